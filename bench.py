@@ -1,0 +1,290 @@
+#!/usr/bin/env python3
+"""Closest-hit ray-casting throughput on one MI355X.
+
+Workload (the ray-cast microbench): 1,000,000 random triangles (centres
+U[0,1]^3, vertices c + l*d, d uniform on the sphere, l log-uniform in
+[1e-3, 2e-2], PCG32 seed 2) in the reference's own SAH BVH (built by the host
+library, byte-identical to yr::BuildBvh); 2048x2048 coherent pinhole rays
+(eye (0.5,0.5,-1.5) -> (0.5,0.5,0.5), 40 degree fov, pixel centres) plus
+4,194,304 incoherent rays (origins U[0,1]^3, directions uniform, seed 3);
+t_min 1e-5, t_max +inf. One step = one closest-hit pass over all 8,388,608
+rays, rays and tree resident in HBM. Prints one JSON line on stdout.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s (closest-hit, 1 GPU) + equal-time MSE, guided vs plain wavefront"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def ensure_built():
+    from yaoray_amd import native
+
+    if not os.path.exists(native.lib_path("libyrh.so")) or not os.path.exists(native.lib_path("libyaoray_host.so")):
+        from yaoray_amd import build
+
+        build.build_all()
+
+
+def algorithmic_bytes(n_rays, node_tests, tri_tests, out_bytes):
+    # 32 B ray in + result out + 32 B per node box test + 36 B per triangle test.
+    return 32.0 * n_rays + out_bytes * n_rays + 32.0 * float(node_tests) + 36.0 * float(tri_tests)
+
+
+def load_traffic(kernel_key):
+    path = os.path.join(ROOT, "profiles", "traffic_c2_microbench.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    entry = data.get(kernel_key)
+    if not entry:
+        return None
+    return (entry["fetch_kib"] + entry["write_kib"]) * 1024.0 * data.get("copy_factor", 1.0)
+
+
+def timed_passes(dev, launch, steps):
+    """Per-launch kernel milliseconds from events on the context stream, plus wall seconds."""
+    starts = [dev.event() for _ in range(steps)]
+    stops = [dev.event() for _ in range(steps)]
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        dev.record(starts[k])
+        launch()
+        dev.record(stops[k])
+    dev.synchronize()
+    wall = time.perf_counter() - t0
+    ms = [dev.elapsed_ms(a, b) for a, b in zip(starts, stops)]
+    for e in starts + stops:
+        dev.destroy_event(e)
+    return ms, wall
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--triangles", type=int, default=1_000_000)
+    ap.add_argument("--resolution", type=int, default=2048)
+    ap.add_argument("--incoherent", type=int, default=4_194_304)
+    ap.add_argument("--cpu-stride", type=int, default=4, help="CPU baseline traces every k-th ray")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--calibration-copy-mb", type=int, default=0,
+                    help="also run a device copy of this size (HBM counter calibration under rocprofv3)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    ensure_built()
+    from yaoray_amd import Device, Scene, random_rays
+
+    t = time.perf_counter()
+    scene = Scene.triangles(args.triangles, seed=2)
+    build_s = scene.build_accel()
+    info = scene.info
+    log(f"scene {args.triangles} tris, {info.node_count} nodes, depth {info.max_depth}, build {build_s:.2f}s")
+    o1, d1 = scene.camera_rays(args.resolution, args.resolution)
+    o2, d2 = random_rays(args.incoherent, seed=3)
+    org = np.ascontiguousarray(np.concatenate([o1, o2]))
+    dirs = np.ascontiguousarray(np.concatenate([d1, d2]))
+    n = len(org)
+    n_coherent = len(o1)
+    log(f"{n} rays ready in {time.perf_counter() - t:.1f}s")
+
+    dev = Device(0)
+    name, cus = dev.info()
+    if args.blocks_per_cu or args.block_size:
+        dev.set_trace_config(args.blocks_per_cu, args.block_size)
+    dev.upload_scene(scene)
+    d_org, d_dir = dev.to_device(org), dev.to_device(dirs)
+    d_hit, d_inst = dev.alloc(16 * n), dev.alloc(4 * n)
+    d_nt, d_tt, d_occ = dev.alloc(4 * n), dev.alloc(4 * n), dev.alloc(4 * n)
+
+    # Counting passes: the per-ray node/triangle test counts (equal to the
+    # reference's counters; checked against it on the CPU sample below).
+    dev.trace_closest_device(d_org, d_dir, n, d_hit, d_inst, d_nt, d_tt)
+    dev.synchronize()
+    nt_closest = d_nt.download(np.uint32, (n,)).astype(np.int64)
+    tt_closest = d_tt.download(np.uint32, (n,)).astype(np.int64)
+    hits = d_hit.download(np.float32, (n, 4))
+    dev.trace_any_device(d_org, d_dir, n, d_occ, d_nt, d_tt)
+    dev.synchronize()
+    nt_any = d_nt.download(np.uint32, (n,)).astype(np.int64).sum()
+    tt_any = d_tt.download(np.uint32, (n,)).astype(np.int64).sum()
+    occluded = d_occ.download(np.uint32, (n,))
+    hit_ids = hits[:, 3].view(np.int32)
+    any_consistent = bool(np.array_equal(occluded.astype(bool), hit_ids >= 0))
+    log(f"counting pass: hit rate {(hit_ids >= 0).mean():.4f}, mean node tests {nt_closest.mean():.1f}, "
+        f"triangle tests {tt_closest.mean():.1f}; any==closest.hit {any_consistent}")
+
+    def closest():
+        dev.trace_closest_device(d_org, d_dir, n, d_hit)
+
+    def anyhit():
+        dev.trace_any_device(d_org, d_dir, n, d_occ)
+
+    for _ in range(args.warmup):
+        closest()
+    dev.synchronize()
+    log(f"warmup {args.warmup} done; timing {args.steps} steps")
+    ms_closest, wall = timed_passes(dev, closest, args.steps)
+    for _ in range(max(1, args.warmup)):
+        anyhit()
+    ms_any, wall_any = timed_passes(dev, anyhit, args.steps)
+    if args.calibration_copy_mb > 0:
+        # A plain elementwise copy of known size: under rocprofv3 --pmc its
+        # FETCH_SIZE / WRITE_SIZE give the counter-to-bytes factor.
+        import torch
+
+        src = torch.ones((args.calibration_copy_mb << 20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy of {args.calibration_copy_mb} MiB done")
+
+    value = n * args.steps / wall / 1e6
+    avg_closest_s = float(np.mean(ms_closest)) / 1e3
+    avg_any_s = float(np.mean(ms_any)) / 1e3
+    b_closest = algorithmic_bytes(n, nt_closest.sum(), tt_closest.sum(), 16)
+    b_any = algorithmic_bytes(n, nt_any, tt_any, 4)
+    achieved = b_closest / avg_closest_s / 1e9
+    log(f"closest: {value:.1f} Mrays/s wall, kernel {avg_closest_s * 1e3:.3f} ms, {achieved:.0f} GB/s algorithmic; "
+        f"any: {n / avg_any_s / 1e6:.1f} Mrays/s")
+
+    cpu = None
+    parity = None
+    if not args.no_cpu_baseline:
+        cpu, parity = cpu_baseline(scene, org, dirs, hits, nt_closest, tt_closest, args.cpu_stride, n_coherent)
+
+    traffic = load_traffic("closest")
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "ray-cast microbench: 1M random triangles (reference SAH BVH), 2048x2048 coherent pinhole "
+                        "+ 4194304 incoherent rays, closest-hit, t in (1e-5, inf]",
+            "triangles": args.triangles,
+            "rays": n,
+            "coherent_rays": n_coherent,
+            "incoherent_rays": n - n_coherent,
+            "bvh_nodes": info.node_count,
+            "bvh_max_depth": info.max_depth,
+            "device": name,
+            "compute_units": cus,
+            "parallelism": "single GPU, persistent lanes with work-counter refill",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "TraceFlatKernel<false> (closest hit)",
+            "algorithmic_bytes_per_launch": b_closest,
+            "avg_kernel_ms": round(avg_closest_s * 1e3, 4),
+            "bytes_model": "32 B ray + 16 B hit + 32 B per node_test + 36 B per triangle_test",
+            "mean_node_tests": round(float(nt_closest.mean()), 3),
+            "mean_triangle_tests": round(float(tt_closest.mean()), 3),
+        },
+        "cpu_baseline": cpu,
+        "any_hit": {
+            "value": round(n / avg_any_s / 1e6, 2),
+            "unit": "Mrays/s (kernel time)",
+            "wall_value": round(n * args.steps / wall_any / 1e6, 2),
+            "avg_kernel_ms": round(avg_any_s * 1e3, 4),
+            "algorithmic_GBps": round(b_any / avg_any_s / 1e9, 1),
+            "occluded_equals_closest_hit": any_consistent,
+        },
+        "closest_kernel_mrays": round(n / avg_closest_s / 1e6, 2),
+        "parity_sample": parity,
+        "guided_vs_plain": None,
+    }
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(scene, org, dirs, hits, nt, tt, stride, n_coherent):
+    """The reference's own IntersectAcceleration (oracle/_ref) on every stride-th ray,
+    timed on the host cores; falls back to the restated oracle where _ref is absent."""
+    from oracle import oracle_py as O
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    sel = np.arange(0, len(org), stride)
+    o, d = np.ascontiguousarray(org[sel]), np.ascontiguousarray(dirs[sel])
+    v, i, p = scene.geometry()
+    t = time.perf_counter()
+    if O.ref_available():
+        kind = "reference"
+        ref = O.RefScene(v, i, p)
+        log(f"cpu baseline: reference BVH rebuilt in {time.perf_counter() - t:.2f}s; tracing {len(sel)} rays "
+            f"on {threads} threads")
+        ref.trace(o[:1024], d[:1024], threads=threads)  # warm-up
+        t0 = time.perf_counter()
+        r = ref.trace(o, d, threads=threads)
+        secs = time.perf_counter() - t0
+    else:
+        kind = "port"
+        bvh = scene.flat_bvh()
+        O.trace_flat(v, i, p, bvh, o[:1024], d[:1024], threads=threads)
+        t0 = time.perf_counter()
+        r = O.trace_flat(v, i, p, bvh, o, d, threads=threads)
+        secs = time.perf_counter() - t0
+    g_id = hits[sel, 3].view(np.int32)
+    g_t = hits[sel, 0]
+    id_match = float(np.mean(g_id == r["triangle"]))
+    both = (g_id >= 0) & (g_id == r["triangle"])
+    rel = np.abs(g_t[both] - r["t"][both]) / np.maximum(np.abs(r["t"][both]), 1e-30)
+    parity = {
+        "rays": int(len(sel)),
+        "checked_against": kind,
+        "hit_id_match": id_match,
+        "t_max_rel_err": float(rel.max()) if len(rel) else 0.0,
+        "t_bit_exact": bool(np.array_equal(g_t.view(np.uint32), r["t"].view(np.uint32))),
+        "node_tests_equal": bool(np.array_equal(nt[sel], r["node_tests"].astype(np.int64))),
+        "triangle_tests_equal": bool(np.array_equal(tt[sel], r["triangle_tests"].astype(np.int64))),
+    }
+    n_coh = int(np.sum(sel < n_coherent))
+    cpu = {
+        "value": round(len(sel) / secs / 1e6, 4),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": f"every {stride}th ray of the same workload ({n_coh} coherent + {len(sel) - n_coh} incoherent), "
+                  f"yr::IntersectAcceleration per ray, {secs:.2f}s",
+    }
+    log(f"cpu baseline ({kind}, {threads} threads): {cpu['value']} Mrays/s; parity {parity}")
+    return cpu, parity
+
+
+if __name__ == "__main__":
+    main()

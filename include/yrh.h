@@ -1,0 +1,230 @@
+/*
+ * yrh.h — C ABI of the MI355X (gfx950) wavefront ray-casting / path-tracing
+ * library `libyrh.so`.
+ *
+ * This is the only surface through which host code (the C++20 HipBackend in
+ * yaoray_amd/host, the Python bindings, tests and benchmarks) reaches HIP.
+ * Signatures use plain C types, pointers and sizes only.
+ *
+ * Conventions
+ *   - Every function returns an int status: YRH_OK (0) or a negative YRH_E_*
+ *     code; yrh_last_error(ctx) returns a human-readable message for the last
+ *     failure on that context (the reference reports failures as `ok = false`
+ *     plus a string: include/yaoray/runtime/backend.hpp:57-79).
+ *   - A context owns one device, one HIP stream and all device copies of the
+ *     scene. Contexts are externally synchronised: one host thread per context,
+ *     matching "no concurrent Render on one prepared scene"
+ *     (src/backends/cpu/cpu_worker_pool.cpp:40-42).
+ *   - Arrays handed to yrh_upload_* are HOST arrays, copied during the call.
+ *     Ray / hit arrays handed to yrh_trace_* are DEVICE arrays (allocate them
+ *     with yrh_device_alloc or any HIP allocator on the context's device).
+ */
+#ifndef YRH_H
+#define YRH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
+#define YRH_OK 0
+#define YRH_E_INVALID_ARGUMENT (-1)
+#define YRH_E_HIP (-2)
+#define YRH_E_UNSUPPORTED (-3)
+#define YRH_E_NO_SCENE (-4)
+#define YRH_E_CANCELLED (-5)
+
+typedef struct yrh_context yrh_context;
+
+/* ---- Scene data, byte-compatible with the reference's IR ---------------- */
+
+/* == yr::RenderBvhNode (include/yaoray/accel/bvh.hpp:34-47), 32 bytes.
+ * Leaf: payload_offset = first primitive slot, primitive_count > 0.
+ * Interior: payload_offset = right child, left child = own index + 1. */
+typedef struct yrh_bvh_node {
+    float bounds_min[3];
+    float bounds_max[3];
+    int32_t payload_offset;
+    int32_t primitive_count;
+} yrh_bvh_node;
+
+/* == yr::RenderBvhPrimitive (include/yaoray/accel/bvh.hpp:49-55), 20 bytes.
+ * kind 0 = triangle, 1 = sphere. Handles are -1 when invalid. */
+typedef struct yrh_bvh_primitive {
+    int32_t kind;
+    int32_t mesh_primitive;
+    int32_t sphere;
+    int32_t local_triangle;
+    int32_t flat_triangle_index;
+} yrh_bvh_primitive;
+
+/* == yr::RenderVertex (include/yaoray/scene/geometry.hpp:12-18), 48 bytes. */
+typedef struct yrh_vertex {
+    float position[3];
+    float normal[3];
+    float uv[2];
+    float tangent[3];
+    float tangent_handedness;
+} yrh_vertex;
+
+/* == yr::RenderPrimitive (include/yaoray/scene/geometry.hpp:20-27), 16 bytes. */
+typedef struct yrh_primitive {
+    uint32_t first_index;
+    uint32_t index_count;
+    int32_t material_index;
+    uint8_t has_normals;
+    uint8_t has_uvs;
+    uint8_t has_tangents;
+    uint8_t reserved;
+} yrh_primitive;
+
+/* One binary BVH: yr::RenderBvh (include/yaoray/accel/bvh.hpp:72-81). */
+typedef struct yrh_bvh_desc {
+    const yrh_bvh_node* nodes;
+    int32_t node_count;
+    const int32_t* primitive_indices;
+    int32_t primitive_index_count;
+    const yrh_bvh_primitive* primitives;
+    int32_t primitive_count;
+    int32_t max_depth;
+} yrh_bvh_desc;
+
+/* == yr::PreparedMeshInstance (include/yaoray/accel/two_level_bvh.hpp:20-26).
+ * Matrices are column-major yr::Mat4f (translation in m[12..14]). The inverse
+ * is the host-computed one (src/core/transform.cpp:119) so object rays match. */
+typedef struct yrh_instance {
+    int32_t primitive;
+    int32_t blas_index;
+    float object_to_world[16];
+    float world_to_object[16];
+    float world_bounds_min[3];
+    float world_bounds_max[3];
+} yrh_instance;
+
+/* == yr::RenderTlasPrimitive (include/yaoray/accel/two_level_bvh.hpp:28-33).
+ * kind 0 = instance, 1 = sphere. */
+typedef struct yrh_tlas_primitive {
+    int32_t kind;
+    int32_t instance;
+    int32_t sphere;
+    float world_bounds_min[3];
+    float world_bounds_max[3];
+} yrh_tlas_primitive;
+
+#define YRH_ACCEL_FLAT 0
+#define YRH_ACCEL_TWO_LEVEL 1
+
+/* GeometryView + RenderAcceleration (include/yaoray/accel/acceleration.hpp:12-19). */
+typedef struct yrh_geometry_desc {
+    const yrh_vertex* vertices;
+    int64_t vertex_count;
+    const uint32_t* indices;
+    int64_t index_count;
+    const yrh_primitive* primitives;
+    int32_t primitive_count;
+    int32_t sphere_count; /* spheres are not supported by this backend: must be 0 */
+
+    int32_t accel_kind; /* YRH_ACCEL_FLAT or YRH_ACCEL_TWO_LEVEL */
+    yrh_bvh_desc flat;  /* used when accel_kind == YRH_ACCEL_FLAT */
+
+    /* used when accel_kind == YRH_ACCEL_TWO_LEVEL */
+    const yrh_bvh_desc* blases; /* one per mesh primitive, in primitive order */
+    int32_t blas_count;
+    const yrh_instance* instances;
+    int32_t instance_count;
+    yrh_bvh_desc tlas; /* tlas.primitives is unused; see tlas_primitives */
+    const yrh_tlas_primitive* tlas_primitives;
+    int32_t tlas_primitive_count;
+} yrh_geometry_desc;
+
+/* ---- Ray / hit queues (device memory, structure of arrays) -------------- */
+
+/* Rays: org_tmin[i] = (origin.xyz, t_min), dir_tmax[i] = (direction.xyz, t_max),
+ * each an array of n float4 (16-byte aligned). */
+typedef struct yrh_ray_soa {
+    const float* org_tmin;
+    const float* dir_tmax;
+} yrh_ray_soa;
+
+/* Closest-hit output: hit[i] = (t, bary_u, bary_v, triangle_index bits) as
+ * float4; triangle_index < 0 means miss (t is then the ray's t_max, as in
+ * yr::BvhHit with hit = false). instance may be NULL; otherwise it receives
+ * the instance index of two-level hits (-1 for flat scenes and misses).
+ * node_tests / triangle_tests may be NULL; otherwise they receive the
+ * per-ray counters with yr::BvhTraceStats semantics
+ * (src/accel/bvh_traversal.cpp:272, 302, 318, 324). */
+typedef struct yrh_hit_soa {
+    float* hit;
+    int32_t* instance;
+    uint32_t* node_tests;
+    uint32_t* triangle_tests;
+} yrh_hit_soa;
+
+/* ---- Context -------------------------------------------------------------- */
+
+int yrh_create(int device, yrh_context** out);
+void yrh_destroy(yrh_context* ctx);
+const char* yrh_last_error(const yrh_context* ctx);
+int yrh_synchronize(yrh_context* ctx);
+/* The context's HIP stream (a hipStream_t), for callers that order their own
+ * work with the library's. */
+void* yrh_stream(yrh_context* ctx);
+/* Device name and compute-unit count of the context's device. */
+int yrh_device_info(yrh_context* ctx, char* name, int name_capacity, int* compute_units);
+
+/* Device memory helpers so callers need no HIP headers. */
+int yrh_device_alloc(yrh_context* ctx, size_t bytes, void** out);
+int yrh_device_free(yrh_context* ctx, void* ptr);
+int yrh_copy_to_device(yrh_context* ctx, void* dst, const void* src, size_t bytes);
+int yrh_copy_to_host(yrh_context* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- Acceleration structure --------------------------------------------- */
+
+/* Replaces the context's geometry. Consumes the reference's own BVH arrays
+ * (flat: yr::RenderBvh; two-level: yr::TwoLevelBvh) exactly as built by
+ * yr::BuildRenderAcceleration (src/accel/acceleration.cpp:28-46) and repacks
+ * them on upload into the device layout (same tree, same visit order). */
+int yrh_upload_geometry(yrh_context* ctx, const yrh_geometry_desc* geometry);
+
+/* Batched closest hit over n rays: the GPU counterpart of calling
+ * yr::IntersectAcceleration (src/accel/acceleration.cpp:48-61) once per ray.
+ * Launched on the context's stream; returns without synchronising. */
+int yrh_trace_closest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n,
+                      const yrh_hit_soa* hits);
+
+/* Batched any-hit (occlusion) over n rays: occluded[i] = 1 if any triangle
+ * is hit with t_min < t <= t_max, else 0. Equal to (closest hit found) on the
+ * same ray; the reference has no any-hit API (its shadow rays are closest-hit
+ * queries, src/integrators/path_direct_lighting.cpp:23-113).
+ * node_tests / triangle_tests may be NULL. */
+int yrh_trace_any(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n,
+                  uint32_t* occluded, uint32_t* node_tests, uint32_t* triangle_tests);
+
+/* Launch geometry of the persistent traversal kernels: workgroups per compute
+ * unit and threads per workgroup (64, 128 or 256); 0 keeps the current value. */
+int yrh_set_trace_config(yrh_context* ctx, int blocks_per_cu, int block_size);
+
+/* ---- Timing on the context's stream --------------------------------------- */
+
+/* HIP events recorded on the context's stream; elapsed milliseconds between
+ * two recorded events (waits for the second one). */
+int yrh_event_create(yrh_context* ctx, void** event);
+int yrh_event_destroy(yrh_context* ctx, void* event);
+int yrh_event_record(yrh_context* ctx, void* event);
+int yrh_event_elapsed_ms(yrh_context* ctx, void* start, void* stop, float* ms);
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* YRH_H */

@@ -1,0 +1,212 @@
+"""ORACLE — test infrastructure only (tests/, __graft_entry__.smoke(), bench.py's
+CPU-baseline leg). ctypes bindings of the CPU restatement (liboracle.so) and
+of the reference compiled from its own sources (_ref/libyrref.so, present
+only where it was built)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REFERENCE_ROOT = os.environ.get("YAORAY_REFERENCE_ROOT", "/root/reference")
+
+_VP, _I32, _I64 = C.c_void_p, C.c_int32, C.c_int64
+
+
+def build(with_reference: bool | None = None, jobs: int = 8):
+    """make -C oracle [ref]; the reference part only where its sources exist."""
+    subprocess.run(["make", "-s", "-C", HERE, f"-j{jobs}", "all"], check=True)
+    if with_reference is None:
+        with_reference = os.path.isdir(os.path.join(REFERENCE_ROOT, "src"))
+    if with_reference:
+        subprocess.run(["make", "-s", "-C", HERE, f"-j{jobs}", "ref", f"REFERENCE_ROOT={REFERENCE_ROOT}"],
+                       check=True)
+
+
+_oracle = None
+_ref = None
+
+
+def oracle_lib():
+    global _oracle
+    if _oracle is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run oracle.oracle_py.build()")
+        lib = C.CDLL(path)
+        lib.oracle_trace_flat.argtypes = [_VP, _I64, _VP, _I64, _VP, _I32, _VP, _VP, _VP, _I64, _I32, _I32,
+                                          _VP, _VP, _VP, _VP]
+        lib.oracle_trace_two_level.argtypes = [_VP, _I64, _VP, _I64, _VP, _I32, _VP, _I32, _VP, _I32, _VP, _VP,
+                                               _I32, _VP, _VP, _I64, _I32, _I32, _VP, _VP, _VP, _VP, _VP]
+        lib.oracle_brute_force.argtypes = [_VP, _I64, _VP, _I64, _VP, _I32, _VP, _VP, _I64, _I32, _VP]
+        _oracle = lib
+    return _oracle
+
+
+def ref_available() -> bool:
+    return os.path.exists(os.path.join(HERE, "_ref", "libyrref.so"))
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        path = os.path.join(HERE, "_ref", "libyrref.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing (the reference is built only where its sources exist)")
+        lib = C.CDLL(path)
+        lib.ref_build.argtypes = [_VP, _I64, _VP, _I64, _VP, _I32, _VP, _VP, _I32, _I32, C.POINTER(_VP),
+                                  C.POINTER(C.c_double)]
+        lib.ref_free.argtypes = [_VP]
+        lib.ref_free.restype = None
+        lib.ref_accel_info.argtypes = [_VP] + [C.POINTER(_I32)] * 7
+        lib.ref_bvh_sizes.argtypes = [_VP, _I32] + [C.POINTER(_I32)] * 4
+        lib.ref_bvh_copy.argtypes = [_VP, _I32, _VP, _VP, _VP]
+        lib.ref_tlas_copy.argtypes = [_VP, _VP, _VP, _VP, _VP]
+        lib.ref_trace.argtypes = [_VP, _VP, _VP, _I64, _I32, _VP, _VP, _VP, _VP]
+        lib.ref_inverse.argtypes = [_VP, _VP]
+        lib.ref_inverse.restype = None
+        _ref = lib
+    return _ref
+
+
+def _p(a):
+    if a is None or len(a) == 0:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be C-contiguous"
+    return a.ctypes.data
+
+
+def _c(a, dtype=np.float32):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# -- restated oracle ------------------------------------------------------------
+
+def trace_flat(vertices, indices, primitives, bvh, org, dirs, occlusion=False, threads=8):
+    """Returns dict(t, u, v, triangle, node_tests, triangle_tests, occluded)."""
+    org, dirs = _c(org), _c(dirs)
+    n = len(org)
+    hit = np.zeros((n, 4), np.float32)
+    occ = np.zeros(n, np.uint32)
+    nt = np.zeros(n, np.uint32)
+    tt = np.zeros(n, np.uint32)
+    desc = bvh.desc()
+    oracle_lib().oracle_trace_flat(_p(vertices), len(vertices), _p(indices), len(indices), _p(primitives),
+                                   len(primitives), C.cast(C.pointer(desc), C.c_void_p),
+                                   org.ctypes.data, dirs.ctypes.data, n, int(occlusion), threads,
+                                   hit.ctypes.data, occ.ctypes.data, nt.ctypes.data, tt.ctypes.data)
+    return {"t": hit[:, 0].copy(), "u": hit[:, 1].copy(), "v": hit[:, 2].copy(),
+            "triangle": hit[:, 3].copy().view(np.int32), "node_tests": nt, "triangle_tests": tt,
+            "occluded": occ}
+
+
+def trace_two_level(vertices, indices, primitives, blases, instances, tlas, tlas_prims, org, dirs,
+                    occlusion=False, threads=8):
+    from yaoray_amd import native as N
+    org, dirs = _c(org), _c(dirs)
+    n = len(org)
+    hit = np.zeros((n, 4), np.float32)
+    inst = np.zeros(n, np.int32)
+    occ = np.zeros(n, np.uint32)
+    nt = np.zeros(n, np.uint32)
+    tt = np.zeros(n, np.uint32)
+    descs = (N.BvhDesc * max(1, len(blases)))(*[b.desc() for b in blases])
+    tdesc = tlas.desc()
+    oracle_lib().oracle_trace_two_level(_p(vertices), len(vertices), _p(indices), len(indices), _p(primitives),
+                                        len(primitives), C.cast(descs, C.c_void_p), len(blases), _p(instances),
+                                        len(instances), C.cast(C.pointer(tdesc), C.c_void_p), _p(tlas_prims),
+                                        len(tlas_prims), org.ctypes.data, dirs.ctypes.data, n, int(occlusion),
+                                        threads, hit.ctypes.data, inst.ctypes.data, occ.ctypes.data,
+                                        nt.ctypes.data, tt.ctypes.data)
+    return {"t": hit[:, 0].copy(), "u": hit[:, 1].copy(), "v": hit[:, 2].copy(),
+            "triangle": hit[:, 3].copy().view(np.int32), "instance": inst, "node_tests": nt,
+            "triangle_tests": tt, "occluded": occ}
+
+
+def brute_force(vertices, indices, primitives, org, dirs, threads=8):
+    org, dirs = _c(org), _c(dirs)
+    n = len(org)
+    hit = np.zeros((n, 4), np.float32)
+    oracle_lib().oracle_brute_force(_p(vertices), len(vertices), _p(indices), len(indices), _p(primitives),
+                                    len(primitives), org.ctypes.data, dirs.ctypes.data, n, threads,
+                                    hit.ctypes.data)
+    return {"t": hit[:, 0].copy(), "u": hit[:, 1].copy(), "v": hit[:, 2].copy(),
+            "triangle": hit[:, 3].copy().view(np.int32)}
+
+
+# -- the reference itself -----------------------------------------------------------
+
+class RefScene:
+    """The reference's RenderSceneIR + RenderAcceleration built by its own code."""
+
+    def __init__(self, vertices, indices, primitives, instance_primitive=None, instance_matrices=None,
+                 threads=0):
+        lib = ref_lib()
+        self._lib = lib
+        ninst = 0 if instance_primitive is None else len(instance_primitive)
+        ip = np.ascontiguousarray(instance_primitive if ninst else np.zeros(1), np.int32)
+        im = np.ascontiguousarray(instance_matrices if ninst else np.zeros((1, 16)), np.float32)
+        h = C.c_void_p()
+        secs = C.c_double()
+        rc = lib.ref_build(_p(vertices), len(vertices), _p(indices), len(indices), _p(primitives), len(primitives),
+                           ip.ctypes.data, im.ctypes.data, ninst, threads, C.byref(h), C.byref(secs))
+        if rc != 0:
+            raise RuntimeError("reference BuildRenderAcceleration reported errors")
+        self._h = h
+        self.build_seconds = secs.value
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._lib.ref_free(self._h)
+        except Exception:
+            pass
+
+    def info(self):
+        vals = [C.c_int32() for _ in range(7)]
+        self._lib.ref_accel_info(self._h, *[C.byref(v) for v in vals])
+        keys = ["kind", "nodes", "prim_indices", "prims", "max_depth", "blas_count", "instance_count"]
+        return {k: v.value for k, v in zip(keys, vals)}
+
+    def bvh(self, blas=-1):
+        from yaoray_amd.api import NODE_DTYPE, PRIM_DTYPE, BvhArrays
+        n, pi, pr, md = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        self._lib.ref_bvh_sizes(self._h, blas, C.byref(n), C.byref(pi), C.byref(pr), C.byref(md))
+        nodes = np.zeros(n.value, NODE_DTYPE)
+        pidx = np.zeros(pi.value, np.int32)
+        prims = np.zeros(pr.value, PRIM_DTYPE)
+        self._lib.ref_bvh_copy(self._h, blas, _p(nodes) or 0, _p(pidx) or 0, _p(prims) or 0)
+        return BvhArrays(nodes, pidx, prims, md.value)
+
+    def tlas(self):
+        from yaoray_amd.api import NODE_DTYPE
+        info = self.info()
+        nodes = np.zeros(info["nodes"], NODE_DTYPE)
+        pidx = np.zeros(info["prim_indices"], np.int32)
+        prim_inst = np.zeros(info["prims"], np.int32)
+        inst = np.zeros((info["instance_count"], 40), np.float32)
+        self._lib.ref_tlas_copy(self._h, _p(nodes) or 0, _p(pidx) or 0, _p(prim_inst) or 0, _p(inst) or 0)
+        return nodes, pidx, prim_inst, inst
+
+    def trace(self, org, dirs, threads=8):
+        org, dirs = _c(org), _c(dirs)
+        n = len(org)
+        hit = np.zeros((n, 4), np.float32)
+        inst = np.zeros(n, np.int32)
+        nt = np.zeros(n, np.uint32)
+        tt = np.zeros(n, np.uint32)
+        self._lib.ref_trace(self._h, org.ctypes.data, dirs.ctypes.data, n, threads, hit.ctypes.data,
+                            inst.ctypes.data, nt.ctypes.data, tt.ctypes.data)
+        return {"t": hit[:, 0].copy(), "u": hit[:, 1].copy(), "v": hit[:, 2].copy(),
+                "triangle": hit[:, 3].copy().view(np.int32), "instance": inst, "node_tests": nt,
+                "triangle_tests": tt}
+
+
+def ref_inverse(m):
+    m = np.ascontiguousarray(m, np.float32).reshape(16)
+    out = np.zeros(16, np.float32)
+    ref_lib().ref_inverse(m.ctypes.data, out.ctypes.data)
+    return out

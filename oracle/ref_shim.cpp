@@ -1,0 +1,196 @@
+// ORACLE — test infrastructure only (see oracle/oracle_trace.cpp).
+//
+// C entry points over the reference itself, compiled from its own sources
+// under /root/reference by oracle/Makefile into oracle/_ref/libyrref.so. Used
+// to pin the restated oracle and the product's BVH builder (byte-identical
+// node arrays), and as the reference CPU baseline. Builds against the
+// reference's headers; contains no reference code.
+#include <yaoray/accel/acceleration.hpp>
+#include <yaoray/core/transform.hpp>
+#include <yaoray/scene/render_scene.hpp>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+struct RefScene {
+    yr::RenderSceneIR scene;
+    yr::RenderAcceleration accel;
+};
+
+}  // namespace
+
+extern "C" {
+#pragma GCC visibility push(default)
+
+// vertices: 48-byte RenderVertex records; primitives: 16-byte RenderPrimitive.
+int ref_build(const void* vertices, int64_t nv, const uint32_t* indices, int64_t ni, const void* primitives,
+              int32_t np, const int32_t* inst_prim, const float* inst_mat, int32_t ninst, int32_t threads,
+              void** out, double* seconds) {
+    static_assert(sizeof(yr::RenderVertex) == 48);
+    static_assert(sizeof(yr::RenderPrimitive) == 16);
+    auto* s = new RefScene();
+    s->scene.vertices.resize(static_cast<size_t>(nv));
+    if (nv > 0) std::memcpy(s->scene.vertices.data(), vertices, static_cast<size_t>(nv) * 48);
+    s->scene.indices.assign(indices, indices + ni);
+    s->scene.primitives.resize(static_cast<size_t>(np));
+    if (np > 0) std::memcpy(s->scene.primitives.data(), primitives, static_cast<size_t>(np) * 16);
+    for (int32_t i = 0; i < ninst; ++i) {
+        yr::RenderInstance inst;
+        inst.primitive = yr::MeshPrimitiveHandle{inst_prim[i]};
+        std::memcpy(inst.object_to_world.m.data(), inst_mat + 16 * static_cast<size_t>(i), 64);
+        s->scene.instances.push_back(inst);
+    }
+    s->scene.materials.push_back(yr::RenderMaterial{});
+    yr::BvhBuildOptions opt;
+    opt.thread_count = threads;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto r = yr::BuildRenderAcceleration(s->scene.Geometry(), opt);
+    const auto t1 = std::chrono::steady_clock::now();
+    if (seconds != nullptr) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (!r.errors.empty()) {
+        delete s;
+        return -1;
+    }
+    s->accel = std::move(r.acceleration);
+    *out = s;
+    return 0;
+}
+
+void ref_free(void* h) { delete static_cast<RefScene*>(h); }
+
+// kind: 0 flat, 1 two-level; counts of the flat BVH or the TLAS.
+int ref_accel_info(void* h, int32_t* kind, int32_t* nodes, int32_t* prim_indices, int32_t* prims, int32_t* max_depth,
+                   int32_t* blas_count, int32_t* instance_count) {
+    const auto* s = static_cast<RefScene*>(h);
+    const bool two = s->accel.kind == yr::RenderAccelerationKind::TwoLevel;
+    *kind = two ? 1 : 0;
+    if (!two) {
+        *nodes = static_cast<int32_t>(s->accel.flat.nodes.size());
+        *prim_indices = static_cast<int32_t>(s->accel.flat.primitive_indices.size());
+        *prims = static_cast<int32_t>(s->accel.flat.primitives.size());
+        *max_depth = s->accel.flat.max_depth;
+        *blas_count = 0;
+        *instance_count = 0;
+    } else {
+        const auto& t = s->accel.two_level.tlas;
+        *nodes = static_cast<int32_t>(t.nodes.size());
+        *prim_indices = static_cast<int32_t>(t.primitive_indices.size());
+        *prims = static_cast<int32_t>(t.primitives.size());
+        *max_depth = t.max_depth;
+        *blas_count = static_cast<int32_t>(s->accel.two_level.blases.size());
+        *instance_count = static_cast<int32_t>(s->accel.two_level.instances.size());
+    }
+    return 0;
+}
+
+// Copies one binary BVH (blas = -1: the flat BVH) as raw reference bytes:
+// nodes 32 B each, primitive_indices int32, primitives 20 B each.
+int ref_bvh_sizes(void* h, int32_t blas, int32_t* nodes, int32_t* prim_indices, int32_t* prims, int32_t* max_depth) {
+    const auto* s = static_cast<RefScene*>(h);
+    const yr::RenderBvh& b = blas < 0 ? s->accel.flat : s->accel.two_level.blases[static_cast<size_t>(blas)].bvh;
+    *nodes = static_cast<int32_t>(b.nodes.size());
+    *prim_indices = static_cast<int32_t>(b.primitive_indices.size());
+    *prims = static_cast<int32_t>(b.primitives.size());
+    *max_depth = b.max_depth;
+    return 0;
+}
+
+int ref_bvh_copy(void* h, int32_t blas, void* nodes, int32_t* prim_indices, void* prims) {
+    const auto* s = static_cast<RefScene*>(h);
+    const yr::RenderBvh& b = blas < 0 ? s->accel.flat : s->accel.two_level.blases[static_cast<size_t>(blas)].bvh;
+    static_assert(sizeof(yr::RenderBvhNode) == 32);
+    std::memcpy(nodes, b.nodes.data(), b.nodes.size() * 32);
+    std::memcpy(prim_indices, b.primitive_indices.data(), b.primitive_indices.size() * 4);
+    auto* out = static_cast<int32_t*>(prims);
+    for (size_t i = 0; i < b.primitives.size(); ++i) {
+        const yr::RenderBvhPrimitive& p = b.primitives[i];
+        out[5 * i + 0] = p.kind == yr::BvhPrimitiveKind::Triangle ? 0 : 1;
+        out[5 * i + 1] = p.mesh_primitive.Value();
+        out[5 * i + 2] = p.sphere.Value();
+        out[5 * i + 3] = p.local_triangle;
+        out[5 * i + 4] = p.flat_triangle_index;
+    }
+    return 0;
+}
+
+// TLAS nodes (32 B each) + primitive indices + instances as
+// (primitive, blas, object_to_world[16], world_to_object[16], bounds[6]) = 40 floats/ints.
+int ref_tlas_copy(void* h, void* nodes, int32_t* prim_indices, int32_t* prim_instance, float* instances) {
+    const auto* s = static_cast<RefScene*>(h);
+    const auto& tl = s->accel.two_level;
+    std::memcpy(nodes, tl.tlas.nodes.data(), tl.tlas.nodes.size() * 32);
+    std::memcpy(prim_indices, tl.tlas.primitive_indices.data(), tl.tlas.primitive_indices.size() * 4);
+    for (size_t i = 0; i < tl.tlas.primitives.size(); ++i) prim_instance[i] = tl.tlas.primitives[i].instance.Value();
+    for (size_t i = 0; i < tl.instances.size(); ++i) {
+        const yr::PreparedMeshInstance& in = tl.instances[i];
+        float* o = instances + 40 * i;
+        const int32_t ids[2] = {in.primitive.Value(), in.blas_index};
+        std::memcpy(o, ids, 8);
+        std::memcpy(o + 2, in.object_to_world.m.data(), 64);
+        std::memcpy(o + 18, in.world_to_object.m.data(), 64);
+        const float b[6] = {in.world_bounds.min.x, in.world_bounds.min.y, in.world_bounds.min.z,
+                            in.world_bounds.max.x, in.world_bounds.max.y, in.world_bounds.max.z};
+        std::memcpy(o + 34, b, 24);
+    }
+    return 0;
+}
+
+// yr::IntersectAcceleration per ray over `threads` threads.
+// hit: float4 (t, u, v, triangle_index bits); inst / counters may be null.
+int ref_trace(void* h, const float* org_tmin, const float* dir_tmax, int64_t n, int32_t threads, float* hit,
+              int32_t* inst, uint32_t* node_tests, uint32_t* tri_tests) {
+    const auto* s = static_cast<RefScene*>(h);
+    const yr::GeometryView g = s->scene.Geometry();
+    std::atomic_int64_t next{0};
+    auto work = [&] {
+        while (true) {
+            const int64_t b = next.fetch_add(256);
+            if (b >= n) break;
+            const int64_t e = std::min(n, b + 256);
+            for (int64_t i = b; i < e; ++i) {
+                const yr::Ray3f ray{yr::Point3f{org_tmin[4 * i], org_tmin[4 * i + 1], org_tmin[4 * i + 2]},
+                                    yr::Vec3f{dir_tmax[4 * i], dir_tmax[4 * i + 1], dir_tmax[4 * i + 2]}};
+                yr::BvhTraceStats st;
+                const yr::BvhHit hh = yr::IntersectAcceleration(g, s->accel, ray, st, org_tmin[4 * i + 3],
+                                                                dir_tmax[4 * i + 3]);
+                if (hit != nullptr) {
+                    const int32_t id = hh.hit ? hh.triangle_index : -1;
+                    float fid;
+                    std::memcpy(&fid, &id, 4);
+                    hit[4 * i] = hh.t;
+                    hit[4 * i + 1] = hh.bary_u;
+                    hit[4 * i + 2] = hh.bary_v;
+                    hit[4 * i + 3] = fid;
+                }
+                if (inst != nullptr) inst[i] = hh.hit ? hh.instance.Value() : -1;
+                if (node_tests != nullptr) node_tests[i] = static_cast<uint32_t>(st.node_tests);
+                if (tri_tests != nullptr) tri_tests[i] = static_cast<uint32_t>(st.triangle_tests);
+            }
+        }
+    };
+    if (threads <= 1) {
+        work();
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t) pool.emplace_back(work);
+        for (auto& th : pool) th.join();
+    }
+    return 0;
+}
+
+void ref_inverse(const float* m, float* out) {
+    yr::Mat4f a;
+    std::memcpy(a.m.data(), m, 64);
+    const yr::Mat4f r = yr::Inverse(a);
+    std::memcpy(out, r.m.data(), 64);
+}
+
+#pragma GCC visibility pop
+}  // extern "C"

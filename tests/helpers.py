@@ -1,0 +1,95 @@
+"""Small in-code scenes mirroring the reference's test scenes
+(tests/accel/bvh_test_scene.hpp:9-53) and array helpers."""
+import numpy as np
+
+from yaoray_amd.api import PRIMITIVE_DTYPE, VERTEX_DTYPE, Scene
+
+
+class MeshArrays:
+    def __init__(self):
+        self.vertices = []
+        self.indices = []
+        self.primitives = []
+
+    def add_test_triangle(self, x_offset, z=0.0, material=0):
+        # AddBvhTestTriangle: (x-0.25,-0.25) (x+0.25,-0.25) (x,0.25) at z, normal +z.
+        first_vertex = len(self.vertices)
+        first_index = len(self.indices)
+        for p in ((x_offset - 0.25, -0.25, z), (x_offset + 0.25, -0.25, z), (x_offset, 0.25, z)):
+            self.vertices.append((p, (0.0, 0.0, 1.0)))
+        self.indices += [first_vertex, first_vertex + 1, first_vertex + 2]
+        self.primitives.append((first_index, 3, material, 1))
+        return self
+
+    def add_triangles(self, tris, material=0):
+        """One primitive holding every triangle of `tris` (n, 3, 3)."""
+        first_index = len(self.indices)
+        for tri in np.asarray(tris, np.float32):
+            base = len(self.vertices)
+            for p in tri:
+                self.vertices.append((tuple(p), (0.0, 0.0, 0.0)))
+            self.indices += [base, base + 1, base + 2]
+        self.primitives.append((first_index, 3 * len(tris), material, 0))
+        return self
+
+    def arrays(self):
+        v = np.zeros(len(self.vertices), VERTEX_DTYPE)
+        for k, (p, n) in enumerate(self.vertices):
+            v[k]["position"] = p
+            v[k]["normal"] = n
+            v[k]["tangent_handedness"] = 1.0
+        i = np.array(self.indices, np.uint32)
+        p = np.zeros(len(self.primitives), PRIMITIVE_DTYPE)
+        for k, (fi, ic, m, hn) in enumerate(self.primitives):
+            p[k]["first_index"] = fi
+            p[k]["index_count"] = ic
+            p[k]["material_index"] = m
+            p[k]["has_normals"] = hn
+        return v, i, p
+
+    def scene(self, instance_primitive=None, instance_matrices=None):
+        v, i, p = self.arrays()
+        return Scene.from_arrays(v, i, p, instance_primitive, instance_matrices)
+
+
+def clustered():
+    m = MeshArrays()
+    for x in (0.0, 0.05, 0.10, 0.15, 5.0, 5.05, 5.10, 5.15, 10.0, 10.05, 10.10, 10.15):
+        m.add_test_triangle(x)
+    return m
+
+
+def rays(origins, directions, t_min=1e-5, t_max=np.inf):
+    o = np.zeros((len(origins), 4), np.float32)
+    d = np.zeros((len(origins), 4), np.float32)
+    o[:, :3] = origins
+    o[:, 3] = t_min
+    d[:, :3] = directions
+    d[:, 3] = t_max
+    return o, d
+
+
+def same_bits(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.dtype == np.float32:
+        a = a.view(np.uint32)
+    if b.dtype == np.float32:
+        b = b.view(np.uint32)
+    return np.array_equal(a, b)
+
+
+def trs(translate, axis, angle_deg, scale):
+    """Column-major 4x4 T * R(axis, angle) * S as float32[16] (computed in float64)."""
+    axis = np.asarray(axis, np.float64)
+    axis = axis / np.linalg.norm(axis)
+    a = np.radians(angle_deg)
+    c, s, t = np.cos(a), np.sin(a), 1 - np.cos(a)
+    x, y, z = axis
+    r = np.array([[t * x * x + c, t * x * y - s * z, t * x * z + s * y],
+                  [t * x * y + s * z, t * y * y + c, t * y * z - s * x],
+                  [t * x * z - s * y, t * y * z + s * x, t * z * z + c]])
+    m = np.eye(4)
+    m[:3, :3] = r @ np.diag(np.broadcast_to(np.asarray(scale, np.float64), 3))
+    m[:3, 3] = translate
+    return m.T.reshape(16).astype(np.float32)  # column-major

@@ -1,0 +1,363 @@
+"""Python face of the backend: scenes (host library) and a device context (libyrh).
+
+Host arrays are numpy; device arrays are ``DeviceBuffer`` objects owned by a
+``Device`` and allocated through the C ABI, so this module needs neither HIP
+headers nor torch.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native as N
+
+NODE_DTYPE = np.dtype([("bounds_min", "<f4", 3), ("bounds_max", "<f4", 3),
+                       ("payload_offset", "<i4"), ("primitive_count", "<i4")])
+PRIM_DTYPE = np.dtype([("kind", "<i4"), ("mesh_primitive", "<i4"), ("sphere", "<i4"),
+                       ("local_triangle", "<i4"), ("flat_triangle_index", "<i4")])
+VERTEX_DTYPE = np.dtype([("position", "<f4", 3), ("normal", "<f4", 3), ("uv", "<f4", 2),
+                         ("tangent", "<f4", 3), ("tangent_handedness", "<f4")])
+PRIMITIVE_DTYPE = np.dtype([("first_index", "<u4"), ("index_count", "<u4"), ("material_index", "<i4"),
+                            ("has_normals", "u1"), ("has_uvs", "u1"), ("has_tangents", "u1"),
+                            ("reserved", "u1")])
+INSTANCE_DTYPE = np.dtype([("primitive", "<i4"), ("blas_index", "<i4"), ("object_to_world", "<f4", 16),
+                           ("world_to_object", "<f4", 16), ("world_bounds_min", "<f4", 3),
+                           ("world_bounds_max", "<f4", 3)])
+TLAS_PRIM_DTYPE = np.dtype([("kind", "<i4"), ("instance", "<i4"), ("sphere", "<i4"),
+                            ("world_bounds_min", "<f4", 3), ("world_bounds_max", "<f4", 3)])
+
+assert NODE_DTYPE.itemsize == 32 and PRIM_DTYPE.itemsize == 20 and VERTEX_DTYPE.itemsize == 48
+assert PRIMITIVE_DTYPE.itemsize == 16 and INSTANCE_DTYPE.itemsize == 160 and TLAS_PRIM_DTYPE.itemsize == 36
+
+
+class YrhError(RuntimeError):
+    pass
+
+
+def _view(ptr, dtype, count):
+    """Copy `count` records of `dtype` from a C pointer into a numpy array."""
+    dtype = np.dtype(dtype)
+    if count == 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    buf = (C.c_char * (dtype.itemsize * count)).from_address(ptr)
+    return np.frombuffer(buf, dtype=dtype, count=count).copy()
+
+
+def _ptr(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+class BvhArrays:
+    """One binary BVH as the reference lays it out (yr::RenderBvh)."""
+
+    def __init__(self, nodes, primitive_indices, primitives, max_depth):
+        self.nodes = nodes
+        self.primitive_indices = primitive_indices
+        self.primitives = primitives
+        self.max_depth = int(max_depth)
+
+    @staticmethod
+    def from_desc(d: N.BvhDesc) -> "BvhArrays":
+        return BvhArrays(_view(d.nodes, NODE_DTYPE, d.node_count),
+                         _view(d.primitive_indices, "<i4", d.primitive_index_count),
+                         _view(d.primitives, PRIM_DTYPE, d.primitive_count), d.max_depth)
+
+    def desc(self) -> N.BvhDesc:
+        d = N.BvhDesc()
+        d.nodes = _ptr(self.nodes) if len(self.nodes) else None
+        d.node_count = len(self.nodes)
+        d.primitive_indices = _ptr(self.primitive_indices) if len(self.primitive_indices) else None
+        d.primitive_index_count = len(self.primitive_indices)
+        d.primitives = _ptr(self.primitives) if len(self.primitives) else None
+        d.primitive_count = len(self.primitives)
+        d.max_depth = self.max_depth
+        return d
+
+
+class Scene:
+    """A RenderSceneIR owned by the host library, with its acceleration structure."""
+
+    def __init__(self, handle):
+        self._lib = N.host()
+        self._h = handle
+
+    # -- construction ---------------------------------------------------
+    @classmethod
+    def _create(cls, kind, params, seed):
+        lib = N.host()
+        arr = (C.c_int64 * max(1, len(params)))(*params)
+        h = C.c_void_p()
+        rc = lib.yrhost_scene_create(kind, arr, len(params), seed, C.byref(h))
+        if rc != 0:
+            raise YrhError(f"yrhost_scene_create({kind}) failed: {rc}")
+        return cls(h)
+
+    @classmethod
+    def cornell(cls, material_seed: int = 1) -> "Scene":
+        return cls._create(N.SCENE_CORNELL, [], material_seed)
+
+    @classmethod
+    def triangles(cls, count: int, seed: int = 2) -> "Scene":
+        return cls._create(N.SCENE_TRIANGLES, [count], seed)
+
+    @classmethod
+    def instanced(cls, instances=10000, subdivisions=4, variants=8, env_width=2048, env_height=1024,
+                  seed: int = 4) -> "Scene":
+        return cls._create(N.SCENE_INSTANCED, [instances, subdivisions, variants, env_width, env_height], seed)
+
+    @classmethod
+    def from_arrays(cls, vertices, indices, primitives, instance_primitive=None, instance_matrices=None):
+        lib = N.host()
+        vertices = np.ascontiguousarray(vertices, dtype=VERTEX_DTYPE)
+        indices = np.ascontiguousarray(indices, dtype="<u4")
+        primitives = np.ascontiguousarray(primitives, dtype=PRIMITIVE_DTYPE)
+        ninst = 0 if instance_primitive is None else len(instance_primitive)
+        ip = np.ascontiguousarray(instance_primitive if ninst else np.zeros(1), dtype="<i4")
+        im = np.ascontiguousarray(instance_matrices if ninst else np.zeros((1, 16)), dtype="<f4")
+        h = C.c_void_p()
+        rc = lib.yrhost_scene_from_arrays(_ptr(vertices) if len(vertices) else None, len(vertices),
+                                          _ptr(indices) if len(indices) else None, len(indices),
+                                          _ptr(primitives) if len(primitives) else None, len(primitives),
+                                          _ptr(ip), _ptr(im), ninst, C.byref(h))
+        if rc != 0:
+            raise YrhError(f"yrhost_scene_from_arrays failed: {rc}")
+        return cls(h)
+
+    def close(self):
+        if self._h:
+            self._lib.yrhost_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- queries ----------------------------------------------------------
+    @property
+    def info(self) -> N.SceneInfo:
+        info = N.SceneInfo()
+        self._lib.yrhost_scene_info_get(self._h, C.byref(info))
+        return info
+
+    def geometry(self):
+        """(vertices, indices, primitives) numpy copies."""
+        info = self.info
+        v, i, p = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        self._lib.yrhost_scene_geometry(self._h, C.byref(v), C.byref(i), C.byref(p))
+        return (_view(v.value, VERTEX_DTYPE, info.vertex_count), _view(i.value, "<u4", info.index_count),
+                _view(p.value, PRIMITIVE_DTYPE, info.primitive_count))
+
+    def build_accel(self, threads: int = 0) -> float:
+        secs = C.c_double()
+        rc = self._lib.yrhost_build_accel(self._h, threads, C.byref(secs))
+        if rc != 0:
+            raise YrhError(self._lib.yrhost_scene_error(self._h).decode())
+        return secs.value
+
+    def flat_bvh(self) -> BvhArrays:
+        d = N.BvhDesc()
+        if self._lib.yrhost_accel_flat(self._h, C.byref(d)) != 0:
+            raise YrhError("no acceleration structure")
+        return BvhArrays.from_desc(d)
+
+    def blas(self, index: int) -> BvhArrays:
+        d = N.BvhDesc()
+        if self._lib.yrhost_accel_blas(self._h, index, C.byref(d)) != 0:
+            raise YrhError(f"no BLAS {index}")
+        return BvhArrays.from_desc(d)
+
+    def tlas(self):
+        """(tlas BvhArrays (no primitives), tlas primitives, instances)."""
+        d = N.BvhDesc()
+        prims, nprims, inst, ninst = C.c_void_p(), C.c_int32(), C.c_void_p(), C.c_int32()
+        if self._lib.yrhost_accel_tlas(self._h, C.byref(d), C.byref(prims), C.byref(nprims), C.byref(inst),
+                                       C.byref(ninst)) != 0:
+            raise YrhError("no acceleration structure")
+        t = BvhArrays(_view(d.nodes, NODE_DTYPE, d.node_count), _view(d.primitive_indices, "<i4",
+                                                                      d.primitive_index_count),
+                      np.zeros(0, PRIM_DTYPE), d.max_depth)
+        return t, _view(prims.value, TLAS_PRIM_DTYPE, nprims.value), _view(inst.value, INSTANCE_DTYPE, ninst.value)
+
+    def camera_rays(self, width: int, height: int, t_min=1e-5, t_max=float("inf")):
+        org = np.empty((width * height, 4), np.float32)
+        dirs = np.empty((width * height, 4), np.float32)
+        rc = self._lib.yrhost_camera_rays(self._h, width, height, t_min, t_max, _ptr(org), _ptr(dirs))
+        if rc != 0:
+            raise YrhError("yrhost_camera_rays failed")
+        return org, dirs
+
+    @property
+    def handle(self):
+        return self._h
+
+
+def random_rays(count: int, seed: int = 3, t_min=1e-5, t_max=float("inf")):
+    org = np.empty((count, 4), np.float32)
+    dirs = np.empty((count, 4), np.float32)
+    rc = N.host().yrhost_random_rays(count, seed, t_min, t_max, _ptr(org) if count else None,
+                                     _ptr(dirs) if count else None)
+    if rc != 0:
+        raise YrhError("yrhost_random_rays failed")
+    return org, dirs
+
+
+class DeviceBuffer:
+    def __init__(self, device: "Device", nbytes: int):
+        self.device = device
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        device._check(device._lib.yrh_device_alloc(device._ctx, max(self.nbytes, 16), C.byref(p)), "alloc")
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        self.device._check(self.device._lib.yrh_copy_to_device(self.device._ctx, self.ptr, _ptr(a), a.nbytes),
+                           "copy_to_device")
+        return self
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        self.device._check(self.device._lib.yrh_copy_to_host(self.device._ctx, _ptr(out), self.ptr, out.nbytes),
+                           "copy_to_host")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.device._lib.yrh_device_free(self.device._ctx, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.device._ctx:
+                self.free()
+        except Exception:
+            pass
+
+
+class Device:
+    """One yrh_context: a GPU, a HIP stream and the uploaded scene."""
+
+    def __init__(self, device: int = 0):
+        self._lib = N.yrh()
+        ctx = C.c_void_p()
+        rc = self._lib.yrh_create(device, C.byref(ctx))
+        if rc != 0:
+            raise YrhError(f"yrh_create({device}) failed with {rc} (no HIP device?)")
+        self._ctx = ctx
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._lib.yrh_last_error(self._ctx)
+            raise YrhError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self._ctx:
+            self._lib.yrh_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    def info(self):
+        name = C.create_string_buffer(256)
+        cus = C.c_int()
+        self._check(self._lib.yrh_device_info(self._ctx, name, 256, C.byref(cus)), "device_info")
+        return name.value.decode(), cus.value
+
+    def synchronize(self):
+        self._check(self._lib.yrh_synchronize(self._ctx), "synchronize")
+
+    def alloc(self, nbytes) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, a: np.ndarray) -> DeviceBuffer:
+        a = np.ascontiguousarray(a)
+        return DeviceBuffer(self, a.nbytes).upload(a)
+
+    def upload_scene(self, scene: Scene):
+        self._check(N.host().yrhost_upload_geometry(scene.handle, self._ctx), "upload_geometry")
+
+    def upload_geometry(self, desc: N.GeometryDesc):
+        self._check(self._lib.yrh_upload_geometry(self._ctx, C.byref(desc)), "upload_geometry")
+
+    def set_trace_config(self, blocks_per_cu=0, block_size=0):
+        self._check(self._lib.yrh_set_trace_config(self._ctx, blocks_per_cu, block_size), "set_trace_config")
+
+    # -- timing on the context's stream ---------------------------------------
+    def event(self):
+        e = C.c_void_p()
+        self._check(self._lib.yrh_event_create(self._ctx, C.byref(e)), "event_create")
+        return e.value
+
+    def record(self, event):
+        self._check(self._lib.yrh_event_record(self._ctx, event), "event_record")
+
+    def elapsed_ms(self, start, stop) -> float:
+        ms = C.c_float()
+        self._check(self._lib.yrh_event_elapsed_ms(self._ctx, start, stop, C.byref(ms)), "event_elapsed")
+        return ms.value
+
+    def destroy_event(self, event):
+        self._lib.yrh_event_destroy(self._ctx, event)
+
+    # -- device-resident launches ------------------------------------------
+    def trace_closest_device(self, org: DeviceBuffer, dirs: DeviceBuffer, n: int, hit: DeviceBuffer,
+                             instance: DeviceBuffer | None = None, node_tests: DeviceBuffer | None = None,
+                             tri_tests: DeviceBuffer | None = None):
+        rays = N.RaySoa(org.ptr, dirs.ptr)
+        hits = N.HitSoa(hit.ptr, instance.ptr if instance else None, node_tests.ptr if node_tests else None,
+                        tri_tests.ptr if tri_tests else None)
+        self._check(self._lib.yrh_trace_closest(self._ctx, C.byref(rays), n, C.byref(hits)), "trace_closest")
+
+    def trace_any_device(self, org: DeviceBuffer, dirs: DeviceBuffer, n: int, occluded: DeviceBuffer,
+                         node_tests: DeviceBuffer | None = None, tri_tests: DeviceBuffer | None = None):
+        rays = N.RaySoa(org.ptr, dirs.ptr)
+        self._check(self._lib.yrh_trace_any(self._ctx, C.byref(rays), n, occluded.ptr,
+                                            node_tests.ptr if node_tests else None,
+                                            tri_tests.ptr if tri_tests else None), "trace_any")
+
+    # -- host convenience ---------------------------------------------------
+    def trace_closest(self, org: np.ndarray, dirs: np.ndarray, counters: bool = False):
+        """Returns dict(t, u, v, triangle, instance[, node_tests, triangle_tests]) as numpy arrays."""
+        n = len(org)
+        d_org, d_dir = self.to_device(org.astype(np.float32)), self.to_device(dirs.astype(np.float32))
+        d_hit = self.alloc(16 * n)
+        d_inst = self.alloc(4 * n)
+        d_nt = self.alloc(4 * n) if counters else None
+        d_tt = self.alloc(4 * n) if counters else None
+        self.trace_closest_device(d_org, d_dir, n, d_hit, d_inst, d_nt, d_tt)
+        self.synchronize()
+        hit = d_hit.download(np.float32, (n, 4))
+        out = {"t": hit[:, 0].copy(), "u": hit[:, 1].copy(), "v": hit[:, 2].copy(),
+               "triangle": hit[:, 3].copy().view(np.int32), "instance": d_inst.download(np.int32, (n,))}
+        if counters:
+            out["node_tests"] = d_nt.download(np.uint32, (n,))
+            out["triangle_tests"] = d_tt.download(np.uint32, (n,))
+        return out
+
+    def trace_any(self, org: np.ndarray, dirs: np.ndarray, counters: bool = False):
+        n = len(org)
+        d_org, d_dir = self.to_device(org.astype(np.float32)), self.to_device(dirs.astype(np.float32))
+        d_occ = self.alloc(4 * n)
+        d_nt = self.alloc(4 * n) if counters else None
+        d_tt = self.alloc(4 * n) if counters else None
+        self.trace_any_device(d_org, d_dir, n, d_occ, d_nt, d_tt)
+        self.synchronize()
+        out = {"occluded": d_occ.download(np.uint32, (n,))}
+        if counters:
+            out["node_tests"] = d_nt.download(np.uint32, (n,))
+            out["triangle_tests"] = d_tt.download(np.uint32, (n,))
+        return out

@@ -1,0 +1,143 @@
+// Internal declarations shared by the libyrh translation units.
+//
+// Device layout of the acceleration structure (MI355X / gfx950):
+//   * A binary BVH is the reference's own tree (include/yaoray/accel/bvh.hpp:34-47),
+//     repacked so that an interior node carries both child boxes: one 64-byte
+//     record (four 16-byte loads issued together) instead of two dependent
+//     32-byte node reads per interior step. Visit order, box arithmetic and the
+//     node_tests counter are unchanged.
+//   * A child is named by a 32-bit "code": >= 0 is an interior node index,
+//     < 0 a leaf: bit 31 set, bits 27..30 = count-1 (15 = escape, count read from
+//     leaf_count[first]), bits 0..26 = first primitive slot.
+//   * Primitive slots follow the leaf order of primitive_indices; each slot
+//     holds the triangle as (p0, e1 = p1 - p0, e2 = p2 - p0) plus its flat
+//     triangle index: 3 x float4 = 48 bytes, one coalesced fetch per test in
+//     place of the reference's 5-level index chase (bvh_traversal.cpp:59-79).
+//     e1/e2 are the same IEEE subtractions the reference performs per test,
+//     so every intersection is bit-identical.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/yrh.h"
+
+namespace yrh {
+
+constexpr int kLeafCountEscape = 15;
+constexpr int32_t kMaxSlot = (1 << 27) - 1;
+
+struct alignas(16) DevNode {
+    float4 a;  // L.min.x L.min.y L.min.z L.max.x
+    float4 b;  // L.max.y L.max.z R.min.x R.min.y
+    float4 c;  // R.min.z R.max.x R.max.y R.max.z
+    int4 d;    // left code, right code, 0, 0
+};
+static_assert(sizeof(DevNode) == 64, "node record is 64 bytes");
+
+struct alignas(16) DevInstance {
+    float4 r0;  // world_to_object m[0], m[4], m[8],  m[12]
+    float4 r1;  //                 m[1], m[5], m[9],  m[13]
+    float4 r2;  //                 m[2], m[6], m[10], m[14]
+    int4 blas;  // blas index, root code, 0, 0
+};
+static_assert(sizeof(DevInstance) == 64, "instance record is 64 bytes");
+
+struct alignas(16) DevBlasRoot {
+    float4 lo;  // root box min, w = root code bits
+    float4 hi;  // root box max
+};
+
+inline int32_t PackLeaf(int32_t first, int32_t count) {
+    const int32_t nib = count <= kLeafCountEscape ? count - 1 : kLeafCountEscape;
+    return static_cast<int32_t>(0x80000000u | (static_cast<uint32_t>(nib) << 27) |
+                                static_cast<uint32_t>(first));
+}
+
+// Device buffer owned by a context.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& other) noexcept : ptr(other.ptr), bytes(other.bytes) {
+        other.ptr = nullptr;
+        other.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& other) noexcept {
+        if (this != &other) {
+            reset();
+            ptr = other.ptr;
+            bytes = other.bytes;
+            other.ptr = nullptr;
+            other.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (ptr != nullptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct DevGeometry {
+    bool loaded = false;
+    int accel_kind = YRH_ACCEL_FLAT;
+    // flat BVH, or every BLAS concatenated (node and slot indices rebased)
+    DevBuf nodes;
+    DevBuf slots;       // float4[3 * slot_count]
+    DevBuf leaf_count;  // int32[slot_count], read only for escaped leaves
+    int64_t slot_count = 0;
+    int32_t root_code = 0;
+    float root_lo[3] = {0, 0, 0};
+    float root_hi[3] = {0, 0, 0};
+    bool empty = true;
+    // two-level
+    DevBuf tlas_nodes;
+    DevBuf tlas_slot_instance;  // int32 per TLAS slot
+    DevBuf tlas_leaf_count;
+    DevBuf instances;           // DevInstance per instance
+    DevBuf blas_roots;          // DevBlasRoot per BLAS
+    int32_t instance_count = 0;
+    int32_t blas_count = 0;
+    int32_t stack_capacity = 0;  // entries needed by the deepest traversal
+};
+
+}  // namespace yrh
+
+struct yrh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int compute_units = 0;
+    char device_name[256] = {0};
+    std::string error;
+    yrh::DevGeometry geo;
+    // persistent traversal scratch
+    // uint32 words: [0], [1] ray counters used by alternate launches (each
+    // launch zeroes the other one for its successor), [2] stack overflow flag
+    yrh::DevBuf work_counter;
+    int launch_parity = 0;
+    yrh::DevBuf stack_spill;    // int2 per lane per spilled stack level
+    int blocks_per_cu = 4;
+    int block_size = 256;
+};
+
+namespace yrh {
+
+int Fail(yrh_context* ctx, int code, const std::string& message);
+int CheckHip(yrh_context* ctx, hipError_t status, const char* what);
+
+// Launches of the traversal kernels (yrh_trace.hip).
+int LaunchTraceClosest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, const yrh_hit_soa* hits);
+int LaunchTraceAny(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, uint32_t* occluded,
+                   uint32_t* node_tests, uint32_t* triangle_tests);
+
+}  // namespace yrh
