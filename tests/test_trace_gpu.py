@@ -116,18 +116,20 @@ def test_empty_scene_and_single_triangle(device):
 
 
 def test_deep_tree_uses_stack_spill(device):
-    # Exponentially spaced triangles make the SAH peel one cluster per level,
-    # so the tree is far deeper than the 16 LDS stack levels.
+    # Triangles at x = 4^k make the SAH peel about one triangle per level, so
+    # the tree is deeper than the 16 LDS stack levels; rays travelling along +x
+    # in the triangles' plane hit both children at every level and fill the
+    # stack to the tree depth.
     tris = []
-    for k in range(120):
-        x = 1.2 ** k
-        tris.append([[x, 0, 0], [x * 1.05, 0, 0], [x, 0.5, 0]])
+    for k in range(59):
+        x = 4.0 ** k
+        tris.append([[x, 0, 0], [x * 1.05, 0, 0], [x, 0.5 * max(1.0, x * 0.01), 0]])
     m = MeshArrays().add_triangles(np.array(tris, np.float32))
     s = m.scene()
     s.build_accel()
     assert s.flat_bvh().max_depth > 20
-    xs = np.array([1.2 ** k * 1.01 for k in range(120)] * 4, np.float32)
-    ys = np.repeat([0.05, 0.1, 0.2, 0.45], 120).astype(np.float32)
+    xs = np.array([4.0 ** k * 1.01 for k in range(59)] * 4, np.float32)
+    ys = np.repeat([0.05, 0.1, 0.2, 0.45], 59).astype(np.float32)
     origins = np.stack([xs, ys, np.full_like(xs, 1.0)], 1)
     org, d = rays(origins, [(0, 0, -1)] * len(xs))
     o2, d2 = rays([(-1.0, 0.1, 0.0)] * 4, [(1, 0, 0), (1, 0.001, 0), (1, 0, 0.001), (1, 0.01, 0)])
