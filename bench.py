@@ -52,7 +52,10 @@ def load_traffic(kernel_key):
     entry = data.get(kernel_key)
     if not entry:
         return None
-    return (entry["fetch_kib"] + entry["write_kib"]) * 1024.0 * data.get("copy_factor", 1.0)
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE (KiB per dispatch), each scaled by what
+    # the counter reads for a device copy of known size.
+    return round(entry["fetch_kib"] * 1024.0 * data["fetch_factor"] + entry["write_kib"] * 1024.0 *
+                 data["write_factor"])
 
 
 def timed_passes(dev, launch, steps):
