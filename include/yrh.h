@@ -210,6 +210,144 @@ int yrh_trace_any(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n,
  * unit and threads per workgroup (64, 128 or 256); 0 keeps the current value. */
 int yrh_set_trace_config(yrh_context* ctx, int blocks_per_cu, int block_size);
 
+/* ---- Shading data ----------------------------------------------------------- */
+
+#define YRH_MATERIAL_DIFFUSE 0
+#define YRH_MATERIAL_CONDUCTOR 1
+#define YRH_MATERIAL_DIELECTRIC 2
+#define YRH_MATERIAL_THIN_DIELECTRIC 3
+
+/* The untextured subset of yr::RenderMaterial (include/yaoray/scene/material.hpp:26-65)
+ * that the in-scope BSDFs read (src/shading/bsdf.cpp:17-329). */
+typedef struct yrh_material {
+    int32_t kind; /* YRH_MATERIAL_* */
+    float reflectance[3];
+    float ior;
+    float uroughness;
+    float vroughness;
+    float alpha; /* must be >= 0.5: alpha cut-outs are not supported */
+    float emission[3];
+    float absorption_distance;
+    float absorption_color[3];
+    float reserved;
+} yrh_material;
+
+/* == yr::EmissivePrimitive (include/yaoray/scene/light.hpp:30-34). */
+typedef struct yrh_emissive {
+    int32_t primitive_index;
+    float radiance[3];
+    float area;
+} yrh_emissive;
+
+/* yr::RenderEnvironment + its RenderTexture (RGBA float, bilinear, repeat)
+ * + RenderEnvironmentDistribution (include/yaoray/scene/light.hpp:10-28). */
+typedef struct yrh_environment_desc {
+    int32_t active;
+    float radiance[3];
+    float strength;
+    float rotation_radians;
+    int32_t width;
+    int32_t height;
+    const float* texels; /* width * height * 4 floats, or NULL */
+    int32_t has_distribution;
+    int32_t dist_width;
+    int32_t dist_height;
+    const float* texel_weights;    /* dist_width * dist_height */
+    const float* row_weights;      /* dist_height */
+    const float* row_cdf;          /* dist_height */
+    const float* conditional_cdfs; /* dist_width * dist_height */
+    float total_weight;
+} yrh_environment_desc;
+
+typedef struct yrh_shading_desc {
+    const yrh_material* materials;
+    int32_t material_count;
+    const yrh_emissive* emissives;
+    int32_t emissive_count;
+    /* LightSamplingCache::emissive_power (src/lighting/light_sampling.cpp:148-216) */
+    const float* alias_probability;
+    const int32_t* alias_index;
+    const float* alias_pmf;
+    yrh_environment_desc environment;
+} yrh_shading_desc;
+
+/* Uploads materials, emitters and the environment for the current geometry
+ * (call after yrh_upload_geometry). */
+int yrh_upload_shading(yrh_context* ctx, const yrh_shading_desc* shading);
+
+/* ---- Wavefront path tracing ----------------------------------------------- */
+
+#define YRH_SAMPLER_INDEPENDENT 0
+#define YRH_SAMPLER_STRATIFIED 1
+#define YRH_SAMPLER_OWEN_SOBOL 2
+#define YRH_SAMPLER_ZSOBOL 3
+
+/* One pass = one sample for every pixel of the frame, the unit the reference
+ * commits per pass (src/backends/cpu/cpu_path_tracer.cpp:109-180). */
+typedef struct yrh_render_desc {
+    float camera_origin[3];
+    float camera_forward[3];
+    float camera_right[3];
+    float camera_up[3];
+    float tan_half_fov; /* std::tan(fov_y_radians * 0.5f), computed on the host */
+    int32_t width;
+    int32_t height;
+    int32_t sampler; /* YRH_SAMPLER_* */
+    int32_t spp;     /* samples per pixel of the whole render */
+    int32_t max_depth;
+    uint64_t seed;
+    float radiance_clamp;
+    int32_t sample_index; /* this pass's sample index (= completed spp) */
+} yrh_render_desc;
+
+/* yr::RenderStats counters (include/yaoray/runtime/backend.hpp:40-55) plus the
+ * film's bad (non-finite) sample count. */
+typedef struct yrh_render_stats {
+    uint64_t rays_traced;
+    uint64_t shadow_rays;
+    uint64_t occluded_shadow_rays;
+    uint64_t triangle_tests;
+    uint64_t sphere_tests;
+    uint64_t bvh_node_tests;
+    uint64_t hits;
+    uint64_t misses;
+    uint64_t bad_samples;
+} yrh_render_stats;
+
+/* Per-pixel FP64 accumulator: yr::FilmPixel (include/yaoray/film/film.hpp:19-28)
+ * with sums and Welford moments in double. */
+typedef struct yrh_film_pixel {
+    double sum[3];
+    double luminance_mean;
+    double luminance_m2;
+    double albedo_sum[3];
+    double normal_sum[3];
+    double depth_sum;
+    uint32_t samples;
+    uint32_t aov_samples;
+} yrh_film_pixel;
+
+/* Allocates (or re-allocates) the film and zeroes it and the render stats. */
+int yrh_film_reset(yrh_context* ctx, int32_t width, int32_t height);
+/* Loads a film (e.g. a resumed render) from host pixels. */
+int yrh_film_load(yrh_context* ctx, const yrh_film_pixel* pixels, int64_t count);
+/* Copies the film to the host (synchronises). */
+int yrh_film_read(yrh_context* ctx, yrh_film_pixel* pixels, int64_t count);
+/* Runs one pass on the context's stream, accumulating into the film. */
+int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* desc);
+/* Reads the accumulated counters (synchronises). */
+int yrh_render_stats_read(yrh_context* ctx, yrh_render_stats* stats);
+
+/* Batched sample generation: out[i] = Sampler(kind, seed, x[i], y[i], sample[i],
+ * spp).Sample1D/2D at absolute dimension dim[i] (src/sampling/sampler.cpp:122-237),
+ * computed on the device; dims = 1 or 2 floats per entry. Host arrays. */
+int yrh_sample_values(yrh_context* ctx, int32_t kind, uint64_t seed, int32_t spp, int64_t n, const int32_t* x,
+                      const int32_t* y, const int32_t* sample, const int32_t* dim, int32_t dims, float* out);
+
+/* Device ray generation for one pass (GeneratePerspectiveCameraRay with the
+ * sampler's Pixel dimension): writes width*height rays into device arrays. */
+int yrh_generate_camera_rays(yrh_context* ctx, const yrh_render_desc* desc, float* org_tmin, float* dir_tmax);
+
 /* ---- Timing on the context's stream --------------------------------------- */
 
 /* HIP events recorded on the context's stream; elapsed milliseconds between

@@ -205,8 +205,88 @@ class RefScene:
                 "triangle_tests": tt}
 
 
+STAT_KEYS = ("rays_traced", "shadow_rays", "occluded_shadow_rays", "triangle_tests", "sphere_tests",
+             "bvh_node_tests", "hits", "misses", "bad_samples")
+
+
+def render(scene, width, height, spp, max_depth=8, sampler=2, seed=0x59414F524159, radiance_clamp=0.0,
+           threads=8, first_sample=0):
+    """The restated CPU path tracer on the scene's own C-ABI descriptors.
+    Returns (FP32 film records, stats dict)."""
+    from yaoray_amd.api import FILM32_DTYPE
+
+    lib = oracle_lib()
+    if not hasattr(lib, "_render_sig"):
+        lib.oracle_render.argtypes = [_VP, _VP, _VP, _I32, _I32, _I32, _VP, _VP]
+        lib.oracle_sample_values.argtypes = [_I32, C.c_uint64, _I32, _I64, _VP, _VP, _VP, _VP, _I32, _VP]
+        lib._render_sig = True
+    gd = scene.geometry_desc()
+    sd = scene.shading_desc()
+    rd = scene.render_desc(width, height, spp, max_depth, sampler, seed, radiance_clamp, 0)
+    film = np.zeros(width * height, FILM32_DTYPE)
+    stats = np.zeros(9, np.uint64)
+    rc = lib.oracle_render(C.cast(C.pointer(gd), _VP), C.cast(C.pointer(sd), _VP), C.cast(C.pointer(rd), _VP),
+                           first_sample, spp - first_sample, threads, film.ctypes.data, stats.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("oracle_render failed")
+    return film, dict(zip(STAT_KEYS, (int(x) for x in stats)))
+
+
+def sample_values(kind, seed, spp, x, y, sample, dim, dims):
+    lib = oracle_lib()
+    lib.oracle_sample_values.argtypes = [_I32, C.c_uint64, _I32, _I64, _VP, _VP, _VP, _VP, _I32, _VP]
+    arrs = [np.ascontiguousarray(a, np.int32) for a in (x, y, sample, dim)]
+    out = np.zeros(len(x) * dims, np.float32)
+    lib.oracle_sample_values(kind, seed, spp, len(x), *[a.ctypes.data for a in arrs], dims, out.ctypes.data)
+    return out.reshape(len(x), dims) if dims == 2 else out
+
+
+def ref_render(scene, width, height, spp, max_depth=8, sampler=2, seed=0x59414F524159, radiance_clamp=0.0,
+               threads=8):
+    """The reference's own PrepareCpuScene + RenderCpuPathTrace on the same inputs.
+    Returns (FP32 film records, stats dict, render seconds)."""
+    from yaoray_amd.api import FILM32_DTYPE
+
+    lib = ref_lib()
+    lib.ref_render.argtypes = [_VP, _I64, _VP, _I64, _VP, _I32, _VP, _VP, _I32, _VP, _VP, _VP, _I32, _VP, _VP,
+                               C.POINTER(C.c_double)]
+    v, i, p = scene.geometry()
+    info = scene.info
+    if info.instance_count:
+        _, _, inst = scene.tlas()
+        ip = np.ascontiguousarray(inst["primitive"], np.int32)
+        im = np.ascontiguousarray(inst["object_to_world"], np.float32)
+    else:
+        ip = np.zeros(1, np.int32)
+        im = np.zeros((1, 16), np.float32)
+    sd = scene.shading_desc()
+    rd = scene.render_desc(width, height, spp, max_depth, sampler, seed, radiance_clamp, 0)
+    cam = np.array(list(info.camera), np.float32)
+    film = np.zeros(width * height, FILM32_DTYPE)
+    stats = np.zeros(10, np.uint64)
+    secs = C.c_double()
+    rc = lib.ref_render(_p(v), len(v), _p(i), len(i), _p(p), len(p), ip.ctypes.data, im.ctypes.data,
+                        info.instance_count, C.cast(C.pointer(sd), _VP), cam.ctypes.data, C.cast(C.pointer(rd), _VP),
+                        threads, film.ctypes.data, stats.ctypes.data, C.byref(secs))
+    if rc != 0:
+        raise RuntimeError(f"reference render failed ({rc})")
+    out = dict(zip(STAT_KEYS, (int(x) for x in stats[:9])))
+    out["threads"] = int(stats[9])
+    return film, out, secs.value
+
+
 def ref_inverse(m):
     m = np.ascontiguousarray(m, np.float32).reshape(16)
     out = np.zeros(16, np.float32)
     ref_lib().ref_inverse(m.ctypes.data, out.ctypes.data)
     return out
+
+
+def camera_rays(render_desc):
+    lib = oracle_lib()
+    lib.oracle_camera_rays.argtypes = [_VP, _VP, _VP]
+    n = render_desc.width * render_desc.height
+    org = np.zeros((n, 4), np.float32)
+    dirs = np.zeros((n, 4), np.float32)
+    lib.oracle_camera_rays(C.cast(C.pointer(render_desc), _VP), org.ctypes.data, dirs.ctypes.data)
+    return org, dirs

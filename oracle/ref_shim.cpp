@@ -6,8 +6,13 @@
 // node arrays), and as the reference CPU baseline. Builds against the
 // reference's headers; contains no reference code.
 #include <yaoray/accel/acceleration.hpp>
+#include <yaoray/backends/cpu/cpu_path_tracer.hpp>
+#include <yaoray/backends/cpu/cpu_prepared_scene.hpp>
 #include <yaoray/core/transform.hpp>
+#include <yaoray/lighting/environment.hpp>
 #include <yaoray/scene/render_scene.hpp>
+
+#include "../include/yrh.h"
 
 #include <algorithm>
 #include <atomic>
@@ -182,6 +187,99 @@ int ref_trace(void* h, const float* org_tmin, const float* dir_tmax, int64_t n, 
         for (int t = 0; t < threads; ++t) pool.emplace_back(work);
         for (auto& th : pool) th.join();
     }
+    return 0;
+}
+
+// The reference's own render: RenderSceneIR built from the same inputs the GPU
+// backend receives, PrepareCpuScene (light tables, BVH) and RenderCpuPathTrace
+// for settings.spp passes. film: width*height FilmPixel (56 B) records.
+// stats: rays, shadow, occluded, triangle tests, sphere tests, node tests,
+// hits, misses, bad samples, threads; seconds = render seconds (excludes prepare).
+int ref_render(const yrh_vertex* vertices, int64_t nv, const uint32_t* indices, int64_t ni,
+               const yrh_primitive* primitives, int32_t np, const int32_t* inst_prim, const float* inst_mat,
+               int32_t ninst, const yrh_shading_desc* sd, const float* camera /* origin, forward, right, up, fov */,
+               const yrh_render_desc* rd, int32_t threads, void* film_out, uint64_t* stats, double* seconds) {
+    yr::RenderJob job;
+    yr::RenderSceneIR& sc = job.scene;
+    sc.vertices.resize(static_cast<size_t>(nv));
+    if (nv > 0) std::memcpy(sc.vertices.data(), vertices, static_cast<size_t>(nv) * 48);
+    sc.indices.assign(indices, indices + ni);
+    sc.primitives.resize(static_cast<size_t>(np));
+    if (np > 0) std::memcpy(sc.primitives.data(), primitives, static_cast<size_t>(np) * 16);
+    for (int32_t i = 0; i < ninst; ++i) {
+        yr::RenderInstance inst;
+        inst.primitive = yr::MeshPrimitiveHandle{inst_prim[i]};
+        std::memcpy(inst.object_to_world.m.data(), inst_mat + 16 * static_cast<size_t>(i), 64);
+        sc.instances.push_back(inst);
+    }
+    for (int32_t k = 0; k < sd->material_count; ++k) {
+        const yrh_material& m = sd->materials[k];
+        yr::RenderMaterial rm;
+        rm.kind = m.kind == YRH_MATERIAL_DIFFUSE ? yr::RenderMaterialKind::Diffuse
+                : m.kind == YRH_MATERIAL_CONDUCTOR ? yr::RenderMaterialKind::Conductor
+                : m.kind == YRH_MATERIAL_DIELECTRIC ? yr::RenderMaterialKind::Dielectric
+                                                    : yr::RenderMaterialKind::ThinDielectric;
+        rm.reflectance.value = yr::Color3f{m.reflectance[0], m.reflectance[1], m.reflectance[2]};
+        rm.ior = m.ior;
+        rm.uroughness.value = m.uroughness;
+        rm.vroughness.value = m.vroughness;
+        rm.alpha.value = m.alpha;
+        rm.emission = yr::Color3f{m.emission[0], m.emission[1], m.emission[2]};
+        rm.absorption_color = yr::Color3f{m.absorption_color[0], m.absorption_color[1], m.absorption_color[2]};
+        rm.absorption_distance = m.absorption_distance;
+        sc.materials.push_back(rm);
+    }
+    for (int32_t k = 0; k < sd->emissive_count; ++k) {
+        const yrh_emissive& e = sd->emissives[k];
+        sc.emissive_primitives.push_back(
+            yr::EmissivePrimitive{e.primitive_index, yr::Color3f{e.radiance[0], e.radiance[1], e.radiance[2]}, e.area});
+    }
+    const yrh_environment_desc& env = sd->environment;
+    sc.environment.active = env.active != 0;
+    sc.environment.radiance = yr::Color3f{env.radiance[0], env.radiance[1], env.radiance[2]};
+    sc.environment.strength = env.strength;
+    sc.environment.rotation_radians = env.rotation_radians;
+    if (env.texels != nullptr && env.width > 0 && env.height > 0) {
+        yr::RenderTexture tex;
+        tex.width = env.width;
+        tex.height = env.height;
+        tex.texels.resize(static_cast<size_t>(env.width) * static_cast<size_t>(env.height));
+        std::memcpy(tex.texels.data(), env.texels, tex.texels.size() * 16);
+        sc.textures.push_back(std::move(tex));
+        sc.environment.texture_index = 0;
+        if (env.has_distribution) {
+            sc.environment_distributions.push_back(yr::BuildEnvironmentDistribution(sc.textures[0]));
+            sc.environment.distribution_index = 0;
+        }
+    }
+    sc.camera.origin = yr::Point3f{camera[0], camera[1], camera[2]};
+    sc.camera.forward = yr::Vec3f{camera[3], camera[4], camera[5]};
+    sc.camera.right = yr::Vec3f{camera[6], camera[7], camera[8]};
+    sc.camera.up = yr::Vec3f{camera[9], camera[10], camera[11]};
+    sc.camera.fov_y_radians = camera[12];
+    yr::RenderSettings& st = job.settings;
+    st.width = rd->width;
+    st.height = rd->height;
+    st.spp = rd->spp;
+    st.max_depth = rd->max_depth;
+    st.seed = rd->seed;
+    st.threads = threads;
+    st.radiance_clamp = rd->radiance_clamp;
+    st.sampler = rd->sampler == YRH_SAMPLER_INDEPENDENT ? yr::RenderSamplerKind::Independent
+               : rd->sampler == YRH_SAMPLER_STRATIFIED ? yr::RenderSamplerKind::Stratified
+               : rd->sampler == YRH_SAMPLER_OWEN_SOBOL ? yr::RenderSamplerKind::OwenSobol
+                                                       : yr::RenderSamplerKind::ZSobol;
+    yr::CpuPrepareResult prep = yr::PrepareCpuScene(std::move(job));
+    if (!prep.ok) return -1;
+    const yr::CpuPathTraceResult r = yr::RenderCpuPathTrace(*prep.scene);
+    if (!r.ok) return -2;
+    static_assert(sizeof(yr::FilmPixel) == 56);
+    std::memcpy(film_out, r.film.Pixels().data(), r.film.Pixels().size() * 56);
+    const uint64_t vals[10] = {r.stats.rays_traced, r.stats.shadow_rays, r.stats.occluded_shadow_rays,
+                               r.stats.triangle_tests, r.stats.sphere_tests, r.stats.bvh_node_tests, r.stats.hits,
+                               r.stats.misses, r.film.BadSampleCount(), static_cast<uint64_t>(r.stats.threads)};
+    std::memcpy(stats, vals, sizeof(vals));
+    if (seconds != nullptr) *seconds = r.stats.elapsed_seconds;
     return 0;
 }
 

@@ -93,3 +93,19 @@ def trs(translate, axis, angle_deg, scale):
     m[:3, :3] = r @ np.diag(np.broadcast_to(np.asarray(scale, np.float64), 3))
     m[:3, 3] = translate
     return m.T.reshape(16).astype(np.float32)  # column-major
+
+
+def base_path_scene():
+    """MakeBasePathJob (tests/backends/cpu/cpu_path_tracer_test_support.hpp:40-66):
+    one diffuse triangle facing a pinhole camera, constant environment."""
+    m = MeshArrays()
+    m.vertices = [((-1.5, -1.0, 0.0), (0.0, 0.0, 1.0)), ((1.5, -1.0, 0.0), (0.0, 0.0, 1.0)),
+                  ((0.0, 1.25, 0.0), (0.0, 0.0, 1.0))]
+    m.indices = [0, 1, 2]
+    m.primitives = [(0, 3, 0, 1)]
+    s = m.scene()
+    s.set_camera((0.0, 0.0, 3.0), (0.0, 0.0, -1.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), 0.8)
+    s.set_materials([{"kind": 0, "reflectance": (0.8, 0.8, 0.8)}])
+    s.set_constant_environment((0.02, 0.03, 0.04), 1.0)
+    s.build_accel()
+    return s

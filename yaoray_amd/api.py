@@ -27,6 +27,15 @@ INSTANCE_DTYPE = np.dtype([("primitive", "<i4"), ("blas_index", "<i4"), ("object
 TLAS_PRIM_DTYPE = np.dtype([("kind", "<i4"), ("instance", "<i4"), ("sphere", "<i4"),
                             ("world_bounds_min", "<f4", 3), ("world_bounds_max", "<f4", 3)])
 
+# yrh_film_pixel (FP64 sums and Welford moments) and yr::FilmPixel (FP32, 56 bytes).
+FILM64_DTYPE = np.dtype([("sum", "<f8", 3), ("luminance_mean", "<f8"), ("luminance_m2", "<f8"),
+                         ("albedo_sum", "<f8", 3), ("normal_sum", "<f8", 3), ("depth_sum", "<f8"),
+                         ("samples", "<u4"), ("aov_samples", "<u4")])
+FILM32_DTYPE = np.dtype([("sum", "<f4", 3), ("samples", "<u4"), ("luminance_mean", "<f4"), ("luminance_m2", "<f4"),
+                         ("albedo_sum", "<f4", 3), ("normal_sum", "<f4", 3), ("depth_sum", "<f4"),
+                         ("aov_samples", "<u4")])
+assert FILM64_DTYPE.itemsize == 104 and FILM32_DTYPE.itemsize == 56
+
 assert NODE_DTYPE.itemsize == 32 and PRIM_DTYPE.itemsize == 20 and VERTEX_DTYPE.itemsize == 48
 assert PRIMITIVE_DTYPE.itemsize == 16 and INSTANCE_DTYPE.itemsize == 160 and TLAS_PRIM_DTYPE.itemsize == 36
 
@@ -190,6 +199,95 @@ class Scene:
             raise YrhError("yrhost_camera_rays failed")
         return org, dirs
 
+    # -- edits for in-code test jobs ---------------------------------------------
+    def set_camera(self, origin, forward, right, up, fov_y_radians):
+        c = np.array([*origin, *forward, *right, *up, fov_y_radians], np.float32)
+        self._lib.yrhost_scene_set_camera(self._h, c.ctypes.data)
+
+    def set_materials(self, materials):
+        """materials: list of dicts with keys of native.Material (kind, reflectance, emission, ...)."""
+        arr = (N.Material * len(materials))()
+        for k, m in enumerate(materials):
+            a = arr[k]
+            a.kind = m.get("kind", 0)
+            a.reflectance[:] = m.get("reflectance", (0.5, 0.5, 0.5))
+            a.ior = m.get("ior", 1.5)
+            a.uroughness = m.get("uroughness", 0.0)
+            a.vroughness = m.get("vroughness", m.get("uroughness", 0.0))
+            a.alpha = m.get("alpha", 1.0)
+            a.emission[:] = m.get("emission", (0.0, 0.0, 0.0))
+            a.absorption_distance = m.get("absorption_distance", 1.0)
+            a.absorption_color[:] = m.get("absorption_color", (1.0, 1.0, 1.0))
+        if self._lib.yrhost_scene_set_materials(self._h, C.cast(arr, C.c_void_p), len(materials)) != 0:
+            raise YrhError("set_materials failed")
+
+    def set_emissives(self, emissives):
+        """emissives: list of (primitive_index, (r, g, b), area)."""
+        arr = (N.Emissive * max(1, len(emissives)))()
+        for k, (prim, rad, area) in enumerate(emissives):
+            arr[k].primitive_index = prim
+            arr[k].radiance[:] = rad
+            arr[k].area = area
+        self._lib.yrhost_scene_set_emissives(self._h, C.cast(arr, C.c_void_p), len(emissives))
+
+    def set_constant_environment(self, radiance, strength=1.0):
+        r = np.array(radiance, np.float32)
+        self._lib.yrhost_scene_set_constant_environment(self._h, r.ctypes.data, strength)
+
+    def geometry_desc(self) -> N.GeometryDesc:
+        """The C-ABI geometry descriptor (pointers into this scene; keep the scene alive)."""
+        g = N.GeometryDesc()
+        if self._lib.yrhost_geometry_desc(self._h, C.byref(g)) != 0:
+            raise YrhError("no acceleration structure")
+        return g
+
+    def shading_desc(self) -> N.ShadingDesc:
+        d = N.ShadingDesc()
+        if self._lib.yrhost_shading_desc(self._h, C.byref(d)) != 0:
+            raise YrhError(self._lib.yrhost_scene_error(self._h).decode())
+        return d
+
+    def render_desc(self, width, height, spp, max_depth=8, sampler=N.SAMPLER_OWEN_SOBOL, seed=0x59414F524159,
+                    radiance_clamp=0.0, sample_index=0) -> N.RenderDesc:
+        d = N.RenderDesc()
+        if self._lib.yrhost_render_desc(self._h, width, height, sampler, spp, max_depth, seed, radiance_clamp,
+                                        sample_index, C.byref(d)) != 0:
+            raise YrhError("yrhost_render_desc failed")
+        return d
+
+    def backend_render(self, width, height, spp, max_depth=8, sampler=N.SAMPLER_OWEN_SOBOL, seed=0x59414F524159,
+                       radiance_clamp=0.0, device=0, requested_backend=0, resume_film=None,
+                       resume_completed_spp=0, stop_before_start=False, cancel_after_passes=0, cancel_error=None):
+        """yr::HipBackend Prepare + Render (the reference's RenderBackend flow).
+
+        Returns (report dict, film or None); the film is a FILM32_DTYPE
+        (yr::FilmPixel) array of shape (height, width)."""
+        st = N.RenderSettings(width, height, sampler, spp, max_depth, radiance_clamp, seed, device,
+                              requested_backend)
+        rq = N.RenderRequest()
+        keep = None
+        if resume_film is not None:
+            keep = np.ascontiguousarray(resume_film, dtype=FILM32_DTYPE)
+            rq.resume_film = keep.ctypes.data
+            rq.resume_height, rq.resume_width = keep.shape
+        rq.resume_completed_spp = resume_completed_spp
+        rq.stop_before_start = 1 if stop_before_start else 0
+        rq.cancel_after_passes = cancel_after_passes
+        rq.cancel_error = cancel_error.encode() if cancel_error else None
+        cap = max(0, width) * max(0, height)
+        film = np.zeros(cap, FILM32_DTYPE)
+        rep = N.RenderReport()
+        rc = self._lib.yrhost_backend_render(self._h, C.byref(st), C.byref(rq), film.ctypes.data if cap else None,
+                                             cap, C.byref(rep))
+        if rc != 0:
+            raise YrhError(f"yrhost_backend_render rejected its arguments ({rc})")
+        report = {k: getattr(rep, k) for k, _ in N.RenderReport._fields_ if k != "error"}
+        report["error"] = rep.error.decode()
+        del keep
+        if not rep.film_valid:
+            return report, None
+        return report, film[: rep.film_width * rep.film_height].reshape(rep.film_height, rep.film_width)
+
     @property
     def handle(self):
         return self._h
@@ -328,6 +426,58 @@ class Device:
         self._check(self._lib.yrh_trace_any(self._ctx, C.byref(rays), n, occluded.ptr,
                                             node_tests.ptr if node_tests else None,
                                             tri_tests.ptr if tri_tests else None), "trace_any")
+
+    # -- wavefront path tracing ----------------------------------------------
+    def upload_shading(self, scene: Scene):
+        self._check(N.host().yrhost_upload_shading(scene.handle, self._ctx), "upload_shading")
+
+    def film_reset(self, width: int, height: int):
+        self._check(self._lib.yrh_film_reset(self._ctx, width, height), "film_reset")
+
+    def render_pass(self, desc: N.RenderDesc):
+        self._check(self._lib.yrh_render_pass(self._ctx, C.byref(desc)), "render_pass")
+
+    def film_read(self, width: int, height: int) -> np.ndarray:
+        out = np.zeros(width * height, FILM64_DTYPE)
+        self._check(self._lib.yrh_film_read(self._ctx, out.ctypes.data, len(out)), "film_read")
+        return out
+
+    def film_load(self, pixels: np.ndarray):
+        pixels = np.ascontiguousarray(pixels, FILM64_DTYPE)
+        self._check(self._lib.yrh_film_load(self._ctx, pixels.ctypes.data, len(pixels)), "film_load")
+
+    def render_stats(self) -> dict:
+        s = N.RenderStats()
+        self._check(self._lib.yrh_render_stats_read(self._ctx, C.byref(s)), "render_stats_read")
+        return {k: int(getattr(s, k)) for k, _ in N.RenderStats._fields_}
+
+    def render(self, scene: Scene, width, height, spp, max_depth=8, sampler=N.SAMPLER_OWEN_SOBOL,
+               seed=0x59414F524159, radiance_clamp=0.0, first_sample=0, upload=True):
+        """Uploads the scene (unless upload=False), renders passes [first_sample, spp) and returns
+        (FP64 film records, stats)."""
+        if upload:
+            self.upload_scene(scene)
+            self.upload_shading(scene)
+        self.film_reset(width, height)
+        for s in range(first_sample, spp):
+            self.render_pass(scene.render_desc(width, height, spp, max_depth, sampler, seed, radiance_clamp, s))
+        self.synchronize()
+        return self.film_read(width, height), self.render_stats()
+
+    def sample_values(self, kind, seed, spp, x, y, sample, dim, dims):
+        n = len(x)
+        arrs = [np.ascontiguousarray(a, np.int32) for a in (x, y, sample, dim)]
+        out = np.zeros(n * dims, np.float32)
+        self._check(self._lib.yrh_sample_values(self._ctx, kind, seed, spp, n, *[a.ctypes.data for a in arrs], dims,
+                                                out.ctypes.data), "sample_values")
+        return out.reshape(n, dims) if dims == 2 else out
+
+    def camera_rays(self, desc: N.RenderDesc):
+        n = desc.width * desc.height
+        d_org, d_dir = self.alloc(16 * n), self.alloc(16 * n)
+        self._check(self._lib.yrh_generate_camera_rays(self._ctx, C.byref(desc), d_org.ptr, d_dir.ptr), "camera_rays")
+        self.synchronize()
+        return d_org.download(np.float32, (n, 4)), d_dir.download(np.float32, (n, 4))
 
     # -- host convenience ---------------------------------------------------
     def trace_closest(self, org: np.ndarray, dirs: np.ndarray, counters: bool = False):

@@ -16,8 +16,9 @@ ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "lib")
 OBJ = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["yrh_context.hip", "yrh_trace.hip"]
-HOST_SOURCES = ["transform.cpp", "bvh_builder.cpp", "light_tables.cpp", "scenes.cpp", "host_capi.cpp"]
+HIP_SOURCES = ["yrh_context.hip", "yrh_trace.hip", "yrh_render.hip"]
+HOST_SOURCES = ["transform.cpp", "bvh_builder.cpp", "light_tables.cpp", "scenes.cpp", "abi_marshal.cpp",
+                "hip_backend.cpp", "host_capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 ops are disabled: hipcc would otherwise emit v_pk_* f32 forms.

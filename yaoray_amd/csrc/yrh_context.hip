@@ -233,6 +233,7 @@ void yrh_destroy(yrh_context* ctx) {
     if (ctx == nullptr) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    DestroyRenderState(ctx);
     ctx->geo = DevGeometry{};
     ctx->work_counter.reset();
     ctx->stack_spill.reset();
@@ -449,6 +450,7 @@ int yrh_upload_geometry(yrh_context* ctx, const yrh_geometry_desc* g) {
     if ((rc = Upload(ctx, geo.nodes, packed.nodes)) != YRH_OK) return rc;
     if ((rc = Upload(ctx, geo.slots, packed.slots)) != YRH_OK) return rc;
     if ((rc = Upload(ctx, geo.leaf_count, packed.leaf_count)) != YRH_OK) return rc;
+    if ((rc = UploadSurfaceGeometry(ctx, g)) != YRH_OK) return rc;
     geo.loaded = true;
     return YRH_OK;
 }

@@ -111,6 +111,8 @@ struct DevGeometry {
     int32_t stack_capacity = 0;  // entries needed by the deepest traversal
 };
 
+struct RenderState;  // yrh_render.hip
+
 }  // namespace yrh
 
 struct yrh_context {
@@ -128,12 +130,36 @@ struct yrh_context {
     yrh::DevBuf stack_spill;    // int2 per lane per spilled stack level
     int blocks_per_cu = 4;
     int block_size = 256;
+    yrh::RenderState* render = nullptr;  // surface/light data, film, path state (yrh_render.hip)
 };
 
 namespace yrh {
 
 int Fail(yrh_context* ctx, int code, const std::string& message);
 int CheckHip(yrh_context* ctx, hipError_t status, const char* what);
+
+// One launch of a traversal kernel over a ray queue (yrh_trace.hip).
+struct TraceRequest {
+    bool any = false;
+    const float4* org = nullptr;
+    const float4* dir = nullptr;
+    int64_t n_host = 0;                  // ray count when n_device is null
+    const uint32_t* n_device = nullptr;  // device-resident ray count (read by the kernel)
+    int64_t capacity = 0;                // upper bound of *n_device (sizes the grid)
+    uint32_t* work_counter = nullptr;    // zeroed ray counter owned by the caller; null = context's
+    float4* hit = nullptr;
+    int32_t* instance = nullptr;
+    uint32_t* node_tests = nullptr;
+    uint32_t* triangle_tests = nullptr;
+    uint32_t* occluded = nullptr;
+    unsigned long long* totals = nullptr;  // [0] node tests, [1] triangle tests
+};
+int LaunchTrace(yrh_context* ctx, const TraceRequest& r);
+
+// Render-side state (yrh_render.hip): shading copy of the geometry, taken at
+// geometry upload; released with the context.
+int UploadSurfaceGeometry(yrh_context* ctx, const yrh_geometry_desc* g);
+void DestroyRenderState(yrh_context* ctx);
 
 // Launches of the traversal kernels (yrh_trace.hip).
 int LaunchTraceClosest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, const yrh_hit_soa* hits);

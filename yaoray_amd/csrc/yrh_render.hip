@@ -1,0 +1,1520 @@
+// Wavefront path tracer for MI355X (gfx950): the GPU counterpart of
+// yr::RenderCpuPathTrace (src/backends/cpu/cpu_path_tracer.cpp:39-229) driving
+// yr::IntegratePathSample (src/integrators/path_integrator.cpp:23-197).
+//
+// One pass = one sample for every pixel. Per bounce, with every count kept in
+// device memory (no host synchronisation inside a pass):
+//   trace    persistent closest-hit kernel over the compacted ray queue
+//   shade    per path: miss/environment, emission x MIS, next-event estimation
+//            (emissive + environment samples -> shadow queue), BSDF sampling,
+//            Russian roulette, next ray appended to the other queue
+//   shadow   any-hit kernel over the shadow queue (opaque scenes), or the
+//            reference's transparent-shadow loop (dielectric scenes)
+//   resolve  per path: adds the visible NEE contributions in reference order
+// and finally `accumulate` adds each path's sample to the FP64 film.
+// Path state is SoA in HBM; queues are appended with one atomic per wave.
+#include "yrh_internal.h"
+#include "yrh_shading.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#pragma clang fp contract(off)
+
+namespace yrh {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxTransparentShadowHits = 16;  // src/integrators/path_direct_lighting.cpp:19
+
+// Counter words of one pass (uint32).
+enum Counter : int {
+    kCountQueue0 = 0,
+    kCountQueue1 = 1,
+    kCountShadow = 2,
+    kCountLoop0 = 3,  // transparent-shadow loop queue counts (two, ping-pong)
+    kCountLoop1 = 4,
+    kWorkExtend = 5,  // per-launch ray counters of the trace kernels
+    kWorkShadow = 6,
+    kWorkLoop = 7,    // kWorkLoop + iteration
+    kCounterWords = kWorkLoop + kMaxTransparentShadowHits + 1,
+};
+
+// 64-bit statistics words.
+enum Stat : int {
+    kStatRays = 0,
+    kStatShadow,
+    kStatOccluded,
+    kStatTriTests,
+    kStatSphereTests,
+    kStatNodeTests,
+    kStatHits,
+    kStatMisses,
+    kStatBad,
+    kStatTraceNodes,  // scratch totals written by the trace kernels: [node, tri]
+    kStatTraceTris,
+    kStatWords,
+};
+
+}  // namespace
+
+struct RenderState {
+    // ---- surface geometry (shading copy) ----
+    DevBuf positions;    // float4 per vertex
+    DevBuf normals;      // float4 per vertex
+    DevBuf indices;      // uint32
+    DevBuf tri_prim;     // int32 per flat triangle: mesh primitive
+    DevBuf prim_info;    // int4 per primitive: material, emissive index, first index, flat offset
+    DevBuf prim_flags;   // int32 per primitive: bit0 has_normals
+    DevBuf inst_normal;  // float4[6] per instance: normal-matrix rows (+inv det, singular), o2w rows
+    int32_t primitive_count = 0;
+    int64_t triangle_count = 0;
+    int64_t vertex_count = 0;
+    std::vector<int32_t> host_prim_first;  // first index per primitive (emitter tables)
+    std::vector<int32_t> host_prim_count;
+    std::vector<float> host_positions;     // xyz per vertex (emitter tables)
+    std::vector<uint32_t> host_indices;
+    std::vector<int32_t> host_prim_material;
+    // ---- shading ----
+    bool shading_loaded = false;
+    bool transparent = false;  // any dielectric / thin dielectric material
+    DevBuf materials;          // DevMaterial
+    int32_t material_count = 0;
+    DevBuf emit;               // float4 (radiance, stored area)
+    DevBuf emit_info;          // int4 (primitive, cdf offset, cdf count, 0)
+    DevBuf emit_area;          // float: summed triangle area (SelectTriangleByArea)
+    DevBuf emit_cdf;           // float: running triangle areas per emitter
+    DevBuf alias_prob, alias_index, alias_pmf;
+    int32_t emissive_count = 0;
+    float total_emissive_area = 0.0f;
+    // environment
+    int32_t env_active = 0;
+    float env_radiance[3] = {0, 0, 0};
+    float env_strength = 1.0f;
+    float env_rotation = 0.0f;
+    DevBuf env_texels;  // float4
+    int32_t env_w = 0, env_h = 0;
+    int32_t env_sampleable = 0;
+    DevBuf env_weights, env_row_weights, env_row_cdf, env_cond_cdf;
+    int32_t env_dw = 0, env_dh = 0;
+    float env_total = 0.0f;
+    // ---- film ----
+    DevBuf film;  // yrh_film_pixel
+    int32_t film_w = 0, film_h = 0;
+    // ---- pass buffers (capacity = paths per pass) ----
+    int64_t capacity = 0;
+    DevBuf p_pixel, p_tp, p_rad, p_prev, p_flags, p_med, p_alb, p_nrm, p_nee, p_slot;
+    DevBuf q_org[2], q_dir[2], q_path[2];
+    DevBuf hits, hit_inst;
+    DevBuf s_org, s_dir, s_li, s_bsdf, s_occ, s_trans;
+    DevBuf l_entry[2], l_org[2], l_dir[2], l_hit, l_state, l_med;
+    DevBuf counters;  // uint32[kCounterWords]
+    DevBuf stats;     // uint64[kStatWords]
+};
+
+namespace {
+
+template <typename T>
+int UploadVec(yrh_context* ctx, DevBuf& buf, const T* data, size_t count) {
+    buf.reset();
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    int rc = CheckHip(ctx, hipMalloc(&buf.ptr, bytes), "hipMalloc");
+    if (rc != YRH_OK) return rc;
+    buf.bytes = bytes;
+    if (count > 0) rc = CheckHip(ctx, hipMemcpy(buf.ptr, data, count * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+    return rc;
+}
+
+int Alloc(yrh_context* ctx, DevBuf& buf, size_t bytes) {
+    buf.reset();
+    bytes = std::max<size_t>(bytes, 16);
+    int rc = CheckHip(ctx, hipMalloc(&buf.ptr, bytes), "hipMalloc");
+    if (rc == YRH_OK) buf.bytes = bytes;
+    return rc;
+}
+
+RenderState& State(yrh_context* ctx) {
+    if (ctx->render == nullptr) ctx->render = new RenderState();
+    return *ctx->render;
+}
+
+// ---- kernel parameter blocks ------------------------------------------------------
+
+struct SurfaceView {
+    const float4* positions;
+    const float4* normals;
+    const uint32_t* indices;
+    const int32_t* tri_prim;
+    const int4* prim_info;
+    const int32_t* prim_flags;
+    const float4* inst_normal;
+};
+
+struct LightView {
+    const DevMaterial* materials;
+    const float4* emit;
+    const int4* emit_info;
+    const float* emit_area;
+    const float* emit_cdf;
+    const float* alias_prob;
+    const int32_t* alias_index;
+    const float* alias_pmf;
+    int32_t emissive_count;
+    float total_emissive_area;
+    int32_t env_active;
+    float3 env_radiance;
+    float env_strength;
+    float env_rotation;
+    const float4* env_texels;
+    int32_t env_w, env_h;
+    int32_t env_sampleable;
+    const float* env_weights;
+    const float* env_row_weights;
+    const float* env_row_cdf;
+    const float* env_cond_cdf;
+    int32_t env_dw, env_dh;
+    float env_total;
+    const float* positions_xyz;  // unused on device (kept for symmetry)
+};
+
+struct PassView {
+    int32_t width, height;
+    int32_t sampler, spp, max_depth, sample_index;
+    uint64_t seed;
+    float radiance_clamp;
+    DevCamera camera;
+    // path state
+    int32_t* p_pixel;
+    float4* p_tp;
+    float4* p_rad;
+    float4* p_prev;
+    int32_t* p_flags;
+    float4* p_med;
+    float4* p_alb;
+    float4* p_nrm;
+    float4* p_nee;
+    int2* p_slot;
+    // queues
+    float4* q_org[2];
+    float4* q_dir[2];
+    int32_t* q_path[2];
+    float4* hits;
+    int32_t* hit_inst;
+    float4* s_org;
+    float4* s_dir;
+    float4* s_li;
+    float4* s_bsdf;
+    uint32_t* s_occ;
+    float4* s_trans;
+    int32_t* l_entry[2];
+    float4* l_org[2];
+    float4* l_dir[2];
+    float4* l_hit;
+    int4* l_state;
+    float4* l_med;
+    uint32_t* counters;
+    unsigned long long* stats;
+    yrh_film_pixel* film;
+};
+
+enum PathFlag : int { kPrevValid = 1, kPrevDelta = 2, kMediumActive = 4, kPrimaryHit = 8 };
+
+__device__ __forceinline__ int Lane() { return static_cast<int>(threadIdx.x & 63u); }
+
+// Appends to a queue with one atomic per wave; returns the slot or -1.
+__device__ __forceinline__ int32_t WaveAppend(uint32_t* counter, bool want) {
+    const uint64_t mask = __ballot(want);
+    if (mask == 0) return -1;
+    const int lane = Lane();
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader);
+    if (!want) return -1;
+    return static_cast<int32_t>(base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull))));
+}
+
+// Wave-summed statistics, one atomic per wave (all lanes must call).
+__device__ __forceinline__ void WaveStat(unsigned long long* stat, unsigned v) {
+    unsigned long long s = v;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (Lane() == 0 && s != 0) atomicAdd(stat, s);
+}
+
+__device__ __forceinline__ float3 XYZ(float4 v) { return V(v.x, v.y, v.z); }
+__device__ __forceinline__ float4 F4(float3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+__device__ __forceinline__ float MaxC(float3 c) {
+    // std::max(c.x, std::max(c.y, c.z))
+    const float yz = c.y < c.z ? c.z : c.y;
+    return c.x < yz ? yz : c.x;
+}
+
+__device__ __forceinline__ DevMaterial LoadMaterial(const DevMaterial* m, int idx) { return m[idx]; }
+
+// TransformNormal (src/core/transform.cpp:46-73) with host-precomputed cofactors.
+__device__ __forceinline__ float3 TransformNormalDev(const float4* rows, float3 n) {
+    const float4 r0 = rows[0], r1 = rows[1], r2 = rows[2];
+    if (r1.w != 0.0f) {
+        const float4 m0 = rows[3], m1 = rows[4], m2 = rows[5];
+        const float3 fb = Normalize(V(m0.x * n.x + m0.y * n.y + m0.z * n.z, m1.x * n.x + m1.y * n.y + m1.z * n.z,
+                                      m2.x * n.x + m2.y * n.y + m2.z * n.z));
+        return LengthSq(fb) > 0.0f ? fb : Normalize(n);
+    }
+    const float inv = r0.w;
+    return Normalize(V((r0.x * n.x + r0.y * n.y + r0.z * n.z) * inv, (r1.x * n.x + r1.y * n.y + r1.z * n.z) * inv,
+                       (r2.x * n.x + r2.y * n.y + r2.z * n.z) * inv));
+}
+
+// ---- environment (src/lighting/environment.cpp:54-219) ------------------------------
+
+__device__ __forceinline__ float Fract(float v) { return v - floorf(v); }
+
+__device__ __forceinline__ float2 DirToUv(float3 d, float rot) {
+    d = Normalize(d);
+    const float phi = atan2f(d.x, d.z);
+    const float u = Fract(0.5f + (phi + rot) / kTwoPi);
+    const float theta = acosf(Clampf(d.y, -1.0f, 1.0f));
+    return make_float2(u, Clampf(theta / kPi, 0.0f, 1.0f));
+}
+
+__device__ __forceinline__ float3 UvToDir(float2 uv, float rot) {
+    const float phi = (uv.x - 0.5f) * kTwoPi - rot;
+    const float theta = Clampf(uv.y, 0.0f, 1.0f) * kPi;
+    const float st = sinf(theta);
+    return Normalize(V(st * sinf(phi), cosf(theta), st * cosf(phi)));
+}
+
+__device__ __forceinline__ int Wrap(int v, int count) {
+    int w = v % count;
+    if (w < 0) w += count;
+    return w;
+}
+
+__device__ __forceinline__ float ApplyRepeat(float v) {
+    const float base = floorf(v);
+    const float f = v - base;
+    return f < 0.0f ? f + 1.0f : f;
+}
+
+// SampleBilinear at level 0 with repeat wrap (src/shading/texture.cpp:116-130).
+__device__ __forceinline__ float3 Bilinear(const float4* tex, int w, int h, float2 uv) {
+    if (w <= 0 || h <= 0) return V(0, 0, 0);
+    if (w == 1 && h == 1) return XYZ(tex[0]);
+    const float s = ApplyRepeat(uv.x) * static_cast<float>(w) - 0.5f;
+    const float t = ApplyRepeat(uv.y) * static_cast<float>(h) - 0.5f;
+    const int s0 = static_cast<int>(floorf(s));
+    const int t0 = static_cast<int>(floorf(t));
+    const float ds = s - static_cast<float>(s0);
+    const float dt = t - static_cast<float>(t0);
+    const float4 c00 = tex[Wrap(t0, h) * w + Wrap(s0, w)];
+    const float4 c10 = tex[Wrap(t0, h) * w + Wrap(s0 + 1, w)];
+    const float4 c01 = tex[Wrap(t0 + 1, h) * w + Wrap(s0, w)];
+    const float4 c11 = tex[Wrap(t0 + 1, h) * w + Wrap(s0 + 1, w)];
+    const float3 a = Add(Mul(XYZ(c00), 1.0f - ds), Mul(XYZ(c10), ds));
+    const float3 b = Add(Mul(XYZ(c01), 1.0f - ds), Mul(XYZ(c11), ds));
+    return Add(Mul(a, 1.0f - dt), Mul(b, dt));
+}
+
+__device__ __forceinline__ float3 EnvColor(const LightView& L, float3 d) {
+    if (!L.env_active) return Mul(L.env_radiance, L.env_strength);
+    if (L.env_strength <= 0.0f || L.env_texels == nullptr) return V(0, 0, 0);
+    const float3 c = Bilinear(L.env_texels, L.env_w, L.env_h, DirToUv(d, L.env_rotation));
+    return Mul(Hadamard(c, L.env_radiance), L.env_strength);
+}
+
+__device__ __forceinline__ float EnvPdf(const LightView& L, float3 d) {
+    if (!L.env_sampleable) return 0.0f;
+    const int w = L.env_dw, h = L.env_dh;
+    if (w <= 0 || h <= 0 || L.env_total <= 0.0f) return 0.0f;
+    const float2 uv = DirToUv(d, L.env_rotation);
+    int x = static_cast<int>(floorf(uv.x * static_cast<float>(w)));
+    int y = static_cast<int>(floorf(uv.y * static_cast<float>(h)));
+    x = x < 0 ? 0 : (x > w - 1 ? w - 1 : x);
+    y = y < 0 ? 0 : (y > h - 1 ? h - 1 : y);
+    const float prob = L.env_weights[static_cast<int64_t>(y) * w + x] / L.env_total;
+    const float st = fmaxf(1.0e-6f, sinf((static_cast<float>(y) + 0.5f) * kPi / static_cast<float>(h)));
+    const float texel_sa = (kTwoPi / static_cast<float>(w)) * (kPi / static_cast<float>(h)) * st;
+    return prob / texel_sa;
+}
+
+// std::lower_bound over a non-decreasing float array.
+__device__ __forceinline__ int LowerBound(const float* a, int n, float target) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < target) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    return lo;
+}
+
+struct EnvSample {
+    float3 dir;
+    float3 radiance;
+    float pdf;
+    bool valid;
+};
+
+__device__ __forceinline__ EnvSample SampleEnv(const LightView& L, float2 s) {
+    EnvSample out;
+    out.valid = false;
+    out.pdf = 0.0f;
+    if (!L.env_sampleable) return out;
+    const int w = L.env_dw, h = L.env_dh;
+    if (w <= 0 || h <= 0 || L.env_total <= 0.0f) return out;
+    const float row_target = Clampf(s.y, 0.0f, kOneMinusEps) * L.env_total;
+    int y = LowerBound(L.env_row_cdf, h, row_target);
+    if (y == h) y = h - 1;
+    const float row_prev = y == 0 ? 0.0f : L.env_row_cdf[y - 1];
+    const float row_w = fmaxf(L.env_row_weights[y], 1.0e-6f);
+    const float row_frac = Clampf((row_target - row_prev) / row_w, 0.0f, 1.0f);
+    const float* cond = L.env_cond_cdf + static_cast<int64_t>(y) * w;
+    const float col_target = Clampf(s.x, 0.0f, kOneMinusEps) * row_w;
+    int x = LowerBound(cond, w, col_target);
+    if (x == w) x = w - 1;
+    const float col_prev = x == 0 ? 0.0f : cond[x - 1];
+    const float tex_w = fmaxf(L.env_weights[static_cast<int64_t>(y) * w + x], 1.0e-6f);
+    const float col_frac = Clampf((col_target - col_prev) / tex_w, 0.0f, 1.0f);
+    const float2 uv = make_float2((static_cast<float>(x) + col_frac) / static_cast<float>(w),
+                                  (static_cast<float>(y) + row_frac) / static_cast<float>(h));
+    const float3 d = UvToDir(uv, L.env_rotation);
+    const float pdf = EnvPdf(L, d);
+    if (pdf <= 0.0f) return out;
+    out.dir = d;
+    out.radiance = EnvColor(L, d);
+    out.pdf = pdf;
+    out.valid = true;
+    return out;
+}
+
+// ---- emissive lights (src/lighting/light_sampling.cpp:148-396) ------------------------
+
+__device__ __forceinline__ float ClampUnitSample(float s) { return isfinite(s) ? Clampf(s, 0.0f, 1.0f) : 0.0f; }
+
+__device__ __forceinline__ float TriArea(float3 a, float3 b, float3 c) {
+    const float area = Length(Cross(Sub(b, a), Sub(c, a))) * 0.5f;
+    return isfinite(area) && area > 0.0f ? area : 0.0f;
+}
+
+__device__ __forceinline__ float3 Pos(const SurfaceView& S, uint32_t vi) { return XYZ(S.positions[vi]); }
+
+struct LightSample {
+    float3 point, normal, radiance;
+    float pdf;
+    int index;
+    bool valid;
+};
+
+__device__ __forceinline__ LightSample SampleEmissive(const SurfaceView& S, const LightView& L, float select, float2 ts) {
+    LightSample out;
+    out.valid = false;
+    out.pdf = 0.0f;
+    out.index = -1;
+    const int n = L.emissive_count;
+    if (n == 0) return out;
+    // SampleAliasTable
+    const float scaled = fminf(ClampUnitSample(select), kOneMinusEps) * static_cast<float>(n);
+    const int column = min(static_cast<int>(scaled), n - 1);
+    const float fraction = scaled - static_cast<float>(column);
+    const float prob = L.alias_prob[column];
+    int index = column;
+    float remapped;
+    if (fraction < prob) {
+        remapped = prob > 0.0f ? fraction / prob : 0.0f;
+    } else {
+        index = L.alias_index[column];
+        remapped = prob < 1.0f ? (fraction - prob) / (1.0f - prob) : 0.0f;
+    }
+    remapped = Clampf(remapped, 0.0f, kOneMinusEps);
+    const float pmf = L.alias_pmf[index];
+    if (index < 0 || pmf <= 0.0f) return out;
+    // SampleEmissivePrimitive
+    const float4 e = L.emit[index];
+    const float area = isfinite(e.w) && e.w > 0.0f ? e.w : 0.0f;
+    if (area <= 0.0f) return out;
+    const int4 info = L.emit_info[index];
+    if (info.x < 0 || info.z <= 0) return out;
+    const float prim_area = L.emit_area[index];
+    if (!isfinite(prim_area) || prim_area <= 0.0f) return out;
+    // SelectTriangleByArea: first running area above the target (zero-area
+    // triangles never start a new running value, so they are never picked).
+    const float target = ClampUnitSample(remapped) * prim_area;
+    const float* cdf = L.emit_cdf + info.y;
+    int tri = 0;
+    {
+        int lo = 0, hi = info.z;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (!(target < cdf[mid])) {
+                lo = mid + 1;
+            } else {
+                hi = mid;
+            }
+        }
+        tri = lo;
+        if (tri >= info.z) {
+            // no running area exceeds the target: the last positive-area triangle
+            tri = info.z - 1;
+            while (tri > 0 && !(cdf[tri] > (tri > 0 ? cdf[tri - 1] : 0.0f))) --tri;
+        }
+    }
+    const int4 pi = S.prim_info[info.x];
+    const int64_t base = static_cast<int64_t>(pi.z) + static_cast<int64_t>(tri) * 3;
+    const float3 p0 = Pos(S, S.indices[base]), p1 = Pos(S, S.indices[base + 1]), p2 = Pos(S, S.indices[base + 2]);
+    const float su = sqrtf(ts.x);
+    const float u = 1.0f - su;
+    const float v = ts.y * su;
+    const float3 point = Add(Add(Mul(p0, 1.0f - u - v), Mul(p1, u)), Mul(p2, v));
+    const float3 nrm = Normalize(Cross(Sub(p1, p0), Sub(p2, p0)));
+    const float ta = TriArea(p0, p1, p2);
+    if (ta <= 0.0f || LengthSq(nrm) == 0.0f) return out;
+    out.point = point;
+    out.normal = nrm;
+    out.radiance = XYZ(e);
+    out.pdf = (1.0f / area) * pmf;
+    out.index = index;
+    out.valid = true;
+    return out;
+}
+
+__device__ __forceinline__ float EmissivePdfSolidAngle(const LightView& L, float3 shading, float3 light, float3 ln,
+                                                       int index) {
+    const int n = L.emissive_count;
+    if (n == 0) return 0.0f;
+    float area_pdf;
+    if (index >= 0 && index < n) {
+        const float w = L.emit[index].w;
+        const float area = isfinite(w) && w > 0.0f ? w : 0.0f;
+        if (area <= 0.0f) return 0.0f;
+        area_pdf = L.alias_pmf[index] / area;
+    } else {
+        const float total = L.total_emissive_area;
+        if (!isfinite(total) || total <= 0.0f) return 0.0f;
+        area_pdf = 1.0f / total;
+    }
+    const float3 dir = Sub(light, shading);
+    const float d2 = LengthSq(dir);
+    if (d2 <= 0.0f) return 0.0f;
+    const float c = fabsf(Dot(ln, Normalize(Sub(shading, light))));
+    if (c <= 0.0f) return 0.0f;
+    return area_pdf * d2 / c;
+}
+
+// ---- kernels -----------------------------------------------------------------------
+
+__global__ void RaygenKernel(PassView P) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t total = static_cast<int64_t>(P.width) * P.height;
+    if (p == 0) P.counters[kCountQueue0] = static_cast<uint32_t>(total);
+    if (p >= total) return;
+    const int x = static_cast<int>(p % P.width);
+    const int y = static_cast<int>(p / P.width);
+    PixelSampler smp;
+    smp.Init(P.sampler, P.seed, x, y, P.sample_index, P.spp);
+    const float2 s = smp.At2D(kDimPixel, 0);
+    const float w = static_cast<float>(P.width < 1 ? 1 : P.width);
+    const float h = static_cast<float>(P.height < 1 ? 1 : P.height);
+    const float3 d = CameraDirection(P.camera, w, h, static_cast<float>(x) + s.x, static_cast<float>(y) + s.y);
+    P.q_org[0][p] = F4(P.camera.origin, 1.0e-5f);
+    P.q_dir[0][p] = F4(d, __int_as_float(0x7f800000));
+    P.q_path[0][p] = static_cast<int32_t>(p);
+    P.p_pixel[p] = static_cast<int32_t>(p);
+    P.p_tp[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    P.p_rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.p_prev[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.p_flags[p] = 0;
+    P.p_med[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    P.p_alb[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.p_nrm[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.p_slot[p] = make_int2(-1, -1);
+}
+
+// Zeroes the counters a bounce appends to before anything uses them.
+__global__ void BeginBounceKernel(uint32_t* counters, int next_queue) {
+    const int i = static_cast<int>(threadIdx.x);
+    if (i == 0) counters[next_queue] = 0u;
+    if (i == 1) counters[kCountShadow] = 0u;
+    if (i >= 2 && i < 2 + (kCounterWords - kWorkExtend)) counters[kWorkExtend + (i - 2)] = 0u;
+    if (i == 63) {
+        counters[kCountLoop0] = 0u;
+        counters[kCountLoop1] = 0u;
+    }
+}
+
+__global__ void CopyTraceTotalsKernel(unsigned long long* stats) {
+    if (threadIdx.x == 0) {
+        stats[kStatNodeTests] += stats[kStatTraceNodes];
+        stats[kStatTriTests] += stats[kStatTraceTris];
+        stats[kStatTraceNodes] = 0;
+        stats[kStatTraceTris] = 0;
+    }
+}
+
+struct ShadeOut {
+    bool next;
+    float3 next_org, next_dir;
+    bool shadow[2];
+    float3 s_org[2], s_dir[2];
+    float s_tmax[2];
+    float3 s_li[2], s_bsdf[2];
+    float s_scale[2];
+};
+
+__device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& S, const LightView& L, int depth,
+                                          int i, int cur, ShadeOut& o, unsigned& hit_count, unsigned& miss_count) {
+    o.next = false;
+    o.shadow[0] = o.shadow[1] = false;
+    const int p = P.q_path[cur][i];
+    const float4 o4 = P.q_org[cur][i];
+    const float4 d4 = P.q_dir[cur][i];
+    const float3 org = XYZ(o4);
+    const float3 dir = XYZ(d4);
+    const float4 h = P.hits[i];
+    const int tri = __float_as_int(h.w);
+    const int inst = P.hit_inst != nullptr ? P.hit_inst[i] : -1;
+    int flags = P.p_flags[p];
+    float3 tp = XYZ(P.p_tp[p]);
+    float3 rad = XYZ(P.p_rad[p]);
+    const float4 prev = P.p_prev[p];
+    P.p_slot[p] = make_int2(-1, -1);
+    const int pix = P.p_pixel[p];
+    PixelSampler smp;
+    smp.Init(P.sampler, P.seed, pix % P.width, pix / P.width, P.sample_index, P.spp);
+    smp.bounce = depth;
+
+    if (tri < 0) {
+        ++miss_count;
+        float w = 1.0f;
+        if ((flags & kPrevValid) && !(flags & kPrevDelta)) w = PowerHeuristic(1, prev.w, 1, EnvPdf(L, dir));
+        rad = Add(rad, Mul(Hadamard(tp, EnvColor(L, dir)), w));
+        P.p_rad[p] = F4(rad, 0.0f);
+        return;
+    }
+    ++hit_count;
+    const float t = h.x;
+    if (flags & kMediumActive) {
+        const float4 med = P.p_med[p];
+        tp = Hadamard(tp, BeerLambert(XYZ(med), med.w, t));
+    }
+    if (IsBlack(tp)) {
+        P.p_tp[p] = F4(tp, 0.0f);
+        return;
+    }
+    const int prim = S.tri_prim[tri];
+    const int4 pinfo = S.prim_info[prim];
+    const DevMaterial mat = LoadMaterial(L.materials, pinfo.x);
+    const float3 hp = Add(org, Mul(dir, t));
+    const float3 wo = Neg(dir);
+    const int64_t base = static_cast<int64_t>(pinfo.z) + static_cast<int64_t>(tri - pinfo.w) * 3;
+    const uint32_t i0 = S.indices[base], i1 = S.indices[base + 1], i2 = S.indices[base + 2];
+    const float3 p0 = Pos(S, i0), p1 = Pos(S, i1), p2 = Pos(S, i2);
+    const float3 obj_geo = Normalize(Cross(Sub(p1, p0), Sub(p2, p0)));
+    const float4* nrows = inst >= 0 ? S.inst_normal + 6 * static_cast<int64_t>(inst) : nullptr;
+    const float3 raw = inst >= 0 ? TransformNormalDev(nrows, obj_geo) : obj_geo;
+    const float3 geo = Dot(raw, wo) < 0.0f ? Neg(raw) : raw;
+    // ResolveShadingNormal (src/geometry/intersection.cpp:84-106) on the object-space
+    // normal for instances, on the face-forwarded normal otherwise.
+    const float3 ref = inst >= 0 ? obj_geo : geo;
+    float3 sn = Normalize(ref);
+    if (S.prim_flags[prim] & 1) {
+        const float u = h.y, v = h.z, w = 1.0f - u - v;
+        const float3 n0 = XYZ(S.normals[i0]), n1 = XYZ(S.normals[i1]), n2 = XYZ(S.normals[i2]);
+        const float3 ni = Normalize(V(n0.x * w + n1.x * u + n2.x * v, n0.y * w + n1.y * u + n2.y * v,
+                                      n0.z * w + n1.z * u + n2.z * v));
+        if (LengthSq(ni) > 0.0f) sn = ni;
+    }
+    if (Dot(sn, ref) < 0.0f) sn = Neg(sn);
+    if (LengthSq(sn) == 0.0f) sn = Normalize(ref);
+    if (inst >= 0) {
+        sn = TransformNormalDev(nrows, sn);
+        if (Dot(sn, geo) < 0.0f) sn = Neg(sn);
+    }
+    if (Dot(sn, wo) < 0.0f) sn = Neg(sn);
+
+    if (depth == 0) {
+        flags |= kPrimaryHit;
+        P.p_alb[p] = F4(mat.reflectance, t);
+        P.p_nrm[p] = F4(sn, 0.0f);
+    }
+    // Emission x MIS (path_integrator.cpp:86-93; light normal in object space
+    // as GeometricNormal returns it).
+    if (!IsBlack(mat.emission)) {
+        float w = 1.0f;
+        if ((flags & kPrevValid) && !(flags & kPrevDelta)) {
+            const float pdf_light = EmissivePdfSolidAngle(L, XYZ(prev), hp, obj_geo, pinfo.y);
+            w = PowerHeuristic(1, prev.w, 1, pdf_light);
+        }
+        rad = Add(rad, Mul(Hadamard(tp, mat.emission), w));
+    }
+    // Next-event estimation (path_direct_lighting.cpp:115-283): the emissive and
+    // environment samples share the DirectLightSurface 2D sample.
+    if (!IsDelta(mat)) {
+        P.p_nee[p] = F4(tp, 0.0f);
+        const float2 surf = smp.Get2D(kDimLightSurface);
+        if (L.emissive_count > 0) {
+            const LightSample ls = SampleEmissive(S, L, smp.Get1D(kDimLightSelect), surf);
+            if (ls.valid && ls.pdf > 0.0f && !IsBlack(ls.radiance)) {
+                const float3 to_light = Sub(ls.point, hp);
+                const float d2 = LengthSq(to_light);
+                if (d2 > 1.0e-4f * 1.0e-4f) {
+                    const float dist = sqrtf(d2);
+                    const float bias = ShadowBias(hp, ls.point, dist);
+                    if (dist > bias) {
+                        const float3 so = Add(hp, Mul(sn, bias));
+                        const float3 stl = Sub(ls.point, so);
+                        const float sd2 = LengthSq(stl);
+                        if (sd2 > 1.0e-4f * 1.0e-4f) {
+                            const float sd = sqrtf(sd2);
+                            const float3 wi = Div(stl, sd);
+                            const float c = fmaxf(0.0f, Dot(sn, wi));
+                            if (c > 0.0f) {
+                                const float pdf_light = EmissivePdfSolidAngle(L, hp, ls.point, ls.normal, ls.index);
+                                if (pdf_light > 0.0f) {
+                                    const float3 f = EvalBsdf(mat, wo, wi, sn);
+                                    const float pdf_bsdf = PdfBsdf(mat, wo, wi, sn);
+                                    const float mis = PowerHeuristic(1, pdf_light, 1, pdf_bsdf);
+                                    o.shadow[0] = true;
+                                    o.s_org[0] = so;
+                                    o.s_dir[0] = wi;
+                                    o.s_tmax[0] = sd - bias;
+                                    o.s_li[0] = ls.radiance;
+                                    o.s_bsdf[0] = f;
+                                    o.s_scale[0] = c * mis / pdf_light;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (L.env_sampleable) {
+            const EnvSample es = SampleEnv(L, surf);
+            if (es.valid && es.pdf > 0.0f && !IsBlack(es.radiance)) {
+                const float c = fmaxf(0.0f, Dot(sn, es.dir));
+                if (c > 0.0f) {
+                    const float3 f = EvalBsdf(mat, wo, es.dir, sn);
+                    if (!IsBlack(f)) {
+                        const float pdf_bsdf = PdfBsdf(mat, wo, es.dir, sn);
+                        const float mis = PowerHeuristic(1, es.pdf, 1, pdf_bsdf);
+                        o.shadow[1] = true;
+                        o.s_org[1] = Add(hp, Mul(sn, SurfaceBias(hp)));
+                        o.s_dir[1] = es.dir;
+                        o.s_tmax[1] = __int_as_float(0x7f800000);
+                        o.s_li[1] = es.radiance;
+                        o.s_bsdf[1] = f;
+                        o.s_scale[1] = c * mis / es.pdf;
+                    }
+                }
+            }
+        }
+    }
+    const int max_depth = P.max_depth < 1 ? 1 : P.max_depth;
+    bool cont = depth + 1 < max_depth;
+    BsdfSample bs = Invalid();
+    if (cont) {
+        bs = SampleBsdf(mat, wo, sn, smp.Get2D(kDimBsdf));
+        cont = bs.valid && !IsBlack(bs.weight);
+    }
+    if (cont) {
+        tp = Hadamard(tp, bs.weight);
+        if (depth >= 3) {
+            const float survival = Clampf(MaxC(tp), 0.05f, 0.95f);
+            if (smp.Get1D(kDimRoulette) >= survival) {
+                cont = false;
+            } else {
+                tp = Div(tp, survival);
+            }
+        }
+    }
+    if (cont) {
+        if (mat.kind == kDielectric && Dot(bs.wi, sn) < 0.0f) {
+            if (flags & kMediumActive) {
+                flags &= ~kMediumActive;
+                P.p_med[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+            } else {
+                flags |= kMediumActive;
+                P.p_med[p] = F4(mat.absorption_color, mat.absorption_distance);
+            }
+        }
+        flags |= kPrevValid;
+        flags = bs.specular ? (flags | kPrevDelta) : (flags & ~kPrevDelta);
+        P.p_prev[p] = F4(hp, bs.pdf);
+        const float3 bn = Dot(bs.wi, sn) >= 0.0f ? sn : Neg(sn);
+        o.next = true;
+        o.next_org = Add(hp, Mul(bn, SurfaceBias(hp)));
+        o.next_dir = bs.wi;
+    }
+    P.p_tp[p] = F4(tp, 0.0f);
+    P.p_rad[p] = F4(rad, 0.0f);
+    P.p_flags[p] = flags;
+}
+
+__global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S, LightView L, int depth, int cur) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int n = static_cast<int>(P.counters[cur]);
+    ShadeOut o;
+    o.next = false;
+    o.shadow[0] = o.shadow[1] = false;
+    unsigned hit_count = 0, miss_count = 0;
+    const bool valid = i < n;
+    if (valid) ShadePath(P, S, L, depth, i, cur, o, hit_count, miss_count);
+    const int nxt = 1 - cur;
+    const int32_t slot = WaveAppend(P.counters + nxt, o.next);
+    if (slot >= 0) {
+        const int p = P.q_path[cur][i];
+        P.q_org[nxt][slot] = F4(o.next_org, 1.0e-5f);
+        P.q_dir[nxt][slot] = F4(o.next_dir, __int_as_float(0x7f800000));
+        P.q_path[nxt][slot] = p;
+    }
+    int2 slots = make_int2(-1, -1);
+    for (int k = 0; k < 2; ++k) {
+        const int32_t s = WaveAppend(P.counters + kCountShadow, o.shadow[k]);
+        if (s >= 0) {
+            P.s_org[s] = F4(o.s_org[k], 1.0e-5f);
+            P.s_dir[s] = F4(o.s_dir[k], o.s_tmax[k]);
+            P.s_li[s] = F4(o.s_li[k], o.s_scale[k]);
+            P.s_bsdf[s] = F4(o.s_bsdf[k], 0.0f);
+            P.s_trans[s] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            if (k == 0) {
+                slots.x = s;
+            } else {
+                slots.y = s;
+            }
+        }
+    }
+    if (valid && (slots.x >= 0 || slots.y >= 0)) P.p_slot[P.q_path[cur][i]] = slots;
+    WaveStat(P.stats + kStatHits, hit_count);
+    WaveStat(P.stats + kStatMisses, miss_count);
+    if (i == 0) atomicAdd(P.stats + kStatRays, static_cast<unsigned long long>(n));
+}
+
+// Adds the visible next-event contributions of this bounce, emissive then
+// environment, as EstimateDirectLight sums them.
+__global__ __launch_bounds__(kBlock) void ResolveKernel(PassView P, int cur) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int n = static_cast<int>(P.counters[cur]);
+    unsigned shadow = 0, occluded = 0;
+    if (i < n) {
+        const int p = P.q_path[cur][i];
+        const int2 slots = P.p_slot[p];
+        if (slots.x >= 0 || slots.y >= 0) {
+            float3 direct = V(0.0f, 0.0f, 0.0f);
+            for (int k = 0; k < 2; ++k) {
+                const int s = k == 0 ? slots.x : slots.y;
+                if (s < 0) continue;
+                ++shadow;
+                if (P.s_occ[s] != 0u) {
+                    ++occluded;
+                    continue;
+                }
+                const float4 li = P.s_li[s];
+                const float3 c = Mul(Hadamard(XYZ(P.s_bsdf[s]), Hadamard(XYZ(P.s_trans[s]), XYZ(li))), li.w);
+                direct = k == 0 ? c : Add(direct, c);
+            }
+            const float3 tp = XYZ(P.p_nee[p]);
+            P.p_rad[p] = F4(Add(XYZ(P.p_rad[p]), Hadamard(tp, direct)), 0.0f);
+        }
+    }
+    WaveStat(P.stats + kStatShadow, shadow);
+    WaveStat(P.stats + kStatOccluded, occluded);
+}
+
+// ---- transparent shadows (TraceShadowVisibility, path_direct_lighting.cpp:23-113) ----
+// l_state[e] = (status 0 active / 1 done, iteration, finite, 0); l_med[e] = medium
+// (color, distance; distance < 0 = inactive); s_trans holds the transmittance and
+// the remaining distance in .w.
+
+__global__ void ShadowLoopInitKernel(PassView P) {
+    const int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int n = static_cast<int>(P.counters[kCountShadow]);
+    if (e == 0) P.counters[kCountLoop0] = static_cast<uint32_t>(n);
+    if (e >= n) return;
+    const float tmax = P.s_dir[e].w;
+    P.l_state[e] = make_int4(0, 0, isfinite(tmax) ? 1 : 0, 0);
+    P.l_med[e] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
+    P.s_trans[e] = make_float4(1.0f, 1.0f, 1.0f, tmax);
+    P.s_occ[e] = 1u;
+    P.l_entry[0][e] = e;
+    P.l_org[0][e] = P.s_org[e];
+    P.l_dir[0][e] = P.s_dir[e];
+}
+
+__device__ __forceinline__ bool TransBlack(float3 c) { return MaxC(c) <= 1.0e-6f; }
+
+__global__ __launch_bounds__(kBlock) void ShadowLoopStepKernel(PassView P, SurfaceView S, LightView L, int src) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int n = static_cast<int>(P.counters[kCountLoop0 + src]);
+    bool again = false;
+    int e = -1;
+    float4 next_org = make_float4(0, 0, 0, 0), next_dir = next_org;
+    if (i < n) {
+        e = P.l_entry[src][i];
+        const float4 h = P.l_hit[i];
+        const float4 lo = P.l_org[src][i], ld = P.l_dir[src][i];
+        int4 st = P.l_state[e];
+        float4 tr = P.s_trans[e];
+        float3 T = XYZ(tr);
+        float remaining = tr.w;
+        float4 med = P.l_med[e];
+        const bool finite = st.z != 0;
+        const int tri = __float_as_int(h.w);
+        uint32_t occ = 1u;
+        if (tri < 0) {
+            bool ok = true;
+            if (med.w >= 0.0f) {
+                if (!finite) {
+                    ok = false;
+                } else {
+                    T = Hadamard(T, BeerLambert(XYZ(med), med.w, remaining));
+                }
+            }
+            occ = ok && !TransBlack(T) ? 0u : 1u;
+        } else {
+            bool ok = true;
+            if (med.w >= 0.0f) {
+                T = Hadamard(T, BeerLambert(XYZ(med), med.w, h.x));
+                if (TransBlack(T)) ok = false;
+            }
+            const int prim = S.tri_prim[tri];
+            const DevMaterial mat = L.materials[S.prim_info[prim].x];
+            if (ok && mat.kind != kDielectric && mat.kind != kThinDielectric) ok = false;
+            if (ok) {
+                if (mat.kind == kThinDielectric) {
+                    T = Hadamard(T, V(Clampf(mat.reflectance.x, 0.0f, 1.0f), Clampf(mat.reflectance.y, 0.0f, 1.0f),
+                                      Clampf(mat.reflectance.z, 0.0f, 1.0f)));
+                    if (TransBlack(T)) ok = false;
+                } else {
+                    med = med.w >= 0.0f ? make_float4(1.0f, 1.0f, 1.0f, -1.0f)
+                                        : F4(mat.absorption_color, mat.absorption_distance);
+                }
+            }
+            if (ok) {
+                const float3 hp = Add(XYZ(lo), Mul(XYZ(ld), h.x));
+                const float bias = SurfaceBias(hp);
+                bool done = false;
+                if (finite) {
+                    remaining -= h.x + bias;
+                    if (remaining <= 0.0f) {
+                        occ = !TransBlack(T) ? 0u : 1u;
+                        done = true;
+                    }
+                }
+                if (!done) {
+                    st.y += 1;
+                    if (st.y >= kMaxTransparentShadowHits) {
+                        occ = 1u;
+                    } else {
+                        again = true;
+                        next_org = F4(Add(hp, Mul(XYZ(ld), bias)), 1.0e-5f);
+                        next_dir = make_float4(ld.x, ld.y, ld.z, remaining);
+                    }
+                }
+            }
+        }
+        P.s_occ[e] = occ;
+        P.s_trans[e] = F4(T, remaining);
+        P.l_med[e] = med;
+        P.l_state[e] = st;
+    }
+    const int dst = 1 - src;
+    const int32_t slot = WaveAppend(P.counters + kCountLoop0 + dst, again);
+    if (slot >= 0) {
+        P.l_entry[dst][slot] = e;
+        P.l_org[dst][slot] = next_org;
+        P.l_dir[dst][slot] = next_dir;
+    }
+}
+
+__global__ void ResetLoopCountKernel(uint32_t* counters, int which) {
+    if (threadIdx.x == 0) counters[kCountLoop0 + which] = 0u;
+}
+
+// Adds one pass's samples to the FP64 film (Film::AddSample, src/film/film.cpp:32-56).
+__global__ __launch_bounds__(kBlock) void AccumulateKernel(PassView P) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t total = static_cast<int64_t>(P.width) * P.height;
+    unsigned bad = 0;
+    if (p < total) {
+        float3 b = XYZ(P.p_rad[p]);
+        if (P.radiance_clamp > 0.0f) {
+            const float m = MaxC(b);
+            if (!(m <= P.radiance_clamp || m <= 0.0f)) b = Mul(b, P.radiance_clamp / m);
+        }
+        if (!Finite3(b)) {
+            bad = 1;
+        } else {
+            yrh_film_pixel& px = P.film[P.p_pixel[p]];
+            px.sum[0] += static_cast<double>(b.x);
+            px.sum[1] += static_cast<double>(b.y);
+            px.sum[2] += static_cast<double>(b.z);
+            px.samples += 1u;
+            const double lum = 0.2126 * static_cast<double>(b.x) + 0.7152 * static_cast<double>(b.y) +
+                               0.0722 * static_cast<double>(b.z);
+            const double delta = lum - px.luminance_mean;
+            px.luminance_mean += delta / static_cast<double>(px.samples);
+            px.luminance_m2 += delta * (lum - px.luminance_mean);
+            const int flags = P.p_flags[p];
+            if (flags & kPrimaryHit) {
+                const float4 a = P.p_alb[p];
+                const float4 nm = P.p_nrm[p];
+                if (Finite3(XYZ(a)) && Finite3(XYZ(nm)) && isfinite(a.w) && a.w >= 0.0f) {
+                    px.albedo_sum[0] += a.x;
+                    px.albedo_sum[1] += a.y;
+                    px.albedo_sum[2] += a.z;
+                    px.normal_sum[0] += nm.x;
+                    px.normal_sum[1] += nm.y;
+                    px.normal_sum[2] += nm.z;
+                    px.depth_sum += a.w;
+                    px.aov_samples += 1u;
+                }
+            }
+        }
+    }
+    WaveStat(P.stats + kStatBad, bad);
+}
+
+__global__ void SampleValuesKernel(int kind, uint64_t seed, int spp, int64_t n, const int32_t* x, const int32_t* y,
+                                   const int32_t* sample, const int32_t* dim, int dims, float* out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PixelSampler s;
+    s.Init(kind, seed, x[i], y[i], sample[i], spp);
+    if (dims == 1) {
+        out[i] = s.At1D(static_cast<uint32_t>(dim[i]), 0);
+    } else {
+        const float2 v = s.At2D(static_cast<uint32_t>(dim[i]), 0);
+        out[2 * i] = v.x;
+        out[2 * i + 1] = v.y;
+    }
+}
+
+int Grid(int64_t n) { return static_cast<int>(std::max<int64_t>(1, (n + kBlock - 1) / kBlock)); }
+
+PassView MakePassView(RenderState& R, const yrh_render_desc& d) {
+    PassView P{};
+    P.width = d.width;
+    P.height = d.height;
+    P.sampler = d.sampler;
+    P.spp = d.spp;
+    P.max_depth = d.max_depth;
+    P.sample_index = d.sample_index;
+    P.seed = d.seed;
+    P.radiance_clamp = d.radiance_clamp;
+    P.camera.origin = make_float3(d.camera_origin[0], d.camera_origin[1], d.camera_origin[2]);
+    P.camera.forward = make_float3(d.camera_forward[0], d.camera_forward[1], d.camera_forward[2]);
+    P.camera.right = make_float3(d.camera_right[0], d.camera_right[1], d.camera_right[2]);
+    P.camera.up = make_float3(d.camera_up[0], d.camera_up[1], d.camera_up[2]);
+    P.camera.tan_half_fov = d.tan_half_fov;
+    P.p_pixel = R.p_pixel.as<int32_t>();
+    P.p_tp = R.p_tp.as<float4>();
+    P.p_rad = R.p_rad.as<float4>();
+    P.p_prev = R.p_prev.as<float4>();
+    P.p_flags = R.p_flags.as<int32_t>();
+    P.p_med = R.p_med.as<float4>();
+    P.p_alb = R.p_alb.as<float4>();
+    P.p_nrm = R.p_nrm.as<float4>();
+    P.p_nee = R.p_nee.as<float4>();
+    P.p_slot = R.p_slot.as<int2>();
+    for (int k = 0; k < 2; ++k) {
+        P.q_org[k] = R.q_org[k].as<float4>();
+        P.q_dir[k] = R.q_dir[k].as<float4>();
+        P.q_path[k] = R.q_path[k].as<int32_t>();
+        P.l_entry[k] = R.l_entry[k].as<int32_t>();
+        P.l_org[k] = R.l_org[k].as<float4>();
+        P.l_dir[k] = R.l_dir[k].as<float4>();
+    }
+    P.hits = R.hits.as<float4>();
+    P.hit_inst = R.hit_inst.as<int32_t>();
+    P.s_org = R.s_org.as<float4>();
+    P.s_dir = R.s_dir.as<float4>();
+    P.s_li = R.s_li.as<float4>();
+    P.s_bsdf = R.s_bsdf.as<float4>();
+    P.s_occ = R.s_occ.as<uint32_t>();
+    P.s_trans = R.s_trans.as<float4>();
+    P.l_hit = R.l_hit.as<float4>();
+    P.l_state = R.l_state.as<int4>();
+    P.l_med = R.l_med.as<float4>();
+    P.counters = R.counters.as<uint32_t>();
+    P.stats = R.stats.as<unsigned long long>();
+    P.film = R.film.as<yrh_film_pixel>();
+    return P;
+}
+
+SurfaceView MakeSurfaceView(RenderState& R) {
+    SurfaceView S{};
+    S.positions = R.positions.as<float4>();
+    S.normals = R.normals.as<float4>();
+    S.indices = R.indices.as<uint32_t>();
+    S.tri_prim = R.tri_prim.as<int32_t>();
+    S.prim_info = R.prim_info.as<int4>();
+    S.prim_flags = R.prim_flags.as<int32_t>();
+    S.inst_normal = R.inst_normal.as<float4>();
+    return S;
+}
+
+LightView MakeLightView(RenderState& R) {
+    LightView L{};
+    L.materials = R.materials.as<DevMaterial>();
+    L.emit = R.emit.as<float4>();
+    L.emit_info = R.emit_info.as<int4>();
+    L.emit_area = R.emit_area.as<float>();
+    L.emit_cdf = R.emit_cdf.as<float>();
+    L.alias_prob = R.alias_prob.as<float>();
+    L.alias_index = R.alias_index.as<int32_t>();
+    L.alias_pmf = R.alias_pmf.as<float>();
+    L.emissive_count = R.emissive_count;
+    L.total_emissive_area = R.total_emissive_area;
+    L.env_active = R.env_active;
+    L.env_radiance = make_float3(R.env_radiance[0], R.env_radiance[1], R.env_radiance[2]);
+    L.env_strength = R.env_strength;
+    L.env_rotation = R.env_rotation;
+    L.env_texels = R.env_w > 0 && R.env_h > 0 ? R.env_texels.as<float4>() : nullptr;
+    L.env_w = R.env_w;
+    L.env_h = R.env_h;
+    L.env_sampleable = R.env_sampleable;
+    L.env_weights = R.env_weights.as<float>();
+    L.env_row_weights = R.env_row_weights.as<float>();
+    L.env_row_cdf = R.env_row_cdf.as<float>();
+    L.env_cond_cdf = R.env_cond_cdf.as<float>();
+    L.env_dw = R.env_dw;
+    L.env_dh = R.env_dh;
+    L.env_total = R.env_total;
+    return L;
+}
+
+int EnsureCapacity(yrh_context* ctx, RenderState& R, int64_t paths) {
+    if (R.capacity >= paths && R.counters.ptr != nullptr) return YRH_OK;
+    int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (rc != YRH_OK) return rc;
+    const size_t P = static_cast<size_t>(paths);
+    const size_t S = 2 * P;
+    struct Item {
+        DevBuf* buf;
+        size_t bytes;
+    } items[] = {
+        {&R.p_pixel, 4 * P}, {&R.p_tp, 16 * P}, {&R.p_rad, 16 * P}, {&R.p_prev, 16 * P}, {&R.p_flags, 4 * P},
+        {&R.p_med, 16 * P}, {&R.p_alb, 16 * P}, {&R.p_nrm, 16 * P}, {&R.p_nee, 16 * P}, {&R.p_slot, 8 * P},
+        {&R.q_org[0], 16 * P}, {&R.q_org[1], 16 * P}, {&R.q_dir[0], 16 * P}, {&R.q_dir[1], 16 * P},
+        {&R.q_path[0], 4 * P}, {&R.q_path[1], 4 * P}, {&R.hits, 16 * P}, {&R.hit_inst, 4 * P},
+        {&R.s_org, 16 * S}, {&R.s_dir, 16 * S}, {&R.s_li, 16 * S}, {&R.s_bsdf, 16 * S}, {&R.s_occ, 4 * S},
+        {&R.s_trans, 16 * S}, {&R.l_entry[0], 4 * S}, {&R.l_entry[1], 4 * S}, {&R.l_org[0], 16 * S},
+        {&R.l_org[1], 16 * S}, {&R.l_dir[0], 16 * S}, {&R.l_dir[1], 16 * S}, {&R.l_hit, 16 * S}, {&R.l_state, 16 * S}, {&R.l_med, 16 * S},
+        {&R.counters, 4 * kCounterWords},
+    };
+    for (const Item& it : items) {
+        if ((rc = Alloc(ctx, *it.buf, it.bytes)) != YRH_OK) return rc;
+    }
+    if (R.stats.ptr == nullptr) {
+        if ((rc = Alloc(ctx, R.stats, 8 * kStatWords)) != YRH_OK) return rc;
+        if ((rc = CheckHip(ctx, hipMemset(R.stats.ptr, 0, 8 * kStatWords), "hipMemset")) != YRH_OK) return rc;
+    }
+    rc = CheckHip(ctx, hipMemset(R.counters.ptr, 0, 4 * kCounterWords), "hipMemset");
+    R.capacity = paths;
+    return rc;
+}
+
+}  // namespace
+
+int UploadSurfaceGeometry(yrh_context* ctx, const yrh_geometry_desc* g) {
+    RenderState& R = State(ctx);
+    R.shading_loaded = false;
+    const int64_t nv = g->vertex_count;
+    // The shading kernels index vertices through every primitive's range.
+    for (int64_t k = 0; k < g->index_count; ++k) {
+        if (static_cast<int64_t>(g->indices[k]) >= nv) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "vertex index out of range");
+    }
+    for (int32_t p = 0; p < g->primitive_count; ++p) {
+        const yrh_primitive& mp = g->primitives[p];
+        if (mp.index_count % 3 != 0 ||
+            static_cast<int64_t>(mp.first_index) + static_cast<int64_t>(mp.index_count) > g->index_count) {
+            return Fail(ctx, YRH_E_INVALID_ARGUMENT, "primitive index range out of bounds");
+        }
+    }
+    if (g->accel_kind == YRH_ACCEL_TWO_LEVEL && g->instance_count > 0 && g->instances == nullptr) {
+        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "instances missing");
+    }
+    std::vector<float4> pos(static_cast<size_t>(nv)), nrm(static_cast<size_t>(nv));
+    R.host_positions.resize(static_cast<size_t>(3 * nv));
+    for (int64_t v = 0; v < nv; ++v) {
+        const yrh_vertex& vx = g->vertices[v];
+        pos[static_cast<size_t>(v)] = make_float4(vx.position[0], vx.position[1], vx.position[2], 0.0f);
+        nrm[static_cast<size_t>(v)] = make_float4(vx.normal[0], vx.normal[1], vx.normal[2], 0.0f);
+        for (int c = 0; c < 3; ++c) R.host_positions[static_cast<size_t>(3 * v + c)] = vx.position[c];
+    }
+    R.host_indices.assign(g->indices, g->indices + g->index_count);
+    std::vector<int4> info(static_cast<size_t>(g->primitive_count));
+    std::vector<int32_t> flags(static_cast<size_t>(g->primitive_count));
+    std::vector<int32_t> tri_prim;
+    R.host_prim_first.assign(static_cast<size_t>(g->primitive_count), 0);
+    R.host_prim_count.assign(static_cast<size_t>(g->primitive_count), 0);
+    R.host_prim_material.assign(static_cast<size_t>(g->primitive_count), 0);
+    int32_t flat = 0;
+    for (int32_t p = 0; p < g->primitive_count; ++p) {
+        const yrh_primitive& mp = g->primitives[p];
+        const int32_t tris = static_cast<int32_t>(mp.index_count / 3);
+        info[static_cast<size_t>(p)] = make_int4(mp.material_index, -1, static_cast<int32_t>(mp.first_index), flat);
+        flags[static_cast<size_t>(p)] = mp.has_normals ? 1 : 0;
+        R.host_prim_first[static_cast<size_t>(p)] = static_cast<int32_t>(mp.first_index);
+        R.host_prim_count[static_cast<size_t>(p)] = tris;
+        R.host_prim_material[static_cast<size_t>(p)] = mp.material_index;
+        for (int32_t t = 0; t < tris; ++t) tri_prim.push_back(p);
+        flat += tris;
+    }
+    R.primitive_count = g->primitive_count;
+    R.triangle_count = flat;
+    R.vertex_count = nv;
+    // Instance normal transforms: cofactor rows of the upper 3x3 of object_to_world
+    // and 1/det, as TransformNormal computes them per call.
+    std::vector<float4> inst;
+    if (g->accel_kind == YRH_ACCEL_TWO_LEVEL) {
+        inst.resize(6 * static_cast<size_t>(g->instance_count));
+        for (int32_t k = 0; k < g->instance_count; ++k) {
+            const float* m = g->instances[k].object_to_world;
+            const float a = m[0], b = m[4], c = m[8], d = m[1], e = m[5], f = m[9], gg = m[2], h = m[6], i = m[10];
+            const float det = a * (e * i - f * h) - b * (d * i - f * gg) + c * (d * h - e * gg);
+            const bool singular = std::fabs(det) <= 1.0e-12f;
+            const float inv = singular ? 0.0f : 1.0f / det;
+            float4* o = &inst[6 * static_cast<size_t>(k)];
+            o[0] = make_float4(e * i - f * h, f * gg - d * i, d * h - e * gg, inv);
+            o[1] = make_float4(c * h - b * i, a * i - c * gg, b * gg - a * h, singular ? 1.0f : 0.0f);
+            o[2] = make_float4(b * f - c * e, c * d - a * f, a * e - b * d, 0.0f);
+            o[3] = make_float4(m[0], m[4], m[8], 0.0f);
+            o[4] = make_float4(m[1], m[5], m[9], 0.0f);
+            o[5] = make_float4(m[2], m[6], m[10], 0.0f);
+        }
+    }
+    int rc;
+    if ((rc = UploadVec(ctx, R.positions, pos.data(), pos.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.normals, nrm.data(), nrm.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.indices, g->indices, static_cast<size_t>(g->index_count))) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.tri_prim, tri_prim.data(), tri_prim.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.prim_info, info.data(), info.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.prim_flags, flags.data(), flags.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.inst_normal, inst.data(), inst.size())) != YRH_OK) return rc;
+    return YRH_OK;
+}
+
+void DestroyRenderState(yrh_context* ctx) {
+    delete ctx->render;
+    ctx->render = nullptr;
+}
+
+}  // namespace yrh
+
+using namespace yrh;
+
+extern "C" {
+
+int yrh_upload_shading(yrh_context* ctx, const yrh_shading_desc* sd) {
+    if (ctx == nullptr || sd == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null argument");
+    if (!ctx->geo.loaded || ctx->render == nullptr) return Fail(ctx, YRH_E_NO_SCENE, "upload geometry first");
+    RenderState& R = *ctx->render;
+    if (sd->material_count <= 0 || sd->materials == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "no materials");
+    for (int32_t m : R.host_prim_material) {
+        if (m < 0 || m >= sd->material_count) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "primitive material index out of range");
+    }
+    (void)hipSetDevice(ctx->device);
+    int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (rc != YRH_OK) return rc;
+    std::vector<DevMaterial> mats(static_cast<size_t>(sd->material_count));
+    R.transparent = false;
+    for (int32_t k = 0; k < sd->material_count; ++k) {
+        const yrh_material& m = sd->materials[k];
+        if (m.kind < YRH_MATERIAL_DIFFUSE || m.kind > YRH_MATERIAL_THIN_DIELECTRIC) {
+            return Fail(ctx, YRH_E_UNSUPPORTED, "material kind not supported by the HIP backend");
+        }
+        if (!(m.alpha >= 0.5f)) return Fail(ctx, YRH_E_UNSUPPORTED, "alpha cut-out materials are not supported");
+        DevMaterial& d = mats[static_cast<size_t>(k)];
+        d.kind = m.kind;
+        d.reflectance = make_float3(m.reflectance[0], m.reflectance[1], m.reflectance[2]);
+        d.ior = m.ior;
+        d.urough = m.uroughness;
+        d.vrough = m.vroughness;
+        d.alpha = m.alpha;
+        d.emission = make_float3(m.emission[0], m.emission[1], m.emission[2]);
+        d.absorption_distance = m.absorption_distance;
+        d.absorption_color = make_float3(m.absorption_color[0], m.absorption_color[1], m.absorption_color[2]);
+        if (m.kind == YRH_MATERIAL_DIELECTRIC || m.kind == YRH_MATERIAL_THIN_DIELECTRIC) R.transparent = true;
+    }
+    if ((rc = UploadVec(ctx, R.materials, mats.data(), mats.size())) != YRH_OK) return rc;
+    R.material_count = sd->material_count;
+
+    // Emitters: per-emitter running triangle areas in index order (the
+    // reference's two linear passes in SelectTriangleByArea).
+    const int32_t ne = std::max(0, sd->emissive_count);
+    if (ne > 0 && (sd->emissives == nullptr || sd->alias_probability == nullptr || sd->alias_index == nullptr ||
+                   sd->alias_pmf == nullptr)) {
+        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "emitter tables missing");
+    }
+    for (int32_t k = 0; k < ne; ++k) {
+        if (sd->alias_index[k] < 0 || sd->alias_index[k] >= ne) {
+            return Fail(ctx, YRH_E_INVALID_ARGUMENT, "alias table index out of range");
+        }
+    }
+    if (sd->environment.texels != nullptr && sd->environment.has_distribution &&
+        (sd->environment.dist_width <= 0 || sd->environment.dist_height <= 0)) {
+        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "environment distribution has no texels");
+    }
+    std::vector<float4> emit(static_cast<size_t>(ne));
+    std::vector<int4> einfo(static_cast<size_t>(ne));
+    std::vector<float> earea(static_cast<size_t>(ne)), cdf;
+    std::vector<int4> prim_info_update;
+    std::vector<int32_t> prim_emissive(static_cast<size_t>(R.primitive_count), -1);
+    float total_area = 0.0f;
+    auto tri_area = [&](int32_t prim, int32_t t) {
+        const size_t base = static_cast<size_t>(R.host_prim_first[static_cast<size_t>(prim)]) + static_cast<size_t>(t) * 3;
+        float p[3][3];
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t vi = R.host_indices[base + static_cast<size_t>(k)];
+            for (int c = 0; c < 3; ++c) p[k][c] = R.host_positions[3 * static_cast<size_t>(vi) + static_cast<size_t>(c)];
+        }
+        const float e1[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
+        const float e2[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
+        const float cx = e1[1] * e2[2] - e1[2] * e2[1];
+        const float cy = e1[2] * e2[0] - e1[0] * e2[2];
+        const float cz = e1[0] * e2[1] - e1[1] * e2[0];
+        const float area = std::sqrt(cx * cx + cy * cy + cz * cz) * 0.5f;
+        return std::isfinite(area) && area > 0.0f ? area : 0.0f;
+    };
+    for (int32_t k = 0; k < ne; ++k) {
+        const yrh_emissive& ep = sd->emissives[k];
+        emit[static_cast<size_t>(k)] = make_float4(ep.radiance[0], ep.radiance[1], ep.radiance[2], ep.area);
+        const float ea = std::isfinite(ep.area) && ep.area > 0.0f ? ep.area : 0.0f;
+        total_area += ea;
+        const int32_t prim = ep.primitive_index;
+        if (prim < 0 || prim >= R.primitive_count) {
+            einfo[static_cast<size_t>(k)] = make_int4(-1, static_cast<int32_t>(cdf.size()), 0, 0);
+            continue;
+        }
+        if (prim_emissive[static_cast<size_t>(prim)] < 0) prim_emissive[static_cast<size_t>(prim)] = k;
+        const int32_t tris = R.host_prim_count[static_cast<size_t>(prim)];
+        float sum = 0.0f;
+        for (int32_t t = 0; t < tris; ++t) sum += tri_area(prim, t);
+        earea[static_cast<size_t>(k)] = sum;
+        einfo[static_cast<size_t>(k)] = make_int4(prim, static_cast<int32_t>(cdf.size()), tris, 0);
+        float running = 0.0f;
+        for (int32_t t = 0; t < tris; ++t) {
+            const float a = tri_area(prim, t);
+            if (a > 0.0f) running += a;
+            cdf.push_back(running);
+        }
+    }
+    R.emissive_count = ne;
+    R.total_emissive_area = total_area;
+    if ((rc = UploadVec(ctx, R.emit, emit.data(), emit.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.emit_info, einfo.data(), einfo.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.emit_area, earea.data(), earea.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.emit_cdf, cdf.data(), cdf.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.alias_prob, sd->alias_probability, static_cast<size_t>(ne))) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.alias_index, sd->alias_index, static_cast<size_t>(ne))) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.alias_pmf, sd->alias_pmf, static_cast<size_t>(ne))) != YRH_OK) return rc;
+    // Primitive -> first emitter record (the reference's linear scan).
+    {
+        std::vector<int4> info(static_cast<size_t>(R.primitive_count));
+        rc = CheckHip(ctx, hipMemcpy(info.data(), R.prim_info.ptr, info.size() * sizeof(int4), hipMemcpyDeviceToHost),
+                      "hipMemcpy");
+        if (rc != YRH_OK) return rc;
+        for (size_t p = 0; p < info.size(); ++p) info[p].y = prim_emissive[p];
+        if ((rc = UploadVec(ctx, R.prim_info, info.data(), info.size())) != YRH_OK) return rc;
+    }
+    // Environment.
+    const yrh_environment_desc& env = sd->environment;
+    R.env_active = env.active ? 1 : 0;
+    for (int c = 0; c < 3; ++c) R.env_radiance[c] = env.radiance[c];
+    R.env_strength = env.strength;
+    R.env_rotation = env.rotation_radians;
+    R.env_w = env.texels != nullptr ? std::max(0, env.width) : 0;
+    R.env_h = env.texels != nullptr ? std::max(0, env.height) : 0;
+    const size_t ntex = static_cast<size_t>(R.env_w) * static_cast<size_t>(R.env_h);
+    if ((rc = UploadVec(ctx, R.env_texels, reinterpret_cast<const float4*>(env.texels), ntex)) != YRH_OK) return rc;
+    const bool has_tex = ntex > 0;
+    R.env_sampleable = (env.active && env.strength > 0.0f && has_tex && env.has_distribution) ? 1 : 0;
+    R.env_dw = env.has_distribution ? env.dist_width : 0;
+    R.env_dh = env.has_distribution ? env.dist_height : 0;
+    R.env_total = env.total_weight;
+    const size_t nd = static_cast<size_t>(std::max(0, R.env_dw)) * static_cast<size_t>(std::max(0, R.env_dh));
+    if (env.has_distribution && (env.texel_weights == nullptr || env.row_weights == nullptr || env.row_cdf == nullptr ||
+                                 env.conditional_cdfs == nullptr)) {
+        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "environment distribution arrays missing");
+    }
+    if ((rc = UploadVec(ctx, R.env_weights, env.texel_weights, env.has_distribution ? nd : 0)) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.env_row_weights, env.row_weights, env.has_distribution ? static_cast<size_t>(R.env_dh) : 0)) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.env_row_cdf, env.row_cdf, env.has_distribution ? static_cast<size_t>(R.env_dh) : 0)) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.env_cond_cdf, env.conditional_cdfs, env.has_distribution ? nd : 0)) != YRH_OK) return rc;
+    R.shading_loaded = true;
+    return YRH_OK;
+}
+
+int yrh_film_reset(yrh_context* ctx, int32_t width, int32_t height) {
+    if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
+    if (width <= 0 || height <= 0) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "Film dimensions must be positive");
+    (void)hipSetDevice(ctx->device);
+    RenderState& R = State(ctx);
+    int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (rc != YRH_OK) return rc;
+    const size_t bytes = static_cast<size_t>(width) * static_cast<size_t>(height) * sizeof(yrh_film_pixel);
+    if (R.film.bytes < bytes) {
+        if ((rc = Alloc(ctx, R.film, bytes)) != YRH_OK) return rc;
+    }
+    if ((rc = CheckHip(ctx, hipMemset(R.film.ptr, 0, bytes), "hipMemset")) != YRH_OK) return rc;
+    R.film_w = width;
+    R.film_h = height;
+    if (R.stats.ptr == nullptr) {
+        if ((rc = Alloc(ctx, R.stats, 8 * kStatWords)) != YRH_OK) return rc;
+    }
+    return CheckHip(ctx, hipMemset(R.stats.ptr, 0, 8 * kStatWords), "hipMemset");
+}
+
+int yrh_film_load(yrh_context* ctx, const yrh_film_pixel* pixels, int64_t count) {
+    if (ctx == nullptr || ctx->render == nullptr || pixels == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "no film");
+    RenderState& R = *ctx->render;
+    if (count != static_cast<int64_t>(R.film_w) * R.film_h) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "film size mismatch");
+    return yrh_copy_to_device(ctx, R.film.ptr, pixels, static_cast<size_t>(count) * sizeof(yrh_film_pixel));
+}
+
+int yrh_film_read(yrh_context* ctx, yrh_film_pixel* pixels, int64_t count) {
+    if (ctx == nullptr || ctx->render == nullptr || pixels == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "no film");
+    RenderState& R = *ctx->render;
+    if (count != static_cast<int64_t>(R.film_w) * R.film_h) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "film size mismatch");
+    return yrh_copy_to_host(ctx, pixels, R.film.ptr, static_cast<size_t>(count) * sizeof(yrh_film_pixel));
+}
+
+int yrh_render_stats_read(yrh_context* ctx, yrh_render_stats* stats) {
+    if (ctx == nullptr || stats == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null argument");
+    if (ctx->render == nullptr || ctx->render->stats.ptr == nullptr) {
+        *stats = yrh_render_stats{};
+        return YRH_OK;
+    }
+    unsigned long long w[kStatWords];
+    int rc = yrh_copy_to_host(ctx, w, ctx->render->stats.ptr, sizeof(w));
+    if (rc != YRH_OK) return rc;
+    stats->rays_traced = w[kStatRays];
+    stats->shadow_rays = w[kStatShadow];
+    stats->occluded_shadow_rays = w[kStatOccluded];
+    stats->triangle_tests = w[kStatTriTests] + w[kStatTraceTris];
+    stats->sphere_tests = w[kStatSphereTests];
+    stats->bvh_node_tests = w[kStatNodeTests] + w[kStatTraceNodes];
+    stats->hits = w[kStatHits];
+    stats->misses = w[kStatMisses];
+    stats->bad_samples = w[kStatBad];
+    return YRH_OK;
+}
+
+int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* d) {
+    if (ctx == nullptr || d == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null argument");
+    if (ctx->render == nullptr || !ctx->render->shading_loaded) return Fail(ctx, YRH_E_NO_SCENE, "upload geometry and shading first");
+    RenderState& R = *ctx->render;
+    if (d->width != R.film_w || d->height != R.film_h) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "film size does not match the pass");
+    if (d->sampler < YRH_SAMPLER_INDEPENDENT || d->sampler > YRH_SAMPLER_ZSOBOL) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "unknown sampler");
+    if (d->sample_index < 0) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "negative sample index");
+    const int64_t paths = static_cast<int64_t>(d->width) * d->height;
+    if (paths > (int64_t{1} << 30)) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "frame too large for one pass");
+    (void)hipSetDevice(ctx->device);
+    int rc = EnsureCapacity(ctx, R, paths);
+    if (rc != YRH_OK) return rc;
+    PassView P = MakePassView(R, *d);
+    const SurfaceView S = MakeSurfaceView(R);
+    const LightView L = MakeLightView(R);
+    hipStream_t st = ctx->stream;
+    const int max_depth = std::max(1, d->max_depth);
+    hipLaunchKernelGGL(RaygenKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P);
+    for (int depth = 0; depth < max_depth; ++depth) {
+        const int cur = depth & 1;
+        hipLaunchKernelGGL(BeginBounceKernel, dim3(1), dim3(64), 0, st, P.counters, 1 - cur);
+        TraceRequest tr;
+        tr.any = false;
+        tr.org = P.q_org[cur];
+        tr.dir = P.q_dir[cur];
+        tr.n_device = P.counters + cur;
+        tr.capacity = paths;
+        tr.work_counter = P.counters + kWorkExtend;
+        tr.hit = P.hits;
+        tr.instance = ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL ? P.hit_inst : nullptr;
+        tr.totals = P.stats + kStatTraceNodes;
+        if ((rc = LaunchTrace(ctx, tr)) != YRH_OK) return rc;
+        PassView Pk = P;
+        if (ctx->geo.accel_kind != YRH_ACCEL_TWO_LEVEL) Pk.hit_inst = nullptr;
+        hipLaunchKernelGGL(ShadeKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, depth, cur);
+        if (!R.transparent) {
+            TraceRequest sr;
+            sr.any = true;
+            sr.org = P.s_org;
+            sr.dir = P.s_dir;
+            sr.n_device = P.counters + kCountShadow;
+            sr.capacity = 2 * paths;
+            sr.work_counter = P.counters + kWorkShadow;
+            sr.occluded = P.s_occ;
+            sr.totals = P.stats + kStatTraceNodes;
+            if ((rc = LaunchTrace(ctx, sr)) != YRH_OK) return rc;
+        } else {
+            hipLaunchKernelGGL(ShadowLoopInitKernel, dim3(Grid(2 * paths)), dim3(kBlock), 0, st, P);
+            for (int it = 0; it < kMaxTransparentShadowHits; ++it) {
+                const int src = it & 1;
+                TraceRequest lr;
+                lr.any = false;
+                lr.org = P.l_org[src];
+                lr.dir = P.l_dir[src];
+                lr.n_device = P.counters + kCountLoop0 + src;
+                lr.capacity = 2 * paths;
+                lr.work_counter = P.counters + kWorkLoop + it;
+                lr.hit = P.l_hit;
+                lr.totals = P.stats + kStatTraceNodes;
+                if ((rc = LaunchTrace(ctx, lr)) != YRH_OK) return rc;
+                hipLaunchKernelGGL(ResetLoopCountKernel, dim3(1), dim3(64), 0, st, P.counters, 1 - src);
+                hipLaunchKernelGGL(ShadowLoopStepKernel, dim3(Grid(2 * paths)), dim3(kBlock), 0, st, P, S, L, src);
+            }
+        }
+        hipLaunchKernelGGL(ResolveKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P, cur);
+    }
+    hipLaunchKernelGGL(AccumulateKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P);
+    hipLaunchKernelGGL(CopyTraceTotalsKernel, dim3(1), dim3(64), 0, st, P.stats);
+    return CheckHip(ctx, hipGetLastError(), "render pass launch");
+}
+
+int yrh_sample_values(yrh_context* ctx, int32_t kind, uint64_t seed, int32_t spp, int64_t n, const int32_t* x,
+                      const int32_t* y, const int32_t* sample, const int32_t* dim, int32_t dims, float* out) {
+    if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
+    if (n < 0 || (n > 0 && (x == nullptr || y == nullptr || sample == nullptr || dim == nullptr || out == nullptr)) ||
+        (dims != 1 && dims != 2) || kind < 0 || kind > 3) {
+        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "bad sample_values arguments");
+    }
+    if (n == 0) return YRH_OK;
+    (void)hipSetDevice(ctx->device);
+    DevBuf bx, by, bs, bd, bo;
+    int rc;
+    if ((rc = UploadVec(ctx, bx, x, static_cast<size_t>(n))) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, by, y, static_cast<size_t>(n))) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, bs, sample, static_cast<size_t>(n))) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, bd, dim, static_cast<size_t>(n))) != YRH_OK) return rc;
+    if ((rc = Alloc(ctx, bo, static_cast<size_t>(n) * static_cast<size_t>(dims) * sizeof(float))) != YRH_OK) return rc;
+    hipLaunchKernelGGL(SampleValuesKernel, dim3(Grid(n)), dim3(kBlock), 0, ctx->stream, kind, seed, spp, n,
+                       bx.as<int32_t>(), by.as<int32_t>(), bs.as<int32_t>(), bd.as<int32_t>(), dims, bo.as<float>());
+    if ((rc = CheckHip(ctx, hipGetLastError(), "sample kernel launch")) != YRH_OK) return rc;
+    return yrh_copy_to_host(ctx, out, bo.ptr, static_cast<size_t>(n) * static_cast<size_t>(dims) * sizeof(float));
+}
+
+int yrh_generate_camera_rays(yrh_context* ctx, const yrh_render_desc* d, float* org_tmin, float* dir_tmax) {
+    if (ctx == nullptr || d == nullptr || org_tmin == nullptr || dir_tmax == nullptr) {
+        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null argument");
+    }
+    if (d->width <= 0 || d->height <= 0) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "bad frame size");
+    (void)hipSetDevice(ctx->device);
+    RenderState& R = State(ctx);
+    const int64_t paths = static_cast<int64_t>(d->width) * d->height;
+    int rc = EnsureCapacity(ctx, R, paths);
+    if (rc != YRH_OK) return rc;
+    PassView P = MakePassView(R, *d);
+    P.q_org[0] = reinterpret_cast<float4*>(org_tmin);
+    P.q_dir[0] = reinterpret_cast<float4*>(dir_tmax);
+    hipLaunchKernelGGL(RaygenKernel, dim3(Grid(paths)), dim3(kBlock), 0, ctx->stream, P);
+    return CheckHip(ctx, hipGetLastError(), "raygen launch");
+}
+
+}  // extern "C"

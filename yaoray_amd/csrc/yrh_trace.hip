@@ -36,6 +36,8 @@ struct TraceArgs {
     const float4* org;
     const float4* dir;
     int32_t n;
+    const uint32_t* n_device;  // when set, the ray count is read from device memory
+    unsigned long long* totals;  // optional [0] node tests, [1] triangle tests of the launch
     // outputs
     float4* hit;
     int32_t* inst_out;
@@ -57,7 +59,7 @@ struct TraceArgs {
     const DevBlasRoot* blas_roots;
     // scheduling
     uint32_t* counter;       // this launch's ray counter (zero at launch)
-    uint32_t* next_counter;  // the next launch's counter, zeroed here
+    uint32_t* next_counter;  // the next launch's counter, zeroed here (may be null)
     uint32_t* overflow;      // stack overflow flag
     int2* spill;
     int64_t spill_stride;
@@ -213,6 +215,18 @@ __device__ __forceinline__ void InteriorStep(const DevNode* __restrict__ nodes, 
 
 __device__ __forceinline__ uint64_t LanesBelow(int lane) { return (1ull << lane) - 1ull; }
 
+// Wave-level sum of the per-lane test counts, one atomic per wave.
+__device__ __forceinline__ void AddTotals(unsigned long long* totals, uint64_t nt, uint64_t tt, int lane) {
+    for (int off = 32; off > 0; off >>= 1) {
+        nt += __shfl_xor(static_cast<unsigned long long>(nt), off);
+        tt += __shfl_xor(static_cast<unsigned long long>(tt), off);
+    }
+    if (lane == 0) {
+        atomicAdd(totals, static_cast<unsigned long long>(nt));
+        atomicAdd(totals + 1, static_cast<unsigned long long>(tt));
+    }
+}
+
 // Claims one ray index per wanting lane with a single atomic per wave.
 __device__ __forceinline__ int32_t ClaimRay(uint32_t* counter, bool want, int lane, int32_t n, bool& exhausted) {
     const uint64_t mask = __ballot(want);
@@ -238,7 +252,9 @@ __global__ __launch_bounds__(kMaxBlock) void TraceFlatKernel(TraceArgs a) {
     Stack st{lds_stack + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
              a.capacity, 0, false};
 
-    if (blockIdx.x == 0 && threadIdx.x == 0) *a.next_counter = 0u;
+    if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.next_counter = 0u;
+    const int32_t n = a.n_device != nullptr ? static_cast<int32_t>(*a.n_device) : a.n;
+    uint64_t sum_nt = 0, sum_tt = 0;
     int32_t ray = -1;
     bool exhausted = false;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, t_min = 0;
@@ -256,10 +272,12 @@ __global__ __launch_bounds__(kMaxBlock) void TraceFlatKernel(TraceArgs a) {
             }
             if (a.node_tests != nullptr) a.node_tests[ray] = nt;
             if (a.triangle_tests != nullptr) a.triangle_tests[ray] = tt;
+            sum_nt += nt;
+            sum_tt += tt;
             ray = -1;
         }
         const bool want = st.sp == 0 && !exhausted;
-        const int32_t claimed = ClaimRay(a.counter, want, lane, a.n, exhausted);
+        const int32_t claimed = ClaimRay(a.counter, want, lane, n, exhausted);
         if (claimed >= 0) {
             ray = claimed;
             const float4 o4 = a.org[ray];
@@ -309,6 +327,7 @@ __global__ __launch_bounds__(kMaxBlock) void TraceFlatKernel(TraceArgs a) {
         }
     }
     if (st.overflow) atomicOr(a.overflow, 1u);
+    if (a.totals != nullptr) AddTotals(a.totals, sum_nt, sum_tt, lane);
 }
 
 __device__ __forceinline__ void TransformRay(const DevInstance& m, float ox, float oy, float oz, float dx, float dy,
@@ -331,7 +350,9 @@ __global__ __launch_bounds__(kMaxBlock) void TraceTwoLevelKernel(TraceArgs a) {
     Stack st{lds_stack + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
              a.capacity, 0, false};
 
-    if (blockIdx.x == 0 && threadIdx.x == 0) *a.next_counter = 0u;
+    if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.next_counter = 0u;
+    const int32_t n = a.n_device != nullptr ? static_cast<int32_t>(*a.n_device) : a.n;
+    uint64_t sum_nt = 0, sum_tt = 0;
     int32_t ray = -1;
     bool exhausted = false;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, t_min = 0;
@@ -360,10 +381,12 @@ __global__ __launch_bounds__(kMaxBlock) void TraceTwoLevelKernel(TraceArgs a) {
             }
             if (a.node_tests != nullptr) a.node_tests[ray] = nt;
             if (a.triangle_tests != nullptr) a.triangle_tests[ray] = tt;
+            sum_nt += nt;
+            sum_tt += tt;
             ray = -1;
         }
         const bool want = !busy && !exhausted;
-        const int32_t claimed = ClaimRay(a.counter, want, lane, a.n, exhausted);
+        const int32_t claimed = ClaimRay(a.counter, want, lane, n, exhausted);
         if (claimed >= 0) {
             ray = claimed;
             const float4 o4 = a.org[ray];
@@ -459,29 +482,41 @@ __global__ __launch_bounds__(kMaxBlock) void TraceTwoLevelKernel(TraceArgs a) {
         }
     }
     if (st.overflow) atomicOr(a.overflow, 1u);
+    if (a.totals != nullptr) AddTotals(a.totals, sum_nt, sum_tt, lane);
 }
 
-int PrepareArgs(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, TraceArgs& a, int& grid, int& block) {
+int PrepareArgs(yrh_context* ctx, const TraceRequest& r, TraceArgs& a, int& grid, int& block) {
     const DevGeometry& g = ctx->geo;
     block = ctx->block_size;
     grid = ctx->compute_units * ctx->blocks_per_cu;
     const int64_t lanes = static_cast<int64_t>(grid) * block;
-    // Do not launch more lanes than rays (rounded up to whole workgroups).
-    if (lanes > n) grid = static_cast<int>(std::max<int64_t>(1, (n + block - 1) / block));
+    // Do not launch more lanes than rays can need (rounded up to whole workgroups).
+    const int64_t bound = r.n_device != nullptr ? r.capacity : r.n_host;
+    if (lanes > bound) grid = static_cast<int>(std::max<int64_t>(1, (bound + block - 1) / block));
     const int64_t total = static_cast<int64_t>(grid) * block;
     const int32_t capacity = g.stack_capacity;
     const int64_t spill_levels = std::max<int64_t>(0, capacity - kLdsStack);
     const size_t spill_bytes = static_cast<size_t>(std::max<int64_t>(1, spill_levels * total)) * sizeof(int2);
     if (ctx->stack_spill.bytes < spill_bytes) {
+        // Earlier launches on the stream may still use the old region.
+        int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        if (rc != YRH_OK) return rc;
         ctx->stack_spill.reset();
-        int rc = CheckHip(ctx, hipMalloc(&ctx->stack_spill.ptr, spill_bytes), "hipMalloc stack spill");
+        rc = CheckHip(ctx, hipMalloc(&ctx->stack_spill.ptr, spill_bytes), "hipMalloc stack spill");
         if (rc != YRH_OK) return rc;
         ctx->stack_spill.bytes = spill_bytes;
     }
     a = TraceArgs{};
-    a.org = reinterpret_cast<const float4*>(rays->org_tmin);
-    a.dir = reinterpret_cast<const float4*>(rays->dir_tmax);
-    a.n = static_cast<int32_t>(n);
+    a.org = r.org;
+    a.dir = r.dir;
+    a.n = static_cast<int32_t>(r.n_host);
+    a.n_device = r.n_device;
+    a.totals = r.totals;
+    a.hit = r.hit;
+    a.inst_out = r.instance;
+    a.node_tests = r.node_tests;
+    a.triangle_tests = r.triangle_tests;
+    a.occluded = r.occluded;
     a.nodes = g.nodes.as<DevNode>();
     a.slots = g.slots.as<float4>();
     a.leaf_count = g.leaf_count.as<int32_t>();
@@ -499,8 +534,13 @@ int PrepareArgs(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, TraceArgs&
     a.instances = g.instances.as<DevInstance>();
     a.blas_roots = g.blas_roots.as<DevBlasRoot>();
     uint32_t* words = ctx->work_counter.as<uint32_t>();
-    a.counter = words + ctx->launch_parity;
-    a.next_counter = words + (1 - ctx->launch_parity);
+    if (r.work_counter != nullptr) {
+        a.counter = r.work_counter;
+        a.next_counter = nullptr;
+    } else {
+        a.counter = words + ctx->launch_parity;
+        a.next_counter = words + (1 - ctx->launch_parity);
+    }
     a.overflow = words + 2;
     a.spill = ctx->stack_spill.as<int2>();
     a.spill_stride = total;
@@ -512,50 +552,59 @@ bool Aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) ==
 
 }  // namespace
 
-int LaunchTraceClosest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, const yrh_hit_soa* hits) {
-    if (!Aligned16(rays->org_tmin) || !Aligned16(rays->dir_tmax) || !Aligned16(hits->hit)) {
+int LaunchTrace(yrh_context* ctx, const TraceRequest& r) {
+    if (!Aligned16(r.org) || !Aligned16(r.dir) || (r.hit != nullptr && !Aligned16(r.hit))) {
         return Fail(ctx, YRH_E_INVALID_ARGUMENT, "ray and hit arrays must be 16-byte aligned");
     }
+    if (r.any ? r.occluded == nullptr : r.hit == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "missing output array");
     (void)hipSetDevice(ctx->device);
     TraceArgs a;
     int grid = 0, block = 0;
-    int rc = PrepareArgs(ctx, rays, n, a, grid, block);
+    int rc = PrepareArgs(ctx, r, a, grid, block);
     if (rc != YRH_OK) return rc;
-    a.hit = reinterpret_cast<float4*>(hits->hit);
-    a.inst_out = hits->instance;
-    a.node_tests = hits->node_tests;
-    a.triangle_tests = hits->triangle_tests;
-    if (ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL) {
-        hipLaunchKernelGGL(TraceTwoLevelKernel<false>, dim3(grid), dim3(block), 0, ctx->stream, a);
+    const bool two = ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL;
+    if (r.any) {
+        if (two) {
+            hipLaunchKernelGGL(TraceTwoLevelKernel<true>, dim3(grid), dim3(block), 0, ctx->stream, a);
+        } else {
+            hipLaunchKernelGGL(TraceFlatKernel<true>, dim3(grid), dim3(block), 0, ctx->stream, a);
+        }
     } else {
-        hipLaunchKernelGGL(TraceFlatKernel<false>, dim3(grid), dim3(block), 0, ctx->stream, a);
+        if (two) {
+            hipLaunchKernelGGL(TraceTwoLevelKernel<false>, dim3(grid), dim3(block), 0, ctx->stream, a);
+        } else {
+            hipLaunchKernelGGL(TraceFlatKernel<false>, dim3(grid), dim3(block), 0, ctx->stream, a);
+        }
     }
-    rc = CheckHip(ctx, hipGetLastError(), "trace closest launch");
-    if (rc == YRH_OK) ctx->launch_parity = 1 - ctx->launch_parity;
+    rc = CheckHip(ctx, hipGetLastError(), r.any ? "trace any launch" : "trace closest launch");
+    if (rc == YRH_OK && r.work_counter == nullptr) ctx->launch_parity = 1 - ctx->launch_parity;
     return rc;
+}
+
+int LaunchTraceClosest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, const yrh_hit_soa* hits) {
+    TraceRequest r;
+    r.any = false;
+    r.org = reinterpret_cast<const float4*>(rays->org_tmin);
+    r.dir = reinterpret_cast<const float4*>(rays->dir_tmax);
+    r.n_host = n;
+    r.hit = reinterpret_cast<float4*>(hits->hit);
+    r.instance = hits->instance;
+    r.node_tests = hits->node_tests;
+    r.triangle_tests = hits->triangle_tests;
+    return LaunchTrace(ctx, r);
 }
 
 int LaunchTraceAny(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, uint32_t* occluded, uint32_t* node_tests,
                    uint32_t* triangle_tests) {
-    if (!Aligned16(rays->org_tmin) || !Aligned16(rays->dir_tmax)) {
-        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "ray arrays must be 16-byte aligned");
-    }
-    (void)hipSetDevice(ctx->device);
-    TraceArgs a;
-    int grid = 0, block = 0;
-    int rc = PrepareArgs(ctx, rays, n, a, grid, block);
-    if (rc != YRH_OK) return rc;
-    a.occluded = occluded;
-    a.node_tests = node_tests;
-    a.triangle_tests = triangle_tests;
-    if (ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL) {
-        hipLaunchKernelGGL(TraceTwoLevelKernel<true>, dim3(grid), dim3(block), 0, ctx->stream, a);
-    } else {
-        hipLaunchKernelGGL(TraceFlatKernel<true>, dim3(grid), dim3(block), 0, ctx->stream, a);
-    }
-    rc = CheckHip(ctx, hipGetLastError(), "trace any launch");
-    if (rc == YRH_OK) ctx->launch_parity = 1 - ctx->launch_parity;
-    return rc;
+    TraceRequest r;
+    r.any = true;
+    r.org = reinterpret_cast<const float4*>(rays->org_tmin);
+    r.dir = reinterpret_cast<const float4*>(rays->dir_tmax);
+    r.n_host = n;
+    r.occluded = occluded;
+    r.node_tests = node_tests;
+    r.triangle_tests = triangle_tests;
+    return LaunchTrace(ctx, r);
 }
 
 }  // namespace yrh

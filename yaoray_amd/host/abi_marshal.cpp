@@ -1,0 +1,132 @@
+#include "abi_marshal.hpp"
+
+#include <cstddef>
+#include <string>
+
+namespace yrhip {
+
+static_assert(sizeof(yr::RenderVertex) == sizeof(yrh_vertex));
+static_assert(sizeof(yr::RenderPrimitive) == sizeof(yrh_primitive));
+static_assert(offsetof(yr::RenderPrimitive, material_index) == offsetof(yrh_primitive, material_index));
+static_assert(offsetof(yr::RenderPrimitive, has_normals) == offsetof(yrh_primitive, has_normals));
+static_assert(sizeof(yr::RenderBvhNode) == sizeof(yrh_bvh_node));
+static_assert(offsetof(yr::RenderBvhNode, payload_offset) == offsetof(yrh_bvh_node, payload_offset));
+static_assert(sizeof(yr::RenderBvhPrimitive) == sizeof(yrh_bvh_primitive));
+static_assert(sizeof(yr::PreparedMeshInstance) == sizeof(yrh_instance));
+static_assert(offsetof(yr::PreparedMeshInstance, world_to_object) == offsetof(yrh_instance, world_to_object));
+static_assert(offsetof(yr::PreparedMeshInstance, world_bounds) == offsetof(yrh_instance, world_bounds_min));
+static_assert(sizeof(yr::RenderTlasPrimitive) == sizeof(yrh_tlas_primitive));
+static_assert(sizeof(yr::Color4f) == 16);
+
+yrh_bvh_desc DescribeBvh(const yr::RenderBvh& b) {
+    yrh_bvh_desc d{};
+    d.nodes = reinterpret_cast<const yrh_bvh_node*>(b.nodes.data());
+    d.node_count = static_cast<int32_t>(b.nodes.size());
+    d.primitive_indices = b.primitive_indices.data();
+    d.primitive_index_count = static_cast<int32_t>(b.primitive_indices.size());
+    d.primitives = reinterpret_cast<const yrh_bvh_primitive*>(b.primitives.data());
+    d.primitive_count = static_cast<int32_t>(b.primitives.size());
+    d.max_depth = b.max_depth;
+    return d;
+}
+
+void DescribeGeometry(const yr::RenderSceneIR& ir, const yr::RenderAcceleration& accel, Marshal& m,
+                      yrh_geometry_desc& g) {
+    g = yrh_geometry_desc{};
+    g.vertices = reinterpret_cast<const yrh_vertex*>(ir.vertices.data());
+    g.vertex_count = static_cast<int64_t>(ir.vertices.size());
+    g.indices = ir.indices.data();
+    g.index_count = static_cast<int64_t>(ir.indices.size());
+    g.primitives = reinterpret_cast<const yrh_primitive*>(ir.primitives.data());
+    g.primitive_count = static_cast<int32_t>(ir.primitives.size());
+    if (accel.kind == yr::RenderAccelerationKind::FlatReference) {
+        g.accel_kind = YRH_ACCEL_FLAT;
+        g.flat = DescribeBvh(accel.flat);
+        return;
+    }
+    const yr::TwoLevelBvh& tl = accel.two_level;
+    m.blas.clear();
+    for (const yr::RenderBlas& b : tl.blases) m.blas.push_back(DescribeBvh(b.bvh));
+    g.accel_kind = YRH_ACCEL_TWO_LEVEL;
+    g.blases = m.blas.data();
+    g.blas_count = static_cast<int32_t>(m.blas.size());
+    g.instances = reinterpret_cast<const yrh_instance*>(tl.instances.data());
+    g.instance_count = static_cast<int32_t>(tl.instances.size());
+    g.tlas = yrh_bvh_desc{};
+    g.tlas.nodes = reinterpret_cast<const yrh_bvh_node*>(tl.tlas.nodes.data());
+    g.tlas.node_count = static_cast<int32_t>(tl.tlas.nodes.size());
+    g.tlas.primitive_indices = tl.tlas.primitive_indices.data();
+    g.tlas.primitive_index_count = static_cast<int32_t>(tl.tlas.primitive_indices.size());
+    g.tlas.max_depth = tl.tlas.max_depth;
+    g.tlas_primitives = reinterpret_cast<const yrh_tlas_primitive*>(tl.tlas.primitives.data());
+    g.tlas_primitive_count = static_cast<int32_t>(tl.tlas.primitives.size());
+}
+
+std::string DescribeShading(const yr::RenderSceneIR& ir, Marshal& m, yrh_shading_desc& d) {
+    m.materials.clear();
+    for (const yr::RenderMaterial& mat : ir.materials) {
+        yrh_material a{};
+        switch (mat.kind) {
+            case yr::RenderMaterialKind::Diffuse: a.kind = YRH_MATERIAL_DIFFUSE; break;
+            case yr::RenderMaterialKind::Conductor: a.kind = YRH_MATERIAL_CONDUCTOR; break;
+            case yr::RenderMaterialKind::Dielectric: a.kind = YRH_MATERIAL_DIELECTRIC; break;
+            case yr::RenderMaterialKind::ThinDielectric: a.kind = YRH_MATERIAL_THIN_DIELECTRIC; break;
+            default: return "material kind outside the HIP backend's set";
+        }
+        a.reflectance[0] = mat.reflectance.x;
+        a.reflectance[1] = mat.reflectance.y;
+        a.reflectance[2] = mat.reflectance.z;
+        a.ior = mat.ior;
+        a.uroughness = mat.uroughness;
+        a.vroughness = mat.vroughness;
+        a.alpha = mat.alpha;
+        a.emission[0] = mat.emission.x;
+        a.emission[1] = mat.emission.y;
+        a.emission[2] = mat.emission.z;
+        a.absorption_distance = mat.absorption_distance;
+        a.absorption_color[0] = mat.absorption_color.x;
+        a.absorption_color[1] = mat.absorption_color.y;
+        a.absorption_color[2] = mat.absorption_color.z;
+        m.materials.push_back(a);
+    }
+    m.emissives.clear();
+    for (const yr::EmissivePrimitive& e : ir.emissive_primitives) {
+        m.emissives.push_back(yrh_emissive{e.primitive_index, {e.radiance.x, e.radiance.y, e.radiance.z}, e.area});
+    }
+    d = yrh_shading_desc{};
+    d.materials = m.materials.data();
+    d.material_count = static_cast<int32_t>(m.materials.size());
+    d.emissives = m.emissives.data();
+    d.emissive_count = static_cast<int32_t>(m.emissives.size());
+    d.alias_probability = ir.emissive_power.probability.data();
+    d.alias_index = ir.emissive_power.alias.data();
+    d.alias_pmf = ir.emissive_power.pmf.data();
+    yrh_environment_desc& e = d.environment;
+    const yr::RenderEnvironment& env = ir.environment;
+    e.active = env.active ? 1 : 0;
+    e.radiance[0] = env.radiance.x;
+    e.radiance[1] = env.radiance.y;
+    e.radiance[2] = env.radiance.z;
+    e.strength = env.strength;
+    e.rotation_radians = env.rotation_radians;
+    if (env.texture_index >= 0 && env.texture_index < static_cast<int>(ir.textures.size())) {
+        const yr::RenderTexture& t = ir.textures[static_cast<size_t>(env.texture_index)];
+        e.width = t.width;
+        e.height = t.height;
+        e.texels = reinterpret_cast<const float*>(t.texels.data());
+    }
+    if (env.distribution_index >= 0 && env.distribution_index < static_cast<int>(ir.environment_distributions.size())) {
+        const yr::RenderEnvironmentDistribution& dist = ir.environment_distributions[static_cast<size_t>(env.distribution_index)];
+        e.has_distribution = 1;
+        e.dist_width = dist.width;
+        e.dist_height = dist.height;
+        e.texel_weights = dist.texel_weights.data();
+        e.row_weights = dist.row_weights.data();
+        e.row_cdf = dist.row_cdf.data();
+        e.conditional_cdfs = dist.conditional_cdfs.data();
+        e.total_weight = dist.total_weight;
+    }
+    return {};
+}
+
+}  // namespace yrhip

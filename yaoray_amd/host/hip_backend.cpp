@@ -1,0 +1,297 @@
+// HipBackend: Prepare / Render over the C ABI, with the pass, progress,
+// cancel and resume semantics of the reference's CPU backend
+// (src/backends/cpu/cpu_backend.cpp:54-102, cpu_path_tracer.cpp:39-229).
+#include "hip_backend.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/yrh.h"
+#include "abi_marshal.hpp"
+
+namespace yr {
+
+Film::Film(int width, int height) : width_(width), height_(height) {
+    if (width <= 0 || height <= 0) throw std::invalid_argument("Film dimensions must be positive");
+    pixels_.resize(static_cast<size_t>(width) * static_cast<size_t>(height));
+}
+
+Color3f Film::LinearPixel(int x, int y) const {
+    if (x < 0 || y < 0 || x >= width_ || y >= height_) throw std::out_of_range("Film pixel coordinate out of range");
+    const FilmPixel& p = pixels_[static_cast<size_t>(y) * static_cast<size_t>(width_) + static_cast<size_t>(x)];
+    return p.samples == 0 ? Color3f{} : p.sum / static_cast<float>(p.samples);
+}
+
+std::uint32_t Film::SampleCount(int x, int y) const {
+    if (x < 0 || y < 0 || x >= width_ || y >= height_) throw std::out_of_range("Film pixel coordinate out of range");
+    return pixels_[static_cast<size_t>(y) * static_cast<size_t>(width_) + static_cast<size_t>(x)].samples;
+}
+
+float Film::LuminanceVariance(int x, int y) const {
+    const FilmPixel& p = pixels_[static_cast<size_t>(y) * static_cast<size_t>(width_) + static_cast<size_t>(x)];
+    return p.samples > 1 ? p.luminance_m2 / static_cast<float>(p.samples - 1) : 0.0f;
+}
+
+std::uint64_t Film::TotalSampleCount() const {
+    std::uint64_t total = 0;
+    for (const FilmPixel& p : pixels_) total += p.samples;
+    return total;
+}
+
+void Film::SetPixelForCheckpoint(int x, int y, FilmPixel pixel) {
+    if (x < 0 || y < 0 || x >= width_ || y >= height_) return;
+    pixels_[static_cast<size_t>(y) * static_cast<size_t>(width_) + static_cast<size_t>(x)] = pixel;
+}
+
+class HipPreparedScene final : public PreparedScene {
+public:
+    HipPreparedScene(RenderJob j, RenderAcceleration a, yrh_context* c, int cus)
+        : job(std::move(j)), accel(std::move(a)), ctx(c), compute_units(cus) {}
+    ~HipPreparedScene() override { yrh_destroy(ctx); }
+    HipPreparedScene(const HipPreparedScene&) = delete;
+    HipPreparedScene& operator=(const HipPreparedScene&) = delete;
+    RenderBackendKind Kind() const override { return RenderBackendKind::Hip; }
+    const RenderSceneIR& SourceScene() const override { return job.scene; }
+    const RenderSettings& Settings() const override { return job.settings; }
+
+    RenderJob job;
+    RenderAcceleration accel;
+    yrh_context* ctx = nullptr;
+    int compute_units = 0;
+    mutable bool rendering = false;
+};
+
+namespace {
+
+std::string LastError(yrh_context* ctx, const char* what) {
+    return std::string(what) + ": " + (ctx != nullptr ? yrh_last_error(ctx) : "no context");
+}
+
+void ToFilm(const std::vector<yrh_film_pixel>& src, Film& film, std::uint64_t bad) {
+    for (int y = 0; y < film.Height(); ++y) {
+        for (int x = 0; x < film.Width(); ++x) {
+            const yrh_film_pixel& s = src[static_cast<size_t>(y) * static_cast<size_t>(film.Width()) + static_cast<size_t>(x)];
+            FilmPixel p;
+            p.sum = Color3f{static_cast<float>(s.sum[0]), static_cast<float>(s.sum[1]), static_cast<float>(s.sum[2])};
+            p.samples = s.samples;
+            p.luminance_mean = static_cast<float>(s.luminance_mean);
+            p.luminance_m2 = static_cast<float>(s.luminance_m2);
+            p.albedo_sum = Color3f{static_cast<float>(s.albedo_sum[0]), static_cast<float>(s.albedo_sum[1]),
+                                   static_cast<float>(s.albedo_sum[2])};
+            p.normal_sum = Vec3f{static_cast<float>(s.normal_sum[0]), static_cast<float>(s.normal_sum[1]),
+                                 static_cast<float>(s.normal_sum[2])};
+            p.depth_sum = static_cast<float>(s.depth_sum);
+            p.aov_samples = s.aov_samples;
+            film.SetPixelForCheckpoint(x, y, p);
+        }
+    }
+    film.SetBadSampleCount(bad);
+}
+
+std::vector<yrh_film_pixel> FromFilm(const Film& film) {
+    std::vector<yrh_film_pixel> out(film.Pixels().size());
+    for (size_t i = 0; i < out.size(); ++i) {
+        const FilmPixel& p = film.Pixels()[i];
+        yrh_film_pixel& d = out[i];
+        d.sum[0] = p.sum.x;
+        d.sum[1] = p.sum.y;
+        d.sum[2] = p.sum.z;
+        d.samples = p.samples;
+        d.luminance_mean = p.luminance_mean;
+        d.luminance_m2 = p.luminance_m2;
+        d.albedo_sum[0] = p.albedo_sum.x;
+        d.albedo_sum[1] = p.albedo_sum.y;
+        d.albedo_sum[2] = p.albedo_sum.z;
+        d.normal_sum[0] = p.normal_sum.x;
+        d.normal_sum[1] = p.normal_sum.y;
+        d.normal_sum[2] = p.normal_sum.z;
+        d.depth_sum = p.depth_sum;
+        d.aov_samples = p.aov_samples;
+    }
+    return out;
+}
+
+int SamplerCode(RenderSamplerKind k) {
+    switch (k) {
+        case RenderSamplerKind::Independent: return YRH_SAMPLER_INDEPENDENT;
+        case RenderSamplerKind::Stratified: return YRH_SAMPLER_STRATIFIED;
+        case RenderSamplerKind::OwenSobol: return YRH_SAMPLER_OWEN_SOBOL;
+        case RenderSamplerKind::ZSobol: return YRH_SAMPLER_ZSOBOL;
+    }
+    return YRH_SAMPLER_INDEPENDENT;
+}
+
+}  // namespace
+
+std::unique_ptr<RenderBackend> CreateRenderBackend(RenderBackendKind kind) {
+    if (kind == RenderBackendKind::Hip) return std::make_unique<HipBackend>(0);
+    return nullptr;
+}
+
+RenderBackendCapabilities HipBackend::Capabilities() const {
+    return RenderBackendCapabilities{RenderBackendKind::Hip, true, false, true, true, false};
+}
+
+BackendPrepareResult HipBackend::Prepare(RenderJob job) {
+    BackendPrepareResult result;
+    if (job.settings.requested_backend != RenderBackendKind::Hip) {
+        result.error = "HIP backend cannot prepare a scene requested for another backend";
+        return result;
+    }
+    const auto start = std::chrono::steady_clock::now();
+    PrepareLightSampling(job.scene);
+    const auto bvh_start = std::chrono::steady_clock::now();
+    RenderAccelerationBuildResult build = BuildRenderAcceleration(job.scene.Geometry());
+    const auto bvh_end = std::chrono::steady_clock::now();
+    result.acceleration_build_seconds = std::chrono::duration<double>(bvh_end - bvh_start).count();
+    if (!build.errors.empty()) {
+        result.error = build.errors.front();
+        return result;
+    }
+    yrh_context* ctx = nullptr;
+    if (yrh_create(device_, &ctx) != YRH_OK) {
+        result.error = "no HIP device " + std::to_string(device_);
+        return result;
+    }
+    yrhip::Marshal m;
+    yrh_geometry_desc g{};
+    yrhip::DescribeGeometry(job.scene, build.acceleration, m, g);
+    if (yrh_upload_geometry(ctx, &g) != YRH_OK) {
+        result.error = LastError(ctx, "geometry upload");
+        yrh_destroy(ctx);
+        return result;
+    }
+    yrh_shading_desc sd{};
+    const std::string err = yrhip::DescribeShading(job.scene, m, sd);
+    if (!err.empty() || yrh_upload_shading(ctx, &sd) != YRH_OK) {
+        result.error = err.empty() ? LastError(ctx, "shading upload") : err;
+        yrh_destroy(ctx);
+        return result;
+    }
+    int cus = 0;
+    yrh_device_info(ctx, nullptr, 0, &cus);
+    result.scene = std::make_unique<HipPreparedScene>(std::move(job), std::move(build.acceleration), ctx, cus);
+    result.ok = true;
+    result.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    return result;
+}
+
+RenderResult HipBackend::Render(const PreparedScene& scene, const RenderRequest& request) {
+    RenderResult result;
+    const auto* hip = dynamic_cast<const HipPreparedScene*>(&scene);
+    if (scene.Kind() != RenderBackendKind::Hip || hip == nullptr) {
+        result.error = "HIP backend received a non-HIP prepared scene.";
+        return result;
+    }
+    if (hip->rendering) throw std::logic_error("HIP prepared scene is already rendering");
+    struct Guard {
+        const HipPreparedScene* s;
+        ~Guard() { s->rendering = false; }
+    } guard{hip};
+    hip->rendering = true;
+
+    const RenderSettings& settings = hip->Settings();
+    yrh_context* ctx = hip->ctx;
+    Film film = request.resume_film != nullptr ? *request.resume_film : Film(settings.width, settings.height);
+    result.stats.bvh_nodes = hip->accel.NodeCount();
+    result.stats.bvh_max_depth = hip->accel.MaxDepth();
+    result.stats.threads = hip->compute_units;
+    auto fail = [&](std::string message) {
+        result.ok = false;
+        result.error = std::move(message);
+        result.film.emplace(film);
+        return result;
+    };
+    const int spp = std::max(1, settings.spp);
+    if (request.resume_completed_spp < 0 || request.resume_completed_spp > spp) return fail("invalid resume completed spp");
+    if (request.resume_film != nullptr) {
+        if (request.resume_film->Width() != settings.width || request.resume_film->Height() != settings.height) {
+            return fail("resume film dimensions do not match render scene");
+        }
+        for (const FilmPixel& p : request.resume_film->Pixels()) {
+            if (p.samples != static_cast<std::uint32_t>(request.resume_completed_spp)) {
+                return fail("resume film sample counts do not match resume completed spp");
+            }
+        }
+    }
+    const auto start = std::chrono::steady_clock::now();
+    if (request.stop_token.stop_requested()) return fail("render cancelled");
+    if (yrh_film_reset(ctx, settings.width, settings.height) != YRH_OK) return fail(LastError(ctx, "film reset"));
+    const int64_t npix = static_cast<int64_t>(settings.width) * settings.height;
+    std::vector<yrh_film_pixel> device_film(static_cast<size_t>(npix));
+    if (request.resume_film != nullptr) {
+        device_film = FromFilm(film);
+        if (yrh_film_load(ctx, device_film.data(), npix) != YRH_OK) return fail(LastError(ctx, "film load"));
+    }
+    yrh_render_desc d{};
+    const RenderCamera& c = hip->job.scene.camera;
+    const float cam[12] = {c.origin.x, c.origin.y, c.origin.z, c.forward.x, c.forward.y, c.forward.z,
+                           c.right.x,  c.right.y,  c.right.z,  c.up.x,      c.up.y,      c.up.z};
+    std::copy(cam, cam + 3, d.camera_origin);
+    std::copy(cam + 3, cam + 6, d.camera_forward);
+    std::copy(cam + 6, cam + 9, d.camera_right);
+    std::copy(cam + 9, cam + 12, d.camera_up);
+    d.tan_half_fov = std::tan(c.fov_y_radians * 0.5f);
+    d.width = settings.width;
+    d.height = settings.height;
+    d.sampler = SamplerCode(settings.sampler);
+    d.spp = spp;
+    d.max_depth = settings.max_depth;
+    d.seed = settings.seed;
+    d.radiance_clamp = settings.radiance_clamp;
+
+    yrh_render_stats st{};
+    const std::uint64_t resumed_bad = request.resume_film != nullptr ? request.resume_film->BadSampleCount() : 0;
+    auto sync_film = [&]() -> bool {
+        if (yrh_film_read(ctx, device_film.data(), npix) != YRH_OK) return false;
+        if (yrh_render_stats_read(ctx, &st) != YRH_OK) return false;
+        ToFilm(device_film, film, resumed_bad + st.bad_samples);
+        return true;
+    };
+    auto fill_stats = [&]() {
+        result.stats.rays_traced = st.rays_traced;
+        result.stats.shadow_rays = st.shadow_rays;
+        result.stats.occluded_shadow_rays = st.occluded_shadow_rays;
+        result.stats.triangle_tests = st.triangle_tests;
+        result.stats.sphere_tests = st.sphere_tests;
+        result.stats.bvh_node_tests = st.bvh_node_tests;
+        result.stats.hits = st.hits;
+        result.stats.misses = st.misses;
+        result.stats.samples_rendered = film.TotalSampleCount();
+        result.stats.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    };
+    for (int pass = request.resume_completed_spp; pass < spp; ++pass) {
+        if (request.stop_token.stop_requested()) {
+            if (!sync_film()) return fail(LastError(ctx, "film read"));
+            fill_stats();
+            return fail("render cancelled");
+        }
+        d.sample_index = pass;
+        if (yrh_render_pass(ctx, &d) != YRH_OK) return fail(LastError(ctx, "render pass"));
+        if (request.progress_callback) {
+            if (!sync_film()) return fail(LastError(ctx, "film read"));
+            fill_stats();
+            const RenderProgress progress{pass + 1,
+                                          spp,
+                                          film.TotalSampleCount(),
+                                          static_cast<std::uint64_t>(spp) * static_cast<std::uint64_t>(npix),
+                                          pass + 1 < spp ? static_cast<std::uint64_t>(npix) : 0,
+                                          0,
+                                          st.rays_traced,
+                                          result.stats.elapsed_seconds};
+            const RenderProgressDecision decision = request.progress_callback(progress, film);
+            if (decision.cancel) {
+                return fail(decision.error.empty() ? "render cancelled by progress callback" : decision.error);
+            }
+        }
+    }
+    if (!sync_film()) return fail(LastError(ctx, "film read"));
+    fill_stats();
+    result.ok = true;
+    result.film.emplace(std::move(film));
+    return result;
+}
+
+}  // namespace yr

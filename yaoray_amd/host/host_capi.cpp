@@ -2,35 +2,27 @@
 #include "yaoray_host.h"
 
 #include <chrono>
+#include <cmath>
 #include <cstddef>
 #include <cstring>
 #include <memory>
 #include <new>
+#include <optional>
+#include <stop_token>
 #include <string>
 
+#include "abi_marshal.hpp"
+#include "hip_backend.hpp"
 #include "scenes.hpp"
 #include "yr_ir.hpp"
-
-// The reference IR arrays are handed to libyrh as-is: the C structs in yrh.h
-// must be byte-compatible with the yr:: types.
-static_assert(sizeof(yr::RenderVertex) == sizeof(yrh_vertex));
-static_assert(sizeof(yr::RenderPrimitive) == sizeof(yrh_primitive));
-static_assert(offsetof(yr::RenderPrimitive, material_index) == offsetof(yrh_primitive, material_index));
-static_assert(offsetof(yr::RenderPrimitive, has_normals) == offsetof(yrh_primitive, has_normals));
-static_assert(sizeof(yr::RenderBvhNode) == sizeof(yrh_bvh_node));
-static_assert(offsetof(yr::RenderBvhNode, payload_offset) == offsetof(yrh_bvh_node, payload_offset));
-static_assert(sizeof(yr::RenderBvhPrimitive) == sizeof(yrh_bvh_primitive));
-static_assert(sizeof(yr::PreparedMeshInstance) == sizeof(yrh_instance));
-static_assert(offsetof(yr::PreparedMeshInstance, world_to_object) == offsetof(yrh_instance, world_to_object));
-static_assert(offsetof(yr::PreparedMeshInstance, world_bounds) == offsetof(yrh_instance, world_bounds_min));
-static_assert(sizeof(yr::RenderTlasPrimitive) == sizeof(yrh_tlas_primitive));
 
 struct yrhost_scene {
     yr::RenderSceneIR ir;
     yr::RenderAcceleration accel;
     bool accel_built = false;
     std::string error;
-    std::vector<yrh_bvh_desc> blas_desc;
+    // Converted tables the C-ABI descriptors point into.
+    mutable yrhip::Marshal marshal;
 };
 
 namespace {
@@ -38,18 +30,6 @@ namespace {
 int SetError(yrhost_scene* s, const std::string& msg) {
     if (s != nullptr) s->error = msg;
     return YRH_E_INVALID_ARGUMENT;
-}
-
-yrh_bvh_desc Describe(const yr::RenderBvh& b) {
-    yrh_bvh_desc d{};
-    d.nodes = reinterpret_cast<const yrh_bvh_node*>(b.nodes.data());
-    d.node_count = static_cast<int32_t>(b.nodes.size());
-    d.primitive_indices = b.primitive_indices.data();
-    d.primitive_index_count = static_cast<int32_t>(b.primitive_indices.size());
-    d.primitives = reinterpret_cast<const yrh_bvh_primitive*>(b.primitives.data());
-    d.primitive_count = static_cast<int32_t>(b.primitives.size());
-    d.max_depth = b.max_depth;
-    return d;
 }
 
 }  // namespace
@@ -128,6 +108,65 @@ int yrhost_scene_from_arrays(const yrh_vertex* vertices, int64_t vertex_count, c
     return YRH_OK;
 }
 
+int yrhost_scene_set_camera(yrhost_scene* s, const float* c) {
+    if (s == nullptr || c == nullptr) return YRH_E_INVALID_ARGUMENT;
+    yr::RenderCamera& cam = s->ir.camera;
+    cam.origin = yr::Point3f{c[0], c[1], c[2]};
+    cam.forward = yr::Vec3f{c[3], c[4], c[5]};
+    cam.right = yr::Vec3f{c[6], c[7], c[8]};
+    cam.up = yr::Vec3f{c[9], c[10], c[11]};
+    cam.fov_y_radians = c[12];
+    return YRH_OK;
+}
+
+int yrhost_scene_set_materials(yrhost_scene* s, const yrh_material* m, int32_t count) {
+    if (s == nullptr || m == nullptr || count <= 0) return YRH_E_INVALID_ARGUMENT;
+    s->ir.materials.clear();
+    for (int32_t k = 0; k < count; ++k) {
+        yr::RenderMaterial r;
+        switch (m[k].kind) {
+            case YRH_MATERIAL_DIFFUSE: r.kind = yr::RenderMaterialKind::Diffuse; break;
+            case YRH_MATERIAL_CONDUCTOR: r.kind = yr::RenderMaterialKind::Conductor; break;
+            case YRH_MATERIAL_DIELECTRIC: r.kind = yr::RenderMaterialKind::Dielectric; break;
+            case YRH_MATERIAL_THIN_DIELECTRIC: r.kind = yr::RenderMaterialKind::ThinDielectric; break;
+            default: return SetError(s, "unknown material kind");
+        }
+        r.reflectance = yr::Color3f{m[k].reflectance[0], m[k].reflectance[1], m[k].reflectance[2]};
+        r.ior = m[k].ior;
+        r.uroughness = m[k].uroughness;
+        r.vroughness = m[k].vroughness;
+        r.alpha = m[k].alpha;
+        r.emission = yr::Color3f{m[k].emission[0], m[k].emission[1], m[k].emission[2]};
+        r.absorption_distance = m[k].absorption_distance;
+        r.absorption_color = yr::Color3f{m[k].absorption_color[0], m[k].absorption_color[1], m[k].absorption_color[2]};
+        s->ir.materials.push_back(r);
+    }
+    return YRH_OK;
+}
+
+int yrhost_scene_set_emissives(yrhost_scene* s, const yrh_emissive* e, int32_t count) {
+    if (s == nullptr || count < 0 || (count > 0 && e == nullptr)) return YRH_E_INVALID_ARGUMENT;
+    s->ir.emissive_primitives.clear();
+    for (int32_t k = 0; k < count; ++k) {
+        s->ir.emissive_primitives.push_back(
+            yr::EmissivePrimitive{e[k].primitive_index, yr::Color3f{e[k].radiance[0], e[k].radiance[1], e[k].radiance[2]},
+                                  e[k].area});
+    }
+    yr::PrepareLightSampling(s->ir);
+    return YRH_OK;
+}
+
+int yrhost_scene_set_constant_environment(yrhost_scene* s, const float* radiance3, float strength) {
+    if (s == nullptr || radiance3 == nullptr) return YRH_E_INVALID_ARGUMENT;
+    s->ir.environment = yr::RenderEnvironment{};
+    s->ir.environment.active = false;
+    s->ir.environment.radiance = yr::Color3f{radiance3[0], radiance3[1], radiance3[2]};
+    s->ir.environment.strength = strength;
+    s->ir.textures.clear();
+    s->ir.environment_distributions.clear();
+    return YRH_OK;
+}
+
 void yrhost_scene_destroy(yrhost_scene* scene) { delete scene; }
 
 const char* yrhost_scene_error(const yrhost_scene* scene) { return scene == nullptr ? "null scene" : scene->error.c_str(); }
@@ -179,21 +218,21 @@ int yrhost_build_accel(yrhost_scene* s, int thread_count, double* seconds) {
     }
     s->accel = std::move(r.acceleration);
     s->accel_built = true;
-    s->blas_desc.clear();
-    for (const yr::RenderBlas& b : s->accel.two_level.blases) s->blas_desc.push_back(Describe(b.bvh));
+    s->marshal.blas.clear();
+    for (const yr::RenderBlas& b : s->accel.two_level.blases) s->marshal.blas.push_back(yrhip::DescribeBvh(b.bvh));
     return YRH_OK;
 }
 
 int yrhost_accel_flat(const yrhost_scene* s, yrh_bvh_desc* out) {
     if (s == nullptr || out == nullptr || !s->accel_built) return YRH_E_INVALID_ARGUMENT;
-    *out = Describe(s->accel.flat);
+    *out = yrhip::DescribeBvh(s->accel.flat);
     return YRH_OK;
 }
 
 int yrhost_accel_blas(const yrhost_scene* s, int32_t index, yrh_bvh_desc* out) {
     if (s == nullptr || out == nullptr || !s->accel_built) return YRH_E_INVALID_ARGUMENT;
-    if (index < 0 || index >= static_cast<int32_t>(s->blas_desc.size())) return YRH_E_INVALID_ARGUMENT;
-    *out = s->blas_desc[static_cast<size_t>(index)];
+    if (index < 0 || index >= static_cast<int32_t>(s->marshal.blas.size())) return YRH_E_INVALID_ARGUMENT;
+    *out = s->marshal.blas[static_cast<size_t>(index)];
     return YRH_OK;
 }
 
@@ -216,25 +255,164 @@ int yrhost_accel_tlas(const yrhost_scene* s, yrh_bvh_desc* tlas, const yrh_tlas_
     return YRH_OK;
 }
 
+int yrhost_geometry_desc(const yrhost_scene* s, yrh_geometry_desc* out) {
+    if (s == nullptr || out == nullptr || !s->accel_built) return YRH_E_INVALID_ARGUMENT;
+    yrhip::DescribeGeometry(s->ir, s->accel, s->marshal, *out);
+    return YRH_OK;
+}
+
 int yrhost_upload_geometry(const yrhost_scene* s, yrh_context* ctx) {
-    if (s == nullptr || ctx == nullptr || !s->accel_built) return YRH_E_INVALID_ARGUMENT;
+    if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
     yrh_geometry_desc g{};
-    g.vertices = reinterpret_cast<const yrh_vertex*>(s->ir.vertices.data());
-    g.vertex_count = static_cast<int64_t>(s->ir.vertices.size());
-    g.indices = s->ir.indices.data();
-    g.index_count = static_cast<int64_t>(s->ir.indices.size());
-    g.primitives = reinterpret_cast<const yrh_primitive*>(s->ir.primitives.data());
-    g.primitive_count = static_cast<int32_t>(s->ir.primitives.size());
-    if (s->accel.kind == yr::RenderAccelerationKind::FlatReference) {
-        g.accel_kind = YRH_ACCEL_FLAT;
-        g.flat = Describe(s->accel.flat);
-    } else {
-        g.accel_kind = YRH_ACCEL_TWO_LEVEL;
-        g.blases = s->blas_desc.data();
-        g.blas_count = static_cast<int32_t>(s->blas_desc.size());
-        yrhost_accel_tlas(s, &g.tlas, &g.tlas_primitives, &g.tlas_primitive_count, &g.instances, &g.instance_count);
-    }
+    const int rc = yrhost_geometry_desc(s, &g);
+    if (rc != YRH_OK) return rc;
     return yrh_upload_geometry(ctx, &g);
+}
+
+int yrhost_shading_desc(yrhost_scene* s, yrh_shading_desc* out) {
+    if (s == nullptr || out == nullptr) return YRH_E_INVALID_ARGUMENT;
+    const std::string err = yrhip::DescribeShading(s->ir, s->marshal, *out);
+    return err.empty() ? YRH_OK : SetError(s, err);
+}
+
+int yrhost_upload_shading(yrhost_scene* s, yrh_context* ctx) {
+    if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
+    yrh_shading_desc d{};
+    const int rc = yrhost_shading_desc(s, &d);
+    if (rc != YRH_OK) return rc;
+    return yrh_upload_shading(ctx, &d);
+}
+
+int yrhost_render_desc(const yrhost_scene* s, int32_t width, int32_t height, int32_t sampler, int32_t spp,
+                       int32_t max_depth, uint64_t seed, float radiance_clamp, int32_t sample_index,
+                       yrh_render_desc* out) {
+    if (s == nullptr || out == nullptr) return YRH_E_INVALID_ARGUMENT;
+    const yr::RenderCamera& c = s->ir.camera;
+    yrh_render_desc& d = *out;
+    d = yrh_render_desc{};
+    const float cam[12] = {c.origin.x, c.origin.y, c.origin.z, c.forward.x, c.forward.y, c.forward.z,
+                           c.right.x,  c.right.y,  c.right.z,  c.up.x,      c.up.y,      c.up.z};
+    std::memcpy(d.camera_origin, cam, 12);
+    std::memcpy(d.camera_forward, cam + 3, 12);
+    std::memcpy(d.camera_right, cam + 6, 12);
+    std::memcpy(d.camera_up, cam + 9, 12);
+    // The reference evaluates std::tan(fov_y_radians * 0.5f) per ray (camera.cpp:14).
+    d.tan_half_fov = std::tan(c.fov_y_radians * 0.5f);
+    d.width = width;
+    d.height = height;
+    d.sampler = sampler;
+    d.spp = spp;
+    d.max_depth = max_depth;
+    d.seed = seed;
+    d.radiance_clamp = radiance_clamp;
+    d.sample_index = sample_index;
+    return YRH_OK;
+}
+
+int yrhost_backend_render(const yrhost_scene* s, const yrhost_render_settings* settings,
+                          const yrhost_render_request* request, yrhost_film_pixel* film_out, int64_t film_capacity,
+                          yrhost_render_report* report) {
+    static_assert(sizeof(yrhost_film_pixel) == sizeof(yr::FilmPixel));
+    if (s == nullptr || settings == nullptr || report == nullptr) return YRH_E_INVALID_ARGUMENT;
+    *report = yrhost_render_report{};
+    auto set_error = [&](const std::string& e) {
+        std::strncpy(report->error, e.c_str(), sizeof(report->error) - 1);
+        report->error[sizeof(report->error) - 1] = '\0';
+    };
+    yr::RenderJob job;
+    job.scene = s->ir;
+    job.settings.requested_backend = settings->requested_backend == 0 ? yr::RenderBackendKind::Hip
+                                                                      : yr::RenderBackendKind::Cpu;
+    switch (settings->sampler) {
+        case YRH_SAMPLER_INDEPENDENT: job.settings.sampler = yr::RenderSamplerKind::Independent; break;
+        case YRH_SAMPLER_STRATIFIED: job.settings.sampler = yr::RenderSamplerKind::Stratified; break;
+        case YRH_SAMPLER_OWEN_SOBOL: job.settings.sampler = yr::RenderSamplerKind::OwenSobol; break;
+        case YRH_SAMPLER_ZSOBOL: job.settings.sampler = yr::RenderSamplerKind::ZSobol; break;
+        default: return YRH_E_INVALID_ARGUMENT;
+    }
+    job.settings.width = settings->width;
+    job.settings.height = settings->height;
+    job.settings.spp = settings->spp;
+    job.settings.max_depth = settings->max_depth;
+    job.settings.seed = settings->seed;
+    job.settings.radiance_clamp = settings->radiance_clamp;
+
+    yr::HipBackend backend(settings->device);
+    yr::BackendPrepareResult prepared = backend.Prepare(std::move(job));
+    report->prepare_seconds = prepared.elapsed_seconds;
+    report->acceleration_build_seconds = prepared.acceleration_build_seconds;
+    if (!prepared.ok) {
+        set_error(prepared.error);
+        return YRH_OK;
+    }
+    report->prepare_ok = 1;
+
+    std::optional<yr::Film> resume;
+    yr::RenderRequest req;
+    std::stop_source stop;
+    req.stop_token = stop.get_token();
+    std::string cancel_error;
+    if (request != nullptr) {
+        if (request->resume_film != nullptr) {
+            try {
+                resume.emplace(request->resume_width, request->resume_height);
+            } catch (const std::exception& e) {
+                set_error(e.what());
+                return YRH_OK;
+            }
+            for (int y = 0; y < request->resume_height; ++y) {
+                for (int x = 0; x < request->resume_width; ++x) {
+                    yr::FilmPixel p;
+                    std::memcpy(&p, &request->resume_film[static_cast<size_t>(y) * request->resume_width + x], sizeof(p));
+                    resume->SetPixelForCheckpoint(x, y, p);
+                }
+            }
+            req.resume_film = &*resume;
+        }
+        req.resume_completed_spp = request->resume_completed_spp;
+        if (request->stop_before_start != 0) stop.request_stop();
+        if (request->cancel_error != nullptr) cancel_error = request->cancel_error;
+    }
+    const int cancel_after = request != nullptr ? request->cancel_after_passes : 0;
+    req.progress_callback = [&](const yr::RenderProgress& p, const yr::Film&) {
+        ++report->progress_calls;
+        report->last_progress_spp = p.completed_spp;
+        report->last_progress_samples = p.completed_samples;
+        yr::RenderProgressDecision d;
+        if (cancel_after > 0 && report->progress_calls >= cancel_after) {
+            d.cancel = true;
+            d.error = cancel_error;
+        }
+        return d;
+    };
+    const yr::RenderResult r = backend.Render(*prepared.scene, req);
+    report->ok = r.ok ? 1 : 0;
+    set_error(r.error);
+    const yr::RenderStats& st = r.stats;
+    report->threads = st.threads;
+    report->bvh_nodes = st.bvh_nodes;
+    report->bvh_max_depth = st.bvh_max_depth;
+    report->samples_rendered = st.samples_rendered;
+    report->rays_traced = st.rays_traced;
+    report->shadow_rays = st.shadow_rays;
+    report->occluded_shadow_rays = st.occluded_shadow_rays;
+    report->triangle_tests = st.triangle_tests;
+    report->sphere_tests = st.sphere_tests;
+    report->bvh_node_tests = st.bvh_node_tests;
+    report->hits = st.hits;
+    report->misses = st.misses;
+    report->elapsed_seconds = st.elapsed_seconds;
+    if (r.film.has_value()) {
+        report->film_valid = 1;
+        report->film_width = r.film->Width();
+        report->film_height = r.film->Height();
+        report->bad_samples = r.film->BadSampleCount();
+        const std::vector<yr::FilmPixel>& px = r.film->Pixels();
+        if (film_out != nullptr && film_capacity >= static_cast<int64_t>(px.size())) {
+            std::memcpy(film_out, px.data(), px.size() * sizeof(yr::FilmPixel));
+        }
+    }
+    return YRH_OK;
 }
 
 int yrhost_camera_rays(const yrhost_scene* s, int width, int height, float t_min, float t_max, float* org_tmin,

@@ -81,12 +81,79 @@ class HitSoa(C.Structure):
                 ("triangle_tests", C.c_void_p)]
 
 
+class Material(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reflectance", C.c_float * 3), ("ior", C.c_float), ("uroughness", C.c_float),
+                ("vroughness", C.c_float), ("alpha", C.c_float), ("emission", C.c_float * 3),
+                ("absorption_distance", C.c_float), ("absorption_color", C.c_float * 3), ("reserved", C.c_float)]
+
+
+class Emissive(C.Structure):
+    _fields_ = [("primitive_index", C.c_int32), ("radiance", C.c_float * 3), ("area", C.c_float)]
+
+
+class EnvironmentDesc(C.Structure):
+    _fields_ = [("active", C.c_int32), ("radiance", C.c_float * 3), ("strength", C.c_float),
+                ("rotation_radians", C.c_float), ("width", C.c_int32), ("height", C.c_int32),
+                ("texels", C.c_void_p), ("has_distribution", C.c_int32), ("dist_width", C.c_int32),
+                ("dist_height", C.c_int32), ("texel_weights", C.c_void_p), ("row_weights", C.c_void_p),
+                ("row_cdf", C.c_void_p), ("conditional_cdfs", C.c_void_p), ("total_weight", C.c_float)]
+
+
+class ShadingDesc(C.Structure):
+    _fields_ = [("materials", C.c_void_p), ("material_count", C.c_int32), ("emissives", C.c_void_p),
+                ("emissive_count", C.c_int32), ("alias_probability", C.c_void_p), ("alias_index", C.c_void_p),
+                ("alias_pmf", C.c_void_p), ("environment", EnvironmentDesc)]
+
+
+class RenderDesc(C.Structure):
+    _fields_ = [("camera_origin", C.c_float * 3), ("camera_forward", C.c_float * 3),
+                ("camera_right", C.c_float * 3), ("camera_up", C.c_float * 3), ("tan_half_fov", C.c_float),
+                ("width", C.c_int32), ("height", C.c_int32), ("sampler", C.c_int32), ("spp", C.c_int32),
+                ("max_depth", C.c_int32), ("seed", C.c_uint64), ("radiance_clamp", C.c_float),
+                ("sample_index", C.c_int32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("rays_traced", "shadow_rays", "occluded_shadow_rays", "triangle_tests",
+                                            "sphere_tests", "bvh_node_tests", "hits", "misses", "bad_samples")]
+
+
+SAMPLER_INDEPENDENT = 0
+SAMPLER_STRATIFIED = 1
+SAMPLER_OWEN_SOBOL = 2
+SAMPLER_ZSOBOL = 3
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("vertex_count", C.c_int64), ("index_count", C.c_int64), ("primitive_count", C.c_int32),
                 ("instance_count", C.c_int32), ("material_count", C.c_int32), ("emissive_count", C.c_int32),
                 ("accel_built", C.c_int32), ("accel_kind", C.c_int32), ("node_count", C.c_int32),
                 ("max_depth", C.c_int32), ("blas_count", C.c_int32), ("environment_active", C.c_int32),
                 ("camera", C.c_float * 13)]
+
+
+class RenderSettings(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("sampler", C.c_int32), ("spp", C.c_int32),
+                ("max_depth", C.c_int32), ("radiance_clamp", C.c_float), ("seed", C.c_uint64),
+                ("device", C.c_int32), ("requested_backend", C.c_int32)]
+
+
+class RenderRequest(C.Structure):
+    _fields_ = [("resume_film", C.c_void_p), ("resume_width", C.c_int32), ("resume_height", C.c_int32),
+                ("resume_completed_spp", C.c_int32), ("stop_before_start", C.c_int32),
+                ("cancel_after_passes", C.c_int32), ("cancel_error", C.c_char_p)]
+
+
+class RenderReport(C.Structure):
+    _fields_ = ([(k, C.c_int32) for k in ("prepare_ok", "ok", "film_valid", "film_width", "film_height",
+                                          "progress_calls", "threads", "bvh_nodes", "bvh_max_depth",
+                                          "last_progress_spp")]
+                + [(k, C.c_uint64) for k in ("samples_rendered", "rays_traced", "shadow_rays",
+                                             "occluded_shadow_rays", "triangle_tests", "sphere_tests",
+                                             "bvh_node_tests", "hits", "misses", "bad_samples",
+                                             "last_progress_samples")]
+                + [(k, C.c_double) for k in ("elapsed_seconds", "prepare_seconds", "acceleration_build_seconds")]
+                + [("error", C.c_char * 256)])
 
 
 def _sig(lib, name, restype, argtypes):
@@ -123,12 +190,24 @@ YRH_SIGNATURES = {
     "yrh_event_destroy": (_I, [_VP, _VP]),
     "yrh_event_record": (_I, [_VP, _VP]),
     "yrh_event_elapsed_ms": (_I, [_VP, _VP, _VP, C.POINTER(_F)]),
+    "yrh_upload_shading": (_I, [_VP, C.POINTER(ShadingDesc)]),
+    "yrh_film_reset": (_I, [_VP, _I32, _I32]),
+    "yrh_film_load": (_I, [_VP, _VP, _I64]),
+    "yrh_film_read": (_I, [_VP, _VP, _I64]),
+    "yrh_render_pass": (_I, [_VP, C.POINTER(RenderDesc)]),
+    "yrh_render_stats_read": (_I, [_VP, C.POINTER(RenderStats)]),
+    "yrh_sample_values": (_I, [_VP, _I32, _U64, _I32, _I64, _VP, _VP, _VP, _VP, _I32, _VP]),
+    "yrh_generate_camera_rays": (_I, [_VP, C.POINTER(RenderDesc), _VP, _VP]),
 }
 
 HOST_SIGNATURES = {
     "yrhost_scene_create": (_I, [_I, C.POINTER(_I64), _I, _U64, C.POINTER(_VP)]),
     "yrhost_scene_from_arrays": (_I, [_VP, _I64, _VP, _I64, _VP, _I32, _VP, _VP, _I32, C.POINTER(_VP)]),
     "yrhost_scene_destroy": (None, [_VP]),
+    "yrhost_scene_set_camera": (_I, [_VP, _VP]),
+    "yrhost_scene_set_materials": (_I, [_VP, _VP, _I32]),
+    "yrhost_scene_set_emissives": (_I, [_VP, _VP, _I32]),
+    "yrhost_scene_set_constant_environment": (_I, [_VP, _VP, _F]),
     "yrhost_scene_error": (C.c_char_p, [_VP]),
     "yrhost_scene_info_get": (_I, [_VP, C.POINTER(SceneInfo)]),
     "yrhost_scene_geometry": (_I, [_VP, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP)]),
@@ -137,7 +216,13 @@ HOST_SIGNATURES = {
     "yrhost_accel_blas": (_I, [_VP, _I32, C.POINTER(BvhDesc)]),
     "yrhost_accel_tlas": (_I, [_VP, C.POINTER(BvhDesc), C.POINTER(_VP), C.POINTER(_I32), C.POINTER(_VP),
                                C.POINTER(_I32)]),
+    "yrhost_geometry_desc": (_I, [_VP, C.POINTER(GeometryDesc)]),
     "yrhost_upload_geometry": (_I, [_VP, _VP]),
+    "yrhost_shading_desc": (_I, [_VP, C.POINTER(ShadingDesc)]),
+    "yrhost_upload_shading": (_I, [_VP, _VP]),
+    "yrhost_render_desc": (_I, [_VP, _I32, _I32, _I32, _I32, _I32, _U64, _F, _I32, C.POINTER(RenderDesc)]),
+    "yrhost_backend_render": (_I, [_VP, C.POINTER(RenderSettings), C.POINTER(RenderRequest), _VP, _I64,
+                                   C.POINTER(RenderReport)]),
     "yrhost_camera_rays": (_I, [_VP, _I, _I, _F, _F, _VP, _VP]),
     "yrhost_random_rays": (_I, [_I64, _U64, _F, _F, _VP, _VP]),
 }
