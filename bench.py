@@ -9,6 +9,12 @@ library, byte-identical to yr::BuildBvh); 2048x2048 coherent pinhole rays
 4,194,304 incoherent rays (origins U[0,1]^3, directions uniform, seed 3);
 t_min 1e-5, t_max +inf. One step = one closest-hit pass over all 8,388,608
 rays, rays and tree resident in HBM. Prints one JSON line on stdout.
+
+After it, the render workloads (`renders` in the JSON line): the seeded
+Cornell box (256x256, 16 spp, depth 8) and the two-level instanced scene
+(1920x1080, 64 spp, depth 16); one step = one whole frame, timed with events
+on the context stream, beside the reference's RenderCpuPathTrace on the host
+cores (the whole Cornell frame; a 480x270 x 8 spp sample of the two-level one).
 """
 import argparse
 import json
@@ -86,6 +92,9 @@ def main():
     ap.add_argument("--incoherent", type=int, default=4_194_304)
     ap.add_argument("--cpu-stride", type=int, default=4, help="CPU baseline traces every k-th ray")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--renders", default="cornell,two_level",
+                    help="render workloads timed after the ray-cast microbench (comma list, empty for none)")
+    ap.add_argument("--render-frames", type=int, default=3)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
@@ -179,6 +188,11 @@ def main():
     if not args.no_cpu_baseline:
         cpu, parity = cpu_baseline(scene, org, dirs, hits, nt_closest, tt_closest, args.cpu_stride, n_coherent)
 
+    renders = {}
+    for key in [k for k in args.renders.split(",") if k]:
+        renders[key] = bench_render(dev, key, args.render_frames, min(16, len(os.sched_getaffinity(0))),
+                                    not args.no_cpu_baseline)
+
     traffic = load_traffic("closest")
     result = {
         "metric": METRIC,
@@ -231,9 +245,99 @@ def main():
         },
         "closest_kernel_mrays": round(n / avg_closest_s / 1e6, 2),
         "parity_sample": parity,
+        "renders": renders,
         "guided_vs_plain": None,
     }
     print(json.dumps(result), flush=True)
+
+
+RENDER_WORKLOADS = {
+    "cornell": dict(desc="seeded Cornell box (34 tris, ceiling emitter L=17), 256x256, 16 spp OwenSobol, depth 8",
+                    width=256, height=256, spp=16, max_depth=8, batch=16,
+                    cpu=dict(width=256, height=256, spp=16)),
+    "two_level": dict(desc="ground + 10k instanced 4x-subdivided icospheres (51.2M instanced tris; diffuse / GGX "
+                           "conductor / dielectric), 2048x1024 sky+sun env, 1920x1080, 64 spp OwenSobol, depth 16",
+                      width=1920, height=1080, spp=64, max_depth=16, batch=4,
+                      cpu=dict(width=480, height=270, spp=8)),
+}
+
+
+def bench_render(dev, key, frames, cpu_threads, with_cpu):
+    """One step = one whole frame (all spp) of the render workload; rays counted as the
+    reference counts rays_traced (camera + continuation segments), shadow rays apart."""
+    from yaoray_amd import Scene
+
+    w = RENDER_WORKLOADS[key]
+    t = time.perf_counter()
+    scene = Scene.cornell(1) if key == "cornell" else Scene.instanced()
+    scene.build_accel()
+    log(f"render {key}: scene ready in {time.perf_counter() - t:.2f}s")
+    dev.upload_scene(scene)
+    dev.upload_shading(scene)
+    kw = dict(max_depth=w["max_depth"], upload=False, batch=w["batch"])
+    dev.render(scene, w["width"], w["height"], min(w["spp"], w["batch"]), **kw)  # warm-up
+    dev.synchronize()
+    secs = []
+    st = None
+    film = None
+    for _ in range(frames):
+        e0, e1 = dev.event(), dev.event()
+        dev.record(e0)
+        film, st = dev.render(scene, w["width"], w["height"], w["spp"], **kw)
+        dev.record(e1)
+        dev.synchronize()
+        secs.append(dev.elapsed_ms(e0, e1) / 1e3)
+        dev.destroy_event(e0)
+        dev.destroy_event(e1)
+    sec = float(np.median(secs))
+    out = {
+        "workload": w["desc"],
+        "frames": frames,
+        "ms_per_frame": round(sec * 1e3, 3),
+        "value": round(st["rays_traced"] / sec / 1e6, 2),
+        "unit": "Mrays/s (path segments, rays_traced)",
+        "shadow_Mrays_s": round(st["shadow_rays"] / sec / 1e6, 2),
+        "samples_per_s": round(w["width"] * w["height"] * w["spp"] / sec, 1),
+        "rays_traced": st["rays_traced"],
+        "shadow_rays": st["shadow_rays"],
+        "mean_node_tests_per_ray": round(st["bvh_node_tests"] / max(1, st["rays_traced"] + st["shadow_rays"]), 2),
+        "samples_per_pass": w["batch"],
+        "cpu_baseline": None,
+    }
+    log(f"render {key}: {out['ms_per_frame']} ms/frame, {out['value']} Mrays/s (+{out['shadow_Mrays_s']} shadow)")
+    if with_cpu:
+        from oracle import oracle_py as O
+
+        c = w["cpu"]
+        if O.ref_available():
+            kind = "reference"
+            rfilm, rst, rsec = O.ref_render(scene, c["width"], c["height"], c["spp"], max_depth=w["max_depth"],
+                                            threads=cpu_threads)
+        else:
+            kind = "port"
+            t0 = time.perf_counter()
+            rfilm, rst = O.render(scene, c["width"], c["height"], c["spp"], max_depth=w["max_depth"],
+                                  threads=cpu_threads)
+            rsec = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": round(rst["rays_traced"] / rsec / 1e6, 4),
+            "unit": "Mrays/s (path segments, rays_traced)",
+            "cores": cpu_threads,
+            "kind": kind,
+            "sample": f"{c['width']}x{c['height']} at {c['spp']} spp of the same scene and settings, "
+                      f"RenderCpuPathTrace, {rsec:.2f}s",
+        }
+        if (c["width"], c["height"], c["spp"]) == (w["width"], w["height"], w["spp"]):
+            g = film["sum"] / np.maximum(film["samples"], 1)[:, None]
+            r = rfilm["sum"] / np.maximum(rfilm["samples"], 1)[:, None]
+            out["parity"] = {
+                "checked_against": kind,
+                "rel_mean_diff": float(abs(g.mean() - r.mean()) / max(abs(r.mean()), 1e-30)),
+                "pixels_bit_equal_fp32": float(np.mean(np.all(g.astype(np.float32) == r.astype(np.float32), axis=1))),
+                "rays_traced_equal": bool(rst["rays_traced"] == st["rays_traced"]),
+            }
+        log(f"render {key}: cpu {kind} {out['cpu_baseline']['value']} Mrays/s on {cpu_threads} threads")
+    return out
 
 
 def cpu_baseline(scene, org, dirs, hits, nt, tt, stride, n_coherent):

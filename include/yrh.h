@@ -283,7 +283,8 @@ int yrh_upload_shading(yrh_context* ctx, const yrh_shading_desc* shading);
 #define YRH_SAMPLER_ZSOBOL 3
 
 /* One pass = one sample for every pixel of the frame, the unit the reference
- * commits per pass (src/backends/cpu/cpu_path_tracer.cpp:109-180). */
+ * commits per pass (src/backends/cpu/cpu_path_tracer.cpp:109-180); a pass may
+ * carry several consecutive sample indices (sample_count) to fill the GPU. */
 typedef struct yrh_render_desc {
     float camera_origin[3];
     float camera_forward[3];
@@ -297,7 +298,9 @@ typedef struct yrh_render_desc {
     int32_t max_depth;
     uint64_t seed;
     float radiance_clamp;
-    int32_t sample_index; /* this pass's sample index (= completed spp) */
+    int32_t sample_index; /* this pass's first sample index (= completed spp) */
+    int32_t sample_count; /* consecutive sample indices rendered by this pass (0 means 1); the
+                             film receives them in index order, as that many passes would add them */
 } yrh_render_desc;
 
 /* yr::RenderStats counters (include/yaoray/runtime/backend.hpp:40-55) plus the

@@ -116,3 +116,21 @@ def test_two_level_environment_statistical_parity(device):
     assert rel_mean < 0.005
     assert rmse < 0.35 * noise
     assert max(counts.values()) < 0.01
+
+
+@pytest.mark.parametrize("kind", ["cornell", "two_level"])
+def test_multi_sample_pass_equals_single_passes(device, kind):
+    """A pass carrying several sample indices leaves the same FP64 film and counters
+    as one pass per sample index."""
+    if kind == "cornell":
+        s = Scene.cornell(1)
+        w, h, spp, depth = 40, 30, 7, 8
+    else:
+        s = Scene.instanced(instances=40, subdivisions=2, variants=3, env_width=64, env_height=32, seed=5)
+        w, h, spp, depth = 32, 20, 5, 10
+    s.build_accel()
+    one, st1 = device.render(s, w, h, spp, max_depth=depth)
+    many, st3 = device.render(s, w, h, spp, max_depth=depth, batch=3)
+    for field in one.dtype.names:
+        assert np.array_equal(one[field], many[field]), field
+    assert st1 == st3

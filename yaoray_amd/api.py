@@ -452,15 +452,17 @@ class Device:
         return {k: int(getattr(s, k)) for k, _ in N.RenderStats._fields_}
 
     def render(self, scene: Scene, width, height, spp, max_depth=8, sampler=N.SAMPLER_OWEN_SOBOL,
-               seed=0x59414F524159, radiance_clamp=0.0, first_sample=0, upload=True):
-        """Uploads the scene (unless upload=False), renders passes [first_sample, spp) and returns
-        (FP64 film records, stats)."""
+               seed=0x59414F524159, radiance_clamp=0.0, first_sample=0, upload=True, batch=1):
+        """Uploads the scene (unless upload=False), renders samples [first_sample, spp), `batch`
+        sample indices per pass, and returns (FP64 film records, stats)."""
         if upload:
             self.upload_scene(scene)
             self.upload_shading(scene)
         self.film_reset(width, height)
-        for s in range(first_sample, spp):
-            self.render_pass(scene.render_desc(width, height, spp, max_depth, sampler, seed, radiance_clamp, s))
+        for s in range(first_sample, spp, batch):
+            d = scene.render_desc(width, height, spp, max_depth, sampler, seed, radiance_clamp, s)
+            d.sample_count = min(batch, spp - s)
+            self.render_pass(d)
         self.synchronize()
         return self.film_read(width, height), self.render_stats()
 

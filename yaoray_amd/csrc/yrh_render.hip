@@ -183,6 +183,8 @@ struct LightView {
 struct PassView {
     int32_t width, height;
     int32_t sampler, spp, max_depth, sample_index;
+    int32_t sample_count;  // paths = npix * sample_count; path p renders sample sample_index + p / npix
+    int64_t npix;
     uint64_t seed;
     float radiance_clamp;
     DevCamera camera;
@@ -511,13 +513,14 @@ __device__ __forceinline__ float EmissivePdfSolidAngle(const LightView& L, float
 
 __global__ void RaygenKernel(PassView P) {
     const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t total = static_cast<int64_t>(P.width) * P.height;
+    const int64_t total = P.npix * P.sample_count;
     if (p == 0) P.counters[kCountQueue0] = static_cast<uint32_t>(total);
     if (p >= total) return;
-    const int x = static_cast<int>(p % P.width);
-    const int y = static_cast<int>(p / P.width);
+    const int64_t pixel = p % P.npix;
+    const int x = static_cast<int>(pixel % P.width);
+    const int y = static_cast<int>(pixel / P.width);
     PixelSampler smp;
-    smp.Init(P.sampler, P.seed, x, y, P.sample_index, P.spp);
+    smp.Init(P.sampler, P.seed, x, y, P.sample_index + static_cast<int>(p / P.npix), P.spp);
     const float2 s = smp.At2D(kDimPixel, 0);
     const float w = static_cast<float>(P.width < 1 ? 1 : P.width);
     const float h = static_cast<float>(P.height < 1 ? 1 : P.height);
@@ -525,7 +528,7 @@ __global__ void RaygenKernel(PassView P) {
     P.q_org[0][p] = F4(P.camera.origin, 1.0e-5f);
     P.q_dir[0][p] = F4(d, __int_as_float(0x7f800000));
     P.q_path[0][p] = static_cast<int32_t>(p);
-    P.p_pixel[p] = static_cast<int32_t>(p);
+    P.p_pixel[p] = static_cast<int32_t>(pixel);
     P.p_tp[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     P.p_rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.p_prev[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -586,7 +589,7 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
     P.p_slot[p] = make_int2(-1, -1);
     const int pix = P.p_pixel[p];
     PixelSampler smp;
-    smp.Init(P.sampler, P.seed, pix % P.width, pix / P.width, P.sample_index, P.spp);
+    smp.Init(P.sampler, P.seed, pix % P.width, pix / P.width, P.sample_index + static_cast<int>(p / P.npix), P.spp);
     smp.bounce = depth;
 
     if (tri < 0) {
@@ -936,21 +939,22 @@ __global__ void ResetLoopCountKernel(uint32_t* counters, int which) {
     if (threadIdx.x == 0) counters[kCountLoop0 + which] = 0u;
 }
 
-// Adds one pass's samples to the FP64 film (Film::AddSample, src/film/film.cpp:32-56).
+// Adds the pass's samples to the FP64 film (Film::AddSample, src/film/film.cpp:32-56):
+// one thread per pixel, samples in index order.
 __global__ __launch_bounds__(kBlock) void AccumulateKernel(PassView P) {
-    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t total = static_cast<int64_t>(P.width) * P.height;
+    const int64_t pixel = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     unsigned bad = 0;
-    if (p < total) {
+    for (int k = 0; pixel < P.npix && k < P.sample_count; ++k) {
+        const int64_t p = static_cast<int64_t>(k) * P.npix + pixel;
         float3 b = XYZ(P.p_rad[p]);
         if (P.radiance_clamp > 0.0f) {
             const float m = MaxC(b);
             if (!(m <= P.radiance_clamp || m <= 0.0f)) b = Mul(b, P.radiance_clamp / m);
         }
         if (!Finite3(b)) {
-            bad = 1;
+            ++bad;
         } else {
-            yrh_film_pixel& px = P.film[P.p_pixel[p]];
+            yrh_film_pixel& px = P.film[pixel];
             px.sum[0] += static_cast<double>(b.x);
             px.sum[1] += static_cast<double>(b.y);
             px.sum[2] += static_cast<double>(b.z);
@@ -1005,6 +1009,8 @@ PassView MakePassView(RenderState& R, const yrh_render_desc& d) {
     P.spp = d.spp;
     P.max_depth = d.max_depth;
     P.sample_index = d.sample_index;
+    P.sample_count = d.sample_count < 1 ? 1 : d.sample_count;
+    P.npix = static_cast<int64_t>(d.width) * d.height;
     P.seed = d.seed;
     P.radiance_clamp = d.radiance_clamp;
     P.camera.origin = make_float3(d.camera_origin[0], d.camera_origin[1], d.camera_origin[2]);
@@ -1413,9 +1419,14 @@ int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* d) {
     RenderState& R = *ctx->render;
     if (d->width != R.film_w || d->height != R.film_h) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "film size does not match the pass");
     if (d->sampler < YRH_SAMPLER_INDEPENDENT || d->sampler > YRH_SAMPLER_ZSOBOL) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "unknown sampler");
-    if (d->sample_index < 0) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "negative sample index");
-    const int64_t paths = static_cast<int64_t>(d->width) * d->height;
-    if (paths > (int64_t{1} << 30)) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "frame too large for one pass");
+    if (d->sample_index < 0 || d->sample_count < 0) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "negative sample index or count");
+    const int64_t count = d->sample_count < 1 ? 1 : d->sample_count;
+    if (static_cast<int64_t>(d->sample_index) + count > (int64_t{1} << 31) - 1) {
+        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "sample index overflow");
+    }
+    const int64_t npix = static_cast<int64_t>(d->width) * d->height;
+    const int64_t paths = npix * count;
+    if (paths > (int64_t{1} << 30)) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "pass too large (pixels x samples > 2^30)");
     (void)hipSetDevice(ctx->device);
     int rc = EnsureCapacity(ctx, R, paths);
     if (rc != YRH_OK) return rc;
@@ -1473,7 +1484,7 @@ int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* d) {
         }
         hipLaunchKernelGGL(ResolveKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P, cur);
     }
-    hipLaunchKernelGGL(AccumulateKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P);
+    hipLaunchKernelGGL(AccumulateKernel, dim3(Grid(npix)), dim3(kBlock), 0, st, P);
     hipLaunchKernelGGL(CopyTraceTotalsKernel, dim3(1), dim3(64), 0, st, P.stats);
     return CheckHip(ctx, hipGetLastError(), "render pass launch");
 }
@@ -1511,6 +1522,7 @@ int yrh_generate_camera_rays(yrh_context* ctx, const yrh_render_desc* d, float* 
     int rc = EnsureCapacity(ctx, R, paths);
     if (rc != YRH_OK) return rc;
     PassView P = MakePassView(R, *d);
+    P.sample_count = 1;  // the caller's arrays hold one ray per pixel
     P.q_org[0] = reinterpret_cast<float4*>(org_tmin);
     P.q_dir[0] = reinterpret_cast<float4*>(dir_tmax);
     hipLaunchKernelGGL(RaygenKernel, dim3(Grid(paths)), dim3(kBlock), 0, ctx->stream, P);

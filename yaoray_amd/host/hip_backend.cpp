@@ -3,6 +3,7 @@
 // (src/backends/cpu/cpu_backend.cpp:54-102, cpu_path_tracer.cpp:39-229).
 #include "hip_backend.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <stdexcept>
@@ -262,13 +263,21 @@ RenderResult HipBackend::Render(const PreparedScene& scene, const RenderRequest&
         result.stats.samples_rendered = film.TotalSampleCount();
         result.stats.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     };
-    for (int pass = request.resume_completed_spp; pass < spp; ++pass) {
+    // Without a progress callback nothing observes the film between passes, so
+    // consecutive passes are batched into one wavefront of up to kBatchPaths
+    // paths (same per-pixel sample order, so the same film).
+    constexpr int64_t kBatchPaths = int64_t{1} << 23;
+    const int batch = request.progress_callback
+        ? 1
+        : static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(spp, kBatchPaths / std::max<int64_t>(1, npix))));
+    for (int pass = request.resume_completed_spp; pass < spp; pass += d.sample_count) {
         if (request.stop_token.stop_requested()) {
             if (!sync_film()) return fail(LastError(ctx, "film read"));
             fill_stats();
             return fail("render cancelled");
         }
         d.sample_index = pass;
+        d.sample_count = std::min(batch, spp - pass);
         if (yrh_render_pass(ctx, &d) != YRH_OK) return fail(LastError(ctx, "render pass"));
         if (request.progress_callback) {
             if (!sync_film()) return fail(LastError(ctx, "film read"));

@@ -110,7 +110,7 @@ class RenderDesc(C.Structure):
                 ("camera_right", C.c_float * 3), ("camera_up", C.c_float * 3), ("tan_half_fov", C.c_float),
                 ("width", C.c_int32), ("height", C.c_int32), ("sampler", C.c_int32), ("spp", C.c_int32),
                 ("max_depth", C.c_int32), ("seed", C.c_uint64), ("radiance_clamp", C.c_float),
-                ("sample_index", C.c_int32)]
+                ("sample_index", C.c_int32), ("sample_count", C.c_int32)]
 
 
 class RenderStats(C.Structure):
