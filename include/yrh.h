@@ -303,6 +303,27 @@ typedef struct yrh_render_desc {
                              film receives them in index order, as that many passes would add them */
 } yrh_render_desc;
 
+/* ---- learned path guiding (no counterpart in the reference) -------------------
+ * A hash-grid + MLP proposal of 8 vMF lobes over directions, conditioned on
+ * position, shading normal and outgoing direction, trained online on the
+ * integrator's own path vertices and combined with BSDF sampling by
+ * one-sample MIS (so renders stay unbiased). */
+typedef struct yrh_guide_desc {
+    float bounds_min[3];      /* scene box the position encoding spans */
+    float bounds_max[3];
+    int32_t log2_table_size;  /* hash-grid entries per level = 2^log2_table_size (10..22) */
+    int32_t min_resolution;   /* coarsest level resolution (>= 2) */
+    int32_t max_resolution;   /* finest level resolution (>= min_resolution) */
+    float learning_rate;      /* Adam step size */
+    float guide_fraction;     /* probability of sampling the guide at a guidable vertex (0..1) */
+    int32_t sample_capacity;  /* training samples kept between yrh_guide_train calls */
+    uint64_t seed;            /* weight initialisation */
+} yrh_guide_desc;
+
+#define YRH_GUIDE_OFF 0     /* plain integrator (the reference's estimator) */
+#define YRH_GUIDE_SAMPLE 1  /* guided sampling */
+#define YRH_GUIDE_RECORD 2  /* guided sampling + training-sample recording */
+
 /* yr::RenderStats counters (include/yaoray/runtime/backend.hpp:40-55) plus the
  * film's bad (non-finite) sample count. */
 typedef struct yrh_render_stats {
@@ -350,6 +371,41 @@ int yrh_sample_values(yrh_context* ctx, int32_t kind, uint64_t seed, int32_t spp
 /* Device ray generation for one pass (GeneratePerspectiveCameraRay with the
  * sampler's Pixel dimension): writes width*height rays into device arrays. */
 int yrh_generate_camera_rays(yrh_context* ctx, const yrh_render_desc* desc, float* org_tmin, float* dir_tmax);
+
+/* ---- Learned path guiding (no counterpart in the reference) ---------------- */
+
+/* Allocates and initialises the proposal (weights from desc->seed). */
+int yrh_guide_configure(yrh_context* ctx, const yrh_guide_desc* desc);
+/* YRH_GUIDE_OFF / _SAMPLE / _RECORD for the following render passes. */
+int yrh_guide_set_mode(yrh_context* ctx, int32_t mode);
+/* `steps` Adam steps of `batch` recorded samples each (consecutive windows of
+ * the recorded set), then forgets the recorded samples. */
+int yrh_guide_train(yrh_context* ctx, int32_t steps, int32_t batch);
+/* Appends n host samples of 13 floats (position, normal, outgoing, sampled
+ * direction, weight > 0) to the recorded set, as if a render pass had
+ * recorded them (samples beyond the capacity are dropped). */
+int yrh_guide_add_samples(yrh_context* ctx, int64_t n, const float* samples);
+/* Recorded training samples and their mean weight (synchronises). */
+int yrh_guide_sample_stats(yrh_context* ctx, int64_t* count, float* mean_weight);
+/* Sizes: MLP parameters (FP32 [in][out] matrices W0..W3), grid parameters
+ * (levels x entries x 2), per-level resolutions and dense flags (8 each). */
+int yrh_guide_info(yrh_context* ctx, int32_t* mlp_params, int64_t* grid_params, int32_t* resolutions,
+                   int32_t* dense);
+/* FP32 master parameters to / from host arrays. */
+int yrh_guide_params_read(yrh_context* ctx, float* mlp, float* grid);
+int yrh_guide_params_write(yrh_context* ctx, const float* mlp, const float* grid);
+/* Network evaluation for n host rows of 9 floats (position, normal, outgoing
+ * direction): writes 40 floats per row (pi[8], kappa[8], mu[8][3]). */
+int yrh_guide_query(yrh_context* ctx, int64_t n, const float* inputs, float* mixture);
+/* One forward + backward over n host samples of 13 floats (position, normal,
+ * outgoing, sampled direction, weight): loss = mean(-(w / w_norm) log q(wi)).
+ * Writes the FP32 gradients of every parameter; no update. */
+int yrh_guide_gradients(yrh_context* ctx, int64_t n, const float* samples, float w_norm, float* grad_mlp,
+                        float* grad_grid, float* loss);
+/* Mixture pdf at dirs (n x 3) and mixture samples from u (n x 3) written as
+ * (direction, pdf) rows of 4 floats. */
+int yrh_guide_mixture_eval(yrh_context* ctx, int64_t n, const float* mixture, const float* dirs, const float* u,
+                           float* pdf, float* samples);
 
 /* ---- Timing on the context's stream --------------------------------------- */
 

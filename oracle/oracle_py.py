@@ -290,3 +290,61 @@ def camera_rays(render_desc):
     dirs = np.zeros((n, 4), np.float32)
     lib.oracle_camera_rays(C.cast(C.pointer(render_desc), _VP), org.ctypes.data, dirs.ctypes.data)
     return org, dirs
+
+
+# ---- learned guiding (oracle_guide.cpp) ------------------------------------------------
+
+
+def _guide_lib():
+    lib = oracle_lib()
+    if not hasattr(lib, "_guide_sig"):
+        lib.oracle_guide_config.argtypes = [_VP, _VP, _VP]
+        lib.oracle_guide_forward.argtypes = [_VP, _VP, _VP, _I64, _VP, _I32, _VP]
+        lib.oracle_guide_gradients.argtypes = [_VP, _VP, _VP, _I64, _VP, C.c_double, _I32, _VP, _VP, _VP]
+        lib.oracle_guide_loss.argtypes = [_VP, _VP, _VP, _I64, _VP, C.c_double]
+        lib.oracle_guide_loss.restype = C.c_double
+        lib.oracle_guide_mixture_eval.argtypes = [_I64, _VP, _VP, _VP, _VP, _VP]
+        lib._guide_sig = True
+    return lib
+
+
+def guide_config(desc):
+    res, dense = np.zeros(8, np.int32), np.zeros(8, np.int32)
+    _guide_lib().oracle_guide_config(C.cast(C.pointer(desc), _VP), res.ctypes.data, dense.ctypes.data)
+    return res, dense
+
+
+def guide_forward(desc, mlp, grid, rows, bf16=False):
+    r = _c(np.asarray(rows, np.float32).reshape(-1, 9))
+    m, g = _c(np.asarray(mlp, np.float32)), _c(np.asarray(grid, np.float32))
+    out = np.zeros((len(r), 40), np.float32)
+    _guide_lib().oracle_guide_forward(C.cast(C.pointer(desc), _VP), m.ctypes.data, g.ctypes.data, len(r),
+                                      r.ctypes.data, 1 if bf16 else 0, out.ctypes.data)
+    return out
+
+
+def guide_gradients(desc, mlp, grid, samples, w_norm, bf16=False):
+    s = _c(np.asarray(samples, np.float32).reshape(-1, 13))
+    m, g = _c(np.asarray(mlp, np.float32)), _c(np.asarray(grid, np.float32))
+    gm, gg, loss = np.zeros(len(m)), np.zeros(len(g)), C.c_double()
+    _guide_lib().oracle_guide_gradients(C.cast(C.pointer(desc), _VP), m.ctypes.data, g.ctypes.data, len(s),
+                                        s.ctypes.data, w_norm, 1 if bf16 else 0, gm.ctypes.data, gg.ctypes.data,
+                                        C.byref(loss))
+    return gm, gg, loss.value
+
+
+def guide_loss(desc, mlp, grid, samples, w_norm):
+    s = _c(np.asarray(samples, np.float32).reshape(-1, 13))
+    m, g = _c(np.asarray(mlp, np.float32)), _c(np.asarray(grid, np.float32))
+    return _guide_lib().oracle_guide_loss(C.cast(C.pointer(desc), _VP), m.ctypes.data, g.ctypes.data, len(s),
+                                          s.ctypes.data, w_norm)
+
+
+def guide_mixture_eval(mixture, dirs, u):
+    m = _c(np.asarray(mixture, np.float32).reshape(-1, 40))
+    d = _c(np.asarray(dirs, np.float32).reshape(-1, 3))
+    uu = _c(np.asarray(u, np.float32).reshape(-1, 3))
+    pdf, smp = np.zeros(len(m), np.float32), np.zeros((len(m), 4), np.float32)
+    _guide_lib().oracle_guide_mixture_eval(len(m), m.ctypes.data, d.ctypes.data, uu.ctypes.data, pdf.ctypes.data,
+                                           smp.ctypes.data)
+    return pdf, smp

@@ -1,7 +1,10 @@
 """Small in-code scenes mirroring the reference's test scenes
 (tests/accel/bvh_test_scene.hpp:9-53) and array helpers."""
+import ctypes as C
+
 import numpy as np
 
+from yaoray_amd import native as N
 from yaoray_amd.api import PRIMITIVE_DTYPE, VERTEX_DTYPE, Scene
 
 
@@ -109,3 +112,40 @@ def base_path_scene():
     s.set_constant_environment((0.02, 0.03, 0.04), 1.0)
     s.build_accel()
     return s
+
+
+# ---- learned guiding ---------------------------------------------------------------
+
+
+def guide_desc(log2_table=12, rmin=4, rmax=64, bounds=((0, 0, 0), (1, 1, 1)), capacity=1 << 17, lr=1e-2):
+    return N.GuideDesc((C.c_float * 3)(*bounds[0]), (C.c_float * 3)(*bounds[1]), log2_table, rmin, rmax, lr, 0.5,
+                       capacity, 1)
+
+
+def guide_params(d, seed, grid_scale=0.5):
+    rng = np.random.default_rng(seed)
+    mlp = np.zeros(N.GUIDE_MLP_PARAMS, np.float32)
+    dims = [(32, 64), (64, 64), (64, 64), (64, 48)]
+    off = 0
+    for k, (i, o) in enumerate(dims):
+        a = np.sqrt(6.0 / i) * (0.5 if k == 3 else 1.0)
+        w = rng.uniform(-a, a, (i, o)).astype(np.float32)
+        if k == 3:
+            w[:, 40:] = 0.0
+        mlp[off:off + i * o] = w.ravel()
+        off += i * o
+    grid = rng.uniform(-grid_scale, grid_scale, 8 * (1 << d.log2_table_size) * 2).astype(np.float32)
+    return mlp, grid
+
+
+def guide_samples(n, seed):
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(0.05, 0.95, (n, 3))
+    nrm = rng.normal(size=(n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    wo = rng.normal(size=(n, 3))
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    wi = rng.normal(size=(n, 3))
+    wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    w = rng.uniform(0.1, 2.0, (n, 1))
+    return np.hstack([p, nrm, wo, wi, w]).astype(np.float32)

@@ -117,6 +117,11 @@ class Scene:
         return cls._create(N.SCENE_INSTANCED, [instances, subdivisions, variants, env_width, env_height], seed)
 
     @classmethod
+    def room(cls, furniture_triangles=200000, env_width=1024, env_height=512, seed: int = 6) -> "Scene":
+        """Guiding workload: window-lit room with a hidden ceiling lamp (flat BVH)."""
+        return cls._create(N.SCENE_ROOM, [furniture_triangles, env_width, env_height], seed)
+
+    @classmethod
     def from_arrays(cls, vertices, indices, primitives, instance_primitive=None, instance_matrices=None):
         lib = N.host()
         vertices = np.ascontiguousarray(vertices, dtype=VERTEX_DTYPE)
@@ -480,6 +485,73 @@ class Device:
         self._check(self._lib.yrh_generate_camera_rays(self._ctx, C.byref(desc), d_org.ptr, d_dir.ptr), "camera_rays")
         self.synchronize()
         return d_org.download(np.float32, (n, 4)), d_dir.download(np.float32, (n, 4))
+
+    # -- learned guiding -----------------------------------------------------
+    def guide_configure(self, bounds_min, bounds_max, log2_table_size=16, min_resolution=8, max_resolution=256,
+                        learning_rate=1e-2, guide_fraction=0.5, sample_capacity=1 << 20, seed=7):
+        d = N.GuideDesc((C.c_float * 3)(*bounds_min), (C.c_float * 3)(*bounds_max), log2_table_size, min_resolution,
+                        max_resolution, learning_rate, guide_fraction, sample_capacity, seed)
+        self._check(self._lib.yrh_guide_configure(self._ctx, C.byref(d)), "guide_configure")
+        return d
+
+    def guide_set_mode(self, mode: int):
+        self._check(self._lib.yrh_guide_set_mode(self._ctx, mode), "guide_set_mode")
+
+    def guide_train(self, steps: int, batch: int):
+        self._check(self._lib.yrh_guide_train(self._ctx, steps, batch), "guide_train")
+
+    def guide_add_samples(self, samples: np.ndarray):
+        smp = np.ascontiguousarray(samples, np.float32).reshape(-1, 13)
+        self._check(self._lib.yrh_guide_add_samples(self._ctx, len(smp), smp.ctypes.data), "guide_add_samples")
+
+    def guide_sample_stats(self):
+        n, w = C.c_int64(), C.c_float()
+        self._check(self._lib.yrh_guide_sample_stats(self._ctx, C.byref(n), C.byref(w)), "guide_sample_stats")
+        return n.value, w.value
+
+    def guide_info(self):
+        m, g = C.c_int32(), C.c_int64()
+        res, dense = np.zeros(8, np.int32), np.zeros(8, np.int32)
+        self._check(self._lib.yrh_guide_info(self._ctx, C.byref(m), C.byref(g), res.ctypes.data, dense.ctypes.data),
+                    "guide_info")
+        return m.value, g.value, res, dense
+
+    def guide_params(self):
+        m, g, _, _ = self.guide_info()
+        mlp, grid = np.zeros(m, np.float32), np.zeros(g, np.float32)
+        self._check(self._lib.yrh_guide_params_read(self._ctx, mlp.ctypes.data, grid.ctypes.data), "guide_params")
+        return mlp, grid
+
+    def guide_set_params(self, mlp=None, grid=None):
+        m = None if mlp is None else np.ascontiguousarray(mlp, np.float32)
+        g = None if grid is None else np.ascontiguousarray(grid, np.float32)
+        self._check(self._lib.yrh_guide_params_write(self._ctx, None if m is None else m.ctypes.data,
+                                                      None if g is None else g.ctypes.data), "guide_set_params")
+
+    def guide_query(self, rows: np.ndarray) -> np.ndarray:
+        """rows: (n, 9) position, normal, outgoing -> (n, 40) mixtures."""
+        r = np.ascontiguousarray(rows, np.float32).reshape(-1, 9)
+        out = np.zeros((len(r), 40), np.float32)
+        self._check(self._lib.yrh_guide_query(self._ctx, len(r), r.ctypes.data, out.ctypes.data), "guide_query")
+        return out
+
+    def guide_gradients(self, samples: np.ndarray, w_norm: float):
+        """samples: (n, 13) position, normal, outgoing, direction, weight -> (grad_mlp, grad_grid, loss)."""
+        smp = np.ascontiguousarray(samples, np.float32).reshape(-1, 13)
+        m, g, _, _ = self.guide_info()
+        gm, gg, loss = np.zeros(m, np.float32), np.zeros(g, np.float32), C.c_float()
+        self._check(self._lib.yrh_guide_gradients(self._ctx, len(smp), smp.ctypes.data, w_norm, gm.ctypes.data,
+                                                   gg.ctypes.data, C.byref(loss)), "guide_gradients")
+        return gm, gg, loss.value
+
+    def guide_mixture_eval(self, mixture: np.ndarray, dirs: np.ndarray, u: np.ndarray):
+        m = np.ascontiguousarray(mixture, np.float32).reshape(-1, 40)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        uu = np.ascontiguousarray(u, np.float32).reshape(-1, 3)
+        pdf, smp = np.zeros(len(m), np.float32), np.zeros((len(m), 4), np.float32)
+        self._check(self._lib.yrh_guide_mixture_eval(self._ctx, len(m), m.ctypes.data, d.ctypes.data, uu.ctypes.data,
+                                                      pdf.ctypes.data, smp.ctypes.data), "guide_mixture_eval")
+        return pdf, smp
 
     # -- host convenience ---------------------------------------------------
     def trace_closest(self, org: np.ndarray, dirs: np.ndarray, counters: bool = False):

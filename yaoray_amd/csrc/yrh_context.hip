@@ -234,6 +234,7 @@ void yrh_destroy(yrh_context* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     DestroyRenderState(ctx);
+    DestroyGuideState(ctx);
     ctx->geo = DevGeometry{};
     ctx->work_counter.reset();
     ctx->stack_spill.reset();

@@ -112,8 +112,13 @@ struct DevGeometry {
 };
 
 struct RenderState;  // yrh_render.hip
+struct GuideState;   // yrh_guide.hip
 
 }  // namespace yrh
+
+namespace yrg {
+struct GuideView;  // yrh_guide.h
+}
 
 struct yrh_context {
     int device = 0;
@@ -131,6 +136,7 @@ struct yrh_context {
     int blocks_per_cu = 4;
     int block_size = 256;
     yrh::RenderState* render = nullptr;  // surface/light data, film, path state (yrh_render.hip)
+    yrh::GuideState* guide = nullptr;    // learned guiding proposal (yrh_guide.hip)
 };
 
 namespace yrh {
@@ -160,6 +166,12 @@ int LaunchTrace(yrh_context* ctx, const TraceRequest& r);
 // geometry upload; released with the context.
 int UploadSurfaceGeometry(yrh_context* ctx, const yrh_geometry_desc* g);
 void DestroyRenderState(yrh_context* ctx);
+
+// Guiding state for one render pass (yrh_guide.hip): fills `view` (mode off
+// when no proposal is configured), sizing the per-row buffers for `paths`
+// rows and, when recording, `paths * max_depth` vertex records.
+int GuidePrepareView(yrh_context* ctx, int64_t paths, int max_depth, yrg::GuideView* view);
+void DestroyGuideState(yrh_context* ctx);
 
 // Launches of the traversal kernels (yrh_trace.hip).
 int LaunchTraceClosest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, const yrh_hit_soa* hits);

@@ -14,7 +14,9 @@
 // and finally `accumulate` adds each path's sample to the FP64 film.
 // Path state is SoA in HBM; queues are appended with one atomic per wave.
 #include "yrh_internal.h"
+#include "yrh_guide.h"
 #include "yrh_shading.h"
+#include "yrh_traverse.h"
 
 #include <algorithm>
 #include <cmath>
@@ -35,12 +37,9 @@ enum Counter : int {
     kCountQueue0 = 0,
     kCountQueue1 = 1,
     kCountShadow = 2,
-    kCountLoop0 = 3,  // transparent-shadow loop queue counts (two, ping-pong)
-    kCountLoop1 = 4,
-    kWorkExtend = 5,  // per-launch ray counters of the trace kernels
-    kWorkShadow = 6,
-    kWorkLoop = 7,    // kWorkLoop + iteration
-    kCounterWords = kWorkLoop + kMaxTransparentShadowHits + 1,
+    kWorkExtend = 3,  // per-launch ray counters of the trace kernels
+    kWorkShadow = 4,
+    kCounterWords = 5,
 };
 
 // 64-bit statistics words.
@@ -110,7 +109,7 @@ struct RenderState {
     DevBuf q_org[2], q_dir[2], q_path[2];
     DevBuf hits, hit_inst;
     DevBuf s_org, s_dir, s_li, s_bsdf, s_occ, s_trans;
-    DevBuf l_entry[2], l_org[2], l_dir[2], l_hit, l_state, l_med;
+    DevBuf l_state, l_med;
     DevBuf counters;  // uint32[kCounterWords]
     DevBuf stats;     // uint64[kStatWords]
 };
@@ -211,10 +210,6 @@ struct PassView {
     float4* s_bsdf;
     uint32_t* s_occ;
     float4* s_trans;
-    int32_t* l_entry[2];
-    float4* l_org[2];
-    float4* l_dir[2];
-    float4* l_hit;
     int4* l_state;
     float4* l_med;
     uint32_t* counters;
@@ -545,10 +540,6 @@ __global__ void BeginBounceKernel(uint32_t* counters, int next_queue) {
     if (i == 0) counters[next_queue] = 0u;
     if (i == 1) counters[kCountShadow] = 0u;
     if (i >= 2 && i < 2 + (kCounterWords - kWorkExtend)) counters[kWorkExtend + (i - 2)] = 0u;
-    if (i == 63) {
-        counters[kCountLoop0] = 0u;
-        counters[kCountLoop1] = 0u;
-    }
 }
 
 __global__ void CopyTraceTotalsKernel(unsigned long long* stats) {
@@ -558,6 +549,58 @@ __global__ void CopyTraceTotalsKernel(unsigned long long* stats) {
         stats[kStatTraceNodes] = 0;
         stats[kStatTraceTris] = 0;
     }
+}
+
+struct SurfacePoint {
+    float3 hp, wo, sn, obj_geo;
+    int4 pinfo;
+    DevMaterial mat;
+};
+
+// TraceVisibleSurface's hit resolution (src/integrators/surface_query.cpp:101-200):
+// material, hit point, object-space geometric normal and the face-forwarded
+// shading normal.
+__device__ __forceinline__ SurfacePoint ResolveSurface(const SurfaceView& S, const LightView& L, float3 org,
+                                                       float3 dir, float4 h, int tri, int inst) {
+    const float t = h.x;
+    const int prim = S.tri_prim[tri];
+    const int4 pinfo = S.prim_info[prim];
+    const DevMaterial mat = LoadMaterial(L.materials, pinfo.x);
+    const float3 hp = Add(org, Mul(dir, t));
+    const float3 wo = Neg(dir);
+    const int64_t base = static_cast<int64_t>(pinfo.z) + static_cast<int64_t>(tri - pinfo.w) * 3;
+    const uint32_t i0 = S.indices[base], i1 = S.indices[base + 1], i2 = S.indices[base + 2];
+    const float3 p0 = Pos(S, i0), p1 = Pos(S, i1), p2 = Pos(S, i2);
+    const float3 obj_geo = Normalize(Cross(Sub(p1, p0), Sub(p2, p0)));
+    const float4* nrows = inst >= 0 ? S.inst_normal + 6 * static_cast<int64_t>(inst) : nullptr;
+    const float3 raw = inst >= 0 ? TransformNormalDev(nrows, obj_geo) : obj_geo;
+    const float3 geo = Dot(raw, wo) < 0.0f ? Neg(raw) : raw;
+    // ResolveShadingNormal (src/geometry/intersection.cpp:84-106) on the object-space
+    // normal for instances, on the face-forwarded normal otherwise.
+    const float3 ref = inst >= 0 ? obj_geo : geo;
+    float3 sn = Normalize(ref);
+    if (S.prim_flags[prim] & 1) {
+        const float u = h.y, v = h.z, w = 1.0f - u - v;
+        const float3 n0 = XYZ(S.normals[i0]), n1 = XYZ(S.normals[i1]), n2 = XYZ(S.normals[i2]);
+        const float3 ni = Normalize(V(n0.x * w + n1.x * u + n2.x * v, n0.y * w + n1.y * u + n2.y * v,
+                                      n0.z * w + n1.z * u + n2.z * v));
+        if (LengthSq(ni) > 0.0f) sn = ni;
+    }
+    if (Dot(sn, ref) < 0.0f) sn = Neg(sn);
+    if (LengthSq(sn) == 0.0f) sn = Normalize(ref);
+    if (inst >= 0) {
+        sn = TransformNormalDev(nrows, sn);
+        if (Dot(sn, geo) < 0.0f) sn = Neg(sn);
+    }
+    if (Dot(sn, wo) < 0.0f) sn = Neg(sn);
+    SurfacePoint out;
+    out.hp = hp;
+    out.wo = wo;
+    out.sn = sn;
+    out.obj_geo = obj_geo;
+    out.pinfo = pinfo;
+    out.mat = mat;
+    return out;
 }
 
 struct ShadeOut {
@@ -570,8 +613,47 @@ struct ShadeOut {
     float s_scale[2];
 };
 
-__device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& S, const LightView& L, int depth,
-                                          int i, int cur, ShadeOut& o, unsigned& hit_count, unsigned& miss_count) {
+// ---- guided sampling (learned proposal, yrh_guide.h) ----------------------------
+
+__device__ __forceinline__ float Lum(float3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+// Vertices the proposal guides: non-delta reflective BSDFs.
+__device__ __forceinline__ bool Guidable(const DevMaterial& m) {
+    return (m.kind == kDiffuse || m.kind == kConductor) && !IsDelta(m);
+}
+
+__device__ __forceinline__ float GuidePdf(const float* gm, float3 w) { return yrg::MixturePdf(gm, gm + 8, gm + 16, w); }
+
+// One-sample MIS of the proposal (chosen with probability alpha) and the BSDF:
+// the direction comes from the chosen strategy, the pdf is the mixture
+// alpha q + (1 - alpha) p_bsdf, so the estimate stays unbiased.
+__device__ __forceinline__ BsdfSample SampleGuided(const DevMaterial& m, float3 wo, float3 n, const float* gm,
+                                                   float alpha, float u, float2 s) {
+    BsdfSample b = Invalid();
+    float3 wi;
+    if (u < alpha) {
+        wi = Normalize(yrg::MixtureSample(gm, gm + 8, gm + 16, u / alpha, s.x, s.y));
+    } else {
+        const BsdfSample bs = SampleBsdf(m, wo, n, s);
+        if (!bs.valid) return b;
+        wi = bs.wi;
+    }
+    const float c = Dot(wi, n);
+    if (!(c > 0.0f)) return b;
+    const float pdf = alpha * GuidePdf(gm, wi) + (1.0f - alpha) * PdfBsdf(m, wo, wi, n);
+    if (!(pdf > 0.0f)) return b;
+    b.wi = wi;
+    b.weight = Mul(EvalBsdf(m, wo, wi, n), c / pdf);
+    b.pdf = pdf;
+    b.valid = true;
+    b.specular = false;
+    return b;
+}
+
+template <bool kGuided>
+__device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& S, const LightView& L,
+                                          const yrg::GuideView& G, int depth, int i, int cur, ShadeOut& o,
+                                          unsigned& hit_count, unsigned& miss_count) {
     o.next = false;
     o.shadow[0] = o.shadow[1] = false;
     const int p = P.q_path[cur][i];
@@ -610,36 +692,20 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
         P.p_tp[p] = F4(tp, 0.0f);
         return;
     }
-    const int prim = S.tri_prim[tri];
-    const int4 pinfo = S.prim_info[prim];
-    const DevMaterial mat = LoadMaterial(L.materials, pinfo.x);
-    const float3 hp = Add(org, Mul(dir, t));
-    const float3 wo = Neg(dir);
-    const int64_t base = static_cast<int64_t>(pinfo.z) + static_cast<int64_t>(tri - pinfo.w) * 3;
-    const uint32_t i0 = S.indices[base], i1 = S.indices[base + 1], i2 = S.indices[base + 2];
-    const float3 p0 = Pos(S, i0), p1 = Pos(S, i1), p2 = Pos(S, i2);
-    const float3 obj_geo = Normalize(Cross(Sub(p1, p0), Sub(p2, p0)));
-    const float4* nrows = inst >= 0 ? S.inst_normal + 6 * static_cast<int64_t>(inst) : nullptr;
-    const float3 raw = inst >= 0 ? TransformNormalDev(nrows, obj_geo) : obj_geo;
-    const float3 geo = Dot(raw, wo) < 0.0f ? Neg(raw) : raw;
-    // ResolveShadingNormal (src/geometry/intersection.cpp:84-106) on the object-space
-    // normal for instances, on the face-forwarded normal otherwise.
-    const float3 ref = inst >= 0 ? obj_geo : geo;
-    float3 sn = Normalize(ref);
-    if (S.prim_flags[prim] & 1) {
-        const float u = h.y, v = h.z, w = 1.0f - u - v;
-        const float3 n0 = XYZ(S.normals[i0]), n1 = XYZ(S.normals[i1]), n2 = XYZ(S.normals[i2]);
-        const float3 ni = Normalize(V(n0.x * w + n1.x * u + n2.x * v, n0.y * w + n1.y * u + n2.y * v,
-                                      n0.z * w + n1.z * u + n2.z * v));
-        if (LengthSq(ni) > 0.0f) sn = ni;
-    }
-    if (Dot(sn, ref) < 0.0f) sn = Neg(sn);
-    if (LengthSq(sn) == 0.0f) sn = Normalize(ref);
-    if (inst >= 0) {
-        sn = TransformNormalDev(nrows, sn);
-        if (Dot(sn, geo) < 0.0f) sn = Neg(sn);
-    }
-    if (Dot(sn, wo) < 0.0f) sn = Neg(sn);
+    const SurfacePoint sp = ResolveSurface(S, L, org, dir, h, tri, inst);
+    const int4 pinfo = sp.pinfo;
+    const DevMaterial& mat = sp.mat;
+    const float3 hp = sp.hp;
+    const float3 wo = sp.wo;
+    const float3 obj_geo = sp.obj_geo;
+    const float3 sn = sp.sn;
+    // Guided vertex: the proposal's mixture of this queue row (GuideQueryKernel).
+    // Only vertices that may continue are guided: at the last bounce no
+    // continuation ray is traced, so next-event MIS keeps the BSDF pdf there,
+    // and the guided estimator has the plain one's expectation at every depth.
+    const bool guided = kGuided && G.valid[i] != 0;
+    const float* gm = guided ? G.mix + static_cast<int64_t>(yrg::kGuideMixture) * i : nullptr;
+    const float alpha = guided ? G.alpha : 0.0f;
 
     if (depth == 0) {
         flags |= kPrimaryHit;
@@ -681,7 +747,8 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
                                 const float pdf_light = EmissivePdfSolidAngle(L, hp, ls.point, ls.normal, ls.index);
                                 if (pdf_light > 0.0f) {
                                     const float3 f = EvalBsdf(mat, wo, wi, sn);
-                                    const float pdf_bsdf = PdfBsdf(mat, wo, wi, sn);
+                                    float pdf_bsdf = PdfBsdf(mat, wo, wi, sn);
+                                    if (guided) pdf_bsdf = alpha * GuidePdf(gm, wi) + (1.0f - alpha) * pdf_bsdf;
                                     const float mis = PowerHeuristic(1, pdf_light, 1, pdf_bsdf);
                                     o.shadow[0] = true;
                                     o.s_org[0] = so;
@@ -704,7 +771,8 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
                 if (c > 0.0f) {
                     const float3 f = EvalBsdf(mat, wo, es.dir, sn);
                     if (!IsBlack(f)) {
-                        const float pdf_bsdf = PdfBsdf(mat, wo, es.dir, sn);
+                        float pdf_bsdf = PdfBsdf(mat, wo, es.dir, sn);
+                        if (guided) pdf_bsdf = alpha * GuidePdf(gm, es.dir) + (1.0f - alpha) * pdf_bsdf;
                         const float mis = PowerHeuristic(1, es.pdf, 1, pdf_bsdf);
                         o.shadow[1] = true;
                         o.s_org[1] = Add(hp, Mul(sn, SurfaceBias(hp)));
@@ -722,9 +790,14 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
     bool cont = depth + 1 < max_depth;
     BsdfSample bs = Invalid();
     if (cont) {
-        bs = SampleBsdf(mat, wo, sn, smp.Get2D(kDimBsdf));
+        if (guided) {
+            bs = SampleGuided(mat, wo, sn, gm, alpha, smp.Independent1D(kDimGuide), smp.Get2D(kDimBsdf));
+        } else {
+            bs = SampleBsdf(mat, wo, sn, smp.Get2D(kDimBsdf));
+        }
         cont = bs.valid && !IsBlack(bs.weight);
     }
+    const float beta_lum = Lum(tp);  // throughput arriving at this vertex
     if (cont) {
         tp = Hadamard(tp, bs.weight);
         if (depth >= 3) {
@@ -750,6 +823,15 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
         flags = bs.specular ? (flags | kPrevDelta) : (flags & ~kPrevDelta);
         P.p_prev[p] = F4(hp, bs.pdf);
         const float3 bn = Dot(bs.wi, sn) >= 0.0f ? sn : Neg(sn);
+        if (guided && G.rec != nullptr) {
+            // Training record: the radiance this direction brings back is
+            // known once the path ends (GuideCollectKernel).
+            float4* r = G.rec + 4 * (static_cast<int64_t>(depth) * G.rec_paths + p);
+            r[0] = F4(hp, beta_lum);
+            r[1] = F4(sn, Lum(rad));
+            r[2] = F4(wo, __int_as_float(G.stamp));
+            r[3] = F4(bs.wi, 0.0f);
+        }
         o.next = true;
         o.next_org = Add(hp, Mul(bn, SurfaceBias(hp)));
         o.next_dir = bs.wi;
@@ -759,7 +841,9 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
     P.p_flags[p] = flags;
 }
 
-__global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S, LightView L, int depth, int cur) {
+template <bool kGuided>
+__global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S, LightView L, yrg::GuideView G,
+                                                     int depth, int cur) {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const int n = static_cast<int>(P.counters[cur]);
     ShadeOut o;
@@ -767,7 +851,7 @@ __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S,
     o.shadow[0] = o.shadow[1] = false;
     unsigned hit_count = 0, miss_count = 0;
     const bool valid = i < n;
-    if (valid) ShadePath(P, S, L, depth, i, cur, o, hit_count, miss_count);
+    if (valid) ShadePath<kGuided>(P, S, L, G, depth, i, cur, o, hit_count, miss_count);
     const int nxt = 1 - cur;
     const int32_t slot = WaveAppend(P.counters + nxt, o.next);
     if (slot >= 0) {
@@ -798,9 +882,124 @@ __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S,
     if (i == 0) atomicAdd(P.stats + kStatRays, static_cast<unsigned long long>(n));
 }
 
+// Proposal for every queue row of this bounce: each lane resolves its row's
+// surface (position, shading normal, outgoing direction), the wave evaluates
+// the network on its 64 rows with MFMA, and each guidable row gets its mixture.
+__global__ __launch_bounds__(kBlock) void GuideQueryKernel(PassView P, SurfaceView S, LightView L, yrg::GuideView G,
+                                                          int cur) {
+    __shared__ float lds_in[4][64][10];
+    __shared__ float lds_o[4][64 * yrg::kGuideOut];
+    const int n = static_cast<int>(P.counters[cur]);
+    if (static_cast<int64_t>(blockIdx.x) * blockDim.x >= n) return;  // whole block idle
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const int wave = static_cast<int>(threadIdx.x >> 6);
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    bool guidable = false;
+    float3 hp = V(0.0f, 0.0f, 0.0f), sn = hp, wo = hp;
+    if (i < n) {
+        const float4 h = P.hits[i];
+        const int tri = __float_as_int(h.w);
+        if (tri >= 0) {
+            const int inst = P.hit_inst != nullptr ? P.hit_inst[i] : -1;
+            const SurfacePoint sp = ResolveSurface(S, L, XYZ(P.q_org[cur][i]), XYZ(P.q_dir[cur][i]), h, tri, inst);
+            guidable = Guidable(sp.mat);
+            hp = sp.hp;
+            sn = sp.sn;
+            wo = sp.wo;
+        }
+    }
+    float* in = lds_in[wave][lane];
+    in[0] = hp.x; in[1] = hp.y; in[2] = hp.z;
+    in[3] = sn.x; in[4] = sn.y; in[5] = sn.z;
+    in[6] = wo.x; in[7] = wo.y; in[8] = wo.z;
+    in[9] = guidable ? 1.0f : 0.0f;
+    __syncthreads();
+    const int q = lane / 16;
+    yrg::bf16x4_t x[4][2];
+    for (int rb = 0; rb < 4; ++rb) {
+        const float* r = lds_in[wave][rb * 16 + lane % 16];
+        float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (r[9] != 0.0f) {
+            const float pp[3] = {r[0], r[1], r[2]};
+            const float nn[3] = {r[3], r[4], r[5]};
+            const float ww[3] = {r[6], r[7], r[8]};
+            yrg::EncodeShare(G.grid, G.table, pp, q, f);
+            x[rb][1] = yrg::DirectionShare(q, nn, ww);
+        } else {
+            x[rb][1] = yrg::PackBf16(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        x[rb][0] = yrg::PackBf16(f[0], f[1], f[2], f[3]);
+    }
+    yrg::bf16x4_t h0[4][4], h1[4][4], h2[4][4];
+    yrg::f32x4_t o[4][3];
+    yrg::MlpForward<4>(G.frag, lane, x, h0, h1, h2, o);
+    float* lo = lds_o[wave];
+    for (int rb = 0; rb < 4; ++rb) {
+        for (int mb = 0; mb < 3; ++mb) {
+            for (int j = 0; j < 4; ++j) lo[(rb * 16 + lane % 16) * yrg::kGuideOut + mb * 16 + 4 * q + j] = o[rb][mb][j];
+        }
+    }
+    __syncthreads();
+    if (i < n) {
+        G.valid[i] = guidable ? 1 : 0;
+        if (guidable) {
+            yrg::Mixture m;
+            yrg::HeadFromOutputs(lo + lane * yrg::kGuideOut, m);
+            float* dst = G.mix + static_cast<int64_t>(yrg::kGuideMixture) * i;
+            for (int k = 0; k < yrg::kGuideLobes; ++k) {
+                dst[k] = m.pi[k];
+                dst[8 + k] = m.kappa[k];
+                dst[16 + 3 * k] = m.mu[k][0];
+                dst[16 + 3 * k + 1] = m.mu[k][1];
+                dst[16 + 3 * k + 2] = m.mu[k][2];
+            }
+        }
+    }
+}
+
+// After the last bounce: each record of this pass becomes a training sample
+// (position, normal, outgoing, sampled direction, weight) with weight =
+// luminance of the radiance the path gathered after the vertex over the
+// luminance of the throughput arriving at it, i.e. f cos L / p of the sampled
+// direction. Zero-weight samples carry no gradient and are skipped.
+__global__ __launch_bounds__(kBlock) void GuideCollectKernel(PassView P, yrg::GuideView G) {
+    const int64_t slot = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    bool keep = false;
+    float w = 0.0f;
+    float4 r0 = make_float4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
+    if (slot < G.rec_paths * G.rec_depths) {
+        const float4* r = G.rec + 4 * slot;
+        r2 = r[2];
+        if (__float_as_int(r2.w) == G.stamp) {
+            r0 = r[0];
+            r1 = r[1];
+            r3 = r[3];
+            const float final_lum = Lum(XYZ(P.p_rad[slot % G.rec_paths]));
+            w = (final_lum - r1.w) / r0.w;
+            keep = isfinite(w) && w > 0.0f;
+        }
+    }
+    const uint64_t mask = __ballot(keep);
+    if (mask == 0) return;
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(G.sample_count, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader);
+    if (!keep) return;
+    const uint32_t idx = base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+    if (idx >= static_cast<uint32_t>(G.sample_capacity)) return;
+    float4* d = G.samples + 4 * static_cast<int64_t>(idx);
+    d[0] = F4(XYZ(r0), w);
+    d[1] = F4(XYZ(r1), 0.0f);
+    d[2] = F4(XYZ(r2), 0.0f);
+    d[3] = F4(XYZ(r3), 0.0f);
+    atomicAdd(G.sample_wsum, w);
+}
+
 // Adds the visible next-event contributions of this bounce, emissive then
 // environment, as EstimateDirectLight sums them.
-__global__ __launch_bounds__(kBlock) void ResolveKernel(PassView P, int cur) {
+__global__ __launch_bounds__(kBlock) void ResolveKernel(PassView P, yrg::GuideView G, int depth, int cur) {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const int n = static_cast<int>(P.counters[cur]);
     unsigned shadow = 0, occluded = 0;
@@ -822,7 +1021,14 @@ __global__ __launch_bounds__(kBlock) void ResolveKernel(PassView P, int cur) {
                 direct = k == 0 ? c : Add(direct, c);
             }
             const float3 tp = XYZ(P.p_nee[p]);
-            P.p_rad[p] = F4(Add(XYZ(P.p_rad[p]), Hadamard(tp, direct)), 0.0f);
+            const float3 rad = Add(XYZ(P.p_rad[p]), Hadamard(tp, direct));
+            P.p_rad[p] = F4(rad, 0.0f);
+            if (G.rec != nullptr) {
+                // This vertex's own next-event estimate is not radiance along
+                // its sampled direction: move the record's snapshot past it.
+                float4* r = G.rec + 4 * (static_cast<int64_t>(depth) * G.rec_paths + p);
+                if (__float_as_int(r[2].w) == G.stamp) r[1].w = Lum(rad);
+            }
         }
     }
     WaveStat(P.stats + kStatShadow, shadow);
@@ -830,46 +1036,40 @@ __global__ __launch_bounds__(kBlock) void ResolveKernel(PassView P, int cur) {
 }
 
 // ---- transparent shadows (TraceShadowVisibility, path_direct_lighting.cpp:23-113) ----
-// l_state[e] = (status 0 active / 1 done, iteration, finite, 0); l_med[e] = medium
-// (color, distance; distance < 0 = inactive); s_trans holds the transmittance and
-// the remaining distance in .w.
-
-__global__ void ShadowLoopInitKernel(PassView P) {
-    const int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    const int n = static_cast<int>(P.counters[kCountShadow]);
-    if (e == 0) P.counters[kCountLoop0] = static_cast<uint32_t>(n);
-    if (e >= n) return;
-    const float tmax = P.s_dir[e].w;
-    P.l_state[e] = make_int4(0, 0, isfinite(tmax) ? 1 : 0, 0);
-    P.l_med[e] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
-    P.s_trans[e] = make_float4(1.0f, 1.0f, 1.0f, tmax);
-    P.s_occ[e] = 1u;
-    P.l_entry[0][e] = e;
-    P.l_org[0][e] = P.s_org[e];
-    P.l_dir[0][e] = P.s_dir[e];
-}
+// One persistent launch walks every shadow ray through its dielectric surfaces:
+// the traversal hands each closest hit to ShadowWalk::Finish, which applies the
+// reference's per-hit rule and returns the next segment (same lane) until the
+// ray is decided. l_state[e] = (0, transparent hits so far, finite, 0);
+// l_med[e] = medium (color, distance; distance < 0 = inactive); s_trans holds
+// the transmittance and the remaining distance in .w.
 
 __device__ __forceinline__ bool TransBlack(float3 c) { return MaxC(c) <= 1.0e-6f; }
 
-__global__ __launch_bounds__(kBlock) void ShadowLoopStepKernel(PassView P, SurfaceView S, LightView L, int src) {
-    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    const int n = static_cast<int>(P.counters[kCountLoop0 + src]);
-    bool again = false;
-    int e = -1;
-    float4 next_org = make_float4(0, 0, 0, 0), next_dir = next_org;
-    if (i < n) {
-        e = P.l_entry[src][i];
-        const float4 h = P.l_hit[i];
-        const float4 lo = P.l_org[src][i], ld = P.l_dir[src][i];
-        int4 st = P.l_state[e];
-        float4 tr = P.s_trans[e];
+struct ShadowWalk {
+    const PassView* P;
+    const SurfaceView* S;
+    const LightView* L;
+
+    __device__ __forceinline__ void Begin(int32_t e, float4& o, float4& d) const {
+        o = P->s_org[e];
+        d = P->s_dir[e];
+        P->l_state[e] = make_int4(0, 0, isfinite(d.w) ? 1 : 0, 0);
+        P->l_med[e] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
+        P->s_trans[e] = make_float4(1.0f, 1.0f, 1.0f, d.w);
+    }
+
+    __device__ __forceinline__ bool Finish(int32_t e, const trav::Best& best, int32_t, uint32_t, uint32_t,
+                                           const float4& lo, const float4& ld, float4& next_org,
+                                           float4& next_dir) const {
+        int4 st = P->l_state[e];
+        const float4 tr = P->s_trans[e];
         float3 T = XYZ(tr);
         float remaining = tr.w;
-        float4 med = P.l_med[e];
+        float4 med = P->l_med[e];
         const bool finite = st.z != 0;
-        const int tri = __float_as_int(h.w);
         uint32_t occ = 1u;
-        if (tri < 0) {
+        bool again = false;
+        if (!best.hit) {
             bool ok = true;
             if (med.w >= 0.0f) {
                 if (!finite) {
@@ -882,11 +1082,11 @@ __global__ __launch_bounds__(kBlock) void ShadowLoopStepKernel(PassView P, Surfa
         } else {
             bool ok = true;
             if (med.w >= 0.0f) {
-                T = Hadamard(T, BeerLambert(XYZ(med), med.w, h.x));
+                T = Hadamard(T, BeerLambert(XYZ(med), med.w, best.t));
                 if (TransBlack(T)) ok = false;
             }
-            const int prim = S.tri_prim[tri];
-            const DevMaterial mat = L.materials[S.prim_info[prim].x];
+            const int prim = S->tri_prim[best.id];
+            const DevMaterial mat = L->materials[S->prim_info[prim].x];
             if (ok && mat.kind != kDielectric && mat.kind != kThinDielectric) ok = false;
             if (ok) {
                 if (mat.kind == kThinDielectric) {
@@ -899,11 +1099,11 @@ __global__ __launch_bounds__(kBlock) void ShadowLoopStepKernel(PassView P, Surfa
                 }
             }
             if (ok) {
-                const float3 hp = Add(XYZ(lo), Mul(XYZ(ld), h.x));
+                const float3 hp = Add(XYZ(lo), Mul(XYZ(ld), best.t));
                 const float bias = SurfaceBias(hp);
                 bool done = false;
                 if (finite) {
-                    remaining -= h.x + bias;
+                    remaining -= best.t + bias;
                     if (remaining <= 0.0f) {
                         occ = !TransBlack(T) ? 0u : 1u;
                         done = true;
@@ -921,22 +1121,24 @@ __global__ __launch_bounds__(kBlock) void ShadowLoopStepKernel(PassView P, Surfa
                 }
             }
         }
-        P.s_occ[e] = occ;
-        P.s_trans[e] = F4(T, remaining);
-        P.l_med[e] = med;
-        P.l_state[e] = st;
+        P->s_occ[e] = occ;
+        P->s_trans[e] = F4(T, remaining);
+        P->l_med[e] = med;
+        P->l_state[e] = st;
+        return again;
     }
-    const int dst = 1 - src;
-    const int32_t slot = WaveAppend(P.counters + kCountLoop0 + dst, again);
-    if (slot >= 0) {
-        P.l_entry[dst][slot] = e;
-        P.l_org[dst][slot] = next_org;
-        P.l_dir[dst][slot] = next_dir;
-    }
-}
+};
 
-__global__ void ResetLoopCountKernel(uint32_t* counters, int which) {
-    if (threadIdx.x == 0) counters[kCountLoop0 + which] = 0u;
+template <bool kTwoLevel>
+__global__ __launch_bounds__(trav::kMaxBlock) void ShadowWalkKernel(trav::TraceArgs a, PassView P, SurfaceView S,
+                                                                    LightView L) {
+    __shared__ int2 lds_stack[trav::kLdsStack * trav::kMaxBlock];
+    const ShadowWalk pol{&P, &S, &L};
+    if (kTwoLevel) {
+        trav::TraceTwoLevelLoop<false>(a, pol, lds_stack);
+    } else {
+        trav::TraceFlatLoop<false>(a, pol, lds_stack);
+    }
 }
 
 // Adds the pass's samples to the FP64 film (Film::AddSample, src/film/film.cpp:32-56):
@@ -1032,9 +1234,6 @@ PassView MakePassView(RenderState& R, const yrh_render_desc& d) {
         P.q_org[k] = R.q_org[k].as<float4>();
         P.q_dir[k] = R.q_dir[k].as<float4>();
         P.q_path[k] = R.q_path[k].as<int32_t>();
-        P.l_entry[k] = R.l_entry[k].as<int32_t>();
-        P.l_org[k] = R.l_org[k].as<float4>();
-        P.l_dir[k] = R.l_dir[k].as<float4>();
     }
     P.hits = R.hits.as<float4>();
     P.hit_inst = R.hit_inst.as<int32_t>();
@@ -1044,7 +1243,6 @@ PassView MakePassView(RenderState& R, const yrh_render_desc& d) {
     P.s_bsdf = R.s_bsdf.as<float4>();
     P.s_occ = R.s_occ.as<uint32_t>();
     P.s_trans = R.s_trans.as<float4>();
-    P.l_hit = R.l_hit.as<float4>();
     P.l_state = R.l_state.as<int4>();
     P.l_med = R.l_med.as<float4>();
     P.counters = R.counters.as<uint32_t>();
@@ -1110,8 +1308,7 @@ int EnsureCapacity(yrh_context* ctx, RenderState& R, int64_t paths) {
         {&R.q_org[0], 16 * P}, {&R.q_org[1], 16 * P}, {&R.q_dir[0], 16 * P}, {&R.q_dir[1], 16 * P},
         {&R.q_path[0], 4 * P}, {&R.q_path[1], 4 * P}, {&R.hits, 16 * P}, {&R.hit_inst, 4 * P},
         {&R.s_org, 16 * S}, {&R.s_dir, 16 * S}, {&R.s_li, 16 * S}, {&R.s_bsdf, 16 * S}, {&R.s_occ, 4 * S},
-        {&R.s_trans, 16 * S}, {&R.l_entry[0], 4 * S}, {&R.l_entry[1], 4 * S}, {&R.l_org[0], 16 * S},
-        {&R.l_org[1], 16 * S}, {&R.l_dir[0], 16 * S}, {&R.l_dir[1], 16 * S}, {&R.l_hit, 16 * S}, {&R.l_state, 16 * S}, {&R.l_med, 16 * S},
+        {&R.s_trans, 16 * S}, {&R.l_state, 16 * S}, {&R.l_med, 16 * S},
         {&R.counters, 4 * kCounterWords},
     };
     for (const Item& it : items) {
@@ -1435,6 +1632,9 @@ int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* d) {
     const LightView L = MakeLightView(R);
     hipStream_t st = ctx->stream;
     const int max_depth = std::max(1, d->max_depth);
+    yrg::GuideView GV;
+    if ((rc = GuidePrepareView(ctx, paths, max_depth, &GV)) != YRH_OK) return rc;
+    const bool guided = GV.mode != YRH_GUIDE_OFF;
     hipLaunchKernelGGL(RaygenKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P);
     for (int depth = 0; depth < max_depth; ++depth) {
         const int cur = depth & 1;
@@ -1452,7 +1652,12 @@ int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* d) {
         if ((rc = LaunchTrace(ctx, tr)) != YRH_OK) return rc;
         PassView Pk = P;
         if (ctx->geo.accel_kind != YRH_ACCEL_TWO_LEVEL) Pk.hit_inst = nullptr;
-        hipLaunchKernelGGL(ShadeKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, depth, cur);
+        if (guided && depth + 1 < max_depth) {
+            hipLaunchKernelGGL(GuideQueryKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, cur);
+            hipLaunchKernelGGL(ShadeKernel<true>, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
+        } else {
+            hipLaunchKernelGGL(ShadeKernel<false>, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
+        }
         if (!R.transparent) {
             TraceRequest sr;
             sr.any = true;
@@ -1465,24 +1670,28 @@ int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* d) {
             sr.totals = P.stats + kStatTraceNodes;
             if ((rc = LaunchTrace(ctx, sr)) != YRH_OK) return rc;
         } else {
-            hipLaunchKernelGGL(ShadowLoopInitKernel, dim3(Grid(2 * paths)), dim3(kBlock), 0, st, P);
-            for (int it = 0; it < kMaxTransparentShadowHits; ++it) {
-                const int src = it & 1;
-                TraceRequest lr;
-                lr.any = false;
-                lr.org = P.l_org[src];
-                lr.dir = P.l_dir[src];
-                lr.n_device = P.counters + kCountLoop0 + src;
-                lr.capacity = 2 * paths;
-                lr.work_counter = P.counters + kWorkLoop + it;
-                lr.hit = P.l_hit;
-                lr.totals = P.stats + kStatTraceNodes;
-                if ((rc = LaunchTrace(ctx, lr)) != YRH_OK) return rc;
-                hipLaunchKernelGGL(ResetLoopCountKernel, dim3(1), dim3(64), 0, st, P.counters, 1 - src);
-                hipLaunchKernelGGL(ShadowLoopStepKernel, dim3(Grid(2 * paths)), dim3(kBlock), 0, st, P, S, L, src);
+            TraceRequest wr;
+            wr.any = false;
+            wr.org = P.s_org;
+            wr.dir = P.s_dir;
+            wr.n_device = P.counters + kCountShadow;
+            wr.capacity = 2 * paths;
+            wr.work_counter = P.counters + kWorkShadow;
+            wr.hit = P.s_trans;  // not written by the walk; LaunchTrace's checks want an output
+            wr.totals = P.stats + kStatTraceNodes;
+            trav::TraceArgs wa;
+            int wgrid = 0, wblock = 0;
+            if ((rc = PrepareTraceArgs(ctx, wr, wa, wgrid, wblock)) != YRH_OK) return rc;
+            if (ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL) {
+                hipLaunchKernelGGL(ShadowWalkKernel<true>, dim3(wgrid), dim3(wblock), 0, st, wa, P, S, L);
+            } else {
+                hipLaunchKernelGGL(ShadowWalkKernel<false>, dim3(wgrid), dim3(wblock), 0, st, wa, P, S, L);
             }
         }
-        hipLaunchKernelGGL(ResolveKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P, cur);
+        hipLaunchKernelGGL(ResolveKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P, GV, depth, cur);
+    }
+    if (GV.rec != nullptr) {
+        hipLaunchKernelGGL(GuideCollectKernel, dim3(Grid(paths * max_depth)), dim3(kBlock), 0, st, P, GV);
     }
     hipLaunchKernelGGL(AccumulateKernel, dim3(Grid(npix)), dim3(kBlock), 0, st, P);
     hipLaunchKernelGGL(CopyTraceTotalsKernel, dim3(1), dim3(64), 0, st, P.stats);

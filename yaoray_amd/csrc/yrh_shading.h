@@ -53,7 +53,7 @@ enum Dim : uint32_t {
     kDimLightSurface = 3,
     kDimBsdf = 5,
     kDimRoulette = 7,
-    kDimGuide = 15,  // lobe choice of the guided proposal (a dimension the reference leaves unused)
+    kDimGuide = 15,  // guide-or-BSDF choice (and lobe) of the guided proposal; the reference leaves it unused
 };
 
 __device__ __forceinline__ uint64_t Mix64(uint64_t v) {
@@ -180,6 +180,15 @@ struct PixelSampler {
     }
 
     __device__ __forceinline__ float Get1D(uint32_t dim) const { return At1D(Absolute(dim), 0); }
+    // A hashed uniform of (pixel, sample, dimension) outside the
+    // low-discrepancy sequence. Every 1D dimension of the Sobol samplers is a
+    // differently scrambled copy of Sobol dimension 0, so a choice drawn from
+    // one is correlated with the first coordinate of any 2D sample of the same
+    // bounce; a strategy choice that selects how that 2D sample is used must
+    // come from here instead.
+    __device__ __forceinline__ float Independent1D(uint32_t dim) const {
+        return UnitFloat(Hash32(pixel_seed ^ static_cast<uint64_t>(static_cast<int64_t>(sample_index)), Absolute(dim), 0));
+    }
     __device__ __forceinline__ float2 Get2D(uint32_t dim) const { return At2D(Absolute(dim), 0); }
 };
 

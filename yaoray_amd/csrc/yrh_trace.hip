@@ -1,491 +1,33 @@
-// Closest-hit and any-hit BVH traversal kernels for MI355X (gfx950).
-//
-// Arithmetic follows the reference operation by operation, with FP
-// contraction disabled (the reference x86-64 build has no FMA):
-//   slab test      include/yaoray/core/bounds.hpp:50-90
-//   triangle test  src/accel/bvh_traversal.cpp:52-121 (Moller-Trumbore, two-sided,
-//                  |det| < 1e-8 reject, equal-t tie -> lower flat triangle index)
-//   traversal      src/accel/bvh_traversal.cpp:253-345 (root box, pop-skip when
-//                  entry_t > best t, both child boxes, push far then near, ties
-//                  near = left)
-//   two-level      src/accel/two_level_bvh.cpp:365-481 (object ray through the
-//                  host inverse, unnormalised direction so t is shared, BLAS run
-//                  with t_max = best t, accepted only if strictly closer)
-// so hit ids, t, barycentrics and the node/triangle test counters equal the
-// reference's on the same tree.
-//
-// Execution model: a persistent grid sized to the device (blocks_per_cu per
-// CU). Each wave64 refills idle lanes from a global ray counter with one
-// atomic per wave (ballot + mbcnt), so divergent ray lengths do not idle the
-// wave. The traversal stack lives in LDS (kLdsStack entries per lane, laid out
-// [level][lane] so a wave's pushes hit consecutive banks) and spills deeper
-// levels to a per-lane global region; its capacity is the proven bound
-// max_depth + 1 (one pending sibling per level plus the top pair).
-#include "yrh_internal.h"
-
-#pragma clang fp contract(off)
+// Closest-hit and any-hit ray-cast kernels for MI355X (gfx950) over a ray
+// queue: the traversal loops of yrh_traverse.h with the ray-array policy.
+#include "yrh_traverse.h"
 
 namespace yrh {
 namespace {
 
-constexpr int kLdsStack = 16;
-constexpr int kMaxBlock = 256;
-constexpr int kRefillIdleLanes = 16;
-
-struct TraceArgs {
-    const float4* org;
-    const float4* dir;
-    int32_t n;
-    const uint32_t* n_device;  // when set, the ray count is read from device memory
-    unsigned long long* totals;  // optional [0] node tests, [1] triangle tests of the launch
-    // outputs
-    float4* hit;
-    int32_t* inst_out;
-    uint32_t* node_tests;
-    uint32_t* triangle_tests;
-    uint32_t* occluded;
-    // flat BVH or concatenated BLASes
-    const DevNode* nodes;
-    const float4* slots;
-    const int32_t* leaf_count;
-    int32_t root_code;
-    float root_lo_x, root_lo_y, root_lo_z, root_hi_x, root_hi_y, root_hi_z;
-    int32_t empty;
-    // two-level
-    const DevNode* tnodes;
-    const int32_t* tslot_instance;
-    const int32_t* tleaf_count;
-    const DevInstance* instances;
-    const DevBlasRoot* blas_roots;
-    // scheduling
-    uint32_t* counter;       // this launch's ray counter (zero at launch)
-    uint32_t* next_counter;  // the next launch's counter, zeroed here (may be null)
-    uint32_t* overflow;      // stack overflow flag
-    int2* spill;
-    int64_t spill_stride;
-    int32_t capacity;
-};
-
-struct RayPre {
-    float ox, oy, oz;
-    float ix, iy, iz;
-    bool nx, ny, nz;
-};
-
-__device__ __forceinline__ RayPre Precompute(float ox, float oy, float oz, float dx, float dy, float dz) {
-    RayPre r;
-    r.ox = ox;
-    r.oy = oy;
-    r.oz = oz;
-    r.ix = 1.0f / dx;
-    r.iy = 1.0f / dy;
-    r.iz = 1.0f / dz;
-    r.nx = signbit(r.ix);
-    r.ny = signbit(r.iy);
-    r.nz = signbit(r.iz);
-    return r;
-}
-
-// IntersectBounds: comparison form keeps the interval on a 0*inf NaN.
-__device__ __forceinline__ bool Slab(const RayPre& r, float lx, float ly, float lz, float hx, float hy, float hz,
-                                     float t_min, float t_max, float& entry) {
-    float nb = r.nx ? hx : lx;
-    float fb = r.nx ? lx : hx;
-    float an = (nb - r.ox) * r.ix;
-    float af = (fb - r.ox) * r.ix;
-    t_min = an > t_min ? an : t_min;
-    t_max = af < t_max ? af : t_max;
-    nb = r.ny ? hy : ly;
-    fb = r.ny ? ly : hy;
-    an = (nb - r.oy) * r.iy;
-    af = (fb - r.oy) * r.iy;
-    t_min = an > t_min ? an : t_min;
-    t_max = af < t_max ? af : t_max;
-    nb = r.nz ? hz : lz;
-    fb = r.nz ? lz : hz;
-    an = (nb - r.oz) * r.iz;
-    af = (fb - r.oz) * r.iz;
-    t_min = an > t_min ? an : t_min;
-    t_max = af < t_max ? af : t_max;
-    entry = t_min;
-    return t_max >= t_min;
-}
-
-struct Best {
-    float t;
-    float u;
-    float v;
-    int32_t id;
-    bool hit;
-};
-
-// IntersectTriangle on a (p0, e1, e2) slot. Returns true when `best` changed.
-__device__ __forceinline__ bool TriangleTest(const float4& s0, const float4& s1, const float4& s2,
-                                             float ox, float oy, float oz, float dx, float dy, float dz,
-                                             float t_min, bool any_hit, Best& best) {
-    const float e1x = s1.x, e1y = s1.y, e1z = s1.z;
-    const float e2x = s2.x, e2y = s2.y, e2z = s2.z;
-    // pvec = Cross(d, e2)
-    const float px = dy * e2z - dz * e2y;
-    const float py = dz * e2x - dx * e2z;
-    const float pz = dx * e2y - dy * e2x;
-    const float det = e1x * px + e1y * py + e1z * pz;
-    if (fabsf(det) < 1.0e-8f) return false;
-    const float inv_det = 1.0f / det;
-    const float tx = ox - s0.x;
-    const float ty = oy - s0.y;
-    const float tz = oz - s0.z;
-    const float u = (tx * px + ty * py + tz * pz) * inv_det;
-    if (u < 0.0f || u > 1.0f) return false;
-    // qvec = Cross(tvec, e1)
-    const float qx = ty * e1z - tz * e1y;
-    const float qy = tz * e1x - tx * e1z;
-    const float qz = tx * e1y - ty * e1x;
-    const float v = (dx * qx + dy * qy + dz * qz) * inv_det;
-    if (v < 0.0f || u + v > 1.0f) return false;
-    const float t = (e2x * qx + e2y * qy + e2z * qz) * inv_det;
-    if (t <= t_min || t > best.t) return false;
-    const int32_t id = __float_as_int(s0.w);
-    if (!any_hit && t == best.t && best.hit && best.id >= 0 && id >= best.id) return false;
-    best.hit = true;
-    best.t = t;
-    best.id = id;
-    best.u = u;
-    best.v = v;
-    return true;
-}
-
-struct Stack {
-    int2* lds;        // this lane's column: lds[level * stride]
-    int lds_stride;
-    int2* spill;      // this lane's column in the global spill region
-    int64_t spill_stride;
-    int capacity;
-    int sp;
-    bool overflow;
-
-    __device__ __forceinline__ void Push(int32_t code, float entry) {
-        if (sp >= capacity) {
-            overflow = true;
-            return;
-        }
-        const int2 e = make_int2(code, __float_as_int(entry));
-        if (sp < kLdsStack) {
-            lds[sp * lds_stride] = e;
-        } else {
-            spill[static_cast<int64_t>(sp - kLdsStack) * spill_stride] = e;
-        }
-        ++sp;
-    }
-    __device__ __forceinline__ int2 Pop() {
-        --sp;
-        if (sp < kLdsStack) return lds[sp * lds_stride];
-        return spill[static_cast<int64_t>(sp - kLdsStack) * spill_stride];
-    }
-};
-
-__device__ __forceinline__ void DecodeLeaf(int32_t code, const int32_t* leaf_count, int32_t& first, int32_t& count) {
-    first = code & kMaxSlot;
-    const int32_t nib = (code >> 27) & 0xF;
-    count = nib < kLeafCountEscape ? nib + 1 : leaf_count[first];
-}
-
-// One interior step: test both child boxes and push them as the reference does.
-__device__ __forceinline__ void InteriorStep(const DevNode* __restrict__ nodes, int32_t code, const RayPre& r,
-                                             float t_min, float t_max, uint32_t& nt, Stack& st) {
-    const DevNode nd = nodes[code];
-    nt += 2;
-    float el, er;
-    const bool hl = Slab(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, t_min, t_max, el);
-    const bool hr = Slab(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, t_min, t_max, er);
-    if (hl && hr) {
-        if (el <= er) {
-            st.Push(nd.d.y, er);
-            st.Push(nd.d.x, el);
-        } else {
-            st.Push(nd.d.x, el);
-            st.Push(nd.d.y, er);
-        }
-    } else if (hl) {
-        st.Push(nd.d.x, el);
-    } else if (hr) {
-        st.Push(nd.d.y, er);
-    }
-}
-
-__device__ __forceinline__ uint64_t LanesBelow(int lane) { return (1ull << lane) - 1ull; }
-
-// Wave-level sum of the per-lane test counts, one atomic per wave.
-__device__ __forceinline__ void AddTotals(unsigned long long* totals, uint64_t nt, uint64_t tt, int lane) {
-    for (int off = 32; off > 0; off >>= 1) {
-        nt += __shfl_xor(static_cast<unsigned long long>(nt), off);
-        tt += __shfl_xor(static_cast<unsigned long long>(tt), off);
-    }
-    if (lane == 0) {
-        atomicAdd(totals, static_cast<unsigned long long>(nt));
-        atomicAdd(totals + 1, static_cast<unsigned long long>(tt));
-    }
-}
-
-// Claims one ray index per wanting lane with a single atomic per wave.
-__device__ __forceinline__ int32_t ClaimRay(uint32_t* counter, bool want, int lane, int32_t n, bool& exhausted) {
-    const uint64_t mask = __ballot(want);
-    if (mask == 0) return -1;
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
-    base = __shfl(base, leader);
-    if (!want) return -1;
-    const uint32_t mine = base + static_cast<uint32_t>(__popcll(mask & LanesBelow(lane)));
-    if (mine >= static_cast<uint32_t>(n)) {
-        exhausted = true;
-        return -1;
-    }
-    return static_cast<int32_t>(mine);
-}
+using trav::kLdsStack;
+using trav::kMaxBlock;
+using trav::TraceArgs;
 
 template <bool kAny>
 __global__ __launch_bounds__(kMaxBlock) void TraceFlatKernel(TraceArgs a) {
     __shared__ int2 lds_stack[kLdsStack * kMaxBlock];
-    const int lane = static_cast<int>(threadIdx.x & 63u);
-    const int64_t gtid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    Stack st{lds_stack + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
-             a.capacity, 0, false};
-
-    if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.next_counter = 0u;
-    const int32_t n = a.n_device != nullptr ? static_cast<int32_t>(*a.n_device) : a.n;
-    uint64_t sum_nt = 0, sum_tt = 0;
-    int32_t ray = -1;
-    bool exhausted = false;
-    float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, t_min = 0;
-    RayPre pre{};
-    Best best{0, 0, 0, -1, false};
-    uint32_t nt = 0, tt = 0;
-
-    while (true) {
-        if (st.sp == 0 && ray >= 0) {
-            if (kAny) {
-                a.occluded[ray] = best.hit ? 1u : 0u;
-            } else {
-                a.hit[ray] = make_float4(best.t, best.u, best.v, __int_as_float(best.hit ? best.id : -1));
-                if (a.inst_out != nullptr) a.inst_out[ray] = -1;
-            }
-            if (a.node_tests != nullptr) a.node_tests[ray] = nt;
-            if (a.triangle_tests != nullptr) a.triangle_tests[ray] = tt;
-            sum_nt += nt;
-            sum_tt += tt;
-            ray = -1;
-        }
-        const bool want = st.sp == 0 && !exhausted;
-        const int32_t claimed = ClaimRay(a.counter, want, lane, n, exhausted);
-        if (claimed >= 0) {
-            ray = claimed;
-            const float4 o4 = a.org[ray];
-            const float4 d4 = a.dir[ray];
-            ox = o4.x; oy = o4.y; oz = o4.z; t_min = o4.w;
-            dx = d4.x; dy = d4.y; dz = d4.z;
-            best = Best{d4.w, 0.0f, 0.0f, -1, false};
-            nt = 0;
-            tt = 0;
-            pre = Precompute(ox, oy, oz, dx, dy, dz);
-            if (!a.empty) {
-                ++nt;
-                float entry;
-                if (Slab(pre, a.root_lo_x, a.root_lo_y, a.root_lo_z, a.root_hi_x, a.root_hi_y, a.root_hi_z,
-                         t_min, best.t, entry)) {
-                    st.Push(a.root_code, entry);
-                }
-            }
-        }
-        if (__ballot(st.sp > 0 || ray >= 0 || !exhausted) == 0) break;
-
-        while (true) {
-            if (st.sp > 0) {
-                const int2 e = st.Pop();
-                const int32_t code = e.x;
-                if (!(__int_as_float(e.y) > best.t)) {
-                    if (code < 0) {
-                        int32_t first, count;
-                        DecodeLeaf(code, a.leaf_count, first, count);
-                        for (int32_t k = 0; k < count; ++k) {
-                            const float4* s = a.slots + 3 * static_cast<int64_t>(first + k);
-                            ++tt;
-                            if (TriangleTest(s[0], s[1], s[2], ox, oy, oz, dx, dy, dz, t_min, kAny, best) && kAny) {
-                                st.sp = 0;
-                                break;
-                            }
-                        }
-                    } else {
-                        InteriorStep(a.nodes, code, pre, t_min, best.t, nt, st);
-                    }
-                }
-            }
-            const uint64_t active = __ballot(st.sp > 0);
-            if (active == 0) break;
-            const uint64_t idle = __ballot(st.sp == 0 && (ray >= 0 || !exhausted));
-            if (__popcll(idle) >= kRefillIdleLanes) break;
-        }
-    }
-    if (st.overflow) atomicOr(a.overflow, 1u);
-    if (a.totals != nullptr) AddTotals(a.totals, sum_nt, sum_tt, lane);
-}
-
-__device__ __forceinline__ void TransformRay(const DevInstance& m, float ox, float oy, float oz, float dx, float dy,
-                                             float dz, float& rox, float& roy, float& roz, float& rdx, float& rdy,
-                                             float& rdz) {
-    // TransformPoint / TransformVector (src/core/transform.cpp:30-44), left to right.
-    rox = m.r0.x * ox + m.r0.y * oy + m.r0.z * oz + m.r0.w;
-    roy = m.r1.x * ox + m.r1.y * oy + m.r1.z * oz + m.r1.w;
-    roz = m.r2.x * ox + m.r2.y * oy + m.r2.z * oz + m.r2.w;
-    rdx = m.r0.x * dx + m.r0.y * dy + m.r0.z * dz;
-    rdy = m.r1.x * dx + m.r1.y * dy + m.r1.z * dz;
-    rdz = m.r2.x * dx + m.r2.y * dy + m.r2.z * dz;
+    const trav::RayArrayPolicy<kAny> pol{&a};
+    trav::TraceFlatLoop<kAny>(a, pol, lds_stack);
 }
 
 template <bool kAny>
 __global__ __launch_bounds__(kMaxBlock) void TraceTwoLevelKernel(TraceArgs a) {
     __shared__ int2 lds_stack[kLdsStack * kMaxBlock];
-    const int lane = static_cast<int>(threadIdx.x & 63u);
-    const int64_t gtid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    Stack st{lds_stack + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
-             a.capacity, 0, false};
-
-    if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.next_counter = 0u;
-    const int32_t n = a.n_device != nullptr ? static_cast<int32_t>(*a.n_device) : a.n;
-    uint64_t sum_nt = 0, sum_tt = 0;
-    int32_t ray = -1;
-    bool exhausted = false;
-    float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, t_min = 0;
-    RayPre pre{};
-    Best best{0, 0, 0, -1, false};
-    int32_t best_inst = -1;
-    uint32_t nt = 0, tt = 0;
-    // object-space state of the BLAS being traversed
-    bool in_blas = false;
-    int blas_base = 0;
-    int32_t cur_inst = -1;
-    float bx = 0, by = 0, bz = 0, bdx = 0, bdy = 0, bdz = 0;
-    RayPre bpre{};
-    Best bbest{0, 0, 0, -1, false};
-    // TLAS leaf being expanded
-    int32_t cursor = 0, leaf_end = 0;
-
-    while (true) {
-        const bool busy = in_blas || cursor < leaf_end || st.sp > 0;
-        if (!busy && ray >= 0) {
-            if (kAny) {
-                a.occluded[ray] = best.hit ? 1u : 0u;
-            } else {
-                a.hit[ray] = make_float4(best.t, best.u, best.v, __int_as_float(best.hit ? best.id : -1));
-                if (a.inst_out != nullptr) a.inst_out[ray] = best.hit ? best_inst : -1;
-            }
-            if (a.node_tests != nullptr) a.node_tests[ray] = nt;
-            if (a.triangle_tests != nullptr) a.triangle_tests[ray] = tt;
-            sum_nt += nt;
-            sum_tt += tt;
-            ray = -1;
-        }
-        const bool want = !busy && !exhausted;
-        const int32_t claimed = ClaimRay(a.counter, want, lane, n, exhausted);
-        if (claimed >= 0) {
-            ray = claimed;
-            const float4 o4 = a.org[ray];
-            const float4 d4 = a.dir[ray];
-            ox = o4.x; oy = o4.y; oz = o4.z; t_min = o4.w;
-            dx = d4.x; dy = d4.y; dz = d4.z;
-            best = Best{d4.w, 0.0f, 0.0f, -1, false};
-            best_inst = -1;
-            nt = 0;
-            tt = 0;
-            in_blas = false;
-            cursor = leaf_end = 0;
-            pre = Precompute(ox, oy, oz, dx, dy, dz);
-            if (!a.empty) {
-                ++nt;
-                float entry;
-                if (Slab(pre, a.root_lo_x, a.root_lo_y, a.root_lo_z, a.root_hi_x, a.root_hi_y, a.root_hi_z,
-                         t_min, best.t, entry)) {
-                    st.Push(a.root_code, entry);
-                }
-            }
-        }
-        if (__ballot(in_blas || cursor < leaf_end || st.sp > 0 || ray >= 0 || !exhausted) == 0) break;
-
-        while (true) {
-            if (in_blas && st.sp == blas_base) {
-                // BLAS finished: accept only a strictly closer hit (two_level_bvh.cpp:474);
-                // an occlusion query accepts any hit within [t_min, t_max].
-                if (bbest.hit && (kAny || bbest.t < best.t)) {
-                    best = bbest;
-                    best_inst = cur_inst;
-                    if (kAny) {
-                        st.sp = 0;
-                        cursor = leaf_end;
-                    }
-                }
-                in_blas = false;
-                ++cursor;
-            } else if (!in_blas && cursor < leaf_end) {
-                // Next instance of the TLAS leaf: object ray, BLAS root box.
-                cur_inst = a.tslot_instance[cursor];
-                const DevInstance m = a.instances[cur_inst];
-                const DevBlasRoot root = a.blas_roots[m.blas.x];
-                TransformRay(m, ox, oy, oz, dx, dy, dz, bx, by, bz, bdx, bdy, bdz);
-                bpre = Precompute(bx, by, bz, bdx, bdy, bdz);
-                ++nt;
-                float entry;
-                if (Slab(bpre, root.lo.x, root.lo.y, root.lo.z, root.hi.x, root.hi.y, root.hi.z, t_min, best.t,
-                         entry)) {
-                    bbest = Best{best.t, 0.0f, 0.0f, -1, false};
-                    blas_base = st.sp;
-                    st.Push(__float_as_int(root.lo.w), entry);
-                    in_blas = true;
-                } else {
-                    ++cursor;
-                }
-            } else if (st.sp > 0) {
-                const int2 e = st.Pop();
-                const int32_t code = e.x;
-                const float entry = __int_as_float(e.y);
-                if (in_blas) {
-                    if (!(entry > bbest.t)) {
-                        if (code < 0) {
-                            int32_t first, count;
-                            DecodeLeaf(code, a.leaf_count, first, count);
-                            for (int32_t k = 0; k < count; ++k) {
-                                const float4* s = a.slots + 3 * static_cast<int64_t>(first + k);
-                                ++tt;
-                                if (TriangleTest(s[0], s[1], s[2], bx, by, bz, bdx, bdy, bdz, t_min, kAny, bbest) &&
-                                    kAny) {
-                                    break;
-                                }
-                            }
-                            if (kAny && bbest.hit) st.sp = blas_base;  // finish this BLAS now
-                        } else {
-                            InteriorStep(a.nodes, code, bpre, t_min, bbest.t, nt, st);
-                        }
-                    }
-                } else if (!(entry > best.t)) {
-                    if (code < 0) {
-                        DecodeLeaf(code, a.tleaf_count, cursor, leaf_end);
-                        leaf_end += cursor;
-                    } else {
-                        InteriorStep(a.tnodes, code, pre, t_min, best.t, nt, st);
-                    }
-                }
-            }
-            const bool lane_busy = in_blas || cursor < leaf_end || st.sp > 0;
-            const uint64_t active = __ballot(lane_busy);
-            if (active == 0) break;
-            const uint64_t idle = __ballot(!lane_busy && (ray >= 0 || !exhausted));
-            if (__popcll(idle) >= kRefillIdleLanes) break;
-        }
-    }
-    if (st.overflow) atomicOr(a.overflow, 1u);
-    if (a.totals != nullptr) AddTotals(a.totals, sum_nt, sum_tt, lane);
+    const trav::RayArrayPolicy<kAny> pol{&a};
+    trav::TraceTwoLevelLoop<kAny>(a, pol, lds_stack);
 }
 
-int PrepareArgs(yrh_context* ctx, const TraceRequest& r, TraceArgs& a, int& grid, int& block) {
+bool Aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+}  // namespace
+
+int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a, int& grid, int& block) {
     const DevGeometry& g = ctx->geo;
     block = ctx->block_size;
     grid = ctx->compute_units * ctx->blocks_per_cu;
@@ -548,10 +90,6 @@ int PrepareArgs(yrh_context* ctx, const TraceRequest& r, TraceArgs& a, int& grid
     return YRH_OK;
 }
 
-bool Aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
-
-}  // namespace
-
 int LaunchTrace(yrh_context* ctx, const TraceRequest& r) {
     if (!Aligned16(r.org) || !Aligned16(r.dir) || (r.hit != nullptr && !Aligned16(r.hit))) {
         return Fail(ctx, YRH_E_INVALID_ARGUMENT, "ray and hit arrays must be 16-byte aligned");
@@ -560,7 +98,7 @@ int LaunchTrace(yrh_context* ctx, const TraceRequest& r) {
     (void)hipSetDevice(ctx->device);
     TraceArgs a;
     int grid = 0, block = 0;
-    int rc = PrepareArgs(ctx, r, a, grid, block);
+    int rc = PrepareTraceArgs(ctx, r, a, grid, block);
     if (rc != YRH_OK) return rc;
     const bool two = ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL;
     if (r.any) {

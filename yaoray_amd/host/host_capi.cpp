@@ -60,6 +60,12 @@ int yrhost_scene_create(int kind, const int64_t* params, int param_count, uint64
                                               seed, static_cast<int>(ew), static_cast<int>(eh));
             break;
         }
+        case YRHOST_SCENE_ROOM: {
+            const int64_t tris = param(0, 200000), ew = param(1, 1024), eh = param(2, 512);
+            if (tris < 0 || tris > 20000000 || ew < 0 || eh < 0) return YRH_E_INVALID_ARGUMENT;
+            s->ir = yrhip::MakeGuidingRoomScene(static_cast<int>(tris), seed, static_cast<int>(ew), static_cast<int>(eh));
+            break;
+        }
         default:
             return YRH_E_INVALID_ARGUMENT;
     }

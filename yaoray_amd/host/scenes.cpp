@@ -109,7 +109,9 @@ struct MeshBuilder {
 
 float TriangleArea(Point3f a, Point3f b, Point3f c) { return Length(Cross(b - a, c - a)) * 0.5f; }
 
-void AddIcosphere(MeshBuilder& mb, int subdivisions) {
+// Unit icosphere subdivided `subdivisions` times, scaled by `radius` about
+// `centre`, with its smooth normals.
+void AddIcosphere(MeshBuilder& mb, int subdivisions, Point3f centre = {}, float radius = 1.0f) {
     const float t = (1.0f + std::sqrt(5.0f)) * 0.5f;
     std::vector<Vec3f> v = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0}, {0, -1, t}, {0, 1, t},
                             {0, -1, -t}, {0, 1, -t}, {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};
@@ -141,7 +143,7 @@ void AddIcosphere(MeshBuilder& mb, int subdivisions) {
         f.swap(nf);
     }
     const std::uint32_t base = static_cast<std::uint32_t>(mb.s.vertices.size());
-    for (const Vec3f& p : v) mb.Vertex(p, p);
+    for (const Vec3f& p : v) mb.Vertex(centre + p * radius, p);
     for (const auto& tri : f) {
         mb.Tri(base + static_cast<std::uint32_t>(tri[0]), base + static_cast<std::uint32_t>(tri[1]),
                base + static_cast<std::uint32_t>(tri[2]));
@@ -236,6 +238,8 @@ RenderSceneIR MakeRandomTriangleScene(int count, std::uint64_t seed) {
     return s;
 }
 
+void AddSkyEnvironment(RenderSceneIR& s, int env_width, int env_height);
+
 RenderSceneIR MakeInstancedScene(int instance_count, int subdivisions, int variants, std::uint64_t seed, int env_width,
                                  int env_height) {
     RenderSceneIR s;
@@ -285,7 +289,15 @@ RenderSceneIR MakeInstancedScene(int instance_count, int subdivisions, int varia
                                    Multiply(RotationAxisMatrix(angle, axis), ScaleMatrix({scale, scale, scale})));
         s.instances.push_back(RenderInstance{prim, o2w});
     }
-    // Procedural sky + sun (equirect, u from atan2(x, z), v = theta / pi).
+    AddSkyEnvironment(s, env_width, env_height);
+    s.camera = LookAtCamera({0.0f, 6.0f, 40.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, 40.0f);
+    PrepareLightSampling(s);
+    return s;
+}
+
+// Procedural sky + sun (equirect, u from atan2(x, z), v = theta / pi) as the
+// scene's environment, with its sampling distribution.
+void AddSkyEnvironment(RenderSceneIR& s, int env_width, int env_height) {
     if (env_width > 0 && env_height > 0) {
         RenderTexture tex;
         tex.width = env_width;
@@ -306,10 +318,106 @@ RenderSceneIR MakeInstancedScene(int instance_count, int subdivisions, int varia
         s.environment.active = true;
         s.environment.radiance = Color3f{1.0f, 1.0f, 1.0f};
         s.environment.strength = 1.0f;
-        s.environment.texture_index = 0;
-        s.environment.distribution_index = 0;
+        s.environment.texture_index = static_cast<int>(s.textures.size()) - 1;
+        s.environment.distribution_index = static_cast<int>(s.environment_distributions.size()) - 1;
     }
-    s.camera = LookAtCamera({0.0f, 6.0f, 40.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, 40.0f);
+}
+
+RenderSceneIR MakeGuidingRoomScene(int furniture_triangles, std::uint64_t seed, int env_width, int env_height) {
+    RenderSceneIR s;
+    Pcg32 rng(seed);
+    MeshBuilder mb{s};
+    auto diffuse = [&](Color3f c) {
+        RenderMaterial m;
+        m.kind = RenderMaterialKind::Diffuse;
+        m.reflectance = c;
+        s.materials.push_back(m);
+        return static_cast<int>(s.materials.size()) - 1;
+    };
+    const int walls = diffuse({0.75f, 0.75f, 0.72f});
+    const int floor = diffuse({0.55f, 0.5f, 0.45f});
+    const int ceiling = diffuse({0.8f, 0.8f, 0.8f});
+    const int blocker = diffuse({0.6f, 0.6f, 0.6f});
+    RenderMaterial lamp;
+    lamp.reflectance = Color3f{0.0f, 0.0f, 0.0f};
+    lamp.emission = Color3f{30.0f, 28.0f, 25.0f};
+    s.materials.push_back(lamp);
+    const int lamp_mat = static_cast<int>(s.materials.size()) - 1;
+    // Furniture materials: diffuse and GGX conductors (alpha in [0.05, 0.5]).
+    std::vector<int> furniture;
+    for (int k = 0; k < 8; ++k) {
+        if (k % 2 == 0) {
+            furniture.push_back(diffuse({0.15f + 0.7f * rng.NextFloat(), 0.15f + 0.7f * rng.NextFloat(),
+                                         0.15f + 0.7f * rng.NextFloat()}));
+        } else {
+            RenderMaterial m;
+            m.kind = RenderMaterialKind::Conductor;
+            m.reflectance = Color3f{0.5f + 0.5f * rng.NextFloat(), 0.5f + 0.5f * rng.NextFloat(),
+                                    0.5f + 0.5f * rng.NextFloat()};
+            const float alpha = 0.05f + 0.45f * rng.NextFloat();
+            m.uroughness = std::sqrt(alpha);
+            m.vroughness = m.uroughness;
+            s.materials.push_back(m);
+            furniture.push_back(static_cast<int>(s.materials.size()) - 1);
+        }
+    }
+    // Room 10 x 10 x 4: floor, ceiling, three solid walls and a wall at x = 10
+    // with a 1 x 1 window (y in [2, 3], z in [4.5, 5.5]).
+    mb.BeginPrimitive(floor);
+    mb.Quad({0, 0, 10}, {10, 0, 10}, {10, 0, 0}, {0, 0, 0});
+    mb.BeginPrimitive(ceiling);
+    mb.Quad({0, 4, 0}, {10, 4, 0}, {10, 4, 10}, {0, 4, 10});
+    mb.BeginPrimitive(walls);
+    mb.Quad({0, 0, 0}, {10, 0, 0}, {10, 4, 0}, {0, 4, 0});
+    mb.Quad({10, 0, 10}, {0, 0, 10}, {0, 4, 10}, {10, 4, 10});
+    mb.Quad({0, 0, 10}, {0, 0, 0}, {0, 4, 0}, {0, 4, 10});
+    mb.Quad({10, 0, 0}, {10, 0, 10}, {10, 2, 10}, {10, 2, 0});
+    mb.Quad({10, 3, 0}, {10, 3, 10}, {10, 4, 10}, {10, 4, 0});
+    mb.Quad({10, 2, 0}, {10, 2, 4.5f}, {10, 3, 4.5f}, {10, 3, 0});
+    mb.Quad({10, 2, 5.5f}, {10, 2, 10}, {10, 3, 10}, {10, 3, 5.5f});
+    // Ceiling lamp hidden behind a panel hanging below it: it lights the room
+    // only through the ceiling and the gap around the panel.
+    const int light = mb.BeginPrimitive(lamp_mat);
+    const Point3f la{2, 3.98f, 8}, lb{3, 3.98f, 8}, lc{3, 3.98f, 7}, ld{2, 3.98f, 7};
+    mb.Quad(la, lb, lc, ld);
+    s.emissive_primitives.push_back(
+        EmissivePrimitive{light, lamp.emission, TriangleArea(la, lb, lc) + TriangleArea(la, lc, ld)});
+    mb.BeginPrimitive(blocker);
+    mb.Box({2.5f, 3.6f, 7.5f}, {0.9f, 0.02f, 0.9f}, 0.0f);
+    // Tables (top + four legs) and spheres on the floor and on the tables, up
+    // to `furniture_triangles` triangles.
+    int tris = 0;
+    std::vector<Point3f> tops;
+    for (int t = 0; t < 10 && tris < furniture_triangles; ++t) {
+        const Point3f c{1.5f + 7.0f * rng.NextFloat(), 0.0f, 1.5f + 7.0f * rng.NextFloat()};
+        const float yaw = 90.0f * rng.NextFloat();
+        const Vec3f half{0.5f + 0.4f * rng.NextFloat(), 0.03f, 0.35f + 0.3f * rng.NextFloat()};
+        mb.BeginPrimitive(furniture[static_cast<size_t>(t) % furniture.size()]);
+        mb.Box({c.x, 0.75f, c.z}, half, yaw);
+        const Mat4f rot = RotationAxisMatrix(yaw, Vec3f{0, 1, 0});
+        for (int l = 0; l < 4; ++l) {
+            const float sx = (l & 1) ? 1.0f : -1.0f, sz = (l & 2) ? 1.0f : -1.0f;
+            const Point3f off = TransformPoint(rot, Point3f{sx * (half.x - 0.05f), 0.0f, sz * (half.z - 0.05f)});
+            mb.Box({c.x + off.x, 0.36f, c.z + off.z}, {0.03f, 0.36f, 0.03f}, yaw);
+        }
+        tops.push_back(Point3f{c.x, 0.78f, c.z});
+        tris += 60;
+    }
+    while (tris < furniture_triangles) {
+        const float radius = 0.15f + 0.35f * rng.NextFloat();
+        Point3f c;
+        if (!tops.empty() && rng.NextFloat() < 0.4f) {
+            const Point3f& top = tops[rng.NextU32() % static_cast<std::uint32_t>(tops.size())];
+            c = Point3f{top.x + 0.3f * (rng.NextFloat() - 0.5f), top.y + radius, top.z + 0.3f * (rng.NextFloat() - 0.5f)};
+        } else {
+            c = Point3f{0.8f + 8.4f * rng.NextFloat(), radius, 0.8f + 8.4f * rng.NextFloat()};
+        }
+        mb.BeginPrimitive(furniture[rng.NextU32() % static_cast<std::uint32_t>(furniture.size())]);
+        AddIcosphere(mb, 4, c, radius);
+        tris += 5120;
+    }
+    AddSkyEnvironment(s, env_width, env_height);
+    s.camera = LookAtCamera({1.0f, 1.6f, 1.0f}, {6.0f, 1.2f, 6.5f}, {0.0f, 1.0f, 0.0f}, 65.0f);
     PrepareLightSampling(s);
     return s;
 }

@@ -42,4 +42,10 @@ yr::RenderSceneIR MakeRandomTriangleScene(int count, std::uint64_t seed);
 yr::RenderSceneIR MakeInstancedScene(int instance_count, int subdivisions, int variants, std::uint64_t seed,
                                      int env_width, int env_height);
 
+// Interior for learned guiding: a 10 x 10 x 4 room lit through one 1 x 1
+// window by the procedural sky + sun, and by a ceiling lamp hidden behind a
+// hanging panel (indirect-dominated); tables and icosphere objects, diffuse
+// and GGX conductor, up to `furniture_triangles` triangles; one flat BVH.
+yr::RenderSceneIR MakeGuidingRoomScene(int furniture_triangles, std::uint64_t seed, int env_width, int env_height);
+
 }  // namespace yrhip

@@ -24,6 +24,7 @@ typedef struct yrhost_scene yrhost_scene;
 #define YRHOST_SCENE_CORNELL 1   /* params: none (seed = material seed) */
 #define YRHOST_SCENE_TRIANGLES 2 /* params: [triangle count] */
 #define YRHOST_SCENE_INSTANCED 3 /* params: [instances, subdivisions, variants, env_w, env_h] */
+#define YRHOST_SCENE_ROOM 4      /* params: [furniture triangles, env_w, env_h] */
 
 typedef struct yrhost_scene_info {
     int64_t vertex_count;

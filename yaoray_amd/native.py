@@ -22,6 +22,7 @@ YRH_ACCEL_TWO_LEVEL = 1
 SCENE_CORNELL = 1
 SCENE_TRIANGLES = 2
 SCENE_INSTANCED = 3
+SCENE_ROOM = 4
 
 
 class BvhNode(C.Structure):
@@ -132,6 +133,18 @@ class SceneInfo(C.Structure):
                 ("camera", C.c_float * 13)]
 
 
+class GuideDesc(C.Structure):
+    _fields_ = [("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3), ("log2_table_size", C.c_int32),
+                ("min_resolution", C.c_int32), ("max_resolution", C.c_int32), ("learning_rate", C.c_float),
+                ("guide_fraction", C.c_float), ("sample_capacity", C.c_int32), ("seed", C.c_uint64)]
+
+
+GUIDE_OFF = 0
+GUIDE_SAMPLE = 1
+GUIDE_RECORD = 2
+GUIDE_MLP_PARAMS = 13312
+
+
 class RenderSettings(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("sampler", C.c_int32), ("spp", C.c_int32),
                 ("max_depth", C.c_int32), ("radiance_clamp", C.c_float), ("seed", C.c_uint64),
@@ -198,6 +211,17 @@ YRH_SIGNATURES = {
     "yrh_render_stats_read": (_I, [_VP, C.POINTER(RenderStats)]),
     "yrh_sample_values": (_I, [_VP, _I32, _U64, _I32, _I64, _VP, _VP, _VP, _VP, _I32, _VP]),
     "yrh_generate_camera_rays": (_I, [_VP, C.POINTER(RenderDesc), _VP, _VP]),
+    "yrh_guide_configure": (_I, [_VP, C.POINTER(GuideDesc)]),
+    "yrh_guide_set_mode": (_I, [_VP, _I32]),
+    "yrh_guide_train": (_I, [_VP, _I32, _I32]),
+    "yrh_guide_add_samples": (_I, [_VP, _I64, _VP]),
+    "yrh_guide_sample_stats": (_I, [_VP, C.POINTER(_I64), C.POINTER(_F)]),
+    "yrh_guide_info": (_I, [_VP, C.POINTER(_I32), C.POINTER(_I64), _VP, _VP]),
+    "yrh_guide_params_read": (_I, [_VP, _VP, _VP]),
+    "yrh_guide_params_write": (_I, [_VP, _VP, _VP]),
+    "yrh_guide_query": (_I, [_VP, _I64, _VP, _VP]),
+    "yrh_guide_gradients": (_I, [_VP, _I64, _VP, _F, _VP, _VP, C.POINTER(_F)]),
+    "yrh_guide_mixture_eval": (_I, [_VP, _I64, _VP, _VP, _VP, _VP, _VP]),
 }
 
 HOST_SIGNATURES = {
