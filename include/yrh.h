@@ -362,6 +362,42 @@ int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* desc);
 /* Reads the accumulated counters (synchronises). */
 int yrh_render_stats_read(yrh_context* ctx, yrh_render_stats* stats);
 
+/* ---- Batched multi-camera rendering (no counterpart in the reference, which
+ * renders one job per Render call) ------------------------------------------ */
+
+typedef struct yrh_batch_job {
+    float camera_origin[3];
+    float camera_forward[3];
+    float camera_right[3];
+    float camera_up[3];
+    float tan_half_fov; /* std::tan(fov_y_radians * 0.5f) */
+    int32_t width;
+    int32_t height;
+    int32_t spp;
+} yrh_batch_job;
+
+typedef struct yrh_batch_desc {
+    const yrh_batch_job* jobs;
+    int32_t job_count;
+    int32_t sampler; /* YRH_SAMPLER_*, shared by the jobs */
+    int32_t max_depth;
+    float radiance_clamp;
+    uint64_t seed;
+    int64_t paths_per_pass; /* wavefront size; 0 = min(2^23, total pixels) */
+    int32_t use_graph;      /* 1: capture one pass as a HIP graph and replay it per pass */
+    int32_t reserved;
+} yrh_batch_desc;
+
+/* Renders every job to its full spp in one pass schedule: each pass carries
+ * the next pixel range of the current sample of as many jobs as fit (a job at
+ * most once per pass), so each job's FP64 film equals rendering that job
+ * alone with yrh_render_pass. Synchronises. job_ms (host, job_count entries,
+ * or NULL) receives each job's completion time since the start in ms;
+ * passes_out (or NULL) the number of passes. Counters: yrh_render_stats_read. */
+int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* desc, float* job_ms, int32_t* passes_out);
+/* Copies one job's film (width*height pixels) of the last batch to the host. */
+int yrh_batch_film_read(yrh_context* ctx, int32_t job, yrh_film_pixel* pixels, int64_t count);
+
 /* Batched sample generation: out[i] = Sampler(kind, seed, x[i], y[i], sample[i],
  * spp).Sample1D/2D at absolute dimension dim[i] (src/sampling/sampler.cpp:122-237),
  * computed on the device; dims = 1 or 2 floats per entry. Host arrays. */

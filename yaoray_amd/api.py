@@ -486,6 +486,33 @@ class Device:
         self.synchronize()
         return d_org.download(np.float32, (n, 4)), d_dir.download(np.float32, (n, 4))
 
+    # -- batched multi-camera rendering --------------------------------------
+    def render_batch(self, jobs, max_depth=8, sampler=N.SAMPLER_OWEN_SOBOL, seed=0x59414F524159, radiance_clamp=0.0,
+                     paths_per_pass=0, use_graph=True):
+        """jobs: list of RenderDesc (camera, tan_half_fov, width, height, spp; e.g. from Scene.render_desc).
+        Returns (FP64 film records per job, job completion ms, passes, stats)."""
+        arr = (N.BatchJob * len(jobs))()
+        for k, j in enumerate(jobs):
+            b = arr[k]
+            b.camera_origin[:] = list(j.camera_origin)
+            b.camera_forward[:] = list(j.camera_forward)
+            b.camera_right[:] = list(j.camera_right)
+            b.camera_up[:] = list(j.camera_up)
+            b.tan_half_fov = j.tan_half_fov
+            b.width, b.height, b.spp = j.width, j.height, j.spp
+        d = N.BatchDesc(C.cast(arr, C.c_void_p), len(jobs), sampler, max_depth, radiance_clamp, seed, paths_per_pass,
+                        1 if use_graph else 0, 0)
+        ms = np.zeros(len(jobs), np.float32)
+        passes = C.c_int32()
+        self._check(self._lib.yrh_batch_render(self._ctx, C.byref(d), ms.ctypes.data, C.byref(passes)), "render_batch")
+        films = []
+        for k, j in enumerate(jobs):
+            n = j.width * j.height
+            f = np.zeros(n, FILM64_DTYPE)
+            self._check(self._lib.yrh_batch_film_read(self._ctx, k, f.ctypes.data, n), "batch_film_read")
+            films.append(f)
+        return films, ms, passes.value, self.render_stats()
+
     # -- learned guiding -----------------------------------------------------
     def guide_configure(self, bounds_min, bounds_max, log2_table_size=16, min_resolution=8, max_resolution=256,
                         learning_rate=1e-2, guide_fraction=0.5, sample_capacity=1 << 20, seed=7):

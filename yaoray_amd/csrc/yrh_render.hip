@@ -37,9 +37,11 @@ enum Counter : int {
     kCountQueue0 = 0,
     kCountQueue1 = 1,
     kCountShadow = 2,
-    kWorkExtend = 3,  // per-launch ray counters of the trace kernels
-    kWorkShadow = 4,
-    kCounterWords = 5,
+    kCountPassPaths = 3,  // paths of the current pass (batched passes)
+    kPassIndex = 4,       // current pass of a batch schedule (advanced by EndPassKernel)
+    kWorkExtend = 5,      // per-launch ray counters of the trace kernels
+    kWorkShadow = 6,
+    kCounterWords = 7,
 };
 
 // 64-bit statistics words.
@@ -112,6 +114,11 @@ struct RenderState {
     DevBuf l_state, l_med;
     DevBuf counters;  // uint32[kCounterWords]
     DevBuf stats;     // uint64[kStatWords]
+    // batched passes
+    DevBuf p_job, p_sample;
+    DevBuf b_jobs, b_items, b_pass_items, b_film;
+    std::vector<int64_t> b_job_pixels;  // per job: pixel count
+    std::vector<int64_t> b_job_offset;  // per job: first pixel in b_film
 };
 
 namespace {
@@ -179,6 +186,19 @@ struct LightView {
     const float* positions_xyz;  // unused on device (kept for symmetry)
 };
 
+// Batched multi-camera passes: the camera jobs, and per pass a list of work
+// items (job, sample index, pixel range) laid end to end over the pass's paths.
+struct DevJob {
+    DevCamera camera;
+    int32_t width, height, spp;
+    int64_t film_offset;  // first pixel of the job in the batch film
+};
+
+struct DevItem {
+    int32_t job, sample, pixel_begin, pixel_count;
+    int64_t path_begin;  // first path of the item within its pass
+};
+
 struct PassView {
     int32_t width, height;
     int32_t sampler, spp, max_depth, sample_index;
@@ -215,6 +235,12 @@ struct PassView {
     uint32_t* counters;
     unsigned long long* stats;
     yrh_film_pixel* film;
+    // batched passes (jobs == nullptr: one camera, width x height x sample_count paths)
+    const DevJob* jobs;
+    const DevItem* items;
+    const int32_t* pass_items;  // items of pass k: [pass_items[k], pass_items[k + 1])
+    int32_t* p_job;
+    int32_t* p_sample;
 };
 
 enum PathFlag : int { kPrevValid = 1, kPrevDelta = 2, kMediumActive = 4, kPrimaryHit = 8 };
@@ -508,19 +534,62 @@ __device__ __forceinline__ float EmissivePdfSolidAngle(const LightView& L, float
 
 __global__ void RaygenKernel(PassView P) {
     const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t total = P.npix * P.sample_count;
-    if (p == 0) P.counters[kCountQueue0] = static_cast<uint32_t>(total);
-    if (p >= total) return;
-    const int64_t pixel = p % P.npix;
-    const int x = static_cast<int>(pixel % P.width);
-    const int y = static_cast<int>(pixel / P.width);
+    int64_t pixel;
+    int x, y, sample, spp, width, height;
+    DevCamera camera;
+    if (P.jobs != nullptr) {
+        // Batched pass: find the work item holding path p (items are ordered by path_begin).
+        const int32_t pass = static_cast<int32_t>(P.counters[kPassIndex]);
+        const int32_t ib = P.pass_items[pass], ie = P.pass_items[pass + 1];
+        const DevItem last = P.items[ie - 1];
+        const int64_t total = last.path_begin + last.pixel_count;
+        if (p == 0) {
+            P.counters[kCountQueue0] = static_cast<uint32_t>(total);
+            P.counters[kCountPassPaths] = static_cast<uint32_t>(total);
+        }
+        if (p >= total) return;
+        int32_t lo = ib, hi = ie - 1;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (P.items[mid].path_begin <= p) {
+                lo = mid;
+            } else {
+                hi = mid - 1;
+            }
+        }
+        const DevItem it = P.items[lo];
+        const DevJob& J = P.jobs[it.job];
+        const int64_t local = it.pixel_begin + (p - it.path_begin);
+        x = static_cast<int>(local % J.width);
+        y = static_cast<int>(local / J.width);
+        sample = it.sample;
+        spp = J.spp;
+        width = J.width;
+        height = J.height;
+        camera = J.camera;
+        pixel = J.film_offset + local;
+        P.p_job[p] = it.job;
+        P.p_sample[p] = it.sample;
+    } else {
+        const int64_t total = P.npix * P.sample_count;
+        if (p == 0) P.counters[kCountQueue0] = static_cast<uint32_t>(total);
+        if (p >= total) return;
+        pixel = p % P.npix;
+        x = static_cast<int>(pixel % P.width);
+        y = static_cast<int>(pixel / P.width);
+        sample = P.sample_index + static_cast<int>(p / P.npix);
+        spp = P.spp;
+        width = P.width;
+        height = P.height;
+        camera = P.camera;
+    }
     PixelSampler smp;
-    smp.Init(P.sampler, P.seed, x, y, P.sample_index + static_cast<int>(p / P.npix), P.spp);
+    smp.Init(P.sampler, P.seed, x, y, sample, spp);
     const float2 s = smp.At2D(kDimPixel, 0);
-    const float w = static_cast<float>(P.width < 1 ? 1 : P.width);
-    const float h = static_cast<float>(P.height < 1 ? 1 : P.height);
-    const float3 d = CameraDirection(P.camera, w, h, static_cast<float>(x) + s.x, static_cast<float>(y) + s.y);
-    P.q_org[0][p] = F4(P.camera.origin, 1.0e-5f);
+    const float w = static_cast<float>(width < 1 ? 1 : width);
+    const float h = static_cast<float>(height < 1 ? 1 : height);
+    const float3 d = CameraDirection(camera, w, h, static_cast<float>(x) + s.x, static_cast<float>(y) + s.y);
+    P.q_org[0][p] = F4(camera.origin, 1.0e-5f);
     P.q_dir[0][p] = F4(d, __int_as_float(0x7f800000));
     P.q_path[0][p] = static_cast<int32_t>(p);
     P.p_pixel[p] = static_cast<int32_t>(pixel);
@@ -671,7 +740,14 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
     P.p_slot[p] = make_int2(-1, -1);
     const int pix = P.p_pixel[p];
     PixelSampler smp;
-    smp.Init(P.sampler, P.seed, pix % P.width, pix / P.width, P.sample_index + static_cast<int>(p / P.npix), P.spp);
+    if (P.jobs != nullptr) {
+        const DevJob& J = P.jobs[P.p_job[p]];
+        const int64_t local = pix - J.film_offset;
+        smp.Init(P.sampler, P.seed, static_cast<int>(local % J.width), static_cast<int>(local / J.width), P.p_sample[p],
+                 J.spp);
+    } else {
+        smp.Init(P.sampler, P.seed, pix % P.width, pix / P.width, P.sample_index + static_cast<int>(p / P.npix), P.spp);
+    }
     smp.bounce = depth;
 
     if (tri < 0) {
@@ -1141,13 +1217,11 @@ __global__ __launch_bounds__(trav::kMaxBlock) void ShadowWalkKernel(trav::TraceA
     }
 }
 
-// Adds the pass's samples to the FP64 film (Film::AddSample, src/film/film.cpp:32-56):
-// one thread per pixel, samples in index order.
-__global__ __launch_bounds__(kBlock) void AccumulateKernel(PassView P) {
-    const int64_t pixel = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+// Film::AddSample (src/film/film.cpp:32-56) of path p's sample into pixel px,
+// in FP64; returns 1 for a dropped non-finite sample.
+__device__ __forceinline__ unsigned AddSample(const PassView& P, int64_t p, yrh_film_pixel& px) {
     unsigned bad = 0;
-    for (int k = 0; pixel < P.npix && k < P.sample_count; ++k) {
-        const int64_t p = static_cast<int64_t>(k) * P.npix + pixel;
+    {
         float3 b = XYZ(P.p_rad[p]);
         if (P.radiance_clamp > 0.0f) {
             const float m = MaxC(b);
@@ -1156,7 +1230,6 @@ __global__ __launch_bounds__(kBlock) void AccumulateKernel(PassView P) {
         if (!Finite3(b)) {
             ++bad;
         } else {
-            yrh_film_pixel& px = P.film[pixel];
             px.sum[0] += static_cast<double>(b.x);
             px.sum[1] += static_cast<double>(b.y);
             px.sum[2] += static_cast<double>(b.z);
@@ -1183,7 +1256,31 @@ __global__ __launch_bounds__(kBlock) void AccumulateKernel(PassView P) {
             }
         }
     }
+    return bad;
+}
+
+// One thread per pixel, the pass's samples in index order.
+__global__ __launch_bounds__(kBlock) void AccumulateKernel(PassView P) {
+    const int64_t pixel = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned bad = 0;
+    for (int k = 0; pixel < P.npix && k < P.sample_count; ++k) {
+        bad += AddSample(P, static_cast<int64_t>(k) * P.npix + pixel, P.film[pixel]);
+    }
     WaveStat(P.stats + kStatBad, bad);
+}
+
+// Batched pass: one thread per path; a pass holds each (job, pixel) at most
+// once, so the film updates do not collide and every pixel receives its
+// samples in index order over the passes.
+__global__ __launch_bounds__(kBlock) void AccumulateBatchKernel(PassView P) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned bad = 0;
+    if (p < static_cast<int64_t>(P.counters[kCountPassPaths])) bad = AddSample(P, p, P.film[P.p_pixel[p]]);
+    WaveStat(P.stats + kStatBad, bad);
+}
+
+__global__ void EndPassKernel(uint32_t* counters) {
+    if (threadIdx.x == 0) counters[kPassIndex] += 1u;
 }
 
 __global__ void SampleValuesKernel(int kind, uint64_t seed, int spp, int64_t n, const int32_t* x, const int32_t* y,
@@ -1248,6 +1345,8 @@ PassView MakePassView(RenderState& R, const yrh_render_desc& d) {
     P.counters = R.counters.as<uint32_t>();
     P.stats = R.stats.as<unsigned long long>();
     P.film = R.film.as<yrh_film_pixel>();
+    P.p_job = R.p_job.as<int32_t>();
+    P.p_sample = R.p_sample.as<int32_t>();
     return P;
 }
 
@@ -1303,7 +1402,7 @@ int EnsureCapacity(yrh_context* ctx, RenderState& R, int64_t paths) {
         DevBuf* buf;
         size_t bytes;
     } items[] = {
-        {&R.p_pixel, 4 * P}, {&R.p_tp, 16 * P}, {&R.p_rad, 16 * P}, {&R.p_prev, 16 * P}, {&R.p_flags, 4 * P},
+        {&R.p_pixel, 4 * P}, {&R.p_job, 4 * P}, {&R.p_sample, 4 * P}, {&R.p_tp, 16 * P}, {&R.p_rad, 16 * P}, {&R.p_prev, 16 * P}, {&R.p_flags, 4 * P},
         {&R.p_med, 16 * P}, {&R.p_alb, 16 * P}, {&R.p_nrm, 16 * P}, {&R.p_nee, 16 * P}, {&R.p_slot, 8 * P},
         {&R.q_org[0], 16 * P}, {&R.q_org[1], 16 * P}, {&R.q_dir[0], 16 * P}, {&R.q_dir[1], 16 * P},
         {&R.q_path[0], 4 * P}, {&R.q_path[1], 4 * P}, {&R.hits, 16 * P}, {&R.hit_inst, 4 * P},
@@ -1321,6 +1420,74 @@ int EnsureCapacity(yrh_context* ctx, RenderState& R, int64_t paths) {
     rc = CheckHip(ctx, hipMemset(R.counters.ptr, 0, 4 * kCounterWords), "hipMemset");
     R.capacity = paths;
     return rc;
+}
+
+// Raygen and the bounces of one pass over `paths` path slots (the queue
+// capacity; the kernels read the live counts from the counter words).
+int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, const SurfaceView& S, const LightView& L,
+                       const yrg::GuideView& GV, int64_t paths, int max_depth) {
+    hipStream_t st = ctx->stream;
+    const bool guided = GV.mode != YRH_GUIDE_OFF;
+    int rc;
+    hipLaunchKernelGGL(RaygenKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P);
+    for (int depth = 0; depth < max_depth; ++depth) {
+        const int cur = depth & 1;
+        hipLaunchKernelGGL(BeginBounceKernel, dim3(1), dim3(64), 0, st, P.counters, 1 - cur);
+        TraceRequest tr;
+        tr.any = false;
+        tr.org = P.q_org[cur];
+        tr.dir = P.q_dir[cur];
+        tr.n_device = P.counters + cur;
+        tr.capacity = paths;
+        tr.work_counter = P.counters + kWorkExtend;
+        tr.hit = P.hits;
+        tr.instance = ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL ? P.hit_inst : nullptr;
+        tr.totals = P.stats + kStatTraceNodes;
+        if ((rc = LaunchTrace(ctx, tr)) != YRH_OK) return rc;
+        PassView Pk = P;
+        if (ctx->geo.accel_kind != YRH_ACCEL_TWO_LEVEL) Pk.hit_inst = nullptr;
+        if (guided && depth + 1 < max_depth) {
+            hipLaunchKernelGGL(GuideQueryKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, cur);
+            hipLaunchKernelGGL(ShadeKernel<true>, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
+        } else {
+            hipLaunchKernelGGL(ShadeKernel<false>, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
+        }
+        if (!R.transparent) {
+            TraceRequest sr;
+            sr.any = true;
+            sr.org = P.s_org;
+            sr.dir = P.s_dir;
+            sr.n_device = P.counters + kCountShadow;
+            sr.capacity = 2 * paths;
+            sr.work_counter = P.counters + kWorkShadow;
+            sr.occluded = P.s_occ;
+            sr.totals = P.stats + kStatTraceNodes;
+            if ((rc = LaunchTrace(ctx, sr)) != YRH_OK) return rc;
+        } else {
+            TraceRequest wr;
+            wr.any = false;
+            wr.org = P.s_org;
+            wr.dir = P.s_dir;
+            wr.n_device = P.counters + kCountShadow;
+            wr.capacity = 2 * paths;
+            wr.work_counter = P.counters + kWorkShadow;
+            wr.hit = P.s_trans;  // not written by the walk; LaunchTrace's checks want an output
+            wr.totals = P.stats + kStatTraceNodes;
+            trav::TraceArgs wa;
+            int wgrid = 0, wblock = 0;
+            if ((rc = PrepareTraceArgs(ctx, wr, wa, wgrid, wblock)) != YRH_OK) return rc;
+            if (ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL) {
+                hipLaunchKernelGGL(ShadowWalkKernel<true>, dim3(wgrid), dim3(wblock), 0, st, wa, P, S, L);
+            } else {
+                hipLaunchKernelGGL(ShadowWalkKernel<false>, dim3(wgrid), dim3(wblock), 0, st, wa, P, S, L);
+            }
+        }
+        hipLaunchKernelGGL(ResolveKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P, GV, depth, cur);
+    }
+    if (GV.rec != nullptr) {
+        hipLaunchKernelGGL(GuideCollectKernel, dim3(Grid(paths * max_depth)), dim3(kBlock), 0, st, P, GV);
+    }
+    return YRH_OK;
 }
 
 }  // namespace
@@ -1634,68 +1801,189 @@ int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* d) {
     const int max_depth = std::max(1, d->max_depth);
     yrg::GuideView GV;
     if ((rc = GuidePrepareView(ctx, paths, max_depth, &GV)) != YRH_OK) return rc;
-    const bool guided = GV.mode != YRH_GUIDE_OFF;
-    hipLaunchKernelGGL(RaygenKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P);
-    for (int depth = 0; depth < max_depth; ++depth) {
-        const int cur = depth & 1;
-        hipLaunchKernelGGL(BeginBounceKernel, dim3(1), dim3(64), 0, st, P.counters, 1 - cur);
-        TraceRequest tr;
-        tr.any = false;
-        tr.org = P.q_org[cur];
-        tr.dir = P.q_dir[cur];
-        tr.n_device = P.counters + cur;
-        tr.capacity = paths;
-        tr.work_counter = P.counters + kWorkExtend;
-        tr.hit = P.hits;
-        tr.instance = ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL ? P.hit_inst : nullptr;
-        tr.totals = P.stats + kStatTraceNodes;
-        if ((rc = LaunchTrace(ctx, tr)) != YRH_OK) return rc;
-        PassView Pk = P;
-        if (ctx->geo.accel_kind != YRH_ACCEL_TWO_LEVEL) Pk.hit_inst = nullptr;
-        if (guided && depth + 1 < max_depth) {
-            hipLaunchKernelGGL(GuideQueryKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, cur);
-            hipLaunchKernelGGL(ShadeKernel<true>, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
-        } else {
-            hipLaunchKernelGGL(ShadeKernel<false>, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
-        }
-        if (!R.transparent) {
-            TraceRequest sr;
-            sr.any = true;
-            sr.org = P.s_org;
-            sr.dir = P.s_dir;
-            sr.n_device = P.counters + kCountShadow;
-            sr.capacity = 2 * paths;
-            sr.work_counter = P.counters + kWorkShadow;
-            sr.occluded = P.s_occ;
-            sr.totals = P.stats + kStatTraceNodes;
-            if ((rc = LaunchTrace(ctx, sr)) != YRH_OK) return rc;
-        } else {
-            TraceRequest wr;
-            wr.any = false;
-            wr.org = P.s_org;
-            wr.dir = P.s_dir;
-            wr.n_device = P.counters + kCountShadow;
-            wr.capacity = 2 * paths;
-            wr.work_counter = P.counters + kWorkShadow;
-            wr.hit = P.s_trans;  // not written by the walk; LaunchTrace's checks want an output
-            wr.totals = P.stats + kStatTraceNodes;
-            trav::TraceArgs wa;
-            int wgrid = 0, wblock = 0;
-            if ((rc = PrepareTraceArgs(ctx, wr, wa, wgrid, wblock)) != YRH_OK) return rc;
-            if (ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL) {
-                hipLaunchKernelGGL(ShadowWalkKernel<true>, dim3(wgrid), dim3(wblock), 0, st, wa, P, S, L);
-            } else {
-                hipLaunchKernelGGL(ShadowWalkKernel<false>, dim3(wgrid), dim3(wblock), 0, st, wa, P, S, L);
-            }
-        }
-        hipLaunchKernelGGL(ResolveKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P, GV, depth, cur);
-    }
-    if (GV.rec != nullptr) {
-        hipLaunchKernelGGL(GuideCollectKernel, dim3(Grid(paths * max_depth)), dim3(kBlock), 0, st, P, GV);
-    }
+    if ((rc = LaunchPassSequence(ctx, R, P, S, L, GV, paths, max_depth)) != YRH_OK) return rc;
     hipLaunchKernelGGL(AccumulateKernel, dim3(Grid(npix)), dim3(kBlock), 0, st, P);
     hipLaunchKernelGGL(CopyTraceTotalsKernel, dim3(1), dim3(64), 0, st, P.stats);
     return CheckHip(ctx, hipGetLastError(), "render pass launch");
+}
+
+// ---- batched multi-camera rendering ----------------------------------------------
+
+int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* d, float* job_ms, int32_t* passes_out) {
+    if (ctx == nullptr || d == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null argument");
+    if (ctx->render == nullptr || !ctx->render->shading_loaded) return Fail(ctx, YRH_E_NO_SCENE, "upload geometry and shading first");
+    if (d->job_count <= 0 || d->jobs == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "no jobs");
+    if (d->sampler < YRH_SAMPLER_INDEPENDENT || d->sampler > YRH_SAMPLER_ZSOBOL) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "unknown sampler");
+    RenderState& R = *ctx->render;
+    // Jobs and their film layout.
+    std::vector<DevJob> jobs(static_cast<size_t>(d->job_count));
+    R.b_job_pixels.assign(jobs.size(), 0);
+    R.b_job_offset.assign(jobs.size(), 0);
+    int64_t film_pixels = 0, largest = 0;
+    for (int32_t j = 0; j < d->job_count; ++j) {
+        const yrh_batch_job& b = d->jobs[j];
+        if (b.width <= 0 || b.height <= 0 || b.spp <= 0) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "bad job size or spp");
+        DevJob& J = jobs[static_cast<size_t>(j)];
+        J.camera.origin = make_float3(b.camera_origin[0], b.camera_origin[1], b.camera_origin[2]);
+        J.camera.forward = make_float3(b.camera_forward[0], b.camera_forward[1], b.camera_forward[2]);
+        J.camera.right = make_float3(b.camera_right[0], b.camera_right[1], b.camera_right[2]);
+        J.camera.up = make_float3(b.camera_up[0], b.camera_up[1], b.camera_up[2]);
+        J.camera.tan_half_fov = b.tan_half_fov;
+        J.width = b.width;
+        J.height = b.height;
+        J.spp = b.spp;
+        J.film_offset = film_pixels;
+        const int64_t npix = static_cast<int64_t>(b.width) * b.height;
+        R.b_job_pixels[static_cast<size_t>(j)] = npix;
+        R.b_job_offset[static_cast<size_t>(j)] = film_pixels;
+        film_pixels += npix;
+        largest = std::max(largest, npix);
+    }
+    if (film_pixels > (int64_t{1} << 31) - 1) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "batch film too large");
+    const int64_t capacity = d->paths_per_pass > 0 ? d->paths_per_pass : std::min<int64_t>(int64_t{1} << 23, film_pixels);
+    if (capacity > (int64_t{1} << 30)) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "paths_per_pass too large");
+    // Schedule: each pass takes, in order of remaining work, the next pixel
+    // range of the current sample of as many jobs as fit; a job appears at most
+    // once per pass, so its pixels see their samples in index order.
+    std::vector<DevItem> items;
+    std::vector<int32_t> pass_items{0};
+    std::vector<int32_t> job_last_pass(jobs.size(), 0);
+    std::vector<int32_t> sample(jobs.size(), 0);
+    std::vector<int64_t> next_pixel(jobs.size(), 0);
+    std::vector<int32_t> order(jobs.size());
+    for (size_t j = 0; j < order.size(); ++j) order[j] = static_cast<int32_t>(j);
+    auto remaining = [&](int32_t j) {
+        const size_t k = static_cast<size_t>(j);
+        return R.b_job_pixels[k] * (jobs[k].spp - sample[k]) - next_pixel[k];
+    };
+    while (true) {
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return remaining(a) > remaining(b); });
+        if (remaining(order[0]) <= 0) break;
+        int64_t used = 0;
+        const int32_t pass = static_cast<int32_t>(pass_items.size()) - 1;
+        for (const int32_t j : order) {
+            const size_t k = static_cast<size_t>(j);
+            if (used >= capacity || remaining(j) <= 0) continue;
+            const int64_t take = std::min(capacity - used, R.b_job_pixels[k] - next_pixel[k]);
+            items.push_back(DevItem{j, sample[k], static_cast<int32_t>(next_pixel[k]), static_cast<int32_t>(take), used});
+            used += take;
+            next_pixel[k] += take;
+            if (next_pixel[k] == R.b_job_pixels[k]) {
+                next_pixel[k] = 0;
+                ++sample[k];
+            }
+            job_last_pass[k] = pass;
+        }
+        pass_items.push_back(static_cast<int32_t>(items.size()));
+    }
+    const int32_t passes = static_cast<int32_t>(pass_items.size()) - 1;
+    if (passes_out != nullptr) *passes_out = passes;
+    (void)hipSetDevice(ctx->device);
+    int rc;
+    if ((rc = EnsureCapacity(ctx, R, capacity)) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.b_jobs, jobs.data(), jobs.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.b_items, items.data(), items.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.b_pass_items, pass_items.data(), pass_items.size())) != YRH_OK) return rc;
+    if ((rc = Alloc(ctx, R.b_film, static_cast<size_t>(film_pixels) * sizeof(yrh_film_pixel))) != YRH_OK) return rc;
+    if ((rc = CheckHip(ctx, hipMemset(R.b_film.ptr, 0, R.b_film.bytes), "hipMemset")) != YRH_OK) return rc;
+    if ((rc = CheckHip(ctx, hipMemset(R.stats.ptr, 0, 8 * kStatWords), "hipMemset")) != YRH_OK) return rc;
+    if ((rc = CheckHip(ctx, hipMemset(R.counters.ptr, 0, 4 * kCounterWords), "hipMemset")) != YRH_OK) return rc;
+    yrh_render_desc rd{};
+    rd.sampler = d->sampler;
+    rd.max_depth = d->max_depth;
+    rd.seed = d->seed;
+    rd.radiance_clamp = d->radiance_clamp;
+    rd.width = 1;
+    rd.height = 1;
+    rd.spp = 1;
+    PassView P = MakePassView(R, rd);
+    P.jobs = R.b_jobs.as<DevJob>();
+    P.items = R.b_items.as<DevItem>();
+    P.pass_items = R.b_pass_items.as<int32_t>();
+    P.film = R.b_film.as<yrh_film_pixel>();
+    const SurfaceView S = MakeSurfaceView(R);
+    const LightView L = MakeLightView(R);
+    const int max_depth = std::max(1, d->max_depth);
+    yrg::GuideView GV;
+    if ((rc = GuidePrepareView(ctx, capacity, max_depth, &GV)) != YRH_OK) return rc;
+    // Size the traversal scratch now: nothing may allocate inside a capture.
+    {
+        TraceRequest warm;
+        warm.org = P.s_org;
+        warm.dir = P.s_dir;
+        warm.n_device = P.counters + kCountShadow;
+        warm.capacity = 2 * capacity;
+        trav::TraceArgs wa;
+        int g = 0, b = 0;
+        if ((rc = PrepareTraceArgs(ctx, warm, wa, g, b)) != YRH_OK) return rc;
+    }
+    hipStream_t st = ctx->stream;
+    auto launch_pass = [&]() -> int {
+        int r = LaunchPassSequence(ctx, R, P, S, L, GV, capacity, max_depth);
+        if (r != YRH_OK) return r;
+        hipLaunchKernelGGL(AccumulateBatchKernel, dim3(Grid(capacity)), dim3(kBlock), 0, st, P);
+        hipLaunchKernelGGL(CopyTraceTotalsKernel, dim3(1), dim3(64), 0, st, P.stats);
+        hipLaunchKernelGGL(EndPassKernel, dim3(1), dim3(64), 0, st, P.counters);
+        return CheckHip(ctx, hipGetLastError(), "batch pass launch");
+    };
+    std::vector<hipEvent_t> ev;
+    if (job_ms != nullptr) {
+        ev.resize(static_cast<size_t>(passes) + 1);
+        for (hipEvent_t& e : ev) {
+            if ((rc = CheckHip(ctx, hipEventCreate(&e), "hipEventCreate")) != YRH_OK) return rc;
+        }
+        if ((rc = CheckHip(ctx, hipEventRecord(ev[0], st), "hipEventRecord")) != YRH_OK) return rc;
+    }
+    if (d->use_graph) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if ((rc = CheckHip(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) != YRH_OK) {
+            return rc;
+        }
+        const int lrc = launch_pass();
+        const int crc = CheckHip(ctx, hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
+        if (lrc != YRH_OK || crc != YRH_OK) {
+            if (graph != nullptr) (void)hipGraphDestroy(graph);
+            return lrc != YRH_OK ? lrc : crc;
+        }
+        rc = CheckHip(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        for (int32_t k = 0; rc == YRH_OK && k < passes; ++k) {
+            rc = CheckHip(ctx, hipGraphLaunch(exec, st), "hipGraphLaunch");
+            if (rc == YRH_OK && job_ms != nullptr) rc = CheckHip(ctx, hipEventRecord(ev[static_cast<size_t>(k) + 1], st), "hipEventRecord");
+        }
+        const int src = CheckHip(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
+        if (exec != nullptr) (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        if (rc == YRH_OK) rc = src;
+    } else {
+        rc = YRH_OK;
+        for (int32_t k = 0; rc == YRH_OK && k < passes; ++k) {
+            rc = launch_pass();
+            if (rc == YRH_OK && job_ms != nullptr) rc = CheckHip(ctx, hipEventRecord(ev[static_cast<size_t>(k) + 1], st), "hipEventRecord");
+        }
+        const int src = CheckHip(ctx, hipStreamSynchronize(st), "hipStreamSynchronize");
+        if (rc == YRH_OK) rc = src;
+    }
+    if (job_ms != nullptr) {
+        for (int32_t j = 0; rc == YRH_OK && j < d->job_count; ++j) {
+            float ms = 0.0f;
+            rc = CheckHip(ctx, hipEventElapsedTime(&ms, ev[0], ev[static_cast<size_t>(job_last_pass[static_cast<size_t>(j)]) + 1]),
+                          "hipEventElapsedTime");
+            job_ms[j] = ms;
+        }
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    return rc;
+}
+
+int yrh_batch_film_read(yrh_context* ctx, int32_t job, yrh_film_pixel* pixels, int64_t count) {
+    if (ctx == nullptr || pixels == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null argument");
+    if (ctx->render == nullptr || ctx->render->b_film.ptr == nullptr) return Fail(ctx, YRH_E_NO_SCENE, "no batch rendered");
+    RenderState& R = *ctx->render;
+    if (job < 0 || job >= static_cast<int32_t>(R.b_job_pixels.size())) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "bad job");
+    if (count != R.b_job_pixels[static_cast<size_t>(job)]) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "pixel count does not match the job");
+    return yrh_copy_to_host(ctx, pixels, R.b_film.as<yrh_film_pixel>() + R.b_job_offset[static_cast<size_t>(job)],
+                            static_cast<size_t>(count) * sizeof(yrh_film_pixel));
 }
 
 int yrh_sample_values(yrh_context* ctx, int32_t kind, uint64_t seed, int32_t spp, int64_t n, const int32_t* x,

@@ -133,6 +133,18 @@ class SceneInfo(C.Structure):
                 ("camera", C.c_float * 13)]
 
 
+class BatchJob(C.Structure):
+    _fields_ = [("camera_origin", C.c_float * 3), ("camera_forward", C.c_float * 3), ("camera_right", C.c_float * 3),
+                ("camera_up", C.c_float * 3), ("tan_half_fov", C.c_float), ("width", C.c_int32),
+                ("height", C.c_int32), ("spp", C.c_int32)]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [("jobs", C.c_void_p), ("job_count", C.c_int32), ("sampler", C.c_int32), ("max_depth", C.c_int32),
+                ("radiance_clamp", C.c_float), ("seed", C.c_uint64), ("paths_per_pass", C.c_int64),
+                ("use_graph", C.c_int32), ("reserved", C.c_int32)]
+
+
 class GuideDesc(C.Structure):
     _fields_ = [("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3), ("log2_table_size", C.c_int32),
                 ("min_resolution", C.c_int32), ("max_resolution", C.c_int32), ("learning_rate", C.c_float),
@@ -211,6 +223,8 @@ YRH_SIGNATURES = {
     "yrh_render_stats_read": (_I, [_VP, C.POINTER(RenderStats)]),
     "yrh_sample_values": (_I, [_VP, _I32, _U64, _I32, _I64, _VP, _VP, _VP, _VP, _I32, _VP]),
     "yrh_generate_camera_rays": (_I, [_VP, C.POINTER(RenderDesc), _VP, _VP]),
+    "yrh_batch_render": (_I, [_VP, C.POINTER(BatchDesc), _VP, C.POINTER(_I32)]),
+    "yrh_batch_film_read": (_I, [_VP, _I32, _VP, _I64]),
     "yrh_guide_configure": (_I, [_VP, C.POINTER(GuideDesc)]),
     "yrh_guide_set_mode": (_I, [_VP, _I32]),
     "yrh_guide_train": (_I, [_VP, _I32, _I32]),
