@@ -15,6 +15,11 @@ Cornell box (256x256, 16 spp, depth 8) and the two-level instanced scene
 (1920x1080, 64 spp, depth 16); one step = one whole frame, timed with events
 on the context stream, beside the reference's RenderCpuPathTrace on the host
 cores (the whole Cornell frame; a 480x270 x 8 spp sample of the two-level one).
+
+Then (`batch`) 32 cameras of uneven size and spp on the two-level scene in
+one pass schedule with HIP-graph replay, wavefront size autotuned; and
+(`guided_vs_plain`) the equal-time relMSE of the guided and the plain
+integrator on the window-lit room, guided time including its training.
 """
 import argparse
 import json
@@ -95,6 +100,16 @@ def main():
     ap.add_argument("--renders", default="cornell,two_level",
                     help="render workloads timed after the ray-cast microbench (comma list, empty for none)")
     ap.add_argument("--render-frames", type=int, default=3)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-camera workload")
+    ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain comparison")
+    ap.add_argument("--guide-width", type=int, default=1280)
+    ap.add_argument("--guide-height", type=int, default=720)
+    ap.add_argument("--guide-ref-spp", type=int, default=1024)
+    ap.add_argument("--guide-spp", default="4,8,16,32,64")
+    ap.add_argument("--guide-train-passes", type=int, default=8)
+    ap.add_argument("--guide-train-steps", type=int, default=16)
+    ap.add_argument("--guide-train-batch", type=int, default=16384)
+    ap.add_argument("--guide-threshold", type=float, default=0.05)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
@@ -192,6 +207,12 @@ def main():
     for key in [k for k in args.renders.split(",") if k]:
         renders[key] = bench_render(dev, key, args.render_frames, min(16, len(os.sched_getaffinity(0))),
                                     not args.no_cpu_baseline)
+    batch = bench_batch(dev) if not args.no_batch else None
+    guiding = None
+    if not args.no_guiding:
+        guiding = bench_guiding(dev, args.guide_width, args.guide_height, args.guide_ref_spp,
+                                [int(v) for v in args.guide_spp.split(",")], args.guide_train_passes,
+                                args.guide_train_steps, args.guide_train_batch, args.guide_threshold)
 
     traffic = load_traffic("closest")
     result = {
@@ -246,7 +267,8 @@ def main():
         "closest_kernel_mrays": round(n / avg_closest_s / 1e6, 2),
         "parity_sample": parity,
         "renders": renders,
-        "guided_vs_plain": None,
+        "batch": batch,
+        "guided_vs_plain": guiding,
     }
     print(json.dumps(result), flush=True)
 
@@ -260,6 +282,172 @@ RENDER_WORKLOADS = {
                       width=1920, height=1080, spp=64, max_depth=16, batch=4,
                       cpu=dict(width=480, height=270, spp=8)),
 }
+
+
+def batch_jobs(scene, count=32, seed=5):
+    """Cameras of the batched workload: pixel counts log-uniform in [128^2, 1920x1080]
+    at 16:9 (clamped), spp log-uniform over powers of two in {4..256}, eyes on a ring
+    25..50 units out looking at points near the centre, fov 30..60 degrees."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(count):
+        n = float(np.exp(rng.uniform(np.log(128 * 128), np.log(1920 * 1080))))
+        w = int(np.clip(round(np.sqrt(n * 16.0 / 9.0)), 128, 1920))
+        h = int(np.clip(round(w * 9.0 / 16.0), 72, 1080))
+        spp = int(2 ** rng.integers(2, 9))
+        az, dist, el = rng.uniform(0, 2 * np.pi), rng.uniform(25, 50), rng.uniform(3, 15)
+        eye = np.array([dist * np.sin(az), el, dist * np.cos(az)])
+        target = np.array([rng.uniform(-5, 5), 1.0, rng.uniform(-5, 5)])
+        fov = rng.uniform(30, 60)
+        f = (target - eye) / np.linalg.norm(target - eye)
+        r = np.cross(f, [0.0, 1.0, 0.0])
+        r /= np.linalg.norm(r)
+        u = np.cross(r, f)
+        scene.set_camera(eye, f, r, u, np.radians(fov))
+        jobs.append(scene.render_desc(w, h, spp, 16))
+    return jobs
+
+
+def bench_batch(dev):
+    """32 cameras on the two-level scene in one pass schedule with HIP-graph replay;
+    wavefront size and launch shape from the autotuner."""
+    from yaoray_amd import Scene, autotune
+
+    scene = Scene.instanced()
+    scene.build_accel()
+    jobs = batch_jobs(scene)
+    dev.upload_scene(scene)
+    dev.upload_shading(scene)
+    t = time.perf_counter()
+    best, trials = autotune.tune(dev, jobs, "two_level_32_cameras", max_depth=16, probe_spp=1)
+    tune_s = time.perf_counter() - t
+    log(f"batch: autotune {tune_s:.1f}s -> {best}")
+    films, ms, passes, st = dev.render_batch(jobs, max_depth=16, paths_per_pass=best["paths_per_pass"],
+                                             use_graph=True)
+    total_ms = float(ms.max())
+    samples = int(sum(j.width * j.height * j.spp for j in jobs))
+    out = {
+        "workload": "32 cameras on the two-level scene (pixels log-uniform 128^2..1920x1080 at 16:9, spp 4..256, "
+                    "seed 5), depth 16, one pass schedule, HIP graph replayed per pass",
+        "value": round(st["rays_traced"] / (total_ms / 1e3) / 1e6, 2),
+        "unit": "Mrays/s (path segments, rays_traced)",
+        "shadow_Mrays_s": round(st["shadow_rays"] / (total_ms / 1e3) / 1e6, 2),
+        "ms_total": round(total_ms, 2),
+        "samples": samples,
+        "samples_per_s": round(samples / (total_ms / 1e3), 1),
+        "passes": passes,
+        "paths_per_pass": best["paths_per_pass"],
+        "blocks_per_cu": best["blocks_per_cu"],
+        "per_camera": [{"width": j.width, "height": j.height, "spp": j.spp, "done_ms": round(float(m), 2)}
+                       for j, m in zip(jobs, ms)],
+        "autotune_trials": trials,
+    }
+    log(f"batch: {total_ms:.0f} ms, {out['value']} Mrays/s over {passes} passes")
+    return out
+
+
+def relmse(film, ref):
+    """mean over pixels and channels of (x - r)^2 / (r^2 + 0.01)."""
+    x = film["sum"] / np.maximum(film["samples"], 1)[:, None]
+    r = ref["sum"] / np.maximum(ref["samples"], 1)[:, None]
+    return float(np.mean((x - r) ** 2 / (r ** 2 + 0.01)))
+
+
+def interp_loglog(xs, ys, x):
+    """log-log interpolation (linear extrapolation past the ends) of y(x)."""
+    lx, ly = np.log(np.asarray(xs, float)), np.log(np.asarray(ys, float))
+    if x <= xs[0]:
+        k = 0
+    elif x >= xs[-1]:
+        k = len(xs) - 2
+    else:
+        k = int(np.searchsorted(xs, x)) - 1
+    t = (np.log(x) - lx[k]) / (lx[k + 1] - lx[k])
+    return float(np.exp(ly[k] + t * (ly[k + 1] - ly[k])))
+
+
+def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_steps, train_batch, threshold):
+    """Equal-time relMSE of the guided and the plain wavefront integrator on the room
+    workload, guided time including its training, against a plain FP64 reference."""
+    from yaoray_amd import Scene
+    from yaoray_amd import native as N
+
+    scene = Scene.room()
+    scene.build_accel()
+    dev.upload_scene(scene)
+    dev.upload_shading(scene)
+    depth = 8
+
+    def timed(spp, seed, batch):
+        dev.synchronize()
+        t0 = time.perf_counter()
+        f, _ = dev.render(scene, width, height, spp, max_depth=depth, upload=False, batch=batch, seed=seed)
+        dev.synchronize()
+        return f, time.perf_counter() - t0
+
+    npix = width * height
+    big = max(1, min(64, (1 << 23) // npix))
+    ref, ref_s = timed(ref_spp, 0x5EED, big)
+    log(f"guiding: reference {ref_spp} spp in {ref_s:.1f}s")
+    plain = []
+    for spp in spp_list:
+        f, t = timed(spp, 0x59414F524159, max(1, min(spp, big)))
+        plain.append({"spp": spp, "seconds": round(t, 4), "relmse": relmse(f, ref)})
+    dev.guide_configure([-0.1, -0.1, -0.1], [10.1, 4.1, 10.1], log2_table_size=18, min_resolution=8,
+                        max_resolution=512, learning_rate=1e-2, guide_fraction=0.5, sample_capacity=1 << 21, seed=7)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    dev.guide_set_mode(N.GUIDE_RECORD)
+    for k in range(train_passes):
+        dev.render(scene, width, height, 1, max_depth=depth, upload=False, seed=7000 + k)
+        dev.guide_train(train_steps, train_batch)
+    dev.synchronize()
+    train_s = time.perf_counter() - t0
+    dev.guide_set_mode(N.GUIDE_SAMPLE)
+    guided = []
+    try:
+        for spp in spp_list:
+            f, t = timed(spp, 0x59414F524159, max(1, min(spp, big)))
+            guided.append({"spp": spp, "seconds": round(t + train_s, 4), "render_seconds": round(t, 4),
+                           "relmse": relmse(f, ref)})
+    finally:
+        dev.guide_set_mode(N.GUIDE_OFF)
+    px = [p["seconds"] for p in plain]
+    py = [p["relmse"] for p in plain]
+    for g in guided:
+        g["plain_relmse_at_equal_time"] = interp_loglog(px, py, g["seconds"])
+        g["improvement"] = round(g["plain_relmse_at_equal_time"] / g["relmse"], 3)
+
+    def time_to(points, thr):
+        xs = [p["seconds"] for p in points]
+        ys = [p["relmse"] for p in points]
+        lo, hi = min(xs) / 8, max(xs) * 8
+        for _ in range(60):  # bisection on the monotone interpolant
+            mid = np.sqrt(lo * hi)
+            if interp_loglog(xs, ys, mid) > thr:
+                lo = mid
+            else:
+                hi = mid
+        return round(hi, 4)
+
+    out = {
+        "workload": f"window-lit room with a hidden ceiling lamp (200k furniture tris, diffuse + GGX), "
+                    f"{width}x{height}, depth 8, OwenSobol",
+        "reference": f"plain, {ref_spp} spp, FP64 film, {ref_s:.1f}s",
+        "relmse": "mean over pixels and channels of (x - r)^2 / (r^2 + 0.01)",
+        "training": {"passes": train_passes, "steps_per_pass": train_steps, "batch": train_batch,
+                     "seconds": round(train_s, 4)},
+        "plain": plain,
+        "guided": guided,
+        "equal_time_improvement": guided[-1]["improvement"],
+        "threshold": threshold,
+        "time_to_threshold_plain_s": time_to(plain, threshold),
+        "time_to_threshold_guided_s": time_to(guided, threshold),
+    }
+    out["time_to_threshold_speedup"] = round(out["time_to_threshold_plain_s"] / out["time_to_threshold_guided_s"], 3)
+    log(f"guiding: equal-time relMSE improvement {out['equal_time_improvement']}x; time to relMSE <= {threshold}: "
+        f"plain {out['time_to_threshold_plain_s']}s, guided {out['time_to_threshold_guided_s']}s")
+    return out
 
 
 def bench_render(dev, key, frames, cpu_threads, with_cpu):

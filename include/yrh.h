@@ -421,6 +421,10 @@ int yrh_guide_train(yrh_context* ctx, int32_t steps, int32_t batch);
  * direction, weight > 0) to the recorded set, as if a render pass had
  * recorded them (samples beyond the capacity are dropped). */
 int yrh_guide_add_samples(yrh_context* ctx, int64_t n, const float* samples);
+/* Copies up to n recorded samples to the host as rows of 13 floats (position,
+ * normal, outgoing, sampled direction, weight); returns the number copied in
+ * *copied (synchronises). */
+int yrh_guide_read_samples(yrh_context* ctx, int64_t n, float* samples, int64_t* copied);
 /* Recorded training samples and their mean weight (synchronises). */
 int yrh_guide_sample_stats(yrh_context* ctx, int64_t* count, float* mean_weight);
 /* Sizes: MLP parameters (FP32 [in][out] matrices W0..W3), grid parameters

@@ -531,6 +531,12 @@ class Device:
         smp = np.ascontiguousarray(samples, np.float32).reshape(-1, 13)
         self._check(self._lib.yrh_guide_add_samples(self._ctx, len(smp), smp.ctypes.data), "guide_add_samples")
 
+    def guide_read_samples(self, n: int) -> np.ndarray:
+        out = np.zeros((n, 13), np.float32)
+        got = C.c_int64()
+        self._check(self._lib.yrh_guide_read_samples(self._ctx, n, out.ctypes.data, C.byref(got)), "guide_read_samples")
+        return out[:got.value]
+
     def guide_sample_stats(self):
         n, w = C.c_int64(), C.c_float()
         self._check(self._lib.yrh_guide_sample_stats(self._ctx, C.byref(n), C.byref(w)), "guide_sample_stats")

@@ -688,6 +688,28 @@ int yrh_guide_add_samples(yrh_context* ctx, int64_t n, const float* samples) {
     return CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 
+int yrh_guide_read_samples(yrh_context* ctx, int64_t n, float* samples, int64_t* copied) {
+    int rc = NeedGuide(ctx);
+    if (rc != YRH_OK) return rc;
+    if (n < 0 || (n > 0 && samples == nullptr)) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "bad samples");
+    int64_t count = 0;
+    if ((rc = yrh_guide_sample_stats(ctx, &count, nullptr)) != YRH_OK) return rc;
+    const int64_t m = std::min(n, count);
+    std::vector<float> raw(static_cast<size_t>(m) * 16);
+    if (m > 0 && (rc = yrh_copy_to_host(ctx, raw.data(), ctx->guide->samples.ptr, raw.size() * 4)) != YRH_OK) return rc;
+    for (int64_t r = 0; r < m; ++r) {
+        const float* s = raw.data() + 16 * r;
+        float* d = samples + 13 * r;
+        d[0] = s[0]; d[1] = s[1]; d[2] = s[2];
+        d[3] = s[4]; d[4] = s[5]; d[5] = s[6];
+        d[6] = s[8]; d[7] = s[9]; d[8] = s[10];
+        d[9] = s[12]; d[10] = s[13]; d[11] = s[14];
+        d[12] = s[3];
+    }
+    if (copied != nullptr) *copied = m;
+    return YRH_OK;
+}
+
 int yrh_guide_sample_stats(yrh_context* ctx, int64_t* count, float* mean_weight) {
     int rc = NeedGuide(ctx);
     if (rc != YRH_OK) return rc;

@@ -229,6 +229,7 @@ YRH_SIGNATURES = {
     "yrh_guide_set_mode": (_I, [_VP, _I32]),
     "yrh_guide_train": (_I, [_VP, _I32, _I32]),
     "yrh_guide_add_samples": (_I, [_VP, _I64, _VP]),
+    "yrh_guide_read_samples": (_I, [_VP, _I64, _VP, C.POINTER(_I64)]),
     "yrh_guide_sample_stats": (_I, [_VP, C.POINTER(_I64), C.POINTER(_F)]),
     "yrh_guide_info": (_I, [_VP, C.POINTER(_I32), C.POINTER(_I64), _VP, _VP]),
     "yrh_guide_params_read": (_I, [_VP, _VP, _VP]),
