@@ -105,12 +105,14 @@ def main():
     ap.add_argument("--guide-width", type=int, default=1280)
     ap.add_argument("--guide-height", type=int, default=720)
     ap.add_argument("--guide-ref-spp", type=int, default=1024)
-    ap.add_argument("--guide-spp", default="4,8,16,32,64")
-    ap.add_argument("--guide-train-passes", type=int, default=8)
-    ap.add_argument("--guide-train-steps", type=int, default=16)
-    ap.add_argument("--guide-train-batch", type=int, default=16384)
-    ap.add_argument("--guide-threshold", type=float, default=0.05)
+    ap.add_argument("--guide-spp", default="16,32,64,128,256")
+    ap.add_argument("--guide-train-passes", type=int, default=16)
+    ap.add_argument("--guide-train-steps", type=int, default=64)
+    ap.add_argument("--guide-train-batch", type=int, default=32768)
+    ap.add_argument("--guide-lr", type=float, default=3e-3)
+    ap.add_argument("--guide-threshold", type=float, default=0.2)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--lds-levels", type=int, default=0, help="traversal stack levels kept in LDS (1..16)")
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="also run a device copy of this size (HBM counter calibration under rocprofv3)")
@@ -141,6 +143,8 @@ def main():
     name, cus = dev.info()
     if args.blocks_per_cu or args.block_size:
         dev.set_trace_config(args.blocks_per_cu, args.block_size)
+    if args.lds_levels:
+        dev.set_trace_lds_levels(args.lds_levels)
     dev.upload_scene(scene)
     d_org, d_dir = dev.to_device(org), dev.to_device(dirs)
     d_hit, d_inst = dev.alloc(16 * n), dev.alloc(4 * n)
@@ -212,7 +216,7 @@ def main():
     if not args.no_guiding:
         guiding = bench_guiding(dev, args.guide_width, args.guide_height, args.guide_ref_spp,
                                 [int(v) for v in args.guide_spp.split(",")], args.guide_train_passes,
-                                args.guide_train_steps, args.guide_train_batch, args.guide_threshold)
+                                args.guide_train_steps, args.guide_train_batch, args.guide_lr, args.guide_threshold)
 
     traffic = load_traffic("closest")
     result = {
@@ -366,7 +370,7 @@ def interp_loglog(xs, ys, x):
     return float(np.exp(ly[k] + t * (ly[k + 1] - ly[k])))
 
 
-def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_steps, train_batch, threshold):
+def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_steps, train_batch, lr, threshold):
     """Equal-time relMSE of the guided and the plain wavefront integrator on the room
     workload, guided time including its training, against a plain FP64 reference."""
     from yaoray_amd import Scene
@@ -394,7 +398,7 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
         f, t = timed(spp, 0x59414F524159, max(1, min(spp, big)))
         plain.append({"spp": spp, "seconds": round(t, 4), "relmse": relmse(f, ref)})
     dev.guide_configure([-0.1, -0.1, -0.1], [10.1, 4.1, 10.1], log2_table_size=18, min_resolution=8,
-                        max_resolution=512, learning_rate=1e-2, guide_fraction=0.5, sample_capacity=1 << 21, seed=7)
+                        max_resolution=512, learning_rate=lr, guide_fraction=0.5, sample_capacity=1 << 21, seed=7)
     dev.synchronize()
     t0 = time.perf_counter()
     dev.guide_set_mode(N.GUIDE_RECORD)
@@ -419,16 +423,15 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
         g["improvement"] = round(g["plain_relmse_at_equal_time"] / g["relmse"], 3)
 
     def time_to(points, thr):
+        """Time at which the measured curve reaches thr (log-log interpolation
+        between measured points only; None when thr is outside them)."""
         xs = [p["seconds"] for p in points]
         ys = [p["relmse"] for p in points]
-        lo, hi = min(xs) / 8, max(xs) * 8
-        for _ in range(60):  # bisection on the monotone interpolant
-            mid = np.sqrt(lo * hi)
-            if interp_loglog(xs, ys, mid) > thr:
-                lo = mid
-            else:
-                hi = mid
-        return round(hi, 4)
+        for k in range(len(xs) - 1):
+            if ys[k] >= thr >= ys[k + 1]:
+                t = (np.log(thr) - np.log(ys[k])) / (np.log(ys[k + 1]) - np.log(ys[k]))
+                return round(float(np.exp(np.log(xs[k]) + t * (np.log(xs[k + 1]) - np.log(xs[k])))), 4)
+        return None
 
     out = {
         "workload": f"window-lit room with a hidden ceiling lamp (200k furniture tris, diffuse + GGX), "
@@ -436,7 +439,7 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
         "reference": f"plain, {ref_spp} spp, FP64 film, {ref_s:.1f}s",
         "relmse": "mean over pixels and channels of (x - r)^2 / (r^2 + 0.01)",
         "training": {"passes": train_passes, "steps_per_pass": train_steps, "batch": train_batch,
-                     "seconds": round(train_s, 4)},
+                     "learning_rate": lr, "seconds": round(train_s, 4)},
         "plain": plain,
         "guided": guided,
         "equal_time_improvement": guided[-1]["improvement"],
@@ -444,7 +447,8 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
         "time_to_threshold_plain_s": time_to(plain, threshold),
         "time_to_threshold_guided_s": time_to(guided, threshold),
     }
-    out["time_to_threshold_speedup"] = round(out["time_to_threshold_plain_s"] / out["time_to_threshold_guided_s"], 3)
+    tp, tg = out["time_to_threshold_plain_s"], out["time_to_threshold_guided_s"]
+    out["time_to_threshold_speedup"] = round(tp / tg, 3) if tp and tg else None
     log(f"guiding: equal-time relMSE improvement {out['equal_time_improvement']}x; time to relMSE <= {threshold}: "
         f"plain {out['time_to_threshold_plain_s']}s, guided {out['time_to_threshold_guided_s']}s")
     return out

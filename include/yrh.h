@@ -209,6 +209,10 @@ int yrh_trace_any(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n,
 /* Launch geometry of the persistent traversal kernels: workgroups per compute
  * unit and threads per workgroup (64, 128 or 256); 0 keeps the current value. */
 int yrh_set_trace_config(yrh_context* ctx, int blocks_per_cu, int block_size);
+/* Traversal stack levels kept in LDS per lane (1..16, default 8); deeper
+ * levels spill to a per-lane global region. Fewer levels use less LDS per
+ * workgroup, so more workgroups fit on a CU. */
+int yrh_set_trace_lds_levels(yrh_context* ctx, int levels);
 
 /* ---- Shading data ----------------------------------------------------------- */
 
@@ -389,9 +393,9 @@ typedef struct yrh_batch_desc {
 } yrh_batch_desc;
 
 /* Renders every job to its full spp in one pass schedule: each pass carries
- * the next pixel range of the current sample of as many jobs as fit (a job at
- * most once per pass), so each job's FP64 film equals rendering that job
- * alone with yrh_render_pass. Synchronises. job_ms (host, job_count entries,
+ * consecutive pixel ranges / sample layers of the jobs, in order of remaining
+ * work, until it is full; a pixel receives its samples in index order, so each
+ * job's FP64 film equals rendering that job alone with yrh_render_pass. Synchronises. job_ms (host, job_count entries,
  * or NULL) receives each job's completion time since the start in ms;
  * passes_out (or NULL) the number of passes. Counters: yrh_render_stats_read. */
 int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* desc, float* job_ms, int32_t* passes_out);

@@ -1,7 +1,8 @@
 """Batched multi-camera rendering (one pass schedule for many cameras, one
 HIP-graph-captured pass replayed per pass): every job's FP64 film equals the
 film of rendering that job alone, bit for bit, with and without the graph,
-for any wavefront size; counters add up; completion times are ordered."""
+for any wavefront size (passes may hold several sample layers of one job);
+counters add up; every pass but the last is full."""
 import numpy as np
 import pytest
 
@@ -67,11 +68,10 @@ def test_batch_films_equal_single_renders(device, kind):
                 for field in r.dtype.names:
                     assert np.array_equal(f[field], r[field]), (graph, ppp, k, field)
             assert st == totals, (graph, ppp)
-            assert passes >= max(j.spp for j in jobs)
+            total = sum(j.width * j.height * j.spp for j in jobs)
+            if ppp:
+                assert passes == -(-total // ppp)  # every pass but the last is full
             assert np.all(ms > 0)
-    # Smaller wavefronts need more passes; the job with most work finishes last.
-    _, ms, passes_small, _ = device.render_batch(jobs, max_depth=depth, paths_per_pass=500)
-    _, _, passes_big, _ = device.render_batch(jobs, max_depth=depth, paths_per_pass=100000)
-    assert passes_small > passes_big == max(j.spp for j in jobs)
-    work = [j.width * j.height * j.spp for j in jobs]
-    assert ms[int(np.argmax(work))] == ms.max()
+    # One pass holds several sample layers of a job when they fit.
+    _, ms, passes_big, _ = device.render_batch(jobs, max_depth=depth, paths_per_pass=100000)
+    assert passes_big == 1 and np.allclose(ms, ms[0])

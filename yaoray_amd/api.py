@@ -396,6 +396,9 @@ class Device:
     def upload_geometry(self, desc: N.GeometryDesc):
         self._check(self._lib.yrh_upload_geometry(self._ctx, C.byref(desc)), "upload_geometry")
 
+    def set_trace_lds_levels(self, levels: int):
+        self._check(self._lib.yrh_set_trace_lds_levels(self._ctx, levels), "set_trace_lds_levels")
+
     def set_trace_config(self, blocks_per_cu=0, block_size=0):
         self._check(self._lib.yrh_set_trace_config(self._ctx, blocks_per_cu, block_size), "set_trace_config")
 

@@ -16,7 +16,7 @@ import time
 from . import native as N
 
 DEFAULT_PATHS = (1 << 20, 1 << 21, 1 << 22, 1 << 23)
-DEFAULT_BLOCKS_PER_CU = (2, 3, 4, 5)
+DEFAULT_BLOCKS_PER_CU = (4, 6, 8)
 
 
 def cache_path() -> str:
@@ -47,13 +47,13 @@ def tune(device, jobs, key, max_depth=8, sampler=N.SAMPLER_OWEN_SOBOL, probe_spp
         C.memmove(C.byref(d), C.byref(j), C.sizeof(N.RenderDesc))
         d.spp = min(j.spp, probe_spp)
         probe.append(d)
-    total_pixels = sum(j.width * j.height for j in jobs)
+    total = sum(j.width * j.height * j.spp for j in probe)
     trials = []
     for bpc in blocks_candidates:
         device.set_trace_config(bpc, 0)
         for ppp in paths_candidates:
-            if ppp > max(paths_candidates[0], 2 * total_pixels):
-                continue  # larger wavefronts than two layers of every job gain nothing
+            if ppp > max(paths_candidates[0], total):
+                continue  # a wavefront larger than the whole probe gains nothing
             best_t = None
             for _ in range(max(1, repeats)):
                 device.synchronize()

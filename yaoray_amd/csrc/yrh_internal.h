@@ -133,8 +133,12 @@ struct yrh_context {
     yrh::DevBuf work_counter;
     int launch_parity = 0;
     yrh::DevBuf stack_spill;    // int2 per lane per spilled stack level
-    int blocks_per_cu = 4;
+    // Launch shape of the persistent traversal kernels: 8 LDS stack levels per
+    // lane (16 KB per 256-thread workgroup) let 8 workgroups share a CU, the
+    // fastest shape measured on MI355X for the ray-cast microbench.
+    int blocks_per_cu = 8;
     int block_size = 256;
+    int lds_levels = 8;  // traversal stack levels kept in LDS per lane
     yrh::RenderState* render = nullptr;  // surface/light data, film, path state (yrh_render.hip)
     yrh::GuideState* guide = nullptr;    // learned guiding proposal (yrh_guide.hip)
 };

@@ -196,6 +196,8 @@ struct DevJob {
 
 struct DevItem {
     int32_t job, sample, pixel_begin, pixel_count;
+    int32_t occurrence;  // earlier items of the same job in this pass (their pixels may repeat)
+    int32_t pad;
     int64_t path_begin;  // first path of the item within its pass
 };
 
@@ -568,7 +570,7 @@ __global__ void RaygenKernel(PassView P) {
         height = J.height;
         camera = J.camera;
         pixel = J.film_offset + local;
-        P.p_job[p] = it.job;
+        P.p_job[p] = it.job | (it.occurrence << 16);
         P.p_sample[p] = it.sample;
     } else {
         const int64_t total = P.npix * P.sample_count;
@@ -741,7 +743,7 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
     const int pix = P.p_pixel[p];
     PixelSampler smp;
     if (P.jobs != nullptr) {
-        const DevJob& J = P.jobs[P.p_job[p]];
+        const DevJob& J = P.jobs[P.p_job[p] & 0xFFFF];
         const int64_t local = pix - J.film_offset;
         smp.Init(P.sampler, P.seed, static_cast<int>(local % J.width), static_cast<int>(local / J.width), P.p_sample[p],
                  J.spp);
@@ -878,7 +880,11 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
         tp = Hadamard(tp, bs.weight);
         if (depth >= 3) {
             const float survival = Clampf(MaxC(tp), 0.05f, 0.95f);
-            if (smp.Get1D(kDimRoulette) >= survival) {
+            // A guided throughput depends on the sampled direction, so the
+            // roulette decision must not come from a Sobol dimension that
+            // correlates with that direction's 2D sample (see Independent1D).
+            const float xi = guided ? smp.Independent1D(kDimRoulette) : smp.Get1D(kDimRoulette);
+            if (xi >= survival) {
                 cont = false;
             } else {
                 tp = Div(tp, survival);
@@ -964,9 +970,11 @@ __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S,
 __global__ __launch_bounds__(kBlock) void GuideQueryKernel(PassView P, SurfaceView S, LightView L, yrg::GuideView G,
                                                           int cur) {
     __shared__ float lds_in[4][64][10];
-    __shared__ float lds_o[4][64 * yrg::kGuideOut];
+    __shared__ float lds_o[4][16 * yrg::kGuideOut];
+    __shared__ yrg::bf16x4_t lds_w[yrg::kFragCount * 64];  // weight fragments, shared by the block
     const int n = static_cast<int>(P.counters[cur]);
     if (static_cast<int64_t>(blockIdx.x) * blockDim.x >= n) return;  // whole block idle
+    for (int k = static_cast<int>(threadIdx.x); k < yrg::kFragCount * 64; k += kBlock) lds_w[k] = G.frag[k];
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int wave = static_cast<int>(threadIdx.x >> 6);
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -991,36 +999,34 @@ __global__ __launch_bounds__(kBlock) void GuideQueryKernel(PassView P, SurfaceVi
     in[9] = guidable ? 1.0f : 0.0f;
     __syncthreads();
     const int q = lane / 16;
-    yrg::bf16x4_t x[4][2];
+    float* lo = lds_o[wave];
+    // One block of 16 rows at a time keeps registers and LDS small; the lanes
+    // of that block's rows then read their own row's 48 outputs.
+#pragma unroll 1
     for (int rb = 0; rb < 4; ++rb) {
         const float* r = lds_in[wave][rb * 16 + lane % 16];
+        yrg::bf16x4_t x[1][2];
         float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         if (r[9] != 0.0f) {
             const float pp[3] = {r[0], r[1], r[2]};
             const float nn[3] = {r[3], r[4], r[5]};
             const float ww[3] = {r[6], r[7], r[8]};
             yrg::EncodeShare(G.grid, G.table, pp, q, f);
-            x[rb][1] = yrg::DirectionShare(q, nn, ww);
+            x[0][1] = yrg::DirectionShare(q, nn, ww);
         } else {
-            x[rb][1] = yrg::PackBf16(0.0f, 0.0f, 0.0f, 0.0f);
+            x[0][1] = yrg::PackBf16(0.0f, 0.0f, 0.0f, 0.0f);
         }
-        x[rb][0] = yrg::PackBf16(f[0], f[1], f[2], f[3]);
-    }
-    yrg::bf16x4_t h0[4][4], h1[4][4], h2[4][4];
-    yrg::f32x4_t o[4][3];
-    yrg::MlpForward<4>(G.frag, lane, x, h0, h1, h2, o);
-    float* lo = lds_o[wave];
-    for (int rb = 0; rb < 4; ++rb) {
+        x[0][0] = yrg::PackBf16(f[0], f[1], f[2], f[3]);
+        yrg::bf16x4_t h0[1][4], h1[1][4], h2[1][4];
+        yrg::f32x4_t o[1][3];
+        yrg::MlpForward<1>(lds_w, lane, x, h0, h1, h2, o);
         for (int mb = 0; mb < 3; ++mb) {
-            for (int j = 0; j < 4; ++j) lo[(rb * 16 + lane % 16) * yrg::kGuideOut + mb * 16 + 4 * q + j] = o[rb][mb][j];
+            for (int j = 0; j < 4; ++j) lo[(lane % 16) * yrg::kGuideOut + mb * 16 + 4 * q + j] = o[0][mb][j];
         }
-    }
-    __syncthreads();
-    if (i < n) {
-        G.valid[i] = guidable ? 1 : 0;
-        if (guidable) {
+        __syncthreads();
+        if (lane / 16 == rb && i < n && guidable) {
             yrg::Mixture m;
-            yrg::HeadFromOutputs(lo + lane * yrg::kGuideOut, m);
+            yrg::HeadFromOutputs(lo + (lane % 16) * yrg::kGuideOut, m);
             float* dst = G.mix + static_cast<int64_t>(yrg::kGuideMixture) * i;
             for (int k = 0; k < yrg::kGuideLobes; ++k) {
                 dst[k] = m.pi[k];
@@ -1030,7 +1036,9 @@ __global__ __launch_bounds__(kBlock) void GuideQueryKernel(PassView P, SurfaceVi
                 dst[16 + 3 * k + 2] = m.mu[k][2];
             }
         }
+        __syncthreads();
     }
+    if (i < n) G.valid[i] = guidable ? 1 : 0;
 }
 
 // After the last bounce: each record of this pass becomes a training sample
@@ -1208,7 +1216,7 @@ struct ShadowWalk {
 template <bool kTwoLevel>
 __global__ __launch_bounds__(trav::kMaxBlock) void ShadowWalkKernel(trav::TraceArgs a, PassView P, SurfaceView S,
                                                                     LightView L) {
-    __shared__ int2 lds_stack[trav::kLdsStack * trav::kMaxBlock];
+    extern __shared__ int2 lds_stack[];
     const ShadowWalk pol{&P, &S, &L};
     if (kTwoLevel) {
         trav::TraceTwoLevelLoop<false>(a, pol, lds_stack);
@@ -1269,13 +1277,17 @@ __global__ __launch_bounds__(kBlock) void AccumulateKernel(PassView P) {
     WaveStat(P.stats + kStatBad, bad);
 }
 
-// Batched pass: one thread per path; a pass holds each (job, pixel) at most
-// once, so the film updates do not collide and every pixel receives its
-// samples in index order over the passes.
-__global__ __launch_bounds__(kBlock) void AccumulateBatchKernel(PassView P) {
+// Batched pass: one thread per path of the given occurrence. The items of a
+// job within a pass are consecutive sample layers, numbered by occurrence;
+// within one occurrence a (job, pixel) appears once, and the occurrences are
+// accumulated in order, so the film updates never collide and every pixel
+// receives its samples in index order.
+__global__ __launch_bounds__(kBlock) void AccumulateBatchKernel(PassView P, int occurrence) {
     const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     unsigned bad = 0;
-    if (p < static_cast<int64_t>(P.counters[kCountPassPaths])) bad = AddSample(P, p, P.film[P.p_pixel[p]]);
+    if (p < static_cast<int64_t>(P.counters[kCountPassPaths]) && (P.p_job[p] >> 16) == occurrence) {
+        bad = AddSample(P, p, P.film[P.p_pixel[p]]);
+    }
     WaveStat(P.stats + kStatBad, bad);
 }
 
@@ -1477,9 +1489,11 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
             int wgrid = 0, wblock = 0;
             if ((rc = PrepareTraceArgs(ctx, wr, wa, wgrid, wblock)) != YRH_OK) return rc;
             if (ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL) {
-                hipLaunchKernelGGL(ShadowWalkKernel<true>, dim3(wgrid), dim3(wblock), 0, st, wa, P, S, L);
+                hipLaunchKernelGGL(ShadowWalkKernel<true>, dim3(wgrid), dim3(wblock), LdsBytes(wa, wblock), st, wa, P, S,
+                                   L);
             } else {
-                hipLaunchKernelGGL(ShadowWalkKernel<false>, dim3(wgrid), dim3(wblock), 0, st, wa, P, S, L);
+                hipLaunchKernelGGL(ShadowWalkKernel<false>, dim3(wgrid), dim3(wblock), LdsBytes(wa, wblock), st, wa, P, S,
+                                   L);
             }
         }
         hipLaunchKernelGGL(ResolveKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P, GV, depth, cur);
@@ -1813,6 +1827,7 @@ int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* d, float* job_ms, i
     if (ctx == nullptr || d == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null argument");
     if (ctx->render == nullptr || !ctx->render->shading_loaded) return Fail(ctx, YRH_E_NO_SCENE, "upload geometry and shading first");
     if (d->job_count <= 0 || d->jobs == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "no jobs");
+    if (d->job_count > 0xFFFF) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "too many jobs (at most 65535)");
     if (d->sampler < YRH_SAMPLER_INDEPENDENT || d->sampler > YRH_SAMPLER_ZSOBOL) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "unknown sampler");
     RenderState& R = *ctx->render;
     // Jobs and their film layout.
@@ -1842,9 +1857,12 @@ int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* d, float* job_ms, i
     if (film_pixels > (int64_t{1} << 31) - 1) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "batch film too large");
     const int64_t capacity = d->paths_per_pass > 0 ? d->paths_per_pass : std::min<int64_t>(int64_t{1} << 23, film_pixels);
     if (capacity > (int64_t{1} << 30)) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "paths_per_pass too large");
-    // Schedule: each pass takes, in order of remaining work, the next pixel
-    // range of the current sample of as many jobs as fit; a job appears at most
-    // once per pass, so its pixels see their samples in index order.
+    // Schedule: each pass takes, in order of remaining work, consecutive pixel
+    // ranges / sample layers of the jobs until it is full. Items of one job in
+    // a pass are numbered by occurrence; the pass accumulates occurrence by
+    // occurrence, so each pixel sees its samples in index order.
+    constexpr int32_t kMaxOccurrences = 64;
+    int32_t max_occurrence = 0;
     std::vector<DevItem> items;
     std::vector<int32_t> pass_items{0};
     std::vector<int32_t> job_last_pass(jobs.size(), 0);
@@ -1863,16 +1881,19 @@ int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* d, float* job_ms, i
         const int32_t pass = static_cast<int32_t>(pass_items.size()) - 1;
         for (const int32_t j : order) {
             const size_t k = static_cast<size_t>(j);
-            if (used >= capacity || remaining(j) <= 0) continue;
-            const int64_t take = std::min(capacity - used, R.b_job_pixels[k] - next_pixel[k]);
-            items.push_back(DevItem{j, sample[k], static_cast<int32_t>(next_pixel[k]), static_cast<int32_t>(take), used});
-            used += take;
-            next_pixel[k] += take;
-            if (next_pixel[k] == R.b_job_pixels[k]) {
-                next_pixel[k] = 0;
-                ++sample[k];
+            for (int32_t occ = 0; used < capacity && remaining(j) > 0 && occ < kMaxOccurrences; ++occ) {
+                const int64_t take = std::min(capacity - used, R.b_job_pixels[k] - next_pixel[k]);
+                items.push_back(DevItem{j, sample[k], static_cast<int32_t>(next_pixel[k]), static_cast<int32_t>(take), occ,
+                                        0, used});
+                used += take;
+                next_pixel[k] += take;
+                if (next_pixel[k] == R.b_job_pixels[k]) {
+                    next_pixel[k] = 0;
+                    ++sample[k];
+                }
+                job_last_pass[k] = pass;
+                max_occurrence = std::max(max_occurrence, occ);
             }
-            job_last_pass[k] = pass;
         }
         pass_items.push_back(static_cast<int32_t>(items.size()));
     }
@@ -1921,7 +1942,9 @@ int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* d, float* job_ms, i
     auto launch_pass = [&]() -> int {
         int r = LaunchPassSequence(ctx, R, P, S, L, GV, capacity, max_depth);
         if (r != YRH_OK) return r;
-        hipLaunchKernelGGL(AccumulateBatchKernel, dim3(Grid(capacity)), dim3(kBlock), 0, st, P);
+        for (int32_t occ = 0; occ <= max_occurrence; ++occ) {
+            hipLaunchKernelGGL(AccumulateBatchKernel, dim3(Grid(capacity)), dim3(kBlock), 0, st, P, occ);
+        }
         hipLaunchKernelGGL(CopyTraceTotalsKernel, dim3(1), dim3(64), 0, st, P.stats);
         hipLaunchKernelGGL(EndPassKernel, dim3(1), dim3(64), 0, st, P.counters);
         return CheckHip(ctx, hipGetLastError(), "batch pass launch");

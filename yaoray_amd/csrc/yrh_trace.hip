@@ -5,20 +5,19 @@
 namespace yrh {
 namespace {
 
-using trav::kLdsStack;
 using trav::kMaxBlock;
 using trav::TraceArgs;
 
 template <bool kAny>
 __global__ __launch_bounds__(kMaxBlock) void TraceFlatKernel(TraceArgs a) {
-    __shared__ int2 lds_stack[kLdsStack * kMaxBlock];
+    extern __shared__ int2 lds_stack[];
     const trav::RayArrayPolicy<kAny> pol{&a};
     trav::TraceFlatLoop<kAny>(a, pol, lds_stack);
 }
 
 template <bool kAny>
 __global__ __launch_bounds__(kMaxBlock) void TraceTwoLevelKernel(TraceArgs a) {
-    __shared__ int2 lds_stack[kLdsStack * kMaxBlock];
+    extern __shared__ int2 lds_stack[];
     const trav::RayArrayPolicy<kAny> pol{&a};
     trav::TraceTwoLevelLoop<kAny>(a, pol, lds_stack);
 }
@@ -26,6 +25,10 @@ __global__ __launch_bounds__(kMaxBlock) void TraceTwoLevelKernel(TraceArgs a) {
 bool Aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 }  // namespace
+
+size_t LdsBytes(const trav::TraceArgs& a, int block) {
+    return static_cast<size_t>(a.lds_levels) * static_cast<size_t>(block) * sizeof(int2);
+}
 
 int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a, int& grid, int& block) {
     const DevGeometry& g = ctx->geo;
@@ -37,7 +40,7 @@ int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a
     if (lanes > bound) grid = static_cast<int>(std::max<int64_t>(1, (bound + block - 1) / block));
     const int64_t total = static_cast<int64_t>(grid) * block;
     const int32_t capacity = g.stack_capacity;
-    const int64_t spill_levels = std::max<int64_t>(0, capacity - kLdsStack);
+    const int64_t spill_levels = std::max<int64_t>(0, capacity - ctx->lds_levels);
     const size_t spill_bytes = static_cast<size_t>(std::max<int64_t>(1, spill_levels * total)) * sizeof(int2);
     if (ctx->stack_spill.bytes < spill_bytes) {
         // Earlier launches on the stream may still use the old region.
@@ -87,6 +90,7 @@ int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a
     a.spill = ctx->stack_spill.as<int2>();
     a.spill_stride = total;
     a.capacity = capacity;
+    a.lds_levels = ctx->lds_levels;
     return YRH_OK;
 }
 
@@ -103,15 +107,15 @@ int LaunchTrace(yrh_context* ctx, const TraceRequest& r) {
     const bool two = ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL;
     if (r.any) {
         if (two) {
-            hipLaunchKernelGGL(TraceTwoLevelKernel<true>, dim3(grid), dim3(block), 0, ctx->stream, a);
+            hipLaunchKernelGGL(TraceTwoLevelKernel<true>, dim3(grid), dim3(block), LdsBytes(a, block), ctx->stream, a);
         } else {
-            hipLaunchKernelGGL(TraceFlatKernel<true>, dim3(grid), dim3(block), 0, ctx->stream, a);
+            hipLaunchKernelGGL(TraceFlatKernel<true>, dim3(grid), dim3(block), LdsBytes(a, block), ctx->stream, a);
         }
     } else {
         if (two) {
-            hipLaunchKernelGGL(TraceTwoLevelKernel<false>, dim3(grid), dim3(block), 0, ctx->stream, a);
+            hipLaunchKernelGGL(TraceTwoLevelKernel<false>, dim3(grid), dim3(block), LdsBytes(a, block), ctx->stream, a);
         } else {
-            hipLaunchKernelGGL(TraceFlatKernel<false>, dim3(grid), dim3(block), 0, ctx->stream, a);
+            hipLaunchKernelGGL(TraceFlatKernel<false>, dim3(grid), dim3(block), LdsBytes(a, block), ctx->stream, a);
         }
     }
     rc = CheckHip(ctx, hipGetLastError(), r.any ? "trace any launch" : "trace closest launch");

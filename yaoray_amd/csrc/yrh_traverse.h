@@ -19,9 +19,9 @@
 // Execution model: a persistent grid sized to the device. Each wave64 refills
 // idle lanes from a global ray counter with one atomic per wave (ballot +
 // mbcnt), so divergent ray lengths do not idle the wave. The traversal stack
-// lives in LDS (kLdsStack entries per lane, laid out [level][lane] so a
-// wave's pushes hit consecutive banks) and spills deeper levels to a per-lane
-// global region; its capacity is the proven bound max_depth + 1 (one pending
+// lives in LDS (the first lds_levels entries per lane, a launch option up to
+// kMaxLdsStack; laid out [level][lane] so a wave's pushes hit consecutive
+// banks) and spills deeper levels to a per-lane global region; its capacity is the proven bound max_depth + 1 (one pending
 // sibling per level plus the top pair).
 //
 // A policy object supplies each ray's first segment (Begin) and receives each
@@ -39,7 +39,7 @@
 namespace yrh {
 namespace trav {
 
-constexpr int kLdsStack = 16;
+constexpr int kMaxLdsStack = 16;  // LDS stack levels per lane, at most (launch option)
 constexpr int kMaxBlock = 256;
 constexpr int kRefillIdleLanes = 16;
 
@@ -75,6 +75,7 @@ struct TraceArgs {
     int2* spill;
     int64_t spill_stride;
     int32_t capacity;
+    int32_t lds_levels;  // stack levels held in LDS (dynamic shared memory: lds_levels x block int2)
 };
 
 struct RayPre {
@@ -172,6 +173,7 @@ struct Stack {
     int2* spill;  // this lane's column in the global spill region
     int64_t spill_stride;
     int capacity;
+    int lds_levels;
     int sp;
     bool overflow;
 
@@ -181,17 +183,17 @@ struct Stack {
             return;
         }
         const int2 e = make_int2(code, __float_as_int(entry));
-        if (sp < kLdsStack) {
+        if (sp < lds_levels) {
             lds[sp * lds_stride] = e;
         } else {
-            spill[static_cast<int64_t>(sp - kLdsStack) * spill_stride] = e;
+            spill[static_cast<int64_t>(sp - lds_levels) * spill_stride] = e;
         }
         ++sp;
     }
     __device__ __forceinline__ int2 Pop() {
         --sp;
-        if (sp < kLdsStack) return lds[sp * lds_stride];
-        return spill[static_cast<int64_t>(sp - kLdsStack) * spill_stride];
+        if (sp < lds_levels) return lds[sp * lds_stride];
+        return spill[static_cast<int64_t>(sp - lds_levels) * spill_stride];
     }
 };
 
@@ -294,7 +296,7 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int64_t gtid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     Stack st{lds_stack + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
-             a.capacity, 0, false};
+             a.capacity, a.lds_levels, 0, false};
 
     if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.next_counter = 0u;
     const int32_t n = a.n_device != nullptr ? static_cast<int32_t>(*a.n_device) : a.n;
@@ -381,7 +383,7 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int64_t gtid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     Stack st{lds_stack + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
-             a.capacity, 0, false};
+             a.capacity, a.lds_levels, 0, false};
 
     if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.next_counter = 0u;
     const int32_t n = a.n_device != nullptr ? static_cast<int32_t>(*a.n_device) : a.n;
@@ -525,5 +527,7 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
 // Fills the launch arguments (grid, stack spill region, counters) of one
 // traversal over `r`'s queue (yrh_trace.hip).
 int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a, int& grid, int& block);
+// Dynamic LDS bytes of a traversal launch with `block` threads.
+size_t LdsBytes(const trav::TraceArgs& a, int block);
 
 }  // namespace yrh

@@ -211,6 +211,7 @@ YRH_SIGNATURES = {
     "yrh_trace_closest": (_I, [_VP, C.POINTER(RaySoa), _I64, C.POINTER(HitSoa)]),
     "yrh_trace_any": (_I, [_VP, C.POINTER(RaySoa), _I64, _VP, _VP, _VP]),
     "yrh_set_trace_config": (_I, [_VP, _I, _I]),
+    "yrh_set_trace_lds_levels": (_I, [_VP, _I]),
     "yrh_event_create": (_I, [_VP, C.POINTER(_VP)]),
     "yrh_event_destroy": (_I, [_VP, _VP]),
     "yrh_event_record": (_I, [_VP, _VP]),
