@@ -209,9 +209,10 @@ int yrh_trace_any(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n,
 /* Launch geometry of the persistent traversal kernels: workgroups per compute
  * unit and threads per workgroup (64, 128 or 256); 0 keeps the current value. */
 int yrh_set_trace_config(yrh_context* ctx, int blocks_per_cu, int block_size);
-/* Traversal stack levels kept in LDS per lane (1..16, default 8); deeper
- * levels spill to a per-lane global region. Fewer levels use less LDS per
- * workgroup, so more workgroups fit on a CU. */
+/* Traversal stack levels kept in LDS per lane (1..16; 0 = automatic: 8 for a
+ * flat tree, 16 for two-level); deeper levels spill to a per-lane global
+ * region. Fewer levels use less LDS per workgroup, so more workgroups fit on
+ * a CU (by default as many as the LDS allows, at most 8). */
 int yrh_set_trace_lds_levels(yrh_context* ctx, int levels);
 
 /* ---- Shading data ----------------------------------------------------------- */

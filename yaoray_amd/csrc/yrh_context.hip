@@ -305,7 +305,7 @@ int yrh_set_trace_config(yrh_context* ctx, int blocks_per_cu, int block_size) {
 
 int yrh_set_trace_lds_levels(yrh_context* ctx, int levels) {
     if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
-    if (levels < 1 || levels > 16) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "lds levels must be in [1, 16]");
+    if (levels < 0 || levels > 16) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "lds levels must be in [0, 16]");
     ctx->lds_levels = levels;
     return YRH_OK;
 }

@@ -133,12 +133,12 @@ struct yrh_context {
     yrh::DevBuf work_counter;
     int launch_parity = 0;
     yrh::DevBuf stack_spill;    // int2 per lane per spilled stack level
-    // Launch shape of the persistent traversal kernels: 8 LDS stack levels per
-    // lane (16 KB per 256-thread workgroup) let 8 workgroups share a CU, the
-    // fastest shape measured on MI355X for the ray-cast microbench.
-    int blocks_per_cu = 8;
+    // Launch shape of the persistent traversal kernels; 0 = chosen per launch
+    // (yrh_trace.hip PrepareTraceArgs: 8 LDS stack levels and 8 workgroups per
+    // CU for flat trees, 16 levels and 5 workgroups for two-level ones).
+    int blocks_per_cu = 0;
     int block_size = 256;
-    int lds_levels = 8;  // traversal stack levels kept in LDS per lane
+    int lds_levels = 0;  // traversal stack levels kept in LDS per lane
     yrh::RenderState* render = nullptr;  // surface/light data, film, path state (yrh_render.hip)
     yrh::GuideState* guide = nullptr;    // learned guiding proposal (yrh_guide.hip)
 };
@@ -155,7 +155,8 @@ struct TraceRequest {
     const float4* dir = nullptr;
     int64_t n_host = 0;                  // ray count when n_device is null
     const uint32_t* n_device = nullptr;  // device-resident ray count (read by the kernel)
-    int64_t capacity = 0;                // upper bound of *n_device (sizes the grid)
+    int64_t capacity = 0;                // upper bound of *n_device
+    int64_t expected = 0;                // typical live count (sizes the grid; 0 = capacity)
     uint32_t* work_counter = nullptr;    // zeroed ray counter owned by the caller; null = context's
     float4* hit = nullptr;
     int32_t* instance = nullptr;

@@ -1451,6 +1451,7 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
         tr.dir = P.q_dir[cur];
         tr.n_device = P.counters + cur;
         tr.capacity = paths;
+        tr.expected = paths;
         tr.work_counter = P.counters + kWorkExtend;
         tr.hit = P.hits;
         tr.instance = ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL ? P.hit_inst : nullptr;
@@ -1471,6 +1472,7 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
             sr.dir = P.s_dir;
             sr.n_device = P.counters + kCountShadow;
             sr.capacity = 2 * paths;
+            sr.expected = paths;  // about one shadow ray per path in flight
             sr.work_counter = P.counters + kWorkShadow;
             sr.occluded = P.s_occ;
             sr.totals = P.stats + kStatTraceNodes;
@@ -1482,6 +1484,7 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
             wr.dir = P.s_dir;
             wr.n_device = P.counters + kCountShadow;
             wr.capacity = 2 * paths;
+            wr.expected = paths;
             wr.work_counter = P.counters + kWorkShadow;
             wr.hit = P.s_trans;  // not written by the walk; LaunchTrace's checks want an output
             wr.totals = P.stats + kStatTraceNodes;

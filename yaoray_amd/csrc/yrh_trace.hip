@@ -33,14 +33,26 @@ size_t LdsBytes(const trav::TraceArgs& a, int block) {
 int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a, int& grid, int& block) {
     const DevGeometry& g = ctx->geo;
     block = ctx->block_size;
-    grid = ctx->compute_units * ctx->blocks_per_cu;
+    // Launch shape: LDS stack levels per lane (flat trees are shallow enough
+    // for 8; two-level stacks hold a TLAS and a BLAS path, so 16) and as many
+    // workgroups per CU as that LDS allows (at most 8), unless set explicitly.
+    const int lds_levels = ctx->lds_levels > 0 ? ctx->lds_levels : (g.accel_kind == YRH_ACCEL_TWO_LEVEL ? 16 : 8);
+    const int per_cu = ctx->blocks_per_cu > 0
+                           ? ctx->blocks_per_cu
+                           : std::max(1, std::min(8, (160 * 1024) / (lds_levels * block * static_cast<int>(sizeof(int2)))));
+    grid = ctx->compute_units * per_cu;
     const int64_t lanes = static_cast<int64_t>(grid) * block;
-    // Do not launch more lanes than rays can need (rounded up to whole workgroups).
-    const int64_t bound = r.n_device != nullptr ? r.capacity : r.n_host;
-    if (lanes > bound) grid = static_cast<int>(std::max<int64_t>(1, (bound + block - 1) / block));
+    // Small queues: keep at least 4 rays per lane (the persistent lanes refill
+    // from the counter, so fewer, fuller workgroups drain faster), and never
+    // more lanes than rays.
+    const int64_t bound = r.n_device != nullptr ? (r.expected > 0 ? std::min(r.expected, r.capacity) : r.capacity)
+                                                : r.n_host;
+    const int64_t want = std::max<int64_t>(std::min<int64_t>(bound, static_cast<int64_t>(ctx->compute_units) * block),
+                                           (bound + 3) / 4);
+    if (lanes > want) grid = static_cast<int>(std::max<int64_t>(1, (want + block - 1) / block));
     const int64_t total = static_cast<int64_t>(grid) * block;
     const int32_t capacity = g.stack_capacity;
-    const int64_t spill_levels = std::max<int64_t>(0, capacity - ctx->lds_levels);
+    const int64_t spill_levels = std::max<int64_t>(0, capacity - lds_levels);
     const size_t spill_bytes = static_cast<size_t>(std::max<int64_t>(1, spill_levels * total)) * sizeof(int2);
     if (ctx->stack_spill.bytes < spill_bytes) {
         // Earlier launches on the stream may still use the old region.
@@ -90,7 +102,7 @@ int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a
     a.spill = ctx->stack_spill.as<int2>();
     a.spill_stride = total;
     a.capacity = capacity;
-    a.lds_levels = ctx->lds_levels;
+    a.lds_levels = lds_levels;
     return YRH_OK;
 }
 
