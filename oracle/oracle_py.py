@@ -275,6 +275,16 @@ def ref_render(scene, width, height, spp, max_depth=8, sampler=2, seed=0x59414F5
     return film, out, secs.value
 
 
+def ref_sample_values(kind, seed, spp, x, y, sample, dim, dims):
+    """The reference's yr::Sampler at bounce 0 (Sample1D / Sample2D of raw dimensions)."""
+    lib = ref_lib()
+    lib.ref_sample_values.argtypes = [_I32, C.c_uint64, _I32, _I64, _VP, _VP, _VP, _VP, _I32, _VP]
+    arrs = [np.ascontiguousarray(a, np.int32) for a in (x, y, sample, dim)]
+    out = np.zeros(len(x) * dims, np.float32)
+    lib.ref_sample_values(kind, seed, spp, len(x), *[a.ctypes.data for a in arrs], dims, out.ctypes.data)
+    return out.reshape(len(x), dims) if dims == 2 else out
+
+
 def ref_inverse(m):
     m = np.ascontiguousarray(m, np.float32).reshape(16)
     out = np.zeros(16, np.float32)

@@ -10,6 +10,7 @@
 #include <yaoray/backends/cpu/cpu_prepared_scene.hpp>
 #include <yaoray/core/transform.hpp>
 #include <yaoray/lighting/environment.hpp>
+#include <yaoray/sampling/sampler.hpp>
 #include <yaoray/scene/render_scene.hpp>
 
 #include "../include/yrh.h"
@@ -288,6 +289,28 @@ void ref_inverse(const float* m, float* out) {
     std::memcpy(a.m.data(), m, 64);
     const yr::Mat4f r = yr::Inverse(a);
     std::memcpy(out, r.m.data(), 64);
+}
+
+// yr::Sampler values at bounce 0 (include/yaoray/sampling/sampler.hpp:25-50):
+// dims == 1 -> Sample1D(dim[i]), dims == 2 -> Sample2D(dim[i]) into out[2i..].
+int ref_sample_values(int32_t kind, uint64_t seed, int32_t spp, int64_t n, const int32_t* x, const int32_t* y,
+                      const int32_t* sample, const int32_t* dim, int32_t dims, float* out) {
+    const yr::RenderSamplerKind k = kind == YRH_SAMPLER_INDEPENDENT ? yr::RenderSamplerKind::Independent
+                                    : kind == YRH_SAMPLER_STRATIFIED ? yr::RenderSamplerKind::Stratified
+                                    : kind == YRH_SAMPLER_OWEN_SOBOL ? yr::RenderSamplerKind::OwenSobol
+                                                                     : yr::RenderSamplerKind::ZSobol;
+    for (int64_t i = 0; i < n; ++i) {
+        const yr::Sampler s(k, seed, x[i], y[i], sample[i], spp, 1);
+        const auto d = static_cast<yr::SampleDimension>(static_cast<uint32_t>(dim[i]));
+        if (dims == 1) {
+            out[i] = s.Sample1D(d);
+        } else {
+            const yr::Vec2f v = s.Sample2D(d);
+            out[2 * i] = v.x;
+            out[2 * i + 1] = v.y;
+        }
+    }
+    return 0;
 }
 
 #pragma GCC visibility pop

@@ -1,0 +1,146 @@
+"""Writes the golden fixtures of tests/golden/ from the reference itself.
+
+Run in a container where the reference's sources are present and
+oracle/_ref/libyrref.so is built from them (`make -C oracle ref`):
+
+    python tests/golden/make_golden.py
+
+Each fixture holds its inputs (or, for the procedural Cornell workload, a
+digest of them) and the reference's outputs on them, so tests can pin the
+oracle and the device path where the reference is absent. Sizes follow the
+reference-only pins of the traversal and path-tracing path: SAH node arrays
+(src/accel/bvh_split_policy.cpp:105-112 order), per-ray BvhHit + node /
+triangle counters (src/accel/bvh_traversal.cpp:253-345,
+src/accel/two_level_bvh.cpp:365-481), yr::Sampler bits
+(src/sampling/sampler.cpp) and an FP32 film + stats of RenderCpuPathTrace
+(src/backends/cpu/cpu_path_tracer.cpp).
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle_py as O  # noqa: E402
+from tests.helpers import MeshArrays  # noqa: E402
+from yaoray_amd import Scene, random_rays  # noqa: E402
+
+SEED = 0x59414F524159
+
+
+def soup(count, seed, edge_min, edge_max):
+    """Random triangles: centres uniform in the unit cube, edge lengths
+    log-uniform in [edge_min, edge_max], random orientation (the ray-cast
+    workload's rule, with longer edges so a few thousand triangles still
+    stop a good share of the rays)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.0, 1.0, (count, 1, 3))
+    edge = np.exp(rng.uniform(np.log(edge_min), np.log(edge_max), (count, 1, 1)))
+    offs = rng.normal(size=(count, 3, 3))
+    offs /= np.linalg.norm(offs, axis=2, keepdims=True)
+    return (c + 0.5 * edge * offs).astype(np.float32)
+
+
+def geometry_digest(scene):
+    v, i, p = scene.geometry()
+    h = hashlib.sha256()
+    for a in (v, i, p):
+        h.update(np.ascontiguousarray(a).tobytes())
+    h.update(np.array(list(scene.info.camera), np.float32).tobytes())
+    return h.hexdigest()
+
+
+def sampler_fixture():
+    rng = np.random.default_rng(101)
+    n = 1024
+    x = rng.integers(0, 2048, n).astype(np.int32)
+    y = rng.integers(0, 2048, n).astype(np.int32)
+    sample = rng.integers(0, 16, n).astype(np.int32)
+    dim = rng.integers(0, 48, n).astype(np.int32)
+    out = {"x": x, "y": y, "sample": sample, "dim": dim, "seed": np.uint64(SEED)}
+    for kind in range(4):
+        for spp in (1, 13, 16):
+            s = np.minimum(sample, spp - 1).astype(np.int32)
+            out[f"v1_{kind}_{spp}"] = O.ref_sample_values(kind, SEED, spp, x, y, s, dim, 1)
+            out[f"v2_{kind}_{spp}"] = O.ref_sample_values(kind, SEED, spp, x, y, s, dim, 2)
+    return out
+
+
+def trace_outputs(r):
+    hit = np.stack([r["t"], r["u"], r["v"], r["triangle"].view(np.float32)], axis=1)
+    return {"hit": hit, "instance": r["instance"], "node_tests": r["node_tests"],
+            "triangle_tests": r["triangle_tests"]}
+
+
+def flat_fixture():
+    tris = soup(3000, 202, 0.02, 0.12)
+    s = MeshArrays().add_triangles(tris).scene()
+    v, i, p = s.geometry()
+    ref = O.RefScene(v, i, p)
+    bvh = ref.bvh()
+    o1, d1 = random_rays(2048, seed=203)
+    # Pinhole rays from outside the cube towards it.
+    g = np.random.default_rng(204)
+    o2 = np.zeros((2048, 4), np.float32)
+    d2 = np.zeros((2048, 4), np.float32)
+    o2[:, :3] = (0.5, 0.5, 2.5)
+    target = np.stack([g.uniform(0, 1, 2048), g.uniform(0, 1, 2048), np.zeros(2048)], axis=1)
+    dd = target - o2[:, :3]
+    d2[:, :3] = dd / np.linalg.norm(dd, axis=1, keepdims=True)
+    o2[:, 3] = 1e-5
+    d2[:, 3] = np.inf
+    org = np.concatenate([o1, o2])
+    dirs = np.concatenate([d1, d2])
+    out = {"triangles": tris, "nodes": bvh.nodes.view(np.uint8).reshape(-1),
+           "primitive_indices": bvh.primitive_indices, "max_depth": np.int32(bvh.max_depth),
+           "org": org, "dir": dirs}
+    out.update(trace_outputs(ref.trace(org, dirs)))
+    return out
+
+
+def two_level_fixture():
+    s = Scene.instanced(instances=60, subdivisions=1, variants=3, env_width=8, env_height=4, seed=205)
+    s.build_accel()
+    v, i, p = s.geometry()
+    _, _, inst = s.tlas()
+    ip = inst["primitive"].copy()
+    im = inst["object_to_world"].copy()
+    ref = O.RefScene(v, i, p, ip, im)
+    tn, tpi, _, _ = ref.tlas()
+    org, dirs = s.camera_rays(80, 45)
+    out = {"vertices": v.view(np.uint8).reshape(-1), "indices": i, "primitives": p.view(np.uint8).reshape(-1),
+           "instance_primitive": ip, "instance_matrices": im, "tlas_nodes": tn.view(np.uint8).reshape(-1),
+           "tlas_primitive_indices": tpi, "org": org, "dir": dirs}
+    out.update(trace_outputs(ref.trace(org, dirs)))
+    return out
+
+
+def cornell_fixture():
+    s = Scene.cornell(1)
+    s.build_accel()
+    film8, st8, _ = O.ref_render(s, 64, 64, 16, max_depth=8, sampler=2, seed=SEED)
+    film1, st1, _ = O.ref_render(s, 64, 64, 4, max_depth=1, sampler=2, seed=SEED)
+    keys = O.STAT_KEYS
+    return {"digest": np.array(geometry_digest(s)), "width": np.int32(64), "height": np.int32(64),
+            "film8": film8.view(np.uint8).reshape(-1), "stats8": np.array([st8[k] for k in keys], np.uint64),
+            "spp8": np.int32(16), "film1": film1.view(np.uint8).reshape(-1),
+            "stats1": np.array([st1[k] for k in keys], np.uint64), "spp1": np.int32(4),
+            "seed": np.uint64(SEED)}
+
+
+def main():
+    if not O.ref_available():
+        raise SystemExit("oracle/_ref/libyrref.so is missing: build it with `make -C oracle ref`")
+    for name, fn in (("sampler", sampler_fixture), ("flat_trace", flat_fixture),
+                     ("two_level_trace", two_level_fixture), ("cornell_film", cornell_fixture)):
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, **fn())
+        print(f"{path}: {os.path.getsize(path)} bytes")
+
+
+if __name__ == "__main__":
+    main()

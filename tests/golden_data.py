@@ -1,0 +1,68 @@
+"""Loaders of the golden fixtures in tests/golden/ (written from the reference
+by tests/golden/make_golden.py): inputs rebuilt into scenes, outputs as arrays."""
+import hashlib
+import os
+
+import numpy as np
+
+from tests.helpers import MeshArrays
+from yaoray_amd import Scene
+from yaoray_amd.api import FILM32_DTYPE, NODE_DTYPE, PRIMITIVE_DTYPE, VERTEX_DTYPE
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+STAT_KEYS = ("rays_traced", "shadow_rays", "occluded_shadow_rays", "triangle_tests", "sphere_tests",
+             "bvh_node_tests", "hits", "misses", "bad_samples")
+
+
+def load(name):
+    with np.load(os.path.join(HERE, name + ".npz"), allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}
+
+
+def expected_hits(f):
+    h = f["hit"]
+    return {"t": h[:, 0].copy(), "u": h[:, 1].copy(), "v": h[:, 2].copy(),
+            "triangle": h[:, 3].copy().view(np.int32), "instance": f["instance"],
+            "node_tests": f["node_tests"], "triangle_tests": f["triangle_tests"]}
+
+
+def flat_scene(f):
+    s = MeshArrays().add_triangles(f["triangles"]).scene()
+    s.build_accel()
+    return s
+
+
+def two_level_scene(f):
+    v = f["vertices"].view(VERTEX_DTYPE)
+    p = f["primitives"].view(PRIMITIVE_DTYPE)
+    s = Scene.from_arrays(v, f["indices"], p, f["instance_primitive"], f["instance_matrices"])
+    s.build_accel()
+    return s
+
+
+def nodes(raw):
+    return raw.view(NODE_DTYPE)
+
+
+def film(raw):
+    return raw.view(FILM32_DTYPE)
+
+
+def stats(arr):
+    return dict(zip(STAT_KEYS, (int(x) for x in arr)))
+
+
+def geometry_digest(scene):
+    v, i, p = scene.geometry()
+    h = hashlib.sha256()
+    for a in (v, i, p):
+        h.update(np.ascontiguousarray(a).tobytes())
+    h.update(np.array(list(scene.info.camera), np.float32).tobytes())
+    return h.hexdigest()
+
+
+def cornell_scene(f):
+    s = Scene.cornell(1)
+    s.build_accel()
+    assert geometry_digest(s) == str(f["digest"]), "Cornell workload inputs differ from the fixture's"
+    return s

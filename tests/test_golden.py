@@ -1,0 +1,70 @@
+"""CPU pins against the golden fixtures (the reference's own outputs, written
+by tests/golden/make_golden.py from oracle/_ref): they hold where the
+reference is absent, so the oracle and the host builder stay pinned on any
+machine.
+
+* yr::Sampler values, all four kinds, three sample counts, 1D and 2D;
+* SAH node arrays of the host builder, byte for byte (flat and TLAS);
+* per-ray hits and node / triangle counters of the oracle, bit for bit;
+* the Cornell workload's FP32 film and statistics from the oracle, bit for bit.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from tests import golden_data as G
+from tests.helpers import same_bits
+
+TRACE_KEYS = ("t", "u", "v", "triangle", "node_tests", "triangle_tests")
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_oracle_sampler_matches_fixture(kind):
+    f = G.load("sampler")
+    seed = int(f["seed"])
+    for spp in (1, 13, 16):
+        s = np.minimum(f["sample"], spp - 1)
+        o1 = O.sample_values(kind, seed, spp, f["x"], f["y"], s, f["dim"], 1)
+        o2 = O.sample_values(kind, seed, spp, f["x"], f["y"], s, f["dim"], 2)
+        assert same_bits(o1, f[f"v1_{kind}_{spp}"]), spp
+        assert same_bits(o2, f[f"v2_{kind}_{spp}"]), spp
+
+
+def test_builder_and_oracle_match_flat_fixture():
+    f = G.load("flat_trace")
+    s = G.flat_scene(f)
+    bvh = s.flat_bvh()
+    assert bvh.nodes.tobytes() == G.nodes(f["nodes"]).tobytes()
+    assert np.array_equal(bvh.primitive_indices, f["primitive_indices"])
+    assert bvh.max_depth == int(f["max_depth"])
+    v, i, p = s.geometry()
+    o = O.trace_flat(v, i, p, bvh, f["org"], f["dir"])
+    exp = G.expected_hits(f)
+    for k in TRACE_KEYS:
+        assert same_bits(o[k], exp[k]), k
+    assert 0.2 < (exp["triangle"] >= 0).mean() < 0.9
+
+
+def test_builder_and_oracle_match_two_level_fixture():
+    f = G.load("two_level_trace")
+    s = G.two_level_scene(f)
+    tl, tp, inst = s.tlas()
+    assert tl.nodes.tobytes() == G.nodes(f["tlas_nodes"]).tobytes()
+    assert np.array_equal(tl.primitive_indices, f["tlas_primitive_indices"])
+    v, i, p = s.geometry()
+    blases = [s.blas(k) for k in range(s.info.blas_count)]
+    o = O.trace_two_level(v, i, p, blases, inst, tl, tp, f["org"], f["dir"])
+    exp = G.expected_hits(f)
+    for k in TRACE_KEYS + ("instance",):
+        assert same_bits(o[k], exp[k]), k
+
+
+@pytest.mark.parametrize("depth", [1, 8])
+def test_oracle_render_matches_cornell_fixture(depth):
+    f = G.load("cornell_film")
+    s = G.cornell_scene(f)
+    w, h = int(f["width"]), int(f["height"])
+    spp = int(f[f"spp{depth}"])
+    film, st = O.render(s, w, h, spp, max_depth=depth, sampler=2, seed=int(f["seed"]))
+    assert film.tobytes() == G.film(f[f"film{depth}"]).tobytes()
+    assert st == G.stats(f[f"stats{depth}"])
