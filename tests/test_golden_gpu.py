@@ -4,8 +4,9 @@ written from oracle/_ref by tests/golden/make_golden.py), through the C ABI.
 * sampler values, camera-independent: bit-exact;
 * closest-hit traversal, flat and two-level: hit ids, t, barycentrics,
   instance and node / triangle counters bit-exact;
-* Cornell max-depth-1 film (emission + emissive NEE): bit-exact after FP32
-  rounding of the device's FP64 film, identical counters;
+* Cornell max-depth-1 film (emission + emissive NEE) at 4 spp: identical
+  counters and sample counts, sums equal up to the reference's FP32
+  summation rounding (each sample value is exact);
 * Cornell max-depth-8 film at equal spp: beyond the first bounce the device's
   transcendentals differ from glibc in the last bits, so the bar is the
   statistical one of the oracle comparison — mean luminance within 0.5 %,
@@ -60,8 +61,11 @@ def test_device_depth1_film_matches_fixture(device):
     w, h = int(f["width"]), int(f["height"])
     g, st = device.render(s, w, h, int(f["spp1"]), max_depth=1, sampler=2, seed=int(f["seed"]))
     ref = G.film(f["film1"])
-    assert same_bits(g["sum"].astype(np.float32), ref["sum"])
+    # Every sample value is bit-exact; the reference sums them in FP32 (three
+    # roundings of at most half an ulp for four samples), the device in FP64.
     assert np.array_equal(g["samples"], ref["samples"])
+    assert np.allclose(g["sum"], ref["sum"], rtol=4 * 2.0 ** -24, atol=0.0)
+    assert np.array_equal(g["sum"] == 0.0, ref["sum"] == 0.0)
     assert np.array_equal(g["aov_samples"], ref["aov_samples"])
     rs = G.stats(f["stats1"])
     for k in ("rays_traced", "hits", "misses", "shadow_rays", "occluded_shadow_rays", "bad_samples"):

@@ -8,8 +8,10 @@ namespace {
 using trav::kMaxBlock;
 using trav::TraceArgs;
 
+// At most 64 VGPRs (8 waves per SIMD): with 8 workgroups of 256 per CU every
+// workgroup of the persistent grid is resident at once.
 template <bool kAny>
-__global__ __launch_bounds__(kMaxBlock) void TraceFlatKernel(TraceArgs a) {
+__global__ __launch_bounds__(kMaxBlock, 8) void TraceFlatKernel(TraceArgs a) {
     extern __shared__ int2 lds_stack[];
     const trav::RayArrayPolicy<kAny> pol{&a};
     trav::TraceFlatLoop<kAny>(a, pol, lds_stack);

@@ -167,6 +167,11 @@ __device__ __forceinline__ bool TriangleTest(const float4& s0, const float4& s1,
     return true;
 }
 
+// Per-lane traversal stack. The top entry is held in registers: a push writes
+// the previous top to memory and a pop of the register top reads nothing, so
+// the common interior step (push far, push near, pop near) costs one LDS
+// write instead of two writes and a read. Entries 0 .. sp-1; when top_valid
+// the entry sp-1 is `top` and only 0 .. sp-2 are in memory.
 struct Stack {
     int2* lds;  // this lane's column: lds[level * stride]
     int lds_stride;
@@ -176,24 +181,41 @@ struct Stack {
     int lds_levels;
     int sp;
     bool overflow;
+    bool top_valid;
+    int2 top;
 
+    __device__ __forceinline__ void Store(int level, int2 e) {
+        if (level < lds_levels) {
+            lds[level * lds_stride] = e;
+        } else {
+            spill[static_cast<int64_t>(level - lds_levels) * spill_stride] = e;
+        }
+    }
     __device__ __forceinline__ void Push(int32_t code, float entry) {
         if (sp >= capacity) {
             overflow = true;
             return;
         }
-        const int2 e = make_int2(code, __float_as_int(entry));
-        if (sp < lds_levels) {
-            lds[sp * lds_stride] = e;
-        } else {
-            spill[static_cast<int64_t>(sp - lds_levels) * spill_stride] = e;
-        }
+        if (top_valid) Store(sp - 1, top);
+        top = make_int2(code, __float_as_int(entry));
+        top_valid = true;
         ++sp;
     }
     __device__ __forceinline__ int2 Pop() {
         --sp;
+        if (top_valid) {
+            top_valid = false;
+            return top;
+        }
         if (sp < lds_levels) return lds[sp * lds_stride];
         return spill[static_cast<int64_t>(sp - lds_levels) * spill_stride];
+    }
+    // Drops every entry at or above `level`.
+    __device__ __forceinline__ void Truncate(int level) {
+        if (level < sp) {
+            sp = level;
+            top_valid = false;
+        }
     }
 };
 
@@ -296,7 +318,7 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int64_t gtid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     Stack st{lds_stack + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
-             a.capacity, a.lds_levels, 0, false};
+             a.capacity, a.lds_levels, 0, false, false, make_int2(0, 0)};
 
     if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.next_counter = 0u;
     const int32_t n = a.n_device != nullptr ? static_cast<int32_t>(*a.n_device) : a.n;
@@ -359,7 +381,7 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
                             const float4* s = a.slots + 3 * static_cast<int64_t>(first + k);
                             ++tt;
                             if (TriangleTest(s[0], s[1], s[2], ox, oy, oz, dx, dy, dz, t_min, kAny, best) && kAny) {
-                                st.sp = 0;
+                                st.Truncate(0);
                                 break;
                             }
                         }
@@ -383,7 +405,7 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int64_t gtid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     Stack st{lds_stack + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
-             a.capacity, a.lds_levels, 0, false};
+             a.capacity, a.lds_levels, 0, false, false, make_int2(0, 0)};
 
     if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.next_counter = 0u;
     const int32_t n = a.n_device != nullptr ? static_cast<int32_t>(*a.n_device) : a.n;
@@ -456,7 +478,7 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
                     best = bbest;
                     best_inst = cur_inst;
                     if (kAny) {
-                        st.sp = 0;
+                        st.Truncate(0);
                         cursor = leaf_end;
                     }
                 }
@@ -497,7 +519,7 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
                                     break;
                                 }
                             }
-                            if (kAny && bbest.hit) st.sp = blas_base;  // finish this BLAS now
+                            if (kAny && bbest.hit) st.Truncate(blas_base);  // finish this BLAS now
                         } else {
                             InteriorStep(a.nodes, code, bpre, t_min, bbest.t, nt, st);
                         }
