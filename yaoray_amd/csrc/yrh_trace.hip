@@ -8,17 +8,20 @@ namespace {
 using trav::kMaxBlock;
 using trav::TraceArgs;
 
-// At most 64 VGPRs (8 waves per SIMD): with 8 workgroups of 256 per CU every
-// workgroup of the persistent grid is resident at once.
+// The flat kernels compile to at most 64 VGPRs (8 waves per SIMD), so with 8
+// workgroups of 256 per CU every workgroup of the persistent grid is resident
+// at once. (A min-waves launch bound asking for the same limit makes this
+// ROCm 7.2 compiler fail instruction verification on the closest-hit kernel.)
 template <bool kAny>
-__global__ __launch_bounds__(kMaxBlock, 8) void TraceFlatKernel(TraceArgs a) {
+__global__ __launch_bounds__(kMaxBlock) void TraceFlatKernel(TraceArgs a) {
     extern __shared__ int2 lds_stack[];
     const trav::RayArrayPolicy<kAny> pol{&a};
     trav::TraceFlatLoop<kAny>(a, pol, lds_stack);
 }
 
+// At most 96 VGPRs (5 waves per SIMD, the most that 16 LDS stack levels allow).
 template <bool kAny>
-__global__ __launch_bounds__(kMaxBlock) void TraceTwoLevelKernel(TraceArgs a) {
+__global__ __launch_bounds__(kMaxBlock, 5) void TraceTwoLevelKernel(TraceArgs a) {
     extern __shared__ int2 lds_stack[];
     const trav::RayArrayPolicy<kAny> pol{&a};
     trav::TraceTwoLevelLoop<kAny>(a, pol, lds_stack);

@@ -225,28 +225,51 @@ __device__ __forceinline__ void DecodeLeaf(int32_t code, const int32_t* leaf_cou
     count = nib < kLeafCountEscape ? nib + 1 : leaf_count[first];
 }
 
-// One interior step: test both child boxes and push them as the reference does.
-__device__ __forceinline__ void InteriorStep(const DevNode* __restrict__ nodes, int32_t code, const RayPre& r,
-                                             float t_min, float t_max, uint32_t& nt, Stack& st) {
-    const DevNode nd = nodes[code];
+// One interior step on a fetched node record (a, b, c = both child boxes,
+// d = child codes): test both boxes and push the children as the reference
+// does (far first, so the near child is popped next; ties go left).
+__device__ __forceinline__ void InteriorStep(const float4& na, const float4& nb, const float4& nc, const float4& nd,
+                                             const RayPre& r, float t_min, float t_max, uint32_t& nt, Stack& st) {
     nt += 2;
     float el, er;
-    const bool hl = Slab(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, t_min, t_max, el);
-    const bool hr = Slab(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, t_min, t_max, er);
+    const bool hl = Slab(r, na.x, na.y, na.z, na.w, nb.x, nb.y, t_min, t_max, el);
+    const bool hr = Slab(r, nb.z, nb.w, nc.x, nc.y, nc.z, nc.w, t_min, t_max, er);
+    const int32_t cl = __float_as_int(nd.x);
+    const int32_t cr = __float_as_int(nd.y);
     if (hl && hr) {
         if (el <= er) {
-            st.Push(nd.d.y, er);
-            st.Push(nd.d.x, el);
+            st.Push(cr, er);
+            st.Push(cl, el);
         } else {
-            st.Push(nd.d.x, el);
-            st.Push(nd.d.y, er);
+            st.Push(cl, el);
+            st.Push(cr, er);
         }
     } else if (hl) {
-        st.Push(nd.d.x, el);
+        st.Push(cl, el);
     } else if (hr) {
-        st.Push(nd.d.y, er);
+        st.Push(cr, er);
     }
 }
+
+__device__ __forceinline__ void InteriorStep(const DevNode* __restrict__ nodes, int32_t code, const RayPre& r,
+                                             float t_min, float t_max, uint32_t& nt, Stack& st) {
+    const float4* q = reinterpret_cast<const float4*>(nodes + code);
+    InteriorStep(q[0], q[1], q[2], q[3], r, t_min, t_max, nt, st);
+}
+
+// The record a lane fetches in one traversal iteration: a 64-byte node or a
+// 48-byte triangle slot, loaded before either is processed, so a wave whose
+// lanes are split between interior nodes and leaves waits for one memory
+// latency per iteration instead of one per branch and per triangle.
+struct Fetch {
+    float4 q0, q1, q2, q3;
+    __device__ __forceinline__ void Load(const float4* p, bool node) {
+        q0 = p[0];
+        q1 = p[1];
+        q2 = p[2];
+        if (node) q3 = p[3];
+    }
+};
 
 __device__ __forceinline__ uint64_t LanesBelow(int lane) { return (1ull << lane) - 1ull; }
 
@@ -329,6 +352,7 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
     RayPre pre{};
     Best best{0, 0, 0, -1, false};
     uint32_t nt = 0, tt = 0;
+    int32_t cursor = 0, leaf_end = 0;  // triangle slots of the leaf being tested
 
     auto start = [&](const float4& o4, const float4& d4) {
         ox = o4.x; oy = o4.y; oz = o4.z; t_min = o4.w;
@@ -336,6 +360,7 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
         best = Best{d4.w, 0.0f, 0.0f, -1, false};
         nt = 0;
         tt = 0;
+        cursor = leaf_end = 0;
         pre = Precompute(ox, oy, oz, dx, dy, dz);
         if (!a.empty) {
             ++nt;
@@ -348,7 +373,7 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
     };
 
     while (true) {
-        if (st.sp == 0 && ray >= 0) {
+        if (st.sp == 0 && cursor >= leaf_end && ray >= 0) {
             sum_nt += nt;
             sum_tt += tt;
             float4 no, nd;
@@ -367,32 +392,49 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
             pol.Begin(ray, o4, d4);
             start(o4, d4);
         }
-        if (__ballot(st.sp > 0 || ray >= 0 || !exhausted) == 0) break;
+        if (__ballot(st.sp > 0 || cursor < leaf_end || ray >= 0 || !exhausted) == 0) break;
 
         while (true) {
-            if (st.sp > 0) {
+            // Pick this iteration's record: the next triangle of the current
+            // leaf, else the next stack entry (a popped leaf starts its first
+            // triangle in the same iteration; an entry beyond the best hit
+            // is dropped as the reference's pop-skip does).
+            bool node = false, tri = false;
+            const float4* src = nullptr;
+            if (cursor < leaf_end) {
+                tri = true;
+            } else if (st.sp > 0) {
                 const int2 e = st.Pop();
-                const int32_t code = e.x;
                 if (!(__int_as_float(e.y) > best.t)) {
-                    if (code < 0) {
+                    if (e.x < 0) {
                         int32_t first, count;
-                        DecodeLeaf(code, a.leaf_count, first, count);
-                        for (int32_t k = 0; k < count; ++k) {
-                            const float4* s = a.slots + 3 * static_cast<int64_t>(first + k);
-                            ++tt;
-                            if (TriangleTest(s[0], s[1], s[2], ox, oy, oz, dx, dy, dz, t_min, kAny, best) && kAny) {
-                                st.Truncate(0);
-                                break;
-                            }
-                        }
+                        DecodeLeaf(e.x, a.leaf_count, first, count);
+                        cursor = first;
+                        leaf_end = first + count;
+                        tri = count > 0;
                     } else {
-                        InteriorStep(a.nodes, code, pre, t_min, best.t, nt, st);
+                        node = true;
+                        src = reinterpret_cast<const float4*>(a.nodes + e.x);
                     }
                 }
             }
-            const uint64_t active = __ballot(st.sp > 0);
+            if (tri) src = a.slots + 3 * static_cast<int64_t>(cursor);
+            Fetch f;
+            if (node || tri) f.Load(src, node);
+            if (node) {
+                InteriorStep(f.q0, f.q1, f.q2, f.q3, pre, t_min, best.t, nt, st);
+            } else if (tri) {
+                ++tt;
+                ++cursor;
+                if (TriangleTest(f.q0, f.q1, f.q2, ox, oy, oz, dx, dy, dz, t_min, kAny, best) && kAny) {
+                    st.Truncate(0);
+                    cursor = leaf_end;
+                }
+            }
+            const bool lane_busy = cursor < leaf_end || st.sp > 0;
+            const uint64_t active = __ballot(lane_busy);
             if (active == 0) break;
-            const uint64_t idle = __ballot(st.sp == 0 && (ray >= 0 || !exhausted));
+            const uint64_t idle = __ballot(!lane_busy && (ray >= 0 || !exhausted));
             if (__popcll(idle) >= kRefillIdleLanes) break;
         }
     }
@@ -426,6 +468,8 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
     Best bbest{0, 0, 0, -1, false};
     // TLAS leaf being expanded
     int32_t cursor = 0, leaf_end = 0;
+    // triangle slots of the BLAS leaf being tested
+    int32_t tcur = 0, tend = 0;
 
     auto start = [&](const float4& o4, const float4& d4) {
         ox = o4.x; oy = o4.y; oz = o4.z; t_min = o4.w;
@@ -436,6 +480,7 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
         tt = 0;
         in_blas = false;
         cursor = leaf_end = 0;
+        tcur = tend = 0;
         pre = Precompute(ox, oy, oz, dx, dy, dz);
         if (!a.empty) {
             ++nt;
@@ -471,7 +516,14 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
         if (__ballot(in_blas || cursor < leaf_end || st.sp > 0 || ray >= 0 || !exhausted) == 0) break;
 
         while (true) {
-            if (in_blas && st.sp == blas_base) {
+            // One record per iteration, as in the flat loop: a BLAS leaf's
+            // triangles one by one, BLAS completion, the TLAS leaf's next
+            // instance, or a stack entry.
+            bool node = false, tri = false;
+            const float4* src = nullptr;
+            if (in_blas && tcur < tend) {
+                tri = true;
+            } else if (in_blas && st.sp == blas_base) {
                 // BLAS finished: accept only a strictly closer hit (two_level_bvh.cpp:474);
                 // an occlusion query accepts any hit within [t_min, t_max].
                 if (bbest.hit && (kAny || bbest.t < best.t)) {
@@ -511,17 +563,12 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
                         if (code < 0) {
                             int32_t first, count;
                             DecodeLeaf(code, a.leaf_count, first, count);
-                            for (int32_t k = 0; k < count; ++k) {
-                                const float4* s = a.slots + 3 * static_cast<int64_t>(first + k);
-                                ++tt;
-                                if (TriangleTest(s[0], s[1], s[2], bx, by, bz, bdx, bdy, bdz, t_min, kAny, bbest) &&
-                                    kAny) {
-                                    break;
-                                }
-                            }
-                            if (kAny && bbest.hit) st.Truncate(blas_base);  // finish this BLAS now
+                            tcur = first;
+                            tend = first + count;
+                            tri = count > 0;
                         } else {
-                            InteriorStep(a.nodes, code, bpre, t_min, bbest.t, nt, st);
+                            node = true;
+                            src = reinterpret_cast<const float4*>(a.nodes + code);
                         }
                     }
                 } else if (!(entry > best.t)) {
@@ -529,8 +576,26 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
                         DecodeLeaf(code, a.tleaf_count, cursor, leaf_end);
                         leaf_end += cursor;
                     } else {
-                        InteriorStep(a.tnodes, code, pre, t_min, best.t, nt, st);
+                        node = true;
+                        src = reinterpret_cast<const float4*>(a.tnodes + code);
                     }
+                }
+            }
+            if (tri) src = a.slots + 3 * static_cast<int64_t>(tcur);
+            Fetch f;
+            if (node || tri) f.Load(src, node);
+            if (node) {
+                if (in_blas) {
+                    InteriorStep(f.q0, f.q1, f.q2, f.q3, bpre, t_min, bbest.t, nt, st);
+                } else {
+                    InteriorStep(f.q0, f.q1, f.q2, f.q3, pre, t_min, best.t, nt, st);
+                }
+            } else if (tri) {
+                ++tt;
+                ++tcur;
+                if (TriangleTest(f.q0, f.q1, f.q2, bx, by, bz, bdx, bdy, bdz, t_min, kAny, bbest) && kAny) {
+                    tcur = tend;
+                    st.Truncate(blas_base);  // finish this BLAS now
                 }
             }
             const bool lane_busy = in_blas || cursor < leaf_end || st.sp > 0;
