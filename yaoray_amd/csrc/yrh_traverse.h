@@ -98,27 +98,25 @@ __device__ __forceinline__ RayPre Precompute(float ox, float oy, float oz, float
     return r;
 }
 
-// IntersectBounds: comparison form keeps the interval on a 0*inf NaN.
+// IntersectBounds. The reference narrows the interval with comparisons
+// (t_min = an > t_min ? an : t_min, t_max = af < t_max ? af : t_max), which
+// keep the bound when a 0*inf product is NaN. fmaxf / fminf (IEEE maxNum /
+// minNum, v_max3_f32 / v_min3_f32 here) return the same value for every
+// input: a NaN operand yields the other operand, and the bounds themselves are
+// never NaN; only the sign of a zero result may differ, which no comparison
+// below can observe. Near / far planes are still chosen by direction sign, as
+// the reference does, not by min / max of the two plane distances (that
+// would differ on a NaN distance).
 __device__ __forceinline__ bool Slab(const RayPre& r, float lx, float ly, float lz, float hx, float hy, float hz,
                                      float t_min, float t_max, float& entry) {
-    float nb = r.nx ? hx : lx;
-    float fb = r.nx ? lx : hx;
-    float an = (nb - r.ox) * r.ix;
-    float af = (fb - r.ox) * r.ix;
-    t_min = an > t_min ? an : t_min;
-    t_max = af < t_max ? af : t_max;
-    nb = r.ny ? hy : ly;
-    fb = r.ny ? ly : hy;
-    an = (nb - r.oy) * r.iy;
-    af = (fb - r.oy) * r.iy;
-    t_min = an > t_min ? an : t_min;
-    t_max = af < t_max ? af : t_max;
-    nb = r.nz ? hz : lz;
-    fb = r.nz ? lz : hz;
-    an = (nb - r.oz) * r.iz;
-    af = (fb - r.oz) * r.iz;
-    t_min = an > t_min ? an : t_min;
-    t_max = af < t_max ? af : t_max;
+    const float anx = ((r.nx ? hx : lx) - r.ox) * r.ix;
+    const float afx = ((r.nx ? lx : hx) - r.ox) * r.ix;
+    const float any = ((r.ny ? hy : ly) - r.oy) * r.iy;
+    const float afy = ((r.ny ? ly : hy) - r.oy) * r.iy;
+    const float anz = ((r.nz ? hz : lz) - r.oz) * r.iz;
+    const float afz = ((r.nz ? lz : hz) - r.oz) * r.iz;
+    t_min = fmaxf(fmaxf(anx, any), fmaxf(anz, t_min));
+    t_max = fminf(fminf(afx, afy), fminf(afz, t_max));
     entry = t_min;
     return t_max >= t_min;
 }
@@ -210,6 +208,26 @@ struct Stack {
         if (sp < lds_levels) return lds[sp * lds_stride];
         return spill[static_cast<int64_t>(sp - lds_levels) * spill_stride];
     }
+    // The children of an interior node, pushed as the reference does: with
+    // both hit, the far one first and the near one (popped next) on top;
+    // ties go left. Branch-light form of the two Push calls, with the same
+    // stack contents; on overflow both are dropped (the launch then fails).
+    __device__ __forceinline__ void PushChildren(bool hl, bool hr, int32_t cl, float el, int32_t cr, float er) {
+        const int k = static_cast<int>(hl) + static_cast<int>(hr);
+        if (k == 0) return;
+        if (sp + k > capacity) {
+            overflow = true;
+            return;
+        }
+        const bool left_near = hl && (!hr || el <= er);
+        const int2 left = make_int2(cl, __float_as_int(el));
+        const int2 right = make_int2(cr, __float_as_int(er));
+        if (top_valid) Store(sp - 1, top);
+        if (k == 2) Store(sp, left_near ? right : left);
+        top = left_near ? left : right;
+        top_valid = true;
+        sp += k;
+    }
     // Drops every entry at or above `level`.
     __device__ __forceinline__ void Truncate(int level) {
         if (level < sp) {
@@ -234,21 +252,7 @@ __device__ __forceinline__ void InteriorStep(const float4& na, const float4& nb,
     float el, er;
     const bool hl = Slab(r, na.x, na.y, na.z, na.w, nb.x, nb.y, t_min, t_max, el);
     const bool hr = Slab(r, nb.z, nb.w, nc.x, nc.y, nc.z, nc.w, t_min, t_max, er);
-    const int32_t cl = __float_as_int(nd.x);
-    const int32_t cr = __float_as_int(nd.y);
-    if (hl && hr) {
-        if (el <= er) {
-            st.Push(cr, er);
-            st.Push(cl, el);
-        } else {
-            st.Push(cl, el);
-            st.Push(cr, er);
-        }
-    } else if (hl) {
-        st.Push(cl, el);
-    } else if (hr) {
-        st.Push(cr, er);
-    }
+    st.PushChildren(hl, hr, __float_as_int(nd.x), el, __float_as_int(nd.y), er);
 }
 
 __device__ __forceinline__ void InteriorStep(const DevNode* __restrict__ nodes, int32_t code, const RayPre& r,
@@ -455,6 +459,9 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
     int32_t ray = -1;
     bool exhausted = false;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, t_min = 0;
+    // Precomputed reciprocals of the ray being traversed: the world ray in the
+    // TLAS, the object ray inside a BLAS (recomputed, bit-identically, when
+    // the BLAS is left), so one set of registers serves both levels.
     RayPre pre{};
     Best best{0, 0, 0, -1, false};
     int32_t best_inst = -1;
@@ -464,7 +471,6 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
     int blas_base = 0;
     int32_t cur_inst = -1;
     float bx = 0, by = 0, bz = 0, bdx = 0, bdy = 0, bdz = 0;
-    RayPre bpre{};
     Best bbest{0, 0, 0, -1, false};
     // TLAS leaf being expanded
     int32_t cursor = 0, leaf_end = 0;
@@ -535,6 +541,7 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
                     }
                 }
                 in_blas = false;
+                pre = Precompute(ox, oy, oz, dx, dy, dz);
                 ++cursor;
             } else if (!in_blas && cursor < leaf_end) {
                 // Next instance of the TLAS leaf: object ray, BLAS root box.
@@ -542,11 +549,12 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
                 const DevInstance m = a.instances[cur_inst];
                 const DevBlasRoot root = a.blas_roots[m.blas.x];
                 TransformRay(m, ox, oy, oz, dx, dy, dz, bx, by, bz, bdx, bdy, bdz);
-                bpre = Precompute(bx, by, bz, bdx, bdy, bdz);
+                const RayPre bpre = Precompute(bx, by, bz, bdx, bdy, bdz);
                 ++nt;
                 float entry;
                 if (Slab(bpre, root.lo.x, root.lo.y, root.lo.z, root.hi.x, root.hi.y, root.hi.z, t_min, best.t,
                          entry)) {
+                    pre = bpre;
                     bbest = Best{best.t, 0.0f, 0.0f, -1, false};
                     blas_base = st.sp;
                     st.Push(__float_as_int(root.lo.w), entry);
@@ -585,11 +593,7 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
             Fetch f;
             if (node || tri) f.Load(src, node);
             if (node) {
-                if (in_blas) {
-                    InteriorStep(f.q0, f.q1, f.q2, f.q3, bpre, t_min, bbest.t, nt, st);
-                } else {
-                    InteriorStep(f.q0, f.q1, f.q2, f.q3, pre, t_min, best.t, nt, st);
-                }
+                InteriorStep(f.q0, f.q1, f.q2, f.q3, pre, t_min, in_blas ? bbest.t : best.t, nt, st);
             } else if (tri) {
                 ++tt;
                 ++tcur;
