@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--lds-levels", type=int, default=0, help="traversal stack levels kept in LDS (1..16)")
     ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--trace-order", choices=("origin", "input"), default="origin",
+                    help="order in which the ray-cast kernels claim rays (origin cells, or input order)")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="also run a device copy of this size (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -145,6 +147,8 @@ def main():
         dev.set_trace_config(args.blocks_per_cu, args.block_size)
     if args.lds_levels:
         dev.set_trace_lds_levels(args.lds_levels)
+    from yaoray_amd import native as N
+    dev.set_trace_order(N.TRACE_ORDER_ORIGIN if args.trace_order == "origin" else N.TRACE_ORDER_INPUT)
     dev.upload_scene(scene)
     d_org, d_dir = dev.to_device(org), dev.to_device(dirs)
     d_hit, d_inst = dev.alloc(16 * n), dev.alloc(4 * n)
@@ -193,14 +197,25 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy of {args.calibration_copy_mb} MiB done")
 
+    # Phase split of the same call (ray-ordering pass, traversal kernel), from
+    # events the library records around each on the context stream.
+    dev.set_trace_timing(True)
+    phases = []
+    for _ in range(args.steps):
+        closest()
+        phases.append(dev.trace_phase_ms())
+    dev.set_trace_timing(False)
+    order_ms = float(np.mean([p[0] for p in phases]))
+    trace_ms = float(np.mean([p[1] for p in phases]))
+
     value = n * args.steps / wall / 1e6
     avg_closest_s = float(np.mean(ms_closest)) / 1e3
     avg_any_s = float(np.mean(ms_any)) / 1e3
     b_closest = algorithmic_bytes(n, nt_closest.sum(), tt_closest.sum(), 16)
     b_any = algorithmic_bytes(n, nt_any, tt_any, 4)
-    achieved = b_closest / avg_closest_s / 1e9
-    log(f"closest: {value:.1f} Mrays/s wall, kernel {avg_closest_s * 1e3:.3f} ms, {achieved:.0f} GB/s algorithmic; "
-        f"any: {n / avg_any_s / 1e6:.1f} Mrays/s")
+    achieved = b_closest / (trace_ms / 1e3) / 1e9
+    log(f"closest: {value:.1f} Mrays/s wall, call {avg_closest_s * 1e3:.3f} ms (ordering {order_ms:.3f} ms, "
+        f"traversal kernel {trace_ms:.3f} ms, {achieved:.0f} GB/s algorithmic); any: {n / avg_any_s / 1e6:.1f} Mrays/s")
 
     cpu = None
     parity = None
@@ -244,6 +259,7 @@ def main():
             "device": name,
             "compute_units": cus,
             "parallelism": "single GPU, persistent lanes with work-counter refill",
+            "ray_order": args.trace_order,
         },
         "roofline": {
             "bound": "hbm",
@@ -254,7 +270,9 @@ def main():
             "traffic": traffic,
             "kernel": "TraceFlatKernel<false> (closest hit)",
             "algorithmic_bytes_per_launch": b_closest,
-            "avg_kernel_ms": round(avg_closest_s * 1e3, 4),
+            "avg_kernel_ms": round(trace_ms, 4),
+            "avg_call_ms": round(avg_closest_s * 1e3, 4),
+            "avg_ordering_ms": round(order_ms, 4),
             "bytes_model": "32 B ray + 16 B hit + 32 B per node_test + 36 B per triangle_test",
             "mean_node_tests": round(float(nt_closest.mean()), 3),
             "mean_triangle_tests": round(float(tt_closest.mean()), 3),

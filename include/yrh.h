@@ -215,6 +215,17 @@ int yrh_set_trace_config(yrh_context* ctx, int blocks_per_cu, int block_size);
  * a CU (by default as many as the LDS allows, at most 8). */
 int yrh_set_trace_lds_levels(yrh_context* ctx, int levels);
 
+/* Order in which yrh_trace_closest / yrh_trace_any claim their rays.
+ * YRH_TRACE_ORDER_ORIGIN (the default): launches of at least 65536 rays first
+ * sort the ray indices by the Morton code of the origin's cell in the scene
+ * box (16 cells per axis; rays of one cell keep their input order, e.g. a
+ * camera's pixel order), so rays starting close together share a wave and
+ * its cached nodes; results still go to each ray's own index and are
+ * identical. YRH_TRACE_ORDER_INPUT: rays are claimed in input order. */
+#define YRH_TRACE_ORDER_INPUT 0
+#define YRH_TRACE_ORDER_ORIGIN 1
+int yrh_set_trace_order(yrh_context* ctx, int order);
+
 /* ---- Shading data ----------------------------------------------------------- */
 
 #define YRH_MATERIAL_DIFFUSE 0
@@ -460,6 +471,14 @@ int yrh_event_create(yrh_context* ctx, void** event);
 int yrh_event_destroy(yrh_context* ctx, void* event);
 int yrh_event_record(yrh_context* ctx, void* event);
 int yrh_event_elapsed_ms(yrh_context* ctx, void* start, void* stop, float* ms);
+
+/* Phase timing of the ray-cast calls (off by default). With timing on, each
+ * yrh_trace_closest / yrh_trace_any records events on the context's stream
+ * around its ray-ordering pass and around its traversal kernel;
+ * yrh_trace_phase_ms waits for the last such call and returns both durations
+ * in milliseconds (order_ms = 0 when the call ran no ordering pass). */
+int yrh_set_trace_timing(yrh_context* ctx, int enable);
+int yrh_trace_phase_ms(yrh_context* ctx, float* order_ms, float* trace_ms);
 
 #if defined(__GNUC__)
 #pragma GCC visibility pop

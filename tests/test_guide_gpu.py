@@ -173,8 +173,10 @@ def test_guided_render_keeps_closed_form(device, mode):
 def test_guided_cornell_matches_plain_mean(device):
     """Guided and plain renders have the same expectation. Per-pixel sample
     variances understate the error of scrambled-Sobol pixel means, so the
-    error is estimated from paired differences over independent sampler seeds
-    (a t statistic with 5 degrees of freedom)."""
+    error is estimated from paired differences over 16 independent sampler
+    seeds (a t statistic with 15 degrees of freedom, two-sided 0.1 % bound:
+    the paired differences are heavy-tailed, and fewer seeds made the
+    bound trip on unbiased runs). A 1 % bias gives |t| above 10 here."""
     s = Scene.cornell(1)
     s.build_accel()
     w, h, spp = 64, 64, 16
@@ -190,7 +192,7 @@ def test_guided_cornell_matches_plain_mean(device):
         device.guide_set_mode(N.GUIDE_OFF)
     lum = np.array([0.2126, 0.7152, 0.0722])
     diffs, means = [], []
-    for seed in range(6):
+    for seed in range(16):
         plain, _ = device.render(s, w, h, spp, max_depth=8, upload=False, seed=seed)
         device.guide_set_mode(N.GUIDE_SAMPLE)
         try:
@@ -205,5 +207,5 @@ def test_guided_cornell_matches_plain_mean(device):
     t = diffs.mean() / (diffs.std(ddof=1) / np.sqrt(len(diffs)))
     print(f"cornell guided - plain: {diffs.mean():.5f} +- {diffs.std(ddof=1) / np.sqrt(len(diffs)):.5f} "
           f"(plain {np.mean(means):.5f}), t = {t:.2f}")
-    assert abs(t) < 4.03  # two-sided 1 % for 5 degrees of freedom
+    assert abs(t) < 4.073  # two-sided 0.1 % for 15 degrees of freedom
     assert abs(diffs.mean()) < 0.01 * np.mean(means)

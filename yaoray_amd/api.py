@@ -399,6 +399,19 @@ class Device:
     def set_trace_lds_levels(self, levels: int):
         self._check(self._lib.yrh_set_trace_lds_levels(self._ctx, levels), "set_trace_lds_levels")
 
+    def set_trace_timing(self, enable: bool):
+        self._check(self._lib.yrh_set_trace_timing(self._ctx, int(bool(enable))), "set_trace_timing")
+
+    def trace_phase_ms(self):
+        """(ordering ms, traversal ms) of the last timed ray-cast call (waits for it)."""
+        a, b = C.c_float(), C.c_float()
+        self._check(self._lib.yrh_trace_phase_ms(self._ctx, C.byref(a), C.byref(b)), "trace_phase_ms")
+        return a.value, b.value
+
+    def set_trace_order(self, order: int):
+        """N.TRACE_ORDER_ORIGIN (default) or N.TRACE_ORDER_INPUT for the ray-cast calls."""
+        self._check(self._lib.yrh_set_trace_order(self._ctx, order), "set_trace_order")
+
     def set_trace_config(self, blocks_per_cu=0, block_size=0):
         self._check(self._lib.yrh_set_trace_config(self._ctx, blocks_per_cu, block_size), "set_trace_config")
 

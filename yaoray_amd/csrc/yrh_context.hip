@@ -238,6 +238,9 @@ void yrh_destroy(yrh_context* ctx) {
     ctx->geo = DevGeometry{};
     ctx->work_counter.reset();
     ctx->stack_spill.reset();
+    for (hipEvent_t e : ctx->phase_events) {
+        if (e != nullptr) (void)hipEventDestroy(e);
+    }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -308,6 +311,28 @@ int yrh_set_trace_lds_levels(yrh_context* ctx, int levels) {
     if (levels < 0 || levels > 16) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "lds levels must be in [0, 16]");
     ctx->lds_levels = levels;
     return YRH_OK;
+}
+
+int yrh_set_trace_order(yrh_context* ctx, int order) {
+    if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
+    if (order != YRH_TRACE_ORDER_INPUT && order != YRH_TRACE_ORDER_ORIGIN) {
+        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "trace order must be YRH_TRACE_ORDER_INPUT or YRH_TRACE_ORDER_ORIGIN");
+    }
+    ctx->trace_order = order;
+    return YRH_OK;
+}
+
+int yrh_set_trace_timing(yrh_context* ctx, int enable) {
+    if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
+    ctx->trace_timing = enable != 0;
+    return YRH_OK;
+}
+
+int yrh_trace_phase_ms(yrh_context* ctx, float* order_ms, float* trace_ms) {
+    if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
+    if (order_ms == nullptr || trace_ms == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null output");
+    (void)hipSetDevice(ctx->device);
+    return TracePhaseMs(ctx, order_ms, trace_ms);
 }
 
 int yrh_event_create(yrh_context* ctx, void** event) {

@@ -139,6 +139,16 @@ struct yrh_context {
     int blocks_per_cu = 0;
     int block_size = 256;
     int lds_levels = 0;  // traversal stack levels kept in LDS per lane
+    // Ray-cast launches of at least kOrderMinRays rays claim their rays in
+    // origin-cell order (yrh_order.hip) unless switched off.
+    int trace_order = YRH_TRACE_ORDER_ORIGIN;
+    yrh::DevBuf ray_order;          // int32 per ray: claim position -> ray index
+    yrh::DevBuf ray_order_scratch;  // cell histograms and scan tile sums
+    // Phase timing of ray-cast calls: events before the ordering pass, before
+    // and after the traversal kernel of the last call.
+    bool trace_timing = false;
+    bool last_call_ordered = false;
+    hipEvent_t phase_events[3] = {nullptr, nullptr, nullptr};
     yrh::RenderState* render = nullptr;  // surface/light data, film, path state (yrh_render.hip)
     yrh::GuideState* guide = nullptr;    // learned guiding proposal (yrh_guide.hip)
 };
@@ -164,8 +174,18 @@ struct TraceRequest {
     uint32_t* triangle_tests = nullptr;
     uint32_t* occluded = nullptr;
     unsigned long long* totals = nullptr;  // [0] node tests, [1] triangle tests
+    const int32_t* order = nullptr;        // claim position -> ray index (null = identity)
 };
 int LaunchTrace(yrh_context* ctx, const TraceRequest& r);
+
+// Order of n rays by the Morton code of their origin's cell in the scene box
+// (yrh_order.hip), input order kept within a cell except among one wave's
+// rays of mixed cells; the order lives in a context buffer that stays valid
+// until the next call.
+constexpr int64_t kOrderMinRays = 1 << 16;
+int OrderRaysByOrigin(yrh_context* ctx, const float4* org, int64_t n, const int32_t** order);
+// Durations of the last timed ray-cast call's phases (yrh_trace.hip).
+int TracePhaseMs(yrh_context* ctx, float* order_ms, float* trace_ms);
 
 // Render-side state (yrh_render.hip): shading copy of the geometry, taken at
 // geometry upload; released with the context.

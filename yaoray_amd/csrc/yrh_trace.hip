@@ -103,6 +103,7 @@ int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a
         a.counter = words + ctx->launch_parity;
         a.next_counter = words + (1 - ctx->launch_parity);
     }
+    a.order = r.order;
     a.overflow = words + 2;
     a.spill = ctx->stack_spill.as<int2>();
     a.spill_stride = total;
@@ -140,6 +141,29 @@ int LaunchTrace(yrh_context* ctx, const TraceRequest& r) {
     return rc;
 }
 
+// Ray-cast launches claim large batches in origin-cell order; with phase
+// timing on, events bracket the ordering pass and the traversal kernel.
+int LaunchRayCast(yrh_context* ctx, TraceRequest& r) {
+    const bool order = ctx->trace_order == YRH_TRACE_ORDER_ORIGIN && r.n_host >= kOrderMinRays && Aligned16(r.org);
+    int rc = YRH_OK;
+    if (ctx->trace_timing) {
+        for (hipEvent_t& e : ctx->phase_events) {
+            if (e == nullptr && (rc = CheckHip(ctx, hipEventCreate(&e), "hipEventCreate")) != YRH_OK) return rc;
+        }
+        if ((rc = CheckHip(ctx, hipEventRecord(ctx->phase_events[0], ctx->stream), "hipEventRecord")) != YRH_OK) return rc;
+    }
+    if (order && (rc = OrderRaysByOrigin(ctx, r.org, r.n_host, &r.order)) != YRH_OK) return rc;
+    if (ctx->trace_timing) {
+        if ((rc = CheckHip(ctx, hipEventRecord(ctx->phase_events[1], ctx->stream), "hipEventRecord")) != YRH_OK) return rc;
+    }
+    if ((rc = LaunchTrace(ctx, r)) != YRH_OK) return rc;
+    if (ctx->trace_timing) {
+        if ((rc = CheckHip(ctx, hipEventRecord(ctx->phase_events[2], ctx->stream), "hipEventRecord")) != YRH_OK) return rc;
+        ctx->last_call_ordered = order;
+    }
+    return YRH_OK;
+}
+
 int LaunchTraceClosest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, const yrh_hit_soa* hits) {
     TraceRequest r;
     r.any = false;
@@ -150,7 +174,7 @@ int LaunchTraceClosest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, con
     r.instance = hits->instance;
     r.node_tests = hits->node_tests;
     r.triangle_tests = hits->triangle_tests;
-    return LaunchTrace(ctx, r);
+    return LaunchRayCast(ctx, r);
 }
 
 int LaunchTraceAny(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, uint32_t* occluded, uint32_t* node_tests,
@@ -163,7 +187,19 @@ int LaunchTraceAny(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, uint32_
     r.occluded = occluded;
     r.node_tests = node_tests;
     r.triangle_tests = triangle_tests;
-    return LaunchTrace(ctx, r);
+    return LaunchRayCast(ctx, r);
+}
+
+int TracePhaseMs(yrh_context* ctx, float* order_ms, float* trace_ms) {
+    if (ctx->phase_events[2] == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "no timed ray-cast call yet");
+    int rc = CheckHip(ctx, hipEventSynchronize(ctx->phase_events[2]), "hipEventSynchronize");
+    if (rc != YRH_OK) return rc;
+    float a = 0.0f, b = 0.0f;
+    if ((rc = CheckHip(ctx, hipEventElapsedTime(&a, ctx->phase_events[0], ctx->phase_events[1]), "hipEventElapsedTime")) != YRH_OK) return rc;
+    if ((rc = CheckHip(ctx, hipEventElapsedTime(&b, ctx->phase_events[1], ctx->phase_events[2]), "hipEventElapsedTime")) != YRH_OK) return rc;
+    *order_ms = ctx->last_call_ordered ? a : 0.0f;
+    *trace_ms = b;
+    return YRH_OK;
 }
 
 }  // namespace yrh

@@ -71,6 +71,7 @@ struct TraceArgs {
     // scheduling
     uint32_t* counter;       // this launch's ray counter (zero at launch)
     uint32_t* next_counter;  // the next launch's counter, zeroed here (may be null)
+    const int32_t* order;    // claim position -> ray index (null = identity)
     uint32_t* overflow;      // stack overflow flag
     int2* spill;
     int64_t spill_stride;
@@ -392,6 +393,10 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
         const int32_t claimed = ClaimRay(a.counter, want, lane, n, exhausted);
         if (claimed >= 0) {
             ray = claimed;
+            if (a.order != nullptr) {
+                const int32_t mapped = a.order[claimed];
+                if (static_cast<uint32_t>(mapped) < static_cast<uint32_t>(n)) ray = mapped;
+            }
             float4 o4, d4;
             pol.Begin(ray, o4, d4);
             start(o4, d4);
@@ -515,6 +520,10 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
         const int32_t claimed = ClaimRay(a.counter, want, lane, n, exhausted);
         if (claimed >= 0) {
             ray = claimed;
+            if (a.order != nullptr) {
+                const int32_t mapped = a.order[claimed];
+                if (static_cast<uint32_t>(mapped) < static_cast<uint32_t>(n)) ray = mapped;
+            }
             float4 o4, d4;
             pol.Begin(ray, o4, d4);
             start(o4, d4);

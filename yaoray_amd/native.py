@@ -119,6 +119,8 @@ class RenderStats(C.Structure):
                                             "sphere_tests", "bvh_node_tests", "hits", "misses", "bad_samples")]
 
 
+TRACE_ORDER_INPUT = 0
+TRACE_ORDER_ORIGIN = 1
 SAMPLER_INDEPENDENT = 0
 SAMPLER_STRATIFIED = 1
 SAMPLER_OWEN_SOBOL = 2
@@ -212,6 +214,9 @@ YRH_SIGNATURES = {
     "yrh_trace_any": (_I, [_VP, C.POINTER(RaySoa), _I64, _VP, _VP, _VP]),
     "yrh_set_trace_config": (_I, [_VP, _I, _I]),
     "yrh_set_trace_lds_levels": (_I, [_VP, _I]),
+    "yrh_set_trace_order": (_I, [_VP, _I]),
+    "yrh_set_trace_timing": (_I, [_VP, _I]),
+    "yrh_trace_phase_ms": (_I, [_VP, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "yrh_event_create": (_I, [_VP, C.POINTER(_VP)]),
     "yrh_event_destroy": (_I, [_VP, _VP]),
     "yrh_event_record": (_I, [_VP, _VP]),
