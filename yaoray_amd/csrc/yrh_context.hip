@@ -213,13 +213,13 @@ int yrh_create(int device, yrh_context** out) {
         std::strncpy(ctx->device_name, prop.gcnArchName, sizeof(ctx->device_name) - 1);
     }
     if (ctx->compute_units <= 0) ctx->compute_units = 256;
-    if (hipMalloc(&ctx->work_counter.ptr, 256) != hipSuccess) {
+    if (hipMalloc(&ctx->work_counter.ptr, kWorkCounterBytes) != hipSuccess) {
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return YRH_E_HIP;
     }
-    ctx->work_counter.bytes = 256;
-    if (hipMemset(ctx->work_counter.ptr, 0, 256) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    ctx->work_counter.bytes = kWorkCounterBytes;
+    if (hipMemset(ctx->work_counter.ptr, 0, kWorkCounterBytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
         ctx->work_counter.reset();
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;

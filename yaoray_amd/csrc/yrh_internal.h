@@ -28,6 +28,10 @@
 namespace yrh {
 
 constexpr int kLeafCountEscape = 15;
+// Context work-counter words (yrh_context::work_counter).
+constexpr int kCounterSetBase = 256;
+constexpr int kCounterSetWords = 256;  // 8 ranges x 32 words
+constexpr size_t kWorkCounterBytes = (kCounterSetBase + 2 * kCounterSetWords) * sizeof(uint32_t);
 constexpr int32_t kMaxSlot = (1 << 27) - 1;
 
 struct alignas(16) DevNode {
@@ -128,8 +132,9 @@ struct yrh_context {
     std::string error;
     yrh::DevGeometry geo;
     // persistent traversal scratch
-    // uint32 words: [0], [1] ray counters used by alternate launches (each
-    // launch zeroes the other one for its successor), [2] stack overflow flag
+    // uint32 words: [2] stack overflow flag; from kCounterSetBase two sets of
+    // per-range ray counters (kCounterSetWords each) used by alternate
+    // launches, each launch zeroing the other set for its successor
     yrh::DevBuf work_counter;
     int launch_parity = 0;
     yrh::DevBuf stack_spill;    // int2 per lane per spilled stack level

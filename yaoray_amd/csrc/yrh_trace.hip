@@ -8,12 +8,10 @@ namespace {
 using trav::kMaxBlock;
 using trav::TraceArgs;
 
-// The flat kernels compile to at most 64 VGPRs (8 waves per SIMD), so with 8
-// workgroups of 256 per CU every workgroup of the persistent grid is resident
-// at once. (A min-waves launch bound asking for the same limit makes this
-// ROCm 7.2 compiler fail instruction verification on the closest-hit kernel.)
+// 8 waves per SIMD (at most 64 VGPRs and 100 SGPRs): with 8 workgroups of 256
+// per CU every workgroup of the persistent grid is resident at once.
 template <bool kAny>
-__global__ __launch_bounds__(kMaxBlock) void TraceFlatKernel(TraceArgs a) {
+__global__ __launch_bounds__(kMaxBlock, 8) void TraceFlatKernel(TraceArgs a) {
     extern __shared__ int2 lds_stack[];
     const trav::RayArrayPolicy<kAny> pol{&a};
     trav::TraceFlatLoop<kAny>(a, pol, lds_stack);
@@ -99,9 +97,13 @@ int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a
     if (r.work_counter != nullptr) {
         a.counter = r.work_counter;
         a.next_counter = nullptr;
+        a.parts = 1;
     } else {
-        a.counter = words + ctx->launch_parity;
-        a.next_counter = words + (1 - ctx->launch_parity);
+        // Two sets of per-range counters; each launch zeroes the other set.
+        uint32_t* sets = words + kCounterSetBase;
+        a.counter = sets + ctx->launch_parity * kCounterSetWords;
+        a.next_counter = sets + (1 - ctx->launch_parity) * kCounterSetWords;
+        a.parts = trav::kParts;
     }
     a.order = r.order;
     a.overflow = words + 2;

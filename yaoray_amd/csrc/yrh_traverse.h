@@ -70,7 +70,8 @@ struct TraceArgs {
     const DevBlasRoot* blas_roots;
     // scheduling
     uint32_t* counter;       // this launch's ray counter (zero at launch)
-    uint32_t* next_counter;  // the next launch's counter, zeroed here (may be null)
+    uint32_t* next_counter;  // the next launch's counter(s), zeroed here (may be null)
+    int32_t parts;           // 1: one counter; kParts: per-range counters kPartStride words apart
     const int32_t* order;    // claim position -> ray index (null = identity)
     uint32_t* overflow;      // stack overflow flag
     int2* spill;
@@ -290,6 +291,47 @@ __device__ __forceinline__ void AddTotals(unsigned long long* totals, uint64_t n
     }
 }
 
+// Ray claiming split into kParts ranges of the claim order, one counter each
+// (kPartStride words apart, so no two share a cache line). Workgroups are
+// dispatched to the 8 XCDs round-robin, so workgroup b starts on range
+// b % kParts: each XCD works through its own contiguous stretch of the
+// (origin-sorted) rays and its L2 keeps the nodes those rays share. A wave
+// whose range runs out moves on to the next range, so every ray is taken.
+constexpr int kParts = 8;
+constexpr int kPartStride = 32;
+
+__device__ __forceinline__ int64_t PartBegin(int part, int32_t n) {
+    return static_cast<int64_t>(part) * n / kParts;
+}
+
+__device__ __forceinline__ int32_t ClaimRayParts(uint32_t* counters, int& part_state, bool want, int lane, int32_t n,
+                                                 bool& exhausted) {
+    int part = part_state & 0xFF;
+    int parts_left = part_state >> 8;
+    uint64_t mask = __ballot(want);
+    int32_t result = -1;
+    while (mask != 0 && parts_left > 0) {
+        const int64_t lo = PartBegin(part, n);
+        const int64_t size = PartBegin(part + 1, n) - lo;
+        const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(counters + part * kPartStride, static_cast<uint32_t>(__popcll(mask)));
+        base = __shfl(base, leader);
+        if ((mask >> lane) & 1ull) {
+            const uint64_t mine = static_cast<uint64_t>(base) + static_cast<uint64_t>(__popcll(mask & LanesBelow(lane)));
+            if (mine < static_cast<uint64_t>(size)) result = static_cast<int32_t>(lo + static_cast<int64_t>(mine));
+        }
+        mask = __ballot(want && result < 0);
+        if (mask != 0) {  // this range is used up
+            part = part + 1 == kParts ? 0 : part + 1;
+            --parts_left;
+        }
+    }
+    part_state = part | (parts_left << 8);
+    if (want && result < 0) exhausted = true;
+    return result;
+}
+
 // Claims one ray index per wanting lane with a single atomic per wave.
 __device__ __forceinline__ int32_t ClaimRay(uint32_t* counter, bool want, int lane, int32_t n, bool& exhausted) {
     const uint64_t mask = __ballot(want);
@@ -348,9 +390,13 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
     Stack st{lds_stack + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
              a.capacity, a.lds_levels, 0, false, false, make_int2(0, 0)};
 
-    if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.next_counter = 0u;
+    if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x < static_cast<unsigned>(a.parts)) {
+        a.next_counter[threadIdx.x * (a.parts > 1 ? kPartStride : 1)] = 0u;
+    }
     const int32_t n = a.n_device != nullptr ? static_cast<int32_t>(*a.n_device) : a.n;
-    uint64_t sum_nt = 0, sum_tt = 0;
+    // Claim range (low byte) and ranges left to try (next byte), wave-uniform.
+    int part_state = static_cast<int>(blockIdx.x % kParts) | (kParts << 8);
+    uint32_t sum_nt = 0, sum_tt = 0;  // this lane's test counts since its last flush
     int32_t ray = -1;
     bool exhausted = false;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, t_min = 0;
@@ -381,6 +427,11 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
         if (st.sp == 0 && cursor >= leaf_end && ray >= 0) {
             sum_nt += nt;
             sum_tt += tt;
+            if (a.totals != nullptr && (sum_nt | sum_tt) >= 0x80000000u) {  // keep 32-bit sums from wrapping
+                atomicAdd(a.totals, static_cast<unsigned long long>(sum_nt));
+                atomicAdd(a.totals + 1, static_cast<unsigned long long>(sum_tt));
+                sum_nt = sum_tt = 0;
+            }
             float4 no, nd;
             if (pol.Finish(ray, best, -1, nt, tt, make_float4(ox, oy, oz, t_min), make_float4(dx, dy, dz, 0.0f), no,
                            nd)) {
@@ -390,7 +441,8 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
             }
         }
         const bool want = ray < 0 && !exhausted;
-        const int32_t claimed = ClaimRay(a.counter, want, lane, n, exhausted);
+        const int32_t claimed = a.parts > 1 ? ClaimRayParts(a.counter, part_state, want, lane, n, exhausted)
+                                            : ClaimRay(a.counter, want, lane, n, exhausted);
         if (claimed >= 0) {
             ray = claimed;
             if (a.order != nullptr) {
@@ -458,9 +510,13 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
     Stack st{lds_stack + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
              a.capacity, a.lds_levels, 0, false, false, make_int2(0, 0)};
 
-    if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.next_counter = 0u;
+    if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x < static_cast<unsigned>(a.parts)) {
+        a.next_counter[threadIdx.x * (a.parts > 1 ? kPartStride : 1)] = 0u;
+    }
     const int32_t n = a.n_device != nullptr ? static_cast<int32_t>(*a.n_device) : a.n;
-    uint64_t sum_nt = 0, sum_tt = 0;
+    // Claim range (low byte) and ranges left to try (next byte), wave-uniform.
+    int part_state = static_cast<int>(blockIdx.x % kParts) | (kParts << 8);
+    uint32_t sum_nt = 0, sum_tt = 0;  // this lane's test counts since its last flush
     int32_t ray = -1;
     bool exhausted = false;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, t_min = 0;
@@ -508,6 +564,11 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
         if (!busy && ray >= 0) {
             sum_nt += nt;
             sum_tt += tt;
+            if (a.totals != nullptr && (sum_nt | sum_tt) >= 0x80000000u) {  // keep 32-bit sums from wrapping
+                atomicAdd(a.totals, static_cast<unsigned long long>(sum_nt));
+                atomicAdd(a.totals + 1, static_cast<unsigned long long>(sum_tt));
+                sum_nt = sum_tt = 0;
+            }
             float4 no, nd;
             if (pol.Finish(ray, best, best_inst, nt, tt, make_float4(ox, oy, oz, t_min),
                            make_float4(dx, dy, dz, 0.0f), no, nd)) {
@@ -517,7 +578,8 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
             }
         }
         const bool want = ray < 0 && !exhausted;
-        const int32_t claimed = ClaimRay(a.counter, want, lane, n, exhausted);
+        const int32_t claimed = a.parts > 1 ? ClaimRayParts(a.counter, part_state, want, lane, n, exhausted)
+                                            : ClaimRay(a.counter, want, lane, n, exhausted);
         if (claimed >= 0) {
             ray = claimed;
             if (a.order != nullptr) {
