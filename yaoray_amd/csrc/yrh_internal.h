@@ -172,7 +172,8 @@ struct TraceRequest {
     const uint32_t* n_device = nullptr;  // device-resident ray count (read by the kernel)
     int64_t capacity = 0;                // upper bound of *n_device
     int64_t expected = 0;                // typical live count (sizes the grid; 0 = capacity)
-    uint32_t* work_counter = nullptr;    // zeroed ray counter owned by the caller; null = context's
+    uint32_t* work_counter = nullptr;    // zeroed ray counter(s) owned by the caller; null = context's
+    int parts = 1;                       // caller's counters: 1, or trav::kParts ranges kPartStride words apart
     float4* hit = nullptr;
     int32_t* instance = nullptr;
     uint32_t* node_tests = nullptr;

@@ -39,9 +39,11 @@ enum Counter : int {
     kCountShadow = 2,
     kCountPassPaths = 3,  // paths of the current pass (batched passes)
     kPassIndex = 4,       // current pass of a batch schedule (advanced by EndPassKernel)
-    kWorkExtend = 5,      // per-launch ray counters of the trace kernels
-    kWorkShadow = 6,
-    kCounterWords = 7,
+    // Per-launch ray counters of the trace kernels: one per claim range
+    // (trav::kParts ranges, trav::kPartStride words apart).
+    kWorkExtend = 32,
+    kWorkShadow = kWorkExtend + trav::kParts * trav::kPartStride,
+    kCounterWords = kWorkShadow + trav::kParts * trav::kPartStride,
 };
 
 // 64-bit statistics words.
@@ -610,7 +612,10 @@ __global__ void BeginBounceKernel(uint32_t* counters, int next_queue) {
     const int i = static_cast<int>(threadIdx.x);
     if (i == 0) counters[next_queue] = 0u;
     if (i == 1) counters[kCountShadow] = 0u;
-    if (i >= 2 && i < 2 + (kCounterWords - kWorkExtend)) counters[kWorkExtend + (i - 2)] = 0u;
+    if (i >= 2 && i < 2 + 2 * trav::kParts) {
+        const int k = i - 2;
+        counters[(k < trav::kParts ? kWorkExtend : kWorkShadow) + (k % trav::kParts) * trav::kPartStride] = 0u;
+    }
 }
 
 __global__ void CopyTraceTotalsKernel(unsigned long long* stats) {
@@ -1453,6 +1458,7 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
         tr.capacity = paths;
         tr.expected = paths;
         tr.work_counter = P.counters + kWorkExtend;
+        tr.parts = trav::kParts;
         tr.hit = P.hits;
         tr.instance = ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL ? P.hit_inst : nullptr;
         tr.totals = P.stats + kStatTraceNodes;
@@ -1474,6 +1480,7 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
             sr.capacity = 2 * paths;
             sr.expected = paths;  // about one shadow ray per path in flight
             sr.work_counter = P.counters + kWorkShadow;
+            sr.parts = trav::kParts;
             sr.occluded = P.s_occ;
             sr.totals = P.stats + kStatTraceNodes;
             if ((rc = LaunchTrace(ctx, sr)) != YRH_OK) return rc;
@@ -1486,6 +1493,7 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
             wr.capacity = 2 * paths;
             wr.expected = paths;
             wr.work_counter = P.counters + kWorkShadow;
+            wr.parts = trav::kParts;
             wr.hit = P.s_trans;  // not written by the walk; LaunchTrace's checks want an output
             wr.totals = P.stats + kStatTraceNodes;
             trav::TraceArgs wa;

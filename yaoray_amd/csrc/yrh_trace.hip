@@ -97,7 +97,7 @@ int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a
     if (r.work_counter != nullptr) {
         a.counter = r.work_counter;
         a.next_counter = nullptr;
-        a.parts = 1;
+        a.parts = r.parts;
     } else {
         // Two sets of per-range counters; each launch zeroes the other set.
         uint32_t* sets = words + kCounterSetBase;
