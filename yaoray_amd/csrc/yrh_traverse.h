@@ -41,7 +41,7 @@ namespace trav {
 
 constexpr int kMaxLdsStack = 16;  // LDS stack levels per lane, at most (launch option)
 constexpr int kMaxBlock = 256;
-constexpr int kRefillIdleLanes = 16;
+constexpr int kRefillIdleLanes = 32;  // refill once half the wave is idle (swept: 8..56)
 
 struct TraceArgs {
     const float4* org;
