@@ -167,13 +167,18 @@ __device__ __forceinline__ bool TriangleTest(const float4& s0, const float4& s1,
     return true;
 }
 
+// LDS word of the traversal stack, in the LDS address space so that its
+// stores stay ds_write (a generic pointer lets the compiler merge them with
+// the spill stores into flat stores).
+typedef __attribute__((address_space(3))) uint64_t LdsWord;
+
 // Per-lane traversal stack. The top entry is held in registers: a push writes
 // the previous top to memory and a pop of the register top reads nothing, so
 // the common interior step (push far, push near, pop near) costs one LDS
 // write instead of two writes and a read. Entries 0 .. sp-1; when top_valid
 // the entry sp-1 is `top` and only 0 .. sp-2 are in memory.
 struct Stack {
-    int2* lds;  // this lane's column: lds[level * stride]
+    LdsWord* lds;  // this lane's column: lds[level * stride], (code, entry bits) packed in 64 bits
     int lds_stride;
     int2* spill;  // this lane's column in the global spill region
     int64_t spill_stride;
@@ -186,7 +191,8 @@ struct Stack {
 
     __device__ __forceinline__ void Store(int level, int2 e) {
         if (level < lds_levels) {
-            lds[level * lds_stride] = e;
+            lds[level * lds_stride] = static_cast<uint64_t>(static_cast<uint32_t>(e.x)) |
+                                      (static_cast<uint64_t>(static_cast<uint32_t>(e.y)) << 32);
         } else {
             spill[static_cast<int64_t>(level - lds_levels) * spill_stride] = e;
         }
@@ -207,7 +213,10 @@ struct Stack {
             top_valid = false;
             return top;
         }
-        if (sp < lds_levels) return lds[sp * lds_stride];
+        if (sp < lds_levels) {
+            const uint64_t w = lds[sp * lds_stride];
+            return make_int2(static_cast<int32_t>(static_cast<uint32_t>(w)), static_cast<int32_t>(w >> 32));
+        }
         return spill[static_cast<int64_t>(sp - lds_levels) * spill_stride];
     }
     // The children of an interior node, pushed as the reference does: with
@@ -387,7 +396,7 @@ template <bool kAny, class Policy>
 __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& pol, int2* lds_stack) {
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int64_t gtid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    Stack st{lds_stack + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
+    Stack st{(LdsWord*)(lds_stack) + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
              a.capacity, a.lds_levels, 0, false, false, make_int2(0, 0)};
 
     if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x < static_cast<unsigned>(a.parts)) {
@@ -507,7 +516,7 @@ template <bool kAny, class Policy>
 __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Policy& pol, int2* lds_stack) {
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int64_t gtid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    Stack st{lds_stack + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
+    Stack st{(LdsWord*)(lds_stack) + threadIdx.x, static_cast<int>(blockDim.x), a.spill + gtid, a.spill_stride,
              a.capacity, a.lds_levels, 0, false, false, make_int2(0, 0)};
 
     if (a.next_counter != nullptr && blockIdx.x == 0 && threadIdx.x < static_cast<unsigned>(a.parts)) {
