@@ -211,7 +211,11 @@ __global__ __launch_bounds__(kBlock) void GuideBackwardKernel(TrainArgs A) {
         float loss = 0.0f;
         HeadGradient(out, wi[0], wi[1], wi[2], c, d, loss);
         for (int j = 0; j < yrg::kGuideOut; ++j) lo[row * yrg::kGuideOut + j] = d[j];
-        if (valid && loss != 0.0f) atomicAdd(A.loss, loss);
+        // One atomic per wave for the loss (lanes 0..15 hold the 16 rows):
+        // every row adding to the one word serialised the whole step.
+        if (!valid) loss = 0.0f;
+        for (int off = 8; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+        if (lane == 0 && loss != 0.0f) atomicAdd(A.loss, loss);
     }
     __syncthreads();
     bf16x4_t dz3[3], dz2[4], dz1[4], dz0[4];
@@ -238,18 +242,42 @@ __global__ __launch_bounds__(kBlock) void GuideBackwardKernel(TrainArgs A) {
     // 4q..4q+3 of its row, i.e. its own encoding share (levels 2q, 2q+1).
     f32x4_t dx = yrg::Zero4();
     for (int kb = 0; kb < 4; ++kb) dx = yrg::Mfma(A.bwd[(yrg::kFragBase[0] + 0 * 4 + kb) * 64 + lane], dz0[kb], dx);
-    if (valid) {
+    // Hash-grid gradients. The 16 lanes of a quarter (lanes 16q..16q+15) hold
+    // 16 consecutive samples' shares of the same two levels; consecutive
+    // samples are neighbouring path vertices, so on the coarse levels they
+    // often touch the same corner entries. Where all 16 do (or add nothing),
+    // the quarter sums its contributions and adds them with one atomic per
+    // feature instead of 16.
+    {
+        const uint64_t quarter = 0xFFFFull << (16 * q);
+        const int quarter_lead = lane & ~15;
         for (int lv = 0; lv < 2; ++lv) {
-            const float ga = dx[2 * lv], gb = dx[2 * lv + 1];
-            if (ga == 0.0f && gb == 0.0f) continue;
             const int level = 2 * q + lv;
+            const float ga = valid ? dx[2 * lv] : 0.0f;
+            const float gb = valid ? dx[2 * lv + 1] : 0.0f;
             const yrg::GridCell cell = yrg::LocateCell(A.grid, level, p);
             float* g = A.table_grad + static_cast<size_t>(level) * A.grid.table_size * 2;
             for (int k = 0; k < 8; ++k) {
                 const float cw = yrg::CornerWeight(cell, k);
-                const uint32_t e = yrg::CornerIndex(A.grid, level, cell, k);
-                unsafeAtomicAdd(g + 2 * e, cw * ga);
-                unsafeAtomicAdd(g + 2 * e + 1, cw * gb);
+                float va = cw * ga, vb = cw * gb;
+                const bool adds = va != 0.0f || vb != 0.0f;
+                const uint32_t e = adds ? yrg::CornerIndex(A.grid, level, cell, k) : 0xFFFFFFFFu;
+                uint32_t e0 = e;
+                for (int off = 1; off < 16; off <<= 1) e0 = min(e0, static_cast<uint32_t>(__shfl_xor(static_cast<int>(e0), off)));
+                const uint64_t same = __ballot(!adds || e == e0);
+                if ((same & quarter) == quarter) {
+                    for (int off = 8; off > 0; off >>= 1) {
+                        va += __shfl_xor(va, off);
+                        vb += __shfl_xor(vb, off);
+                    }
+                    if (lane == quarter_lead && e0 != 0xFFFFFFFFu) {
+                        unsafeAtomicAdd(g + 2 * e0, va);
+                        unsafeAtomicAdd(g + 2 * e0 + 1, vb);
+                    }
+                } else if (adds) {
+                    unsafeAtomicAdd(g + 2 * e, va);
+                    unsafeAtomicAdd(g + 2 * e + 1, vb);
+                }
             }
         }
     }

@@ -1075,15 +1075,19 @@ __global__ __launch_bounds__(kBlock) void GuideCollectKernel(PassView P, yrg::Gu
     uint32_t base = 0;
     if (lane == leader) base = atomicAdd(G.sample_count, static_cast<uint32_t>(__popcll(mask)));
     base = __shfl(base, leader);
-    if (!keep) return;
     const uint32_t idx = base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
-    if (idx >= static_cast<uint32_t>(G.sample_capacity)) return;
-    float4* d = G.samples + 4 * static_cast<int64_t>(idx);
-    d[0] = F4(XYZ(r0), w);
-    d[1] = F4(XYZ(r1), 0.0f);
-    d[2] = F4(XYZ(r2), 0.0f);
-    d[3] = F4(XYZ(r3), 0.0f);
-    atomicAdd(G.sample_wsum, w);
+    const bool stored = keep && idx < static_cast<uint32_t>(G.sample_capacity);
+    if (stored) {
+        float4* d = G.samples + 4 * static_cast<int64_t>(idx);
+        d[0] = F4(XYZ(r0), w);
+        d[1] = F4(XYZ(r1), 0.0f);
+        d[2] = F4(XYZ(r2), 0.0f);
+        d[3] = F4(XYZ(r3), 0.0f);
+    }
+    // The weight sum of the stored samples: one atomic per wave, not per sample.
+    float ws = stored ? w : 0.0f;
+    for (int off = 32; off > 0; off >>= 1) ws += __shfl_xor(ws, off);
+    if (lane == leader && ws != 0.0f) atomicAdd(G.sample_wsum, ws);
 }
 
 // Adds the visible next-event contributions of this bounce, emissive then
