@@ -446,15 +446,19 @@ int yrh_upload_geometry(yrh_context* ctx, const yrh_geometry_desc* g) {
             return true;
         };
         if (!PackBvh(tdesc, tl, error, no_slot)) return Fail(ctx, YRH_E_INVALID_ARGUMENT, error);
-        std::vector<DevInstance> inst(static_cast<size_t>(g->instance_count));
-        for (int32_t i = 0; i < g->instance_count; ++i) {
+        std::vector<DevInstanceSlot> slots(resolved.size());
+        for (size_t s = 0; s < resolved.size(); ++s) {
+            const int32_t i = resolved[s];
             const float* m = g->instances[i].world_to_object;
-            DevInstance& d = inst[static_cast<size_t>(i)];
+            const DevBlasRoot& root = roots[static_cast<size_t>(g->instances[i].blas_index)];
+            DevInstanceSlot& d = slots[s];
             d.r0 = make_float4(m[0], m[4], m[8], m[12]);
             d.r1 = make_float4(m[1], m[5], m[9], m[13]);
             d.r2 = make_float4(m[2], m[6], m[10], m[14]);
-            const int32_t bi = g->instances[i].blas_index;
-            d.blas = make_int4(bi, 0, 0, 0);
+            float fi;
+            std::memcpy(&fi, &i, 4);
+            d.lo = root.lo;
+            d.hi = make_float4(root.hi.x, root.hi.y, root.hi.z, fi);
         }
         geo.instance_count = g->instance_count;
         geo.blas_count = g->blas_count;
@@ -465,9 +469,7 @@ int yrh_upload_geometry(yrh_context* ctx, const yrh_geometry_desc* g) {
         geo.stack_capacity = std::max(1, t.max_depth) + blas_depth + 3;
         if ((rc = Upload(ctx, geo.tlas_nodes, tl.nodes)) != YRH_OK) return rc;
         if ((rc = Upload(ctx, geo.tlas_leaf_count, tl.leaf_count)) != YRH_OK) return rc;
-        if ((rc = Upload(ctx, geo.tlas_slot_instance, resolved)) != YRH_OK) return rc;
-        if ((rc = Upload(ctx, geo.instances, inst)) != YRH_OK) return rc;
-        if ((rc = Upload(ctx, geo.blas_roots, roots)) != YRH_OK) return rc;
+        if ((rc = Upload(ctx, geo.instance_slots, slots)) != YRH_OK) return rc;
     } else {
         return Fail(ctx, YRH_E_INVALID_ARGUMENT, "unknown acceleration kind");
     }

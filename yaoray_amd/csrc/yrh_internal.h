@@ -42,18 +42,22 @@ struct alignas(16) DevNode {
 };
 static_assert(sizeof(DevNode) == 64, "node record is 64 bytes");
 
-struct alignas(16) DevInstance {
-    float4 r0;  // world_to_object m[0], m[4], m[8],  m[12]
-    float4 r1;  //                 m[1], m[5], m[9],  m[13]
-    float4 r2;  //                 m[2], m[6], m[10], m[14]
-    int4 blas;  // blas index, root code, 0, 0
-};
-static_assert(sizeof(DevInstance) == 64, "instance record is 64 bytes");
 
 struct alignas(16) DevBlasRoot {
     float4 lo;  // root box min, w = root code bits
     float4 hi;  // root box max
 };
+
+// One TLAS leaf slot: the instance's world-to-object rows and its BLAS root,
+// so a traversal step into an instance is one 80-byte fetch.
+struct alignas(16) DevInstanceSlot {
+    float4 r0;  // world_to_object m[0], m[4], m[8],  m[12]
+    float4 r1;  //                 m[1], m[5], m[9],  m[13]
+    float4 r2;  //                 m[2], m[6], m[10], m[14]
+    float4 lo;  // BLAS root box min, w = root code bits
+    float4 hi;  // BLAS root box max, w = instance index bits
+};
+static_assert(sizeof(DevInstanceSlot) == 80, "instance slot record is 80 bytes");
 
 inline int32_t PackLeaf(int32_t first, int32_t count) {
     const int32_t nib = count <= kLeafCountEscape ? count - 1 : kLeafCountEscape;
@@ -106,10 +110,8 @@ struct DevGeometry {
     bool empty = true;
     // two-level
     DevBuf tlas_nodes;
-    DevBuf tlas_slot_instance;  // int32 per TLAS slot
     DevBuf tlas_leaf_count;
-    DevBuf instances;           // DevInstance per instance
-    DevBuf blas_roots;          // DevBlasRoot per BLAS
+    DevBuf instance_slots;      // DevInstanceSlot per TLAS slot
     int32_t instance_count = 0;
     int32_t blas_count = 0;
     int32_t stack_capacity = 0;  // entries needed by the deepest traversal

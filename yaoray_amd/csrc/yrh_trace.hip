@@ -89,10 +89,8 @@ int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a
     a.root_hi_z = g.root_hi[2];
     a.empty = g.empty ? 1 : 0;
     a.tnodes = g.tlas_nodes.as<DevNode>();
-    a.tslot_instance = g.tlas_slot_instance.as<int32_t>();
     a.tleaf_count = g.tlas_leaf_count.as<int32_t>();
-    a.instances = g.instances.as<DevInstance>();
-    a.blas_roots = g.blas_roots.as<DevBlasRoot>();
+    a.instance_slots = g.instance_slots.as<DevInstanceSlot>();
     uint32_t* words = ctx->work_counter.as<uint32_t>();
     if (r.work_counter != nullptr) {
         a.counter = r.work_counter;

@@ -64,10 +64,8 @@ struct TraceArgs {
     int32_t empty;
     // two-level
     const DevNode* tnodes;
-    const int32_t* tslot_instance;
     const int32_t* tleaf_count;
-    const DevInstance* instances;
-    const DevBlasRoot* blas_roots;
+    const DevInstanceSlot* instance_slots;  // per TLAS leaf slot
     // scheduling
     uint32_t* counter;       // this launch's ray counter (zero at launch)
     uint32_t* next_counter;  // the next launch's counter(s), zeroed here (may be null)
@@ -358,16 +356,16 @@ __device__ __forceinline__ int32_t ClaimRay(uint32_t* counter, bool want, int la
     return static_cast<int32_t>(mine);
 }
 
-__device__ __forceinline__ void TransformRay(const DevInstance& m, float ox, float oy, float oz, float dx, float dy,
-                                             float dz, float& rox, float& roy, float& roz, float& rdx, float& rdy,
-                                             float& rdz) {
+__device__ __forceinline__ void TransformRay(const float4& r0, const float4& r1, const float4& r2, float ox, float oy,
+                                             float oz, float dx, float dy, float dz, float& rox, float& roy,
+                                             float& roz, float& rdx, float& rdy, float& rdz) {
     // TransformPoint / TransformVector (src/core/transform.cpp:30-44), left to right.
-    rox = m.r0.x * ox + m.r0.y * oy + m.r0.z * oz + m.r0.w;
-    roy = m.r1.x * ox + m.r1.y * oy + m.r1.z * oz + m.r1.w;
-    roz = m.r2.x * ox + m.r2.y * oy + m.r2.z * oz + m.r2.w;
-    rdx = m.r0.x * dx + m.r0.y * dy + m.r0.z * dz;
-    rdy = m.r1.x * dx + m.r1.y * dy + m.r1.z * dz;
-    rdz = m.r2.x * dx + m.r2.y * dy + m.r2.z * dz;
+    rox = r0.x * ox + r0.y * oy + r0.z * oz + r0.w;
+    roy = r1.x * ox + r1.y * oy + r1.z * oz + r1.w;
+    roz = r2.x * ox + r2.y * oy + r2.z * oz + r2.w;
+    rdx = r0.x * dx + r0.y * dy + r0.z * dz;
+    rdy = r1.x * dx + r1.y * dy + r1.z * dz;
+    rdz = r2.x * dx + r2.y * dy + r2.z * dz;
 }
 
 // The ray-cast policy: rays from (org, dir) arrays, results to the output arrays.
@@ -625,10 +623,10 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
                 ++cursor;
             } else if (!in_blas && cursor < leaf_end) {
                 // Next instance of the TLAS leaf: object ray, BLAS root box.
-                cur_inst = a.tslot_instance[cursor];
-                const DevInstance m = a.instances[cur_inst];
-                const DevBlasRoot root = a.blas_roots[m.blas.x];
-                TransformRay(m, ox, oy, oz, dx, dy, dz, bx, by, bz, bdx, bdy, bdz);
+                const DevInstanceSlot m = a.instance_slots[cursor];
+                cur_inst = __float_as_int(m.hi.w);
+                const DevInstanceSlot& root = m;
+                TransformRay(m.r0, m.r1, m.r2, ox, oy, oz, dx, dy, dz, bx, by, bz, bdx, bdy, bdz);
                 const RayPre bpre = Precompute(bx, by, bz, bdx, bdy, bdz);
                 ++nt;
                 float entry;
