@@ -313,11 +313,11 @@ def main():
 
 RENDER_WORKLOADS = {
     "cornell": dict(desc="seeded Cornell box (34 tris, ceiling emitter L=17), 256x256, 16 spp OwenSobol, depth 8",
-                    width=256, height=256, spp=16, max_depth=8, batch=16,
+                    width=256, height=256, spp=16, max_depth=8,
                     cpu=dict(width=256, height=256, spp=16)),
     "two_level": dict(desc="ground + 10k instanced 4x-subdivided icospheres (51.2M instanced tris; diffuse / GGX "
                            "conductor / dielectric), 2048x1024 sky+sun env, 1920x1080, 64 spp OwenSobol, depth 16",
-                      width=1920, height=1080, spp=64, max_depth=16, batch=4,
+                      width=1920, height=1080, spp=64, max_depth=16,
                       cpu=dict(width=480, height=270, spp=8)),
 }
 
@@ -357,11 +357,12 @@ def bench_batch(dev):
     dev.upload_scene(scene)
     dev.upload_shading(scene)
     t = time.perf_counter()
-    best, trials = autotune.tune(dev, jobs, "two_level_32_cameras", max_depth=16, probe_spp=1)
+    best, trials = autotune.tune(dev, jobs, "two_level_32_cameras", max_depth=16)
     tune_s = time.perf_counter() - t
     log(f"batch: autotune {tune_s:.1f}s -> {best}")
     films, ms, passes, st = dev.render_batch(jobs, max_depth=16, paths_per_pass=best["paths_per_pass"],
                                              use_graph=True)
+    dev.set_trace_config(-1, 0)  # later sections use the automatic launch shape
     total_ms = float(ms.max())
     samples = int(sum(j.width * j.height * j.spp for j in jobs))
     out = {
@@ -421,7 +422,7 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
     depth = 8
     total = max(spp_list)
     npix = width * height
-    big = max(1, min(64, (1 << 23) // npix))
+    big = max(1, min(total, dev.suggest_paths_per_pass(depth) // npix))
 
     ref, _ = dev.render(scene, width, height, 1, max_depth=depth, upload=False)  # warm-up
     dev.synchronize()
@@ -521,8 +522,11 @@ def bench_render(dev, key, frames, cpu_threads, with_cpu):
     log(f"render {key}: scene ready in {time.perf_counter() - t:.2f}s")
     dev.upload_scene(scene)
     dev.upload_shading(scene)
-    kw = dict(max_depth=w["max_depth"], upload=False, batch=w["batch"])
-    dev.render(scene, w["width"], w["height"], min(w["spp"], w["batch"]), **kw)  # warm-up
+    # Sample indices per pass: as many as the suggested paths per pass hold
+    # (the whole frame fits on one MI355X for both workloads).
+    batch = int(max(1, min(w["spp"], dev.suggest_paths_per_pass(w["max_depth"]) // (w["width"] * w["height"]))))
+    kw = dict(max_depth=w["max_depth"], upload=False, batch=batch)
+    dev.render(scene, w["width"], w["height"], batch, **kw)  # warm-up, allocates the pass buffers
     dev.synchronize()
     secs = []
     st = None
@@ -548,7 +552,7 @@ def bench_render(dev, key, frames, cpu_threads, with_cpu):
         "rays_traced": st["rays_traced"],
         "shadow_rays": st["shadow_rays"],
         "mean_node_tests_per_ray": round(st["bvh_node_tests"] / max(1, st["rays_traced"] + st["shadow_rays"]), 2),
-        "samples_per_pass": w["batch"],
+        "samples_per_pass": batch,
         "cpu_baseline": None,
     }
     log(f"render {key}: {out['ms_per_frame']} ms/frame, {out['value']} Mrays/s (+{out['shadow_Mrays_s']} shadow)")

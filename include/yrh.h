@@ -207,7 +207,9 @@ int yrh_trace_any(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n,
                   uint32_t* occluded, uint32_t* node_tests, uint32_t* triangle_tests);
 
 /* Launch geometry of the persistent traversal kernels: workgroups per compute
- * unit and threads per workgroup (64, 128 or 256); 0 keeps the current value. */
+ * unit and threads per workgroup (64, 128 or 256); 0 keeps the current value;
+ * blocks_per_cu = -1 returns to the automatic choice (as many workgroups as
+ * the LDS stack allows, at most 8). */
 int yrh_set_trace_config(yrh_context* ctx, int blocks_per_cu, int block_size);
 /* Traversal stack levels kept in LDS per lane (1..16; 0 = automatic: 8 for a
  * flat tree, 16 for two-level); deeper levels spill to a per-lane global
@@ -375,6 +377,14 @@ int yrh_film_load(yrh_context* ctx, const yrh_film_pixel* pixels, int64_t count)
 int yrh_film_read(yrh_context* ctx, yrh_film_pixel* pixels, int64_t count);
 /* Runs one pass on the context's stream, accumulating into the film. */
 int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* desc);
+/* Paths (pixels x sample indices) one pass should hold: as many as half of
+ * the device memory not in use (plus what this context's path state already
+ * holds) can keep, at ~460 bytes per path (+ the guiding buffers when a
+ * proposal is configured), capped at 2^27. Every bounce of a pass waits for
+ * its slowest paths, so fewer, larger passes spend less time in nearly empty
+ * late bounces; the film is the same for any pass size. No counterpart in the
+ * reference (its CPU passes are one sample index each). */
+int yrh_suggest_paths_per_pass(yrh_context* ctx, int32_t max_depth, int64_t* paths);
 /* Reads the accumulated counters (synchronises). */
 int yrh_render_stats_read(yrh_context* ctx, yrh_render_stats* stats);
 

@@ -75,3 +75,24 @@ def test_batch_films_equal_single_renders(device, kind):
     # One pass holds several sample layers of a job when they fit.
     _, ms, passes_big, _ = device.render_batch(jobs, max_depth=depth, paths_per_pass=100000)
     assert passes_big == 1 and np.allclose(ms, ms[0])
+
+
+def test_autotune_picks_the_fastest_probe_and_caches_it(device, tmp_path):
+    from yaoray_amd import autotune
+
+    s = Scene.cornell(1)
+    s.build_accel()
+    device.upload_scene(s)
+    device.upload_shading(s)
+    jobs = cameras(s, SPECS)
+    cache = str(tmp_path / "tune.json")
+    best, trials = autotune.tune(device, jobs, "cornell_3", max_depth=8, paths_candidates=(1000, 4000),
+                                 blocks_candidates=(4, 8), cache=cache)
+    assert len(trials) == 4
+    assert best == max(trials, key=lambda r: r["paths_per_s"])
+    for t in trials:
+        # Each probe holds about two passes of its candidate size.
+        assert t["probe_paths"] >= min(2 * t["paths_per_pass"], sum(j.width * j.height * j.spp for j in jobs))
+        assert t["paths_per_s"] > 0
+    assert autotune.lookup(device, "cornell_3", cache) == best
+    device.set_trace_config(-1, 256)  # back to the automatic launch shape

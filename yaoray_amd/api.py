@@ -487,6 +487,12 @@ class Device:
         self.synchronize()
         return self.film_read(width, height), self.render_stats()
 
+    def suggest_paths_per_pass(self, max_depth: int = 8) -> int:
+        """Paths one render pass should hold for the free device memory (yrh_suggest_paths_per_pass)."""
+        v = C.c_int64()
+        self._check(self._lib.yrh_suggest_paths_per_pass(self._ctx, max_depth, C.byref(v)), "suggest_paths_per_pass")
+        return v.value
+
     def sample_values(self, kind, seed, spp, x, y, sample, dim, dims):
         n = len(x)
         arrs = [np.ascontiguousarray(a, np.int32) for a in (x, y, sample, dim)]

@@ -1,10 +1,12 @@
 """Autotuner for the batched multi-camera pipeline.
 
-Times a short version of a batch (every job at `probe_spp` samples) for each
-candidate wavefront size (paths per pass) and traversal launch shape (blocks
-per CU), keeps the fastest, applies its launch shape to the device and, when a
-cache path is given, stores the choice per device and workload key in a JSON
-file so later runs start from it.
+For each candidate wavefront size (paths per pass) and traversal launch shape
+(blocks per CU) it renders a probe of about two passes of that size (every
+job at the same reduced sample count) and measures paths per second: a pass
+ends when its slowest paths do, so the rate depends on the pass size and a
+probe much smaller than a pass would not show it. The fastest rate wins; its
+launch shape is applied to the device and, when a cache path is given, the
+choice is stored per device and workload key in a JSON file.
 """
 from __future__ import annotations
 
@@ -15,8 +17,8 @@ import time
 
 from . import native as N
 
-DEFAULT_PATHS = (1 << 20, 1 << 21, 1 << 22, 1 << 23)
-DEFAULT_BLOCKS_PER_CU = (4, 6, 8)
+DEFAULT_PATHS = (1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27)
+DEFAULT_BLOCKS_PER_CU = (-1, 4)  # -1: the automatic launch shape
 
 
 def cache_path() -> str:
@@ -38,22 +40,25 @@ def lookup(device, key, path=None):
     return data.get(device.info()[0], {}).get(key)
 
 
-def tune(device, jobs, key, max_depth=8, sampler=N.SAMPLER_OWEN_SOBOL, probe_spp=2,
-         paths_candidates=DEFAULT_PATHS, blocks_candidates=DEFAULT_BLOCKS_PER_CU, cache=None, repeats=1):
-    """Returns (best, trials); best = dict(paths_per_pass, blocks_per_cu, seconds)."""
-    probe = []
-    for j in jobs:
-        d = N.RenderDesc()
-        C.memmove(C.byref(d), C.byref(j), C.sizeof(N.RenderDesc))
-        d.spp = min(j.spp, probe_spp)
-        probe.append(d)
-    total = sum(j.width * j.height * j.spp for j in probe)
+def tune(device, jobs, key, max_depth=8, sampler=N.SAMPLER_OWEN_SOBOL, paths_candidates=DEFAULT_PATHS,
+         blocks_candidates=DEFAULT_BLOCKS_PER_CU, cache=None, repeats=1, passes_per_probe=2):
+    """Returns (best, trials); best = dict(paths_per_pass, blocks_per_cu, paths_per_s)."""
+    pixels = sum(j.width * j.height for j in jobs)
+    full = sum(j.width * j.height * j.spp for j in jobs)
     trials = []
-    for bpc in blocks_candidates:
-        device.set_trace_config(bpc, 0)
-        for ppp in paths_candidates:
-            if ppp > max(paths_candidates[0], total):
-                continue  # a wavefront larger than the whole probe gains nothing
+    for ppp in paths_candidates:
+        if ppp > 2 * full:
+            continue  # larger than the whole schedule
+        spp = max(1, -(-passes_per_probe * ppp // pixels))
+        probe = []
+        for j in jobs:
+            d = N.RenderDesc()
+            C.memmove(C.byref(d), C.byref(j), C.sizeof(N.RenderDesc))
+            d.spp = min(j.spp, spp)
+            probe.append(d)
+        paths = sum(d.width * d.height * d.spp for d in probe)
+        for bpc in blocks_candidates:
+            device.set_trace_config(bpc, 0)
             best_t = None
             for _ in range(max(1, repeats)):
                 device.synchronize()
@@ -61,8 +66,9 @@ def tune(device, jobs, key, max_depth=8, sampler=N.SAMPLER_OWEN_SOBOL, probe_spp
                 device.render_batch(probe, max_depth=max_depth, sampler=sampler, paths_per_pass=ppp)
                 t = time.perf_counter() - t0
                 best_t = t if best_t is None else min(best_t, t)
-            trials.append({"paths_per_pass": ppp, "blocks_per_cu": bpc, "seconds": round(best_t, 5)})
-    best = min(trials, key=lambda r: r["seconds"])
+            trials.append({"paths_per_pass": ppp, "blocks_per_cu": bpc, "probe_paths": paths,
+                           "seconds": round(best_t, 5), "paths_per_s": round(paths / best_t, 1)})
+    best = max(trials, key=lambda r: r["paths_per_s"])
     device.set_trace_config(best["blocks_per_cu"], 0)
     if cache:
         data = _load(cache)

@@ -297,11 +297,11 @@ int yrh_copy_to_host(yrh_context* ctx, void* dst, const void* src, size_t bytes)
 
 int yrh_set_trace_config(yrh_context* ctx, int blocks_per_cu, int block_size) {
     if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
-    if (blocks_per_cu < 0 || blocks_per_cu > 64) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "blocks_per_cu must be in [0, 64]");
+    if (blocks_per_cu < -1 || blocks_per_cu > 64) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "blocks_per_cu must be in [-1, 64]");
     if (block_size != 0 && block_size != 64 && block_size != 128 && block_size != 256) {
         return Fail(ctx, YRH_E_INVALID_ARGUMENT, "block_size must be 64, 128 or 256");
     }
-    if (blocks_per_cu > 0) ctx->blocks_per_cu = blocks_per_cu;
+    if (blocks_per_cu != 0) ctx->blocks_per_cu = blocks_per_cu < 0 ? 0 : blocks_per_cu;
     if (block_size > 0) ctx->block_size = block_size;
     return YRH_OK;
 }

@@ -1413,6 +1413,12 @@ LightView MakeLightView(RenderState& R) {
     return L;
 }
 
+// Bytes per path slot and per shadow slot of the arrays EnsureCapacity
+// allocates, and per guided row (mixture + valid flag).
+constexpr size_t kPathSlotBytes = 4 + 4 + 4 + 16 + 16 + 16 + 4 + 16 + 16 + 16 + 16 + 8 + 2 * 16 + 2 * 16 + 2 * 4 + 16 + 4;
+constexpr size_t kShadowSlotBytes = 16 + 16 + 16 + 16 + 4 + 16 + 16 + 16;
+constexpr size_t kGuideRowBytes = yrg::kGuideMixture * 4 + 4;
+
 int EnsureCapacity(yrh_context* ctx, RenderState& R, int64_t paths) {
     if (R.capacity >= paths && R.counters.ptr != nullptr) return YRH_OK;
     int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
@@ -1803,6 +1809,22 @@ int yrh_render_stats_read(yrh_context* ctx, yrh_render_stats* stats) {
     stats->hits = w[kStatHits];
     stats->misses = w[kStatMisses];
     stats->bad_samples = w[kStatBad];
+    return YRH_OK;
+}
+
+int yrh_suggest_paths_per_pass(yrh_context* ctx, int32_t max_depth, int64_t* paths) {
+    if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
+    if (paths == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null output");
+    (void)hipSetDevice(ctx->device);
+    size_t free_bytes = 0, total_bytes = 0;
+    int rc = CheckHip(ctx, hipMemGetInfo(&free_bytes, &total_bytes), "hipMemGetInfo");
+    if (rc != YRH_OK) return rc;
+    // Per path: the path slot state (EnsureCapacity) and its two shadow slots.
+    size_t per_path = kPathSlotBytes + 2 * kShadowSlotBytes;
+    if (ctx->guide != nullptr) per_path += kGuideRowBytes + 64 * static_cast<size_t>(std::max(1, max_depth));
+    const size_t held = ctx->render != nullptr ? static_cast<size_t>(ctx->render->capacity) * per_path : 0;
+    const int64_t fit = static_cast<int64_t>((free_bytes + held) / 2 / per_path);
+    *paths = std::max<int64_t>(int64_t{1} << 16, std::min<int64_t>(fit, int64_t{1} << 27));
     return YRH_OK;
 }
 

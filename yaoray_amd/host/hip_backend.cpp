@@ -264,12 +264,15 @@ RenderResult HipBackend::Render(const PreparedScene& scene, const RenderRequest&
         result.stats.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     };
     // Without a progress callback nothing observes the film between passes, so
-    // consecutive passes are batched into one wavefront of up to kBatchPaths
-    // paths (same per-pixel sample order, so the same film).
-    constexpr int64_t kBatchPaths = int64_t{1} << 23;
+    // consecutive passes are batched into one wavefront as large as the device
+    // memory suggests (same per-pixel sample order, so the same film).
+    int64_t batch_paths = int64_t{1} << 23;
+    if (!request.progress_callback && yrh_suggest_paths_per_pass(ctx, settings.max_depth, &batch_paths) != YRH_OK) {
+        return fail(LastError(ctx, "suggest paths per pass"));
+    }
     const int batch = request.progress_callback
         ? 1
-        : static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(spp, kBatchPaths / std::max<int64_t>(1, npix))));
+        : static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(spp, batch_paths / std::max<int64_t>(1, npix))));
     for (int pass = request.resume_completed_spp; pass < spp; pass += d.sample_count) {
         if (request.stop_token.stop_requested()) {
             if (!sync_film()) return fail(LastError(ctx, "film read"));
