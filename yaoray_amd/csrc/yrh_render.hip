@@ -61,6 +61,14 @@ enum Stat : int {
     kStatTraceTris,
     kStatWords,
 };
+// The statistics live in kStatSlots copies kStatStride words (128 B) apart;
+// workgroup b adds to copy b % kStatSlots and the readback sums them: one
+// counter word for every wave of a 100-million-path launch made its atomics
+// a quarter of the render time.
+constexpr int kStatSlots = 64;
+constexpr int kStatStride = 16;
+static_assert(kStatWords <= kStatStride, "statistics words fit a slot");
+constexpr size_t kStatBytes = sizeof(unsigned long long) * kStatStride * kStatSlots;
 
 }  // namespace
 
@@ -101,7 +109,12 @@ struct RenderState {
     DevBuf env_texels;  // float4
     int32_t env_w = 0, env_h = 0;
     int32_t env_sampleable = 0;
-    DevBuf env_weights, env_row_weights, env_row_cdf, env_cond_cdf;
+    DevBuf env_weights, env_row_weights;
+    // CDFs padded with +inf to a multiple of kCdfBlock entries (rows of the
+    // conditional CDFs env_cstride apart), and every kCdfBlock-th entry of
+    // each (padded with +inf to a multiple of 4) for the two-level search.
+    DevBuf env_row_cdf, env_row_top, env_cond_cdf, env_cond_top;
+    int32_t env_rtop = 0, env_cstride = 0, env_ctop = 0;
     int32_t env_dw = 0, env_dh = 0;
     float env_total = 0.0f;
     // ---- film ----
@@ -115,7 +128,7 @@ struct RenderState {
     DevBuf s_org, s_dir, s_li, s_bsdf, s_occ, s_trans;
     DevBuf l_state, l_med;
     DevBuf counters;  // uint32[kCounterWords]
-    DevBuf stats;     // uint64[kStatWords]
+    DevBuf stats;     // uint64[kStatStride * kStatSlots]: kStatSlots copies of the statistics words
     // batched passes
     DevBuf p_job, p_sample;
     DevBuf b_jobs, b_items, b_pass_items, b_film;
@@ -182,7 +195,10 @@ struct LightView {
     const float* env_weights;
     const float* env_row_weights;
     const float* env_row_cdf;
+    const float* env_row_top;
     const float* env_cond_cdf;
+    const float* env_cond_top;
+    int32_t env_rtop, env_cstride, env_ctop;
     int32_t env_dw, env_dh;
     float env_total;
     const float* positions_xyz;  // unused on device (kept for symmetry)
@@ -251,24 +267,40 @@ enum PathFlag : int { kPrevValid = 1, kPrevDelta = 2, kMediumActive = 4, kPrimar
 
 __device__ __forceinline__ int Lane() { return static_cast<int>(threadIdx.x & 63u); }
 
-// Appends to a queue with one atomic per wave; returns the slot or -1.
-__device__ __forceinline__ int32_t WaveAppend(uint32_t* counter, bool want) {
-    const uint64_t mask = __ballot(want);
-    if (mask == 0) return -1;
+// Appends `count` (0..2) entries per thread to a queue with one atomic per
+// workgroup (all threads must call; uses `lds`, kBlock / 64 + 1 words);
+// returns the first slot of this thread's entries, consecutive, or -1. The
+// queue counters see one atomic per workgroup instead of one per wave and
+// entry kind, which on 100-million-path launches contended on one word.
+__device__ __forceinline__ int32_t BlockAppend(uint32_t* counter, uint32_t count, uint32_t* lds) {
+    constexpr int kWaves = kBlock / 64;
     const int lane = Lane();
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
-    base = __shfl(base, leader);
-    if (!want) return -1;
-    return static_cast<int32_t>(base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull))));
+    const int wave = static_cast<int>(threadIdx.x >> 6);
+    uint32_t inc = count;  // inclusive prefix over the wave
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t up = __shfl_up(inc, off);
+        if (lane >= off) inc += up;
+    }
+    if (lane == 63) lds[wave] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (int w = 0; w < kWaves; ++w) total += lds[w];
+        lds[kWaves] = total != 0u ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    uint32_t base = lds[kWaves];
+    for (int w = 0; w < wave; ++w) base += lds[w];
+    __syncthreads();  // lds is reused by the next call
+    return count != 0u ? static_cast<int32_t>(base + inc - count) : -1;
 }
 
-// Wave-summed statistics, one atomic per wave (all lanes must call).
-__device__ __forceinline__ void WaveStat(unsigned long long* stat, unsigned v) {
+// Wave-summed statistics word `word`, one atomic per wave into this
+// workgroup's copy (all lanes must call).
+__device__ __forceinline__ void WaveStat(unsigned long long* stats, int word, unsigned v) {
     unsigned long long s = v;
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (Lane() == 0 && s != 0) atomicAdd(stat, s);
+    if (Lane() == 0 && s != 0) atomicAdd(stats + word + kStatStride * (blockIdx.x % kStatSlots), s);
 }
 
 __device__ __forceinline__ float3 XYZ(float4 v) { return V(v.x, v.y, v.z); }
@@ -368,18 +400,33 @@ __device__ __forceinline__ float EnvPdf(const LightView& L, float3 d) {
     return prob / texel_sa;
 }
 
-// std::lower_bound over a non-decreasing float array.
-__device__ __forceinline__ int LowerBound(const float* a, int n, float target) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] < target) {
-            lo = mid + 1;
-        } else {
-            hi = mid;
-        }
-    }
-    return lo;
+// std::lower_bound over a non-decreasing float array, in two rounds of
+// independent loads instead of log2(n) dependent ones. `a` holds the n values
+// and +inf up to a multiple of kCdfBlock; top[j] = a[kCdfBlock * j] for the
+// ntop (a multiple of 4) block starts, +inf-padded. The index is the number
+// of entries below the target: k block starts lie below it, so it falls in
+// block k - 1, whose entries below the target are counted. (A finite target
+// never counts the +inf padding; the caller clamps n to n - 1 as the
+// reference does for a target above every entry.)
+constexpr int kCdfBlock = 32;
+
+__device__ __forceinline__ int Below4(const float4& v, float t) {
+    return static_cast<int>(v.x < t) + static_cast<int>(v.y < t) + static_cast<int>(v.z < t) + static_cast<int>(v.w < t);
+}
+
+__device__ __forceinline__ int BlockLowerBound(const float* __restrict__ a, const float* __restrict__ top, int ntop,
+                                               float target) {
+    const float4* t4 = reinterpret_cast<const float4*>(top);
+    int k = 0;
+#pragma unroll 4
+    for (int j = 0; j < ntop / 4; ++j) k += Below4(t4[j], target);
+    if (k == 0) return 0;
+    const int base = kCdfBlock * (k - 1);
+    const float4* a4 = reinterpret_cast<const float4*>(a + base);
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < kCdfBlock / 4; ++j) c += Below4(a4[j], target);
+    return base + c;
 }
 
 struct EnvSample {
@@ -397,14 +444,14 @@ __device__ __forceinline__ EnvSample SampleEnv(const LightView& L, float2 s) {
     const int w = L.env_dw, h = L.env_dh;
     if (w <= 0 || h <= 0 || L.env_total <= 0.0f) return out;
     const float row_target = Clampf(s.y, 0.0f, kOneMinusEps) * L.env_total;
-    int y = LowerBound(L.env_row_cdf, h, row_target);
+    int y = BlockLowerBound(L.env_row_cdf, L.env_row_top, L.env_rtop, row_target);
     if (y == h) y = h - 1;
     const float row_prev = y == 0 ? 0.0f : L.env_row_cdf[y - 1];
     const float row_w = fmaxf(L.env_row_weights[y], 1.0e-6f);
     const float row_frac = Clampf((row_target - row_prev) / row_w, 0.0f, 1.0f);
-    const float* cond = L.env_cond_cdf + static_cast<int64_t>(y) * w;
+    const float* cond = L.env_cond_cdf + static_cast<int64_t>(y) * L.env_cstride;
     const float col_target = Clampf(s.x, 0.0f, kOneMinusEps) * row_w;
-    int x = LowerBound(cond, w, col_target);
+    int x = BlockLowerBound(cond, L.env_cond_top + static_cast<int64_t>(y) * L.env_ctop, L.env_ctop, col_target);
     if (x == w) x = w - 1;
     const float col_prev = x == 0 ? 0.0f : cond[x - 1];
     const float tex_w = fmaxf(L.env_weights[static_cast<int64_t>(y) * w + x], 1.0e-6f);
@@ -940,7 +987,8 @@ __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S,
     const bool valid = i < n;
     if (valid) ShadePath<kGuided>(P, S, L, G, depth, i, cur, o, hit_count, miss_count);
     const int nxt = 1 - cur;
-    const int32_t slot = WaveAppend(P.counters + nxt, o.next);
+    __shared__ uint32_t append_lds[kBlock / 64 + 1];
+    const int32_t slot = BlockAppend(P.counters + nxt, o.next ? 1u : 0u, append_lds);
     if (slot >= 0) {
         const int p = P.q_path[cur][i];
         P.q_org[nxt][slot] = F4(o.next_org, 1.0e-5f);
@@ -948,9 +996,9 @@ __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S,
         P.q_path[nxt][slot] = p;
     }
     int2 slots = make_int2(-1, -1);
+    int32_t s = BlockAppend(P.counters + kCountShadow, (o.shadow[0] ? 1u : 0u) + (o.shadow[1] ? 1u : 0u), append_lds);
     for (int k = 0; k < 2; ++k) {
-        const int32_t s = WaveAppend(P.counters + kCountShadow, o.shadow[k]);
-        if (s >= 0) {
+        if (o.shadow[k]) {
             P.s_org[s] = F4(o.s_org[k], 1.0e-5f);
             P.s_dir[s] = F4(o.s_dir[k], o.s_tmax[k]);
             P.s_li[s] = F4(o.s_li[k], o.s_scale[k]);
@@ -961,11 +1009,12 @@ __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S,
             } else {
                 slots.y = s;
             }
+            ++s;
         }
     }
     if (valid && (slots.x >= 0 || slots.y >= 0)) P.p_slot[P.q_path[cur][i]] = slots;
-    WaveStat(P.stats + kStatHits, hit_count);
-    WaveStat(P.stats + kStatMisses, miss_count);
+    WaveStat(P.stats, kStatHits, hit_count);
+    WaveStat(P.stats, kStatMisses, miss_count);
     if (i == 0) atomicAdd(P.stats + kStatRays, static_cast<unsigned long long>(n));
 }
 
@@ -1124,8 +1173,8 @@ __global__ __launch_bounds__(kBlock) void ResolveKernel(PassView P, yrg::GuideVi
             }
         }
     }
-    WaveStat(P.stats + kStatShadow, shadow);
-    WaveStat(P.stats + kStatOccluded, occluded);
+    WaveStat(P.stats, kStatShadow, shadow);
+    WaveStat(P.stats, kStatOccluded, occluded);
 }
 
 // ---- transparent shadows (TraceShadowVisibility, path_direct_lighting.cpp:23-113) ----
@@ -1283,7 +1332,7 @@ __global__ __launch_bounds__(kBlock) void AccumulateKernel(PassView P) {
     for (int k = 0; pixel < P.npix && k < P.sample_count; ++k) {
         bad += AddSample(P, static_cast<int64_t>(k) * P.npix + pixel, P.film[pixel]);
     }
-    WaveStat(P.stats + kStatBad, bad);
+    WaveStat(P.stats, kStatBad, bad);
 }
 
 // Batched pass: one thread per path of the given occurrence. The items of a
@@ -1297,7 +1346,7 @@ __global__ __launch_bounds__(kBlock) void AccumulateBatchKernel(PassView P, int 
     if (p < static_cast<int64_t>(P.counters[kCountPassPaths]) && (P.p_job[p] >> 16) == occurrence) {
         bad = AddSample(P, p, P.film[P.p_pixel[p]]);
     }
-    WaveStat(P.stats + kStatBad, bad);
+    WaveStat(P.stats, kStatBad, bad);
 }
 
 __global__ void EndPassKernel(uint32_t* counters) {
@@ -1406,7 +1455,12 @@ LightView MakeLightView(RenderState& R) {
     L.env_weights = R.env_weights.as<float>();
     L.env_row_weights = R.env_row_weights.as<float>();
     L.env_row_cdf = R.env_row_cdf.as<float>();
+    L.env_row_top = R.env_row_top.as<float>();
     L.env_cond_cdf = R.env_cond_cdf.as<float>();
+    L.env_cond_top = R.env_cond_top.as<float>();
+    L.env_rtop = R.env_rtop;
+    L.env_cstride = R.env_cstride;
+    L.env_ctop = R.env_ctop;
     L.env_dw = R.env_dw;
     L.env_dh = R.env_dh;
     L.env_total = R.env_total;
@@ -1441,8 +1495,8 @@ int EnsureCapacity(yrh_context* ctx, RenderState& R, int64_t paths) {
         if ((rc = Alloc(ctx, *it.buf, it.bytes)) != YRH_OK) return rc;
     }
     if (R.stats.ptr == nullptr) {
-        if ((rc = Alloc(ctx, R.stats, 8 * kStatWords)) != YRH_OK) return rc;
-        if ((rc = CheckHip(ctx, hipMemset(R.stats.ptr, 0, 8 * kStatWords), "hipMemset")) != YRH_OK) return rc;
+        if ((rc = Alloc(ctx, R.stats, kStatBytes)) != YRH_OK) return rc;
+        if ((rc = CheckHip(ctx, hipMemset(R.stats.ptr, 0, kStatBytes), "hipMemset")) != YRH_OK) return rc;
     }
     rc = CheckHip(ctx, hipMemset(R.counters.ptr, 0, 4 * kCounterWords), "hipMemset");
     R.capacity = paths;
@@ -1751,8 +1805,32 @@ int yrh_upload_shading(yrh_context* ctx, const yrh_shading_desc* sd) {
     }
     if ((rc = UploadVec(ctx, R.env_weights, env.texel_weights, env.has_distribution ? nd : 0)) != YRH_OK) return rc;
     if ((rc = UploadVec(ctx, R.env_row_weights, env.row_weights, env.has_distribution ? static_cast<size_t>(R.env_dh) : 0)) != YRH_OK) return rc;
-    if ((rc = UploadVec(ctx, R.env_row_cdf, env.row_cdf, env.has_distribution ? static_cast<size_t>(R.env_dh) : 0)) != YRH_OK) return rc;
-    if ((rc = UploadVec(ctx, R.env_cond_cdf, env.conditional_cdfs, env.has_distribution ? nd : 0)) != YRH_OK) return rc;
+    {
+        // Padded CDFs and their block starts for BlockLowerBound.
+        const float inf = std::numeric_limits<float>::infinity();
+        const int h = env.has_distribution ? std::max(0, R.env_dh) : 0;
+        const int w = env.has_distribution ? std::max(0, R.env_dw) : 0;
+        const auto padded = [](int n) { return (n + kCdfBlock - 1) / kCdfBlock * kCdfBlock; };
+        const auto tops = [](int n) { return ((n + kCdfBlock - 1) / kCdfBlock + 3) / 4 * 4; };
+        std::vector<float> row(static_cast<size_t>(padded(h)), inf), row_top(static_cast<size_t>(tops(h)), inf);
+        for (int i = 0; i < h; ++i) row[static_cast<size_t>(i)] = env.row_cdf[i];
+        for (int j = 0; j * kCdfBlock < h; ++j) row_top[static_cast<size_t>(j)] = row[static_cast<size_t>(j) * kCdfBlock];
+        R.env_cstride = padded(w);
+        R.env_ctop = tops(w);
+        R.env_rtop = tops(h);
+        std::vector<float> cond(static_cast<size_t>(R.env_cstride) * static_cast<size_t>(h), inf);
+        std::vector<float> cond_top(static_cast<size_t>(R.env_ctop) * static_cast<size_t>(h), inf);
+        for (int y = 0; y < h; ++y) {
+            float* c = cond.data() + static_cast<size_t>(y) * static_cast<size_t>(R.env_cstride);
+            for (int x = 0; x < w; ++x) c[x] = env.conditional_cdfs[static_cast<size_t>(y) * static_cast<size_t>(w) + static_cast<size_t>(x)];
+            float* t = cond_top.data() + static_cast<size_t>(y) * static_cast<size_t>(R.env_ctop);
+            for (int j = 0; j * kCdfBlock < w; ++j) t[j] = c[j * kCdfBlock];
+        }
+        if ((rc = UploadVec(ctx, R.env_row_cdf, row.data(), row.size())) != YRH_OK) return rc;
+        if ((rc = UploadVec(ctx, R.env_row_top, row_top.data(), row_top.size())) != YRH_OK) return rc;
+        if ((rc = UploadVec(ctx, R.env_cond_cdf, cond.data(), cond.size())) != YRH_OK) return rc;
+        if ((rc = UploadVec(ctx, R.env_cond_top, cond_top.data(), cond_top.size())) != YRH_OK) return rc;
+    }
     R.shading_loaded = true;
     return YRH_OK;
 }
@@ -1772,9 +1850,9 @@ int yrh_film_reset(yrh_context* ctx, int32_t width, int32_t height) {
     R.film_w = width;
     R.film_h = height;
     if (R.stats.ptr == nullptr) {
-        if ((rc = Alloc(ctx, R.stats, 8 * kStatWords)) != YRH_OK) return rc;
+        if ((rc = Alloc(ctx, R.stats, kStatBytes)) != YRH_OK) return rc;
     }
-    return CheckHip(ctx, hipMemset(R.stats.ptr, 0, 8 * kStatWords), "hipMemset");
+    return CheckHip(ctx, hipMemset(R.stats.ptr, 0, kStatBytes), "hipMemset");
 }
 
 int yrh_film_load(yrh_context* ctx, const yrh_film_pixel* pixels, int64_t count) {
@@ -1797,9 +1875,13 @@ int yrh_render_stats_read(yrh_context* ctx, yrh_render_stats* stats) {
         *stats = yrh_render_stats{};
         return YRH_OK;
     }
-    unsigned long long w[kStatWords];
-    int rc = yrh_copy_to_host(ctx, w, ctx->render->stats.ptr, sizeof(w));
+    std::vector<unsigned long long> slots(static_cast<size_t>(kStatStride) * kStatSlots);
+    int rc = yrh_copy_to_host(ctx, slots.data(), ctx->render->stats.ptr, kStatBytes);
     if (rc != YRH_OK) return rc;
+    unsigned long long w[kStatWords] = {};
+    for (int k = 0; k < kStatSlots; ++k) {
+        for (int j = 0; j < kStatWords; ++j) w[j] += slots[static_cast<size_t>(k) * kStatStride + j];
+    }
     stats->rays_traced = w[kStatRays];
     stats->shadow_rays = w[kStatShadow];
     stats->occluded_shadow_rays = w[kStatOccluded];
@@ -1944,7 +2026,7 @@ int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* d, float* job_ms, i
     if ((rc = UploadVec(ctx, R.b_pass_items, pass_items.data(), pass_items.size())) != YRH_OK) return rc;
     if ((rc = Alloc(ctx, R.b_film, static_cast<size_t>(film_pixels) * sizeof(yrh_film_pixel))) != YRH_OK) return rc;
     if ((rc = CheckHip(ctx, hipMemset(R.b_film.ptr, 0, R.b_film.bytes), "hipMemset")) != YRH_OK) return rc;
-    if ((rc = CheckHip(ctx, hipMemset(R.stats.ptr, 0, 8 * kStatWords), "hipMemset")) != YRH_OK) return rc;
+    if ((rc = CheckHip(ctx, hipMemset(R.stats.ptr, 0, kStatBytes), "hipMemset")) != YRH_OK) return rc;
     if ((rc = CheckHip(ctx, hipMemset(R.counters.ptr, 0, 4 * kCounterWords), "hipMemset")) != YRH_OK) return rc;
     yrh_render_desc rd{};
     rd.sampler = d->sampler;
