@@ -119,7 +119,7 @@ def main():
     ap.add_argument("--guide-ref-spp", type=int, default=1024)
     ap.add_argument("--guide-spp", default="16,32,64,128,256")
     ap.add_argument("--guide-train-passes", type=int, default=16)
-    ap.add_argument("--guide-train-steps", type=int, default=64)
+    ap.add_argument("--guide-train-steps", type=int, default=8)
     ap.add_argument("--guide-train-batch", type=int, default=32768)
     ap.add_argument("--guide-lr", type=float, default=3e-3)
     ap.add_argument("--guide-threshold", type=float, default=0.2)

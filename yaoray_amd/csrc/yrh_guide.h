@@ -137,15 +137,36 @@ __device__ inline void HeadFromOutputs(const float* o, Mixture& m) {
 }
 
 // vMF density with the normaliser written as kappa / (2 pi (1 - e^{-2 kappa})).
-__device__ inline float VmfPdf(float kappa, float cos_mu) {
-    return kappa / (2.0f * 3.14159265358979f * (-expm1f(-2.0f * kappa))) * expf(kappa * (cos_mu - 1.0f));
-}
+__device__ inline float VmfNorm(float kappa) { return kappa / (2.0f * 3.14159265358979f * (-expm1f(-2.0f * kappa))); }
+__device__ inline float VmfPdf(float kappa, float cos_mu) { return VmfNorm(kappa) * expf(kappa * (cos_mu - 1.0f)); }
 
-__device__ inline float MixturePdf(const float* pi, const float* kappa, const float* mu, float3 w) {
+// A mixture's 40 values: pi[8], kappa[8], mu[8][3], as a row (stride 1) or
+// as column i of per-row arrays stride apart (the render's per-queue-row
+// mixtures, so a wave's reads of one value are contiguous).
+struct MixtureView {
+    const float* p;
+    int64_t stride;
+    __device__ float operator[](int k) const { return p[k * stride]; }
+};
+
+__device__ inline float MixturePdf(const MixtureView& m, float3 w) {
     float q = 0.0f;
     for (int k = 0; k < kGuideLobes; ++k) {
-        const float c = mu[3 * k] * w.x + mu[3 * k + 1] * w.y + mu[3 * k + 2] * w.z;
-        q += pi[k] * VmfPdf(kappa[k], c);
+        const float c = m[16 + 3 * k] * w.x + m[16 + 3 * k + 1] * w.y + m[16 + 3 * k + 2] * w.z;
+        q += m[k] * VmfPdf(m[8 + k], c);
+    }
+    return q;
+}
+
+// The same density from a row that also carries each lobe's normaliser
+// VmfNorm(kappa) at 40 + k (the render's per-queue-row mixtures): the same
+// operations, so the same value, with one exponential per lobe.
+constexpr int kGuideMixtureRow = kGuideMixture + kGuideLobes;
+__device__ inline float MixturePdfWithNorm(const MixtureView& m, float3 w) {
+    float q = 0.0f;
+    for (int k = 0; k < kGuideLobes; ++k) {
+        const float c = m[16 + 3 * k] * w.x + m[16 + 3 * k + 1] * w.y + m[16 + 3 * k + 2] * w.z;
+        q += m[k] * (m[kGuideMixture + k] * expf(m[8 + k] * (c - 1.0f)));
     }
     return q;
 }
@@ -153,22 +174,21 @@ __device__ inline float MixturePdf(const float* pi, const float* kappa, const fl
 // Samples the mixture: u0 picks the lobe, (u1, u2) the direction within it
 // (W. Jakob, "Numerically stable sampling of the von Mises-Fisher
 // distribution on S^2", 2012; frame of Duff et al., JCGT 2017).
-__device__ inline float3 MixtureSample(const float* pi, const float* kappa, const float* mu, float u0, float u1,
-                                       float u2) {
+__device__ inline float3 MixtureSample(const MixtureView& m, float u0, float u1, float u2) {
     int k = 0;
-    float acc = pi[0];
+    float acc = m[0];
     while (k < kGuideLobes - 1 && u0 >= acc) {
         ++k;
-        acc += pi[k];
+        acc += m[k];
     }
-    const float kap = kappa[k];
+    const float kap = m[8 + k];
     const float e = expf(-2.0f * kap);
     float w = 1.0f + logf(fmaxf(u1 + (1.0f - u1) * e, 1.0e-30f)) / kap;
     w = fminf(fmaxf(w, -1.0f), 1.0f);
     const float st = sqrtf(fmaxf(0.0f, 1.0f - w * w));
     const float phi = 2.0f * 3.14159265358979f * u2;
     const float lx = st * cosf(phi), ly = st * sinf(phi);
-    const float nx = mu[3 * k], ny = mu[3 * k + 1], nz = mu[3 * k + 2];
+    const float nx = m[16 + 3 * k], ny = m[16 + 3 * k + 1], nz = m[16 + 3 * k + 2];
     const float sign = nz >= 0.0f ? 1.0f : -1.0f;
     const float a = -1.0f / (sign + nz);
     const float b = nx * ny * a;
@@ -311,7 +331,8 @@ struct GuideView {
     GuideGrid grid;
     const bf16x4_t* frag;  // forward fragments
     const float2* table;   // hash-grid features
-    float* mix;            // per queue row: pi[8], kappa[8], mu[8][3]
+    float* mix;            // per queue row i: value k (pi[8], kappa[8], mu[8][3], VmfNorm(kappa)[8]) at mix[k * mix_stride + i]
+    int64_t mix_stride;
     int32_t* valid;        // per queue row: 1 when the row carries a mixture
     float4* rec;           // vertex records [depth][path][4]
     int64_t rec_paths;

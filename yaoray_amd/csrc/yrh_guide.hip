@@ -462,14 +462,15 @@ __global__ void GuideMixtureEvalKernel(const float* __restrict__ mixture, const 
     const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const float* m = mixture + 40 * r;
-    pdf[r] = yrg::MixturePdf(m, m + 8, m + 16, make_float3(dirs[3 * r], dirs[3 * r + 1], dirs[3 * r + 2]));
-    float3 w = yrg::MixtureSample(m, m + 8, m + 16, u[3 * r], u[3 * r + 1], u[3 * r + 2]);
+    const yrg::MixtureView mv{m, 1};
+    pdf[r] = yrg::MixturePdf(mv, make_float3(dirs[3 * r], dirs[3 * r + 1], dirs[3 * r + 2]));
+    float3 w = yrg::MixtureSample(mv, u[3 * r], u[3 * r + 1], u[3 * r + 2]);
     const float len = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
     w = make_float3(w.x / len, w.y / len, w.z / len);
     samples[4 * r] = w.x;
     samples[4 * r + 1] = w.y;
     samples[4 * r + 2] = w.z;
-    samples[4 * r + 3] = yrg::MixturePdf(m, m + 8, m + 16, w);
+    samples[4 * r + 3] = yrg::MixturePdf(mv, w);
 }
 
 // ---- host helpers ------------------------------------------------------------------
@@ -557,7 +558,7 @@ int GuidePrepareView(yrh_context* ctx, int64_t paths, int max_depth, yrg::GuideV
     if (G == nullptr || G->mode == YRH_GUIDE_OFF) return YRH_OK;
     int rc;
     if (G->rows < paths) {
-        if ((rc = Ensure(ctx, G->mix, static_cast<size_t>(paths) * yrg::kGuideMixture * 4)) != YRH_OK) return rc;
+        if ((rc = Ensure(ctx, G->mix, static_cast<size_t>(paths) * yrg::kGuideMixtureRow * 4)) != YRH_OK) return rc;
         if ((rc = Ensure(ctx, G->valid, static_cast<size_t>(paths) * 4)) != YRH_OK) return rc;
         G->rows = paths;
     }
@@ -578,6 +579,7 @@ int GuidePrepareView(yrh_context* ctx, int64_t paths, int max_depth, yrg::GuideV
     view->frag = G->frag_fwd.as<bf16x4_t>();
     view->table = G->table.as<float2>();
     view->mix = G->mix.as<float>();
+    view->mix_stride = G->rows;
     view->valid = G->valid.as<int32_t>();
     view->rec = G->mode == YRH_GUIDE_RECORD ? G->rec.as<float4>() : nullptr;
     view->rec_paths = paths;

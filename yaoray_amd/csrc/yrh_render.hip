@@ -745,17 +745,17 @@ __device__ __forceinline__ bool Guidable(const DevMaterial& m) {
     return (m.kind == kDiffuse || m.kind == kConductor) && !IsDelta(m);
 }
 
-__device__ __forceinline__ float GuidePdf(const float* gm, float3 w) { return yrg::MixturePdf(gm, gm + 8, gm + 16, w); }
+__device__ __forceinline__ float GuidePdf(const yrg::MixtureView& gm, float3 w) { return yrg::MixturePdfWithNorm(gm, w); }
 
 // One-sample MIS of the proposal (chosen with probability alpha) and the BSDF:
 // the direction comes from the chosen strategy, the pdf is the mixture
 // alpha q + (1 - alpha) p_bsdf, so the estimate stays unbiased.
-__device__ __forceinline__ BsdfSample SampleGuided(const DevMaterial& m, float3 wo, float3 n, const float* gm,
+__device__ __forceinline__ BsdfSample SampleGuided(const DevMaterial& m, float3 wo, float3 n, const yrg::MixtureView& gm,
                                                    float alpha, float u, float2 s) {
     BsdfSample b = Invalid();
     float3 wi;
     if (u < alpha) {
-        wi = Normalize(yrg::MixtureSample(gm, gm + 8, gm + 16, u / alpha, s.x, s.y));
+        wi = Normalize(yrg::MixtureSample(gm, u / alpha, s.x, s.y));
     } else {
         const BsdfSample bs = SampleBsdf(m, wo, n, s);
         if (!bs.valid) return b;
@@ -834,7 +834,7 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
     // continuation ray is traced, so next-event MIS keeps the BSDF pdf there,
     // and the guided estimator has the plain one's expectation at every depth.
     const bool guided = kGuided && G.valid[i] != 0;
-    const float* gm = guided ? G.mix + static_cast<int64_t>(yrg::kGuideMixture) * i : nullptr;
+    const yrg::MixtureView gm{guided ? G.mix + i : nullptr, G.mix_stride};
     const float alpha = guided ? G.alpha : 0.0f;
 
     if (depth == 0) {
@@ -1021,14 +1021,23 @@ __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S,
 // Proposal for every queue row of this bounce: each lane resolves its row's
 // surface (position, shading normal, outgoing direction), the wave evaluates
 // the network on its 64 rows with MFMA, and each guidable row gets its mixture.
-__global__ __launch_bounds__(kBlock) void GuideQueryKernel(PassView P, SurfaceView S, LightView L, yrg::GuideView G,
-                                                          int cur) {
-    __shared__ float lds_in[4][64][10];
-    __shared__ float lds_o[4][16 * yrg::kGuideOut];
+// Per-bounce proposal inference: kQueryBlock lanes share one copy of the
+// weight fragments in LDS; each wave runs its 64 rows through the MLP 16 at a
+// time, and all 64 lanes then form the 16 rows' mixtures (lane: row l % 16,
+// lobes 2(l / 16) and 2(l / 16) + 1, with the same operations as
+// HeadFromOutputs), written column-wise (value k of row i at
+// mix[k * mix_stride + i]).
+constexpr int kQueryBlock = 512;
+constexpr int kQueryWaves = kQueryBlock / 64;
+
+__global__ __launch_bounds__(kQueryBlock) void GuideQueryKernel(PassView P, SurfaceView S, LightView L,
+                                                               yrg::GuideView G, int cur) {
+    __shared__ float lds_in[kQueryWaves][64][10];
+    __shared__ float lds_o[kQueryWaves][16 * yrg::kGuideOut];
     __shared__ yrg::bf16x4_t lds_w[yrg::kFragCount * 64];  // weight fragments, shared by the block
     const int n = static_cast<int>(P.counters[cur]);
     if (static_cast<int64_t>(blockIdx.x) * blockDim.x >= n) return;  // whole block idle
-    for (int k = static_cast<int>(threadIdx.x); k < yrg::kFragCount * 64; k += kBlock) lds_w[k] = G.frag[k];
+    for (int k = static_cast<int>(threadIdx.x); k < yrg::kFragCount * 64; k += kQueryBlock) lds_w[k] = G.frag[k];
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int wave = static_cast<int>(threadIdx.x >> 6);
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -1051,14 +1060,15 @@ __global__ __launch_bounds__(kBlock) void GuideQueryKernel(PassView P, SurfaceVi
     in[3] = sn.x; in[4] = sn.y; in[5] = sn.z;
     in[6] = wo.x; in[7] = wo.y; in[8] = wo.z;
     in[9] = guidable ? 1.0f : 0.0f;
+    if (i < n) G.valid[i] = guidable ? 1 : 0;
     __syncthreads();
     const int q = lane / 16;
+    const int row = lane % 16;
     float* lo = lds_o[wave];
-    // One block of 16 rows at a time keeps registers and LDS small; the lanes
-    // of that block's rows then read their own row's 48 outputs.
+    const int64_t wave_row0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + wave * 64;
 #pragma unroll 1
     for (int rb = 0; rb < 4; ++rb) {
-        const float* r = lds_in[wave][rb * 16 + lane % 16];
+        const float* r = lds_in[wave][rb * 16 + row];
         yrg::bf16x4_t x[1][2];
         float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         if (r[9] != 0.0f) {
@@ -1075,24 +1085,33 @@ __global__ __launch_bounds__(kBlock) void GuideQueryKernel(PassView P, SurfaceVi
         yrg::f32x4_t o[1][3];
         yrg::MlpForward<1>(lds_w, lane, x, h0, h1, h2, o);
         for (int mb = 0; mb < 3; ++mb) {
-            for (int j = 0; j < 4; ++j) lo[(lane % 16) * yrg::kGuideOut + mb * 16 + 4 * q + j] = o[0][mb][j];
+            for (int j = 0; j < 4; ++j) lo[row * yrg::kGuideOut + mb * 16 + 4 * q + j] = o[0][mb][j];
         }
         __syncthreads();
-        if (lane / 16 == rb && i < n && guidable) {
-            yrg::Mixture m;
-            yrg::HeadFromOutputs(lo + (lane % 16) * yrg::kGuideOut, m);
-            float* dst = G.mix + static_cast<int64_t>(yrg::kGuideMixture) * i;
-            for (int k = 0; k < yrg::kGuideLobes; ++k) {
-                dst[k] = m.pi[k];
-                dst[8 + k] = m.kappa[k];
-                dst[16 + 3 * k] = m.mu[k][0];
-                dst[16 + 3 * k + 1] = m.mu[k][1];
-                dst[16 + 3 * k + 2] = m.mu[k][2];
+        const int64_t ri = wave_row0 + rb * 16 + row;
+        if (ri < n && r[9] != 0.0f) {
+            const float* out = lo + row * yrg::kGuideOut;
+            float mx = out[0];
+            for (int k = 1; k < yrg::kGuideLobes; ++k) mx = fmaxf(mx, out[5 * k]);
+            float sum = 0.0f;
+            for (int k = 0; k < yrg::kGuideLobes; ++k) sum += expf(out[5 * k] - mx);
+            float* col = G.mix + ri;
+            for (int k = 2 * q; k < 2 * q + 2; ++k) {
+                col[k * G.mix_stride] = expf(out[5 * k] - mx) / sum;
+                const float s = fminf(fmaxf(out[5 * k + 1], yrg::kLogKappaMin), yrg::kLogKappaMax);
+                const float kappa = expf(s);
+                col[(8 + k) * G.mix_stride] = kappa;
+                col[(yrg::kGuideMixture + k) * G.mix_stride] = yrg::VmfNorm(kappa);
+                const float ux = out[5 * k + 2], uy = out[5 * k + 3], uz = out[5 * k + 4];
+                const float len = sqrtf(ux * ux + uy * uy + uz * uz);
+                const bool ok = len > 1.0e-6f;
+                col[(16 + 3 * k) * G.mix_stride] = ok ? ux / len : 0.0f;
+                col[(16 + 3 * k + 1) * G.mix_stride] = ok ? uy / len : 0.0f;
+                col[(16 + 3 * k + 2) * G.mix_stride] = ok ? uz / len : 1.0f;
             }
         }
         __syncthreads();
     }
-    if (i < n) G.valid[i] = guidable ? 1 : 0;
 }
 
 // After the last bounce: each record of this pass becomes a training sample
@@ -1368,7 +1387,7 @@ __global__ void SampleValuesKernel(int kind, uint64_t seed, int spp, int64_t n, 
     }
 }
 
-int Grid(int64_t n) { return static_cast<int>(std::max<int64_t>(1, (n + kBlock - 1) / kBlock)); }
+int Grid(int64_t n, int block = kBlock) { return static_cast<int>(std::max<int64_t>(1, (n + block - 1) / block)); }
 
 PassView MakePassView(RenderState& R, const yrh_render_desc& d) {
     PassView P{};
@@ -1471,7 +1490,7 @@ LightView MakeLightView(RenderState& R) {
 // allocates, and per guided row (mixture + valid flag).
 constexpr size_t kPathSlotBytes = 4 + 4 + 4 + 16 + 16 + 16 + 4 + 16 + 16 + 16 + 16 + 8 + 2 * 16 + 2 * 16 + 2 * 4 + 16 + 4;
 constexpr size_t kShadowSlotBytes = 16 + 16 + 16 + 16 + 4 + 16 + 16 + 16;
-constexpr size_t kGuideRowBytes = yrg::kGuideMixture * 4 + 4;
+constexpr size_t kGuideRowBytes = yrg::kGuideMixtureRow * 4 + 4;
 
 int EnsureCapacity(yrh_context* ctx, RenderState& R, int64_t paths) {
     if (R.capacity >= paths && R.counters.ptr != nullptr) return YRH_OK;
@@ -1530,7 +1549,8 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
         PassView Pk = P;
         if (ctx->geo.accel_kind != YRH_ACCEL_TWO_LEVEL) Pk.hit_inst = nullptr;
         if (guided && depth + 1 < max_depth) {
-            hipLaunchKernelGGL(GuideQueryKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, cur);
+            hipLaunchKernelGGL(GuideQueryKernel, dim3(Grid(paths, kQueryBlock)), dim3(kQueryBlock), 0, st, Pk, S, L, GV,
+                               cur);
             hipLaunchKernelGGL(ShadeKernel<true>, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
         } else {
             hipLaunchKernelGGL(ShadeKernel<false>, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
