@@ -162,6 +162,10 @@ __device__ inline float MixturePdf(const MixtureView& m, float3 w) {
 // VmfNorm(kappa) at 40 + k (the render's per-queue-row mixtures): the same
 // operations, so the same value, with one exponential per lobe.
 constexpr int kGuideMixtureRow = kGuideMixture + kGuideLobes;
+// Row stride of network outputs staged in LDS: odd, so the 16 rows of a
+// block read one output column from 16 different banks (a stride of 48
+// words put them in 2 banks, an 8-way conflict).
+constexpr int kOutStride = kGuideOut + 1;
 __device__ inline float MixturePdfWithNorm(const MixtureView& m, float3 w) {
     float q = 0.0f;
     for (int k = 0; k < kGuideLobes; ++k) {

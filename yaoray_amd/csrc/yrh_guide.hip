@@ -168,7 +168,7 @@ __device__ void HeadGradient(const float* o, float wx, float wy, float wz, float
 }
 
 __global__ __launch_bounds__(kBlock) void GuideBackwardKernel(TrainArgs A) {
-    __shared__ float lds_o[4][16 * yrg::kGuideOut];
+    __shared__ float lds_o[4][16 * yrg::kOutStride];
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int wave = static_cast<int>(threadIdx.x >> 6);
     const int64_t rb = static_cast<int64_t>(blockIdx.x) * 4 + wave;
@@ -199,18 +199,18 @@ __global__ __launch_bounds__(kBlock) void GuideBackwardKernel(TrainArgs A) {
     yrg::MlpForward<1>(A.fwd, lane, x, h0, h1, h2, o);
     float* lo = lds_o[wave];
     for (int mb = 0; mb < 3; ++mb) {
-        for (int j = 0; j < 4; ++j) lo[row * yrg::kGuideOut + mb * 16 + 4 * q + j] = o[0][mb][j];
+        for (int j = 0; j < 4; ++j) lo[row * yrg::kOutStride + mb * 16 + 4 * q + j] = o[0][mb][j];
     }
     __syncthreads();
     if (q == 0) {
         float out[yrg::kGuideOut], d[yrg::kGuideOut];
-        for (int j = 0; j < yrg::kGuideOut; ++j) out[j] = lo[row * yrg::kGuideOut + j];
+        for (int j = 0; j < yrg::kGuideOut; ++j) out[j] = lo[row * yrg::kOutStride + j];
         float norm = A.w_norm;
         if (!(norm > 0.0f)) norm = count > 0 ? *A.wsum / static_cast<float>(count) : 1.0f;
         const float c = valid && norm > 0.0f ? w / norm / static_cast<float>(A.rows) : 0.0f;
         float loss = 0.0f;
         HeadGradient(out, wi[0], wi[1], wi[2], c, d, loss);
-        for (int j = 0; j < yrg::kGuideOut; ++j) lo[row * yrg::kGuideOut + j] = d[j];
+        for (int j = 0; j < yrg::kGuideOut; ++j) lo[row * yrg::kOutStride + j] = d[j];
         // One atomic per wave for the loss (lanes 0..15 hold the 16 rows):
         // every row adding to the one word serialised the whole step.
         if (!valid) loss = 0.0f;
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void GuideBackwardKernel(TrainArgs A) {
     __syncthreads();
     bf16x4_t dz3[3], dz2[4], dz1[4], dz0[4];
     for (int kb = 0; kb < 3; ++kb) {
-        const float* src = lo + row * yrg::kGuideOut + kb * 16 + 4 * q;
+        const float* src = lo + row * yrg::kOutStride + kb * 16 + 4 * q;
         dz3[kb] = yrg::PackBf16(src[0], src[1], src[2], src[3]);
     }
     for (int mb = 0; mb < 4; ++mb) {
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(kBlock) void GuideQueryRowsKernel(const float* __re
                                                                const bf16x4_t* __restrict__ fwd,
                                                                const float2* __restrict__ table, yrg::GuideGrid grid,
                                                                float* __restrict__ mixture) {
-    __shared__ float lds_o[4][64 * yrg::kGuideOut];
+    __shared__ float lds_o[4][64 * yrg::kOutStride];
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int wave = static_cast<int>(threadIdx.x >> 6);
     const int64_t base = (static_cast<int64_t>(blockIdx.x) * 4 + wave) * 64;
@@ -437,14 +437,14 @@ __global__ __launch_bounds__(kBlock) void GuideQueryRowsKernel(const float* __re
     float* lo = lds_o[wave];
     for (int rb = 0; rb < 4; ++rb) {
         for (int mb = 0; mb < 3; ++mb) {
-            for (int j = 0; j < 4; ++j) lo[(rb * 16 + lane % 16) * yrg::kGuideOut + mb * 16 + 4 * q + j] = o[rb][mb][j];
+            for (int j = 0; j < 4; ++j) lo[(rb * 16 + lane % 16) * yrg::kOutStride + mb * 16 + 4 * q + j] = o[rb][mb][j];
         }
     }
     __syncthreads();
     const int64_t r = base + lane;
     if (r < n) {
         yrg::Mixture m;
-        yrg::HeadFromOutputs(lo + lane * yrg::kGuideOut, m);
+        yrg::HeadFromOutputs(lo + lane * yrg::kOutStride, m);
         float* dst = mixture + 40 * r;
         for (int k = 0; k < yrg::kGuideLobes; ++k) {
             dst[k] = m.pi[k];

@@ -1032,8 +1032,8 @@ constexpr int kQueryWaves = kQueryBlock / 64;
 
 __global__ __launch_bounds__(kQueryBlock) void GuideQueryKernel(PassView P, SurfaceView S, LightView L,
                                                                yrg::GuideView G, int cur) {
-    __shared__ float lds_in[kQueryWaves][64][10];
-    __shared__ float lds_o[kQueryWaves][16 * yrg::kGuideOut];
+    __shared__ float lds_in[kQueryWaves][64][11];  // 11: an odd lane stride, no bank conflicts
+    __shared__ float lds_o[kQueryWaves][16 * yrg::kOutStride];
     __shared__ yrg::bf16x4_t lds_w[yrg::kFragCount * 64];  // weight fragments, shared by the block
     const int n = static_cast<int>(P.counters[cur]);
     if (static_cast<int64_t>(blockIdx.x) * blockDim.x >= n) return;  // whole block idle
@@ -1085,12 +1085,12 @@ __global__ __launch_bounds__(kQueryBlock) void GuideQueryKernel(PassView P, Surf
         yrg::f32x4_t o[1][3];
         yrg::MlpForward<1>(lds_w, lane, x, h0, h1, h2, o);
         for (int mb = 0; mb < 3; ++mb) {
-            for (int j = 0; j < 4; ++j) lo[row * yrg::kGuideOut + mb * 16 + 4 * q + j] = o[0][mb][j];
+            for (int j = 0; j < 4; ++j) lo[row * yrg::kOutStride + mb * 16 + 4 * q + j] = o[0][mb][j];
         }
         __syncthreads();
         const int64_t ri = wave_row0 + rb * 16 + row;
         if (ri < n && r[9] != 0.0f) {
-            const float* out = lo + row * yrg::kGuideOut;
+            const float* out = lo + row * yrg::kOutStride;
             float mx = out[0];
             for (int k = 1; k < yrg::kGuideLobes; ++k) mx = fmaxf(mx, out[5 * k]);
             float sum = 0.0f;
