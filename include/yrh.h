@@ -440,6 +440,12 @@ int yrh_generate_camera_rays(yrh_context* ctx, const yrh_render_desc* desc, floa
 int yrh_guide_configure(yrh_context* ctx, const yrh_guide_desc* desc);
 /* YRH_GUIDE_OFF / _SAMPLE / _RECORD for the following render passes. */
 int yrh_guide_set_mode(yrh_context* ctx, int32_t mode);
+/* Device memory (bytes) that sampling passes may spend on per-cell copies of
+ * the hash grid: levels are copied coarse first while they fit, each cell's 8
+ * corners in one 64-byte record, so a lookup is one load instead of 8 gathers.
+ * The features, and so the renders, are identical with any budget; 0 turns the
+ * copies off. Default 512 MiB. */
+int yrh_guide_set_cell_budget(yrh_context* ctx, int64_t bytes);
 /* `steps` Adam steps of `batch` recorded samples each (consecutive windows of
  * the recorded set), then forgets the recorded samples. */
 int yrh_guide_train(yrh_context* ctx, int32_t steps, int32_t batch);

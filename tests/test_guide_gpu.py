@@ -253,3 +253,35 @@ def test_guided_room_converges_to_64k_spp_reference(device):
     assert rel(g) < 0.9 * rel(p)
     assert abs((g @ lum).mean() / rl - 1.0) < 0.05
     assert abs((p @ lum).mean() / rl - 1.0) < 0.05
+
+
+def test_cell_copies_leave_guided_render_unchanged(device):
+    """Per-cell copies of the grid change how sampling passes read the
+    features, not their values: a guided render of the room is bit-identical
+    with no copies, the default budget (dense and coarse hashed levels
+    copied) and a budget that copies every level."""
+    s = Scene.room()
+    s.build_accel()
+    w, h = 160, 90
+    device.guide_configure([-0.1, -0.1, -0.1], [10.1, 4.1, 10.1], log2_table_size=18, min_resolution=8,
+                           max_resolution=512, learning_rate=3e-3, guide_fraction=0.5, sample_capacity=1 << 18,
+                           seed=7)
+    device.upload_scene(s)
+    device.upload_shading(s)
+    device.guide_set_mode(N.GUIDE_RECORD)
+    try:
+        for k in range(2):
+            device.render(s, w, h, 4, max_depth=8, upload=False, seed=700 + k)
+            device.guide_train(16, 8192)
+        films = []
+        for budget in (0, 512 << 20, 16 << 30, 0):
+            device.guide_set_cell_budget(budget)
+            device.guide_set_mode(N.GUIDE_SAMPLE)
+            film, _ = device.render(s, w, h, 8, max_depth=8, upload=False, seed=11, batch=8)
+            films.append(film)
+    finally:
+        device.guide_set_mode(N.GUIDE_OFF)
+        device.guide_set_cell_budget(512 << 20)
+    assert films[0]["samples"].sum() == w * h * 8
+    for f in films[1:]:
+        assert np.array_equal(f.view(np.uint8), films[0].view(np.uint8))

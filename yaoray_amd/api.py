@@ -546,6 +546,10 @@ class Device:
     def guide_set_mode(self, mode: int):
         self._check(self._lib.yrh_guide_set_mode(self._ctx, mode), "guide_set_mode")
 
+    def guide_set_cell_budget(self, nbytes: int):
+        """Device memory sampling passes may spend on per-cell grid copies (0 = off)."""
+        self._check(self._lib.yrh_guide_set_cell_budget(self._ctx, nbytes), "guide_set_cell_budget")
+
     def guide_train(self, steps: int, batch: int):
         self._check(self._lib.yrh_guide_train(self._ctx, steps, batch), "guide_train")
 

@@ -346,6 +346,47 @@ struct GuideView {
     uint32_t* sample_count;
     float* sample_wsum;
     int32_t sample_capacity;
+    // Per-cell copies of the grid for sampling passes (nullptr: none). Level l
+    // is copied when bit l of cell_mask is set: its cell (x, y, z), with res
+    // cells per axis, holds the features of its 8 corners in corner order as
+    // 16 floats (one 64-byte line) at cells + cell_off[l] + 4 (x + res (y + res z)).
+    const float4* cells;
+    int64_t cell_off[kGuideLevels];
+    uint32_t cell_mask;
 };
+
+// EncodeShare over the view: a copied level is one 64-byte load of the cell's
+// 8 corners instead of 8 gathers from the table. The corner values, weights
+// and summation order are EncodeShare's, so the features are identical.
+__device__ __forceinline__ void EncodeShareView(const GuideView& G, const float p[3], int q, float f[4]) {
+    for (int lv = 0; lv < 2; ++lv) {
+        const int level = 2 * q + lv;
+        const GridCell c = LocateCell(G.grid, level, p);
+        float a = 0.0f, b = 0.0f;
+        if ((G.cell_mask >> level) & 1u) {
+            const uint32_t r = static_cast<uint32_t>(G.grid.res[level]);
+            const float4* rec = G.cells + G.cell_off[level] +
+                                4 * static_cast<int64_t>(c.base[0] + r * (c.base[1] + r * c.base[2]));
+            const float4 v0 = rec[0], v1 = rec[1], v2 = rec[2], v3 = rec[3];
+            const float vx[8] = {v0.x, v0.z, v1.x, v1.z, v2.x, v2.z, v3.x, v3.z};
+            const float vy[8] = {v0.y, v0.w, v1.y, v1.w, v2.y, v2.w, v3.y, v3.w};
+            for (int k = 0; k < 8; ++k) {
+                const float w = CornerWeight(c, k);
+                a += w * vx[k];
+                b += w * vy[k];
+            }
+        } else {
+            const float2* table = G.table + static_cast<size_t>(level) * G.grid.table_size;
+            for (int k = 0; k < 8; ++k) {
+                const float w = CornerWeight(c, k);
+                const float2 v = table[CornerIndex(G.grid, level, c, k)];
+                a += w * v.x;
+                b += w * v.y;
+            }
+        }
+        f[2 * lv] = a;
+        f[2 * lv + 1] = b;
+    }
+}
 
 }  // namespace yrg

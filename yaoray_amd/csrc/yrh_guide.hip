@@ -13,6 +13,10 @@
 //   GuideAdamMlpKernel    sum of the partials + Adam on the FP32 master weights
 //   GuideAdamGridKernel   Adam on the touched hash-grid entries
 //   GuidePackKernel       master weights -> BF16 A-operand fragments
+//
+// Before the first sampling pass after a change of the parameters,
+// GuideCellPackKernel copies each level that fits the cell budget into
+// per-cell records (the 8 corners of a cell in one 64-byte line).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -25,6 +29,9 @@ namespace yrh {
 
 using yrg::bf16x4_t;
 using yrg::f32x4_t;
+
+// Device memory the per-cell grid copies may take (levels coarse first).
+constexpr int64_t kGuideCellBudgetDefault = static_cast<int64_t>(512) << 20;
 
 struct GuideState {
     yrh_guide_desc desc{};
@@ -46,6 +53,13 @@ struct GuideState {
     int64_t rec_slots = 0;
     int32_t stamp = 0;
     int64_t adam_step = 0;
+    // Per-cell copies of the grid levels in cell_mask; valid while
+    // cells_fresh (cleared by every change of the grid parameters).
+    DevBuf cells;
+    int64_t cell_off[yrg::kGuideLevels] = {};
+    uint32_t cell_mask = 0;
+    bool cells_fresh = false;
+    int64_t cell_budget = kGuideCellBudgetDefault;
 };
 
 namespace {
@@ -74,6 +88,21 @@ int Ensure(yrh_context* ctx, DevBuf& b, size_t bytes) {
 
 // Layer of a fragment / tile index and its local index.
 __device__ __forceinline__ int LayerOfFrag(int f) { return f < 8 ? 0 : (f < 24 ? 1 : (f < 40 ? 2 : 3)); }
+
+// Cell i of `level` (x fastest): the table entries of its 8 corners in corner
+// order, 4 float4 per cell.
+__global__ void GuideCellPackKernel(const float2* __restrict__ table, yrg::GuideGrid g, int level, uint32_t cells,
+                                    float4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    const uint32_t r = static_cast<uint32_t>(g.res[level]);
+    const uint32_t x = i % r, y = (i / r) % r, z = i / (r * r);
+    const float2* t = table + static_cast<size_t>(level) * g.table_size;
+    float2 v[8];
+    for (int k = 0; k < 8; ++k) v[k] = t[yrg::GridIndex(g, level, x + (k & 1), y + ((k >> 1) & 1), z + ((k >> 2) & 1))];
+    float4* o = out + 4 * static_cast<size_t>(i);
+    for (int j = 0; j < 4; ++j) o[j] = make_float4(v[2 * j].x, v[2 * j].y, v[2 * j + 1].x, v[2 * j + 1].y);
+}
 
 __global__ void GuidePackKernel(const float* __restrict__ mlp, bf16x4_t* __restrict__ fwd, bf16x4_t* __restrict__ bwd) {
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -502,6 +531,44 @@ int Pack(yrh_context* ctx, GuideState& G) {
     return CheckHip(ctx, hipGetLastError(), "guide pack");
 }
 
+// Per-cell copies for the sampling passes: levels are taken coarse first while
+// their records fit the budget. If the device cannot hold them the levels stay
+// on the table (a slower lookup, the same features).
+int PackCells(yrh_context* ctx, GuideState& G) {
+    G.cells_fresh = true;
+    G.cell_mask = 0;
+    int64_t total = 0;
+    uint32_t mask = 0;
+    for (int l = 0; l < yrg::kGuideLevels; ++l) {
+        const int64_t r = G.grid.res[l];
+        const int64_t bytes = r * r * r * 64;
+        if (r * r * r > 0x7FFFFFFF || total + bytes > G.cell_budget) break;
+        G.cell_off[l] = total / 16;
+        total += bytes;
+        mask |= 1u << l;
+    }
+    if (mask == 0) return YRH_OK;
+    if (G.cells.bytes < static_cast<size_t>(total)) {
+        G.cells.reset();
+        void* p = nullptr;
+        if (hipMalloc(&p, static_cast<size_t>(total)) != hipSuccess) {
+            (void)hipGetLastError();
+            return YRH_OK;
+        }
+        G.cells.ptr = p;
+        G.cells.bytes = static_cast<size_t>(total);
+    }
+    for (int l = 0; l < yrg::kGuideLevels; ++l) {
+        if (!((mask >> l) & 1u)) continue;
+        const int64_t r = G.grid.res[l];
+        const uint32_t n = static_cast<uint32_t>(r * r * r);
+        hipLaunchKernelGGL(GuideCellPackKernel, dim3(Grid(n)), dim3(kBlock), 0, ctx->stream, G.table.as<float2>(),
+                           G.grid, l, n, G.cells.as<float4>() + G.cell_off[l]);
+    }
+    G.cell_mask = mask;
+    return CheckHip(ctx, hipGetLastError(), "guide cell pack");
+}
+
 // Phase A + B of one step over `rows` rows from `samples` (count on device).
 int BackwardPass(yrh_context* ctx, GuideState& G, const float4* samples, const uint32_t* count, uint32_t capacity,
                  const float* wsum, float w_norm, int64_t offset, int64_t rows, float* loss) {
@@ -572,6 +639,7 @@ int GuidePrepareView(yrh_context* ctx, int64_t paths, int max_depth, yrg::GuideV
             G->rec_slots = slots;
         }
     }
+    if (G->mode == YRH_GUIDE_SAMPLE && !G->cells_fresh && (rc = PackCells(ctx, *G)) != YRH_OK) return rc;
     G->stamp = G->stamp == 0x3FFFFFFF ? 1 : G->stamp + 1;
     view->mode = G->mode;
     view->alpha = G->desc.guide_fraction;
@@ -589,6 +657,12 @@ int GuidePrepareView(yrh_context* ctx, int64_t paths, int max_depth, yrg::GuideV
     view->sample_count = G->sample_hdr.as<uint32_t>();
     view->sample_wsum = G->sample_hdr.as<float>() + 2;
     view->sample_capacity = G->desc.sample_capacity;
+    // Recording passes interleave with training steps: they read the table.
+    if (G->mode == YRH_GUIDE_SAMPLE && G->cell_mask != 0) {
+        view->cells = G->cells.as<float4>();
+        for (int l = 0; l < yrg::kGuideLevels; ++l) view->cell_off[l] = G->cell_off[l];
+        view->cell_mask = G->cell_mask;
+    }
     return CheckHip(ctx, hipGetLastError(), "guide view");
 }
 
@@ -676,6 +750,22 @@ int yrh_guide_set_mode(yrh_context* ctx, int32_t mode) {
     return YRH_OK;
 }
 
+int yrh_guide_set_cell_budget(yrh_context* ctx, int64_t bytes) {
+    int rc = NeedGuide(ctx);
+    if (rc != YRH_OK) return rc;
+    if (bytes < 0) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "negative cell budget");
+    GuideState& G = *ctx->guide;
+    if (bytes != G.cell_budget) {
+        // Earlier passes may still read the copies.
+        if ((rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize")) != YRH_OK) return rc;
+        G.cell_budget = bytes;
+        G.cells.reset();
+        G.cell_mask = 0;
+        G.cells_fresh = false;
+    }
+    return YRH_OK;
+}
+
 int yrh_guide_train(yrh_context* ctx, int32_t steps, int32_t batch) {
     int rc = NeedGuide(ctx);
     if (rc != YRH_OK) return rc;
@@ -699,6 +789,7 @@ int yrh_guide_train(yrh_context* ctx, int32_t steps, int32_t batch) {
                            static_cast<float*>(nullptr), G.grid_params, 1, a);
         if ((rc = Pack(ctx, G)) != YRH_OK) return rc;
     }
+    if (steps > 0) G.cells_fresh = false;
     hipLaunchKernelGGL(ResetHeaderKernel, dim3(1), dim3(64), 0, ctx->stream, hdr);
     return CheckHip(ctx, hipGetLastError(), "guide train");
 }
@@ -789,6 +880,7 @@ int yrh_guide_params_write(yrh_context* ctx, const float* mlp, const float* grid
     if (grid != nullptr && (rc = yrh_copy_to_device(ctx, G.table.ptr, grid, static_cast<size_t>(G.grid_params) * 4)) != YRH_OK) {
         return rc;
     }
+    G.cells_fresh = false;
     if ((rc = Pack(ctx, G)) != YRH_OK) return rc;
     return CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }

@@ -1028,12 +1028,22 @@ __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S,
 // HeadFromOutputs), written column-wise (value k of row i at
 // mix[k * mix_stride + i]).
 constexpr int kQueryBlock = 512;
+
+// Orders a wave's LDS writes before its own later LDS reads (the staging
+// areas below are private to one wave, so no workgroup barrier is needed).
+__device__ __forceinline__ void WaveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 constexpr int kQueryWaves = kQueryBlock / 64;
 
 __global__ __launch_bounds__(kQueryBlock) void GuideQueryKernel(PassView P, SurfaceView S, LightView L,
                                                                yrg::GuideView G, int cur) {
-    __shared__ float lds_in[kQueryWaves][64][11];  // 11: an odd lane stride, no bank conflicts
-    __shared__ float lds_o[kQueryWaves][16 * yrg::kOutStride];
+    // Per wave, first the inputs of its 64 rows (stride 11: odd, no bank
+    // conflicts), then the network outputs of 16 rows at a time.
+    __shared__ float lds_stage[kQueryWaves][16 * yrg::kOutStride];
+    static_assert(16 * yrg::kOutStride >= 64 * 11, "staging area holds the inputs");
     __shared__ yrg::bf16x4_t lds_w[yrg::kFragCount * 64];  // weight fragments, shared by the block
     const int n = static_cast<int>(P.counters[cur]);
     if (static_cast<int64_t>(blockIdx.x) * blockDim.x >= n) return;  // whole block idle
@@ -1055,7 +1065,7 @@ __global__ __launch_bounds__(kQueryBlock) void GuideQueryKernel(PassView P, Surf
             wo = sp.wo;
         }
     }
-    float* in = lds_in[wave][lane];
+    float* in = lds_stage[wave] + lane * 11;
     in[0] = hp.x; in[1] = hp.y; in[2] = hp.z;
     in[3] = sn.x; in[4] = sn.y; in[5] = sn.z;
     in[6] = wo.x; in[7] = wo.y; in[8] = wo.z;
@@ -1064,32 +1074,41 @@ __global__ __launch_bounds__(kQueryBlock) void GuideQueryKernel(PassView P, Surf
     __syncthreads();
     const int q = lane / 16;
     const int row = lane % 16;
-    float* lo = lds_o[wave];
+    float* lo = lds_stage[wave];
     const int64_t wave_row0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + wave * 64;
-#pragma unroll 1
+    // Encodings of all 4 row blocks first, so their grid loads are in flight
+    // together.
+    yrg::bf16x4_t xin[4][2];
+    bool use[4];
+#pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
-        const float* r = lds_in[wave][rb * 16 + row];
-        yrg::bf16x4_t x[1][2];
+        const float* r = lds_stage[wave] + (rb * 16 + row) * 11;
         float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (r[9] != 0.0f) {
+        use[rb] = r[9] != 0.0f;
+        if (use[rb]) {
             const float pp[3] = {r[0], r[1], r[2]};
             const float nn[3] = {r[3], r[4], r[5]};
             const float ww[3] = {r[6], r[7], r[8]};
-            yrg::EncodeShare(G.grid, G.table, pp, q, f);
-            x[0][1] = yrg::DirectionShare(q, nn, ww);
+            yrg::EncodeShareView(G, pp, q, f);
+            xin[rb][1] = yrg::DirectionShare(q, nn, ww);
         } else {
-            x[0][1] = yrg::PackBf16(0.0f, 0.0f, 0.0f, 0.0f);
+            xin[rb][1] = yrg::PackBf16(0.0f, 0.0f, 0.0f, 0.0f);
         }
-        x[0][0] = yrg::PackBf16(f[0], f[1], f[2], f[3]);
+        xin[rb][0] = yrg::PackBf16(f[0], f[1], f[2], f[3]);
+    }
+    WaveSync();  // the inputs are read; the area now takes the outputs
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+        const yrg::bf16x4_t x[1][2] = {{xin[rb][0], xin[rb][1]}};
         yrg::bf16x4_t h0[1][4], h1[1][4], h2[1][4];
         yrg::f32x4_t o[1][3];
         yrg::MlpForward<1>(lds_w, lane, x, h0, h1, h2, o);
         for (int mb = 0; mb < 3; ++mb) {
             for (int j = 0; j < 4; ++j) lo[row * yrg::kOutStride + mb * 16 + 4 * q + j] = o[0][mb][j];
         }
-        __syncthreads();
+        WaveSync();
         const int64_t ri = wave_row0 + rb * 16 + row;
-        if (ri < n && r[9] != 0.0f) {
+        if (ri < n && use[rb]) {
             const float* out = lo + row * yrg::kOutStride;
             float mx = out[0];
             for (int k = 1; k < yrg::kGuideLobes; ++k) mx = fmaxf(mx, out[5 * k]);
@@ -1110,7 +1129,7 @@ __global__ __launch_bounds__(kQueryBlock) void GuideQueryKernel(PassView P, Surf
                 col[(16 + 3 * k + 2) * G.mix_stride] = ok ? uz / len : 1.0f;
             }
         }
-        __syncthreads();
+        WaveSync();
     }
 }
 
