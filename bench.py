@@ -116,13 +116,17 @@ def main():
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain comparison")
     ap.add_argument("--guide-width", type=int, default=1280)
     ap.add_argument("--guide-height", type=int, default=720)
-    ap.add_argument("--guide-ref-spp", type=int, default=1024)
-    ap.add_argument("--guide-spp", default="16,32,64,128,256")
-    ap.add_argument("--guide-train-passes", type=int, default=16)
-    ap.add_argument("--guide-train-steps", type=int, default=8)
+    ap.add_argument("--guide-ref-spp", type=int, default=4096)
+    ap.add_argument("--guide-spp", default="16,32,64,128,256,512")
+    ap.add_argument("--guide-train-passes", type=int, default=8)
+    ap.add_argument("--guide-record-spp", type=int, default=2, help="sample indices per recording pass")
+    ap.add_argument("--guide-depths", type=int, default=3, help="guided bounces (0 = all that may continue)")
+    ap.add_argument("--guide-fraction", type=float, default=0.5)
+    ap.add_argument("--guide-train-steps", type=int, default=16)
     ap.add_argument("--guide-train-batch", type=int, default=32768)
     ap.add_argument("--guide-lr", type=float, default=3e-3)
-    ap.add_argument("--guide-threshold", type=float, default=0.2)
+    ap.add_argument("--guide-threshold", type=float, default=0.1,
+                    help="relMSE threshold of the headline time-to-threshold figure (0.2 is reported beside it)")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--lds-levels", type=int, default=0, help="traversal stack levels kept in LDS (1..16)")
     ap.add_argument("--block-size", type=int, default=0)
@@ -243,7 +247,8 @@ def main():
     if not args.no_guiding:
         guiding = bench_guiding(dev, args.guide_width, args.guide_height, args.guide_ref_spp,
                                 [int(v) for v in args.guide_spp.split(",")], args.guide_train_passes,
-                                args.guide_train_steps, args.guide_train_batch, args.guide_lr, args.guide_threshold)
+                                args.guide_train_steps, args.guide_train_batch, args.guide_lr, args.guide_threshold,
+                                args.guide_record_spp, args.guide_depths, args.guide_fraction)
 
     traffic = load_traffic("closest")
     result = {
@@ -405,13 +410,14 @@ def interp_loglog(xs, ys, x):
     return float(np.exp(ly[k] + t * (ly[k + 1] - ly[k])))
 
 
-def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_steps, train_batch, lr, threshold):
+def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_steps, train_batch, lr, threshold,
+                  record_spp=1, guide_depths=0, guide_fraction=0.5):
     """Equal-time relMSE of the guided and the plain wavefront integrator on the room
     workload against a plain FP64 reference. Each mode is one progressive render whose
     film is read at the spp checkpoints (readback excluded from the clock); the guided
-    render's first `train_passes` passes record training samples and each is followed
-    by `train_steps` Adam steps, all inside its clock, and those passes stay in its film
-    (every pass is an unbiased estimate)."""
+    render's first `train_passes` passes (`record_spp` sample indices each) record
+    training samples and each is followed by `train_steps` Adam steps, all inside its
+    clock, and those passes stay in its film (every pass is an unbiased estimate)."""
     from yaoray_amd import Scene
     from yaoray_amd import native as N
 
@@ -439,16 +445,17 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
         done = 0
         if guided:
             dev.guide_configure([-0.1, -0.1, -0.1], [10.1, 4.1, 10.1], log2_table_size=18, min_resolution=8,
-                                max_resolution=512, learning_rate=lr, guide_fraction=0.5,
-                                sample_capacity=1 << 21, seed=7)
+                                max_resolution=512, learning_rate=lr, guide_fraction=guide_fraction,
+                                sample_capacity=1 << 22, seed=7)
+            dev.guide_set_depth_limit(guide_depths)
         try:
             for checkpoint in sorted(spp_list):
                 dev.synchronize()
                 t = time.perf_counter()
                 while done < checkpoint:
-                    if guided and done < train_passes:
+                    if guided and done < train_passes * record_spp:
                         dev.guide_set_mode(N.GUIDE_RECORD)
-                        count = 1
+                        count = min(record_spp, checkpoint - done)
                     else:
                         if guided:
                             dev.guide_set_mode(N.GUIDE_SAMPLE)
@@ -457,7 +464,7 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
                     d.sample_count = count
                     dev.render_pass(d)
                     done += count
-                    if guided and done <= train_passes:
+                    if guided and done <= train_passes * record_spp:
                         dev.guide_train(train_steps, train_batch)
                 dev.synchronize()
                 clock += time.perf_counter() - t
@@ -494,7 +501,10 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
                     f"{width}x{height}, depth 8, OwenSobol",
         "reference": f"plain, {ref_spp} spp, FP64 film, {ref_s:.1f}s",
         "relmse": "mean over pixels and channels of (x - r)^2 / (r^2 + 0.01)",
-        "training": {"passes": train_passes, "steps_per_pass": train_steps, "batch": train_batch,
+        "guide_fraction": guide_fraction,
+        "guided_bounces": guide_depths if guide_depths > 0 else "all that may continue",
+        "training": {"passes": train_passes, "spp_per_pass": record_spp, "steps_per_pass": train_steps,
+                     "batch": train_batch,
                      "learning_rate": lr, "note": "inside the guided clock; its passes stay in the film"},
         "plain": plain,
         "guided": guided,
@@ -502,6 +512,8 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
         "threshold": threshold,
         "time_to_threshold_plain_s": time_to(plain, threshold),
         "time_to_threshold_guided_s": time_to(guided, threshold),
+        "time_to_threshold_by_level": {str(t): {"plain_s": time_to(plain, t), "guided_s": time_to(guided, t)}
+                                       for t in sorted({0.2, threshold}, reverse=True)},
     }
     tp, tg = out["time_to_threshold_plain_s"], out["time_to_threshold_guided_s"]
     out["time_to_threshold_speedup"] = round(tp / tg, 3) if tp and tg else None

@@ -446,6 +446,10 @@ int yrh_guide_set_mode(yrh_context* ctx, int32_t mode);
  * The features, and so the renders, are identical with any budget; 0 turns the
  * copies off. Default 512 MiB. */
 int yrh_guide_set_cell_budget(yrh_context* ctx, int64_t bytes);
+/* Guides only the vertices of the first `depths` bounces (0, the default:
+ * every vertex that may continue); deeper vertices sample their BSDF as the
+ * plain integrator does. Renders stay unbiased for any limit. */
+int yrh_guide_set_depth_limit(yrh_context* ctx, int32_t depths);
 /* `steps` Adam steps of `batch` recorded samples each (consecutive windows of
  * the recorded set), then forgets the recorded samples. */
 int yrh_guide_train(yrh_context* ctx, int32_t steps, int32_t batch);

@@ -170,8 +170,11 @@ def test_guided_render_keeps_closed_form(device, mode):
         assert count > 0 and mean_w > 0.0
 
 
-def test_guided_cornell_matches_plain_mean(device):
-    """Guided and plain renders have the same expectation. Per-pixel sample
+@pytest.mark.parametrize("depth_limit", [0, 2])
+def test_guided_cornell_matches_plain_mean(device, depth_limit):
+    """Guided and plain renders have the same expectation, with every
+    continuing vertex guided or only those of the first two bounces (plain
+    vertices after guided ones). Per-pixel sample
     variances understate the error of scrambled-Sobol pixel means, so the
     error is estimated from paired differences over 16 independent sampler
     seeds (a t statistic with 15 degrees of freedom, two-sided 0.1 % bound:
@@ -183,6 +186,7 @@ def test_guided_cornell_matches_plain_mean(device):
     configure(device, guide_desc(log2_table=16, rmin=8, rmax=128, bounds=((-1.05, -0.05, -1.05), (1.05, 2.05, 1.05))))
     device.upload_scene(s)
     device.upload_shading(s)
+    device.guide_set_depth_limit(depth_limit)
     device.guide_set_mode(N.GUIDE_RECORD)
     try:
         for k in range(4):  # a few training rounds on the integrator's own samples

@@ -550,6 +550,10 @@ class Device:
         """Device memory sampling passes may spend on per-cell grid copies (0 = off)."""
         self._check(self._lib.yrh_guide_set_cell_budget(self._ctx, nbytes), "guide_set_cell_budget")
 
+    def guide_set_depth_limit(self, depths: int):
+        """Guide only the first `depths` bounces (0 = every bounce that may continue)."""
+        self._check(self._lib.yrh_guide_set_depth_limit(self._ctx, depths), "guide_set_depth_limit")
+
     def guide_train(self, steps: int, batch: int):
         self._check(self._lib.yrh_guide_train(self._ctx, steps, batch), "guide_train")
 

@@ -353,6 +353,7 @@ struct GuideView {
     const float4* cells;
     int64_t cell_off[kGuideLevels];
     uint32_t cell_mask;
+    int32_t depth_limit;  // guided bounces: depth < depth_limit (0: every bounce that may continue)
 };
 
 // EncodeShare over the view: a copied level is one 64-byte load of the cell's

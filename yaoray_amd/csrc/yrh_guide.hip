@@ -60,6 +60,7 @@ struct GuideState {
     uint32_t cell_mask = 0;
     bool cells_fresh = false;
     int64_t cell_budget = kGuideCellBudgetDefault;
+    int32_t depth_limit = 0;
 };
 
 namespace {
@@ -137,6 +138,7 @@ struct TrainArgs {
     float w_norm;
     int64_t offset;  // first sample of this batch (mod count)
     int64_t rows;    // batch rows
+    int32_t spread;  // 1: row k of the run takes sample floor(frac(k phi) count), k = offset + row
     const bf16x4_t* fwd;
     const bf16x4_t* bwd;
     const float2* table;
@@ -209,7 +211,13 @@ __global__ __launch_bounds__(kBlock) void GuideBackwardKernel(TrainArgs A) {
     float p[3] = {0, 0, 0}, n[3] = {0, 0, 0}, wo[3] = {0, 0, 0}, wi[3] = {0, 0, 1};
     float w = 0.0f;
     if (valid) {
-        const int64_t s = (A.offset + idx) % static_cast<int64_t>(count);
+        // Spread batches: the recorded set is in path-slot order (depth-major,
+        // pixels in order), so a contiguous batch would see one depth of one
+        // image band. Row k takes frac(k phi) of the set instead (a Weyl
+        // sequence: every batch and every run of batches is spread evenly).
+        const uint64_t k = static_cast<uint64_t>(A.offset + idx);
+        const int64_t s = A.spread ? static_cast<int64_t>((((k * 0x9E3779B97F4A7C15ull) >> 32) * count) >> 32)
+                                   : static_cast<int64_t>(k % count);
         const float4 a = A.samples[4 * s], b = A.samples[4 * s + 1], c = A.samples[4 * s + 2], d = A.samples[4 * s + 3];
         p[0] = a.x; p[1] = a.y; p[2] = a.z; w = a.w;
         n[0] = b.x; n[1] = b.y; n[2] = b.z;
@@ -571,7 +579,7 @@ int PackCells(yrh_context* ctx, GuideState& G) {
 
 // Phase A + B of one step over `rows` rows from `samples` (count on device).
 int BackwardPass(yrh_context* ctx, GuideState& G, const float4* samples, const uint32_t* count, uint32_t capacity,
-                 const float* wsum, float w_norm, int64_t offset, int64_t rows, float* loss) {
+                 const float* wsum, float w_norm, int64_t offset, int64_t rows, float* loss, int spread) {
     const int64_t nrb = (rows + 15) / 16;
     int rc;
     if (G.scratch_blocks < nrb) {
@@ -591,6 +599,7 @@ int BackwardPass(yrh_context* ctx, GuideState& G, const float4* samples, const u
     A.wsum = wsum;
     A.w_norm = w_norm;
     A.offset = offset;
+    A.spread = spread;
     A.rows = rows;
     A.fwd = G.frag_fwd.as<bf16x4_t>();
     A.bwd = G.frag_bwd.as<bf16x4_t>();
@@ -657,6 +666,7 @@ int GuidePrepareView(yrh_context* ctx, int64_t paths, int max_depth, yrg::GuideV
     view->sample_count = G->sample_hdr.as<uint32_t>();
     view->sample_wsum = G->sample_hdr.as<float>() + 2;
     view->sample_capacity = G->desc.sample_capacity;
+    view->depth_limit = G->depth_limit;
     // Recording passes interleave with training steps: they read the table.
     if (G->mode == YRH_GUIDE_SAMPLE && G->cell_mask != 0) {
         view->cells = G->cells.as<float4>();
@@ -750,6 +760,14 @@ int yrh_guide_set_mode(yrh_context* ctx, int32_t mode) {
     return YRH_OK;
 }
 
+int yrh_guide_set_depth_limit(yrh_context* ctx, int32_t depths) {
+    int rc = NeedGuide(ctx);
+    if (rc != YRH_OK) return rc;
+    if (depths < 0) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "negative guide depth limit");
+    ctx->guide->depth_limit = depths;
+    return YRH_OK;
+}
+
 int yrh_guide_set_cell_budget(yrh_context* ctx, int64_t bytes) {
     int rc = NeedGuide(ctx);
     if (rc != YRH_OK) return rc;
@@ -775,7 +793,7 @@ int yrh_guide_train(yrh_context* ctx, int32_t steps, int32_t batch) {
     float* fh = G.sample_hdr.as<float>();
     for (int s = 0; s < steps; ++s) {
         if ((rc = BackwardPass(ctx, G, G.samples.as<float4>(), hdr, static_cast<uint32_t>(G.desc.sample_capacity),
-                               fh + 2, 0.0f, static_cast<int64_t>(s) * batch, batch, fh + 3)) != YRH_OK) {
+                               fh + 2, 0.0f, static_cast<int64_t>(s) * batch, batch, fh + 3, 1)) != YRH_OK) {
             return rc;
         }
         ++G.adam_step;
@@ -925,7 +943,7 @@ int yrh_guide_gradients(yrh_context* ctx, int64_t n, const float* samples, float
     if ((rc = yrh_copy_to_device(ctx, buf.ptr, rows.data(), rows.size() * 4)) != YRH_OK) return rc;
     if ((rc = yrh_copy_to_device(ctx, hdr.ptr, h, 16)) != YRH_OK) return rc;
     if ((rc = BackwardPass(ctx, G, buf.as<float4>(), hdr.as<uint32_t>(), static_cast<uint32_t>(n), hdr.as<float>() + 2,
-                           w_norm, 0, n, hdr.as<float>() + 3)) != YRH_OK) {
+                           w_norm, 0, n, hdr.as<float>() + 3, 0)) != YRH_OK) {
         return rc;
     }
     const int64_t chunks = ((n + 15) / 16 + kChunkBlocks - 1) / kChunkBlocks;

@@ -1138,43 +1138,55 @@ __global__ __launch_bounds__(kQueryBlock) void GuideQueryKernel(PassView P, Surf
 // luminance of the radiance the path gathered after the vertex over the
 // luminance of the throughput arriving at it, i.e. f cos L / p of the sampled
 // direction. Zero-weight samples carry no gradient and are skipped.
+// Each workgroup takes kCollectPer runs of kBlock consecutive slots and
+// appends its kept samples with one atomic (a single counter word taken by
+// every wave had cost more than the reads).
+constexpr int kCollectPer = 8;
+
 __global__ __launch_bounds__(kBlock) void GuideCollectKernel(PassView P, yrg::GuideView G) {
-    const int64_t slot = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int lane = static_cast<int>(threadIdx.x & 63u);
-    bool keep = false;
-    float w = 0.0f;
-    float4 r0 = make_float4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
-    if (slot < G.rec_paths * G.rec_depths) {
-        const float4* r = G.rec + 4 * slot;
-        r2 = r[2];
-        if (__float_as_int(r2.w) == G.stamp) {
-            r0 = r[0];
-            r1 = r[1];
-            r3 = r[3];
-            const float final_lum = Lum(XYZ(P.p_rad[slot % G.rec_paths]));
-            w = (final_lum - r1.w) / r0.w;
-            keep = isfinite(w) && w > 0.0f;
+    __shared__ uint32_t append_lds[kBlock / 64 + 1];
+    __shared__ float wsum_lds[kBlock / 64];
+    const int64_t total = G.rec_paths * G.rec_depths;
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * (kBlock * kCollectPer) + threadIdx.x;
+    float w[kCollectPer];
+    uint32_t keep = 0;
+    for (int j = 0; j < kCollectPer; ++j) {
+        const int64_t slot = first + j * kBlock;
+        w[j] = 0.0f;
+        if (slot < total) {
+            const float4* r = G.rec + 4 * slot;
+            if (__float_as_int(r[2].w) == G.stamp) {
+                const float final_lum = Lum(XYZ(P.p_rad[slot % G.rec_paths]));
+                w[j] = (final_lum - r[1].w) / r[0].w;
+                if (isfinite(w[j]) && w[j] > 0.0f) keep |= 1u << j;
+            }
         }
     }
-    const uint64_t mask = __ballot(keep);
-    if (mask == 0) return;
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(G.sample_count, static_cast<uint32_t>(__popcll(mask)));
-    base = __shfl(base, leader);
-    const uint32_t idx = base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
-    const bool stored = keep && idx < static_cast<uint32_t>(G.sample_capacity);
-    if (stored) {
-        float4* d = G.samples + 4 * static_cast<int64_t>(idx);
-        d[0] = F4(XYZ(r0), w);
-        d[1] = F4(XYZ(r1), 0.0f);
-        d[2] = F4(XYZ(r2), 0.0f);
-        d[3] = F4(XYZ(r3), 0.0f);
+    const int32_t base = BlockAppend(G.sample_count, static_cast<uint32_t>(__popc(keep)), append_lds);
+    float ws = 0.0f;
+    uint32_t idx = static_cast<uint32_t>(base);
+    for (int j = 0; j < kCollectPer; ++j) {
+        if (!((keep >> j) & 1u)) continue;
+        if (idx < static_cast<uint32_t>(G.sample_capacity)) {
+            const float4* r = G.rec + 4 * (first + j * kBlock);
+            float4* d = G.samples + 4 * static_cast<int64_t>(idx);
+            d[0] = F4(XYZ(r[0]), w[j]);
+            d[1] = F4(XYZ(r[1]), 0.0f);
+            d[2] = F4(XYZ(r[2]), 0.0f);
+            d[3] = F4(XYZ(r[3]), 0.0f);
+            ws += w[j];
+        }
+        ++idx;
     }
-    // The weight sum of the stored samples: one atomic per wave, not per sample.
-    float ws = stored ? w : 0.0f;
+    // The weight sum of the stored samples: one atomic per workgroup.
     for (int off = 32; off > 0; off >>= 1) ws += __shfl_xor(ws, off);
-    if (lane == leader && ws != 0.0f) atomicAdd(G.sample_wsum, ws);
+    if ((threadIdx.x & 63u) == 0) wsum_lds[threadIdx.x >> 6] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int k = 0; k < kBlock / 64; ++k) t += wsum_lds[k];
+        if (t != 0.0f) atomicAdd(G.sample_wsum, t);
+    }
 }
 
 // Adds the visible next-event contributions of this bounce, emissive then
@@ -1567,7 +1579,10 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
         if ((rc = LaunchTrace(ctx, tr)) != YRH_OK) return rc;
         PassView Pk = P;
         if (ctx->geo.accel_kind != YRH_ACCEL_TWO_LEVEL) Pk.hit_inst = nullptr;
-        if (guided && depth + 1 < max_depth) {
+        // Guided bounces: all that may continue, or the first depth_limit. A
+        // plain vertex after guided ones is the plain estimator's vertex (its
+        // NEE and emission weights use the pdf its direction was drawn from).
+        if (guided && depth + 1 < max_depth && (GV.depth_limit <= 0 || depth < GV.depth_limit)) {
             hipLaunchKernelGGL(GuideQueryKernel, dim3(Grid(paths, kQueryBlock)), dim3(kQueryBlock), 0, st, Pk, S, L, GV,
                                cur);
             hipLaunchKernelGGL(ShadeKernel<true>, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
@@ -1613,7 +1628,8 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
         hipLaunchKernelGGL(ResolveKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P, GV, depth, cur);
     }
     if (GV.rec != nullptr) {
-        hipLaunchKernelGGL(GuideCollectKernel, dim3(Grid(paths * max_depth)), dim3(kBlock), 0, st, P, GV);
+        hipLaunchKernelGGL(GuideCollectKernel, dim3(Grid(paths * max_depth, kBlock * kCollectPer)), dim3(kBlock), 0, st,
+                           P, GV);
     }
     return YRH_OK;
 }

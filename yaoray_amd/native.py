@@ -235,6 +235,7 @@ YRH_SIGNATURES = {
     "yrh_guide_configure": (_I, [_VP, C.POINTER(GuideDesc)]),
     "yrh_guide_set_mode": (_I, [_VP, _I32]),
     "yrh_guide_set_cell_budget": (_I, [_VP, _I64]),
+    "yrh_guide_set_depth_limit": (_I, [_VP, _I32]),
     "yrh_guide_train": (_I, [_VP, _I32, _I32]),
     "yrh_guide_add_samples": (_I, [_VP, _I64, _VP]),
     "yrh_guide_read_samples": (_I, [_VP, _I64, _VP, C.POINTER(_I64)]),
