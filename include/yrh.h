@@ -319,7 +319,19 @@ typedef struct yrh_render_desc {
     int32_t sample_index; /* this pass's first sample index (= completed spp) */
     int32_t sample_count; /* consecutive sample indices rendered by this pass (0 means 1); the
                              film receives them in index order, as that many passes would add them */
+    int32_t pixel_list;   /* 1: render one sample for each pixel of the list the last
+                             yrh_film_update_active wrote, each at its own next sample index
+                             (its film sample count); sample_index and sample_count are ignored */
 } yrh_render_desc;
+
+/* yr::AdaptiveSamplingSettings (include/yaoray/scene/render_options.hpp:30-36). */
+typedef struct yrh_adaptive_desc {
+    int32_t enabled;
+    int32_t min_spp;
+    float relative_error;
+    float absolute_error;
+    float confidence;
+} yrh_adaptive_desc;
 
 /* ---- learned path guiding (no counterpart in the reference) -------------------
  * A hash-grid + MLP proposal of 8 vMF lobes over directions, conditioned on
@@ -377,6 +389,14 @@ int yrh_film_load(yrh_context* ctx, const yrh_film_pixel* pixels, int64_t count)
 int yrh_film_read(yrh_context* ctx, yrh_film_pixel* pixels, int64_t count);
 /* Runs one pass on the context's stream, accumulating into the film. */
 int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* desc);
+/* The active pixels of RenderCpuPathTrace (src/backends/cpu/cpu_path_tracer.cpp:95-106,
+ * 183-199) from the device film: those with fewer than spp samples that are not
+ * converged by Film::IsConverged (src/film/film.cpp:113-121; only when
+ * adaptive->enabled; the test runs on the pixel statistics rounded to FP32).
+ * The list stays on the device for pixel-list passes. With active or converged
+ * non-null it synchronises and returns the list length and the converged count. */
+int yrh_film_update_active(yrh_context* ctx, const yrh_adaptive_desc* adaptive, int32_t spp, int64_t* active,
+                           int64_t* converged);
 /* Paths (pixels x sample indices) one pass should hold: as many as half of
  * the device memory not in use (plus what this context's path state already
  * holds) can keep, at ~460 bytes per path (+ the guiding buffers when a

@@ -232,6 +232,32 @@ def render(scene, width, height, spp, max_depth=8, sampler=2, seed=0x59414F52415
     return film, dict(zip(STAT_KEYS, (int(x) for x in stats)))
 
 
+def render_adaptive(scene, width, height, spp, adaptive, max_depth=8, sampler=2, seed=0x59414F524159,
+                    radiance_clamp=0.0, threads=8, film=None, first_pass=0):
+    """The restated pass loop with active pixels (adaptive sampling when
+    adaptive = (min_spp, relative_error, absolute_error, confidence); None turns it
+    off). Continues `film` (FP32 records) from pass `first_pass` when given.
+    Returns (film, stats dict, passes run, converged pixels)."""
+    from yaoray_amd.api import FILM32_DTYPE
+
+    lib = oracle_lib()
+    lib.oracle_render_adaptive.argtypes = [_VP, _VP, _VP, _VP, _I32, _I32, _VP, _VP, C.POINTER(_I32),
+                                           C.POINTER(C.c_uint64)]
+    gd = scene.geometry_desc()
+    sd = scene.shading_desc()
+    rd = scene.render_desc(width, height, spp, max_depth, sampler, seed, radiance_clamp, 0)
+    film = np.zeros(width * height, FILM32_DTYPE) if film is None else np.array(film, FILM32_DTYPE)
+    a = np.array([0.0, 0.0, 0.0, 0.0, 0.0] if adaptive is None else [1.0, *adaptive], np.float32)
+    stats = np.zeros(9, np.uint64)
+    passes, conv = _I32(), C.c_uint64()
+    rc = lib.oracle_render_adaptive(C.cast(C.pointer(gd), _VP), C.cast(C.pointer(sd), _VP), C.cast(C.pointer(rd), _VP),
+                                    a.ctypes.data, first_pass, threads, film.ctypes.data, stats.ctypes.data,
+                                    C.byref(passes), C.byref(conv))
+    if rc != 0:
+        raise RuntimeError("oracle_render_adaptive failed")
+    return film, dict(zip(STAT_KEYS, (int(x) for x in stats))), passes.value, conv.value
+
+
 def sample_values(kind, seed, spp, x, y, sample, dim, dims):
     lib = oracle_lib()
     lib.oracle_sample_values.argtypes = [_I32, C.c_uint64, _I32, _I64, _VP, _VP, _VP, _VP, _I32, _VP]
@@ -242,12 +268,19 @@ def sample_values(kind, seed, spp, x, y, sample, dim, dims):
 
 
 def ref_render(scene, width, height, spp, max_depth=8, sampler=2, seed=0x59414F524159, radiance_clamp=0.0,
-               threads=8):
-    """The reference's own PrepareCpuScene + RenderCpuPathTrace on the same inputs.
-    Returns (FP32 film records, stats dict, render seconds)."""
+               threads=8, adaptive=None):
+    """The reference's own PrepareCpuScene + RenderCpuPathTrace on the same inputs, with
+    adaptive = (min_spp, relative_error, absolute_error, confidence) or None (off).
+    Returns (FP32 film records, stats dict with converged_pixels, render seconds)."""
     from yaoray_amd.api import FILM32_DTYPE
 
     lib = ref_lib()
+    lib.ref_set_adaptive.argtypes = [_I32, _I32, C.c_float, C.c_float, C.c_float]
+    lib.ref_last_converged.restype = C.c_uint64
+    if adaptive is None:
+        lib.ref_set_adaptive(0, 16, 0.05, 0.001, 1.96)
+    else:
+        lib.ref_set_adaptive(1, int(adaptive[0]), *[float(v) for v in adaptive[1:]])
     lib.ref_render.argtypes = [_VP, _I64, _VP, _I64, _VP, _I32, _VP, _VP, _I32, _VP, _VP, _VP, _I32, _VP, _VP,
                                C.POINTER(C.c_double)]
     v, i, p = scene.geometry()
@@ -272,6 +305,8 @@ def ref_render(scene, width, height, spp, max_depth=8, sampler=2, seed=0x59414F5
         raise RuntimeError(f"reference render failed ({rc})")
     out = dict(zip(STAT_KEYS, (int(x) for x in stats[:9])))
     out["threads"] = int(stats[9])
+    out["converged_pixels"] = int(lib.ref_last_converged())
+    lib.ref_set_adaptive(0, 16, 0.05, 0.001, 1.96)
     return film, out, secs.value
 
 

@@ -16,6 +16,8 @@
 //                           path_direct_lighting.cpp:23-283, path_integrator_common.cpp:15-201
 //   film                    src/film/film.cpp:32-56 (FP32, as the reference)
 //   pass driver             src/backends/cpu/cpu_path_tracer.cpp:109-180
+//   adaptive sampling       cpu_path_tracer.cpp:95-106, 183-199 (active pixels),
+//                           src/film/film.cpp:98-121 (IsConverged)
 // Same operation order and the same libm calls as the reference, so on one
 // machine its films equal the reference build's bit for bit (pinned by
 // tests/test_oracle_render.py against oracle/_ref).
@@ -1092,16 +1094,9 @@ void AddSample(FilmPixel32& p, const PathOut& s, uint64_t& bad) {
 
 }  // namespace
 
-extern "C" {
-#pragma GCC visibility push(default)
+namespace {
 
-// Renders passes [sample_begin, sample_begin + pass_count) of `rd` into `film`
-// (width*height FilmPixel records of 56 bytes, accumulated in place).
-// stats: rays, shadow, occluded, triangle tests, sphere tests, node tests,
-// hits, misses, bad samples (uint64 each, accumulated).
-int oracle_render(const yrh_geometry_desc* gd, const yrh_shading_desc* sd, const yrh_render_desc* rd,
-                  int32_t sample_begin, int32_t pass_count, int32_t threads, void* film, uint64_t* stats) {
-    if (gd == nullptr || sd == nullptr || rd == nullptr || film == nullptr || stats == nullptr) return -1;
+Scene MakeScene(const yrh_geometry_desc* gd, const yrh_shading_desc* sd) {
     Scene s;
     s.g = orc::Geometry{gd->vertices, gd->vertex_count, gd->indices, gd->index_count, gd->primitives, gd->primitive_count};
     s.gd = gd;
@@ -1109,49 +1104,52 @@ int oracle_render(const yrh_geometry_desc* gd, const yrh_shading_desc* sd, const
     s.two_level = gd->accel_kind == YRH_ACCEL_TWO_LEVEL;
     s.tl = orc::TwoLevel{gd->blases, gd->blas_count, gd->instances, gd->instance_count, &gd->tlas, gd->tlas_primitives,
                          gd->tlas_primitive_count};
-    const int W = rd->width, H = rd->height;
-    const int64_t npix = static_cast<int64_t>(W) * H;
-    auto* px = static_cast<FilmPixel32*>(film);
-    std::vector<PathOut> pass(static_cast<size_t>(npix));
-    std::vector<Stats> per_thread(static_cast<size_t>(std::max(1, threads)));
+    return s;
+}
+
+// One pass: pixels[i] renders sample index sample_of(i) into out[pixels[i]]
+// (camera ray from the Pixel sample, then the path loop), on `threads` workers.
+template <class SampleOf>
+void RenderPixels(const Scene& s, const yrh_render_desc& rd, const std::vector<int64_t>& pixels, SampleOf sample_of,
+                  int threads, std::vector<Stats>& per_thread, std::vector<PathOut>& out) {
+    const int W = rd.width, H = rd.height;
     const float w = static_cast<float>(std::max(1, W)), h = static_cast<float>(std::max(1, H));
-    const V3 co = Load3(rd->camera_origin), cf = Load3(rd->camera_forward), cr = Load3(rd->camera_right),
-             cu = Load3(rd->camera_up);
-    const float hh = rd->tan_half_fov;
-    for (int pi = 0; pi < pass_count; ++pi) {
-        const int sample = sample_begin + pi;
-        std::atomic_int64_t next{0};
-        auto work = [&](int tid) {
-            Stats& st = per_thread[static_cast<size_t>(tid)];
-            while (true) {
-                const int64_t b = next.fetch_add(256);
-                if (b >= npix) break;
-                const int64_t e = std::min(npix, b + 256);
-                for (int64_t p = b; p < e; ++p) {
-                    const int x = static_cast<int>(p % W), y = static_cast<int>(p / W);
-                    Sampler smp(rd->sampler, rd->seed, x, y, sample, rd->spp);
-                    float sx, sy;
-                    smp.At2(DimPixel, 0, sx, sy);
-                    const float fx = static_cast<float>(x) + sx, fy = static_cast<float>(y) + sy;
-                    const float aspect = w / h;
-                    const float scx = (2.0f * fx / w - 1.0f) * aspect * hh;
-                    const float scy = (1.0f - 2.0f * fy / h) * hh;
-                    const V3 dir = Normalize(Add(Add(cf, Mul(cr, scx)), Mul(cu, scy)));
-                    pass[static_cast<size_t>(p)] = Integrate(s, *rd, co, dir, smp, st);
-                }
+    const V3 co = Load3(rd.camera_origin), cf = Load3(rd.camera_forward), cr = Load3(rd.camera_right),
+             cu = Load3(rd.camera_up);
+    const float hh = rd.tan_half_fov;
+    const int64_t n = static_cast<int64_t>(pixels.size());
+    std::atomic_int64_t next{0};
+    auto work = [&](int tid) {
+        Stats& st = per_thread[static_cast<size_t>(tid)];
+        while (true) {
+            const int64_t b = next.fetch_add(256);
+            if (b >= n) break;
+            const int64_t e = std::min(n, b + 256);
+            for (int64_t i = b; i < e; ++i) {
+                const int64_t p = pixels[static_cast<size_t>(i)];
+                const int x = static_cast<int>(p % W), y = static_cast<int>(p / W);
+                Sampler smp(rd.sampler, rd.seed, x, y, sample_of(i), rd.spp);
+                float sx, sy;
+                smp.At2(DimPixel, 0, sx, sy);
+                const float fx = static_cast<float>(x) + sx, fy = static_cast<float>(y) + sy;
+                const float aspect = w / h;
+                const float scx = (2.0f * fx / w - 1.0f) * aspect * hh;
+                const float scy = (1.0f - 2.0f * fy / h) * hh;
+                const V3 dir = Normalize(Add(Add(cf, Mul(cr, scx)), Mul(cu, scy)));
+                out[static_cast<size_t>(p)] = Integrate(s, rd, co, dir, smp, st);
             }
-        };
-        if (threads <= 1) {
-            work(0);
-        } else {
-            std::vector<std::thread> pool;
-            for (int t = 0; t < threads; ++t) pool.emplace_back(work, t);
-            for (auto& th : pool) th.join();
         }
-        uint64_t bad = 0;
-        for (int64_t p = 0; p < npix; ++p) AddSample(px[p], pass[static_cast<size_t>(p)], bad);
-        stats[8] += bad;
+    };
+    if (threads <= 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t) pool.emplace_back(work, t);
+        for (auto& th : pool) th.join();
     }
+}
+
+void AddStats(const std::vector<Stats>& per_thread, uint64_t* stats) {
     for (const Stats& st : per_thread) {
         stats[0] += st.rays;
         stats[1] += st.shadow;
@@ -1162,6 +1160,98 @@ int oracle_render(const yrh_geometry_desc* gd, const yrh_shading_desc* sd, const
         stats[6] += st.hits;
         stats[7] += st.misses;
     }
+}
+
+// Film::IsConverged (src/film/film.cpp:113-121) with LuminanceVariance and
+// LuminanceStandardError (film.cpp:98-111), in FP32.
+bool IsConverged(const FilmPixel32& p, int min_spp, float relative_error, float absolute_error, float confidence) {
+    if (p.samples < static_cast<uint32_t>(std::max(2, min_spp))) return false;
+    const float variance = p.samples > 1 ? p.m2 / static_cast<float>(p.samples - 1) : 0.0f;
+    const float stderr_mean = p.samples > 1 ? std::sqrt(std::max(0.0f, variance) / static_cast<float>(p.samples))
+                                            : std::numeric_limits<float>::infinity();
+    const float bound = std::max(0.0f, confidence) * stderr_mean;
+    const float tolerance = std::max(0.0f, absolute_error) + std::max(0.0f, relative_error) * std::fabs(p.mean);
+    return std::isfinite(bound) && bound <= tolerance;
+}
+
+}  // namespace
+
+extern "C" {
+#pragma GCC visibility push(default)
+
+// Renders passes [sample_begin, sample_begin + pass_count) of `rd` into `film`
+// (width*height FilmPixel records of 56 bytes, accumulated in place).
+// stats: rays, shadow, occluded, triangle tests, sphere tests, node tests,
+// hits, misses, bad samples (uint64 each, accumulated).
+int oracle_render(const yrh_geometry_desc* gd, const yrh_shading_desc* sd, const yrh_render_desc* rd,
+                  int32_t sample_begin, int32_t pass_count, int32_t threads, void* film, uint64_t* stats) {
+    if (gd == nullptr || sd == nullptr || rd == nullptr || film == nullptr || stats == nullptr) return -1;
+    const Scene s = MakeScene(gd, sd);
+    const int64_t npix = static_cast<int64_t>(rd->width) * rd->height;
+    auto* px = static_cast<FilmPixel32*>(film);
+    std::vector<PathOut> pass(static_cast<size_t>(npix));
+    std::vector<Stats> per_thread(static_cast<size_t>(std::max(1, threads)));
+    std::vector<int64_t> all(static_cast<size_t>(npix));
+    for (int64_t p = 0; p < npix; ++p) all[static_cast<size_t>(p)] = p;
+    for (int pi = 0; pi < pass_count; ++pi) {
+        const int sample = sample_begin + pi;
+        RenderPixels(s, *rd, all, [sample](int64_t) { return sample; }, threads, per_thread, pass);
+        uint64_t bad = 0;
+        for (int64_t p = 0; p < npix; ++p) AddSample(px[p], pass[static_cast<size_t>(p)], bad);
+        stats[8] += bad;
+    }
+    AddStats(per_thread, stats);
+    return 0;
+}
+
+// RenderCpuPathTrace's pass loop with its active pixels
+// (src/backends/cpu/cpu_path_tracer.cpp:95-106, 109-199): before each pass the
+// active pixels are those below rd->spp samples that are not converged
+// (IsConverged, when adaptive[0] != 0); each renders its next sample (index =
+// its film sample count); the loop ends after pass rd->spp - 1 or when none is
+// active. adaptive: enabled, min_spp, relative_error, absolute_error,
+// confidence (as floats). first_pass = resume_completed_spp. passes: passes run;
+// converged: converged pixels after the last pass.
+int oracle_render_adaptive(const yrh_geometry_desc* gd, const yrh_shading_desc* sd, const yrh_render_desc* rd,
+                           const float* adaptive, int32_t first_pass, int32_t threads, void* film, uint64_t* stats,
+                           int32_t* passes, uint64_t* converged) {
+    if (gd == nullptr || sd == nullptr || rd == nullptr || adaptive == nullptr || film == nullptr || stats == nullptr) {
+        return -1;
+    }
+    const Scene s = MakeScene(gd, sd);
+    const int64_t npix = static_cast<int64_t>(rd->width) * rd->height;
+    auto* px = static_cast<FilmPixel32*>(film);
+    const int spp = std::max(1, rd->spp);
+    const bool on = adaptive[0] != 0.0f;
+    const int min_spp = static_cast<int>(adaptive[1]);
+    auto collect = [&](std::vector<int64_t>& list) {
+        list.clear();
+        uint64_t conv = 0;
+        for (int64_t p = 0; p < npix; ++p) {
+            const bool c = on && IsConverged(px[p], min_spp, adaptive[2], adaptive[3], adaptive[4]);
+            if (c) ++conv;
+            if (!c && px[p].samples < static_cast<uint32_t>(spp)) list.push_back(p);
+        }
+        return conv;
+    };
+    std::vector<PathOut> pass(static_cast<size_t>(npix));
+    std::vector<Stats> per_thread(static_cast<size_t>(std::max(1, threads)));
+    std::vector<int64_t> active;
+    uint64_t conv = collect(active);
+    int ran = 0;
+    for (int pi = first_pass; pi < spp && !active.empty(); ++pi) {
+        RenderPixels(s, *rd, active,
+                     [&](int64_t i) { return static_cast<int>(px[active[static_cast<size_t>(i)]].samples); },
+                     threads, per_thread, pass);
+        uint64_t bad = 0;
+        for (int64_t p : active) AddSample(px[p], pass[static_cast<size_t>(p)], bad);
+        stats[8] += bad;
+        conv = collect(active);
+        ++ran;
+    }
+    AddStats(per_thread, stats);
+    if (passes != nullptr) *passes = ran;
+    if (converged != nullptr) *converged = conv;
     return 0;
 }
 

@@ -196,6 +196,22 @@ int ref_trace(void* h, const float* org_tmin, const float* dir_tmax, int64_t n, 
 // for settings.spp passes. film: width*height FilmPixel (56 B) records.
 // stats: rays, shadow, occluded, triangle tests, sphere tests, node tests,
 // hits, misses, bad samples, threads; seconds = render seconds (excludes prepare).
+// Adaptive-sampling settings the next ref_render calls use
+// (yr::AdaptiveSamplingSettings, include/yaoray/scene/render_options.hpp:30-36),
+// and the converged-pixel count of the last ref_render.
+static yr::AdaptiveSamplingSettings g_adaptive{};
+static uint64_t g_last_converged = 0;
+
+void ref_set_adaptive(int32_t enabled, int32_t min_spp, float relative_error, float absolute_error, float confidence) {
+    g_adaptive.enabled = enabled != 0;
+    g_adaptive.min_spp = min_spp;
+    g_adaptive.relative_error = relative_error;
+    g_adaptive.absolute_error = absolute_error;
+    g_adaptive.confidence = confidence;
+}
+
+uint64_t ref_last_converged() { return g_last_converged; }
+
 int ref_render(const yrh_vertex* vertices, int64_t nv, const uint32_t* indices, int64_t ni,
                const yrh_primitive* primitives, int32_t np, const int32_t* inst_prim, const float* inst_mat,
                int32_t ninst, const yrh_shading_desc* sd, const float* camera /* origin, forward, right, up, fov */,
@@ -266,6 +282,7 @@ int ref_render(const yrh_vertex* vertices, int64_t nv, const uint32_t* indices, 
     st.seed = rd->seed;
     st.threads = threads;
     st.radiance_clamp = rd->radiance_clamp;
+    st.adaptive_sampling = g_adaptive;
     st.sampler = rd->sampler == YRH_SAMPLER_INDEPENDENT ? yr::RenderSamplerKind::Independent
                : rd->sampler == YRH_SAMPLER_STRATIFIED ? yr::RenderSamplerKind::Stratified
                : rd->sampler == YRH_SAMPLER_OWEN_SOBOL ? yr::RenderSamplerKind::OwenSobol
@@ -274,6 +291,7 @@ int ref_render(const yrh_vertex* vertices, int64_t nv, const uint32_t* indices, 
     if (!prep.ok) return -1;
     const yr::CpuPathTraceResult r = yr::RenderCpuPathTrace(*prep.scene);
     if (!r.ok) return -2;
+    g_last_converged = r.stats.converged_pixels;
     static_assert(sizeof(yr::FilmPixel) == 56);
     std::memcpy(film_out, r.film.Pixels().data(), r.film.Pixels().size() * 56);
     const uint64_t vals[10] = {r.stats.rays_traced, r.stats.shadow_rays, r.stats.occluded_shadow_rays,

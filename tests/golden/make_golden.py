@@ -3,7 +3,7 @@
 Run in a container where the reference's sources are present and
 oracle/_ref/libyrref.so is built from them (`make -C oracle ref`):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture names]   # default: all
 
 Each fixture holds its inputs (or, for the procedural Cornell workload, a
 digest of them) and the reference's outputs on them, so tests can pin the
@@ -12,8 +12,9 @@ reference-only pins of the traversal and path-tracing path: SAH node arrays
 (src/accel/bvh_split_policy.cpp:105-112 order), per-ray BvhHit + node /
 triangle counters (src/accel/bvh_traversal.cpp:253-345,
 src/accel/two_level_bvh.cpp:365-481), yr::Sampler bits
-(src/sampling/sampler.cpp) and an FP32 film + stats of RenderCpuPathTrace
-(src/backends/cpu/cpu_path_tracer.cpp).
+(src/sampling/sampler.cpp) and FP32 films + stats of RenderCpuPathTrace
+(src/backends/cpu/cpu_path_tracer.cpp), plain and with adaptive sampling
+(its active-pixel loop and Film::IsConverged, src/film/film.cpp:113-121).
 """
 import hashlib
 import os
@@ -132,15 +133,40 @@ def cornell_fixture():
             "seed": np.uint64(SEED)}
 
 
-def main():
+# Adaptive sampling settings of the fixture: (min_spp, relative_error,
+# absolute_error, confidence) for depth 8 and depth 1.
+ADAPTIVE8 = (4, 0.1, 0.001, 1.96)
+ADAPTIVE1 = (4, 0.05, 0.001, 1.96)
+
+
+def cornell_adaptive_fixture():
+    s = Scene.cornell(1)
+    s.build_accel()
+    film8, st8, _ = O.ref_render(s, 64, 64, 32, max_depth=8, sampler=2, seed=SEED, adaptive=ADAPTIVE8)
+    film1, st1, _ = O.ref_render(s, 64, 64, 16, max_depth=1, sampler=2, seed=SEED, adaptive=ADAPTIVE1)
+    keys = O.STAT_KEYS
+    return {"digest": np.array(geometry_digest(s)), "width": np.int32(64), "height": np.int32(64),
+            "film8": film8.view(np.uint8).reshape(-1), "stats8": np.array([st8[k] for k in keys], np.uint64),
+            "spp8": np.int32(32), "adaptive8": np.array(ADAPTIVE8, np.float32),
+            "converged8": np.uint64(st8["converged_pixels"]),
+            "film1": film1.view(np.uint8).reshape(-1), "stats1": np.array([st1[k] for k in keys], np.uint64),
+            "spp1": np.int32(16), "adaptive1": np.array(ADAPTIVE1, np.float32),
+            "converged1": np.uint64(st1["converged_pixels"]), "seed": np.uint64(SEED)}
+
+
+FIXTURES = {"sampler": sampler_fixture, "flat_trace": flat_fixture, "two_level_trace": two_level_fixture,
+            "cornell_film": cornell_fixture, "cornell_adaptive": cornell_adaptive_fixture}
+
+
+def main(names=None):
     if not O.ref_available():
         raise SystemExit("oracle/_ref/libyrref.so is missing: build it with `make -C oracle ref`")
-    for name, fn in (("sampler", sampler_fixture), ("flat_trace", flat_fixture),
-                     ("two_level_trace", two_level_fixture), ("cornell_film", cornell_fixture)):
+    for name in names or FIXTURES:
+        fn = FIXTURES[name]
         path = os.path.join(HERE, name + ".npz")
         np.savez_compressed(path, **fn())
         print(f"{path}: {os.path.getsize(path)} bytes")
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -68,3 +68,40 @@ def test_oracle_render_matches_cornell_fixture(depth):
     film, st = O.render(s, w, h, spp, max_depth=depth, sampler=2, seed=int(f["seed"]))
     assert film.tobytes() == G.film(f[f"film{depth}"]).tobytes()
     assert st == G.stats(f[f"stats{depth}"])
+
+
+@pytest.mark.parametrize("depth", [1, 8])
+def test_oracle_adaptive_render_matches_reference_fixture(depth):
+    """The restated active-pixel loop with Film::IsConverged against the
+    reference's adaptive RenderCpuPathTrace: per-pixel sample counts, film and
+    counters bit for bit, and the converged-pixel count."""
+    f = G.load("cornell_adaptive")
+    s = G.cornell_scene(f)
+    w, h = int(f["width"]), int(f["height"])
+    spp = int(f[f"spp{depth}"])
+    a = tuple(float(v) for v in f[f"adaptive{depth}"])
+    film, st, passes, conv = O.render_adaptive(s, w, h, spp, (int(a[0]), *a[1:]), max_depth=depth, sampler=2,
+                                               seed=int(f["seed"]))
+    exp = G.film(f[f"film{depth}"])
+    assert np.array_equal(film["samples"], exp["samples"])
+    assert film.tobytes() == exp.tobytes()
+    assert st == G.stats(f[f"stats{depth}"])
+    assert conv == int(f[f"converged{depth}"])
+    assert 0 < conv < w * h and passes == spp  # some pixels converge early, some run to spp
+
+
+def test_oracle_adaptive_limits():
+    """Closed cases of the convergence test: with adaptive sampling off the
+    loop is the plain pass loop; a tolerance no pixel can miss stops every
+    pixel at max(2, min_spp) samples; a zero tolerance stops only pixels
+    whose luminance never varied."""
+    s = G.cornell_scene(G.load("cornell_film"))
+    plain, pst = O.render(s, 24, 24, 6, max_depth=4)
+    off, ost, passes, conv = O.render_adaptive(s, 24, 24, 6, None, max_depth=4)
+    assert off.tobytes() == plain.tobytes() and ost == pst and passes == 6 and conv == 0
+    loose, _, passes, conv = O.render_adaptive(s, 24, 24, 12, (3, 1e30, 1e30, 1.96), max_depth=4)
+    assert np.all(loose["samples"] == 3) and passes == 3 and conv == 24 * 24
+    tight, _, _, conv = O.render_adaptive(s, 24, 24, 8, (2, 0.0, 0.0, 1.96), max_depth=4)
+    still = tight["luminance_m2"] == 0.0
+    assert np.all(tight["samples"][still] == 2) and np.all(tight["samples"][~still] == 8)
+    assert conv == int(still.sum())

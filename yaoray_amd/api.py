@@ -262,13 +262,18 @@ class Scene:
 
     def backend_render(self, width, height, spp, max_depth=8, sampler=N.SAMPLER_OWEN_SOBOL, seed=0x59414F524159,
                        radiance_clamp=0.0, device=0, requested_backend=0, resume_film=None,
-                       resume_completed_spp=0, stop_before_start=False, cancel_after_passes=0, cancel_error=None):
-        """yr::HipBackend Prepare + Render (the reference's RenderBackend flow).
+                       resume_completed_spp=0, stop_before_start=False, cancel_after_passes=0, cancel_error=None,
+                       adaptive=None):
+        """yr::HipBackend Prepare + Render (the reference's RenderBackend flow); adaptive =
+        (min_spp, relative_error, absolute_error, confidence) turns on adaptive sampling.
 
         Returns (report dict, film or None); the film is a FILM32_DTYPE
         (yr::FilmPixel) array of shape (height, width)."""
         st = N.RenderSettings(width, height, sampler, spp, max_depth, radiance_clamp, seed, device,
                               requested_backend)
+        if adaptive is not None:
+            st.adaptive_enabled = 1
+            st.adaptive_min_spp, st.adaptive_relative_error, st.adaptive_absolute_error, st.adaptive_confidence = adaptive
         rq = N.RenderRequest()
         keep = None
         if resume_film is not None:
@@ -486,6 +491,35 @@ class Device:
             self.render_pass(d)
         self.synchronize()
         return self.film_read(width, height), self.render_stats()
+
+    def film_update_active(self, spp: int, adaptive=None):
+        """Writes the device list of active pixels (fewer than spp samples and, with
+        adaptive = (min_spp, relative_error, absolute_error, confidence), not converged)
+        for the next pixel-list pass; returns (active pixels, converged pixels)."""
+        a = N.AdaptiveDesc(0, 0, 0.0, 0.0, 0.0) if adaptive is None else N.AdaptiveDesc(1, *adaptive)
+        active, converged = C.c_int64(), C.c_int64()
+        self._check(self._lib.yrh_film_update_active(self._ctx, C.byref(a), spp, C.byref(active), C.byref(converged)),
+                    "film_update_active")
+        return active.value, converged.value
+
+    def render_adaptive(self, scene: Scene, width, height, spp, adaptive, max_depth=8, sampler=N.SAMPLER_OWEN_SOBOL,
+                        seed=0x59414F524159, radiance_clamp=0.0, upload=True):
+        """RenderCpuPathTrace's active-pixel loop on the device: each pass renders the next
+        sample of every active pixel, until none is active or spp passes ran. Returns
+        (FP64 film records, stats, passes, converged pixels)."""
+        if upload:
+            self.upload_scene(scene)
+            self.upload_shading(scene)
+        self.film_reset(width, height)
+        active, converged = self.film_update_active(spp, adaptive)
+        passes = 0
+        while passes < spp and active > 0:
+            d = scene.render_desc(width, height, spp, max_depth, sampler, seed, radiance_clamp, passes)
+            d.pixel_list = 1
+            self.render_pass(d)
+            active, converged = self.film_update_active(spp, adaptive)
+            passes += 1
+        return self.film_read(width, height), self.render_stats(), passes, converged
 
     def suggest_paths_per_pass(self, max_depth: int = 8) -> int:
         """Paths one render pass should hold for the free device memory (yrh_suggest_paths_per_pass)."""

@@ -29,9 +29,7 @@ int Upload(yrh_context* ctx, DevBuf& buf, const std::vector<T>& host) {
     if (rc != YRH_OK) return rc;
     buf.bytes = bytes;
     if (!host.empty()) {
-        rc = CheckHip(ctx, hipMemcpy(buf.ptr, host.data(), host.size() * sizeof(T),
-                                     hipMemcpyHostToDevice),
-                      "hipMemcpy H2D");
+        rc = yrh_copy_to_device(ctx, buf.ptr, host.data(), host.size() * sizeof(T));
     }
     return rc;
 }

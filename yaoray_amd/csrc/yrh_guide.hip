@@ -743,10 +743,10 @@ int yrh_guide_configure(yrh_context* ctx, const yrh_guide_desc* d) {
         (rc = Alloc(ctx, G->sample_hdr, 16)) != YRH_OK) {
         return rc;
     }
-    if ((rc = CheckHip(ctx, hipMemcpy(G->mlp.ptr, w.data(), mb, hipMemcpyHostToDevice), "hipMemcpy")) != YRH_OK) return rc;
-    if ((rc = CheckHip(ctx, hipMemcpy(G->table.ptr, t.data(), tb, hipMemcpyHostToDevice), "hipMemcpy")) != YRH_OK) return rc;
+    if ((rc = yrh_copy_to_device(ctx, G->mlp.ptr, w.data(), mb)) != YRH_OK) return rc;
+    if ((rc = yrh_copy_to_device(ctx, G->table.ptr, t.data(), tb)) != YRH_OK) return rc;
     for (DevBuf* b : {&G->mlp_m, &G->mlp_v, &G->table_m, &G->table_v, &G->table_grad, &G->sample_hdr}) {
-        if ((rc = CheckHip(ctx, hipMemset(b->ptr, 0, b->bytes), "hipMemset")) != YRH_OK) return rc;
+        if ((rc = CheckHip(ctx, hipMemsetAsync(b->ptr, 0, b->bytes, ctx->stream), "hipMemsetAsync")) != YRH_OK) return rc;
     }
     if ((rc = Pack(ctx, *G)) != YRH_OK) return rc;
     return CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");

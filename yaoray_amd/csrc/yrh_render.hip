@@ -120,6 +120,10 @@ struct RenderState {
     // ---- film ----
     DevBuf film;  // yrh_film_pixel
     int32_t film_w = 0, film_h = 0;
+    // Active pixels (yrh_film_update_active): the list and the words
+    // [0] list length, [1] converged pixels; act_ready once a list matches the film.
+    DevBuf act_list, act_count;
+    bool act_ready = false;
     // ---- pass buffers (capacity = paths per pass) ----
     int64_t capacity = 0;
     DevBuf p_pixel, p_tp, p_rad, p_prev, p_flags, p_med, p_alb, p_nrm, p_nee, p_slot;
@@ -145,7 +149,7 @@ int UploadVec(yrh_context* ctx, DevBuf& buf, const T* data, size_t count) {
     int rc = CheckHip(ctx, hipMalloc(&buf.ptr, bytes), "hipMalloc");
     if (rc != YRH_OK) return rc;
     buf.bytes = bytes;
-    if (count > 0) rc = CheckHip(ctx, hipMemcpy(buf.ptr, data, count * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+    if (count > 0) rc = yrh_copy_to_device(ctx, buf.ptr, data, count * sizeof(T));
     return rc;
 }
 
@@ -261,6 +265,10 @@ struct PassView {
     const int32_t* pass_items;  // items of pass k: [pass_items[k], pass_items[k + 1])
     int32_t* p_job;
     int32_t* p_sample;
+    // pixel-list passes (adaptive sampling): one path per listed pixel, at the
+    // pixel's next sample index; nullptr otherwise
+    const int32_t* active_list;
+    const uint32_t* active_count;
 };
 
 enum PathFlag : int { kPrevValid = 1, kPrevDelta = 2, kMediumActive = 4, kPrimaryHit = 8 };
@@ -621,6 +629,20 @@ __global__ void RaygenKernel(PassView P) {
         pixel = J.film_offset + local;
         P.p_job[p] = it.job | (it.occurrence << 16);
         P.p_sample[p] = it.sample;
+    } else if (P.active_list != nullptr) {
+        // RenderCpuPathTrace's active-pixel pass: each listed pixel renders
+        // its next sample, index = its film sample count.
+        const int64_t total = *P.active_count;
+        if (p == 0) P.counters[kCountQueue0] = static_cast<uint32_t>(total);
+        if (p >= total) return;
+        pixel = P.active_list[p];
+        x = static_cast<int>(pixel % P.width);
+        y = static_cast<int>(pixel / P.width);
+        sample = static_cast<int>(P.film[pixel].samples);
+        spp = P.spp;
+        width = P.width;
+        height = P.height;
+        camera = P.camera;
     } else {
         const int64_t total = P.npix * P.sample_count;
         if (p == 0) P.counters[kCountQueue0] = static_cast<uint32_t>(total);
@@ -1379,10 +1401,57 @@ __device__ __forceinline__ unsigned AddSample(const PassView& P, int64_t p, yrh_
 __global__ __launch_bounds__(kBlock) void AccumulateKernel(PassView P) {
     const int64_t pixel = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     unsigned bad = 0;
-    for (int k = 0; pixel < P.npix && k < P.sample_count; ++k) {
-        bad += AddSample(P, static_cast<int64_t>(k) * P.npix + pixel, P.film[pixel]);
+    if (P.active_list != nullptr) {
+        // Pixel-list pass: path p belongs to pixel active_list[p] (each listed once).
+        if (pixel < static_cast<int64_t>(*P.active_count)) bad = AddSample(P, pixel, P.film[P.active_list[pixel]]);
+    } else {
+        for (int k = 0; pixel < P.npix && k < P.sample_count; ++k) {
+            bad += AddSample(P, static_cast<int64_t>(k) * P.npix + pixel, P.film[pixel]);
+        }
     }
     WaveStat(P.stats, kStatBad, bad);
+}
+
+// Film::IsConverged (src/film/film.cpp:113-121) on the pixel's statistics
+// rounded to FP32 (the reference keeps its film in FP32): the standard error
+// of the luminance mean, scaled by the confidence, within the absolute plus
+// relative tolerance, after at least max(2, min_spp) samples.
+__device__ __forceinline__ bool PixelConverged(const yrh_film_pixel& px, int32_t min_spp, float rel, float abs_err,
+                                               float confidence) {
+    const uint32_t n = px.samples;
+    if (n < static_cast<uint32_t>(min_spp > 2 ? min_spp : 2)) return false;
+    const float mean = static_cast<float>(px.luminance_mean);
+    const float m2 = static_cast<float>(px.luminance_m2);
+    const float variance = m2 / static_cast<float>(n - 1);  // LuminanceVariance, n > 1
+    const float stderr_mean = sqrtf(fmaxf(0.0f, variance) / static_cast<float>(n));
+    const float bound = fmaxf(0.0f, confidence) * stderr_mean;
+    const float tolerance = fmaxf(0.0f, abs_err) + fmaxf(0.0f, rel) * fabsf(mean);
+    return isfinite(bound) && bound <= tolerance;
+}
+
+__global__ void ZeroWordsKernel(uint32_t* w, int n) {
+    if (static_cast<int>(threadIdx.x) < n) w[threadIdx.x] = 0u;
+}
+
+// The active-pixel list of RenderCpuPathTrace (src/backends/cpu/cpu_path_tracer.cpp:95-106,
+// 183-199): pixels that are not converged (adaptive sampling on) and have
+// fewer than spp samples; counts[1] = converged pixels. One atomic per
+// workgroup for each count; the list holds each workgroup's pixels in order.
+__global__ __launch_bounds__(kBlock) void UpdateActiveKernel(const yrh_film_pixel* film, int64_t npix, int32_t adaptive,
+                                                             int32_t min_spp, float rel, float abs_err,
+                                                             float confidence, uint32_t spp, int32_t* list,
+                                                             uint32_t* counts) {
+    __shared__ uint32_t append_lds[kBlock / 64 + 1];
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool converged = false, active = false;
+    if (k < npix) {
+        const yrh_film_pixel& px = film[k];
+        converged = adaptive != 0 && PixelConverged(px, min_spp, rel, abs_err, confidence);
+        active = !converged && px.samples < spp;
+    }
+    const int32_t slot = BlockAppend(counts, active ? 1u : 0u, append_lds);
+    if (active) list[slot] = static_cast<int32_t>(k);
+    (void)BlockAppend(counts + 1, converged ? 1u : 0u, append_lds);
 }
 
 // Batched pass: one thread per path of the given occurrence. The items of a
@@ -1467,6 +1536,11 @@ PassView MakePassView(RenderState& R, const yrh_render_desc& d) {
     P.film = R.film.as<yrh_film_pixel>();
     P.p_job = R.p_job.as<int32_t>();
     P.p_sample = R.p_sample.as<int32_t>();
+    if (d.pixel_list != 0) {
+        P.active_list = R.act_list.as<int32_t>();
+        P.active_count = R.act_count.as<uint32_t>();
+        P.sample_count = 1;
+    }
     return P;
 }
 
@@ -1546,9 +1620,11 @@ int EnsureCapacity(yrh_context* ctx, RenderState& R, int64_t paths) {
     }
     if (R.stats.ptr == nullptr) {
         if ((rc = Alloc(ctx, R.stats, kStatBytes)) != YRH_OK) return rc;
-        if ((rc = CheckHip(ctx, hipMemset(R.stats.ptr, 0, kStatBytes), "hipMemset")) != YRH_OK) return rc;
+        if ((rc = CheckHip(ctx, hipMemsetAsync(R.stats.ptr, 0, kStatBytes, ctx->stream), "hipMemsetAsync")) != YRH_OK) {
+            return rc;
+        }
     }
-    rc = CheckHip(ctx, hipMemset(R.counters.ptr, 0, 4 * kCounterWords), "hipMemset");
+    rc = CheckHip(ctx, hipMemsetAsync(R.counters.ptr, 0, 4 * kCounterWords, ctx->stream), "hipMemsetAsync");
     R.capacity = paths;
     return rc;
 }
@@ -1901,13 +1977,16 @@ int yrh_film_reset(yrh_context* ctx, int32_t width, int32_t height) {
     if (R.film.bytes < bytes) {
         if ((rc = Alloc(ctx, R.film, bytes)) != YRH_OK) return rc;
     }
-    if ((rc = CheckHip(ctx, hipMemset(R.film.ptr, 0, bytes), "hipMemset")) != YRH_OK) return rc;
+    // The context stream is non-blocking: clears go on it, so they are ordered
+    // before a following film load or pass (a null-stream memset is not).
+    if ((rc = CheckHip(ctx, hipMemsetAsync(R.film.ptr, 0, bytes, ctx->stream), "hipMemsetAsync")) != YRH_OK) return rc;
     R.film_w = width;
     R.film_h = height;
+    R.act_ready = false;
     if (R.stats.ptr == nullptr) {
         if ((rc = Alloc(ctx, R.stats, kStatBytes)) != YRH_OK) return rc;
     }
-    return CheckHip(ctx, hipMemset(R.stats.ptr, 0, kStatBytes), "hipMemset");
+    return CheckHip(ctx, hipMemsetAsync(R.stats.ptr, 0, kStatBytes, ctx->stream), "hipMemsetAsync");
 }
 
 int yrh_film_load(yrh_context* ctx, const yrh_film_pixel* pixels, int64_t count) {
@@ -1922,6 +2001,37 @@ int yrh_film_read(yrh_context* ctx, yrh_film_pixel* pixels, int64_t count) {
     RenderState& R = *ctx->render;
     if (count != static_cast<int64_t>(R.film_w) * R.film_h) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "film size mismatch");
     return yrh_copy_to_host(ctx, pixels, R.film.ptr, static_cast<size_t>(count) * sizeof(yrh_film_pixel));
+}
+
+int yrh_film_update_active(yrh_context* ctx, const yrh_adaptive_desc* adaptive, int32_t spp, int64_t* active,
+                           int64_t* converged) {
+    if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
+    if (ctx->render == nullptr || ctx->render->film.ptr == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "no film");
+    if (spp < 1) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "spp must be positive");
+    (void)hipSetDevice(ctx->device);
+    RenderState& R = *ctx->render;
+    const int64_t npix = static_cast<int64_t>(R.film_w) * R.film_h;
+    int rc;
+    if (R.act_list.bytes < static_cast<size_t>(npix) * 4) {
+        if ((rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize")) != YRH_OK) return rc;
+        if ((rc = Alloc(ctx, R.act_list, static_cast<size_t>(npix) * 4)) != YRH_OK) return rc;
+    }
+    if (R.act_count.ptr == nullptr && (rc = Alloc(ctx, R.act_count, 16)) != YRH_OK) return rc;
+    const bool on = adaptive != nullptr && adaptive->enabled != 0;
+    hipLaunchKernelGGL(ZeroWordsKernel, dim3(1), dim3(64), 0, ctx->stream, R.act_count.as<uint32_t>(), 2);
+    hipLaunchKernelGGL(UpdateActiveKernel, dim3(Grid(npix)), dim3(kBlock), 0, ctx->stream, R.film.as<yrh_film_pixel>(),
+                       npix, on ? 1 : 0, on ? adaptive->min_spp : 0, on ? adaptive->relative_error : 0.0f,
+                       on ? adaptive->absolute_error : 0.0f, on ? adaptive->confidence : 0.0f,
+                       static_cast<uint32_t>(spp), R.act_list.as<int32_t>(), R.act_count.as<uint32_t>());
+    if ((rc = CheckHip(ctx, hipGetLastError(), "update active launch")) != YRH_OK) return rc;
+    R.act_ready = true;
+    if (active != nullptr || converged != nullptr) {
+        uint32_t c[2] = {0, 0};
+        if ((rc = yrh_copy_to_host(ctx, c, R.act_count.ptr, 8)) != YRH_OK) return rc;
+        if (active != nullptr) *active = c[0];
+        if (converged != nullptr) *converged = c[1];
+    }
+    return YRH_OK;
 }
 
 int yrh_render_stats_read(yrh_context* ctx, yrh_render_stats* stats) {
@@ -1972,7 +2082,10 @@ int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* d) {
     if (d->width != R.film_w || d->height != R.film_h) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "film size does not match the pass");
     if (d->sampler < YRH_SAMPLER_INDEPENDENT || d->sampler > YRH_SAMPLER_ZSOBOL) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "unknown sampler");
     if (d->sample_index < 0 || d->sample_count < 0) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "negative sample index or count");
-    const int64_t count = d->sample_count < 1 ? 1 : d->sample_count;
+    if (d->pixel_list != 0 && !R.act_ready) {
+        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "pixel-list pass without an active list (yrh_film_update_active)");
+    }
+    const int64_t count = d->pixel_list != 0 ? 1 : (d->sample_count < 1 ? 1 : d->sample_count);
     if (static_cast<int64_t>(d->sample_index) + count > (int64_t{1} << 31) - 1) {
         return Fail(ctx, YRH_E_INVALID_ARGUMENT, "sample index overflow");
     }
@@ -2080,9 +2193,15 @@ int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* d, float* job_ms, i
     if ((rc = UploadVec(ctx, R.b_items, items.data(), items.size())) != YRH_OK) return rc;
     if ((rc = UploadVec(ctx, R.b_pass_items, pass_items.data(), pass_items.size())) != YRH_OK) return rc;
     if ((rc = Alloc(ctx, R.b_film, static_cast<size_t>(film_pixels) * sizeof(yrh_film_pixel))) != YRH_OK) return rc;
-    if ((rc = CheckHip(ctx, hipMemset(R.b_film.ptr, 0, R.b_film.bytes), "hipMemset")) != YRH_OK) return rc;
-    if ((rc = CheckHip(ctx, hipMemset(R.stats.ptr, 0, kStatBytes), "hipMemset")) != YRH_OK) return rc;
-    if ((rc = CheckHip(ctx, hipMemset(R.counters.ptr, 0, 4 * kCounterWords), "hipMemset")) != YRH_OK) return rc;
+    // Stream-ordered clears (the context stream does not wait for the null stream).
+    if ((rc = CheckHip(ctx, hipMemsetAsync(R.b_film.ptr, 0, R.b_film.bytes, ctx->stream), "hipMemsetAsync")) != YRH_OK) {
+        return rc;
+    }
+    if ((rc = CheckHip(ctx, hipMemsetAsync(R.stats.ptr, 0, kStatBytes, ctx->stream), "hipMemsetAsync")) != YRH_OK) return rc;
+    if ((rc = CheckHip(ctx, hipMemsetAsync(R.counters.ptr, 0, 4 * kCounterWords, ctx->stream), "hipMemsetAsync")) !=
+        YRH_OK) {
+        return rc;
+    }
     yrh_render_desc rd{};
     rd.sampler = d->sampler;
     rd.max_depth = d->max_depth;

@@ -211,8 +211,11 @@ RenderResult HipBackend::Render(const PreparedScene& scene, const RenderRequest&
         if (request.resume_film->Width() != settings.width || request.resume_film->Height() != settings.height) {
             return fail("resume film dimensions do not match render scene");
         }
+        // With adaptive sampling converged pixels may hold fewer samples.
+        const bool adaptive_resume = settings.adaptive_sampling.enabled;
         for (const FilmPixel& p : request.resume_film->Pixels()) {
-            if (p.samples != static_cast<std::uint32_t>(request.resume_completed_spp)) {
+            const auto done = static_cast<std::uint32_t>(request.resume_completed_spp);
+            if (adaptive_resume ? p.samples > done : p.samples != done) {
                 return fail("resume film sample counts do not match resume completed spp");
             }
         }
@@ -263,17 +266,31 @@ RenderResult HipBackend::Render(const PreparedScene& scene, const RenderRequest&
         result.stats.samples_rendered = film.TotalSampleCount();
         result.stats.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     };
-    // Without a progress callback nothing observes the film between passes, so
-    // consecutive passes are batched into one wavefront as large as the device
-    // memory suggests (same per-pixel sample order, so the same film).
+    // Adaptive sampling (RenderCpuPathTrace's active-pixel loop,
+    // cpu_path_tracer.cpp:95-106, 183-199): the device keeps the list of
+    // pixels below spp samples that Film::IsConverged has not stopped, and
+    // each pass renders the next sample of every listed pixel.
+    const AdaptiveSamplingSettings& as = settings.adaptive_sampling;
+    const bool adaptive = as.enabled;
+    const yrh_adaptive_desc ad{1, as.min_spp, as.relative_error, as.absolute_error, as.confidence};
+    int64_t active = npix, converged = 0;
+    if (adaptive) {
+        if (yrh_film_update_active(ctx, &ad, spp, &active, &converged) != YRH_OK) return fail(LastError(ctx, "active pixels"));
+        result.stats.converged_pixels = static_cast<std::uint64_t>(converged);
+        d.pixel_list = 1;
+    }
+    // Otherwise, without a progress callback nothing observes the film between
+    // passes, so consecutive passes are batched into one wavefront as large as
+    // the device memory suggests (same per-pixel sample order, so the same film).
     int64_t batch_paths = int64_t{1} << 23;
-    if (!request.progress_callback && yrh_suggest_paths_per_pass(ctx, settings.max_depth, &batch_paths) != YRH_OK) {
+    if (!adaptive && !request.progress_callback &&
+        yrh_suggest_paths_per_pass(ctx, settings.max_depth, &batch_paths) != YRH_OK) {
         return fail(LastError(ctx, "suggest paths per pass"));
     }
-    const int batch = request.progress_callback
+    const int batch = adaptive || request.progress_callback
         ? 1
         : static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(spp, batch_paths / std::max<int64_t>(1, npix))));
-    for (int pass = request.resume_completed_spp; pass < spp; pass += d.sample_count) {
+    for (int pass = request.resume_completed_spp; pass < spp && active > 0; pass += d.sample_count) {
         if (request.stop_token.stop_requested()) {
             if (!sync_film()) return fail(LastError(ctx, "film read"));
             fill_stats();
@@ -282,6 +299,14 @@ RenderResult HipBackend::Render(const PreparedScene& scene, const RenderRequest&
         d.sample_index = pass;
         d.sample_count = std::min(batch, spp - pass);
         if (yrh_render_pass(ctx, &d) != YRH_OK) return fail(LastError(ctx, "render pass"));
+        if (adaptive) {
+            if (yrh_film_update_active(ctx, &ad, spp, &active, &converged) != YRH_OK) {
+                return fail(LastError(ctx, "active pixels"));
+            }
+            result.stats.converged_pixels = static_cast<std::uint64_t>(converged);
+        } else {
+            active = pass + d.sample_count < spp ? npix : 0;
+        }
         if (request.progress_callback) {
             if (!sync_film()) return fail(LastError(ctx, "film read"));
             fill_stats();
@@ -289,8 +314,8 @@ RenderResult HipBackend::Render(const PreparedScene& scene, const RenderRequest&
                                           spp,
                                           film.TotalSampleCount(),
                                           static_cast<std::uint64_t>(spp) * static_cast<std::uint64_t>(npix),
-                                          pass + 1 < spp ? static_cast<std::uint64_t>(npix) : 0,
-                                          0,
+                                          static_cast<std::uint64_t>(active),
+                                          static_cast<std::uint64_t>(converged),
                                           st.rays_traced,
                                           result.stats.elapsed_seconds};
             const RenderProgressDecision decision = request.progress_callback(progress, film);

@@ -342,6 +342,11 @@ int yrhost_backend_render(const yrhost_scene* s, const yrhost_render_settings* s
     job.settings.max_depth = settings->max_depth;
     job.settings.seed = settings->seed;
     job.settings.radiance_clamp = settings->radiance_clamp;
+    job.settings.adaptive_sampling.enabled = settings->adaptive_enabled != 0;
+    job.settings.adaptive_sampling.min_spp = settings->adaptive_min_spp;
+    job.settings.adaptive_sampling.relative_error = settings->adaptive_relative_error;
+    job.settings.adaptive_sampling.absolute_error = settings->adaptive_absolute_error;
+    job.settings.adaptive_sampling.confidence = settings->adaptive_confidence;
 
     yr::HipBackend backend(settings->device);
     yr::BackendPrepareResult prepared = backend.Prepare(std::move(job));
@@ -384,6 +389,7 @@ int yrhost_backend_render(const yrhost_scene* s, const yrhost_render_settings* s
         ++report->progress_calls;
         report->last_progress_spp = p.completed_spp;
         report->last_progress_samples = p.completed_samples;
+        report->last_progress_active = p.active_pixels;
         yr::RenderProgressDecision d;
         if (cancel_after > 0 && report->progress_calls >= cancel_after) {
             d.cancel = true;
@@ -407,6 +413,7 @@ int yrhost_backend_render(const yrhost_scene* s, const yrhost_render_settings* s
     report->bvh_node_tests = st.bvh_node_tests;
     report->hits = st.hits;
     report->misses = st.misses;
+    report->converged_pixels = st.converged_pixels;
     report->elapsed_seconds = st.elapsed_seconds;
     if (r.film.has_value()) {
         report->film_valid = 1;

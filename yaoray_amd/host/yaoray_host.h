@@ -110,6 +110,12 @@ typedef struct yrhost_render_settings {
     uint64_t seed;
     int32_t device;
     int32_t requested_backend; /* 0 = HIP; anything else is refused by Prepare */
+    /* yr::AdaptiveSamplingSettings (include/yaoray/scene/render_options.hpp:30-36) */
+    int32_t adaptive_enabled;
+    int32_t adaptive_min_spp;
+    float adaptive_relative_error;
+    float adaptive_absolute_error;
+    float adaptive_confidence;
 } yrhost_render_settings;
 
 /* yr::RenderRequest. The stop token and the progress callback are driven
@@ -149,6 +155,8 @@ typedef struct yrhost_render_report {
     uint64_t misses;
     uint64_t bad_samples;
     uint64_t last_progress_samples;
+    uint64_t converged_pixels;
+    uint64_t last_progress_active;
     double elapsed_seconds;
     double prepare_seconds;
     double acceleration_build_seconds;

@@ -97,6 +97,9 @@ struct RenderPrimitive {
     bool has_normals = false;
     bool has_uvs = false;
     bool has_tangents = false;
+    // The reference's padding byte, named so that copies of the array (and
+    // digests of its bytes) are deterministic.
+    std::uint8_t reserved = 0;
 };
 static_assert(sizeof(RenderPrimitive) == 16);
 
@@ -189,6 +192,14 @@ struct RenderCamera {
 enum class RenderBackendKind { Cpu, Cuda, Hip };
 enum class RenderSamplerKind { Independent, Stratified, OwenSobol, ZSobol };
 
+struct AdaptiveSamplingSettings {
+    bool enabled = false;
+    int min_spp = 16;
+    float relative_error = 0.05f;
+    float absolute_error = 0.001f;
+    float confidence = 1.96f;
+};
+
 struct RenderSettings {
     RenderBackendKind requested_backend = RenderBackendKind::Hip;
     RenderSamplerKind sampler = RenderSamplerKind::Independent;
@@ -199,6 +210,7 @@ struct RenderSettings {
     std::uint64_t seed = 0;
     int threads = 0;
     float radiance_clamp = 0.0f;
+    AdaptiveSamplingSettings adaptive_sampling;
 };
 
 struct RenderSceneIR {

@@ -111,7 +111,12 @@ class RenderDesc(C.Structure):
                 ("camera_right", C.c_float * 3), ("camera_up", C.c_float * 3), ("tan_half_fov", C.c_float),
                 ("width", C.c_int32), ("height", C.c_int32), ("sampler", C.c_int32), ("spp", C.c_int32),
                 ("max_depth", C.c_int32), ("seed", C.c_uint64), ("radiance_clamp", C.c_float),
-                ("sample_index", C.c_int32), ("sample_count", C.c_int32)]
+                ("sample_index", C.c_int32), ("sample_count", C.c_int32), ("pixel_list", C.c_int32)]
+
+
+class AdaptiveDesc(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("min_spp", C.c_int32), ("relative_error", C.c_float),
+                ("absolute_error", C.c_float), ("confidence", C.c_float)]
 
 
 class RenderStats(C.Structure):
@@ -162,7 +167,9 @@ GUIDE_MLP_PARAMS = 13312
 class RenderSettings(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("sampler", C.c_int32), ("spp", C.c_int32),
                 ("max_depth", C.c_int32), ("radiance_clamp", C.c_float), ("seed", C.c_uint64),
-                ("device", C.c_int32), ("requested_backend", C.c_int32)]
+                ("device", C.c_int32), ("requested_backend", C.c_int32), ("adaptive_enabled", C.c_int32),
+                ("adaptive_min_spp", C.c_int32), ("adaptive_relative_error", C.c_float),
+                ("adaptive_absolute_error", C.c_float), ("adaptive_confidence", C.c_float)]
 
 
 class RenderRequest(C.Structure):
@@ -178,7 +185,7 @@ class RenderReport(C.Structure):
                 + [(k, C.c_uint64) for k in ("samples_rendered", "rays_traced", "shadow_rays",
                                              "occluded_shadow_rays", "triangle_tests", "sphere_tests",
                                              "bvh_node_tests", "hits", "misses", "bad_samples",
-                                             "last_progress_samples")]
+                                             "last_progress_samples", "converged_pixels", "last_progress_active")]
                 + [(k, C.c_double) for k in ("elapsed_seconds", "prepare_seconds", "acceleration_build_seconds")]
                 + [("error", C.c_char * 256)])
 
@@ -227,6 +234,7 @@ YRH_SIGNATURES = {
     "yrh_film_load": (_I, [_VP, _VP, _I64]),
     "yrh_film_read": (_I, [_VP, _VP, _I64]),
     "yrh_render_pass": (_I, [_VP, C.POINTER(RenderDesc)]),
+    "yrh_film_update_active": (_I, [_VP, C.POINTER(AdaptiveDesc), _I32, C.POINTER(_I64), C.POINTER(_I64)]),
     "yrh_render_stats_read": (_I, [_VP, C.POINTER(RenderStats)]),
     "yrh_sample_values": (_I, [_VP, _I32, _U64, _I32, _I64, _VP, _VP, _VP, _VP, _I32, _VP]),
     "yrh_generate_camera_rays": (_I, [_VP, C.POINTER(RenderDesc), _VP, _VP]),
