@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--incoherent", type=int, default=4_194_304)
     ap.add_argument("--cpu-stride", type=int, default=4, help="CPU baseline traces every k-th ray")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--renders", default="cornell,two_level",
+    ap.add_argument("--renders", default="cornell,two_level,cornell_adaptive",
                     help="render workloads timed after the ray-cast microbench (comma list, empty for none)")
     ap.add_argument("--render-frames", type=int, default=3)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-camera workload")
@@ -324,7 +324,18 @@ RENDER_WORKLOADS = {
                            "conductor / dielectric), 2048x1024 sky+sun env, 1920x1080, 64 spp OwenSobol, depth 16",
                       width=1920, height=1080, spp=64, max_depth=16,
                       cpu=dict(width=480, height=270, spp=8)),
+    # Adaptive sampling with the reference's default settings
+    # (AdaptiveSamplingSettings: min_spp 16, relative 0.05, absolute 0.001, confidence 1.96).
+    "cornell_adaptive": dict(desc="seeded Cornell box, 256x256, up to 256 spp OwenSobol, depth 8, adaptive sampling "
+                                  "(min 16 spp, relative error 0.05, absolute 0.001, confidence 1.96)",
+                             width=256, height=256, spp=256, max_depth=8, adaptive=(16, 0.05, 0.001, 1.96),
+                             cpu=dict(width=256, height=256, spp=256)),
 }
+
+
+# Samples per active pixel in one adaptive pass (ray budget vs pass count: a
+# one-sample pass over the few live pixels of a late pass is latency bound).
+ADAPTIVE_BATCH = 16
 
 
 def batch_jobs(scene, count=32, seed=5):
@@ -529,7 +540,7 @@ def bench_render(dev, key, frames, cpu_threads, with_cpu):
 
     w = RENDER_WORKLOADS[key]
     t = time.perf_counter()
-    scene = Scene.cornell(1) if key == "cornell" else Scene.instanced()
+    scene = Scene.cornell(1) if key.startswith("cornell") else Scene.instanced()
     scene.build_accel()
     log(f"render {key}: scene ready in {time.perf_counter() - t:.2f}s")
     dev.upload_scene(scene)
@@ -538,15 +549,24 @@ def bench_render(dev, key, frames, cpu_threads, with_cpu):
     # (the whole frame fits on one MI355X for both workloads).
     batch = int(max(1, min(w["spp"], dev.suggest_paths_per_pass(w["max_depth"]) // (w["width"] * w["height"]))))
     kw = dict(max_depth=w["max_depth"], upload=False, batch=batch)
+    adaptive = w.get("adaptive")
     dev.render(scene, w["width"], w["height"], batch, **kw)  # warm-up, allocates the pass buffers
     dev.synchronize()
     secs = []
     st = None
     film = None
+    passes = converged = None
     for _ in range(frames):
         e0, e1 = dev.event(), dev.event()
         dev.record(e0)
-        film, st = dev.render(scene, w["width"], w["height"], w["spp"], **kw)
+        if adaptive:
+            # One pass per sample index over the active pixels (the host reads the
+            # active count after each pass, as the reference's loop does).
+            film, st, passes, converged = dev.render_adaptive(scene, w["width"], w["height"], w["spp"], adaptive,
+                                                              max_depth=w["max_depth"], upload=False,
+                                                              batch=ADAPTIVE_BATCH)
+        else:
+            film, st = dev.render(scene, w["width"], w["height"], w["spp"], **kw)
         dev.record(e1)
         dev.synchronize()
         secs.append(dev.elapsed_ms(e0, e1) / 1e3)
@@ -560,13 +580,19 @@ def bench_render(dev, key, frames, cpu_threads, with_cpu):
         "value": round(st["rays_traced"] / sec / 1e6, 2),
         "unit": "Mrays/s (path segments, rays_traced)",
         "shadow_Mrays_s": round(st["shadow_rays"] / sec / 1e6, 2),
-        "samples_per_s": round(w["width"] * w["height"] * w["spp"] / sec, 1),
+        "samples_per_s": round(float(film["samples"].sum()) / sec, 1),
         "rays_traced": st["rays_traced"],
         "shadow_rays": st["shadow_rays"],
         "mean_node_tests_per_ray": round(st["bvh_node_tests"] / max(1, st["rays_traced"] + st["shadow_rays"]), 2),
         "samples_per_pass": batch,
         "cpu_baseline": None,
     }
+    if adaptive:
+        out["samples_per_pass"] = (f"{ADAPTIVE_BATCH} per active pixel (first pass min_spp), added in order until "
+                                   f"the pixel converges; rays_traced includes the samples not added")
+        out["passes"] = passes
+        out["converged_pixels"] = converged
+        out["mean_spp"] = round(float(film["samples"].mean()), 2)
     log(f"render {key}: {out['ms_per_frame']} ms/frame, {out['value']} Mrays/s (+{out['shadow_Mrays_s']} shadow)")
     if with_cpu:
         from oracle import oracle_py as O
@@ -575,12 +601,16 @@ def bench_render(dev, key, frames, cpu_threads, with_cpu):
         if O.ref_available():
             kind = "reference"
             rfilm, rst, rsec = O.ref_render(scene, c["width"], c["height"], c["spp"], max_depth=w["max_depth"],
-                                            threads=cpu_threads)
+                                            threads=cpu_threads, adaptive=adaptive)
         else:
             kind = "port"
             t0 = time.perf_counter()
-            rfilm, rst = O.render(scene, c["width"], c["height"], c["spp"], max_depth=w["max_depth"],
-                                  threads=cpu_threads)
+            if adaptive:
+                rfilm, rst, _, _ = O.render_adaptive(scene, c["width"], c["height"], c["spp"], adaptive,
+                                                     max_depth=w["max_depth"], threads=cpu_threads)
+            else:
+                rfilm, rst = O.render(scene, c["width"], c["height"], c["spp"], max_depth=w["max_depth"],
+                                      threads=cpu_threads)
             rsec = time.perf_counter() - t0
         out["cpu_baseline"] = {
             "value": round(rst["rays_traced"] / rsec / 1e6, 4),
@@ -599,6 +629,9 @@ def bench_render(dev, key, frames, cpu_threads, with_cpu):
                 "pixels_bit_equal_fp32": float(np.mean(np.all(g.astype(np.float32) == r.astype(np.float32), axis=1))),
                 "rays_traced_equal": bool(rst["rays_traced"] == st["rays_traced"]),
             }
+            if adaptive:
+                out["parity"]["pixels_with_equal_sample_counts"] = float(np.mean(film["samples"] == rfilm["samples"]))
+                out["parity"]["rays_traced_vs_reference"] = round(st["rays_traced"] / max(1, rst["rays_traced"]), 4)
         log(f"render {key}: cpu {kind} {out['cpu_baseline']['value']} Mrays/s on {cpu_threads} threads")
     return out
 

@@ -319,9 +319,13 @@ typedef struct yrh_render_desc {
     int32_t sample_index; /* this pass's first sample index (= completed spp) */
     int32_t sample_count; /* consecutive sample indices rendered by this pass (0 means 1); the
                              film receives them in index order, as that many passes would add them */
-    int32_t pixel_list;   /* 1: render one sample for each pixel of the list the last
-                             yrh_film_update_active wrote, each at its own next sample index
-                             (its film sample count); sample_index and sample_count are ignored */
+    int32_t pixel_list;   /* 1: render the pixels of the list the last yrh_film_update_active
+                             wrote, sample_count samples each from the pixel's own next sample
+                             index (its film sample count), added in index order until the pixel
+                             converges under that call's adaptive settings or reaches spp: the film
+                             equals sample_count one-sample passes with the list updated between
+                             them (the render counters include the samples not added);
+                             sample_index is ignored */
 } yrh_render_desc;
 
 /* yr::AdaptiveSamplingSettings (include/yaoray/scene/render_options.hpp:30-36). */

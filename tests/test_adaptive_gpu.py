@@ -114,3 +114,40 @@ def test_backend_adaptive_render_and_resume(device):
     same = fdone["samples"].reshape(-1) == film["samples"]
     print(f"resumed adaptive: equal counts {same.mean():.4f}, last active {done['last_progress_active']}")
     assert same.mean() >= 0.99
+
+
+@pytest.mark.parametrize("batch", [4, 16])
+def test_batched_adaptive_passes_equal_one_sample_passes(device, batch):
+    """A pixel-list pass of several samples per pixel, accepted in order until
+    the pixel converges, leaves the same film (bytes) and sample counts as the
+    one-sample passes; its counters include the samples it did not add."""
+    f = G.load("cornell_adaptive")
+    s = G.cornell_scene(f)
+    a = adaptive_args(f, 8)
+    one, st1, passes1, conv1 = device.render_adaptive(s, 64, 64, 32, a, max_depth=8, sampler=2, seed=int(f["seed"]))
+    many, stk, passesk, convk = device.render_adaptive(s, 64, 64, 32, a, max_depth=8, sampler=2, seed=int(f["seed"]),
+                                                       upload=False, batch=batch)
+    assert many.tobytes() == one.tobytes()
+    assert convk == conv1 == int(f["converged8"]) and passesk == passes1
+    assert np.array_equal(many["samples"], G.film(f["film8"])["samples"])
+    assert stk["rays_traced"] >= st1["rays_traced"] and stk["bad_samples"] == st1["bad_samples"]
+
+
+def test_backend_without_progress_callback(device):
+    """Without a progress callback HipBackend batches passes: plain renders in
+    one wavefront of many sample indices, adaptive ones in 16-sample pixel-list
+    passes. Both leave the films of the one-pass-per-sample renders."""
+    f = G.load("cornell_adaptive")
+    s = G.cornell_scene(f)
+    a = adaptive_args(f, 8)
+    seed = int(f["seed"])
+    rep1, film1 = s.backend_render(64, 64, 32, max_depth=8, sampler=2, seed=seed, adaptive=a)
+    repb, filmb = s.backend_render(64, 64, 32, max_depth=8, sampler=2, seed=seed, adaptive=a, cancel_after_passes=-1)
+    assert rep1["ok"] == 1 and repb["ok"] == 1 and repb["progress_calls"] == 0
+    assert filmb.tobytes() == film1.tobytes()
+    assert repb["converged_pixels"] == rep1["converged_pixels"] == int(f["converged8"])
+    assert repb["rays_traced"] >= rep1["rays_traced"]
+    p1, pf1 = s.backend_render(48, 32, 6, max_depth=6, seed=seed)
+    pb, pfb = s.backend_render(48, 32, 6, max_depth=6, seed=seed, cancel_after_passes=-1)
+    assert pb["ok"] == 1 and pb["progress_calls"] == 0
+    assert pfb.tobytes() == pf1.tobytes() and pb["rays_traced"] == p1["rays_traced"]

@@ -503,10 +503,13 @@ class Device:
         return active.value, converged.value
 
     def render_adaptive(self, scene: Scene, width, height, spp, adaptive, max_depth=8, sampler=N.SAMPLER_OWEN_SOBOL,
-                        seed=0x59414F524159, radiance_clamp=0.0, upload=True):
+                        seed=0x59414F524159, radiance_clamp=0.0, upload=True, batch=1):
         """RenderCpuPathTrace's active-pixel loop on the device: each pass renders the next
-        sample of every active pixel, until none is active or spp passes ran. Returns
-        (FP64 film records, stats, passes, converged pixels)."""
+        sample of every active pixel, until none is active or spp passes ran. With batch > 1
+        one device pass stands for `batch` of them (each pixel's samples are added in order
+        until it converges, so the film is the same; the counters include the samples not
+        added); the first covers max(batch, min_spp) passes, before which no pixel can
+        converge. Returns (FP64 film records, stats, passes, converged pixels)."""
         if upload:
             self.upload_scene(scene)
             self.upload_shading(scene)
@@ -514,11 +517,13 @@ class Device:
         active, converged = self.film_update_active(spp, adaptive)
         passes = 0
         while passes < spp and active > 0:
+            first = max(batch, adaptive[0]) if (passes == 0 and adaptive is not None and batch > 1) else batch
             d = scene.render_desc(width, height, spp, max_depth, sampler, seed, radiance_clamp, passes)
             d.pixel_list = 1
+            d.sample_count = min(first, spp - passes)
             self.render_pass(d)
             active, converged = self.film_update_active(spp, adaptive)
-            passes += 1
+            passes += d.sample_count
         return self.film_read(width, height), self.render_stats(), passes, converged
 
     def suggest_paths_per_pass(self, max_depth: int = 8) -> int:

@@ -121,9 +121,11 @@ struct RenderState {
     DevBuf film;  // yrh_film_pixel
     int32_t film_w = 0, film_h = 0;
     // Active pixels (yrh_film_update_active): the list and the words
-    // [0] list length, [1] converged pixels; act_ready once a list matches the film.
+    // [0] list length, [1] converged pixels; act_ready once a list matches the
+    // film; act_desc: the adaptive settings pixel-list passes accept samples by.
     DevBuf act_list, act_count;
     bool act_ready = false;
+    yrh_adaptive_desc act_desc{};
     // ---- pass buffers (capacity = paths per pass) ----
     int64_t capacity = 0;
     DevBuf p_pixel, p_tp, p_rad, p_prev, p_flags, p_med, p_alb, p_nrm, p_nee, p_slot;
@@ -265,10 +267,13 @@ struct PassView {
     const int32_t* pass_items;  // items of pass k: [pass_items[k], pass_items[k + 1])
     int32_t* p_job;
     int32_t* p_sample;
-    // pixel-list passes (adaptive sampling): one path per listed pixel, at the
-    // pixel's next sample index; nullptr otherwise
+    // pixel-list passes (adaptive sampling): sample_count paths per listed
+    // pixel at its next sample indices, accepted in order until the pixel
+    // converges (act_*: Film::IsConverged settings); nullptr otherwise
     const int32_t* active_list;
     const uint32_t* active_count;
+    int32_t act_on, act_min_spp;
+    float act_rel, act_abs, act_conf;
 };
 
 enum PathFlag : int { kPrevValid = 1, kPrevDelta = 2, kMediumActive = 4, kPrimaryHit = 8 };
@@ -630,15 +635,18 @@ __global__ void RaygenKernel(PassView P) {
         P.p_job[p] = it.job | (it.occurrence << 16);
         P.p_sample[p] = it.sample;
     } else if (P.active_list != nullptr) {
-        // RenderCpuPathTrace's active-pixel pass: each listed pixel renders
-        // its next sample, index = its film sample count.
-        const int64_t total = *P.active_count;
+        // RenderCpuPathTrace's active-pixel passes: listed pixel e renders its
+        // next sample_count samples (indices from its film sample count) as
+        // paths e, e + n, e + 2n, ... (n listed pixels).
+        const int64_t n = *P.active_count;
+        const int64_t total = n * P.sample_count;
         if (p == 0) P.counters[kCountQueue0] = static_cast<uint32_t>(total);
         if (p >= total) return;
-        pixel = P.active_list[p];
+        pixel = P.active_list[p % n];
         x = static_cast<int>(pixel % P.width);
         y = static_cast<int>(pixel / P.width);
-        sample = static_cast<int>(P.film[pixel].samples);
+        sample = static_cast<int>(P.film[pixel].samples) + static_cast<int>(p / n);
+        P.p_sample[p] = sample;  // the bounces re-create the path's sampler from it
         spp = P.spp;
         width = P.width;
         height = P.height;
@@ -821,6 +829,8 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
         const int64_t local = pix - J.film_offset;
         smp.Init(P.sampler, P.seed, static_cast<int>(local % J.width), static_cast<int>(local / J.width), P.p_sample[p],
                  J.spp);
+    } else if (P.active_list != nullptr) {
+        smp.Init(P.sampler, P.seed, pix % P.width, pix / P.width, P.p_sample[p], P.spp);
     } else {
         smp.Init(P.sampler, P.seed, pix % P.width, pix / P.width, P.sample_index + static_cast<int>(p / P.npix), P.spp);
     }
@@ -1398,20 +1408,6 @@ __device__ __forceinline__ unsigned AddSample(const PassView& P, int64_t p, yrh_
 }
 
 // One thread per pixel, the pass's samples in index order.
-__global__ __launch_bounds__(kBlock) void AccumulateKernel(PassView P) {
-    const int64_t pixel = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    unsigned bad = 0;
-    if (P.active_list != nullptr) {
-        // Pixel-list pass: path p belongs to pixel active_list[p] (each listed once).
-        if (pixel < static_cast<int64_t>(*P.active_count)) bad = AddSample(P, pixel, P.film[P.active_list[pixel]]);
-    } else {
-        for (int k = 0; pixel < P.npix && k < P.sample_count; ++k) {
-            bad += AddSample(P, static_cast<int64_t>(k) * P.npix + pixel, P.film[pixel]);
-        }
-    }
-    WaveStat(P.stats, kStatBad, bad);
-}
-
 // Film::IsConverged (src/film/film.cpp:113-121) on the pixel's statistics
 // rounded to FP32 (the reference keeps its film in FP32): the standard error
 // of the luminance mean, scaled by the confidence, within the absolute plus
@@ -1427,6 +1423,36 @@ __device__ __forceinline__ bool PixelConverged(const yrh_film_pixel& px, int32_t
     const float bound = fmaxf(0.0f, confidence) * stderr_mean;
     const float tolerance = fmaxf(0.0f, abs_err) + fmaxf(0.0f, rel) * fabsf(mean);
     return isfinite(bound) && bound <= tolerance;
+}
+
+__global__ __launch_bounds__(kBlock) void AccumulateKernel(PassView P) {
+    const int64_t pixel = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned bad = 0;
+    if (P.active_list != nullptr) {
+        // Pixel-list pass: thread e takes listed pixel e and adds its samples in
+        // index order, as that many one-sample passes would: it stops once the
+        // pixel converges (the reference would stop rendering it) or reaches
+        // spp. A non-finite sample is not added, so the reference would render
+        // the same index in each remaining pass: those count as bad too.
+        const int64_t e = pixel;
+        const int64_t n = *P.active_count;
+        if (e < n) {
+            yrh_film_pixel& px = P.film[P.active_list[e]];
+            for (int k = 0; k < P.sample_count; ++k) {
+                if (px.samples >= static_cast<uint32_t>(P.spp)) break;
+                if (AddSample(P, static_cast<int64_t>(k) * n + e, px) != 0u) {
+                    bad += static_cast<unsigned>(P.sample_count - k);
+                    break;
+                }
+                if (P.act_on && PixelConverged(px, P.act_min_spp, P.act_rel, P.act_abs, P.act_conf)) break;
+            }
+        }
+    } else {
+        for (int k = 0; pixel < P.npix && k < P.sample_count; ++k) {
+            bad += AddSample(P, static_cast<int64_t>(k) * P.npix + pixel, P.film[pixel]);
+        }
+    }
+    WaveStat(P.stats, kStatBad, bad);
 }
 
 __global__ void ZeroWordsKernel(uint32_t* w, int n) {
@@ -1539,7 +1565,11 @@ PassView MakePassView(RenderState& R, const yrh_render_desc& d) {
     if (d.pixel_list != 0) {
         P.active_list = R.act_list.as<int32_t>();
         P.active_count = R.act_count.as<uint32_t>();
-        P.sample_count = 1;
+        P.act_on = R.act_desc.enabled;
+        P.act_min_spp = R.act_desc.min_spp;
+        P.act_rel = R.act_desc.relative_error;
+        P.act_abs = R.act_desc.absolute_error;
+        P.act_conf = R.act_desc.confidence;
     }
     return P;
 }
@@ -2025,6 +2055,7 @@ int yrh_film_update_active(yrh_context* ctx, const yrh_adaptive_desc* adaptive, 
                        static_cast<uint32_t>(spp), R.act_list.as<int32_t>(), R.act_count.as<uint32_t>());
     if ((rc = CheckHip(ctx, hipGetLastError(), "update active launch")) != YRH_OK) return rc;
     R.act_ready = true;
+    R.act_desc = on ? *adaptive : yrh_adaptive_desc{};
     if (active != nullptr || converged != nullptr) {
         uint32_t c[2] = {0, 0};
         if ((rc = yrh_copy_to_host(ctx, c, R.act_count.ptr, 8)) != YRH_OK) return rc;
@@ -2085,7 +2116,7 @@ int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* d) {
     if (d->pixel_list != 0 && !R.act_ready) {
         return Fail(ctx, YRH_E_INVALID_ARGUMENT, "pixel-list pass without an active list (yrh_film_update_active)");
     }
-    const int64_t count = d->pixel_list != 0 ? 1 : (d->sample_count < 1 ? 1 : d->sample_count);
+    const int64_t count = d->sample_count < 1 ? 1 : d->sample_count;
     if (static_cast<int64_t>(d->sample_index) + count > (int64_t{1} << 31) - 1) {
         return Fail(ctx, YRH_E_INVALID_ARGUMENT, "sample index overflow");
     }

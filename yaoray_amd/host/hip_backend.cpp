@@ -290,6 +290,11 @@ RenderResult HipBackend::Render(const PreparedScene& scene, const RenderRequest&
     const int batch = adaptive || request.progress_callback
         ? 1
         : static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(spp, batch_paths / std::max<int64_t>(1, npix))));
+    // Adaptive passes without a progress callback render kAdaptiveBatch samples
+    // of each active pixel, added in order until the pixel converges (the same
+    // film as one-sample passes; the first covers min_spp, before which no
+    // pixel can converge).
+    constexpr int kAdaptiveBatch = 16;
     for (int pass = request.resume_completed_spp; pass < spp && active > 0; pass += d.sample_count) {
         if (request.stop_token.stop_requested()) {
             if (!sync_film()) return fail(LastError(ctx, "film read"));
@@ -297,7 +302,11 @@ RenderResult HipBackend::Render(const PreparedScene& scene, const RenderRequest&
             return fail("render cancelled");
         }
         d.sample_index = pass;
-        d.sample_count = std::min(batch, spp - pass);
+        int take = batch;
+        if (adaptive && !request.progress_callback) {
+            take = pass == request.resume_completed_spp ? std::max(kAdaptiveBatch, as.min_spp - pass) : kAdaptiveBatch;
+        }
+        d.sample_count = std::min(take, spp - pass);
         if (yrh_render_pass(ctx, &d) != YRH_OK) return fail(LastError(ctx, "render pass"));
         if (adaptive) {
             if (yrh_film_update_active(ctx, &ad, spp, &active, &converged) != YRH_OK) {

@@ -385,7 +385,7 @@ int yrhost_backend_render(const yrhost_scene* s, const yrhost_render_settings* s
         if (request->cancel_error != nullptr) cancel_error = request->cancel_error;
     }
     const int cancel_after = request != nullptr ? request->cancel_after_passes : 0;
-    req.progress_callback = [&](const yr::RenderProgress& p, const yr::Film&) {
+    if (cancel_after >= 0) req.progress_callback = [&](const yr::RenderProgress& p, const yr::Film&) {
         ++report->progress_calls;
         report->last_progress_spp = p.completed_spp;
         report->last_progress_samples = p.completed_samples;

@@ -121,7 +121,8 @@ typedef struct yrhost_render_settings {
 /* yr::RenderRequest. The stop token and the progress callback are driven
  * from here: stop_before_start requests the stop before the first pass,
  * cancel_after_passes > 0 makes the progress callback cancel after that many
- * passes (with cancel_error as the message when non-empty). */
+ * passes (with cancel_error as the message when non-empty); < 0 renders
+ * without a progress callback. */
 typedef struct yrhost_render_request {
     const yrhost_film_pixel* resume_film; /* width*height pixels or NULL */
     int32_t resume_width;
