@@ -1,6 +1,6 @@
 // ORACLE — test infrastructure only. Nothing in the product links, loads or
 // calls this file; only tests/, __graft_entry__.smoke() and bench.py's CPU
-// baseline leg use it, as the checker.
+// baseline leg use it, to check the product's results against.
 //
 // CPU restatement of the reference's ray casting over its own BVH arrays:
 //   IntersectBounds      include/yaoray/core/bounds.hpp:64-90

@@ -1,7 +1,7 @@
 // ORACLE — test infrastructure only (see oracle/oracle_trace.cpp).
 //
 // C entry points over the reference itself, compiled from its own sources
-// under /root/reference by oracle/Makefile into oracle/_ref/libyrref.so. Used
+// where they lie by oracle/Makefile into oracle/_ref/libyrref.so. Used
 // to pin the restated oracle and the product's BVH builder (byte-identical
 // node arrays), and as the reference CPU baseline. Builds against the
 // reference's headers; contains no reference code.
