@@ -2023,6 +2023,7 @@ int yrh_film_load(yrh_context* ctx, const yrh_film_pixel* pixels, int64_t count)
     if (ctx == nullptr || ctx->render == nullptr || pixels == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "no film");
     RenderState& R = *ctx->render;
     if (count != static_cast<int64_t>(R.film_w) * R.film_h) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "film size mismatch");
+    R.act_ready = false;  // an active list describes the film it was built from
     return yrh_copy_to_device(ctx, R.film.ptr, pixels, static_cast<size_t>(count) * sizeof(yrh_film_pixel));
 }
 
