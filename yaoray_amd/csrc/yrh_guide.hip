@@ -280,11 +280,12 @@ __global__ __launch_bounds__(kBlock) void GuideBackwardKernel(TrainArgs A) {
     f32x4_t dx = yrg::Zero4();
     for (int kb = 0; kb < 4; ++kb) dx = yrg::Mfma(A.bwd[(yrg::kFragBase[0] + 0 * 4 + kb) * 64 + lane], dz0[kb], dx);
     // Hash-grid gradients. The 16 lanes of a quarter (lanes 16q..16q+15) hold
-    // 16 consecutive samples' shares of the same two levels; consecutive
-    // samples are neighbouring path vertices, so on the coarse levels they
-    // often touch the same corner entries. Where all 16 do (or add nothing),
-    // the quarter sums its contributions and adds them with one atomic per
-    // feature instead of 16.
+    // 16 rows' shares of the same two levels. Where all 16 touch the same
+    // corner entry (or add nothing) — rows from one cell of a coarse level,
+    // e.g. the rows of yrh_guide_gradients in set order — the quarter sums its
+    // contributions and adds them with one atomic per feature instead of 16.
+    // (Training batches are spread over the recorded set, where this is rare;
+    // runs of 16 neighbouring samples instead measured no faster.)
     {
         const uint64_t quarter = 0xFFFFull << (16 * q);
         const int quarter_lead = lane & ~15;
