@@ -523,6 +523,10 @@ int yrh_event_elapsed_ms(yrh_context* ctx, void* start, void* stop, float* ms);
  * in milliseconds (order_ms = 0 when the call ran no ordering pass). */
 int yrh_set_trace_timing(yrh_context* ctx, int enable);
 int yrh_trace_phase_ms(yrh_context* ctx, float* order_ms, float* trace_ms);
+/* Render passes (yrh_render_pass) run each bounce's shadow rays on a second
+ * stream, beside the next bounce's path-segment traversal (default on); the
+ * film is the same either way. No counterpart in the reference. */
+int yrh_set_render_overlap(yrh_context* ctx, int enable);
 
 #if defined(__GNUC__)
 #pragma GCC visibility pop

@@ -407,6 +407,10 @@ class Device:
     def set_trace_timing(self, enable: bool):
         self._check(self._lib.yrh_set_trace_timing(self._ctx, int(bool(enable))), "set_trace_timing")
 
+    def set_render_overlap(self, enable: bool):
+        """Shadow rays of a bounce on a second stream beside the next bounce's traversal."""
+        self._check(self._lib.yrh_set_render_overlap(self._ctx, int(bool(enable))), "set_render_overlap")
+
     def trace_phase_ms(self):
         """(ordering ms, traversal ms) of the last timed ray-cast call (waits for it)."""
         a, b = C.c_float(), C.c_float()

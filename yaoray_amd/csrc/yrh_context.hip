@@ -231,14 +231,20 @@ void yrh_destroy(yrh_context* ctx) {
     if (ctx == nullptr) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->aux_stream != nullptr) (void)hipStreamSynchronize(ctx->aux_stream);
     DestroyRenderState(ctx);
     DestroyGuideState(ctx);
     ctx->geo = DevGeometry{};
     ctx->work_counter.reset();
     ctx->stack_spill.reset();
+    ctx->stack_spill_aux.reset();
     for (hipEvent_t e : ctx->phase_events) {
         if (e != nullptr) (void)hipEventDestroy(e);
     }
+    for (hipEvent_t e : ctx->bounce_events) {
+        if (e != nullptr) (void)hipEventDestroy(e);
+    }
+    if (ctx->aux_stream != nullptr) (void)hipStreamDestroy(ctx->aux_stream);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -323,6 +329,12 @@ int yrh_set_trace_order(yrh_context* ctx, int order) {
 int yrh_set_trace_timing(yrh_context* ctx, int enable) {
     if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
     ctx->trace_timing = enable != 0;
+    return YRH_OK;
+}
+
+int yrh_set_render_overlap(yrh_context* ctx, int enable) {
+    if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
+    ctx->render_overlap = enable != 0;
     return YRH_OK;
 }
 

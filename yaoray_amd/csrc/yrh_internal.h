@@ -140,6 +140,13 @@ struct yrh_context {
     yrh::DevBuf work_counter;
     int launch_parity = 0;
     yrh::DevBuf stack_spill;    // int2 per lane per spilled stack level
+    // Render passes run each bounce's shadow rays on a second stream, beside
+    // the next bounce's path-segment traversal (yrh_render.hip); it has its
+    // own spill region. Default priority, like the main stream.
+    hipStream_t aux_stream = nullptr;
+    yrh::DevBuf stack_spill_aux;
+    hipEvent_t bounce_events[2] = {nullptr, nullptr};  // [0] shaded, [1] shadows resolved
+    bool render_overlap = true;
     // Launch shape of the persistent traversal kernels; 0 = chosen per launch
     // (yrh_trace.hip PrepareTraceArgs: 8 LDS stack levels and 8 workgroups per
     // CU for flat trees, 16 levels and 5 workgroups for two-level ones).
@@ -183,6 +190,7 @@ struct TraceRequest {
     uint32_t* occluded = nullptr;
     unsigned long long* totals = nullptr;  // [0] node tests, [1] triangle tests
     const int32_t* order = nullptr;        // claim position -> ray index (null = identity)
+    bool aux = false;                      // on the context's second stream, with its spill region
 };
 int LaunchTrace(yrh_context* ctx, const TraceRequest& r);
 

@@ -685,14 +685,21 @@ __global__ void RaygenKernel(PassView P) {
 }
 
 // Zeroes the counters a bounce appends to before anything uses them.
-__global__ void BeginBounceKernel(uint32_t* counters, int next_queue) {
+// Zeroes the counters a bounce uses before anything uses them: kBeginTrace
+// the path-segment claim counters (before the bounce's traversal), kBeginShade
+// the next queue's count, the shadow queue's count and its claim counters
+// (before Shade appends to them).
+constexpr int kBeginTrace = 1;
+constexpr int kBeginShade = 2;
+
+__global__ void BeginBounceKernel(uint32_t* counters, int next_queue, int what) {
     const int i = static_cast<int>(threadIdx.x);
-    if (i == 0) counters[next_queue] = 0u;
-    if (i == 1) counters[kCountShadow] = 0u;
-    if (i >= 2 && i < 2 + 2 * trav::kParts) {
-        const int k = i - 2;
-        counters[(k < trav::kParts ? kWorkExtend : kWorkShadow) + (k % trav::kParts) * trav::kPartStride] = 0u;
+    if (what & kBeginShade) {
+        if (i == 0) counters[next_queue] = 0u;
+        if (i == 1) counters[kCountShadow] = 0u;
+        if (i >= 2 && i < 2 + trav::kParts) counters[kWorkShadow + (i - 2) * trav::kPartStride] = 0u;
     }
+    if ((what & kBeginTrace) && i < trav::kParts) counters[kWorkExtend + i * trav::kPartStride] = 0u;
 }
 
 __global__ void CopyTraceTotalsKernel(unsigned long long* stats) {
@@ -1662,14 +1669,35 @@ int EnsureCapacity(yrh_context* ctx, RenderState& R, int64_t paths) {
 // Raygen and the bounces of one pass over `paths` path slots (the queue
 // capacity; the kernels read the live counts from the counter words).
 int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, const SurfaceView& S, const LightView& L,
-                       const yrg::GuideView& GV, int64_t paths, int max_depth) {
+                       const yrg::GuideView& GV, int64_t paths, int max_depth, bool overlap) {
     hipStream_t st = ctx->stream;
     const bool guided = GV.mode != YRH_GUIDE_OFF;
     int rc;
+    // Overlap: a bounce's shadow rays and Resolve run on the second stream
+    // while the main stream traces the next bounce's path segments (and
+    // queries the guide); Shade of the next bounce waits for that Resolve.
+    // Every per-path operation keeps its order, so the film is the same.
+    hipStream_t ss = st;
+    if (overlap) {
+        if (ctx->aux_stream == nullptr &&
+            (rc = CheckHip(ctx, hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking), "hipStreamCreate")) !=
+                YRH_OK) {
+            return rc;
+        }
+        for (hipEvent_t& e : ctx->bounce_events) {
+            if (e == nullptr &&
+                (rc = CheckHip(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate")) != YRH_OK) {
+                return rc;
+            }
+        }
+        ss = ctx->aux_stream;
+    }
+    hipEvent_t shaded = overlap ? ctx->bounce_events[0] : nullptr;
+    hipEvent_t resolved = overlap ? ctx->bounce_events[1] : nullptr;
     hipLaunchKernelGGL(RaygenKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P);
     for (int depth = 0; depth < max_depth; ++depth) {
         const int cur = depth & 1;
-        hipLaunchKernelGGL(BeginBounceKernel, dim3(1), dim3(64), 0, st, P.counters, 1 - cur);
+        hipLaunchKernelGGL(BeginBounceKernel, dim3(1), dim3(64), 0, st, P.counters, 1 - cur, kBeginTrace);
         TraceRequest tr;
         tr.any = false;
         tr.org = P.q_org[cur];
@@ -1688,12 +1716,25 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
         // Guided bounces: all that may continue, or the first depth_limit. A
         // plain vertex after guided ones is the plain estimator's vertex (its
         // NEE and emission weights use the pdf its direction was drawn from).
-        if (guided && depth + 1 < max_depth && (GV.depth_limit <= 0 || depth < GV.depth_limit)) {
+        const bool guide_here = guided && depth + 1 < max_depth && (GV.depth_limit <= 0 || depth < GV.depth_limit);
+        if (guide_here) {
             hipLaunchKernelGGL(GuideQueryKernel, dim3(Grid(paths, kQueryBlock)), dim3(kQueryBlock), 0, st, Pk, S, L, GV,
                                cur);
+        }
+        // The previous bounce's Resolve reads this bounce's path queue count,
+        // the path slots and the shadow queue, which Shade rewrites.
+        if (overlap && depth > 0 && (rc = CheckHip(ctx, hipStreamWaitEvent(st, resolved, 0), "hipStreamWaitEvent")) != YRH_OK) {
+            return rc;
+        }
+        hipLaunchKernelGGL(BeginBounceKernel, dim3(1), dim3(64), 0, st, P.counters, 1 - cur, kBeginShade);
+        if (guide_here) {
             hipLaunchKernelGGL(ShadeKernel<true>, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
         } else {
             hipLaunchKernelGGL(ShadeKernel<false>, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
+        }
+        if (overlap) {
+            if ((rc = CheckHip(ctx, hipEventRecord(shaded, st), "hipEventRecord")) != YRH_OK) return rc;
+            if ((rc = CheckHip(ctx, hipStreamWaitEvent(ss, shaded, 0), "hipStreamWaitEvent")) != YRH_OK) return rc;
         }
         if (!R.transparent) {
             TraceRequest sr;
@@ -1707,6 +1748,7 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
             sr.parts = trav::kParts;
             sr.occluded = P.s_occ;
             sr.totals = P.stats + kStatTraceNodes;
+            sr.aux = overlap;
             if ((rc = LaunchTrace(ctx, sr)) != YRH_OK) return rc;
         } else {
             TraceRequest wr;
@@ -1720,19 +1762,23 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
             wr.parts = trav::kParts;
             wr.hit = P.s_trans;  // not written by the walk; LaunchTrace's checks want an output
             wr.totals = P.stats + kStatTraceNodes;
+            wr.aux = overlap;
             trav::TraceArgs wa;
             int wgrid = 0, wblock = 0;
             if ((rc = PrepareTraceArgs(ctx, wr, wa, wgrid, wblock)) != YRH_OK) return rc;
             if (ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL) {
-                hipLaunchKernelGGL(ShadowWalkKernel<true>, dim3(wgrid), dim3(wblock), LdsBytes(wa, wblock), st, wa, P, S,
+                hipLaunchKernelGGL(ShadowWalkKernel<true>, dim3(wgrid), dim3(wblock), LdsBytes(wa, wblock), ss, wa, P, S,
                                    L);
             } else {
-                hipLaunchKernelGGL(ShadowWalkKernel<false>, dim3(wgrid), dim3(wblock), LdsBytes(wa, wblock), st, wa, P, S,
+                hipLaunchKernelGGL(ShadowWalkKernel<false>, dim3(wgrid), dim3(wblock), LdsBytes(wa, wblock), ss, wa, P, S,
                                    L);
             }
         }
-        hipLaunchKernelGGL(ResolveKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P, GV, depth, cur);
+        hipLaunchKernelGGL(ResolveKernel, dim3(Grid(paths)), dim3(kBlock), 0, ss, P, GV, depth, cur);
+        if (overlap && (rc = CheckHip(ctx, hipEventRecord(resolved, ss), "hipEventRecord")) != YRH_OK) return rc;
     }
+    if (overlap && (rc = CheckHip(ctx, hipStreamWaitEvent(st, resolved, 0), "hipStreamWaitEvent")) != YRH_OK) return rc;
+    if ((rc = CheckHip(ctx, hipGetLastError(), "pass sequence launch")) != YRH_OK) return rc;
     if (GV.rec != nullptr) {
         hipLaunchKernelGGL(GuideCollectKernel, dim3(Grid(paths * max_depth, kBlock * kCollectPer)), dim3(kBlock), 0, st,
                            P, GV);
@@ -2134,7 +2180,7 @@ int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* d) {
     const int max_depth = std::max(1, d->max_depth);
     yrg::GuideView GV;
     if ((rc = GuidePrepareView(ctx, paths, max_depth, &GV)) != YRH_OK) return rc;
-    if ((rc = LaunchPassSequence(ctx, R, P, S, L, GV, paths, max_depth)) != YRH_OK) return rc;
+    if ((rc = LaunchPassSequence(ctx, R, P, S, L, GV, paths, max_depth, ctx->render_overlap)) != YRH_OK) return rc;
     hipLaunchKernelGGL(AccumulateKernel, dim3(Grid(npix)), dim3(kBlock), 0, st, P);
     hipLaunchKernelGGL(CopyTraceTotalsKernel, dim3(1), dim3(64), 0, st, P.stats);
     return CheckHip(ctx, hipGetLastError(), "render pass launch");
@@ -2265,7 +2311,7 @@ int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* d, float* job_ms, i
     }
     hipStream_t st = ctx->stream;
     auto launch_pass = [&]() -> int {
-        int r = LaunchPassSequence(ctx, R, P, S, L, GV, capacity, max_depth);
+        int r = LaunchPassSequence(ctx, R, P, S, L, GV, capacity, max_depth, false);
         if (r != YRH_OK) return r;
         for (int32_t occ = 0; occ <= max_occurrence; ++occ) {
             hipLaunchKernelGGL(AccumulateBatchKernel, dim3(Grid(capacity)), dim3(kBlock), 0, st, P, occ);

@@ -57,14 +57,18 @@ int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a
     const int32_t capacity = g.stack_capacity;
     const int64_t spill_levels = std::max<int64_t>(0, capacity - lds_levels);
     const size_t spill_bytes = static_cast<size_t>(std::max<int64_t>(1, spill_levels * total)) * sizeof(int2);
-    if (ctx->stack_spill.bytes < spill_bytes) {
-        // Earlier launches on the stream may still use the old region.
+    DevBuf& spill = r.aux ? ctx->stack_spill_aux : ctx->stack_spill;
+    if (spill.bytes < spill_bytes) {
+        // Earlier launches on either stream may still use the old region.
         int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        if (rc == YRH_OK && ctx->aux_stream != nullptr) {
+            rc = CheckHip(ctx, hipStreamSynchronize(ctx->aux_stream), "hipStreamSynchronize");
+        }
         if (rc != YRH_OK) return rc;
-        ctx->stack_spill.reset();
-        rc = CheckHip(ctx, hipMalloc(&ctx->stack_spill.ptr, spill_bytes), "hipMalloc stack spill");
+        spill.reset();
+        rc = CheckHip(ctx, hipMalloc(&spill.ptr, spill_bytes), "hipMalloc stack spill");
         if (rc != YRH_OK) return rc;
-        ctx->stack_spill.bytes = spill_bytes;
+        spill.bytes = spill_bytes;
     }
     a = TraceArgs{};
     a.org = r.org;
@@ -105,7 +109,7 @@ int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a
     }
     a.order = r.order;
     a.overflow = words + 2;
-    a.spill = ctx->stack_spill.as<int2>();
+    a.spill = spill.as<int2>();
     a.spill_stride = total;
     a.capacity = capacity;
     a.lds_levels = lds_levels;
@@ -123,17 +127,18 @@ int LaunchTrace(yrh_context* ctx, const TraceRequest& r) {
     int rc = PrepareTraceArgs(ctx, r, a, grid, block);
     if (rc != YRH_OK) return rc;
     const bool two = ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL;
+    hipStream_t st = r.aux ? ctx->aux_stream : ctx->stream;
     if (r.any) {
         if (two) {
-            hipLaunchKernelGGL(TraceTwoLevelKernel<true>, dim3(grid), dim3(block), LdsBytes(a, block), ctx->stream, a);
+            hipLaunchKernelGGL(TraceTwoLevelKernel<true>, dim3(grid), dim3(block), LdsBytes(a, block), st, a);
         } else {
-            hipLaunchKernelGGL(TraceFlatKernel<true>, dim3(grid), dim3(block), LdsBytes(a, block), ctx->stream, a);
+            hipLaunchKernelGGL(TraceFlatKernel<true>, dim3(grid), dim3(block), LdsBytes(a, block), st, a);
         }
     } else {
         if (two) {
-            hipLaunchKernelGGL(TraceTwoLevelKernel<false>, dim3(grid), dim3(block), LdsBytes(a, block), ctx->stream, a);
+            hipLaunchKernelGGL(TraceTwoLevelKernel<false>, dim3(grid), dim3(block), LdsBytes(a, block), st, a);
         } else {
-            hipLaunchKernelGGL(TraceFlatKernel<false>, dim3(grid), dim3(block), LdsBytes(a, block), ctx->stream, a);
+            hipLaunchKernelGGL(TraceFlatKernel<false>, dim3(grid), dim3(block), LdsBytes(a, block), st, a);
         }
     }
     rc = CheckHip(ctx, hipGetLastError(), r.any ? "trace any launch" : "trace closest launch");

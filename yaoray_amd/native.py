@@ -223,6 +223,7 @@ YRH_SIGNATURES = {
     "yrh_set_trace_lds_levels": (_I, [_VP, _I]),
     "yrh_set_trace_order": (_I, [_VP, _I]),
     "yrh_set_trace_timing": (_I, [_VP, _I]),
+    "yrh_set_render_overlap": (_I, [_VP, _I]),
     "yrh_suggest_paths_per_pass": (_I, [_VP, _I, C.POINTER(C.c_int64)]),
     "yrh_trace_phase_ms": (_I, [_VP, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "yrh_event_create": (_I, [_VP, C.POINTER(_VP)]),
