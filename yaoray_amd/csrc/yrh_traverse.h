@@ -601,10 +601,15 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
 
         while (true) {
             // One record per iteration, as in the flat loop: a BLAS leaf's
-            // triangles one by one, BLAS completion, the TLAS leaf's next
-            // instance, or a stack entry.
+            // triangles one by one, a node from the stack, or a BLAS's root.
+            // A lane that finishes a BLAS or expands a TLAS leaf tries its
+            // next instance in the same iteration, and an instance whose root
+            // box is hit goes straight to its root record (no stack round
+            // trip): fewer iterations per instance visit, the same
+            // operations in the same order.
             bool node = false, tri = false;
             const float4* src = nullptr;
+            bool try_instance = false;
             if (in_blas && tcur < tend) {
                 tri = true;
             } else if (in_blas && st.sp == blas_base) {
@@ -621,25 +626,9 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
                 in_blas = false;
                 pre = Precompute(ox, oy, oz, dx, dy, dz);
                 ++cursor;
+                try_instance = true;
             } else if (!in_blas && cursor < leaf_end) {
-                // Next instance of the TLAS leaf: object ray, BLAS root box.
-                const DevInstanceSlot m = a.instance_slots[cursor];
-                cur_inst = __float_as_int(m.hi.w);
-                const DevInstanceSlot& root = m;
-                TransformRay(m.r0, m.r1, m.r2, ox, oy, oz, dx, dy, dz, bx, by, bz, bdx, bdy, bdz);
-                const RayPre bpre = Precompute(bx, by, bz, bdx, bdy, bdz);
-                ++nt;
-                float entry;
-                if (Slab(bpre, root.lo.x, root.lo.y, root.lo.z, root.hi.x, root.hi.y, root.hi.z, t_min, best.t,
-                         entry)) {
-                    pre = bpre;
-                    bbest = Best{best.t, 0.0f, 0.0f, -1, false};
-                    blas_base = st.sp;
-                    st.Push(__float_as_int(root.lo.w), entry);
-                    in_blas = true;
-                } else {
-                    ++cursor;
-                }
+                try_instance = true;
             } else if (st.sp > 0) {
                 const int2 e = st.Pop();
                 const int32_t code = e.x;
@@ -661,10 +650,43 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
                     if (code < 0) {
                         DecodeLeaf(code, a.tleaf_count, cursor, leaf_end);
                         leaf_end += cursor;
+                        try_instance = true;
                     } else {
                         node = true;
                         src = reinterpret_cast<const float4*>(a.tnodes + code);
                     }
+                }
+            }
+            if (try_instance && cursor < leaf_end) {
+                // Next instance of the TLAS leaf: object ray, BLAS root box.
+                const DevInstanceSlot m = a.instance_slots[cursor];
+                cur_inst = __float_as_int(m.hi.w);
+                const DevInstanceSlot& root = m;
+                TransformRay(m.r0, m.r1, m.r2, ox, oy, oz, dx, dy, dz, bx, by, bz, bdx, bdy, bdz);
+                const RayPre bpre = Precompute(bx, by, bz, bdx, bdy, bdz);
+                ++nt;
+                float entry;
+                if (Slab(bpre, root.lo.x, root.lo.y, root.lo.z, root.hi.x, root.hi.y, root.hi.z, t_min, best.t,
+                         entry)) {
+                    // The root's entry distance is within [t_min, best t], so the
+                    // stack pop it replaces would always have taken it.
+                    pre = bpre;
+                    bbest = Best{best.t, 0.0f, 0.0f, -1, false};
+                    blas_base = st.sp;
+                    in_blas = true;
+                    const int32_t code = __float_as_int(root.lo.w);
+                    if (code < 0) {
+                        int32_t first, count;
+                        DecodeLeaf(code, a.leaf_count, first, count);
+                        tcur = first;
+                        tend = first + count;
+                        tri = count > 0;
+                    } else {
+                        node = true;
+                        src = reinterpret_cast<const float4*>(a.nodes + code);
+                    }
+                } else {
+                    ++cursor;
                 }
             }
             if (tri) src = a.slots + 3 * static_cast<int64_t>(tcur);
