@@ -39,6 +39,9 @@ extern "C" {
 #define YRH_E_UNSUPPORTED (-3)
 #define YRH_E_NO_SCENE (-4)
 #define YRH_E_CANCELLED (-5)
+/* A traversal dropped stack entries: a BVH is deeper than the max_depth it was
+ * uploaded with (uploads reject such BVHs; reported at the next sync point). */
+#define YRH_E_OVERFLOW (-6)
 
 typedef struct yrh_context yrh_context;
 
@@ -186,6 +189,14 @@ int yrh_copy_to_host(yrh_context* ctx, void* dst, const void* src, size_t bytes)
 
 /* ---- Acceleration structure --------------------------------------------- */
 
+/* The checks yrh_upload_geometry applies before touching the device: array
+ * sizes and presence, no spheres, child and leaf ranges of every BVH, and each
+ * BVH's declared max_depth at least its real depth (root = 1, as
+ * yr::RenderBvh::max_depth counts it, src/accel/bvh_builder.cpp:226-244): the
+ * traversal stack is sized from it. Needs no device or context. Writes a
+ * message into error (may be NULL) on failure. */
+int yrh_validate_geometry(const yrh_geometry_desc* geometry, char* error, int error_capacity);
+
 /* Replaces the context's geometry. Consumes the reference's own BVH arrays
  * (flat: yr::RenderBvh; two-level: yr::TwoLevelBvh) exactly as built by
  * yr::BuildRenderAcceleration (src/accel/acceleration.cpp:28-46) and repacks
@@ -194,7 +205,10 @@ int yrh_upload_geometry(yrh_context* ctx, const yrh_geometry_desc* geometry);
 
 /* Batched closest hit over n rays: the GPU counterpart of calling
  * yr::IntersectAcceleration (src/accel/acceleration.cpp:48-61) once per ray.
- * Launched on the context's stream; returns without synchronising. */
+ * Launched on the context's stream; returns without synchronising. A
+ * traversal stack overflow (impossible for geometry that passed
+ * yrh_validate_geometry) is reported as YRH_E_OVERFLOW by the next
+ * yrh_synchronize, yrh_film_read, yrh_render_stats_read or yrh_batch_render. */
 int yrh_trace_closest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n,
                       const yrh_hit_soa* hits);
 
@@ -453,6 +467,16 @@ int yrh_batch_film_read(yrh_context* ctx, int32_t job, yrh_film_pixel* pixels, i
  * computed on the device; dims = 1 or 2 floats per entry. Host arrays. */
 int yrh_sample_values(yrh_context* ctx, int32_t kind, uint64_t seed, int32_t spp, int64_t n, const int32_t* x,
                       const int32_t* y, const int32_t* sample, const int32_t* dim, int32_t dims, float* out);
+
+/* Batched BSDF queries on the device, the arithmetic the shading kernels run:
+ * for query i, material materials[material_index[i]] and the 12 floats
+ * queries[12i..] = wo (3), wi (3), shading normal (3), u (2), unused, out[12i..]
+ * receives EvaluateBsdf(wo, wi, n) (3 floats), PdfBsdf(wo, wi, n), then
+ * SampleBsdf(wo, n, u): wi (3), weight (3), pdf, and flags = valid + 2 specular
+ * + 4 IsDeltaBsdf (src/shading/bsdf.cpp:17-329; include/yaoray/shading/bsdf.hpp:10-16).
+ * Host arrays; synchronises. */
+int yrh_bsdf_query(yrh_context* ctx, const yrh_material* materials, int32_t material_count, int64_t n,
+                   const int32_t* material_index, const float* queries, float* out);
 
 /* Device ray generation for one pass (GeneratePerspectiveCameraRay with the
  * sampler's Pixel dimension): writes width*height rays into device arrays. */

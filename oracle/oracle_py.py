@@ -393,3 +393,30 @@ def guide_mixture_eval(mixture, dirs, u):
     _guide_lib().oracle_guide_mixture_eval(len(m), m.ctypes.data, d.ctypes.data, uu.ctypes.data, pdf.ctypes.data,
                                            smp.ctypes.data)
     return pdf, smp
+
+
+# ---- BSDF queries (query layout of oracle_bsdf_query) ---------------------------------
+
+
+def _bsdf(fn_name, lib, materials, material_index, queries):
+    from yaoray_amd.api import material_array
+    arr = material_array(materials)
+    idx = np.ascontiguousarray(material_index, np.int32)
+    q = np.ascontiguousarray(queries, np.float32).reshape(-1, 12)
+    out = np.zeros((len(q), 12), np.float32)
+    fn = getattr(lib, fn_name)
+    fn.argtypes = [_VP, _I32, _I64, _VP, _VP, _VP]
+    if fn(C.cast(arr, _VP), len(materials), len(q), idx.ctypes.data, q.ctypes.data, out.ctypes.data) != 0:
+        raise RuntimeError(f"{fn_name} failed")
+    return out
+
+
+def bsdf_query(materials, material_index, queries):
+    """The restated EvaluateBsdf / PdfBsdf / SampleBsdf / IsDeltaBsdf per query row
+    (wo, wi, normal, u0, u1, unused) -> (f, pdf, sampled wi, weight, pdf, flags)."""
+    return _bsdf("oracle_bsdf_query", oracle_lib(), materials, material_index, queries)
+
+
+def ref_bsdf_query(materials, material_index, queries):
+    """The reference's own BSDF functions on the same query rows."""
+    return _bsdf("ref_bsdf_query", ref_lib(), materials, material_index, queries)

@@ -1284,6 +1284,29 @@ int oracle_camera_rays(const yrh_render_desc* rd, float* org_tmin, float* dir_tm
     return 0;
 }
 
+// BSDF queries: for each i, material mats[mat[i]] and the 12 floats q[12i..]
+// = wo(3), wi(3), normal(3), u(2), unused; writes 12 floats out[12i..] =
+// EvaluateBsdf(wo, wi, n) (3), PdfBsdf(wo, wi, n), then SampleBsdf(wo, n, u):
+// wi (3), weight (3), pdf, and flags = valid + 2 specular + 4 IsDeltaBsdf
+// (src/shading/bsdf.cpp:17-329).
+int oracle_bsdf_query(const yrh_material* mats, int32_t nmat, int64_t n, const int32_t* mat, const float* q,
+                      float* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        if (mat[i] < 0 || mat[i] >= nmat) return -1;
+        const yrh_material& m = mats[mat[i]];
+        const float* a = q + 12 * i;
+        const V3 wo = Load3(a), wi = Load3(a + 3), nn = Load3(a + 6);
+        const V3 f = EvalBsdf(m, wo, wi, nn);
+        const float pdf = PdfBsdf(m, wo, wi, nn);
+        const BS s = SampleBsdf(m, wo, nn, a[9], a[10]);
+        float* o = out + 12 * i;
+        const float vals[12] = {f.x, f.y, f.z, pdf, s.wi.x, s.wi.y, s.wi.z, s.weight.x, s.weight.y, s.weight.z, s.pdf,
+                                static_cast<float>((s.valid ? 1 : 0) + (s.specular ? 2 : 0) + (IsDelta(m) ? 4 : 0))};
+        std::memcpy(o, vals, sizeof(vals));
+    }
+    return 0;
+}
+
 // Sampler values for the tests (host restatement of the device kernel).
 int oracle_sample_values(int32_t kind, uint64_t seed, int32_t spp, int64_t n, const int32_t* x, const int32_t* y,
                          const int32_t* sample, const int32_t* dim, int32_t dims, float* out) {

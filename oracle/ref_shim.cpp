@@ -10,7 +10,9 @@
 #include <yaoray/backends/cpu/cpu_prepared_scene.hpp>
 #include <yaoray/core/transform.hpp>
 #include <yaoray/lighting/environment.hpp>
+#include <yaoray/core/rng.hpp>
 #include <yaoray/sampling/sampler.hpp>
+#include <yaoray/shading/bsdf.hpp>
 #include <yaoray/scene/render_scene.hpp>
 
 #include "../include/yrh.h"
@@ -29,6 +31,24 @@ struct RefScene {
     yr::RenderSceneIR scene;
     yr::RenderAcceleration accel;
 };
+
+// The reference material a yrh_material describes (untextured parameters).
+yr::RenderMaterial ToRefMaterial(const yrh_material& m) {
+    yr::RenderMaterial rm;
+    rm.kind = m.kind == YRH_MATERIAL_DIFFUSE ? yr::RenderMaterialKind::Diffuse
+            : m.kind == YRH_MATERIAL_CONDUCTOR ? yr::RenderMaterialKind::Conductor
+            : m.kind == YRH_MATERIAL_DIELECTRIC ? yr::RenderMaterialKind::Dielectric
+                                                : yr::RenderMaterialKind::ThinDielectric;
+    rm.reflectance.value = yr::Color3f{m.reflectance[0], m.reflectance[1], m.reflectance[2]};
+    rm.ior = m.ior;
+    rm.uroughness.value = m.uroughness;
+    rm.vroughness.value = m.vroughness;
+    rm.alpha.value = m.alpha;
+    rm.emission = yr::Color3f{m.emission[0], m.emission[1], m.emission[2]};
+    rm.absorption_color = yr::Color3f{m.absorption_color[0], m.absorption_color[1], m.absorption_color[2]};
+    rm.absorption_distance = m.absorption_distance;
+    return rm;
+}
 
 }  // namespace
 
@@ -229,23 +249,7 @@ int ref_render(const yrh_vertex* vertices, int64_t nv, const uint32_t* indices, 
         std::memcpy(inst.object_to_world.m.data(), inst_mat + 16 * static_cast<size_t>(i), 64);
         sc.instances.push_back(inst);
     }
-    for (int32_t k = 0; k < sd->material_count; ++k) {
-        const yrh_material& m = sd->materials[k];
-        yr::RenderMaterial rm;
-        rm.kind = m.kind == YRH_MATERIAL_DIFFUSE ? yr::RenderMaterialKind::Diffuse
-                : m.kind == YRH_MATERIAL_CONDUCTOR ? yr::RenderMaterialKind::Conductor
-                : m.kind == YRH_MATERIAL_DIELECTRIC ? yr::RenderMaterialKind::Dielectric
-                                                    : yr::RenderMaterialKind::ThinDielectric;
-        rm.reflectance.value = yr::Color3f{m.reflectance[0], m.reflectance[1], m.reflectance[2]};
-        rm.ior = m.ior;
-        rm.uroughness.value = m.uroughness;
-        rm.vroughness.value = m.vroughness;
-        rm.alpha.value = m.alpha;
-        rm.emission = yr::Color3f{m.emission[0], m.emission[1], m.emission[2]};
-        rm.absorption_color = yr::Color3f{m.absorption_color[0], m.absorption_color[1], m.absorption_color[2]};
-        rm.absorption_distance = m.absorption_distance;
-        sc.materials.push_back(rm);
-    }
+    for (int32_t k = 0; k < sd->material_count; ++k) sc.materials.push_back(ToRefMaterial(sd->materials[k]));
     for (int32_t k = 0; k < sd->emissive_count; ++k) {
         const yrh_emissive& e = sd->emissives[k];
         sc.emissive_primitives.push_back(
@@ -327,6 +331,32 @@ int ref_sample_values(int32_t kind, uint64_t seed, int32_t spp, int64_t n, const
             out[2 * i] = v.x;
             out[2 * i + 1] = v.y;
         }
+    }
+    return 0;
+}
+
+// The reference's EvaluateBsdf / PdfBsdf / SampleBsdf / IsDeltaBsdf
+// (src/shading/bsdf.cpp:17-329) on the query layout of oracle_bsdf_query:
+// q[12i..] = wo, wi, normal, u (2), unused; out[12i..] = f (3), pdf, sampled
+// wi (3), weight (3), pdf, flags (valid + 2 specular + 4 delta). A fresh
+// Rng{1} per query (only layered BSDFs read it).
+int ref_bsdf_query(const yrh_material* mats, int32_t nmat, int64_t n, const int32_t* mat, const float* q,
+                   float* out) {
+    std::vector<yr::RenderMaterial> rm;
+    for (int32_t k = 0; k < nmat; ++k) rm.push_back(ToRefMaterial(mats[k]));
+    for (int64_t i = 0; i < n; ++i) {
+        if (mat[i] < 0 || mat[i] >= nmat) return -1;
+        const yr::RenderMaterial& m = rm[static_cast<size_t>(mat[i])];
+        const float* a = q + 12 * i;
+        const yr::Vec3f wo{a[0], a[1], a[2]}, wi{a[3], a[4], a[5]}, nn{a[6], a[7], a[8]};
+        yr::Rng rng{1u};
+        const yr::Color3f f = yr::EvaluateBsdf(m, wo, wi, nn, rng);
+        const float pdf = yr::PdfBsdf(m, wo, wi, nn, rng);
+        const yr::BsdfSample s = yr::SampleBsdf(m, wo, nn, yr::Vec2f{a[9], a[10]}, rng);
+        const float vals[12] = {f.x, f.y, f.z, pdf, s.wi.x, s.wi.y, s.wi.z, s.weight.x, s.weight.y, s.weight.z, s.pdf,
+                                static_cast<float>((s.valid ? 1 : 0) + (s.specular ? 2 : 0) +
+                                                   (yr::IsDeltaBsdf(m) ? 4 : 0))};
+        std::memcpy(out + 12 * i, vals, sizeof(vals));
     }
     return 0;
 }

@@ -14,7 +14,8 @@ triangle counters (src/accel/bvh_traversal.cpp:253-345,
 src/accel/two_level_bvh.cpp:365-481), yr::Sampler bits
 (src/sampling/sampler.cpp) and FP32 films + stats of RenderCpuPathTrace
 (src/backends/cpu/cpu_path_tracer.cpp), plain and with adaptive sampling
-(its active-pixel loop and Film::IsConverged, src/film/film.cpp:113-121).
+(its active-pixel loop and Film::IsConverged, src/film/film.cpp:113-121),
+and BSDF eval / pdf / sample tuples (src/shading/bsdf.cpp:17-329).
 """
 import hashlib
 import os
@@ -154,7 +155,39 @@ def cornell_adaptive_fixture():
             "converged1": np.uint64(st1["converged_pixels"]), "seed": np.uint64(SEED)}
 
 
-FIXTURES = {"sampler": sampler_fixture, "flat_trace": flat_fixture, "two_level_trace": two_level_fixture,
+def bsdf_fixture():
+    """EvaluateBsdf / PdfBsdf / SampleBsdf / IsDeltaBsdf of the reference
+    (src/shading/bsdf.cpp:17-329) on 256 random queries per material of
+    tests/bsdf_cases.py (every in-scope branch) and on the known-answer cases."""
+    from tests import bsdf_cases as B
+    mi, q = B.queries(256, seed=12)
+    cases = B.known_answer_cases()
+    mats = list(B.MATERIALS) + [m for _, m, _ in cases]
+    mi = np.concatenate([mi, np.arange(len(B.MATERIALS), len(mats), dtype=np.int32)])
+    q = np.concatenate([q, np.stack([row for _, _, row in cases])])
+    return {"kind": np.array([m["kind"] for m in mats], np.int32),
+            "reflectance": np.array([m["reflectance"] for m in mats], np.float32),
+            "ior": np.array([m.get("ior", 1.5) for m in mats], np.float32),
+            "uroughness": np.array([m.get("uroughness", 0.0) for m in mats], np.float32),
+            "vroughness": np.array([m.get("vroughness", m.get("uroughness", 0.0)) for m in mats], np.float32),
+            "material_index": mi.astype(np.int32), "queries": q,
+            "results": O.ref_bsdf_query(mats, mi, q)}
+
+
+def zoo_film_fixture():
+    """The reference's FP32 film + statistics of the material-zoo scene (every
+    in-scope BSDF branch, sky environment): 64 x 36, 8 spp, depth 12."""
+    from tests.helpers import material_zoo_scene
+    s = material_zoo_scene()
+    film, st, _ = O.ref_render(s, 64, 36, 8, max_depth=12, sampler=2, seed=SEED)
+    keys = O.STAT_KEYS
+    from tests.golden_data import scene_digest
+    return {"digest": np.array(scene_digest(s)), "width": np.int32(64), "height": np.int32(36),
+            "spp": np.int32(8), "depth": np.int32(12), "film": film.view(np.uint8).reshape(-1),
+            "stats": np.array([st[k] for k in keys], np.uint64), "seed": np.uint64(SEED)}
+
+
+FIXTURES = {"sampler": sampler_fixture, "bsdf": bsdf_fixture, "zoo_film": zoo_film_fixture, "flat_trace": flat_fixture, "two_level_trace": two_level_fixture,
             "cornell_film": cornell_fixture, "cornell_adaptive": cornell_adaptive_fixture}
 
 

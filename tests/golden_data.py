@@ -66,3 +66,30 @@ def cornell_scene(f):
     s.build_accel()
     assert geometry_digest(s) == str(f["digest"]), "Cornell workload inputs differ from the fixture's"
     return s
+
+
+def bsdf_materials(f):
+    """Material dicts of the BSDF fixture (kind, reflectance, ior, roughness)."""
+    return [{"kind": int(k), "reflectance": tuple(float(x) for x in r), "ior": float(i), "uroughness": float(u),
+             "vroughness": float(v)}
+            for k, r, i, u, v in zip(f["kind"], f["reflectance"], f["ior"], f["uroughness"], f["vroughness"])]
+
+
+def scene_digest(scene):
+    """Digest of geometry, camera, instances and materials (the zoo fixture's key)."""
+    import ctypes
+
+    from yaoray_amd import native as N
+    h = hashlib.sha256(geometry_digest(scene).encode())
+    _, _, inst = scene.tlas()
+    h.update(np.ascontiguousarray(inst).tobytes())
+    sd = scene.shading_desc()
+    h.update(ctypes.string_at(sd.materials, sd.material_count * ctypes.sizeof(N.Material)))
+    return h.hexdigest()
+
+
+def zoo_scene(f):
+    from tests.helpers import material_zoo_scene
+    s = material_zoo_scene()
+    assert scene_digest(s) == str(f["digest"]), "material-zoo inputs differ from the fixture's"
+    return s

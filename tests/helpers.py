@@ -149,3 +149,30 @@ def guide_samples(n, seed):
     wi /= np.linalg.norm(wi, axis=1, keepdims=True)
     w = rng.uniform(0.1, 2.0, (n, 1))
     return np.hstack([p, nrm, wo, wi, w]).astype(np.float32)
+
+
+# Ground + one material per icosphere variant of Scene.instanced(variants=8):
+# every BSDF branch of the in-scope kinds, including an absorbing smooth
+# dielectric (Beer-Lambert inside the medium), rough dielectrics, a delta
+# mirror and smooth / rough thin dielectrics.
+ZOO_MATERIALS = [
+    {"kind": 0, "reflectance": (0.5, 0.5, 0.5)},
+    {"kind": 0, "reflectance": (0.7, 0.4, 0.3)},
+    {"kind": 1, "reflectance": (0.9, 0.7, 0.4), "uroughness": 0.4},
+    {"kind": 1, "reflectance": (0.95, 0.95, 0.95), "uroughness": 0.0},
+    {"kind": 2, "reflectance": (1.0, 1.0, 1.0), "ior": 1.5, "absorption_color": (0.6, 0.8, 0.9),
+     "absorption_distance": 0.5},
+    {"kind": 2, "reflectance": (1.0, 1.0, 1.0), "ior": 1.5, "uroughness": 0.3},
+    {"kind": 2, "reflectance": (0.9, 0.85, 0.8), "ior": 1.33, "uroughness": 0.55},
+    {"kind": 3, "reflectance": (1.0, 1.0, 1.0), "ior": 1.5},
+    {"kind": 3, "reflectance": (0.9, 0.95, 1.0), "ior": 1.5, "uroughness": 0.35},
+]
+
+
+def material_zoo_scene(instances=150, subdivisions=2, env_width=64, env_height=32, seed=4):
+    """Two-level environment-lit scene whose instances carry every BSDF branch."""
+    s = Scene.instanced(instances=instances, subdivisions=subdivisions, variants=8, env_width=env_width,
+                        env_height=env_height, seed=seed)
+    s.set_materials(ZOO_MATERIALS)
+    s.build_accel()
+    return s

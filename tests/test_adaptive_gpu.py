@@ -151,3 +151,21 @@ def test_backend_without_progress_callback(device):
     pb, pfb = s.backend_render(48, 32, 6, max_depth=6, seed=seed, cancel_after_passes=-1)
     assert pb["ok"] == 1 and pb["progress_calls"] == 0
     assert pfb.tobytes() == pf1.tobytes() and pb["rays_traced"] == p1["rays_traced"]
+
+
+def test_backend_adaptive_passes_respect_the_path_cap(device, monkeypatch):
+    """Adaptive passes without a progress callback are capped at the paths a pass
+    may hold (device memory; here forced down with YAORAY_HIP_MAX_PATHS_PER_PASS):
+    min_spp x pixels is 16x the cap, so the min_spp samples are split over many
+    pixel-list passes. The film is the uncapped one, byte for byte."""
+    f = G.load("cornell_adaptive")
+    s = G.cornell_scene(f)
+    a = (16, 0.1, 0.001, 1.96)
+    seed = int(f["seed"])
+    rep, film = s.backend_render(64, 64, 24, max_depth=8, sampler=2, seed=seed, adaptive=a, cancel_after_passes=-1)
+    monkeypatch.setenv("YAORAY_HIP_MAX_PATHS_PER_PASS", str(64 * 64))
+    repc, filmc = s.backend_render(64, 64, 24, max_depth=8, sampler=2, seed=seed, adaptive=a, cancel_after_passes=-1)
+    assert rep["ok"] == 1 and repc["ok"] == 1, (rep["error"], repc["error"])
+    assert filmc.tobytes() == film.tobytes()
+    assert repc["converged_pixels"] == rep["converged_pixels"]
+    assert repc["rays_traced"] <= rep["rays_traced"]  # one-sample passes discard no samples

@@ -105,3 +105,15 @@ def test_oracle_adaptive_limits():
     still = tight["luminance_m2"] == 0.0
     assert np.all(tight["samples"][still] == 2) and np.all(tight["samples"][~still] == 8)
     assert conv == int(still.sum())
+
+
+def test_oracle_render_matches_zoo_fixture():
+    """Every in-scope BSDF branch (rough / smooth / absorbing / thin dielectrics,
+    mirror and rough conductors, diffuse) under the sky environment: the oracle's
+    FP32 film and counters equal the reference's bit for bit."""
+    f = G.load("zoo_film")
+    s = G.zoo_scene(f)
+    w, h = int(f["width"]), int(f["height"])
+    film, st = O.render(s, w, h, int(f["spp"]), max_depth=int(f["depth"]), sampler=2, seed=int(f["seed"]))
+    assert film.tobytes() == G.film(f["film"]).tobytes()
+    assert st == G.stats(f["stats"])

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle_py as O
-from tests.helpers import base_path_scene
+from tests.helpers import base_path_scene, material_zoo_scene
 from yaoray_amd import Scene
 
 needs_ref = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference sources absent)")
@@ -82,3 +82,16 @@ def test_known_answer_scene_matches_reference_bits():
     a, sa = O.render(s, 8, 6, 2, max_depth=3, sampler=1, seed=7)
     b, sb, _ = O.ref_render(s, 8, 6, 2, max_depth=3, sampler=1, seed=7)
     assert a.tobytes() == b.tobytes() and sa == {k: sb[k] for k in sa}
+
+
+@needs_ref
+def test_material_zoo_scene_matches_reference_bits():
+    """Every in-scope BSDF branch in a render: rough and smooth dielectrics (one
+    absorbing), a delta mirror, smooth and rough thin dielectrics, diffuse and
+    rough conductor instances under the sky environment."""
+    s = material_zoo_scene()
+    for depth, spp in ((1, 1), (12, 3)):
+        a, sa = O.render(s, 64, 36, spp, max_depth=depth)
+        b, sb, _ = O.ref_render(s, 64, 36, spp, max_depth=depth)
+        assert a.tobytes() == b.tobytes()
+        assert sa == {k: sb[k] for k in sa}

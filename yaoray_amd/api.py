@@ -85,6 +85,32 @@ class BvhArrays:
         return d
 
 
+def validate_geometry(desc: N.GeometryDesc):
+    """yrh_validate_geometry: the device-free checks of a geometry upload.
+    Returns (status, message); status 0 when the geometry would be accepted."""
+    buf = C.create_string_buffer(512)
+    rc = N.yrh().yrh_validate_geometry(C.byref(desc), buf, len(buf))
+    return rc, buf.value.decode()
+
+
+def material_array(materials):
+    """ctypes array of native.Material from dicts (kind, reflectance, ior, uroughness, vroughness,
+    alpha, emission, absorption_distance, absorption_color; missing keys take the defaults)."""
+    arr = (N.Material * max(1, len(materials)))()
+    for k, m in enumerate(materials):
+        a = arr[k]
+        a.kind = m.get("kind", 0)
+        a.reflectance[:] = m.get("reflectance", (0.5, 0.5, 0.5))
+        a.ior = m.get("ior", 1.5)
+        a.uroughness = m.get("uroughness", 0.0)
+        a.vroughness = m.get("vroughness", m.get("uroughness", 0.0))
+        a.alpha = m.get("alpha", 1.0)
+        a.emission[:] = m.get("emission", (0.0, 0.0, 0.0))
+        a.absorption_distance = m.get("absorption_distance", 1.0)
+        a.absorption_color[:] = m.get("absorption_color", (1.0, 1.0, 1.0))
+    return arr
+
+
 class Scene:
     """A RenderSceneIR owned by the host library, with its acceleration structure."""
 
@@ -211,18 +237,7 @@ class Scene:
 
     def set_materials(self, materials):
         """materials: list of dicts with keys of native.Material (kind, reflectance, emission, ...)."""
-        arr = (N.Material * len(materials))()
-        for k, m in enumerate(materials):
-            a = arr[k]
-            a.kind = m.get("kind", 0)
-            a.reflectance[:] = m.get("reflectance", (0.5, 0.5, 0.5))
-            a.ior = m.get("ior", 1.5)
-            a.uroughness = m.get("uroughness", 0.0)
-            a.vroughness = m.get("vroughness", m.get("uroughness", 0.0))
-            a.alpha = m.get("alpha", 1.0)
-            a.emission[:] = m.get("emission", (0.0, 0.0, 0.0))
-            a.absorption_distance = m.get("absorption_distance", 1.0)
-            a.absorption_color[:] = m.get("absorption_color", (1.0, 1.0, 1.0))
+        arr = material_array(materials)
         if self._lib.yrhost_scene_set_materials(self._h, C.cast(arr, C.c_void_p), len(materials)) != 0:
             raise YrhError("set_materials failed")
 
@@ -543,6 +558,18 @@ class Device:
         self._check(self._lib.yrh_sample_values(self._ctx, kind, seed, spp, n, *[a.ctypes.data for a in arrs], dims,
                                                 out.ctypes.data), "sample_values")
         return out.reshape(n, dims) if dims == 2 else out
+
+    def bsdf_query(self, materials, material_index, queries):
+        """Device BSDF queries: materials = list of material dicts (Scene.set_materials keys),
+        queries = (n, 12) float32 rows (wo, wi, normal, u0, u1, unused). Returns (n, 12) rows
+        (f, pdf, sampled wi, weight, pdf, flags = valid + 2 specular + 4 delta)."""
+        arr = material_array(materials)
+        idx = np.ascontiguousarray(material_index, np.int32)
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 12)
+        out = np.zeros((len(q), 12), np.float32)
+        self._check(self._lib.yrh_bsdf_query(self._ctx, C.cast(arr, C.c_void_p), len(materials), len(q),
+                                             idx.ctypes.data, q.ctypes.data, out.ctypes.data), "bsdf_query")
+        return out
 
     def camera_rays(self, desc: N.RenderDesc):
         n = desc.width * desc.height

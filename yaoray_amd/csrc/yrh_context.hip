@@ -19,6 +19,19 @@ int CheckHip(yrh_context* ctx, hipError_t status, const char* what) {
     return Fail(ctx, YRH_E_HIP, std::string(what) + ": " + hipGetErrorString(status));
 }
 
+int CheckStackOverflow(yrh_context* ctx) {
+    if (ctx->work_counter.ptr == nullptr) return YRH_OK;
+    uint32_t flag = 0;
+    const uint32_t* word = ctx->work_counter.as<uint32_t>() + kOverflowWord;
+    int rc = CheckHip(ctx, hipMemcpyAsync(&flag, word, sizeof(flag), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+    if (rc == YRH_OK) rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (rc != YRH_OK || flag == 0) return rc;
+    (void)hipMemsetAsync(const_cast<uint32_t*>(word), 0, sizeof(flag), ctx->stream);
+    return Fail(ctx, YRH_E_OVERFLOW,
+                "traversal stack overflow: a BVH is deeper than the max_depth it was uploaded with; results of the "
+                "traversals since the last check are incomplete");
+}
+
 namespace {
 
 template <typename T>
@@ -44,13 +57,14 @@ struct PackedBvh {
     bool empty = true;
 };
 
-// Validates one reference BVH and appends its repacked form to `out`, rebasing
-// node indices by out.nodes.size() and slots by out.slots.size() / 3.
-// `slot_payload(slot, prim)` writes the three float4 of a slot.
-template <typename SlotFn>
-bool PackBvh(const yrh_bvh_desc& bvh, PackedBvh& out, std::string& error, SlotFn&& slot_payload) {
-    const int32_t node_base = static_cast<int32_t>(out.nodes.size());
-    const int64_t slot_base = static_cast<int64_t>(out.leaf_count.size());
+// Checks one reference BVH's arrays (child and leaf ranges) and that its
+// declared max_depth covers the tree: the traversal stack is sized from the
+// declared depth, so an under-declared one would drop entries. Depth counts
+// the root as 1, as the reference's builder does (src/accel/bvh_builder.cpp:226-244,
+// bvh_serial_builder.cpp:44-55). Children follow their parent in the node
+// array (left = i + 1, right > i), so one pass in index order finds the
+// deepest node.
+bool ValidateBvh(const yrh_bvh_desc& bvh, std::string& error) {
     if (bvh.node_count < 0 || bvh.primitive_index_count < 0 || bvh.primitive_count < 0) {
         error = "negative BVH array size";
         return false;
@@ -59,17 +73,9 @@ bool PackBvh(const yrh_bvh_desc& bvh, PackedBvh& out, std::string& error, SlotFn
         error = "BVH arrays missing";
         return false;
     }
-    if (slot_base + bvh.primitive_index_count > kMaxSlot) {
-        error = "too many BVH primitive slots for the device layout";
-        return false;
-    }
-    const auto code_of = [&](int32_t node) -> int32_t {
-        const yrh_bvh_node& n = bvh.nodes[node];
-        if (n.primitive_count > 0) {
-            return PackLeaf(static_cast<int32_t>(slot_base) + n.payload_offset, n.primitive_count);
-        }
-        return node_base + node;
-    };
+    std::vector<int32_t> depth(static_cast<size_t>(bvh.node_count), 0);
+    int32_t deepest = 0;
+    if (bvh.node_count > 0) depth[0] = 1;
     for (int32_t i = 0; i < bvh.node_count; ++i) {
         const yrh_bvh_node& n = bvh.nodes[i];
         if (n.primitive_count > 0) {
@@ -85,8 +91,79 @@ bool PackBvh(const yrh_bvh_desc& bvh, PackedBvh& out, std::string& error, SlotFn
                 error = "BVH interior node child out of bounds";
                 return false;
             }
+            const int32_t d = depth[static_cast<size_t>(i)] + 1;
+            depth[static_cast<size_t>(left)] = std::max(depth[static_cast<size_t>(left)], d);
+            depth[static_cast<size_t>(right)] = std::max(depth[static_cast<size_t>(right)], d);
         }
+        deepest = std::max(deepest, depth[static_cast<size_t>(i)]);
     }
+    if (bvh.max_depth < deepest) {
+        error = "BVH max_depth " + std::to_string(bvh.max_depth) + " is below the tree's depth " +
+                std::to_string(deepest) + " (the traversal stack is sized from it)";
+        return false;
+    }
+    return true;
+}
+
+// Descriptor checks of yrh_upload_geometry that need no device: array sizes
+// and presence, no spheres, and every BVH (flat, each BLAS, the TLAS) valid
+// with a declared max_depth that covers it.
+int ValidateGeometry(const yrh_geometry_desc* g, std::string& error) {
+    if (g == nullptr) {
+        error = "null argument";
+        return YRH_E_INVALID_ARGUMENT;
+    }
+    if (g->sphere_count != 0) {
+        error = "sphere primitives are not supported by the HIP backend";
+        return YRH_E_UNSUPPORTED;
+    }
+    if (g->vertex_count < 0 || g->index_count < 0 || g->primitive_count < 0) {
+        error = "negative geometry array size";
+        return YRH_E_INVALID_ARGUMENT;
+    }
+    if ((g->vertex_count > 0 && g->vertices == nullptr) || (g->index_count > 0 && g->indices == nullptr) ||
+        (g->primitive_count > 0 && g->primitives == nullptr)) {
+        error = "geometry arrays missing";
+        return YRH_E_INVALID_ARGUMENT;
+    }
+    if (g->accel_kind == YRH_ACCEL_FLAT) {
+        if (!ValidateBvh(g->flat, error)) return YRH_E_INVALID_ARGUMENT;
+    } else if (g->accel_kind == YRH_ACCEL_TWO_LEVEL) {
+        if (g->blas_count < 0 || g->instance_count < 0 || (g->blas_count > 0 && g->blases == nullptr) ||
+            (g->instance_count > 0 && g->instances == nullptr)) {
+            error = "two-level arrays missing";
+            return YRH_E_INVALID_ARGUMENT;
+        }
+        for (int32_t b = 0; b < g->blas_count; ++b) {
+            if (!ValidateBvh(g->blases[b], error)) return YRH_E_INVALID_ARGUMENT;
+        }
+        if (!ValidateBvh(g->tlas, error)) return YRH_E_INVALID_ARGUMENT;
+    } else {
+        error = "unknown acceleration kind";
+        return YRH_E_INVALID_ARGUMENT;
+    }
+    return YRH_OK;
+}
+
+// Validates one reference BVH and appends its repacked form to `out`, rebasing
+// node indices by out.nodes.size() and slots by out.slots.size() / 3.
+// `slot_payload(slot, prim)` writes the three float4 of a slot.
+template <typename SlotFn>
+bool PackBvh(const yrh_bvh_desc& bvh, PackedBvh& out, std::string& error, SlotFn&& slot_payload) {
+    const int32_t node_base = static_cast<int32_t>(out.nodes.size());
+    const int64_t slot_base = static_cast<int64_t>(out.leaf_count.size());
+    if (!ValidateBvh(bvh, error)) return false;
+    if (slot_base + bvh.primitive_index_count > kMaxSlot) {
+        error = "too many BVH primitive slots for the device layout";
+        return false;
+    }
+    const auto code_of = [&](int32_t node) -> int32_t {
+        const yrh_bvh_node& n = bvh.nodes[node];
+        if (n.primitive_count > 0) {
+            return PackLeaf(static_cast<int32_t>(slot_base) + n.payload_offset, n.primitive_count);
+        }
+        return node_base + node;
+    };
     out.nodes.resize(out.nodes.size() + static_cast<size_t>(bvh.node_count));
     for (int32_t i = 0; i < bvh.node_count; ++i) {
         const yrh_bvh_node& n = bvh.nodes[i];
@@ -255,7 +332,8 @@ const char* yrh_last_error(const yrh_context* ctx) {
 
 int yrh_synchronize(yrh_context* ctx) {
     if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
-    return CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    const int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return rc != YRH_OK ? rc : CheckStackOverflow(ctx);
 }
 
 void* yrh_stream(yrh_context* ctx) { return ctx == nullptr ? nullptr : static_cast<void*>(ctx->stream); }
@@ -374,15 +452,22 @@ int yrh_event_elapsed_ms(yrh_context* ctx, void* start, void* stop, float* ms) {
                     "hipEventElapsedTime");
 }
 
+int yrh_validate_geometry(const yrh_geometry_desc* g, char* error, int error_capacity) {
+    std::string msg;
+    const int rc = ValidateGeometry(g, msg);
+    if (rc != YRH_OK && error != nullptr && error_capacity > 0) {
+        std::strncpy(error, msg.c_str(), static_cast<size_t>(error_capacity - 1));
+        error[error_capacity - 1] = '\0';
+    }
+    return rc;
+}
+
 int yrh_upload_geometry(yrh_context* ctx, const yrh_geometry_desc* g) {
     if (ctx == nullptr || g == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null argument");
-    if (g->sphere_count != 0) return Fail(ctx, YRH_E_UNSUPPORTED, "sphere primitives are not supported by the HIP backend");
-    if (g->vertex_count < 0 || g->index_count < 0 || g->primitive_count < 0) {
-        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "negative geometry array size");
-    }
-    if ((g->vertex_count > 0 && g->vertices == nullptr) || (g->index_count > 0 && g->indices == nullptr) ||
-        (g->primitive_count > 0 && g->primitives == nullptr)) {
-        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "geometry arrays missing");
+    {
+        std::string msg;
+        const int vrc = ValidateGeometry(g, msg);
+        if (vrc != YRH_OK) return Fail(ctx, vrc, msg);
     }
     (void)hipSetDevice(ctx->device);
     int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");

@@ -29,6 +29,7 @@ namespace yrh {
 
 constexpr int kLeafCountEscape = 15;
 // Context work-counter words (yrh_context::work_counter).
+constexpr int kOverflowWord = 2;  // traversal stack overflow flag
 constexpr int kCounterSetBase = 256;
 constexpr int kCounterSetWords = 256;  // 8 ranges x 32 words
 constexpr size_t kWorkCounterBytes = (kCounterSetBase + 2 * kCounterSetWords) * sizeof(uint32_t);
@@ -171,6 +172,10 @@ namespace yrh {
 
 int Fail(yrh_context* ctx, int code, const std::string& message);
 int CheckHip(yrh_context* ctx, hipError_t status, const char* what);
+// Reads (and clears) the traversal stack overflow flag after synchronising the
+// context stream: YRH_E_OVERFLOW when a traversal since the last check dropped
+// stack entries.
+int CheckStackOverflow(yrh_context* ctx);
 
 // One launch of a traversal kernel over a ray queue (yrh_trace.hip).
 struct TraceRequest {

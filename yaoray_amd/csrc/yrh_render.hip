@@ -1628,6 +1628,46 @@ LightView MakeLightView(RenderState& R) {
     return L;
 }
 
+DevMaterial ToDevMaterial(const yrh_material& m) {
+    DevMaterial d;
+    d.kind = m.kind;
+    d.reflectance = make_float3(m.reflectance[0], m.reflectance[1], m.reflectance[2]);
+    d.ior = m.ior;
+    d.urough = m.uroughness;
+    d.vrough = m.vroughness;
+    d.alpha = m.alpha;
+    d.emission = make_float3(m.emission[0], m.emission[1], m.emission[2]);
+    d.absorption_distance = m.absorption_distance;
+    d.absorption_color = make_float3(m.absorption_color[0], m.absorption_color[1], m.absorption_color[2]);
+    return d;
+}
+
+// One BSDF query per thread (yrh_bsdf_query): the device EvalBsdf, PdfBsdf,
+// SampleBsdf and IsDelta the shading kernels run, on caller tuples.
+__global__ void BsdfQueryKernel(const DevMaterial* mats, int64_t n, const int32_t* mat, const float* q, float* out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevMaterial m = mats[mat[i]];
+    const float* a = q + 12 * i;
+    const float3 wo = V(a[0], a[1], a[2]), wi = V(a[3], a[4], a[5]), nn = V(a[6], a[7], a[8]);
+    const float3 f = EvalBsdf(m, wo, wi, nn);
+    const float pdf = PdfBsdf(m, wo, wi, nn);
+    const BsdfSample s = SampleBsdf(m, wo, nn, make_float2(a[9], a[10]));
+    float* o = out + 12 * i;
+    o[0] = f.x;
+    o[1] = f.y;
+    o[2] = f.z;
+    o[3] = pdf;
+    o[4] = s.wi.x;
+    o[5] = s.wi.y;
+    o[6] = s.wi.z;
+    o[7] = s.weight.x;
+    o[8] = s.weight.y;
+    o[9] = s.weight.z;
+    o[10] = s.pdf;
+    o[11] = static_cast<float>((s.valid ? 1 : 0) + (s.specular ? 2 : 0) + (IsDelta(m) ? 4 : 0));
+}
+
 // Bytes per path slot and per shadow slot of the arrays EnsureCapacity
 // allocates, and per guided row (mixture + valid flag).
 constexpr size_t kPathSlotBytes = 4 + 4 + 4 + 16 + 16 + 16 + 4 + 16 + 16 + 16 + 16 + 8 + 2 * 16 + 2 * 16 + 2 * 4 + 16 + 4;
@@ -1897,16 +1937,7 @@ int yrh_upload_shading(yrh_context* ctx, const yrh_shading_desc* sd) {
             return Fail(ctx, YRH_E_UNSUPPORTED, "material kind not supported by the HIP backend");
         }
         if (!(m.alpha >= 0.5f)) return Fail(ctx, YRH_E_UNSUPPORTED, "alpha cut-out materials are not supported");
-        DevMaterial& d = mats[static_cast<size_t>(k)];
-        d.kind = m.kind;
-        d.reflectance = make_float3(m.reflectance[0], m.reflectance[1], m.reflectance[2]);
-        d.ior = m.ior;
-        d.urough = m.uroughness;
-        d.vrough = m.vroughness;
-        d.alpha = m.alpha;
-        d.emission = make_float3(m.emission[0], m.emission[1], m.emission[2]);
-        d.absorption_distance = m.absorption_distance;
-        d.absorption_color = make_float3(m.absorption_color[0], m.absorption_color[1], m.absorption_color[2]);
+        mats[static_cast<size_t>(k)] = ToDevMaterial(m);
         if (m.kind == YRH_MATERIAL_DIELECTRIC || m.kind == YRH_MATERIAL_THIN_DIELECTRIC) R.transparent = true;
     }
     if ((rc = UploadVec(ctx, R.materials, mats.data(), mats.size())) != YRH_OK) return rc;
@@ -2077,7 +2108,8 @@ int yrh_film_read(yrh_context* ctx, yrh_film_pixel* pixels, int64_t count) {
     if (ctx == nullptr || ctx->render == nullptr || pixels == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "no film");
     RenderState& R = *ctx->render;
     if (count != static_cast<int64_t>(R.film_w) * R.film_h) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "film size mismatch");
-    return yrh_copy_to_host(ctx, pixels, R.film.ptr, static_cast<size_t>(count) * sizeof(yrh_film_pixel));
+    const int rc = yrh_copy_to_host(ctx, pixels, R.film.ptr, static_cast<size_t>(count) * sizeof(yrh_film_pixel));
+    return rc != YRH_OK ? rc : CheckStackOverflow(ctx);
 }
 
 int yrh_film_update_active(yrh_context* ctx, const yrh_adaptive_desc* adaptive, int32_t spp, int64_t* active,
@@ -2120,6 +2152,7 @@ int yrh_render_stats_read(yrh_context* ctx, yrh_render_stats* stats) {
     }
     std::vector<unsigned long long> slots(static_cast<size_t>(kStatStride) * kStatSlots);
     int rc = yrh_copy_to_host(ctx, slots.data(), ctx->render->stats.ptr, kStatBytes);
+    if (rc == YRH_OK) rc = CheckStackOverflow(ctx);
     if (rc != YRH_OK) return rc;
     unsigned long long w[kStatWords] = {};
     for (int k = 0; k < kStatSlots; ++k) {
@@ -2367,7 +2400,7 @@ int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* d, float* job_ms, i
         }
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     }
-    return rc;
+    return rc != YRH_OK ? rc : CheckStackOverflow(ctx);
 }
 
 int yrh_batch_film_read(yrh_context* ctx, int32_t job, yrh_film_pixel* pixels, int64_t count) {
@@ -2400,6 +2433,39 @@ int yrh_sample_values(yrh_context* ctx, int32_t kind, uint64_t seed, int32_t spp
                        bx.as<int32_t>(), by.as<int32_t>(), bs.as<int32_t>(), bd.as<int32_t>(), dims, bo.as<float>());
     if ((rc = CheckHip(ctx, hipGetLastError(), "sample kernel launch")) != YRH_OK) return rc;
     return yrh_copy_to_host(ctx, out, bo.ptr, static_cast<size_t>(n) * static_cast<size_t>(dims) * sizeof(float));
+}
+
+int yrh_bsdf_query(yrh_context* ctx, const yrh_material* materials, int32_t material_count, int64_t n,
+                   const int32_t* material_index, const float* queries, float* out) {
+    if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
+    if (n < 0 || material_count <= 0 || materials == nullptr ||
+        (n > 0 && (material_index == nullptr || queries == nullptr || out == nullptr))) {
+        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "bad bsdf_query arguments");
+    }
+    std::vector<DevMaterial> mats(static_cast<size_t>(material_count));
+    for (int32_t k = 0; k < material_count; ++k) {
+        if (materials[k].kind < YRH_MATERIAL_DIFFUSE || materials[k].kind > YRH_MATERIAL_THIN_DIELECTRIC) {
+            return Fail(ctx, YRH_E_UNSUPPORTED, "material kind not supported by the HIP backend");
+        }
+        mats[static_cast<size_t>(k)] = ToDevMaterial(materials[k]);
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        if (material_index[i] < 0 || material_index[i] >= material_count) {
+            return Fail(ctx, YRH_E_INVALID_ARGUMENT, "material index out of range");
+        }
+    }
+    if (n == 0) return YRH_OK;
+    (void)hipSetDevice(ctx->device);
+    DevBuf bm, bi, bq, bo;
+    int rc;
+    if ((rc = UploadVec(ctx, bm, mats.data(), mats.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, bi, material_index, static_cast<size_t>(n))) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, bq, queries, static_cast<size_t>(12 * n))) != YRH_OK) return rc;
+    if ((rc = Alloc(ctx, bo, static_cast<size_t>(12 * n) * sizeof(float))) != YRH_OK) return rc;
+    hipLaunchKernelGGL(BsdfQueryKernel, dim3(Grid(n)), dim3(kBlock), 0, ctx->stream, bm.as<DevMaterial>(), n,
+                       bi.as<int32_t>(), bq.as<float>(), bo.as<float>());
+    if ((rc = CheckHip(ctx, hipGetLastError(), "bsdf query launch")) != YRH_OK) return rc;
+    return yrh_copy_to_host(ctx, out, bo.ptr, static_cast<size_t>(12 * n) * sizeof(float));
 }
 
 int yrh_generate_camera_rays(yrh_context* ctx, const yrh_render_desc* d, float* org_tmin, float* dir_tmax) {

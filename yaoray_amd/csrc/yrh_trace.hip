@@ -108,7 +108,7 @@ int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a
         a.parts = trav::kParts;
     }
     a.order = r.order;
-    a.overflow = words + 2;
+    a.overflow = words + kOverflowWord;
     a.spill = spill.as<int2>();
     a.spill_stride = total;
     a.capacity = capacity;

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <stdexcept>
 #include <string>
@@ -283,17 +284,23 @@ RenderResult HipBackend::Render(const PreparedScene& scene, const RenderRequest&
     // passes, so consecutive passes are batched into one wavefront as large as
     // the device memory suggests (same per-pixel sample order, so the same film).
     int64_t batch_paths = int64_t{1} << 23;
-    if (!adaptive && !request.progress_callback &&
-        yrh_suggest_paths_per_pass(ctx, settings.max_depth, &batch_paths) != YRH_OK) {
+    if (!request.progress_callback && yrh_suggest_paths_per_pass(ctx, settings.max_depth, &batch_paths) != YRH_OK) {
         return fail(LastError(ctx, "suggest paths per pass"));
+    }
+    // YAORAY_HIP_MAX_PATHS_PER_PASS caps the wavefront (device memory shared
+    // with other work); any cap gives the same film.
+    if (const char* cap = std::getenv("YAORAY_HIP_MAX_PATHS_PER_PASS"); cap != nullptr && std::atoll(cap) > 0) {
+        batch_paths = std::min<int64_t>(batch_paths, std::atoll(cap));
     }
     const int batch = adaptive || request.progress_callback
         ? 1
         : static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(spp, batch_paths / std::max<int64_t>(1, npix))));
     // Adaptive passes without a progress callback render kAdaptiveBatch samples
     // of each active pixel, added in order until the pixel converges (the same
-    // film as one-sample passes; the first covers min_spp, before which no
-    // pixel can converge).
+    // film as one-sample passes; the first ones cover min_spp, before which no
+    // pixel can converge). A pass holds at most batch_paths paths (active
+    // pixels x samples), so large frames take more, smaller passes; splitting
+    // a pixel-list pass leaves the film unchanged.
     constexpr int kAdaptiveBatch = 16;
     for (int pass = request.resume_completed_spp; pass < spp && active > 0; pass += d.sample_count) {
         if (request.stop_token.stop_requested()) {
@@ -304,7 +311,8 @@ RenderResult HipBackend::Render(const PreparedScene& scene, const RenderRequest&
         d.sample_index = pass;
         int take = batch;
         if (adaptive && !request.progress_callback) {
-            take = pass == request.resume_completed_spp ? std::max(kAdaptiveBatch, as.min_spp - pass) : kAdaptiveBatch;
+            take = std::max(kAdaptiveBatch, as.min_spp - pass);
+            take = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(take, batch_paths / std::max<int64_t>(1, active))));
         }
         d.sample_count = std::min(take, spp - pass);
         if (yrh_render_pass(ctx, &d) != YRH_OK) return fail(LastError(ctx, "render pass"));

@@ -16,6 +16,12 @@ import os
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 
 YRH_OK = 0
+E_INVALID_ARGUMENT = -1
+E_HIP = -2
+E_UNSUPPORTED = -3
+E_NO_SCENE = -4
+E_CANCELLED = -5
+E_OVERFLOW = -6
 YRH_ACCEL_FLAT = 0
 YRH_ACCEL_TWO_LEVEL = 1
 
@@ -238,6 +244,8 @@ YRH_SIGNATURES = {
     "yrh_film_update_active": (_I, [_VP, C.POINTER(AdaptiveDesc), _I32, C.POINTER(_I64), C.POINTER(_I64)]),
     "yrh_render_stats_read": (_I, [_VP, C.POINTER(RenderStats)]),
     "yrh_sample_values": (_I, [_VP, _I32, _U64, _I32, _I64, _VP, _VP, _VP, _VP, _I32, _VP]),
+    "yrh_bsdf_query": (_I, [_VP, _VP, _I32, _I64, _VP, _VP, _VP]),
+    "yrh_validate_geometry": (_I, [_VP, _VP, _I32]),
     "yrh_generate_camera_rays": (_I, [_VP, C.POINTER(RenderDesc), _VP, _VP]),
     "yrh_batch_render": (_I, [_VP, C.POINTER(BatchDesc), _VP, C.POINTER(_I32)]),
     "yrh_batch_film_read": (_I, [_VP, _I32, _VP, _I64]),
