@@ -17,12 +17,13 @@ _VP, _I32, _I64 = C.c_void_p, C.c_int32, C.c_int64
 
 
 def build(with_reference: bool | None = None, jobs: int = 8):
-    """make -C oracle [ref]; the reference part only where its sources exist."""
+    """make -C oracle [ref dropin]; the reference parts only where its sources exist (dropin links
+    yaoray_amd/lib/libyrh.so, so build that first)."""
     subprocess.run(["make", "-s", "-C", HERE, f"-j{jobs}", "all"], check=True)
     if with_reference is None:
         with_reference = os.path.isdir(os.path.join(REFERENCE_ROOT, "src"))
     if with_reference:
-        subprocess.run(["make", "-s", "-C", HERE, f"-j{jobs}", "ref", f"REFERENCE_ROOT={REFERENCE_ROOT}"],
+        subprocess.run(["make", "-s", "-C", HERE, f"-j{jobs}", "ref", "dropin", f"REFERENCE_ROOT={REFERENCE_ROOT}"],
                        check=True)
 
 

@@ -18,7 +18,7 @@ OBJ = os.path.join(ROOT, "_build")
 
 HIP_SOURCES = ["yrh_context.hip", "yrh_trace.hip", "yrh_render.hip", "yrh_guide.hip", "yrh_order.hip"]
 HOST_SOURCES = ["transform.cpp", "bvh_builder.cpp", "light_tables.cpp", "scenes.cpp", "abi_marshal.cpp",
-                "hip_backend.cpp", "host_capi.cpp"]
+                "hip_backend.cpp", "standalone_boundary.cpp", "host_capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 ops are disabled: hipcc would otherwise emit v_pk_* f32 forms.

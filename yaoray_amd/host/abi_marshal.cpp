@@ -73,13 +73,17 @@ std::string DescribeShading(const yr::RenderSceneIR& ir, Marshal& m, yrh_shading
             case yr::RenderMaterialKind::ThinDielectric: a.kind = YRH_MATERIAL_THIN_DIELECTRIC; break;
             default: return "material kind outside the HIP backend's set";
         }
-        a.reflectance[0] = mat.reflectance.x;
-        a.reflectance[1] = mat.reflectance.y;
-        a.reflectance[2] = mat.reflectance.z;
+        if (mat.reflectance.texture != -1 || mat.uroughness.texture != -1 || mat.vroughness.texture != -1 ||
+            mat.alpha.texture != -1 || mat.normal_map != -1) {
+            return "textured material parameters are not supported by the HIP backend";
+        }
+        a.reflectance[0] = mat.reflectance.value.x;
+        a.reflectance[1] = mat.reflectance.value.y;
+        a.reflectance[2] = mat.reflectance.value.z;
         a.ior = mat.ior;
-        a.uroughness = mat.uroughness;
-        a.vroughness = mat.vroughness;
-        a.alpha = mat.alpha;
+        a.uroughness = mat.uroughness.value;
+        a.vroughness = mat.vroughness.value;
+        a.alpha = mat.alpha.value;
         a.emission[0] = mat.emission.x;
         a.emission[1] = mat.emission.y;
         a.emission[2] = mat.emission.z;
@@ -98,9 +102,10 @@ std::string DescribeShading(const yr::RenderSceneIR& ir, Marshal& m, yrh_shading
     d.material_count = static_cast<int32_t>(m.materials.size());
     d.emissives = m.emissives.data();
     d.emissive_count = static_cast<int32_t>(m.emissives.size());
-    d.alias_probability = ir.emissive_power.probability.data();
-    d.alias_index = ir.emissive_power.alias.data();
-    d.alias_pmf = ir.emissive_power.pmf.data();
+    const yr::AliasTable& power = ir.light_sampling.emissive_power;
+    d.alias_probability = power.probability.data();
+    d.alias_index = power.alias.data();
+    d.alias_pmf = power.pmf.data();
     yrh_environment_desc& e = d.environment;
     const yr::RenderEnvironment& env = ir.environment;
     e.active = env.active ? 1 : 0;

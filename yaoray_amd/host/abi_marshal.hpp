@@ -6,7 +6,12 @@
 #include <vector>
 
 #include "../../include/yrh.h"
+#if defined(YAORAY_HIP_DROPIN)
+#include <yaoray/accel/acceleration.hpp>
+#include <yaoray/scene/render_scene.hpp>
+#else
 #include "yr_ir.hpp"
+#endif
 
 namespace yrhip {
 

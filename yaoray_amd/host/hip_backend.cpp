@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cmath>
+#include <limits>
 #include <stdexcept>
 #include <string>
 
@@ -14,38 +15,6 @@
 #include "abi_marshal.hpp"
 
 namespace yr {
-
-Film::Film(int width, int height) : width_(width), height_(height) {
-    if (width <= 0 || height <= 0) throw std::invalid_argument("Film dimensions must be positive");
-    pixels_.resize(static_cast<size_t>(width) * static_cast<size_t>(height));
-}
-
-Color3f Film::LinearPixel(int x, int y) const {
-    if (x < 0 || y < 0 || x >= width_ || y >= height_) throw std::out_of_range("Film pixel coordinate out of range");
-    const FilmPixel& p = pixels_[static_cast<size_t>(y) * static_cast<size_t>(width_) + static_cast<size_t>(x)];
-    return p.samples == 0 ? Color3f{} : p.sum / static_cast<float>(p.samples);
-}
-
-std::uint32_t Film::SampleCount(int x, int y) const {
-    if (x < 0 || y < 0 || x >= width_ || y >= height_) throw std::out_of_range("Film pixel coordinate out of range");
-    return pixels_[static_cast<size_t>(y) * static_cast<size_t>(width_) + static_cast<size_t>(x)].samples;
-}
-
-float Film::LuminanceVariance(int x, int y) const {
-    const FilmPixel& p = pixels_[static_cast<size_t>(y) * static_cast<size_t>(width_) + static_cast<size_t>(x)];
-    return p.samples > 1 ? p.luminance_m2 / static_cast<float>(p.samples - 1) : 0.0f;
-}
-
-std::uint64_t Film::TotalSampleCount() const {
-    std::uint64_t total = 0;
-    for (const FilmPixel& p : pixels_) total += p.samples;
-    return total;
-}
-
-void Film::SetPixelForCheckpoint(int x, int y, FilmPixel pixel) {
-    if (x < 0 || y < 0 || x >= width_ || y >= height_) return;
-    pixels_[static_cast<size_t>(y) * static_cast<size_t>(width_) + static_cast<size_t>(x)] = pixel;
-}
 
 class HipPreparedScene final : public PreparedScene {
 public:
@@ -89,7 +58,11 @@ void ToFilm(const std::vector<yrh_film_pixel>& src, Film& film, std::uint64_t ba
             film.SetPixelForCheckpoint(x, y, p);
         }
     }
-    film.SetBadSampleCount(bad);
+    // Film keeps its bad-sample count private; a non-finite sample only
+    // counts, so adding the new ones brings it to the device's total.
+    FilmSample nan_sample;
+    nan_sample.beauty = Color3f{std::numeric_limits<float>::quiet_NaN(), 0.0f, 0.0f};
+    for (std::uint64_t k = film.BadSampleCount(); k < bad; ++k) film.AddSample(0, 0, nan_sample);
 }
 
 std::vector<yrh_film_pixel> FromFilm(const Film& film) {
@@ -126,11 +99,6 @@ int SamplerCode(RenderSamplerKind k) {
 }
 
 }  // namespace
-
-std::unique_ptr<RenderBackend> CreateRenderBackend(RenderBackendKind kind) {
-    if (kind == RenderBackendKind::Hip) return std::make_unique<HipBackend>(0);
-    return nullptr;
-}
 
 RenderBackendCapabilities HipBackend::Capabilities() const {
     return RenderBackendCapabilities{RenderBackendKind::Hip, true, false, true, true, false};

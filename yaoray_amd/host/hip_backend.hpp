@@ -5,7 +5,21 @@
 // non-positive size, the progress callback runs on the caller's thread after
 // each complete pass and may cancel, a stop token is polled between passes,
 // and a resumed film must carry exactly `resume_completed_spp` samples.
+//
+// Two builds share this file: the standalone library (yaoray_amd/build.py)
+// declares the reference's boundary types itself (yr_ir.hpp and below); the
+// drop-in build inside the reference tree (-DYAORAY_HIP_DROPIN, oracle/Makefile
+// `dropin`) takes them from the reference's own headers instead, and the
+// reference's factory (src/runtime/backend.cpp) gains the Hip case.
 #pragma once
+
+#if defined(YAORAY_HIP_DROPIN)
+
+#include <yaoray/accel/acceleration.hpp>
+#include <yaoray/lighting/light_sampling.hpp>
+#include <yaoray/runtime/backend.hpp>
+
+#else
 
 #include <cstdint>
 #include <functional>
@@ -17,9 +31,15 @@
 
 #include "yr_ir.hpp"
 
-struct yrh_context;
-
 namespace yr {
+
+struct FilmSample {
+    Color3f beauty;
+    Color3f albedo;
+    Vec3f normal;
+    float depth = 0.0f;
+    bool primary_hit = false;
+};
 
 struct FilmPixel {
     Color3f sum;
@@ -45,7 +65,7 @@ public:
     std::uint64_t BadSampleCount() const { return bad_sample_count_; }
     const std::vector<FilmPixel>& Pixels() const { return pixels_; }
     void SetPixelForCheckpoint(int x, int y, FilmPixel pixel);
-    void SetBadSampleCount(std::uint64_t count) { bad_sample_count_ = count; }
+    void AddSample(int x, int y, const FilmSample& sample);
 
 private:
     int width_ = 0;
@@ -145,6 +165,14 @@ public:
 // The factory's new case: RenderBackendKind::Hip -> HipBackend on device 0
 // (src/runtime/backend.cpp:10-18 dispatches the other kinds).
 std::unique_ptr<RenderBackend> CreateRenderBackend(RenderBackendKind kind);
+
+}  // namespace yr
+
+#endif  // YAORAY_HIP_DROPIN
+
+struct yrh_context;
+
+namespace yr {
 
 class HipPreparedScene;
 

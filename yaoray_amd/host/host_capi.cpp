@@ -137,11 +137,11 @@ int yrhost_scene_set_materials(yrhost_scene* s, const yrh_material* m, int32_t c
             case YRH_MATERIAL_THIN_DIELECTRIC: r.kind = yr::RenderMaterialKind::ThinDielectric; break;
             default: return SetError(s, "unknown material kind");
         }
-        r.reflectance = yr::Color3f{m[k].reflectance[0], m[k].reflectance[1], m[k].reflectance[2]};
+        r.reflectance.value = yr::Color3f{m[k].reflectance[0], m[k].reflectance[1], m[k].reflectance[2]};
         r.ior = m[k].ior;
-        r.uroughness = m[k].uroughness;
-        r.vroughness = m[k].vroughness;
-        r.alpha = m[k].alpha;
+        r.uroughness.value = m[k].uroughness;
+        r.vroughness.value = m[k].vroughness;
+        r.alpha.value = m[k].alpha;
         r.emission = yr::Color3f{m[k].emission[0], m[k].emission[1], m[k].emission[2]};
         r.absorption_distance = m[k].absorption_distance;
         r.absorption_color = yr::Color3f{m[k].absorption_color[0], m[k].absorption_color[1], m[k].absorption_color[2]};

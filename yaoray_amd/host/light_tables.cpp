@@ -60,7 +60,7 @@ void PrepareLightSampling(RenderSceneIR& scene) {
     std::vector<float> w;
     w.reserve(scene.emissive_primitives.size());
     for (const EmissivePrimitive& ep : scene.emissive_primitives) w.push_back(EmissiveArea(ep) * Luminance(ep.radiance));
-    scene.emissive_power = BuildAliasTable(w);
+    scene.light_sampling.emissive_power = BuildAliasTable(w);
 }
 
 RenderEnvironmentDistribution BuildEnvironmentDistribution(const RenderTexture& tex) {

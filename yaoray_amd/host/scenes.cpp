@@ -178,7 +178,7 @@ RenderSceneIR MakeCornellScene(std::uint64_t material_seed) {
         const float r = 0.1f + 0.7f * rng.NextFloat();
         const float g = 0.1f + 0.7f * rng.NextFloat();
         const float b = 0.1f + 0.7f * rng.NextFloat();
-        mat.reflectance = Color3f{r, g, b};
+        mat.reflectance.value = Color3f{r, g, b};
         s.materials.push_back(mat);
     }
     const float yaw_tall = -30.0f + 60.0f * rng.NextFloat();
@@ -247,7 +247,7 @@ RenderSceneIR MakeInstancedScene(int instance_count, int subdivisions, int varia
     MeshBuilder mb{s};
     // Primitive 0: ground, grey diffuse.
     RenderMaterial ground;
-    ground.reflectance = Color3f{0.5f, 0.5f, 0.5f};
+    ground.reflectance.value = Color3f{0.5f, 0.5f, 0.5f};
     s.materials.push_back(ground);
     mb.BeginPrimitive(0);
     mb.Quad({-100, 0, 100}, {100, 0, 100}, {100, 0, -100}, {-100, 0, -100});
@@ -257,18 +257,18 @@ RenderSceneIR MakeInstancedScene(int instance_count, int subdivisions, int varia
         const int kind = v % 3;
         if (kind == 0) {
             m.kind = RenderMaterialKind::Diffuse;
-            m.reflectance = Color3f{0.1f + 0.8f * rng.NextFloat(), 0.1f + 0.8f * rng.NextFloat(),
+            m.reflectance.value = Color3f{0.1f + 0.8f * rng.NextFloat(), 0.1f + 0.8f * rng.NextFloat(),
                                     0.1f + 0.8f * rng.NextFloat()};
         } else if (kind == 1) {
             m.kind = RenderMaterialKind::Conductor;
-            m.reflectance = Color3f{0.5f + 0.5f * rng.NextFloat(), 0.5f + 0.5f * rng.NextFloat(),
+            m.reflectance.value = Color3f{0.5f + 0.5f * rng.NextFloat(), 0.5f + 0.5f * rng.NextFloat(),
                                     0.5f + 0.5f * rng.NextFloat()};
             const float alpha = 0.05f + 0.45f * rng.NextFloat();
-            m.uroughness = std::sqrt(alpha);
-            m.vroughness = m.uroughness;
+            m.uroughness.value = std::sqrt(alpha);
+            m.vroughness.value = m.uroughness.value;
         } else {
             m.kind = RenderMaterialKind::Dielectric;
-            m.reflectance = Color3f{1.0f, 1.0f, 1.0f};
+            m.reflectance.value = Color3f{1.0f, 1.0f, 1.0f};
             m.ior = 1.5f;
         }
         s.materials.push_back(m);
@@ -330,7 +330,7 @@ RenderSceneIR MakeGuidingRoomScene(int furniture_triangles, std::uint64_t seed, 
     auto diffuse = [&](Color3f c) {
         RenderMaterial m;
         m.kind = RenderMaterialKind::Diffuse;
-        m.reflectance = c;
+        m.reflectance.value = c;
         s.materials.push_back(m);
         return static_cast<int>(s.materials.size()) - 1;
     };
@@ -339,7 +339,7 @@ RenderSceneIR MakeGuidingRoomScene(int furniture_triangles, std::uint64_t seed, 
     const int ceiling = diffuse({0.8f, 0.8f, 0.8f});
     const int blocker = diffuse({0.6f, 0.6f, 0.6f});
     RenderMaterial lamp;
-    lamp.reflectance = Color3f{0.0f, 0.0f, 0.0f};
+    lamp.reflectance.value = Color3f{0.0f, 0.0f, 0.0f};
     lamp.emission = Color3f{30.0f, 28.0f, 25.0f};
     s.materials.push_back(lamp);
     const int lamp_mat = static_cast<int>(s.materials.size()) - 1;
@@ -352,11 +352,11 @@ RenderSceneIR MakeGuidingRoomScene(int furniture_triangles, std::uint64_t seed, 
         } else {
             RenderMaterial m;
             m.kind = RenderMaterialKind::Conductor;
-            m.reflectance = Color3f{0.5f + 0.5f * rng.NextFloat(), 0.5f + 0.5f * rng.NextFloat(),
+            m.reflectance.value = Color3f{0.5f + 0.5f * rng.NextFloat(), 0.5f + 0.5f * rng.NextFloat(),
                                     0.5f + 0.5f * rng.NextFloat()};
             const float alpha = 0.05f + 0.45f * rng.NextFloat();
-            m.uroughness = std::sqrt(alpha);
-            m.vroughness = m.uroughness;
+            m.uroughness.value = std::sqrt(alpha);
+            m.vroughness.value = m.uroughness.value;
             s.materials.push_back(m);
             furniture.push_back(static_cast<int>(s.materials.size()) - 1);
         }

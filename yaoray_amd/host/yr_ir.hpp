@@ -129,16 +129,30 @@ enum class RenderMaterialKind {
     SubsurfaceExit,
 };
 
-// The in-scope subset of RenderMaterial (material.hpp:26-65): untextured
-// diffuse / conductor / dielectric / thin dielectric with emission and alpha.
+// A material parameter with an optional texture (material.hpp:21-30); this
+// backend reads `value` and rejects texture != -1.
+struct TexParam1f {
+    float value = 0.0f;
+    int texture = -1;
+};
+
+struct TexParam3f {
+    Color3f value{0.0f, 0.0f, 0.0f};
+    int texture = -1;
+};
+
+// The fields of RenderMaterial (material.hpp:32-65) the in-scope kinds read,
+// with the reference's names and types: diffuse / conductor / dielectric /
+// thin dielectric with emission, alpha and interior absorption.
 struct RenderMaterial {
     RenderMaterialKind kind = RenderMaterialKind::Diffuse;
-    Color3f reflectance{0.5f, 0.5f, 0.5f};
+    TexParam3f reflectance{{0.5f, 0.5f, 0.5f}};
     float ior = 1.5f;
-    float uroughness = 0.0f;
-    float vroughness = 0.0f;
+    TexParam1f uroughness{0.0f};
+    TexParam1f vroughness{0.0f};
+    int normal_map = -1;
     Color3f emission{0.0f, 0.0f, 0.0f};
-    float alpha = 1.0f;
+    TexParam1f alpha{1.0f};
     Color3f absorption_color{1.0f, 1.0f, 1.0f};
     float absorption_distance = 1.0f;
 };
@@ -153,6 +167,11 @@ struct AliasTable {
     std::vector<float> probability;
     std::vector<int> alias;
     std::vector<float> pmf;
+};
+
+// LightSamplingCache (light.hpp:61-67): the emissive-power alias table.
+struct LightSamplingCache {
+    AliasTable emissive_power;
 };
 
 struct RenderTexture {
@@ -224,7 +243,7 @@ struct RenderSceneIR {
     std::vector<EmissivePrimitive> emissive_primitives;
     RenderEnvironment environment;
     std::vector<RenderEnvironmentDistribution> environment_distributions;
-    AliasTable emissive_power;  // LightSamplingCache::emissive_power
+    LightSamplingCache light_sampling;
 
     GeometryView Geometry() const { return GeometryView{vertices, indices, primitives, instances}; }
 };
