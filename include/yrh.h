@@ -212,6 +212,18 @@ int yrh_upload_geometry(yrh_context* ctx, const yrh_geometry_desc* geometry);
 int yrh_trace_closest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n,
                       const yrh_hit_soa* hits);
 
+/* yrh_trace_closest without any allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). Same rays, hits
+ * and counters as yrh_trace_closest, bit for bit; rays are claimed in input
+ * order. Its scratch memory (ray-claim counters and the traversal stack spill)
+ * is one 64 MiB device array per GPU, held from library load to process end;
+ * the launch uses fewer workgroups when the tree's spilled stack levels would
+ * not fit, and fails with YRH_E_UNSUPPORTED if one workgroup's would not (a
+ * tree deeper than 32768 + LDS levels). Every call on a GPU shares its array:
+ * calls that overlap on one GPU (two streams or two contexts) race. */
+int yrh_trace_closest_noalloc(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, const yrh_hit_soa* hits);
+
 /* Batched any-hit (occlusion) over n rays: occluded[i] = 1 if any triangle
  * is hit with t_min < t <= t_max, else 0. Equal to (closest hit found) on the
  * same ray; the reference has no any-hit API (its shadow rays are closest-hit

@@ -16,7 +16,8 @@ ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "lib")
 OBJ = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["yrh_context.hip", "yrh_trace.hip", "yrh_render.hip", "yrh_guide.hip", "yrh_order.hip"]
+HIP_SOURCES = ["yrh_context.hip", "yrh_trace.hip", "yrh_render.hip", "yrh_guide.hip", "yrh_order.hip",
+               "yrh_noalloc.hip"]
 HOST_SOURCES = ["transform.cpp", "bvh_builder.cpp", "light_tables.cpp", "scenes.cpp", "abi_marshal.cpp",
                 "hip_backend.cpp", "standalone_boundary.cpp", "host_capi.cpp"]
 

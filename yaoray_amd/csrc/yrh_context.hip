@@ -294,7 +294,8 @@ int yrh_create(int device, yrh_context** out) {
         return YRH_E_HIP;
     }
     ctx->work_counter.bytes = kWorkCounterBytes;
-    if (hipMemset(ctx->work_counter.ptr, 0, kWorkCounterBytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    if (hipMemset(ctx->work_counter.ptr, 0, kWorkCounterBytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        InitNoAllocScratch(ctx) != YRH_OK) {
         ctx->work_counter.reset();
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;

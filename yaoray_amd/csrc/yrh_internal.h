@@ -164,6 +164,9 @@ struct yrh_context {
     bool trace_timing = false;
     bool last_call_ordered = false;
     hipEvent_t phase_events[3] = {nullptr, nullptr, nullptr};
+    // Device address of this GPU's scratch array for yrh_trace_closest_noalloc
+    // (yrh_noalloc.hip), looked up once per device at context creation.
+    void* noalloc_scratch = nullptr;
     yrh::RenderState* render = nullptr;  // surface/light data, film, path state (yrh_render.hip)
     yrh::GuideState* guide = nullptr;    // learned guiding proposal (yrh_guide.hip)
 };
@@ -196,8 +199,17 @@ struct TraceRequest {
     unsigned long long* totals = nullptr;  // [0] node tests, [1] triangle tests
     const int32_t* order = nullptr;        // claim position -> ray index (null = identity)
     bool aux = false;                      // on the context's second stream, with its spill region
+    // Caller-provided stack spill region (never reallocated): the grid is
+    // reduced until the spill of every lane fits in spill_bytes. Null: the
+    // context's region, grown as needed.
+    int2* spill = nullptr;
+    size_t spill_bytes = 0;
 };
 int LaunchTrace(yrh_context* ctx, const TraceRequest& r);
+
+// Looks up this device's scratch array for yrh_trace_closest_noalloc
+// (yrh_noalloc.hip); called by yrh_create with the device current.
+int InitNoAllocScratch(yrh_context* ctx);
 
 // Order of n rays by the Morton code of their origin's cell in the scene box
 // (yrh_order.hip), input order kept within a cell except among one wave's

@@ -53,12 +53,29 @@ int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a
     const int64_t want = std::max<int64_t>(std::min<int64_t>(bound, static_cast<int64_t>(ctx->compute_units) * block),
                                            (bound + 3) / 4);
     if (lanes > want) grid = static_cast<int>(std::max<int64_t>(1, (want + block - 1) / block));
-    const int64_t total = static_cast<int64_t>(grid) * block;
     const int32_t capacity = g.stack_capacity;
     const int64_t spill_levels = std::max<int64_t>(0, capacity - lds_levels);
+    int2* spill_ptr = nullptr;
+    if (r.spill != nullptr) {
+        // A fixed region: fewer workgroups when every lane's spill would not fit
+        // (persistent lanes claim rays until none are left, so any grid is correct).
+        const int64_t per_group = spill_levels * block * static_cast<int64_t>(sizeof(int2));
+        if (per_group > 0) {
+            const int64_t fit = static_cast<int64_t>(r.spill_bytes) / per_group;
+            if (fit < 1) {
+                return Fail(ctx, YRH_E_UNSUPPORTED,
+                            "BVH too deep for the scratch stack region (" + std::to_string(spill_levels) +
+                                " spilled levels x " + std::to_string(block) + " lanes > " +
+                                std::to_string(r.spill_bytes) + " bytes)");
+            }
+            grid = static_cast<int>(std::min<int64_t>(grid, fit));
+        }
+        spill_ptr = r.spill;
+    }
+    const int64_t total = static_cast<int64_t>(grid) * block;
     const size_t spill_bytes = static_cast<size_t>(std::max<int64_t>(1, spill_levels * total)) * sizeof(int2);
     DevBuf& spill = r.aux ? ctx->stack_spill_aux : ctx->stack_spill;
-    if (spill.bytes < spill_bytes) {
+    if (spill_ptr == nullptr && spill.bytes < spill_bytes) {
         // Earlier launches on either stream may still use the old region.
         int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         if (rc == YRH_OK && ctx->aux_stream != nullptr) {
@@ -109,7 +126,7 @@ int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a
     }
     a.order = r.order;
     a.overflow = words + kOverflowWord;
-    a.spill = spill.as<int2>();
+    a.spill = spill_ptr != nullptr ? spill_ptr : spill.as<int2>();
     a.spill_stride = total;
     a.capacity = capacity;
     a.lds_levels = lds_levels;

@@ -246,6 +246,7 @@ YRH_SIGNATURES = {
     "yrh_sample_values": (_I, [_VP, _I32, _U64, _I32, _I64, _VP, _VP, _VP, _VP, _I32, _VP]),
     "yrh_bsdf_query": (_I, [_VP, _VP, _I32, _I64, _VP, _VP, _VP]),
     "yrh_validate_geometry": (_I, [_VP, _VP, _I32]),
+    "yrh_trace_closest_noalloc": (_I, [_VP, C.POINTER(RaySoa), _I64, C.POINTER(HitSoa)]),
     "yrh_generate_camera_rays": (_I, [_VP, C.POINTER(RenderDesc), _VP, _VP]),
     "yrh_batch_render": (_I, [_VP, C.POINTER(BatchDesc), _VP, C.POINTER(_I32)]),
     "yrh_batch_film_read": (_I, [_VP, _I32, _VP, _I64]),
