@@ -77,8 +77,36 @@ def load_counters():
         return None
     with open(path) as f:
         data = json.load(f)
-    keys = ("l2_hit_rate", "l1_hit_rate", "valu_busy", "valu_lane_utilization", "occupancy")
-    return {k: data[k] for k in keys if k in data}
+    keys = ("l2_hit_rate", "l1_hit_rate", "valu_busy", "valu_lane_utilization", "occupancy",
+            "valu_instructions_per_wave", "salu_instructions_per_wave")
+    out = {k: data[k] for k in keys if k in data}
+    if "valu_busy" in out and "valu_lane_utilization" in out:
+        # Issue-bound figure: the fraction of the CU's vector lane-cycles doing work.
+        out["valu_lane_cycles_used"] = round(out["valu_busy"] * out["valu_lane_utilization"], 4)
+    if out.get("valu_instructions_per_wave"):
+        out["salu_per_valu"] = round(out.get("salu_instructions_per_wave", 0) / out["valu_instructions_per_wave"], 3)
+    return out
+
+
+def host_cores():
+    """(threads used, nproc, cores in this process's affinity). The CPU baselines
+    use every core of the affinity mask up to 16: a one-GPU box's share of the
+    host is 16 cores, whatever nproc reports."""
+    aff = len(os.sched_getaffinity(0))
+    return min(16, aff), os.cpu_count(), aff
+
+
+def median_seconds(fn, repeats=5):
+    """One untimed warm-up call, then the median wall time of `repeats` calls
+    (the reference's benchmark convention, README.md:90); returns (median s,
+    all timings, last result)."""
+    fn()
+    times, out = [], None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), [round(t, 4) for t in times], out
 
 
 def timed_passes(dev, launch, steps):
@@ -116,7 +144,7 @@ def main():
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain comparison")
     ap.add_argument("--guide-width", type=int, default=1280)
     ap.add_argument("--guide-height", type=int, default=720)
-    ap.add_argument("--guide-ref-spp", type=int, default=4096)
+    ap.add_argument("--guide-ref-spp", type=int, default=16384)
     ap.add_argument("--guide-spp", default="16,32,64,128,256,512")
     ap.add_argument("--guide-train-passes", type=int, default=8)
     ap.add_argument("--guide-record-spp", type=int, default=2, help="sample indices per recording pass")
@@ -242,7 +270,7 @@ def main():
     for key in [k for k in args.renders.split(",") if k]:
         renders[key] = bench_render(dev, key, args.render_frames, min(16, len(os.sched_getaffinity(0))),
                                     not args.no_cpu_baseline)
-    batch = bench_batch(dev) if not args.no_batch else None
+    batch = bench_batch(dev, not args.no_cpu_baseline, host_cores()[0]) if not args.no_batch else None
     guiding = None
     if not args.no_guiding:
         guiding = bench_guiding(dev, args.guide_width, args.guide_height, args.guide_ref_spp,
@@ -293,10 +321,16 @@ def main():
             "bytes_model": "32 B ray + 16 B hit + 32 B per node_test + 36 B per triangle_test",
             "mean_node_tests": round(float(nt_closest.mean()), 3),
             "mean_triangle_tests": round(float(tt_closest.mean()), 3),
+            "hbm_achieved": round(traffic / (trace_ms / 1e3) / 1e9, 1) if traffic else None,
+            "hbm_frac": round(traffic / (trace_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
             "counters": load_counters(),
+            "binding": "VALU issue: valu_lane_cycles_used = valu_busy x lane utilization is the share of vector "
+                       "lane-cycles doing work; HBM is not the bound (hbm_frac), nor L2 (hit rate)",
             "note": "achieved prices every node and triangle visit as an HBM fetch (32 B / 36 B); L2 and the "
                     "Infinity Cache serve most revisits, so achieved can exceed the HBM peak; traffic is the "
-                    "measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
+                    "measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated), hbm_frac "
+                    "that traffic over the kernel time against the 8 TB/s peak; counters from the committed "
+                    "rocprofv3 summary of the same kernel",
         },
         "cpu_baseline": cpu,
         "any_hit": {
@@ -338,38 +372,33 @@ RENDER_WORKLOADS = {
 ADAPTIVE_BATCH = 16
 
 
-def batch_jobs(scene, count=32, seed=5):
-    """Cameras of the batched workload: pixel counts log-uniform in [128^2, 1920x1080]
-    at 16:9 (clamped), spp log-uniform over powers of two in {4..256}, eyes on a ring
-    25..50 units out looking at points near the centre, fov 30..60 degrees."""
-    rng = np.random.default_rng(seed)
+def batch_jobs(scene, count=32, seed=5, cameras=None):
+    """Cameras of the batched workload (yaoray_amd.api.uneven_cameras): pixel counts
+    log-uniform in [128^2, 1920x1080] at 16:9 (clamped), spp log-uniform over powers
+    of two in {4..256}, eyes on a ring 25..50 units out looking at points near the
+    centre, fov 30..60 degrees; depth 16."""
+    from yaoray_amd.api import uneven_cameras
+
     jobs = []
-    for _ in range(count):
-        n = float(np.exp(rng.uniform(np.log(128 * 128), np.log(1920 * 1080))))
-        w = int(np.clip(round(np.sqrt(n * 16.0 / 9.0)), 128, 1920))
-        h = int(np.clip(round(w * 9.0 / 16.0), 72, 1080))
-        spp = int(2 ** rng.integers(2, 9))
-        az, dist, el = rng.uniform(0, 2 * np.pi), rng.uniform(25, 50), rng.uniform(3, 15)
-        eye = np.array([dist * np.sin(az), el, dist * np.cos(az)])
-        target = np.array([rng.uniform(-5, 5), 1.0, rng.uniform(-5, 5)])
-        fov = rng.uniform(30, 60)
-        f = (target - eye) / np.linalg.norm(target - eye)
-        r = np.cross(f, [0.0, 1.0, 0.0])
-        r /= np.linalg.norm(r)
-        u = np.cross(r, f)
-        scene.set_camera(eye, f, r, u, np.radians(fov))
+    for w, h, spp, cam in uneven_cameras(count, seed):
+        scene.set_camera(*cam)
         jobs.append(scene.render_desc(w, h, spp, 16))
+        if cameras is not None:
+            cameras.append(cam)
     return jobs
 
 
-def bench_batch(dev):
+def bench_batch(dev, with_cpu=True, cpu_threads=16, cpu_scale=8):
     """32 cameras on the two-level scene in one pass schedule with HIP-graph replay;
-    wavefront size and launch shape from the autotuner."""
+    wavefront size and launch shape from the autotuner. CPU baseline: the 32 jobs
+    rendered one after another by the reference's RenderCpuPathTrace, each at
+    1/cpu_scale of its width and height and 1/cpu_scale of its spp."""
     from yaoray_amd import Scene, autotune
 
     scene = Scene.instanced()
     scene.build_accel()
-    jobs = batch_jobs(scene)
+    cameras = []
+    jobs = batch_jobs(scene, cameras=cameras)
     dev.upload_scene(scene)
     dev.upload_shading(scene)
     t = time.perf_counter()
@@ -398,6 +427,38 @@ def bench_batch(dev):
         "autotune_trials": trials,
     }
     log(f"batch: {total_ms:.0f} ms, {out['value']} Mrays/s over {passes} passes")
+    if with_cpu:
+        from oracle import oracle_py as O
+
+        _, nproc, aff = host_cores()
+        rays = samples_cpu = 0
+        secs = 0.0
+        kind = "reference" if O.ref_available() else "port"
+        for j, cam in zip(jobs, cameras):
+            scene.set_camera(*cam)
+            w, h, spp = max(1, j.width // cpu_scale), max(1, j.height // cpu_scale), max(1, j.spp // cpu_scale)
+            if kind == "reference":
+                _, rst, sec = O.ref_render(scene, w, h, spp, max_depth=16, threads=cpu_threads)
+            else:
+                t0 = time.perf_counter()
+                _, rst = O.render(scene, w, h, spp, max_depth=16, threads=cpu_threads)
+                sec = time.perf_counter() - t0
+            rays += rst["rays_traced"]
+            samples_cpu += w * h * spp
+            secs += sec
+        out["cpu_baseline"] = {
+            "value": round(rays / secs / 1e6, 4),
+            "unit": "Mrays/s (path segments, rays_traced)",
+            "cores": cpu_threads,
+            "nproc": nproc,
+            "affinity_cores": aff,
+            "kind": kind,
+            "sample": f"the 32 jobs rendered one after another by RenderCpuPathTrace, each at 1/{cpu_scale} of its "
+                      f"width and height and 1/{cpu_scale} of its spp ({samples_cpu} samples, "
+                      f"{samples_cpu / samples:.4g} of the batch), render seconds summed ({secs:.2f}s)",
+            "batch_seconds_extrapolated": round(secs * samples / samples_cpu, 1),
+        }
+        log(f"batch: cpu {kind} {out['cpu_baseline']['value']} Mrays/s")
     return out
 
 
@@ -406,6 +467,18 @@ def relmse(film, ref):
     x = film["sum"] / np.maximum(film["samples"], 1)[:, None]
     r = ref["sum"] / np.maximum(ref["samples"], 1)[:, None]
     return float(np.mean((x - r) ** 2 / (r ** 2 + 0.01)))
+
+
+def relmse_luminance_corrected(film, ref):
+    """Luminance relMSE with the reference's own noise taken out: with x and r
+    independent, E[(x - r)^2] = E[(x - mu)^2] + Var(r), and Var(r) of each pixel's
+    mean is its Welford M2 / (n - 1) / n from the FP64 film."""
+    lum = np.array([0.2126, 0.7152, 0.0722])
+    x = (film["sum"] / np.maximum(film["samples"], 1)[:, None]) @ lum
+    r = (ref["sum"] / np.maximum(ref["samples"], 1)[:, None]) @ lum
+    n = ref["samples"].astype(np.float64)
+    var_r = ref["luminance_m2"] / np.maximum(n - 1, 1) / np.maximum(n, 1)
+    return float(np.mean(((x - r) ** 2 - var_r) / (r ** 2 + 0.01)))
 
 
 def interp_loglog(xs, ys, x):
@@ -480,7 +553,8 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
                 dev.synchronize()
                 clock += time.perf_counter() - t
                 film = dev.film_read(width, height)
-                points.append({"spp": checkpoint, "seconds": round(clock, 4), "relmse": relmse(film, ref)})
+                points.append({"spp": checkpoint, "seconds": round(clock, 4), "relmse": relmse(film, ref),
+                               "relmse_lum_corrected": relmse_luminance_corrected(film, ref)})
         finally:
             if guided:
                 dev.guide_set_mode(N.GUIDE_OFF)
@@ -496,11 +570,11 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
         p = next(q for q in plain if q["spp"] == g["spp"])
         g["per_sample_improvement"] = round(p["relmse"] / g["relmse"], 3)
 
-    def time_to(points, thr):
+    def time_to(points, thr, key="relmse"):
         """Time at which the measured curve reaches thr (log-log interpolation
         between measured points only; None when thr is outside them)."""
         xs = [p["seconds"] for p in points]
-        ys = [p["relmse"] for p in points]
+        ys = [max(p[key], 1e-12) for p in points]
         for k in range(len(xs) - 1):
             if ys[k] >= thr >= ys[k + 1]:
                 t = (np.log(thr) - np.log(ys[k])) / (np.log(ys[k + 1]) - np.log(ys[k]))
@@ -524,7 +598,14 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
         "time_to_threshold_plain_s": time_to(plain, threshold),
         "time_to_threshold_guided_s": time_to(guided, threshold),
         "time_to_threshold_by_level": {str(t): {"plain_s": time_to(plain, t), "guided_s": time_to(guided, t)}
-                                       for t in sorted({0.2, threshold}, reverse=True)},
+                                       for t in sorted({0.2, threshold, 0.05}, reverse=True)},
+        "relmse_lum_corrected": "luminance relMSE minus the reference's own per-pixel variance of the mean "
+                                "(M2 / (n - 1) / n of its FP64 film) over (r^2 + 0.01): unbiased for the error "
+                                "against the converged image",
+        "time_to_threshold_lum_corrected": {
+            str(t): {"plain_s": time_to(plain, t, "relmse_lum_corrected"),
+                     "guided_s": time_to(guided, t, "relmse_lum_corrected")}
+            for t in sorted({0.2, threshold, 0.05}, reverse=True)},
     }
     tp, tg = out["time_to_threshold_plain_s"], out["time_to_threshold_guided_s"]
     out["time_to_threshold_speedup"] = round(tp / tg, 3) if tp and tg else None
@@ -598,36 +679,66 @@ def bench_render(dev, key, frames, cpu_threads, with_cpu):
         from oracle import oracle_py as O
 
         c = w["cpu"]
+        _, nproc, aff = host_cores()
         if O.ref_available():
             kind = "reference"
-            rfilm, rst, rsec = O.ref_render(scene, c["width"], c["height"], c["spp"], max_depth=w["max_depth"],
+            rsecs = []
+
+            def run():
+                f, st_, sec_ = O.ref_render(scene, c["width"], c["height"], c["spp"], max_depth=w["max_depth"],
                                             threads=cpu_threads, adaptive=adaptive)
+                rsecs.append(sec_)  # RenderCpuPathTrace's own render seconds (prepare excluded)
+                return f, st_
+
+            run()
+            rsecs.clear()
+            results = [run() for _ in range(5)]
+            rsec = float(np.median(rsecs))
+            rfilm, rst = results[-1]
+            runs = [round(x, 4) for x in rsecs]
         else:
             kind = "port"
-            t0 = time.perf_counter()
             if adaptive:
-                rfilm, rst, _, _ = O.render_adaptive(scene, c["width"], c["height"], c["spp"], adaptive,
-                                                     max_depth=w["max_depth"], threads=cpu_threads)
+                rsec, runs, res = median_seconds(lambda: O.render_adaptive(
+                    scene, c["width"], c["height"], c["spp"], adaptive, max_depth=w["max_depth"], threads=cpu_threads))
+                rfilm, rst = res[0], res[1]
             else:
-                rfilm, rst = O.render(scene, c["width"], c["height"], c["spp"], max_depth=w["max_depth"],
-                                      threads=cpu_threads)
-            rsec = time.perf_counter() - t0
+                rsec, runs, (rfilm, rst) = median_seconds(lambda: O.render(
+                    scene, c["width"], c["height"], c["spp"], max_depth=w["max_depth"], threads=cpu_threads))
+        frac = (c["width"] * c["height"] * c["spp"]) / (w["width"] * w["height"] * w["spp"])
         out["cpu_baseline"] = {
             "value": round(rst["rays_traced"] / rsec / 1e6, 4),
             "unit": "Mrays/s (path segments, rays_traced)",
             "cores": cpu_threads,
+            "nproc": nproc,
+            "affinity_cores": aff,
             "kind": kind,
-            "sample": f"{c['width']}x{c['height']} at {c['spp']} spp of the same scene and settings, "
-                      f"RenderCpuPathTrace, {rsec:.2f}s",
+            "sample": f"{c['width']}x{c['height']} at {c['spp']} spp of the same scene and settings "
+                      f"({frac:.4g} of the frame's samples), RenderCpuPathTrace render seconds, warm-up + median "
+                      f"of 5 ({rsec:.3f}s)",
+            "runs_s": runs,
+            "frame_seconds_extrapolated": round(rsec / frac, 2),
         }
         if (c["width"], c["height"], c["spp"]) == (w["width"], w["height"], w["spp"]):
             g = film["sum"] / np.maximum(film["samples"], 1)[:, None]
             r = rfilm["sum"] / np.maximum(rfilm["samples"], 1)[:, None]
+            lum = np.array([0.2126, 0.7152, 0.0722])
+            gl, rl = g @ lum, r @ lum
+            var = rfilm["luminance_m2"] / np.maximum(rfilm["samples"] - 1, 1)
+            noise = float(np.sqrt(np.mean(np.maximum(var, 0) / np.maximum(rfilm["samples"], 1))))
+            rmse = float(np.sqrt(np.mean((gl - rl) ** 2)))
             out["parity"] = {
                 "checked_against": kind,
                 "rel_mean_diff": float(abs(g.mean() - r.mean()) / max(abs(r.mean()), 1e-30)),
+                "per_pixel_rmse_luminance": rmse,
+                "cpu_film_noise_luminance": noise,
+                "rmse_over_noise": round(rmse / max(noise, 1e-30), 4),
                 "pixels_bit_equal_fp32": float(np.mean(np.all(g.astype(np.float32) == r.astype(np.float32), axis=1))),
                 "rays_traced_equal": bool(rst["rays_traced"] == st["rays_traced"]),
+                "rays_traced_rel_diff": round(abs(rst["rays_traced"] - st["rays_traced"]) / max(1, rst["rays_traced"]),
+                                              6),
+                "note": "same sampler streams; paths coincide until a transcendental's last bit differs, so the "
+                        "per-pixel RMSE is a fraction of the noise (statistical parity bound: 0.35)",
             }
             if adaptive:
                 out["parity"]["pixels_with_equal_sample_counts"] = float(np.mean(film["samples"] == rfilm["samples"]))
@@ -641,7 +752,7 @@ def cpu_baseline(scene, org, dirs, hits, nt, tt, stride, n_coherent):
     timed on the host cores; falls back to the restated oracle where _ref is absent."""
     from oracle import oracle_py as O
 
-    threads = min(16, len(os.sched_getaffinity(0)))
+    threads, nproc, aff = host_cores()
     sel = np.arange(0, len(org), stride)
     o, d = np.ascontiguousarray(org[sel]), np.ascontiguousarray(dirs[sel])
     v, i, p = scene.geometry()
@@ -651,17 +762,11 @@ def cpu_baseline(scene, org, dirs, hits, nt, tt, stride, n_coherent):
         ref = O.RefScene(v, i, p)
         log(f"cpu baseline: reference BVH rebuilt in {time.perf_counter() - t:.2f}s; tracing {len(sel)} rays "
             f"on {threads} threads")
-        ref.trace(o[:1024], d[:1024], threads=threads)  # warm-up
-        t0 = time.perf_counter()
-        r = ref.trace(o, d, threads=threads)
-        secs = time.perf_counter() - t0
+        secs, runs, r = median_seconds(lambda: ref.trace(o, d, threads=threads))
     else:
         kind = "port"
         bvh = scene.flat_bvh()
-        O.trace_flat(v, i, p, bvh, o[:1024], d[:1024], threads=threads)
-        t0 = time.perf_counter()
-        r = O.trace_flat(v, i, p, bvh, o, d, threads=threads)
-        secs = time.perf_counter() - t0
+        secs, runs, r = median_seconds(lambda: O.trace_flat(v, i, p, bvh, o, d, threads=threads))
     g_id = hits[sel, 3].view(np.int32)
     g_t = hits[sel, 0]
     id_match = float(np.mean(g_id == r["triangle"]))
@@ -681,9 +786,12 @@ def cpu_baseline(scene, org, dirs, hits, nt, tt, stride, n_coherent):
         "value": round(len(sel) / secs / 1e6, 4),
         "unit": "Mrays/s",
         "cores": threads,
+        "nproc": nproc,
+        "affinity_cores": aff,
         "kind": kind,
         "sample": f"every {stride}th ray of the same workload ({n_coh} coherent + {len(sel) - n_coh} incoherent), "
-                  f"yr::IntersectAcceleration per ray, {secs:.2f}s",
+                  f"yr::IntersectAcceleration per ray; warm-up + median of 5 runs ({secs:.2f}s)",
+        "runs_s": runs,
     }
     log(f"cpu baseline ({kind}, {threads} threads): {cpu['value']} Mrays/s; parity {parity}")
     return cpu, parity

@@ -465,8 +465,8 @@ typedef struct yrh_batch_desc {
 } yrh_batch_desc;
 
 /* Renders every job to its full spp in one pass schedule: each pass carries
- * consecutive pixel ranges / sample layers of the jobs, in order of remaining
- * work, until it is full; a pixel receives its samples in index order, so each
+ * consecutive pixel ranges / sample layers of the jobs, least remaining work
+ * first (small jobs complete early), until it is full; a pixel receives its samples in index order, so each
  * job's FP64 film equals rendering that job alone with yrh_render_pass. Synchronises. job_ms (host, job_count entries,
  * or NULL) receives each job's completion time since the start in ms;
  * passes_out (or NULL) the number of passes. Counters: yrh_render_stats_read. */

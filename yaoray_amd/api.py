@@ -318,6 +318,35 @@ class Scene:
         return self._h
 
 
+def uneven_cameras(count=32, seed=5, min_pixels=128 * 128, max_pixels=1920 * 1080, min_log2_spp=2,
+                   max_log2_spp=8, distance=(25.0, 50.0), target_spread=5.0):
+    """Cameras of uneven size and spp for the batched pipeline: pixel counts
+    log-uniform in [min_pixels, max_pixels] at 16:9 (clamped), spp a power of two
+    2^k with k uniform in [min_log2_spp, max_log2_spp], eyes on a ring `distance`
+    out and 3..15 up, looking at points within target_spread of the centre, fov
+    30..60 degrees. Returns [(width, height, spp, (eye, forward, right, up, fov))]."""
+    rng = np.random.default_rng(seed)
+    max_w = int(round(np.sqrt(max_pixels * 16.0 / 9.0)))
+    min_w = int(round(np.sqrt(min_pixels)))
+    out = []
+    for _ in range(count):
+        n = float(np.exp(rng.uniform(np.log(min_pixels), np.log(max_pixels))))
+        w = int(np.clip(round(np.sqrt(n * 16.0 / 9.0)), min_w, max_w))
+        h = int(np.clip(round(w * 9.0 / 16.0), max(1, (min_w * 9) // 16), (max_w * 9) // 16 + 1))
+        spp = int(2 ** rng.integers(min_log2_spp, max_log2_spp + 1))
+        az, dist, el = rng.uniform(0, 2 * np.pi), rng.uniform(*distance), rng.uniform(3, 15)
+        eye = np.array([dist * np.sin(az), el, dist * np.cos(az)])
+        target = np.array([rng.uniform(-target_spread, target_spread), 1.0,
+                           rng.uniform(-target_spread, target_spread)])
+        fov = rng.uniform(30, 60)
+        f = (target - eye) / np.linalg.norm(target - eye)
+        r = np.cross(f, [0.0, 1.0, 0.0])
+        r /= np.linalg.norm(r)
+        u = np.cross(r, f)
+        out.append((w, h, spp, (eye, f, r, u, np.radians(fov))))
+    return out
+
+
 def random_rays(count: int, seed: int = 3, t_min=1e-5, t_max=float("inf")):
     org = np.empty((count, 4), np.float32)
     dirs = np.empty((count, 4), np.float32)

@@ -2255,10 +2255,13 @@ int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* d, float* job_ms, i
     if (film_pixels > (int64_t{1} << 31) - 1) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "batch film too large");
     const int64_t capacity = d->paths_per_pass > 0 ? d->paths_per_pass : std::min<int64_t>(int64_t{1} << 23, film_pixels);
     if (capacity > (int64_t{1} << 30)) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "paths_per_pass too large");
-    // Schedule: each pass takes, in order of remaining work, consecutive pixel
-    // ranges / sample layers of the jobs until it is full. Items of one job in
-    // a pass are numbered by occurrence; the pass accumulates occurrence by
-    // occurrence, so each pixel sees its samples in index order.
+    // Schedule: each pass takes consecutive pixel ranges / sample layers of the
+    // jobs, the job with the least remaining work first, until it is full:
+    // small cameras complete in the first passes instead of waiting behind the
+    // large ones (every pass but the last is full either way, so the pass count
+    // is the same). Items of one job in a pass are numbered by occurrence; the
+    // pass accumulates occurrence by occurrence, so each pixel sees its samples
+    // in index order.
     constexpr int32_t kMaxOccurrences = 64;
     int32_t max_occurrence = 0;
     std::vector<DevItem> items;
@@ -2273,8 +2276,8 @@ int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* d, float* job_ms, i
         return R.b_job_pixels[k] * (jobs[k].spp - sample[k]) - next_pixel[k];
     };
     while (true) {
-        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return remaining(a) > remaining(b); });
-        if (remaining(order[0]) <= 0) break;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return remaining(a) < remaining(b); });
+        if (remaining(order.back()) <= 0) break;
         int64_t used = 0;
         const int32_t pass = static_cast<int32_t>(pass_items.size()) - 1;
         for (const int32_t j : order) {
