@@ -141,6 +141,8 @@ def main():
                     help="render workloads timed after the ray-cast microbench (comma list, empty for none)")
     ap.add_argument("--render-frames", type=int, default=3)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-camera workload")
+    ap.add_argument("--no-batch-graph", action="store_true",
+                    help="batched passes as plain stream launches instead of HIP-graph replay (same kernels, films)")
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain comparison")
     ap.add_argument("--guide-width", type=int, default=1280)
     ap.add_argument("--guide-height", type=int, default=720)
@@ -270,7 +272,8 @@ def main():
     for key in [k for k in args.renders.split(",") if k]:
         renders[key] = bench_render(dev, key, args.render_frames, min(16, len(os.sched_getaffinity(0))),
                                     not args.no_cpu_baseline)
-    batch = bench_batch(dev, not args.no_cpu_baseline, host_cores()[0]) if not args.no_batch else None
+    batch = (bench_batch(dev, not args.no_cpu_baseline, host_cores()[0], use_graph=not args.no_batch_graph)
+             if not args.no_batch else None)
     guiding = None
     if not args.no_guiding:
         guiding = bench_guiding(dev, args.guide_width, args.guide_height, args.guide_ref_spp,
@@ -388,7 +391,7 @@ def batch_jobs(scene, count=32, seed=5, cameras=None):
     return jobs
 
 
-def bench_batch(dev, with_cpu=True, cpu_threads=16, cpu_scale=8):
+def bench_batch(dev, with_cpu=True, cpu_threads=16, cpu_scale=8, use_graph=True):
     """32 cameras on the two-level scene in one pass schedule with HIP-graph replay;
     wavefront size and launch shape from the autotuner. CPU baseline: the 32 jobs
     rendered one after another by the reference's RenderCpuPathTrace, each at
@@ -406,13 +409,15 @@ def bench_batch(dev, with_cpu=True, cpu_threads=16, cpu_scale=8):
     tune_s = time.perf_counter() - t
     log(f"batch: autotune {tune_s:.1f}s -> {best}")
     films, ms, passes, st = dev.render_batch(jobs, max_depth=16, paths_per_pass=best["paths_per_pass"],
-                                             use_graph=True)
+                                             use_graph=use_graph)
     dev.set_trace_config(-1, 0)  # later sections use the automatic launch shape
     total_ms = float(ms.max())
     samples = int(sum(j.width * j.height * j.spp for j in jobs))
     out = {
         "workload": "32 cameras on the two-level scene (pixels log-uniform 128^2..1920x1080 at 16:9, spp 4..256, "
-                    "seed 5), depth 16, one pass schedule, HIP graph replayed per pass",
+                    "seed 5), depth 16, one pass schedule, "
+                    + ("HIP graph replayed per pass" if use_graph else "passes launched on the stream (no graph)"),
+        "mean_camera_done_ms": round(float(np.mean(ms)), 2),
         "value": round(st["rays_traced"] / (total_ms / 1e3) / 1e6, 2),
         "unit": "Mrays/s (path segments, rays_traced)",
         "shadow_Mrays_s": round(st["shadow_rays"] / (total_ms / 1e3) / 1e6, 2),
