@@ -87,6 +87,19 @@ typedef struct yrh_primitive {
     uint8_t reserved;
 } yrh_primitive;
 
+/* == yr::RenderSphere (include/yaoray/scene/geometry.hpp:29-35), 28 bytes:
+ * world-space centre and radius; flip_normals turns the outward normal
+ * inward. Analytic spheres carry no emission in the reference
+ * (src/integrators/path_integrator.cpp:83-86). */
+typedef struct yrh_sphere {
+    float center[3];
+    float radius;
+    int32_t material_index;
+    int32_t area_light_index;
+    uint8_t flip_normals;
+    uint8_t reserved[3];
+} yrh_sphere;
+
 /* One binary BVH: yr::RenderBvh (include/yaoray/accel/bvh.hpp:72-81). */
 typedef struct yrh_bvh_desc {
     const yrh_bvh_node* nodes;
@@ -131,7 +144,8 @@ typedef struct yrh_geometry_desc {
     int64_t index_count;
     const yrh_primitive* primitives;
     int32_t primitive_count;
-    int32_t sphere_count; /* spheres are not supported by this backend: must be 0 */
+    int32_t sphere_count;
+    const yrh_sphere* spheres; /* flat BVH primitives of kind 1, or TLAS primitives of kind 1 */
 
     int32_t accel_kind; /* YRH_ACCEL_FLAT or YRH_ACCEL_TWO_LEVEL */
     yrh_bvh_desc flat;  /* used when accel_kind == YRH_ACCEL_FLAT */
@@ -155,18 +169,21 @@ typedef struct yrh_ray_soa {
     const float* dir_tmax;
 } yrh_ray_soa;
 
-/* Closest-hit output: hit[i] = (t, bary_u, bary_v, triangle_index bits) as
- * float4; triangle_index < 0 means miss (t is then the ray's t_max, as in
- * yr::BvhHit with hit = false). instance may be NULL; otherwise it receives
- * the instance index of two-level hits (-1 for flat scenes and misses).
- * node_tests / triangle_tests may be NULL; otherwise they receive the
- * per-ray counters with yr::BvhTraceStats semantics
- * (src/accel/bvh_traversal.cpp:272, 302, 318, 324). */
+/* Closest-hit output: hit[i] = (t, bary_u, bary_v, id bits) as float4 with
+ * id = the hit's flat triangle index (>= 0), -1 for a miss (t is then the
+ * ray's t_max, as in yr::BvhHit with hit = false), or -2 - s for a hit on
+ * sphere s, whose (bary_u, bary_v) are then SphereUv of the hit normal
+ * (src/accel/bvh_traversal.cpp:123-150). instance may be NULL; otherwise it
+ * receives the instance index of two-level triangle hits (-1 for flat scenes,
+ * spheres and misses). node_tests / triangle_tests / sphere_tests may be NULL;
+ * otherwise they receive the per-ray counters with yr::BvhTraceStats
+ * semantics (src/accel/bvh_traversal.cpp:272, 302, 305, 318, 324). */
 typedef struct yrh_hit_soa {
     float* hit;
     int32_t* instance;
     uint32_t* node_tests;
     uint32_t* triangle_tests;
+    uint32_t* sphere_tests;
 } yrh_hit_soa;
 
 /* ---- Context -------------------------------------------------------------- */
@@ -190,7 +207,7 @@ int yrh_copy_to_host(yrh_context* ctx, void* dst, const void* src, size_t bytes)
 /* ---- Acceleration structure --------------------------------------------- */
 
 /* The checks yrh_upload_geometry applies before touching the device: array
- * sizes and presence, no spheres, child and leaf ranges of every BVH, and each
+ * sizes and presence, sphere records, child and leaf ranges of every BVH, and each
  * BVH's declared max_depth at least its real depth (root = 1, as
  * yr::RenderBvh::max_depth counts it, src/accel/bvh_builder.cpp:226-244): the
  * traversal stack is sized from it. Needs no device or context. Writes a
@@ -225,7 +242,7 @@ int yrh_trace_closest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n,
 int yrh_trace_closest_noalloc(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, const yrh_hit_soa* hits);
 
 /* Batched any-hit (occlusion) over n rays: occluded[i] = 1 if any triangle
- * is hit with t_min < t <= t_max, else 0. Equal to (closest hit found) on the
+ * or sphere is hit with t_min < t <= t_max, else 0. Equal to (closest hit found) on the
  * same ray; the reference has no any-hit API (its shadow rays are closest-hit
  * queries, src/integrators/path_direct_lighting.cpp:23-113).
  * node_tests / triangle_tests may be NULL. */

@@ -38,11 +38,12 @@ def oracle_lib():
         if not os.path.exists(path):
             raise RuntimeError(f"{path} missing: run oracle.oracle_py.build()")
         lib = C.CDLL(path)
-        lib.oracle_trace_flat.argtypes = [_VP, _I64, _VP, _I64, _VP, _I32, _VP, _VP, _VP, _I64, _I32, _I32,
-                                          _VP, _VP, _VP, _VP]
-        lib.oracle_trace_two_level.argtypes = [_VP, _I64, _VP, _I64, _VP, _I32, _VP, _I32, _VP, _I32, _VP, _VP,
-                                               _I32, _VP, _VP, _I64, _I32, _I32, _VP, _VP, _VP, _VP, _VP]
-        lib.oracle_brute_force.argtypes = [_VP, _I64, _VP, _I64, _VP, _I32, _VP, _VP, _I64, _I32, _VP]
+        lib.oracle_trace_flat.argtypes = [_VP, _I64, _VP, _I64, _VP, _I32, _VP, _I32, _VP, _VP, _VP, _I64, _I32,
+                                          _I32, _VP, _VP, _VP, _VP, _VP]
+        lib.oracle_trace_two_level.argtypes = [_VP, _I64, _VP, _I64, _VP, _I32, _VP, _I32, _VP, _I32, _VP, _I32,
+                                               _VP, _VP, _I32, _VP, _VP, _I64, _I32, _I32, _VP, _VP, _VP, _VP, _VP,
+                                               _VP]
+        lib.oracle_brute_force.argtypes = [_VP, _I64, _VP, _I64, _VP, _I32, _VP, _I32, _VP, _VP, _I64, _I32, _VP]
         _oracle = lib
     return _oracle
 
@@ -66,7 +67,7 @@ def ref_lib():
         lib.ref_bvh_sizes.argtypes = [_VP, _I32] + [C.POINTER(_I32)] * 4
         lib.ref_bvh_copy.argtypes = [_VP, _I32, _VP, _VP, _VP]
         lib.ref_tlas_copy.argtypes = [_VP, _VP, _VP, _VP, _VP]
-        lib.ref_trace.argtypes = [_VP, _VP, _VP, _I64, _I32, _VP, _VP, _VP, _VP]
+        lib.ref_trace.argtypes = [_VP, _VP, _VP, _I64, _I32, _VP, _VP, _VP, _VP, _VP]
         lib.ref_inverse.argtypes = [_VP, _VP]
         lib.ref_inverse.restype = None
         _ref = lib
@@ -86,26 +87,36 @@ def _c(a, dtype=np.float32):
 
 # -- restated oracle ------------------------------------------------------------
 
-def trace_flat(vertices, indices, primitives, bvh, org, dirs, occlusion=False, threads=8):
-    """Returns dict(t, u, v, triangle, node_tests, triangle_tests, occluded)."""
+def _spheres(spheres):
+    if spheres is None or len(spheres) == 0:
+        return None, 0
+    assert spheres.dtype.itemsize == 28 and spheres.flags["C_CONTIGUOUS"]
+    return spheres.ctypes.data, len(spheres)
+
+
+def trace_flat(vertices, indices, primitives, bvh, org, dirs, occlusion=False, threads=8, spheres=None):
+    """Returns dict(t, u, v, triangle, node_tests, triangle_tests, sphere_tests, occluded);
+    `triangle` is the hit id (-1 miss, >= 0 flat triangle, -2 - s sphere s)."""
     org, dirs = _c(org), _c(dirs)
     n = len(org)
     hit = np.zeros((n, 4), np.float32)
     occ = np.zeros(n, np.uint32)
     nt = np.zeros(n, np.uint32)
     tt = np.zeros(n, np.uint32)
+    st = np.zeros(n, np.uint32)
     desc = bvh.desc()
+    sp, ns = _spheres(spheres)
     oracle_lib().oracle_trace_flat(_p(vertices), len(vertices), _p(indices), len(indices), _p(primitives),
-                                   len(primitives), C.cast(C.pointer(desc), C.c_void_p),
+                                   len(primitives), sp, ns, C.cast(C.pointer(desc), C.c_void_p),
                                    org.ctypes.data, dirs.ctypes.data, n, int(occlusion), threads,
-                                   hit.ctypes.data, occ.ctypes.data, nt.ctypes.data, tt.ctypes.data)
+                                   hit.ctypes.data, occ.ctypes.data, nt.ctypes.data, tt.ctypes.data, st.ctypes.data)
     return {"t": hit[:, 0].copy(), "u": hit[:, 1].copy(), "v": hit[:, 2].copy(),
             "triangle": hit[:, 3].copy().view(np.int32), "node_tests": nt, "triangle_tests": tt,
-            "occluded": occ}
+            "sphere_tests": st, "occluded": occ}
 
 
 def trace_two_level(vertices, indices, primitives, blases, instances, tlas, tlas_prims, org, dirs,
-                    occlusion=False, threads=8):
+                    occlusion=False, threads=8, spheres=None):
     from yaoray_amd import native as N
     org, dirs = _c(org), _c(dirs)
     n = len(org)
@@ -114,25 +125,28 @@ def trace_two_level(vertices, indices, primitives, blases, instances, tlas, tlas
     occ = np.zeros(n, np.uint32)
     nt = np.zeros(n, np.uint32)
     tt = np.zeros(n, np.uint32)
+    st = np.zeros(n, np.uint32)
     descs = (N.BvhDesc * max(1, len(blases)))(*[b.desc() for b in blases])
     tdesc = tlas.desc()
+    sp, ns = _spheres(spheres)
     oracle_lib().oracle_trace_two_level(_p(vertices), len(vertices), _p(indices), len(indices), _p(primitives),
-                                        len(primitives), C.cast(descs, C.c_void_p), len(blases), _p(instances),
-                                        len(instances), C.cast(C.pointer(tdesc), C.c_void_p), _p(tlas_prims),
-                                        len(tlas_prims), org.ctypes.data, dirs.ctypes.data, n, int(occlusion),
-                                        threads, hit.ctypes.data, inst.ctypes.data, occ.ctypes.data,
-                                        nt.ctypes.data, tt.ctypes.data)
+                                        len(primitives), sp, ns, C.cast(descs, C.c_void_p), len(blases),
+                                        _p(instances), len(instances), C.cast(C.pointer(tdesc), C.c_void_p),
+                                        _p(tlas_prims), len(tlas_prims), org.ctypes.data, dirs.ctypes.data, n,
+                                        int(occlusion), threads, hit.ctypes.data, inst.ctypes.data,
+                                        occ.ctypes.data, nt.ctypes.data, tt.ctypes.data, st.ctypes.data)
     return {"t": hit[:, 0].copy(), "u": hit[:, 1].copy(), "v": hit[:, 2].copy(),
             "triangle": hit[:, 3].copy().view(np.int32), "instance": inst, "node_tests": nt,
-            "triangle_tests": tt, "occluded": occ}
+            "triangle_tests": tt, "sphere_tests": st, "occluded": occ}
 
 
-def brute_force(vertices, indices, primitives, org, dirs, threads=8):
+def brute_force(vertices, indices, primitives, org, dirs, threads=8, spheres=None):
     org, dirs = _c(org), _c(dirs)
     n = len(org)
     hit = np.zeros((n, 4), np.float32)
+    sp, ns = _spheres(spheres)
     oracle_lib().oracle_brute_force(_p(vertices), len(vertices), _p(indices), len(indices), _p(primitives),
-                                    len(primitives), org.ctypes.data, dirs.ctypes.data, n, threads,
+                                    len(primitives), sp, ns, org.ctypes.data, dirs.ctypes.data, n, threads,
                                     hit.ctypes.data)
     return {"t": hit[:, 0].copy(), "u": hit[:, 1].copy(), "v": hit[:, 2].copy(),
             "triangle": hit[:, 3].copy().view(np.int32)}
@@ -140,13 +154,23 @@ def brute_force(vertices, indices, primitives, org, dirs, threads=8):
 
 # -- the reference itself -----------------------------------------------------------
 
+def _ref_spheres(spheres):
+    lib = ref_lib()
+    lib.ref_set_spheres.argtypes = [_VP, _I32]
+    lib.ref_set_spheres.restype = None
+    sp, n = _spheres(spheres)
+    lib.ref_set_spheres(sp, n)
+
+
 class RefScene:
     """The reference's RenderSceneIR + RenderAcceleration built by its own code."""
 
     def __init__(self, vertices, indices, primitives, instance_primitive=None, instance_matrices=None,
-                 threads=0):
+                 threads=0, spheres=None):
         lib = ref_lib()
         self._lib = lib
+        self._h = None
+        _ref_spheres(spheres)
         ninst = 0 if instance_primitive is None else len(instance_primitive)
         ip = np.ascontiguousarray(instance_primitive if ninst else np.zeros(1), np.int32)
         im = np.ascontiguousarray(instance_matrices if ninst else np.zeros((1, 16)), np.float32)
@@ -154,6 +178,7 @@ class RefScene:
         secs = C.c_double()
         rc = lib.ref_build(_p(vertices), len(vertices), _p(indices), len(indices), _p(primitives), len(primitives),
                            ip.ctypes.data, im.ctypes.data, ninst, threads, C.byref(h), C.byref(secs))
+        _ref_spheres(None)
         if rc != 0:
             raise RuntimeError("reference BuildRenderAcceleration reported errors")
         self._h = h
@@ -199,11 +224,12 @@ class RefScene:
         inst = np.zeros(n, np.int32)
         nt = np.zeros(n, np.uint32)
         tt = np.zeros(n, np.uint32)
+        st = np.zeros(n, np.uint32)
         self._lib.ref_trace(self._h, org.ctypes.data, dirs.ctypes.data, n, threads, hit.ctypes.data,
-                            inst.ctypes.data, nt.ctypes.data, tt.ctypes.data)
+                            inst.ctypes.data, nt.ctypes.data, tt.ctypes.data, st.ctypes.data)
         return {"t": hit[:, 0].copy(), "u": hit[:, 1].copy(), "v": hit[:, 2].copy(),
                 "triangle": hit[:, 3].copy().view(np.int32), "instance": inst, "node_tests": nt,
-                "triangle_tests": tt}
+                "triangle_tests": tt, "sphere_tests": st}
 
 
 STAT_KEYS = ("rays_traced", "shadow_rays", "occluded_shadow_rays", "triangle_tests", "sphere_tests",
@@ -299,9 +325,11 @@ def ref_render(scene, width, height, spp, max_depth=8, sampler=2, seed=0x59414F5
     film = np.zeros(width * height, FILM32_DTYPE)
     stats = np.zeros(10, np.uint64)
     secs = C.c_double()
+    _ref_spheres(scene.spheres())
     rc = lib.ref_render(_p(v), len(v), _p(i), len(i), _p(p), len(p), ip.ctypes.data, im.ctypes.data,
                         info.instance_count, C.cast(C.pointer(sd), _VP), cam.ctypes.data, C.cast(C.pointer(rd), _VP),
                         threads, film.ctypes.data, stats.ctypes.data, C.byref(secs))
+    _ref_spheres(None)
     if rc != 0:
         raise RuntimeError(f"reference render failed ({rc})")
     out = dict(zip(STAT_KEYS, (int(x) for x in stats[:9])))

@@ -268,17 +268,29 @@ struct Surface {
     const yrh_material* mat = nullptr;
     V3 shading;
     int prim = -1;
+    int sphere = -1;
     TriRef tri;
 };
 
 // TraceVisibleSurface without alpha cut-outs (all in-scope materials are opaque
-// to the alpha test).
+// to the alpha test). Sphere hits (src/integrators/surface_query.cpp:129-147):
+// material from the sphere, shading normal = SphereNormal (negated when
+// flip_normals), not face-forwarded.
 Surface TraceSurface(const Scene& s, V3 o, V3 d, float t_min, float t_max, orc::Counters& c) {
     Surface out;
     const Hit h = Trace(s, o, d, t_min, t_max, c);
-    if (!h.hit || h.tri < 0) return out;
+    if (!h.hit || h.tri == -1) return out;
     out.hit = true;
     out.h = h;
+    if (h.tri <= -2) {
+        const yrh_sphere& sp = s.g.sph[-2 - h.tri];
+        const V3 hp = Add(o, Mul(d, h.t));
+        const V3 gn = Normalize(Sub(hp, Load3(sp.center)));
+        out.sphere = -2 - h.tri;
+        out.mat = &s.sd->materials[sp.material_index];
+        out.shading = sp.flip_normals ? Neg(gn) : gn;
+        return out;
+    }
     out.tri = Locate(s, h.tri);
     out.prim = out.tri.prim;
     out.mat = &s.sd->materials[s.g.prim[out.prim].material_index];
@@ -855,6 +867,7 @@ struct Stats {
 void AddTrace(Stats& st, const orc::Counters& c) {
     st.nodes += c.nodes;
     st.tris += c.tris;
+    st.spheres += c.spheres;
 }
 
 bool ShadowVisible(const Scene& s, V3 o, V3 d, float max_dist, Stats& st, V3& T) {
@@ -1011,7 +1024,9 @@ PathOut Integrate(const Scene& s, const yrh_render_desc& rd, V3 o, V3 d, Sampler
             out.normal = n;
             out.depth = sf.h.t;
         }
-        if (!NearBlack(Load3(m.emission))) {
+        // Sphere emission is not supported by the reference (zero contribution,
+        // src/integrators/path_integrator.cpp:83-92).
+        if (!NearBlack(Load3(m.emission)) && sf.sphere < 0) {
             const V3 ln = GeometricNormal(s, sf.tri);
             float w = 1.0f;
             if (prev_valid && !prev_delta) {
@@ -1098,7 +1113,8 @@ namespace {
 
 Scene MakeScene(const yrh_geometry_desc* gd, const yrh_shading_desc* sd) {
     Scene s;
-    s.g = orc::Geometry{gd->vertices, gd->vertex_count, gd->indices, gd->index_count, gd->primitives, gd->primitive_count};
+    s.g = orc::Geometry{gd->vertices, gd->vertex_count, gd->indices, gd->index_count, gd->primitives, gd->primitive_count,
+                        gd->spheres, gd->sphere_count};
     s.gd = gd;
     s.sd = sd;
     s.two_level = gd->accel_kind == YRH_ACCEL_TWO_LEVEL;

@@ -9,6 +9,8 @@
 //                        RenderPrimitive -> indices -> vertices, as the reference)
 //   IntersectBvh         src/accel/bvh_traversal.cpp:253-345
 //   IntersectTwoLevelBvh src/accel/two_level_bvh.cpp:365-481
+//   analytic spheres     src/accel/bvh_traversal.cpp:123-150,
+//                        src/geometry/intersection.cpp:108-151
 // plus a brute-force closest hit over every triangle (same Moller-Trumbore and
 // tie rule) that pins the BVH result independently of any tree, and an
 // early-exit occlusion traversal (same visit order; exits at the first hit).
@@ -59,7 +61,7 @@ Ray LoadRay(const float* org, const float* dir, int64_t i, float& t_min, float& 
 }
 
 void Store(const Hit& h, int64_t i, float* hit, int32_t* inst, uint32_t* occluded, const Counters& c,
-           uint32_t* node_tests, uint32_t* tri_tests) {
+           uint32_t* node_tests, uint32_t* tri_tests, uint32_t* sphere_tests = nullptr) {
     if (hit != nullptr) {
         const int id = h.hit ? h.id : -1;
         float fid;
@@ -73,6 +75,7 @@ void Store(const Hit& h, int64_t i, float* hit, int32_t* inst, uint32_t* occlude
     if (occluded != nullptr) occluded[i] = h.hit ? 1u : 0u;
     if (node_tests != nullptr) node_tests[i] = c.nodes;
     if (tri_tests != nullptr) tri_tests[i] = c.tris;
+    if (sphere_tests != nullptr) sphere_tests[i] = c.spheres;
 }
 
 }  // namespace
@@ -82,36 +85,37 @@ extern "C" {
 
 // Flat closest hit (occlusion = 0) or early-exit occlusion (occlusion = 1).
 int oracle_trace_flat(const yrh_vertex* v, int64_t nv, const uint32_t* idx, int64_t ni, const yrh_primitive* p,
-                      int32_t np, const yrh_bvh_desc* bvh, const float* org_tmin, const float* dir_tmax, int64_t n,
-                      int32_t occlusion, int32_t threads, float* hit, uint32_t* occluded, uint32_t* node_tests,
-                      uint32_t* tri_tests) {
+                      int32_t np, const yrh_sphere* sph, int32_t nsph, const yrh_bvh_desc* bvh, const float* org_tmin,
+                      const float* dir_tmax, int64_t n, int32_t occlusion, int32_t threads, float* hit,
+                      uint32_t* occluded, uint32_t* node_tests, uint32_t* tri_tests, uint32_t* sphere_tests) {
     if (bvh == nullptr || n < 0) return -1;
-    const Geometry g{v, nv, idx, ni, p, np};
+    const Geometry g{v, nv, idx, ni, p, np, sph, nsph};
     ParallelFor(n, threads, [&](int64_t i) {
         float t_min, t_max;
         const Ray r = LoadRay(org_tmin, dir_tmax, i, t_min, t_max);
         Counters c;
         const Hit h = Bvh(g, *bvh, r, t_min, t_max, occlusion != 0, c);
-        Store(h, i, hit, nullptr, occluded, c, node_tests, tri_tests);
+        Store(h, i, hit, nullptr, occluded, c, node_tests, tri_tests, sphere_tests);
     });
     return 0;
 }
 
 int oracle_trace_two_level(const yrh_vertex* v, int64_t nv, const uint32_t* idx, int64_t ni, const yrh_primitive* p,
-                           int32_t np, const yrh_bvh_desc* blases, int32_t nblas, const yrh_instance* inst,
-                           int32_t ninst, const yrh_bvh_desc* tlas, const yrh_tlas_primitive* tprim, int32_t ntprim,
-                           const float* org_tmin, const float* dir_tmax, int64_t n, int32_t occlusion, int32_t threads,
-                           float* hit, int32_t* inst_out, uint32_t* occluded, uint32_t* node_tests,
-                           uint32_t* tri_tests) {
+                           int32_t np, const yrh_sphere* sph, int32_t nsph, const yrh_bvh_desc* blases, int32_t nblas,
+                           const yrh_instance* inst, int32_t ninst, const yrh_bvh_desc* tlas,
+                           const yrh_tlas_primitive* tprim, int32_t ntprim, const float* org_tmin,
+                           const float* dir_tmax, int64_t n, int32_t occlusion, int32_t threads, float* hit,
+                           int32_t* inst_out, uint32_t* occluded, uint32_t* node_tests, uint32_t* tri_tests,
+                           uint32_t* sphere_tests) {
     if (tlas == nullptr || n < 0) return -1;
-    const Geometry g{v, nv, idx, ni, p, np};
+    const Geometry g{v, nv, idx, ni, p, np, sph, nsph};
     const TwoLevel tl{blases, nblas, inst, ninst, tlas, tprim, ntprim};
     ParallelFor(n, threads, [&](int64_t i) {
         float t_min, t_max;
         const Ray r = LoadRay(org_tmin, dir_tmax, i, t_min, t_max);
         Counters c;
         const Hit h = Tlas(g, tl, r, t_min, t_max, occlusion != 0, c);
-        Store(h, i, hit, inst_out, occluded, c, node_tests, tri_tests);
+        Store(h, i, hit, inst_out, occluded, c, node_tests, tri_tests, sphere_tests);
     });
     return 0;
 }
@@ -119,9 +123,9 @@ int oracle_trace_two_level(const yrh_vertex* v, int64_t nv, const uint32_t* idx,
 // Closest hit over every triangle in flat order (no tree): the tie rule makes
 // the result independent of visit order, so it must equal the BVH result.
 int oracle_brute_force(const yrh_vertex* v, int64_t nv, const uint32_t* idx, int64_t ni, const yrh_primitive* p,
-                       int32_t np, const float* org_tmin, const float* dir_tmax, int64_t n, int32_t threads,
-                       float* hit) {
-    const Geometry g{v, nv, idx, ni, p, np};
+                       int32_t np, const yrh_sphere* sph, int32_t nsph, const float* org_tmin, const float* dir_tmax,
+                       int64_t n, int32_t threads, float* hit) {
+    const Geometry g{v, nv, idx, ni, p, np, sph, nsph};
     ParallelFor(n, threads, [&](int64_t i) {
         float t_min, t_max;
         const Ray r = LoadRay(org_tmin, dir_tmax, i, t_min, t_max);
@@ -133,6 +137,10 @@ int oracle_brute_force(const yrh_vertex* v, int64_t nv, const uint32_t* idx, int
             for (int k = 0; k < tris; ++k) Triangle(g, m, k, flat + k, r, t_min, true, best);
             flat += tris;
         }
+        // A sphere takes only a strictly closer hit and a triangle takes an
+        // equal-t hit from a sphere, so the closest hit does not depend on
+        // the visit order except between two spheres at exactly equal t.
+        for (int k = 0; k < nsph; ++k) Sphere(g, k, r, t_min, best);
         Counters c;
         Store(best, i, hit, nullptr, nullptr, c, nullptr, nullptr);
     });

@@ -27,6 +27,8 @@ struct Geometry {
     int64_t ni;
     const yrh_primitive* prim;
     int32_t np;
+    const yrh_sphere* sph = nullptr;
+    int32_t nsph = 0;
 };
 
 struct Ray {
@@ -103,8 +105,45 @@ inline void Triangle(const Geometry& g, int mesh, int local, int flat_id, const 
     best.v = v;
 }
 
+// IntersectAnalyticSphere (src/accel/bvh_traversal.cpp:123-150) over
+// IntersectSphere / SphereNormal / SphereUv (src/geometry/intersection.cpp:108-151):
+// the nearer root in (t_min, best t), else the farther one; a hit records
+// id = -2 - sphere (the device's encoding of a sphere hit) and the uv of the
+// outward normal. A bad handle is skipped.
+inline void Sphere(const Geometry& g, int sphere, const Ray& r, float t_min, Hit& best) {
+    if (sphere < 0 || sphere >= g.nsph) return;
+    const yrh_sphere& s = g.sph[sphere];
+    const V3 c{s.center[0], s.center[1], s.center[2]};
+    const V3 oc = Sub(r.o, c);
+    const float a = Dot(r.d, r.d);
+    const float b = 2.0f * Dot(oc, r.d);
+    const float cc = Dot(oc, oc) - s.radius * s.radius;
+    const float disc = b * b - 4.0f * a * cc;
+    if (disc < 0.0f) return;
+    const float sq = std::sqrt(disc);
+    const float inv_2a = 0.5f / a;
+    float t = (-b - sq) * inv_2a;
+    if (!(t > t_min && t < best.t)) {
+        t = (-b + sq) * inv_2a;
+        if (!(t > t_min && t < best.t)) return;
+    }
+    if (t >= best.t) return;
+    const V3 p{r.o.x + r.d.x * t, r.o.y + r.d.y * t, r.o.z + r.d.z * t};
+    const V3 nv = Sub(p, c);
+    const float len = std::sqrt(Dot(nv, nv));
+    const V3 n = len > 0.0f ? V3{nv.x / len, nv.y / len, nv.z / len} : V3{0.0f, 0.0f, 0.0f};
+    constexpr float kPi = 3.14159265358979323846f;
+    const float y = std::clamp(n.y, -1.0f, 1.0f);
+    best.hit = true;
+    best.t = t;
+    best.id = -2 - sphere;
+    best.u = (std::atan2(n.x, n.z) + kPi) / (2.0f * kPi);
+    best.v = (kPi - std::acos(y)) / kPi;
+    best.inst = -1;
+}
+
 struct Counters {
-    uint32_t nodes = 0, tris = 0;
+    uint32_t nodes = 0, tris = 0, spheres = 0;
 };
 
 // IntersectBvh on one binary BVH. In occlusion mode the traversal stops at the
@@ -132,8 +171,14 @@ inline Hit Bvh(const Geometry& g, const yrh_bvh_desc& b, const Ray& r, float t_m
                 const int pi = b.primitive_indices[i];
                 if (pi < 0 || pi >= b.primitive_count) continue;
                 const yrh_bvh_primitive& bp = b.primitives[pi];
-                ++c.tris;
-                Triangle(g, bp.mesh_primitive, bp.local_triangle, bp.flat_triangle_index, r, t_min, !occlusion, best);
+                if (bp.kind == 0) {
+                    ++c.tris;
+                    Triangle(g, bp.mesh_primitive, bp.local_triangle, bp.flat_triangle_index, r, t_min, !occlusion,
+                             best);
+                } else {
+                    ++c.spheres;
+                    Sphere(g, bp.sphere, r, t_min, best);
+                }
                 if (occlusion && best.hit) return best;
             }
         } else {
@@ -234,7 +279,12 @@ inline Hit Tlas(const Geometry& g, const TwoLevel& tl, const Ray& r, float t_min
             const int pi = t.primitive_indices[i];
             if (pi < 0 || pi >= tl.ntprim) continue;
             const yrh_tlas_primitive& tp = tl.tprim[pi];
-            if (tp.kind != 0) continue;  // spheres are outside this oracle
+            if (tp.kind != 0) {  // IntersectTlasSphere (two_level_bvh.cpp:231-253): world ray, no instance
+                ++c.spheres;
+                Sphere(g, tp.sphere, r, t_min, best);
+                if (occlusion && best.hit) return best;
+                continue;
+            }
             if (tp.instance < 0 || tp.instance >= tl.ninst) continue;
             const yrh_instance& in = tl.inst[tp.instance];
             if (in.blas_index < 0 || in.blas_index >= tl.nblas) continue;

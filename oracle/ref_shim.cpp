@@ -55,6 +55,23 @@ yr::RenderMaterial ToRefMaterial(const yrh_material& m) {
 extern "C" {
 #pragma GCC visibility push(default)
 
+// Spheres the next ref_build / ref_render calls add to their scene (yrh_sphere
+// records, converted field by field into yr::RenderSphere); n = 0 clears them.
+std::vector<yr::RenderSphere> g_spheres;
+
+void ref_set_spheres(const yrh_sphere* sp, int32_t n) {
+    g_spheres.clear();
+    for (int32_t k = 0; k < n; ++k) {
+        yr::RenderSphere r;
+        r.center = yr::Point3f{sp[k].center[0], sp[k].center[1], sp[k].center[2]};
+        r.radius = sp[k].radius;
+        r.material_index = sp[k].material_index;
+        r.area_light_index = sp[k].area_light_index;
+        r.flip_normals = sp[k].flip_normals != 0;
+        g_spheres.push_back(r);
+    }
+}
+
 // vertices: 48-byte RenderVertex records; primitives: 16-byte RenderPrimitive.
 int ref_build(const void* vertices, int64_t nv, const uint32_t* indices, int64_t ni, const void* primitives,
               int32_t np, const int32_t* inst_prim, const float* inst_mat, int32_t ninst, int32_t threads,
@@ -73,6 +90,7 @@ int ref_build(const void* vertices, int64_t nv, const uint32_t* indices, int64_t
         std::memcpy(inst.object_to_world.m.data(), inst_mat + 16 * static_cast<size_t>(i), 64);
         s->scene.instances.push_back(inst);
     }
+    s->scene.spheres = g_spheres;
     s->scene.materials.push_back(yr::RenderMaterial{});
     yr::BvhBuildOptions opt;
     opt.thread_count = threads;
@@ -169,9 +187,10 @@ int ref_tlas_copy(void* h, void* nodes, int32_t* prim_indices, int32_t* prim_ins
 }
 
 // yr::IntersectAcceleration per ray over `threads` threads.
-// hit: float4 (t, u, v, triangle_index bits); inst / counters may be null.
+// hit: float4 (t, u, v, id bits) with id = triangle_index, or -2 - sphere for a
+// sphere hit, or -1 for a miss; inst / counters may be null.
 int ref_trace(void* h, const float* org_tmin, const float* dir_tmax, int64_t n, int32_t threads, float* hit,
-              int32_t* inst, uint32_t* node_tests, uint32_t* tri_tests) {
+              int32_t* inst, uint32_t* node_tests, uint32_t* tri_tests, uint32_t* sphere_tests) {
     const auto* s = static_cast<RefScene*>(h);
     const yr::GeometryView g = s->scene.Geometry();
     std::atomic_int64_t next{0};
@@ -187,7 +206,8 @@ int ref_trace(void* h, const float* org_tmin, const float* dir_tmax, int64_t n, 
                 const yr::BvhHit hh = yr::IntersectAcceleration(g, s->accel, ray, st, org_tmin[4 * i + 3],
                                                                 dir_tmax[4 * i + 3]);
                 if (hit != nullptr) {
-                    const int32_t id = hh.hit ? hh.triangle_index : -1;
+                    const int32_t id =
+                        !hh.hit ? -1 : (hh.sphere.IsValid() ? -2 - hh.sphere.Value() : hh.triangle_index);
                     float fid;
                     std::memcpy(&fid, &id, 4);
                     hit[4 * i] = hh.t;
@@ -198,6 +218,7 @@ int ref_trace(void* h, const float* org_tmin, const float* dir_tmax, int64_t n, 
                 if (inst != nullptr) inst[i] = hh.hit ? hh.instance.Value() : -1;
                 if (node_tests != nullptr) node_tests[i] = static_cast<uint32_t>(st.node_tests);
                 if (tri_tests != nullptr) tri_tests[i] = static_cast<uint32_t>(st.triangle_tests);
+                if (sphere_tests != nullptr) sphere_tests[i] = static_cast<uint32_t>(st.sphere_tests);
             }
         }
     };
@@ -249,6 +270,7 @@ int ref_render(const yrh_vertex* vertices, int64_t nv, const uint32_t* indices, 
         std::memcpy(inst.object_to_world.m.data(), inst_mat + 16 * static_cast<size_t>(i), 64);
         sc.instances.push_back(inst);
     }
+    sc.spheres = g_spheres;
     for (int32_t k = 0; k < sd->material_count; ++k) sc.materials.push_back(ToRefMaterial(sd->materials[k]));
     for (int32_t k = 0; k < sd->emissive_count; ++k) {
         const yrh_emissive& e = sd->emissives[k];

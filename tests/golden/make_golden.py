@@ -15,7 +15,9 @@ src/accel/two_level_bvh.cpp:365-481), yr::Sampler bits
 (src/sampling/sampler.cpp) and FP32 films + stats of RenderCpuPathTrace
 (src/backends/cpu/cpu_path_tracer.cpp), plain and with adaptive sampling
 (its active-pixel loop and Film::IsConverged, src/film/film.cpp:113-121),
-and BSDF eval / pdf / sample tuples (src/shading/bsdf.cpp:17-329).
+BSDF eval / pdf / sample tuples (src/shading/bsdf.cpp:17-329), and hits,
+counters and a film of scenes with analytic spheres
+(src/geometry/intersection.cpp:108-151, src/integrators/surface_query.cpp:129-147).
 """
 import hashlib
 import os
@@ -187,7 +189,49 @@ def zoo_film_fixture():
             "stats": np.array([st[k] for k in keys], np.uint64), "seed": np.uint64(SEED)}
 
 
-FIXTURES = {"sampler": sampler_fixture, "bsdf": bsdf_fixture, "zoo_film": zoo_film_fixture, "flat_trace": flat_fixture, "two_level_trace": two_level_fixture,
+def spheres_fixture():
+    """Analytic spheres (yr::RenderSphere): the reference's per-ray hits (t,
+    SphereUv, id = -2 - sphere) and node / triangle / sphere counters on the
+    sphere Cornell box (flat BVH: spheres in leaves beside triangles) and the
+    sphere zoo (spheres in the TLAS), and its FP32 film + statistics of the
+    sphere Cornell box (48 x 36, 8 spp, depth 8)."""
+    from tests.golden_data import scene_digest
+    from tests.helpers import sphere_cornell, sphere_zoo
+    out = {}
+    c = sphere_cornell()
+    v, i, p = c.geometry()
+    sp = c.spheres()
+    o1, d1 = c.camera_rays(64, 48)
+    g = np.random.default_rng(206)
+    o2 = np.zeros((2048, 4), np.float32)
+    o2[:, :3] = g.uniform((-1, 0, -1), (1, 2, 1), (2048, 3))
+    o2[:, 3] = 1e-5
+    dd = g.normal(size=(2048, 3))
+    d2 = np.zeros((2048, 4), np.float32)
+    d2[:, :3] = dd / np.linalg.norm(dd, axis=1, keepdims=True)
+    d2[:, 3] = np.inf
+    org, dirs = np.concatenate([o1, o2]), np.concatenate([d1, d2])
+    r = O.RefScene(v, i, p, spheres=sp).trace(org, dirs)
+    out.update({"flat_" + k: val for k, val in trace_outputs(r).items()})
+    out.update({"flat_sphere_tests": r["sphere_tests"], "flat_org": org, "flat_dir": dirs,
+                "flat_digest": np.array(scene_digest(c))})
+    film, st, _ = O.ref_render(c, 48, 36, 8, max_depth=8, sampler=2, seed=SEED)
+    out.update({"film": film.view(np.uint8).reshape(-1), "stats": np.array([st[k] for k in O.STAT_KEYS], np.uint64),
+                "width": np.int32(48), "height": np.int32(36), "spp": np.int32(8), "depth": np.int32(8),
+                "seed": np.uint64(SEED)})
+    z = sphere_zoo(60)
+    v, i, p = z.geometry()
+    _, _, inst = z.tlas()
+    ref = O.RefScene(v, i, p, inst["primitive"].copy(), inst["object_to_world"].copy(), spheres=z.spheres())
+    org, dirs = z.camera_rays(80, 45)
+    r = ref.trace(org, dirs)
+    out.update({"tl_" + k: val for k, val in trace_outputs(r).items()})
+    out.update({"tl_sphere_tests": r["sphere_tests"], "tl_org": org, "tl_dir": dirs,
+                "tl_digest": np.array(scene_digest(z))})
+    return out
+
+
+FIXTURES = {"spheres": spheres_fixture, "sampler": sampler_fixture, "bsdf": bsdf_fixture, "zoo_film": zoo_film_fixture, "flat_trace": flat_fixture, "two_level_trace": two_level_fixture,
             "cornell_film": cornell_fixture, "cornell_adaptive": cornell_adaptive_fixture}
 
 

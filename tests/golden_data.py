@@ -19,11 +19,14 @@ def load(name):
         return {k: f[k] for k in f.files}
 
 
-def expected_hits(f):
-    h = f["hit"]
-    return {"t": h[:, 0].copy(), "u": h[:, 1].copy(), "v": h[:, 2].copy(),
-            "triangle": h[:, 3].copy().view(np.int32), "instance": f["instance"],
-            "node_tests": f["node_tests"], "triangle_tests": f["triangle_tests"]}
+def expected_hits(f, prefix=""):
+    h = f[prefix + "hit"]
+    out = {"t": h[:, 0].copy(), "u": h[:, 1].copy(), "v": h[:, 2].copy(),
+           "triangle": h[:, 3].copy().view(np.int32), "instance": f[prefix + "instance"],
+           "node_tests": f[prefix + "node_tests"], "triangle_tests": f[prefix + "triangle_tests"]}
+    if prefix + "sphere_tests" in f:
+        out["sphere_tests"] = f[prefix + "sphere_tests"]
+    return out
 
 
 def flat_scene(f):
@@ -76,13 +79,18 @@ def bsdf_materials(f):
 
 
 def scene_digest(scene):
-    """Digest of geometry, camera, instances and materials (the zoo fixture's key)."""
+    """Digest of geometry, camera, instances, spheres and materials (the key of the
+    fixtures of procedural scenes)."""
     import ctypes
 
     from yaoray_amd import native as N
     h = hashlib.sha256(geometry_digest(scene).encode())
-    _, _, inst = scene.tlas()
-    h.update(np.ascontiguousarray(inst).tobytes())
+    if scene.info.accel_kind == 1:
+        _, _, inst = scene.tlas()
+        h.update(np.ascontiguousarray(inst).tobytes())
+    sph = scene.spheres()
+    if len(sph):
+        h.update(sph.tobytes())
     sd = scene.shading_desc()
     h.update(ctypes.string_at(sd.materials, sd.material_count * ctypes.sizeof(N.Material)))
     return h.hexdigest()
@@ -93,3 +101,12 @@ def zoo_scene(f):
     s = material_zoo_scene()
     assert scene_digest(s) == str(f["digest"]), "material-zoo inputs differ from the fixture's"
     return s
+
+
+def sphere_scenes(f):
+    """The sphere Cornell box and the sphere zoo of the spheres fixture."""
+    from tests.helpers import sphere_cornell, sphere_zoo
+    c, z = sphere_cornell(), sphere_zoo(60)
+    assert scene_digest(c) == str(f["flat_digest"]), "sphere Cornell inputs differ from the fixture's"
+    assert scene_digest(z) == str(f["tl_digest"]), "sphere zoo inputs differ from the fixture's"
+    return c, z

@@ -176,3 +176,47 @@ def material_zoo_scene(instances=150, subdivisions=2, env_width=64, env_height=3
     s.set_materials(ZOO_MATERIALS)
     s.build_accel()
     return s
+
+
+def cornell_materials(scene):
+    """The scene's materials as set_materials dicts."""
+    import ctypes as C
+
+    from yaoray_amd import native as N
+    sd = scene.shading_desc()
+    arr = (N.Material * sd.material_count).from_address(sd.materials)
+    out = []
+    for m in arr:
+        out.append({f: (list(getattr(m, f)) if isinstance(getattr(m, f), C.Array) else getattr(m, f))
+                    for f, _ in N.Material._fields_ if f != "reserved"})
+    return out
+
+
+# Spheres over every sphere path: absorbing glass (a closed medium), mirror, rough conductor, diffuse,
+# one with flipped normals and one carrying the emissive material (a sphere's
+# emission is not added by the reference).
+def sphere_cornell():
+    """Cornell box (flat BVH) plus analytic spheres; materials 8.. are added."""
+    s = Scene.cornell(1)
+    mats = cornell_materials(s) + [
+        {"kind": 2, "reflectance": (1.0, 1.0, 1.0), "ior": 1.5, "absorption_color": (0.7, 0.9, 0.8),
+         "absorption_distance": 0.6},
+        {"kind": 1, "reflectance": (0.95, 0.95, 0.95), "uroughness": 0.0},
+        {"kind": 1, "reflectance": (0.9, 0.7, 0.4), "uroughness": 0.4},
+    ]
+    s.set_materials(mats)
+    s.set_spheres([((-0.45, 0.4, 0.2), 0.4, 8), ((0.5, 0.35, -0.3), 0.35, 9), ((0.45, 1.2, 0.4), 0.2, 10),
+                   ((-0.3, 1.4, -0.4), 0.25, 1, True), ((0.0, 0.15, 0.6), 0.15, 2), ((0.6, 1.7, -0.6), 0.12, 7)])
+    s.build_accel()
+    return s
+
+
+def sphere_zoo(instances=150):
+    """The two-level material-zoo scene plus spheres in front of its camera (TLAS spheres)."""
+    s = Scene.instanced(instances=instances, subdivisions=2, variants=8, env_width=64, env_height=32, seed=4)
+    s.set_materials(ZOO_MATERIALS)
+    s.set_spheres([((-4.0, 2.0, 20.0), 2.0, 7), ((3.0, 1.5, 25.0), 1.5, 3), ((0.0, 3.0, 10.0), 3.0, 0),
+                   ((6.0, 1.0, 30.0), 1.0, 1, True), ((-6.0, 1.5, 18.0), 1.2, 2), ((5.0, 1.2, 16.0), 1.2, 5),
+                   ((-1.0, 1.0, 22.0), 0.9, 4)])
+    s.build_accel()
+    return s

@@ -39,3 +39,11 @@ def test_probe_hip_without_gpu(tmp_path):
     r, res = D.render("hip_room.pbrt", "hip", tmp_path, "4")
     assert r.returncode == 0 and not res["ok"]
     assert res["error"] == "Render backend preparation failed: no HIP device 0"
+
+
+def test_probe_reference_cpu_render_of_spheres(tmp_path):
+    """Shape "sphere" compiles to yr::RenderSphere (centre and radius from the
+    transform) and the reference's CPU backend renders it: sphere tests counted."""
+    r, res = D.render("hip_spheres.pbrt", "cpu", tmp_path, "2")
+    assert r.returncode == 0 and res["ok"], r.stderr
+    assert res["stats"]["sphere_tests"] > 0 and (res["film"]["samples"] == 2).all()

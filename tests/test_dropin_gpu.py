@@ -1,8 +1,8 @@
 """The drop-in build on the GPU: the reference's own PBRT frontend, factory,
 film and checkpoint code driving HipBackend.
 
-* HIP vs the reference's CPU backend on the same PBRT scene (every in-scope
-  material kind as triangle meshes): image mean within 0.5 %, per-pixel RMSE
+* HIP vs the reference's CPU backend on the same PBRT scenes (every in-scope
+  material kind as triangle meshes; analytic spheres from Shape "sphere"): image mean within 0.5 %, per-pixel RMSE
   below 0.35 x the CPU film's standard error, counters within 1 %, sample
   counts equal (same sampler streams; beyond the first bounce the device's
   transcendentals differ from glibc in the last bits).
@@ -31,9 +31,10 @@ def lum_mean(film):
     return (film["sum"].reshape(-1, 3) / np.maximum(film["samples"].reshape(-1), 1)[:, None]) @ LUM
 
 
-def test_hip_matches_reference_cpu_backend(tmp_path):
-    rc, cpu = D.render("hip_room.pbrt", "cpu", tmp_path)
-    rh, hip = D.render("hip_room.pbrt", "hip", tmp_path)
+@pytest.mark.parametrize("scene", ["hip_room.pbrt", "hip_spheres.pbrt"])
+def test_hip_matches_reference_cpu_backend(tmp_path, scene):
+    rc, cpu = D.render(scene, "cpu", tmp_path)
+    rh, hip = D.render(scene, "hip", tmp_path)
     assert cpu["ok"] and hip["ok"], (rc.stderr, rh.stderr, hip and hip["error"])
     assert np.array_equal(hip["film"]["samples"], cpu["film"]["samples"])
     g, c = lum_mean(hip["film"]), lum_mean(cpu["film"])
@@ -42,9 +43,9 @@ def test_hip_matches_reference_cpu_backend(tmp_path):
     noise = float(np.sqrt(np.mean(np.maximum(var, 0) / np.maximum(f["samples"], 1))))
     rel_mean = abs(g.mean() - c.mean()) / c.mean()
     rmse = float(np.sqrt(np.mean((g - c) ** 2)))
-    counts = {k: abs(hip["stats"][k] - cpu["stats"][k]) / max(1, cpu["stats"][k])
-              for k in ("rays_traced", "hits", "misses", "shadow_rays")}
-    print(f"drop-in room: rel mean {rel_mean:.5f} rmse {rmse:.5f} noise {noise:.5f} counts {counts}", flush=True)
+    keys = ("rays_traced", "hits", "misses", "shadow_rays") + (("sphere_tests",) if "spheres" in scene else ())
+    counts = {k: abs(hip["stats"][k] - cpu["stats"][k]) / max(1, cpu["stats"][k]) for k in keys}
+    print(f"drop-in {scene}: rel mean {rel_mean:.5f} rmse {rmse:.5f} noise {noise:.5f} counts {counts}", flush=True)
     assert rel_mean < 0.005 and rmse < 0.35 * noise and max(counts.values()) < 0.01
 
 

@@ -6,7 +6,10 @@ machine.
 * yr::Sampler values, all four kinds, three sample counts, 1D and 2D;
 * SAH node arrays of the host builder, byte for byte (flat and TLAS);
 * per-ray hits and node / triangle counters of the oracle, bit for bit;
-* the Cornell workload's FP32 film and statistics from the oracle, bit for bit.
+* the Cornell workload's FP32 film and statistics from the oracle, bit for bit;
+* analytic spheres: per-ray hits (t, SphereUv, id) and node / triangle /
+  sphere counters of the oracle on flat and TLAS spheres, and the sphere
+  Cornell box's FP32 film and statistics, bit for bit.
 """
 import numpy as np
 import pytest
@@ -115,5 +118,28 @@ def test_oracle_render_matches_zoo_fixture():
     s = G.zoo_scene(f)
     w, h = int(f["width"]), int(f["height"])
     film, st = O.render(s, w, h, int(f["spp"]), max_depth=int(f["depth"]), sampler=2, seed=int(f["seed"]))
+    assert film.tobytes() == G.film(f["film"]).tobytes()
+    assert st == G.stats(f["stats"])
+
+
+def test_oracle_matches_spheres_fixture():
+    f = G.load("spheres")
+    c, z = G.sphere_scenes(f)
+    v, i, p = c.geometry()
+    o = O.trace_flat(v, i, p, c.flat_bvh(), f["flat_org"], f["flat_dir"], spheres=c.spheres())
+    exp = G.expected_hits(f, "flat_")
+    for k in TRACE_KEYS + ("sphere_tests",):
+        assert same_bits(o[k], exp[k]), k
+    assert (exp["triangle"] <= -2).sum() > 500
+    v, i, p = z.geometry()
+    tl, tp, inst = z.tlas()
+    o = O.trace_two_level(v, i, p, [z.blas(k) for k in range(z.info.blas_count)], inst, tl, tp, f["tl_org"],
+                          f["tl_dir"], spheres=z.spheres())
+    exp = G.expected_hits(f, "tl_")
+    for k in TRACE_KEYS + ("instance", "sphere_tests"):
+        assert same_bits(o[k], exp[k]), k
+    assert (exp["triangle"] <= -2).sum() > 300
+    w, h = int(f["width"]), int(f["height"])
+    film, st = O.render(c, w, h, int(f["spp"]), max_depth=int(f["depth"]), sampler=2, seed=int(f["seed"]))
     assert film.tobytes() == G.film(f["film"]).tobytes()
     assert st == G.stats(f["stats"])

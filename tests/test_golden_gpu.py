@@ -11,7 +11,11 @@ written from oracle/_ref by tests/golden/make_golden.py), through the C ABI.
   transcendentals differ from glibc in the last bits, so the bar is the
   statistical one of the oracle comparison — mean luminance within 0.5 %,
   per-pixel RMSE below 0.35 x the reference's per-pixel standard error,
-  ray / hit / shadow counters within 1 %.
+  ray / hit / shadow counters within 1 %;
+* analytic spheres (flat leaves and TLAS): hit ids, t, instance and node /
+  triangle / sphere counters bit-exact, barycentrics of triangle hits
+  bit-exact, a sphere hit's SphereUv within 4e-6 (the device's atan2 / acos of
+  the same bits); the sphere Cornell film statistically, as above.
 """
 import numpy as np
 import pytest
@@ -91,3 +95,36 @@ def test_device_film_matches_fixture_statistically(device):
     assert rel_mean < 0.005
     assert rmse < 0.35 * noise
     assert max(counts.values()) < 0.01
+
+
+def film_parity(g, st, ref, rs):
+    gl = (g["sum"] / np.maximum(g["samples"], 1)[:, None]) @ LUM
+    rl = (ref["sum"] / np.maximum(ref["samples"], 1)[:, None]) @ LUM
+    var = ref["luminance_m2"] / np.maximum(ref["samples"] - 1, 1)
+    stderr = np.sqrt(np.maximum(var, 0) / np.maximum(ref["samples"], 1))
+    rel_mean = abs(gl.mean() - rl.mean()) / rl.mean()
+    rmse = float(np.sqrt(np.mean((gl - rl) ** 2)))
+    noise = float(np.sqrt(np.mean(stderr ** 2)))
+    counts = {k: abs(st[k] - rs[k]) / max(1, rs[k]) for k in ("rays_traced", "hits", "misses", "shadow_rays")}
+    return rel_mean, rmse, noise, counts
+
+
+def test_device_matches_spheres_fixture(device):
+    f = G.load("spheres")
+    c, z = G.sphere_scenes(f)
+    for scene, prefix in ((c, "flat_"), (z, "tl_")):
+        print(f"spheres fixture {prefix}: {len(f[prefix + 'org'])} rays", flush=True)
+        device.upload_scene(scene)
+        g = device.trace_closest(f[prefix + "org"], f[prefix + "dir"], counters=True)
+        exp = G.expected_hits(f, prefix)
+        check_trace(g, exp, ("t", "triangle", "instance", "node_tests", "triangle_tests", "sphere_tests"))
+        tri, sph = exp["triangle"] >= 0, exp["triangle"] <= -2
+        for k in ("u", "v"):
+            assert same_bits(g[k][tri], exp[k][tri]), k
+            assert np.abs(g[k][sph].astype(np.float64) - exp[k][sph]).max() <= 4e-6, k
+    w, h = int(f["width"]), int(f["height"])
+    print(f"spheres fixture film {w}x{h}x{int(f['spp'])}", flush=True)
+    g, st = device.render(c, w, h, int(f["spp"]), max_depth=int(f["depth"]), sampler=2, seed=int(f["seed"]))
+    rel_mean, rmse, noise, counts = film_parity(g, st, G.film(f["film"]), G.stats(f["stats"]))
+    print(f"spheres fixture film: rel mean {rel_mean:.5f} rmse {rmse:.5f} noise {noise:.5f} counts {counts}")
+    assert rel_mean < 0.005 and rmse < 0.35 * noise and max(counts.values()) < 0.01

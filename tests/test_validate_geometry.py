@@ -64,5 +64,6 @@ def test_malformed_trees_are_rejected():
     rc, msg = validate_geometry(g)
     assert rc == N.E_INVALID_ARGUMENT and "child" in msg
     g = s.geometry_desc()
-    g.sphere_count = 1
-    assert validate_geometry(g)[0] == N.E_UNSUPPORTED
+    g.sphere_count = 1  # a count with no sphere array
+    rc, msg = validate_geometry(g)
+    assert rc == N.E_INVALID_ARGUMENT and "sphere" in msg

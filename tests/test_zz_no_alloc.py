@@ -2,14 +2,15 @@
 against yrh_trace_closest through the C ABI: hits (t, u, v, triangle index),
 instance indices and node / triangle counters bit for bit, on two calls in a
 row (the second a random subset of the first's rays, so it reuses the
-scratch array the first left behind), for a flat tree, a two-level scene and a tree deep enough to spill
-its traversal stack into the scratch array."""
+scratch array the first left behind), for a flat tree, a two-level scene, a two-level scene with analytic
+spheres in its TLAS and a tree deep enough to spill its traversal stack into
+the scratch array."""
 import ctypes as C
 
 import numpy as np
 import pytest
 
-from tests.helpers import MeshArrays, rays
+from tests.helpers import MeshArrays, rays, sphere_zoo
 from yaoray_amd import Scene, random_rays
 from yaoray_amd import native as N
 
@@ -58,6 +59,12 @@ def test_noalloc_two_level(device):
     s.build_accel()
     o1, d1 = s.camera_rays(320, 180)
     check_two_calls(device, s, o1, d1, 777)
+
+
+def test_noalloc_spheres(device):
+    s = sphere_zoo(150)
+    o1, d1 = s.camera_rays(200, 120)
+    check_two_calls(device, s, o1, d1, 1500)
 
 
 def test_noalloc_deep_tree_spills_into_scratch(device):

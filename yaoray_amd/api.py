@@ -28,6 +28,8 @@ TLAS_PRIM_DTYPE = np.dtype([("kind", "<i4"), ("instance", "<i4"), ("sphere", "<i
                             ("world_bounds_min", "<f4", 3), ("world_bounds_max", "<f4", 3)])
 
 # yrh_film_pixel (FP64 sums and Welford moments) and yr::FilmPixel (FP32, 56 bytes).
+SPHERE_DTYPE = np.dtype([("center", "<f4", 3), ("radius", "<f4"), ("material_index", "<i4"),
+                         ("area_light_index", "<i4"), ("flip_normals", "u1"), ("reserved", "u1", 3)])
 FILM64_DTYPE = np.dtype([("sum", "<f8", 3), ("luminance_mean", "<f8"), ("luminance_m2", "<f8"),
                          ("albedo_sum", "<f8", 3), ("normal_sum", "<f8", 3), ("depth_sum", "<f8"),
                          ("samples", "<u4"), ("aov_samples", "<u4")])
@@ -240,6 +242,27 @@ class Scene:
         arr = material_array(materials)
         if self._lib.yrhost_scene_set_materials(self._h, C.cast(arr, C.c_void_p), len(materials)) != 0:
             raise YrhError("set_materials failed")
+
+    def set_spheres(self, spheres):
+        """spheres: list of (center (3), radius, material_index[, flip_normals]) (yr::RenderSphere);
+        the acceleration structure must be built again."""
+        arr = (N.Sphere * max(1, len(spheres)))()
+        for k, sp in enumerate(spheres):
+            arr[k].center[:] = [float(c) for c in sp[0]]
+            arr[k].radius = float(sp[1])
+            arr[k].material_index = int(sp[2])
+            arr[k].area_light_index = -1
+            arr[k].flip_normals = 1 if (len(sp) > 3 and sp[3]) else 0
+        if self._lib.yrhost_scene_set_spheres(self._h, C.cast(arr, C.c_void_p), len(spheres)) != 0:
+            raise YrhError("set_spheres failed")
+
+    def spheres(self) -> np.ndarray:
+        """The scene's spheres as a SPHERE_DTYPE array (a copy)."""
+        p, n = C.c_void_p(), C.c_int32()
+        self._lib.yrhost_scene_spheres(self._h, C.byref(p), C.byref(n))
+        if n.value == 0:
+            return np.zeros(0, SPHERE_DTYPE)
+        return _view(p.value, SPHERE_DTYPE, n.value)
 
     def set_emissives(self, emissives):
         """emissives: list of (primitive_index, (r, g, b), area)."""
@@ -488,10 +511,10 @@ class Device:
     # -- device-resident launches ------------------------------------------
     def trace_closest_device(self, org: DeviceBuffer, dirs: DeviceBuffer, n: int, hit: DeviceBuffer,
                              instance: DeviceBuffer | None = None, node_tests: DeviceBuffer | None = None,
-                             tri_tests: DeviceBuffer | None = None):
+                             tri_tests: DeviceBuffer | None = None, sphere_tests: DeviceBuffer | None = None):
         rays = N.RaySoa(org.ptr, dirs.ptr)
         hits = N.HitSoa(hit.ptr, instance.ptr if instance else None, node_tests.ptr if node_tests else None,
-                        tri_tests.ptr if tri_tests else None)
+                        tri_tests.ptr if tri_tests else None, sphere_tests.ptr if sphere_tests else None)
         self._check(self._lib.yrh_trace_closest(self._ctx, C.byref(rays), n, C.byref(hits)), "trace_closest")
 
     def trace_any_device(self, org: DeviceBuffer, dirs: DeviceBuffer, n: int, occluded: DeviceBuffer,
@@ -717,14 +740,16 @@ class Device:
 
     # -- host convenience ---------------------------------------------------
     def trace_closest(self, org: np.ndarray, dirs: np.ndarray, counters: bool = False):
-        """Returns dict(t, u, v, triangle, instance[, node_tests, triangle_tests]) as numpy arrays."""
+        """Returns dict(t, u, v, triangle, instance[, node_tests, triangle_tests, sphere_tests]) as
+        numpy arrays; `triangle` is the hit id (-1 miss, >= 0 flat triangle index, -2 - s sphere s)."""
         n = len(org)
         d_org, d_dir = self.to_device(org.astype(np.float32)), self.to_device(dirs.astype(np.float32))
         d_hit = self.alloc(16 * n)
         d_inst = self.alloc(4 * n)
         d_nt = self.alloc(4 * n) if counters else None
         d_tt = self.alloc(4 * n) if counters else None
-        self.trace_closest_device(d_org, d_dir, n, d_hit, d_inst, d_nt, d_tt)
+        d_st = self.alloc(4 * n) if counters else None
+        self.trace_closest_device(d_org, d_dir, n, d_hit, d_inst, d_nt, d_tt, d_st)
         self.synchronize()
         hit = d_hit.download(np.float32, (n, 4))
         out = {"t": hit[:, 0].copy(), "u": hit[:, 1].copy(), "v": hit[:, 2].copy(),
@@ -732,6 +757,7 @@ class Device:
         if counters:
             out["node_tests"] = d_nt.download(np.uint32, (n,))
             out["triangle_tests"] = d_tt.download(np.uint32, (n,))
+            out["sphere_tests"] = d_st.download(np.uint32, (n,))
         return out
 
     def trace_any(self, org: np.ndarray, dirs: np.ndarray, counters: bool = False):

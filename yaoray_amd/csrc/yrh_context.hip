@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <new>
 
 namespace yrh {
@@ -113,9 +114,9 @@ int ValidateGeometry(const yrh_geometry_desc* g, std::string& error) {
         error = "null argument";
         return YRH_E_INVALID_ARGUMENT;
     }
-    if (g->sphere_count != 0) {
-        error = "sphere primitives are not supported by the HIP backend";
-        return YRH_E_UNSUPPORTED;
+    if (g->sphere_count < 0 || (g->sphere_count > 0 && g->spheres == nullptr)) {
+        error = "sphere array missing";
+        return YRH_E_INVALID_ARGUMENT;
     }
     if (g->vertex_count < 0 || g->index_count < 0 || g->primitive_count < 0) {
         error = "negative geometry array size";
@@ -217,8 +218,25 @@ bool PackBvh(const yrh_bvh_desc& bvh, PackedBvh& out, std::string& error, SlotFn
 struct TriangleSlotWriter {
     const yrh_geometry_desc* g;
     bool operator()(const yrh_bvh_primitive& prim, float4* dst, std::string& error) const {
+        if (prim.kind == 1) {
+            // Sphere slot: (centre, -2 - sphere) (radius, 0, 0, -1) (0, 0, 0, -1).
+            // A bad handle (skipped by the reference, bvh_traversal.cpp:130-133)
+            // gets a NaN centre: counted, never hit.
+            const bool ok = prim.sphere >= 0 && prim.sphere < g->sphere_count;
+            const float nan = std::numeric_limits<float>::quiet_NaN();
+            const yrh_sphere* sp = ok ? &g->spheres[prim.sphere] : nullptr;
+            const int32_t id = -2 - (ok ? prim.sphere : 0);
+            float fid, fneg;
+            const int32_t neg = -1;
+            std::memcpy(&fid, &id, 4);
+            std::memcpy(&fneg, &neg, 4);
+            dst[0] = ok ? make_float4(sp->center[0], sp->center[1], sp->center[2], fid) : make_float4(nan, nan, nan, fid);
+            dst[1] = make_float4(ok ? sp->radius : nan, 0.0f, 0.0f, fneg);
+            dst[2] = make_float4(0.0f, 0.0f, 0.0f, fneg);
+            return true;
+        }
         if (prim.kind != 0) {
-            error = "sphere primitives are not supported by the HIP backend";
+            error = "unknown BVH primitive kind";
             return false;
         }
         float p[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -523,12 +541,20 @@ int yrh_upload_geometry(yrh_context* ctx, const yrh_geometry_desc* g) {
         tdesc.primitives = nullptr;
         tdesc.primitive_count = g->tlas_primitive_count;
         // Reuse PackBvh for nodes/codes with a slot writer that records instances.
+        // Per TLAS slot: an instance index (>= 0) or -2 - sphere index.
         std::vector<int32_t> resolved(static_cast<size_t>(std::max(0, t.primitive_index_count)), -1);
         for (int32_t s = 0; s < t.primitive_index_count; ++s) {
             const int32_t p = t.primitive_indices[s];
             if (p < 0 || p >= g->tlas_primitive_count) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "TLAS primitive index out of bounds");
             const yrh_tlas_primitive& tp = g->tlas_primitives[p];
-            if (tp.kind != 0) return Fail(ctx, YRH_E_UNSUPPORTED, "sphere primitives are not supported by the HIP backend");
+            if (tp.kind == 1) {
+                if (tp.sphere < 0 || tp.sphere >= g->sphere_count) {
+                    return Fail(ctx, YRH_E_INVALID_ARGUMENT, "TLAS sphere out of bounds");
+                }
+                resolved[static_cast<size_t>(s)] = -2 - tp.sphere;
+                continue;
+            }
+            if (tp.kind != 0) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "unknown TLAS primitive kind");
             if (tp.instance < 0 || tp.instance >= g->instance_count) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "TLAS instance out of bounds");
             const int32_t bi = g->instances[tp.instance].blas_index;
             if (bi < 0 || bi >= g->blas_count) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "instance BLAS out of bounds");
@@ -545,6 +571,17 @@ int yrh_upload_geometry(yrh_context* ctx, const yrh_geometry_desc* g) {
         std::vector<DevInstanceSlot> slots(resolved.size());
         for (size_t s = 0; s < resolved.size(); ++s) {
             const int32_t i = resolved[s];
+            if (i <= -2) {
+                // TLAS sphere slot: r0 = (centre, radius), hi.w = -2 - sphere.
+                const yrh_sphere& sp = g->spheres[-2 - i];
+                float fi;
+                std::memcpy(&fi, &i, 4);
+                DevInstanceSlot& d = slots[s];
+                d.r0 = make_float4(sp.center[0], sp.center[1], sp.center[2], sp.radius);
+                d.r1 = d.r2 = d.lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                d.hi = make_float4(0.0f, 0.0f, 0.0f, fi);
+                continue;
+            }
             const float* m = g->instances[i].world_to_object;
             const DevBlasRoot& root = roots[static_cast<size_t>(g->instances[i].blas_index)];
             DevInstanceSlot& d = slots[s];

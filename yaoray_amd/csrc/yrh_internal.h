@@ -195,8 +195,9 @@ struct TraceRequest {
     int32_t* instance = nullptr;
     uint32_t* node_tests = nullptr;
     uint32_t* triangle_tests = nullptr;
+    uint32_t* sphere_tests = nullptr;
     uint32_t* occluded = nullptr;
-    unsigned long long* totals = nullptr;  // [0] node tests, [1] triangle tests
+    unsigned long long* totals = nullptr;  // [0] node tests, [1] triangle tests, [2] sphere tests
     const int32_t* order = nullptr;        // claim position -> ray index (null = identity)
     bool aux = false;                      // on the context's second stream, with its spill region
     // Caller-provided stack spill region (never reallocated): the grid is

@@ -81,6 +81,7 @@ int yrh_trace_closest_noalloc(yrh_context* ctx, const yrh_ray_soa* rays, int64_t
     r.instance = hits->instance;
     r.node_tests = hits->node_tests;
     r.triangle_tests = hits->triangle_tests;
+    r.sphere_tests = hits->sphere_tests;
     r.work_counter = counters;
     r.parts = trav::kParts;
     r.spill = reinterpret_cast<int2*>(base + kCounterBytes);

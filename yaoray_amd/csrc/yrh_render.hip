@@ -57,8 +57,9 @@ enum Stat : int {
     kStatHits,
     kStatMisses,
     kStatBad,
-    kStatTraceNodes,  // scratch totals written by the trace kernels: [node, tri]
+    kStatTraceNodes,  // scratch totals written by the trace kernels: [node, tri, sphere]
     kStatTraceTris,
+    kStatTraceSpheres,
     kStatWords,
 };
 // The statistics live in kStatSlots copies kStatStride words (128 B) apart;
@@ -78,6 +79,8 @@ struct RenderState {
     DevBuf normals;      // float4 per vertex
     DevBuf indices;      // uint32
     DevBuf tri_prim;     // int32 per flat triangle: mesh primitive
+    DevBuf spheres;      // float4 per sphere: centre, radius
+    DevBuf sphere_info;  // int2 per sphere: material index, flip_normals
     DevBuf prim_info;    // int4 per primitive: material, emissive index, first index, flat offset
     DevBuf prim_flags;   // int32 per primitive: bit0 has_normals
     DevBuf inst_normal;  // float4[6] per instance: normal-matrix rows (+inv det, singular), o2w rows
@@ -89,6 +92,7 @@ struct RenderState {
     std::vector<float> host_positions;     // xyz per vertex (emitter tables)
     std::vector<uint32_t> host_indices;
     std::vector<int32_t> host_prim_material;
+    std::vector<int32_t> host_sphere_material;
     // ---- shading ----
     bool shading_loaded = false;
     bool transparent = false;  // any dielectric / thin dielectric material
@@ -178,6 +182,8 @@ struct SurfaceView {
     const int4* prim_info;
     const int32_t* prim_flags;
     const float4* inst_normal;
+    const float4* spheres;     // centre, radius
+    const int2* sphere_info;   // material index, flip_normals
 };
 
 struct LightView {
@@ -706,8 +712,10 @@ __global__ void CopyTraceTotalsKernel(unsigned long long* stats) {
     if (threadIdx.x == 0) {
         stats[kStatNodeTests] += stats[kStatTraceNodes];
         stats[kStatTriTests] += stats[kStatTraceTris];
+        stats[kStatSphereTests] += stats[kStatTraceSpheres];
         stats[kStatTraceNodes] = 0;
         stats[kStatTraceTris] = 0;
+        stats[kStatTraceSpheres] = 0;
     }
 }
 
@@ -723,6 +731,23 @@ struct SurfacePoint {
 __device__ __forceinline__ SurfacePoint ResolveSurface(const SurfaceView& S, const LightView& L, float3 org,
                                                        float3 dir, float4 h, int tri, int inst) {
     const float t = h.x;
+    if (trav::IsSphereId(tri)) {
+        // Sphere hit (surface_query.cpp:129-146): outward normal at ray.At(t)
+        // (flipped for flip_normals) is both the geometric and the shading
+        // normal, not face-forwarded; spheres carry no emission index.
+        const int k = -2 - tri;
+        const float4 sph = S.spheres[k];
+        const int2 info = S.sphere_info[k];
+        SurfacePoint out;
+        out.hp = Add(org, Mul(dir, t));
+        out.wo = Neg(dir);
+        const float3 n = Normalize(Sub(out.hp, XYZ(sph)));
+        out.sn = info.y ? Neg(n) : n;
+        out.obj_geo = out.sn;
+        out.pinfo = make_int4(info.x, -1, 0, 0);
+        out.mat = LoadMaterial(L.materials, info.x);
+        return out;
+    }
     const int prim = S.tri_prim[tri];
     const int4 pinfo = S.prim_info[prim];
     const DevMaterial mat = LoadMaterial(L.materials, pinfo.x);
@@ -843,7 +868,7 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
     }
     smp.bounce = depth;
 
-    if (tri < 0) {
+    if (tri == -1) {
         ++miss_count;
         float w = 1.0f;
         if ((flags & kPrevValid) && !(flags & kPrevDelta)) w = PowerHeuristic(1, prev.w, 1, EnvPdf(L, dir));
@@ -882,8 +907,8 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
         P.p_nrm[p] = F4(sn, 0.0f);
     }
     // Emission x MIS (path_integrator.cpp:86-93; light normal in object space
-    // as GeometricNormal returns it).
-    if (!IsBlack(mat.emission)) {
+    // as GeometricNormal returns it). Sphere hits emit nothing (:83-86).
+    if (!IsBlack(mat.emission) && tri >= 0) {
         float w = 1.0f;
         if ((flags & kPrevValid) && !(flags & kPrevDelta)) {
             const float pdf_light = EmissivePdfSolidAngle(L, XYZ(prev), hp, obj_geo, pinfo.y);
@@ -1095,7 +1120,7 @@ __global__ __launch_bounds__(kQueryBlock) void GuideQueryKernel(PassView P, Surf
     if (i < n) {
         const float4 h = P.hits[i];
         const int tri = __float_as_int(h.w);
-        if (tri >= 0) {
+        if (tri != -1) {
             const int inst = P.hit_inst != nullptr ? P.hit_inst[i] : -1;
             const SurfacePoint sp = ResolveSurface(S, L, XYZ(P.q_org[cur][i]), XYZ(P.q_dir[cur][i]), h, tri, inst);
             guidable = Guidable(sp.mat);
@@ -1289,7 +1314,7 @@ struct ShadowWalk {
         P->s_trans[e] = make_float4(1.0f, 1.0f, 1.0f, d.w);
     }
 
-    __device__ __forceinline__ bool Finish(int32_t e, const trav::Best& best, int32_t, uint32_t, uint32_t,
+    __device__ __forceinline__ bool Finish(int32_t e, const trav::Best& best, int32_t, uint32_t, uint32_t, uint32_t,
                                            const float4& lo, const float4& ld, float4& next_org,
                                            float4& next_dir) const {
         int4 st = P->l_state[e];
@@ -1316,8 +1341,9 @@ struct ShadowWalk {
                 T = Hadamard(T, BeerLambert(XYZ(med), med.w, best.t));
                 if (TransBlack(T)) ok = false;
             }
-            const int prim = S->tri_prim[best.id];
-            const DevMaterial mat = L->materials[S->prim_info[prim].x];
+            const int mat_index = trav::IsSphereId(best.id) ? S->sphere_info[-2 - best.id].x
+                                                             : S->prim_info[S->tri_prim[best.id]].x;
+            const DevMaterial mat = L->materials[mat_index];
             if (ok && mat.kind != kDielectric && mat.kind != kThinDielectric) ok = false;
             if (ok) {
                 if (mat.kind == kThinDielectric) {
@@ -1587,6 +1613,8 @@ SurfaceView MakeSurfaceView(RenderState& R) {
     S.normals = R.normals.as<float4>();
     S.indices = R.indices.as<uint32_t>();
     S.tri_prim = R.tri_prim.as<int32_t>();
+    S.spheres = R.spheres.as<float4>();
+    S.sphere_info = R.sphere_info.as<int2>();
     S.prim_info = R.prim_info.as<int4>();
     S.prim_flags = R.prim_flags.as<int32_t>();
     S.inst_normal = R.inst_normal.as<float4>();
@@ -1904,6 +1932,17 @@ int UploadSurfaceGeometry(yrh_context* ctx, const yrh_geometry_desc* g) {
     if ((rc = UploadVec(ctx, R.prim_info, info.data(), info.size())) != YRH_OK) return rc;
     if ((rc = UploadVec(ctx, R.prim_flags, flags.data(), flags.size())) != YRH_OK) return rc;
     if ((rc = UploadVec(ctx, R.inst_normal, inst.data(), inst.size())) != YRH_OK) return rc;
+    std::vector<float4> sph(static_cast<size_t>(g->sphere_count));
+    std::vector<int2> sph_info(static_cast<size_t>(g->sphere_count));
+    R.host_sphere_material.assign(static_cast<size_t>(g->sphere_count), 0);
+    for (int32_t k = 0; k < g->sphere_count; ++k) {
+        const yrh_sphere& sp = g->spheres[k];
+        sph[static_cast<size_t>(k)] = make_float4(sp.center[0], sp.center[1], sp.center[2], sp.radius);
+        sph_info[static_cast<size_t>(k)] = make_int2(sp.material_index, sp.flip_normals ? 1 : 0);
+        R.host_sphere_material[static_cast<size_t>(k)] = sp.material_index;
+    }
+    if ((rc = UploadVec(ctx, R.spheres, sph.data(), sph.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.sphere_info, sph_info.data(), sph_info.size())) != YRH_OK) return rc;
     return YRH_OK;
 }
 
@@ -1925,6 +1964,9 @@ int yrh_upload_shading(yrh_context* ctx, const yrh_shading_desc* sd) {
     if (sd->material_count <= 0 || sd->materials == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "no materials");
     for (int32_t m : R.host_prim_material) {
         if (m < 0 || m >= sd->material_count) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "primitive material index out of range");
+    }
+    for (int32_t m : R.host_sphere_material) {
+        if (m < 0 || m >= sd->material_count) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "sphere material index out of range");
     }
     (void)hipSetDevice(ctx->device);
     int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
@@ -2162,7 +2204,7 @@ int yrh_render_stats_read(yrh_context* ctx, yrh_render_stats* stats) {
     stats->shadow_rays = w[kStatShadow];
     stats->occluded_shadow_rays = w[kStatOccluded];
     stats->triangle_tests = w[kStatTriTests] + w[kStatTraceTris];
-    stats->sphere_tests = w[kStatSphereTests];
+    stats->sphere_tests = w[kStatSphereTests] + w[kStatTraceSpheres];
     stats->bvh_node_tests = w[kStatNodeTests] + w[kStatTraceNodes];
     stats->hits = w[kStatHits];
     stats->misses = w[kStatMisses];

@@ -97,6 +97,7 @@ int PrepareTraceArgs(yrh_context* ctx, const TraceRequest& r, trav::TraceArgs& a
     a.inst_out = r.instance;
     a.node_tests = r.node_tests;
     a.triangle_tests = r.triangle_tests;
+    a.sphere_tests = r.sphere_tests;
     a.occluded = r.occluded;
     a.nodes = g.nodes.as<DevNode>();
     a.slots = g.slots.as<float4>();
@@ -196,6 +197,7 @@ int LaunchTraceClosest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, con
     r.instance = hits->instance;
     r.node_tests = hits->node_tests;
     r.triangle_tests = hits->triangle_tests;
+    r.sphere_tests = hits->sphere_tests;
     return LaunchRayCast(ctx, r);
 }
 

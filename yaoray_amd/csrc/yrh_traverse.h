@@ -48,12 +48,13 @@ struct TraceArgs {
     const float4* dir;
     int32_t n;
     const uint32_t* n_device;    // when set, the ray count is read from device memory
-    unsigned long long* totals;  // optional [0] node tests, [1] triangle tests of the launch
+    unsigned long long* totals;  // optional [0] node, [1] triangle, [2] sphere tests of the launch
     // outputs
     float4* hit;
     int32_t* inst_out;
     uint32_t* node_tests;
     uint32_t* triangle_tests;
+    uint32_t* sphere_tests;
     uint32_t* occluded;
     // flat BVH or concatenated BLASes
     const DevNode* nodes;
@@ -162,6 +163,61 @@ __device__ __forceinline__ bool TriangleTest(const float4& s0, const float4& s1,
     best.id = id;
     best.u = u;
     best.v = v;
+    return true;
+}
+
+// Sphere slot of a leaf (device layout): (centre, id bits) (radius, 0, 0, -1)
+// (0, 0, 0, -1) with id = -2 - sphere index; triangle slots carry their flat
+// triangle index (>= 0) in the same word.
+__device__ __forceinline__ bool IsSphereId(int32_t id) { return id <= -2; }
+
+// SphereUv (src/geometry/intersection.cpp:136-151) of the outward normal at
+// ray.At(t): the (u, v) the reference stores in a sphere hit.
+__device__ __forceinline__ void SphereUv(float cx, float cy, float cz, float ox, float oy, float oz, float dx, float dy,
+                                         float dz, float t, float& u, float& v) {
+    const float px = ox + dx * t, py = oy + dy * t, pz = oz + dz * t;
+    const float nx = px - cx, ny = py - cy, nz = pz - cz;
+    const float len = sqrtf(nx * nx + ny * ny + nz * nz);
+    float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
+    if (len > 0.0f) {
+        n0 = nx / len;
+        n1 = ny / len;
+        n2 = nz / len;
+    }
+    constexpr float kPi = 3.14159265358979323846f;
+    const float y = fminf(fmaxf(n1, -1.0f), 1.0f);
+    u = (atan2f(n0, n2) + kPi) / (2.0f * kPi);
+    v = (kPi - acosf(y)) / kPi;
+}
+
+// IntersectAnalyticSphere (src/accel/bvh_traversal.cpp:123-150) over
+// IntersectSphere (src/geometry/intersection.cpp:108-128), same operations in
+// the same order: the nearer root if it lies in (t_min, best t), else the
+// farther one; a hit is strictly closer than `best` and clears the triangle
+// id. (u, v) = SphereUv when `with_uv` (ray casts report it; renders
+// recompute the normal). Returns true when `best` changed.
+__device__ __forceinline__ bool SphereTest(float cx, float cy, float cz, float radius, int32_t id, float ox, float oy,
+                                           float oz, float dx, float dy, float dz, float t_min, bool with_uv,
+                                           Best& best) {
+    const float ocx = ox - cx, ocy = oy - cy, ocz = oz - cz;
+    const float a = dx * dx + dy * dy + dz * dz;
+    const float b = 2.0f * (ocx * dx + ocy * dy + ocz * dz);
+    const float c = (ocx * ocx + ocy * ocy + ocz * ocz) - radius * radius;
+    const float disc = b * b - 4.0f * a * c;
+    if (disc < 0.0f) return false;
+    const float sq = sqrtf(disc);
+    const float inv_2a = 0.5f / a;
+    float t = (-b - sq) * inv_2a;
+    if (!(t > t_min && t < best.t)) {
+        t = (-b + sq) * inv_2a;
+        if (!(t > t_min && t < best.t)) return false;
+    }
+    best.hit = true;
+    best.t = t;
+    best.id = id;
+    best.u = 0.0f;
+    best.v = 0.0f;
+    if (with_uv) SphereUv(cx, cy, cz, ox, oy, oz, dx, dy, dz, t, best.u, best.v);
     return true;
 }
 
@@ -286,15 +342,19 @@ struct Fetch {
 
 __device__ __forceinline__ uint64_t LanesBelow(int lane) { return (1ull << lane) - 1ull; }
 
-// Wave-level sum of the per-lane test counts, one atomic per wave.
-__device__ __forceinline__ void AddTotals(unsigned long long* totals, uint64_t nt, uint64_t tt, int lane) {
+// Wave-level sum of the per-lane test counts (node, triangle, sphere), one
+// atomic per wave and count.
+__device__ __forceinline__ void AddTotals(unsigned long long* totals, uint64_t nt, uint64_t tt, uint64_t ts,
+                                          int lane) {
     for (int off = 32; off > 0; off >>= 1) {
         nt += __shfl_xor(static_cast<unsigned long long>(nt), off);
         tt += __shfl_xor(static_cast<unsigned long long>(tt), off);
+        ts += __shfl_xor(static_cast<unsigned long long>(ts), off);
     }
     if (lane == 0) {
         atomicAdd(totals, static_cast<unsigned long long>(nt));
         atomicAdd(totals + 1, static_cast<unsigned long long>(tt));
+        if (ts != 0) atomicAdd(totals + 2, static_cast<unsigned long long>(ts));
     }
 }
 
@@ -377,7 +437,7 @@ struct RayArrayPolicy {
         d = a->dir[ray];
     }
     __device__ __forceinline__ bool Finish(int32_t ray, const Best& best, int32_t inst, uint32_t nt, uint32_t tt,
-                                           const float4&, const float4&, float4&, float4&) const {
+                                           uint32_t ts, const float4&, const float4&, float4&, float4&) const {
         if (kAny) {
             a->occluded[ray] = best.hit ? 1u : 0u;
         } else {
@@ -386,6 +446,7 @@ struct RayArrayPolicy {
         }
         if (a->node_tests != nullptr) a->node_tests[ray] = nt;
         if (a->triangle_tests != nullptr) a->triangle_tests[ray] = tt;
+        if (a->sphere_tests != nullptr) a->sphere_tests[ray] = ts;
         return false;
     }
 };
@@ -403,14 +464,14 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
     const int32_t n = a.n_device != nullptr ? static_cast<int32_t>(*a.n_device) : a.n;
     // Claim range (low byte) and ranges left to try (next byte), wave-uniform.
     int part_state = static_cast<int>(blockIdx.x % kParts) | (kParts << 8);
-    uint32_t sum_nt = 0, sum_tt = 0;  // this lane's test counts since its last flush
+    uint32_t sum_nt = 0, sum_tt = 0, sum_ts = 0;  // this lane's test counts since its last flush
     int32_t ray = -1;
     bool exhausted = false;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, t_min = 0;
     RayPre pre{};
     Best best{0, 0, 0, -1, false};
-    uint32_t nt = 0, tt = 0;
-    int32_t cursor = 0, leaf_end = 0;  // triangle slots of the leaf being tested
+    uint32_t nt = 0, tt = 0, ts = 0;
+    int32_t cursor = 0, leaf_end = 0;  // primitive slots of the leaf being tested
 
     auto start = [&](const float4& o4, const float4& d4) {
         ox = o4.x; oy = o4.y; oz = o4.z; t_min = o4.w;
@@ -418,6 +479,7 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
         best = Best{d4.w, 0.0f, 0.0f, -1, false};
         nt = 0;
         tt = 0;
+        ts = 0;
         cursor = leaf_end = 0;
         pre = Precompute(ox, oy, oz, dx, dy, dz);
         if (!a.empty) {
@@ -434,14 +496,16 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
         if (st.sp == 0 && cursor >= leaf_end && ray >= 0) {
             sum_nt += nt;
             sum_tt += tt;
-            if (a.totals != nullptr && (sum_nt | sum_tt) >= 0x80000000u) {  // keep 32-bit sums from wrapping
+            sum_ts += ts;
+            if (a.totals != nullptr && (sum_nt | sum_tt | sum_ts) >= 0x80000000u) {  // keep 32-bit sums from wrapping
                 atomicAdd(a.totals, static_cast<unsigned long long>(sum_nt));
                 atomicAdd(a.totals + 1, static_cast<unsigned long long>(sum_tt));
-                sum_nt = sum_tt = 0;
+                atomicAdd(a.totals + 2, static_cast<unsigned long long>(sum_ts));
+                sum_nt = sum_tt = sum_ts = 0;
             }
             float4 no, nd;
-            if (pol.Finish(ray, best, -1, nt, tt, make_float4(ox, oy, oz, t_min), make_float4(dx, dy, dz, 0.0f), no,
-                           nd)) {
+            if (pol.Finish(ray, best, -1, nt, tt, ts, make_float4(ox, oy, oz, t_min), make_float4(dx, dy, dz, 0.0f),
+                           no, nd)) {
                 start(no, nd);
             } else {
                 ray = -1;
@@ -492,9 +556,17 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
             if (node) {
                 InteriorStep(f.q0, f.q1, f.q2, f.q3, pre, t_min, best.t, nt, st);
             } else if (tri) {
-                ++tt;
                 ++cursor;
-                if (TriangleTest(f.q0, f.q1, f.q2, ox, oy, oz, dx, dy, dz, t_min, kAny, best) && kAny) {
+                bool changed;
+                if (IsSphereId(__float_as_int(f.q0.w))) {
+                    ++ts;
+                    changed = SphereTest(f.q0.x, f.q0.y, f.q0.z, f.q1.x, __float_as_int(f.q0.w), ox, oy, oz, dx, dy,
+                                         dz, t_min, !kAny, best);
+                } else {
+                    ++tt;
+                    changed = TriangleTest(f.q0, f.q1, f.q2, ox, oy, oz, dx, dy, dz, t_min, kAny, best);
+                }
+                if (changed && kAny) {
                     st.Truncate(0);
                     cursor = leaf_end;
                 }
@@ -507,7 +579,7 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
         }
     }
     if (st.overflow) atomicOr(a.overflow, 1u);
-    if (a.totals != nullptr) AddTotals(a.totals, sum_nt, sum_tt, lane);
+    if (a.totals != nullptr) AddTotals(a.totals, sum_nt, sum_tt, sum_ts, lane);
 }
 
 template <bool kAny, class Policy>
@@ -523,7 +595,7 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
     const int32_t n = a.n_device != nullptr ? static_cast<int32_t>(*a.n_device) : a.n;
     // Claim range (low byte) and ranges left to try (next byte), wave-uniform.
     int part_state = static_cast<int>(blockIdx.x % kParts) | (kParts << 8);
-    uint32_t sum_nt = 0, sum_tt = 0;  // this lane's test counts since its last flush
+    uint32_t sum_nt = 0, sum_tt = 0, sum_ts = 0;  // this lane's test counts since its last flush
     int32_t ray = -1;
     bool exhausted = false;
     float ox = 0, oy = 0, oz = 0, dx = 0, dy = 0, dz = 0, t_min = 0;
@@ -533,7 +605,7 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
     RayPre pre{};
     Best best{0, 0, 0, -1, false};
     int32_t best_inst = -1;
-    uint32_t nt = 0, tt = 0;
+    uint32_t nt = 0, tt = 0, ts = 0;
     // object-space state of the BLAS being traversed
     bool in_blas = false;
     int blas_base = 0;
@@ -552,6 +624,7 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
         best_inst = -1;
         nt = 0;
         tt = 0;
+        ts = 0;
         in_blas = false;
         cursor = leaf_end = 0;
         tcur = tend = 0;
@@ -571,13 +644,15 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
         if (!busy && ray >= 0) {
             sum_nt += nt;
             sum_tt += tt;
-            if (a.totals != nullptr && (sum_nt | sum_tt) >= 0x80000000u) {  // keep 32-bit sums from wrapping
+            sum_ts += ts;
+            if (a.totals != nullptr && (sum_nt | sum_tt | sum_ts) >= 0x80000000u) {  // keep 32-bit sums from wrapping
                 atomicAdd(a.totals, static_cast<unsigned long long>(sum_nt));
                 atomicAdd(a.totals + 1, static_cast<unsigned long long>(sum_tt));
-                sum_nt = sum_tt = 0;
+                atomicAdd(a.totals + 2, static_cast<unsigned long long>(sum_ts));
+                sum_nt = sum_tt = sum_ts = 0;
             }
             float4 no, nd;
-            if (pol.Finish(ray, best, best_inst, nt, tt, make_float4(ox, oy, oz, t_min),
+            if (pol.Finish(ray, best, best_inst, nt, tt, ts, make_float4(ox, oy, oz, t_min),
                            make_float4(dx, dy, dz, 0.0f), no, nd)) {
                 start(no, nd);
             } else {
@@ -658,35 +733,51 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
                 }
             }
             if (try_instance && cursor < leaf_end) {
-                // Next instance of the TLAS leaf: object ray, BLAS root box.
                 const DevInstanceSlot m = a.instance_slots[cursor];
-                cur_inst = __float_as_int(m.hi.w);
-                const DevInstanceSlot& root = m;
-                TransformRay(m.r0, m.r1, m.r2, ox, oy, oz, dx, dy, dz, bx, by, bz, bdx, bdy, bdz);
-                const RayPre bpre = Precompute(bx, by, bz, bdx, bdy, bdz);
-                ++nt;
-                float entry;
-                if (Slab(bpre, root.lo.x, root.lo.y, root.lo.z, root.hi.x, root.hi.y, root.hi.z, t_min, best.t,
-                         entry)) {
-                    // The root's entry distance is within [t_min, best t], so the
-                    // stack pop it replaces would always have taken it.
-                    pre = bpre;
-                    bbest = Best{best.t, 0.0f, 0.0f, -1, false};
-                    blas_base = st.sp;
-                    in_blas = true;
-                    const int32_t code = __float_as_int(root.lo.w);
-                    if (code < 0) {
-                        int32_t first, count;
-                        DecodeLeaf(code, a.leaf_count, first, count);
-                        tcur = first;
-                        tend = first + count;
-                        tri = count > 0;
-                    } else {
-                        node = true;
-                        src = reinterpret_cast<const float4*>(a.nodes + code);
+                const int32_t slot_id = __float_as_int(m.hi.w);
+                if (IsSphereId(slot_id)) {
+                    // A TLAS sphere (IntersectTlasSphere, two_level_bvh.cpp:231-253):
+                    // tested against the world ray, no instance.
+                    ++ts;
+                    ++cursor;
+                    if (SphereTest(m.r0.x, m.r0.y, m.r0.z, m.r0.w, slot_id, ox, oy, oz, dx, dy, dz, t_min, !kAny,
+                                   best)) {
+                        best_inst = -1;
+                        if (kAny) {
+                            st.Truncate(0);
+                            cursor = leaf_end;
+                        }
                     }
                 } else {
-                    ++cursor;
+                    // Next instance of the TLAS leaf: object ray, BLAS root box.
+                    cur_inst = slot_id;
+                    const DevInstanceSlot& root = m;
+                    TransformRay(m.r0, m.r1, m.r2, ox, oy, oz, dx, dy, dz, bx, by, bz, bdx, bdy, bdz);
+                    const RayPre bpre = Precompute(bx, by, bz, bdx, bdy, bdz);
+                    ++nt;
+                    float entry;
+                    if (Slab(bpre, root.lo.x, root.lo.y, root.lo.z, root.hi.x, root.hi.y, root.hi.z, t_min, best.t,
+                             entry)) {
+                        // The root's entry distance is within [t_min, best t], so the
+                        // stack pop it replaces would always have taken it.
+                        pre = bpre;
+                        bbest = Best{best.t, 0.0f, 0.0f, -1, false};
+                        blas_base = st.sp;
+                        in_blas = true;
+                        const int32_t code = __float_as_int(root.lo.w);
+                        if (code < 0) {
+                            int32_t first, count;
+                            DecodeLeaf(code, a.leaf_count, first, count);
+                            tcur = first;
+                            tend = first + count;
+                            tri = count > 0;
+                        } else {
+                            node = true;
+                            src = reinterpret_cast<const float4*>(a.nodes + code);
+                        }
+                    } else {
+                        ++cursor;
+                    }
                 }
             }
             if (tri) src = a.slots + 3 * static_cast<int64_t>(tcur);
@@ -710,7 +801,7 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
         }
     }
     if (st.overflow) atomicOr(a.overflow, 1u);
-    if (a.totals != nullptr) AddTotals(a.totals, sum_nt, sum_tt, lane);
+    if (a.totals != nullptr) AddTotals(a.totals, sum_nt, sum_tt, sum_ts, lane);
 }
 
 }  // namespace trav

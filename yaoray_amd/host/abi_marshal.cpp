@@ -17,6 +17,10 @@ static_assert(offsetof(yr::PreparedMeshInstance, world_to_object) == offsetof(yr
 static_assert(offsetof(yr::PreparedMeshInstance, world_bounds) == offsetof(yrh_instance, world_bounds_min));
 static_assert(sizeof(yr::RenderTlasPrimitive) == sizeof(yrh_tlas_primitive));
 static_assert(sizeof(yr::Color4f) == 16);
+static_assert(sizeof(yr::RenderSphere) == sizeof(yrh_sphere));
+static_assert(offsetof(yr::RenderSphere, radius) == offsetof(yrh_sphere, radius));
+static_assert(offsetof(yr::RenderSphere, material_index) == offsetof(yrh_sphere, material_index));
+static_assert(offsetof(yr::RenderSphere, flip_normals) == offsetof(yrh_sphere, flip_normals));
 
 yrh_bvh_desc DescribeBvh(const yr::RenderBvh& b) {
     yrh_bvh_desc d{};
@@ -39,6 +43,8 @@ void DescribeGeometry(const yr::RenderSceneIR& ir, const yr::RenderAcceleration&
     g.index_count = static_cast<int64_t>(ir.indices.size());
     g.primitives = reinterpret_cast<const yrh_primitive*>(ir.primitives.data());
     g.primitive_count = static_cast<int32_t>(ir.primitives.size());
+    g.spheres = reinterpret_cast<const yrh_sphere*>(ir.spheres.data());
+    g.sphere_count = static_cast<int32_t>(ir.spheres.size());
     if (accel.kind == yr::RenderAccelerationKind::FlatReference) {
         g.accel_kind = YRH_ACCEL_FLAT;
         g.flat = DescribeBvh(accel.flat);

@@ -4,7 +4,7 @@
 // exactly the arrays yr::BuildBvh / yr::BuildTwoLevelBvh emit: same bucket
 // count, cost model, partition and median-fallback calls on the same element
 // sequence, same depth-first preorder node numbering. Reference:
-//   primitive refs        src/accel/bvh_builder.cpp:34-127
+//   primitive refs        src/accel/bvh_builder.cpp:34-140 (triangles, then spheres)
 //   split choice          src/accel/bvh_split_policy.cpp:13-182
 //   subtree recursion     src/accel/bvh_serial_builder.cpp:8-57,
 //                         src/accel/bvh_parallel_builder.cpp:9-106 (same result)
@@ -266,6 +266,11 @@ Bounds3f TransformBounds(const Mat4f& t, const Bounds3f& b) {
     return r;
 }
 
+// SphereBounds (src/geometry/intersection.cpp:130-134).
+Bounds3f SphereBounds(Point3f c, float r) {
+    return Bounds3f{Point3f{c.x - r, c.y - r, c.z - r}, Point3f{c.x + r, c.y + r, c.z + r}};
+}
+
 bool InvertibleAffine(const Mat4f& t) {
     for (float v : t.m) {
         if (!std::isfinite(v)) return false;
@@ -314,6 +319,17 @@ BvhBuildResult BuildBvh(GeometryView g, const BvhBuildOptions& opt) {
             prims.push_back(RenderBvhPrimitive{BvhPrimitiveKind::Triangle, p, -1, tri, flat++});
         }
     }
+    // Spheres after every triangle, centroid = centre (BuildPrimitiveRefs).
+    for (int k = 0; k < static_cast<int>(g.spheres.size()); ++k) {
+        const RenderSphere& sp = g.spheres[static_cast<size_t>(k)];
+        const Bounds3f b = SphereBounds(sp.center, sp.radius);
+        if (!(sp.radius > 0.0f) || !Finite(b) || !Finite(sp.center)) {
+            result.errors.push_back("BVH build encountered invalid sphere data");
+            return result;
+        }
+        refs.push_back(PrimRef{static_cast<int>(refs.size()), b, sp.center});
+        prims.push_back(RenderBvhPrimitive{BvhPrimitiveKind::Sphere, -1, k, -1, -1});
+    }
     RenderBvh& bvh = result.bvh;
     if (refs.empty()) {
         ValidOptions(opt, result.errors);
@@ -321,7 +337,8 @@ BvhBuildResult BuildBvh(GeometryView g, const BvhBuildOptions& opt) {
     }
     bvh.primitives = std::move(prims);
     bvh.total_primitives = static_cast<int>(refs.size());
-    bvh.total_triangles = bvh.total_primitives;
+    bvh.total_spheres = static_cast<int>(g.spheres.size());
+    bvh.total_triangles = bvh.total_primitives - bvh.total_spheres;
     if (!BuildFromRefs(refs, opt, bvh, result.errors)) bvh = RenderBvh{};
     return result;
 }
@@ -355,7 +372,7 @@ RenderAccelerationBuildResult BuildRenderAcceleration(GeometryView g, const BvhB
     int flat_offset = 0;
     for (int p = 0; p < static_cast<int>(g.primitives.size()); ++p) {
         const RenderPrimitive& mp = g.primitives[static_cast<size_t>(p)];
-        GeometryView one{g.vertices, g.indices, std::span<const RenderPrimitive>(&mp, 1), {}};
+        GeometryView one{g.vertices, g.indices, std::span<const RenderPrimitive>(&mp, 1), {}, {}};
         BvhBuildResult b = BuildBvh(one, opt);
         if (!b.errors.empty()) {
             result.errors = std::move(b.errors);
@@ -392,13 +409,26 @@ RenderAccelerationBuildResult BuildRenderAcceleration(GeometryView g, const BvhB
                                                     TransformBounds(src.object_to_world,
                                                                     tl.blases[static_cast<size_t>(p)].object_bounds)});
     }
-    // TLAS over instance world bounds, centroid = box centre.
+    // TLAS over instance world bounds (centroid = box centre), then the
+    // spheres (centroid = centre), two_level_bvh.cpp:51-109.
     std::vector<PrimRef> refs;
-    refs.reserve(tl.instances.size());
+    refs.reserve(tl.instances.size() + g.spheres.size());
     for (int i = 0; i < static_cast<int>(tl.instances.size()); ++i) {
         const Bounds3f& wb = tl.instances[static_cast<size_t>(i)].world_bounds;
         tl.tlas.primitives.push_back(RenderTlasPrimitive{TlasPrimitiveKind::Instance, i, -1, wb});
         refs.push_back(PrimRef{i, wb, (wb.min + wb.max) * 0.5f});
+    }
+    for (int k = 0; k < static_cast<int>(g.spheres.size()); ++k) {
+        const RenderSphere& sp = g.spheres[static_cast<size_t>(k)];
+        const Bounds3f b = SphereBounds(sp.center, sp.radius);
+        if (!(sp.radius > 0.0f) || !Finite(b)) {
+            result.errors.push_back("TLAS build encountered invalid sphere data");
+            result.acceleration.two_level = TwoLevelBvh{};
+            return result;
+        }
+        const int flat_index = static_cast<int>(tl.tlas.primitives.size());
+        tl.tlas.primitives.push_back(RenderTlasPrimitive{TlasPrimitiveKind::Sphere, -1, k, b});
+        refs.push_back(PrimRef{flat_index, b, sp.center});
     }
     RenderBvh hierarchy;
     if (!BuildFromRefs(refs, opt, hierarchy, result.errors)) {

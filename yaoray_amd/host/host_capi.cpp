@@ -188,6 +188,7 @@ int yrhost_scene_info_get(const yrhost_scene* s, yrhost_scene_info* info) {
     info->emissive_count = static_cast<int32_t>(s->ir.emissive_primitives.size());
     info->accel_built = s->accel_built ? 1 : 0;
     info->environment_active = s->ir.environment.active ? 1 : 0;
+    info->sphere_count = static_cast<int32_t>(s->ir.spheres.size());
     if (s->accel_built) {
         info->accel_kind = s->accel.kind == yr::RenderAccelerationKind::TwoLevel ? YRH_ACCEL_TWO_LEVEL : YRH_ACCEL_FLAT;
         info->node_count = s->accel.NodeCount();
@@ -198,6 +199,22 @@ int yrhost_scene_info_get(const yrhost_scene* s, yrhost_scene_info* info) {
     const float cam[13] = {c.origin.x,  c.origin.y,  c.origin.z,  c.forward.x, c.forward.y, c.forward.z, c.right.x,
                            c.right.y,   c.right.z,   c.up.x,      c.up.y,      c.up.z,      c.fov_y_radians};
     std::memcpy(info->camera, cam, sizeof(cam));
+    return YRH_OK;
+}
+
+int yrhost_scene_set_spheres(yrhost_scene* s, const yrh_sphere* spheres, int32_t count) {
+    if (s == nullptr || count < 0 || (count > 0 && spheres == nullptr)) return YRH_E_INVALID_ARGUMENT;
+    static_assert(sizeof(yrh_sphere) == sizeof(yr::RenderSphere));
+    s->ir.spheres.resize(static_cast<size_t>(count));
+    if (count > 0) std::memcpy(s->ir.spheres.data(), spheres, sizeof(yrh_sphere) * static_cast<size_t>(count));
+    s->accel_built = false;
+    return YRH_OK;
+}
+
+int yrhost_scene_spheres(const yrhost_scene* s, const yrh_sphere** spheres, int32_t* count) {
+    if (s == nullptr || spheres == nullptr || count == nullptr) return YRH_E_INVALID_ARGUMENT;
+    *spheres = reinterpret_cast<const yrh_sphere*>(s->ir.spheres.data());
+    *count = static_cast<int32_t>(s->ir.spheres.size());
     return YRH_OK;
 }
 

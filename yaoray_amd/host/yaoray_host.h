@@ -40,6 +40,7 @@ typedef struct yrhost_scene_info {
     int32_t blas_count;
     int32_t environment_active;
     float camera[13];       /* origin, forward, right, up, fov_y_radians */
+    int32_t sphere_count;
 } yrhost_scene_info;
 
 int yrhost_scene_create(int kind, const int64_t* params, int param_count, uint64_t seed, yrhost_scene** out);
@@ -61,6 +62,11 @@ int yrhost_scene_set_camera(yrhost_scene* scene, const float* camera13);
 int yrhost_scene_set_materials(yrhost_scene* scene, const yrh_material* materials, int32_t count);
 int yrhost_scene_set_emissives(yrhost_scene* scene, const yrh_emissive* emissives, int32_t count);
 int yrhost_scene_set_constant_environment(yrhost_scene* scene, const float* radiance3, float strength);
+/* Replaces the analytic spheres (copied; the acceleration structure must be
+ * built again). */
+int yrhost_scene_set_spheres(yrhost_scene* scene, const yrh_sphere* spheres, int32_t count);
+/* Read-only view of the spheres (valid until the scene changes). */
+int yrhost_scene_spheres(const yrhost_scene* scene, const yrh_sphere** spheres, int32_t* count);
 
 /* Read-only views of the scene arrays (valid until the scene is destroyed). */
 int yrhost_scene_geometry(const yrhost_scene* scene, const yrh_vertex** vertices, const uint32_t** indices,

@@ -103,6 +103,16 @@ struct RenderPrimitive {
 };
 static_assert(sizeof(RenderPrimitive) == 16);
 
+// Analytic sphere (geometry.hpp:29-35): world-space centre and radius;
+// flip_normals turns the outward normal inward.
+struct RenderSphere {
+    Point3f center{0.0f, 0.0f, 0.0f};
+    float radius = 1.0f;
+    int material_index = 0;
+    int area_light_index = -1;
+    bool flip_normals = false;
+};
+
 struct RenderInstance {
     int primitive = -1;  // MeshPrimitiveHandle value
     Mat4f object_to_world;
@@ -112,6 +122,7 @@ struct GeometryView {
     std::span<const RenderVertex> vertices;
     std::span<const std::uint32_t> indices;
     std::span<const RenderPrimitive> primitives;
+    std::span<const RenderSphere> spheres;
     std::span<const RenderInstance> instances{};
 };
 
@@ -237,6 +248,7 @@ struct RenderSceneIR {
     std::vector<RenderVertex> vertices;
     std::vector<std::uint32_t> indices;
     std::vector<RenderPrimitive> primitives;
+    std::vector<RenderSphere> spheres;
     std::vector<RenderInstance> instances;
     std::vector<RenderMaterial> materials;
     std::vector<RenderTexture> textures;
@@ -245,7 +257,7 @@ struct RenderSceneIR {
     std::vector<RenderEnvironmentDistribution> environment_distributions;
     LightSamplingCache light_sampling;
 
-    GeometryView Geometry() const { return GeometryView{vertices, indices, primitives, instances}; }
+    GeometryView Geometry() const { return GeometryView{vertices, indices, primitives, spheres, instances}; }
 };
 
 // ---- acceleration ------------------------------------------------------------
@@ -285,6 +297,7 @@ struct RenderBvh {
     int max_depth = 0;
     int total_primitives = 0;
     int total_triangles = 0;
+    int total_spheres = 0;
 };
 
 struct RenderBlas {

@@ -73,10 +73,16 @@ class GeometryDesc(C.Structure):
     _fields_ = [("vertices", C.c_void_p), ("vertex_count", C.c_int64),
                 ("indices", C.c_void_p), ("index_count", C.c_int64),
                 ("primitives", C.c_void_p), ("primitive_count", C.c_int32), ("sphere_count", C.c_int32),
-                ("accel_kind", C.c_int32), ("flat", BvhDesc),
+                ("spheres", C.c_void_p), ("accel_kind", C.c_int32), ("flat", BvhDesc),
                 ("blases", C.c_void_p), ("blas_count", C.c_int32),
                 ("instances", C.c_void_p), ("instance_count", C.c_int32),
                 ("tlas", BvhDesc), ("tlas_primitives", C.c_void_p), ("tlas_primitive_count", C.c_int32)]
+
+
+class Sphere(C.Structure):
+    """yrh_sphere == yr::RenderSphere (28 bytes)."""
+    _fields_ = [("center", C.c_float * 3), ("radius", C.c_float), ("material_index", C.c_int32),
+                ("area_light_index", C.c_int32), ("flip_normals", C.c_uint8), ("reserved", C.c_uint8 * 3)]
 
 
 class RaySoa(C.Structure):
@@ -85,7 +91,7 @@ class RaySoa(C.Structure):
 
 class HitSoa(C.Structure):
     _fields_ = [("hit", C.c_void_p), ("instance", C.c_void_p), ("node_tests", C.c_void_p),
-                ("triangle_tests", C.c_void_p)]
+                ("triangle_tests", C.c_void_p), ("sphere_tests", C.c_void_p)]
 
 
 class Material(C.Structure):
@@ -143,7 +149,7 @@ class SceneInfo(C.Structure):
                 ("instance_count", C.c_int32), ("material_count", C.c_int32), ("emissive_count", C.c_int32),
                 ("accel_built", C.c_int32), ("accel_kind", C.c_int32), ("node_count", C.c_int32),
                 ("max_depth", C.c_int32), ("blas_count", C.c_int32), ("environment_active", C.c_int32),
-                ("camera", C.c_float * 13)]
+                ("camera", C.c_float * 13), ("sphere_count", C.c_int32)]
 
 
 class BatchJob(C.Structure):
@@ -272,6 +278,8 @@ HOST_SIGNATURES = {
     "yrhost_scene_destroy": (None, [_VP]),
     "yrhost_scene_set_camera": (_I, [_VP, _VP]),
     "yrhost_scene_set_materials": (_I, [_VP, _VP, _I32]),
+    "yrhost_scene_set_spheres": (_I, [_VP, _VP, _I32]),
+    "yrhost_scene_spheres": (_I, [_VP, C.POINTER(_VP), C.POINTER(_I32)]),
     "yrhost_scene_set_emissives": (_I, [_VP, _VP, _I32]),
     "yrhost_scene_set_constant_environment": (_I, [_VP, _VP, _F]),
     "yrhost_scene_error": (C.c_char_p, [_VP]),
