@@ -320,6 +320,35 @@ typedef struct yrh_environment_desc {
     float total_weight;
 } yrh_environment_desc;
 
+#define YRH_LIGHT_POINT 0
+#define YRH_LIGHT_SPOT 1
+#define YRH_LIGHT_DISTANT 2
+
+/* == yr::AnalyticLight (include/yaoray/scene/light.hpp:36-45): a point, spot
+ * or distant light (kind YRH_LIGHT_*). Spot lights: direction is the cone
+ * axis, cone_angle the cosine of the outer cone, cone_cos_inner the cosine of
+ * the inner (full-intensity) cone; distant lights shine along direction. */
+typedef struct yrh_analytic_light {
+    int32_t kind;
+    float position[3];
+    float direction[3];
+    float intensity[3];
+    float cone_angle;
+    float cone_cos_inner;
+} yrh_analytic_light;
+
+/* == yr::LightTreeNode (include/yaoray/scene/light.hpp:53-59): bounds of the
+ * light positions below, their summed power, children (-1 for a leaf) and the
+ * leaf's light index (-1 for an interior node). */
+typedef struct yrh_light_tree_node {
+    float bounds_min[3];
+    float bounds_max[3];
+    float power;
+    int32_t left;
+    int32_t right;
+    int32_t light_index;
+} yrh_light_tree_node;
+
 typedef struct yrh_shading_desc {
     const yrh_material* materials;
     int32_t material_count;
@@ -330,6 +359,20 @@ typedef struct yrh_shading_desc {
     const int32_t* alias_index;
     const float* alias_pmf;
     yrh_environment_desc environment;
+    /* RenderSceneIR::analytic_lights and their LightSamplingCache tables
+     * (src/lighting/light_sampling.cpp:206-269): the power alias table
+     * (analytic_light_count entries) and the light tree, which light
+     * selection descends instead when use_analytic_tree is set (more than 8
+     * lights, none distant). NULL / 0 when the scene has none. */
+    const yrh_analytic_light* analytic_lights;
+    int32_t analytic_light_count;
+    const float* analytic_alias_probability;
+    const int32_t* analytic_alias_index;
+    const float* analytic_alias_pmf;
+    const yrh_light_tree_node* analytic_tree;
+    int32_t analytic_tree_count;
+    int32_t analytic_tree_root;
+    int32_t use_analytic_tree;
 } yrh_shading_desc;
 
 /* Uploads materials, emitters and the environment for the current geometry

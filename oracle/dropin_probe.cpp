@@ -13,6 +13,12 @@
 //       Render cancelled by its progress callback after <passes> passes,
 //       WriteFilmCheckpoint (src/film/film_checkpoint.cpp:71), LoadFilmCheckpoint
 //       (:120) with the settings hash, then Render resumed from the loaded film.
+//   dropin_probe dump <scene.pbrt> <out.bin>
+//       The compiled RenderSceneIR and RenderSettings as the C ABI carries them
+//       (raw vertex / index / primitive / sphere / instance arrays, materials,
+//       emitters and analytic lights through the backend's own marshalling):
+//       "YRIR", then sections (16-byte name, int64 count, int32 element size,
+//       bytes) up to a section named "end".
 //
 // out.bin: "YRPF", int32 ok, width, height, spp, uint64 stats[10]
 // (rays, shadow, occluded, triangle tests, sphere tests, node tests, hits,
@@ -24,6 +30,9 @@
 #include <yaoray/runtime/backend.hpp>
 #include <yaoray/scene/diagnostic.hpp>
 #include <yaoray/scene/render_scene_hash.hpp>
+#include <yaoray/lighting/light_sampling.hpp>
+
+#include "../yaoray_amd/host/abi_marshal.hpp"
 
 #include <cstdint>
 #include <cstdio>
@@ -150,12 +159,80 @@ int Checkpoint(int argc, char** argv) {
     return WriteOut(argv[5], r, st.spp, st.width, st.height) ? 0 : 1;
 }
 
+void Section(std::ofstream& f, const char* name, const void* data, int64_t count, int32_t elem) {
+    char tag[16] = {};
+    std::strncpy(tag, name, 15);
+    f.write(tag, 16);
+    f.write(reinterpret_cast<const char*>(&count), 8);
+    f.write(reinterpret_cast<const char*>(&elem), 4);
+    if (count > 0) f.write(static_cast<const char*>(data), static_cast<std::streamsize>(count * elem));
+}
+
+int Dump(int argc, char** argv) {
+    if (argc < 4) return 2;
+    auto loaded = Load(argv[2]);
+    if (!loaded) return 2;
+    yr::RenderSceneIR& ir = loaded->scene;
+    const yr::RenderSettings& st = loaded->settings;
+    if (!ir.textures.empty() || ir.environment.active) {
+        std::cerr << "dump: textures and environment maps are not carried\n";
+        return 1;
+    }
+    yr::PrepareLightSampling(ir);
+    yrhip::Marshal m;
+    yrh_shading_desc sd{};
+    const std::string err = yrhip::DescribeShading(ir, m, sd);
+    if (!err.empty()) {
+        std::cerr << "dump: " << err << '\n';
+        return 1;
+    }
+    std::ofstream f(argv[3], std::ios::binary);
+    f.write("YRIR", 4);
+    const yr::RenderCamera& c = ir.camera;
+    const float cam[13] = {c.origin.x, c.origin.y, c.origin.z, c.forward.x, c.forward.y, c.forward.z, c.right.x,
+                           c.right.y,  c.right.z,  c.up.x,     c.up.y,      c.up.z,      c.fov_y_radians};
+    Section(f, "camera", cam, 13, 4);
+    const int64_t settings[6] = {st.width, st.height, st.spp, st.max_depth, static_cast<int64_t>(st.sampler),
+                                 static_cast<int64_t>(st.seed)};
+    Section(f, "settings", settings, 6, 8);
+    Section(f, "vertices", ir.vertices.data(), static_cast<int64_t>(ir.vertices.size()), 48);
+    Section(f, "indices", ir.indices.data(), static_cast<int64_t>(ir.indices.size()), 4);
+    Section(f, "primitives", ir.primitives.data(), static_cast<int64_t>(ir.primitives.size()), 16);
+    std::vector<yrh_sphere> sph(ir.spheres.size());
+    for (size_t k = 0; k < sph.size(); ++k) {
+        const yr::RenderSphere& r = ir.spheres[k];
+        sph[k] = yrh_sphere{{r.center.x, r.center.y, r.center.z}, r.radius, r.material_index, r.area_light_index,
+                            static_cast<uint8_t>(r.flip_normals ? 1 : 0), {0, 0, 0}};
+    }
+    Section(f, "spheres", sph.data(), static_cast<int64_t>(sph.size()), sizeof(yrh_sphere));
+    std::vector<int32_t> inst_prim;
+    std::vector<float> inst_mat;
+    for (const yr::RenderInstance& in : ir.instances) {
+        inst_prim.push_back(in.primitive.Value());
+        inst_mat.insert(inst_mat.end(), in.object_to_world.m.begin(), in.object_to_world.m.end());
+    }
+    Section(f, "inst_prim", inst_prim.data(), static_cast<int64_t>(inst_prim.size()), 4);
+    Section(f, "inst_mat", inst_mat.data(), static_cast<int64_t>(inst_mat.size()), 4);
+    Section(f, "materials", sd.materials, sd.material_count, sizeof(yrh_material));
+    Section(f, "emissives", sd.emissives, sd.emissive_count, sizeof(yrh_emissive));
+    Section(f, "analytic", sd.analytic_lights, sd.analytic_light_count, sizeof(yrh_analytic_light));
+    const float env[4] = {ir.environment.radiance.x, ir.environment.radiance.y, ir.environment.radiance.z,
+                          ir.environment.strength};
+    Section(f, "environment", env, 4, 4);
+    Section(f, "end", nullptr, 0, 0);
+    std::cout << "dumped " << ir.vertices.size() << " vertices, " << ir.spheres.size() << " spheres, "
+              << ir.analytic_lights.size() << " analytic lights\n";
+    return f ? 0 : 1;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     if (argc >= 2 && std::strcmp(argv[1], "render") == 0) return Render(argc, argv);
     if (argc >= 2 && std::strcmp(argv[1], "checkpoint") == 0) return Checkpoint(argc, argv);
+    if (argc >= 2 && std::strcmp(argv[1], "dump") == 0) return Dump(argc, argv);
     std::cerr << "usage: dropin_probe render <scene.pbrt> <cpu|hip> <out.bin> [spp] [max_depth]\n"
-                 "       dropin_probe checkpoint <scene.pbrt> <cpu|hip> <ckpt> <out.bin> <passes>\n";
+                 "       dropin_probe checkpoint <scene.pbrt> <cpu|hip> <ckpt> <out.bin> <passes>\n"
+                 "       dropin_probe dump <scene.pbrt> <out.bin>\n";
     return 2;
 }

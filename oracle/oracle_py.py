@@ -154,6 +154,62 @@ def brute_force(vertices, indices, primitives, org, dirs, threads=8, spheres=Non
 
 # -- the reference itself -----------------------------------------------------------
 
+def sample_analytic(scene, points, u):
+    """The oracle's SampleAnalyticLight on the scene's analytic lights and
+    tables: dict(wi, distance, radiance, selection_pdf, light_index, valid)."""
+    lib = oracle_lib()
+    lib.oracle_sample_analytic.argtypes = [_VP, _I64, _VP, _VP, _VP]
+    p = _c(np.asarray(points).reshape(-1, 3))
+    uu = _c(np.asarray(u).reshape(-1))
+    out = np.zeros((len(p), 10), np.float32)
+    sd = scene.shading_desc()
+    if lib.oracle_sample_analytic(C.cast(C.pointer(sd), _VP), len(p), p.ctypes.data, uu.ctypes.data,
+                                  out.ctypes.data) != 0:
+        raise RuntimeError("oracle_sample_analytic failed")
+    return {"wi": out[:, 0:3], "distance": out[:, 3], "radiance": out[:, 4:7], "selection_pdf": out[:, 7],
+            "light_index": out[:, 8].astype(np.int32), "valid": out[:, 9] > 0}
+
+
+def _ref_analytic(lights):
+    """lights: ctypes array of native.AnalyticLight (or None)."""
+    lib = ref_lib()
+    lib.ref_set_analytic_lights.argtypes = [_VP, _I32]
+    lib.ref_set_analytic_lights.restype = None
+    if lights is None or len(lights) == 0:
+        lib.ref_set_analytic_lights(None, 0)
+    else:
+        lib.ref_set_analytic_lights(C.cast(lights, _VP), len(lights))
+
+
+def _scene_analytic(scene):
+    from yaoray_amd import native as N
+    sd = scene.shading_desc()
+    n = sd.analytic_light_count
+    if n == 0:
+        return None
+    return (N.AnalyticLight * n).from_buffer_copy(C.string_at(sd.analytic_lights, n * C.sizeof(N.AnalyticLight)))
+
+
+def ref_analytic_tables(lights):
+    """The reference's PrepareLightSampling tables for `lights` (ctypes array of
+    native.AnalyticLight): (probability, alias, pmf, tree nodes (LIGHT_NODE_DTYPE),
+    root, use_tree)."""
+    lib = ref_lib()
+    lib.ref_analytic_tables.argtypes = [_VP, _I32, _VP, _VP, _VP, _VP, C.POINTER(_I32), C.POINTER(_I32)]
+    lib.ref_analytic_tables.restype = _I32
+    n = len(lights)
+    prob, alias, pmf = np.zeros(n, np.float32), np.zeros(n, np.int32), np.zeros(n, np.float32)
+    tree = np.zeros(max(1, 2 * n), LIGHT_NODE_DTYPE)
+    root, use = _I32(), _I32()
+    count = lib.ref_analytic_tables(C.cast(lights, _VP), n, prob.ctypes.data, alias.ctypes.data, pmf.ctypes.data,
+                                    tree.ctypes.data, C.byref(root), C.byref(use))
+    return prob, alias, pmf, tree[:count].copy(), root.value, use.value
+
+
+LIGHT_NODE_DTYPE = np.dtype([("bounds_min", "<f4", 3), ("bounds_max", "<f4", 3), ("power", "<f4"),
+                             ("left", "<i4"), ("right", "<i4"), ("light_index", "<i4")])
+
+
 def _ref_spheres(spheres):
     lib = ref_lib()
     lib.ref_set_spheres.argtypes = [_VP, _I32]
@@ -326,10 +382,12 @@ def ref_render(scene, width, height, spp, max_depth=8, sampler=2, seed=0x59414F5
     stats = np.zeros(10, np.uint64)
     secs = C.c_double()
     _ref_spheres(scene.spheres())
+    _ref_analytic(_scene_analytic(scene))
     rc = lib.ref_render(_p(v), len(v), _p(i), len(i), _p(p), len(p), ip.ctypes.data, im.ctypes.data,
                         info.instance_count, C.cast(C.pointer(sd), _VP), cam.ctypes.data, C.cast(C.pointer(rd), _VP),
                         threads, film.ctypes.data, stats.ctypes.data, C.byref(secs))
     _ref_spheres(None)
+    _ref_analytic(None)
     if rc != 0:
         raise RuntimeError(f"reference render failed ({rc})")
     out = dict(zip(STAT_KEYS, (int(x) for x in stats[:9])))

@@ -168,7 +168,14 @@ struct Sampler {
     void Get2(uint32_t d, float& a, float& b) const { At2(Abs(d), 0, a, b); }
 };
 
-enum : uint32_t { DimPixel = 0, DimLightSelect = 2, DimLightSurface = 3, DimBsdf = 5, DimRoulette = 7 };
+enum : uint32_t {
+    DimPixel = 0,
+    DimLightSelect = 2,
+    DimLightSurface = 3,
+    DimBsdf = 5,
+    DimRoulette = 7,
+    DimAnalyticSelect = 14
+};
 
 // ---- scene ------------------------------------------------------------------------
 
@@ -912,6 +919,109 @@ bool ShadowVisible(const Scene& s, V3 o, V3 d, float max_dist, Stats& st, V3& T)
     return false;
 }
 
+// ---- analytic lights (src/lighting/analytic_light_sampling.cpp:9-140) -------------
+
+struct AS {
+    bool valid = false;
+    V3 wi{0, 0, 0}, radiance{0, 0, 0};
+    float distance = 0.0f, selection_pdf = 0.0f;
+    int light_index = -1;
+};
+
+// DistanceSquaredToBounds / TreeImportance (:9-18).
+float TreeImportance(const yrh_light_tree_node& nd, V3 p) {
+    const float dx = std::max({nd.bounds_min[0] - p.x, 0.0f, p.x - nd.bounds_max[0]});
+    const float dy = std::max({nd.bounds_min[1] - p.y, 0.0f, p.y - nd.bounds_max[1]});
+    const float dz = std::max({nd.bounds_min[2] - p.z, 0.0f, p.z - nd.bounds_max[2]});
+    return nd.power / std::max(1.0e-4f, dx * dx + dy * dy + dz * dz);
+}
+
+// SampleAnalyticLight (:80-140) with SampleAnalyticPoint / Distant / Spot (:22-78).
+AS SampleAnalytic(const Scene& s, V3 p, float select) {
+    const yrh_shading_desc& sd = *s.sd;
+    AS out;
+    const int n = sd.analytic_light_count;
+    if (n <= 0) return out;
+    int light = -1;
+    float pdf = 1.0f;
+    float u = std::clamp(select, 0.0f, std::nextafter(1.0f, 0.0f));
+    if (sd.use_analytic_tree && sd.analytic_tree_root >= 0 && sd.analytic_tree_root < sd.analytic_tree_count) {
+        int node = sd.analytic_tree_root;
+        while (node >= 0) {
+            const yrh_light_tree_node& nd = sd.analytic_tree[node];
+            if (nd.light_index >= 0) {
+                light = nd.light_index;
+                break;
+            }
+            if (nd.left < 0 || nd.right < 0) return out;
+            const float lw = TreeImportance(sd.analytic_tree[nd.left], p);
+            const float rw = TreeImportance(sd.analytic_tree[nd.right], p);
+            const float total = lw + rw;
+            const float lp = total > 0.0f ? lw / total : 0.5f;
+            if (u < lp) {
+                pdf *= lp;
+                u = lp > 0.0f ? u / lp : 0.0f;
+                node = nd.left;
+            } else {
+                const float rp = 1.0f - lp;
+                pdf *= rp;
+                u = rp > 0.0f ? (u - lp) / rp : 0.0f;
+                node = nd.right;
+            }
+        }
+    } else if (sd.analytic_alias_pmf != nullptr) {
+        // SampleAliasTable (src/lighting/light_sampling.cpp:184-204).
+        const float scaled = std::min(ClampUnitSample(u), std::nextafter(1.0f, 0.0f)) * static_cast<float>(n);
+        const int column = std::min(static_cast<int>(scaled), n - 1);
+        const float fraction = scaled - static_cast<float>(column);
+        light = fraction < sd.analytic_alias_probability[column] ? column : sd.analytic_alias_index[column];
+        pdf = sd.analytic_alias_pmf[light];
+    } else {
+        light = std::min(static_cast<int>(u * static_cast<float>(n)), n - 1);
+        pdf = 1.0f / static_cast<float>(n);
+    }
+    if (light < 0 || light >= n || pdf <= 0.0f) return out;
+    const yrh_analytic_light& l = sd.analytic_lights[light];
+    const V3 intensity = Load3(l.intensity);
+    if (l.kind == YRH_LIGHT_POINT) {
+        const V3 delta = Sub(Load3(l.position), p);
+        const float d2 = Dot(delta, delta);
+        if (d2 <= 0.0f) return out;
+        out.distance = std::sqrt(d2);
+        out.wi = Normalize(delta);
+        out.radiance = Div(intensity, d2);
+    } else if (l.kind == YRH_LIGHT_DISTANT) {
+        const V3 to_light = Neg(Load3(l.direction));
+        if (LengthSq(to_light) == 0.0f) return out;
+        out.wi = Normalize(to_light);
+        out.distance = 1.0e6f;
+        out.radiance = intensity;
+    } else {
+        const V3 delta = Sub(Load3(l.position), p);
+        const float d2 = Dot(delta, delta);
+        if (d2 <= 0.0f) return out;
+        const float dist = std::sqrt(d2);
+        const V3 wi = Div(delta, dist);
+        const V3 dir = Load3(l.direction);
+        const V3 axis = LengthSq(dir) > 0.0f ? Normalize(dir) : V3{0.0f, 0.0f, 1.0f};
+        const float cosine = -Dot(wi, axis);
+        float falloff = 0.0f;
+        if (cosine >= l.cone_cos_inner) {
+            falloff = 1.0f;
+        } else if (cosine > l.cone_angle) {
+            const float t = (cosine - l.cone_angle) / (l.cone_cos_inner - l.cone_angle);
+            falloff = t * t * (3.0f - 2.0f * t);
+        }
+        out.wi = wi;
+        out.distance = dist;
+        out.radiance = Mul(intensity, falloff / d2);
+    }
+    out.selection_pdf = pdf;
+    out.light_index = light;
+    out.valid = true;
+    return out;
+}
+
 V3 DirectLight(const Scene& s, const yrh_material& m, V3 hp, V3 n, V3 wo, const Sampler& smp, Stats& st) {
     V3 rad{0, 0, 0};
     if (s.sd->emissive_count > 0) {
@@ -954,6 +1064,26 @@ V3 DirectLight(const Scene& s, const yrh_material& m, V3 hp, V3 n, V3 wo, const 
             }
         }
         rad = Mul(rad, 1.0f);
+    }
+    // Analytic lights: one delta sample, no MIS (path_direct_lighting.cpp:250-281).
+    if (s.sd->analytic_light_count > 0) {
+        const AS as = SampleAnalytic(s, hp, smp.Get1(DimAnalyticSelect));
+        if (as.valid && as.selection_pdf > 0.0f && !NearBlack(as.radiance)) {
+            const float c = std::max(0.0f, Dot(n, as.wi));
+            if (c > 0.0f) {
+                const V3 f = EvalBsdf(m, wo, as.wi, n);
+                if (!NearBlack(f)) {
+                    const V3 so = Add(hp, Mul(n, SurfaceBias(hp)));
+                    ++st.shadow;
+                    V3 T;
+                    if (!ShadowVisible(s, so, as.wi, as.distance - SurfaceBias(hp), st, T)) {
+                        ++st.occluded;
+                    } else {
+                        rad = Add(rad, Mul(Had(f, Had(T, as.radiance)), c / as.selection_pdf));
+                    }
+                }
+            }
+        }
     }
     if (EnvSampleable(s)) {
         V3 er{0, 0, 0};
@@ -1195,8 +1325,10 @@ bool IsConverged(const FilmPixel32& p, int min_spp, float relative_error, float 
 extern "C" {
 #pragma GCC visibility push(default)
 
-// Renders passes [sample_begin, sample_begin + pass_count) of `rd` into `film`
-// (width*height FilmPixel records of 56 bytes, accumulated in place).
+// Renders pass_count passes of `rd` into `film` (width*height FilmPixel
+// records of 56 bytes, accumulated in place): RenderCpuPathTrace's pass loop
+// without adaptive sampling; a pixel's sample index is sample_begin plus its
+// film sample count.
 // stats: rays, shadow, occluded, triangle tests, sphere tests, node tests,
 // hits, misses, bad samples (uint64 each, accumulated).
 int oracle_render(const yrh_geometry_desc* gd, const yrh_shading_desc* sd, const yrh_render_desc* rd,
@@ -1209,9 +1341,13 @@ int oracle_render(const yrh_geometry_desc* gd, const yrh_shading_desc* sd, const
     std::vector<Stats> per_thread(static_cast<size_t>(std::max(1, threads)));
     std::vector<int64_t> all(static_cast<size_t>(npix));
     for (int64_t p = 0; p < npix; ++p) all[static_cast<size_t>(p)] = p;
+    // Each pass renders every pixel's next sample: index = sample_begin + its
+    // film sample count (cpu_path_tracer.cpp:126), so a pixel whose sample was
+    // not finite (not added) renders the same index again.
     for (int pi = 0; pi < pass_count; ++pi) {
-        const int sample = sample_begin + pi;
-        RenderPixels(s, *rd, all, [sample](int64_t) { return sample; }, threads, per_thread, pass);
+        RenderPixels(
+            s, *rd, all, [&](int64_t p) { return sample_begin + static_cast<int>(px[p].samples); }, threads,
+            per_thread, pass);
         uint64_t bad = 0;
         for (int64_t p = 0; p < npix; ++p) AddSample(px[p], pass[static_cast<size_t>(p)], bad);
         stats[8] += bad;
@@ -1319,6 +1455,22 @@ int oracle_bsdf_query(const yrh_material* mats, int32_t nmat, int64_t n, const i
         const float vals[12] = {f.x, f.y, f.z, pdf, s.wi.x, s.wi.y, s.wi.z, s.weight.x, s.weight.y, s.weight.z, s.pdf,
                                 static_cast<float>((s.valid ? 1 : 0) + (s.specular ? 2 : 0) + (IsDelta(m) ? 4 : 0))};
         std::memcpy(o, vals, sizeof(vals));
+    }
+    return 0;
+}
+
+// SampleAnalyticLight at points p (n x 3) with selection samples u (n): per
+// row wi (3), distance, radiance (3), selection pdf, light index, valid.
+int oracle_sample_analytic(const yrh_shading_desc* sd, int64_t n, const float* p, const float* u, float* out) {
+    if (sd == nullptr) return -1;
+    Scene s{};
+    s.sd = sd;
+    for (int64_t i = 0; i < n; ++i) {
+        const AS a = SampleAnalytic(s, Load3(p + 3 * i), u[i]);
+        const float vals[10] = {a.wi.x,         a.wi.y,        a.wi.z, a.distance, a.radiance.x, a.radiance.y,
+                                a.radiance.z,   a.selection_pdf, static_cast<float>(a.light_index),
+                                a.valid ? 1.0f : 0.0f};
+        std::memcpy(out + 10 * i, vals, sizeof(vals));
     }
     return 0;
 }

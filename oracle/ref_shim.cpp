@@ -10,6 +10,7 @@
 #include <yaoray/backends/cpu/cpu_prepared_scene.hpp>
 #include <yaoray/core/transform.hpp>
 #include <yaoray/lighting/environment.hpp>
+#include <yaoray/lighting/light_sampling.hpp>
 #include <yaoray/core/rng.hpp>
 #include <yaoray/sampling/sampler.hpp>
 #include <yaoray/shading/bsdf.hpp>
@@ -70,6 +71,47 @@ void ref_set_spheres(const yrh_sphere* sp, int32_t n) {
         r.flip_normals = sp[k].flip_normals != 0;
         g_spheres.push_back(r);
     }
+}
+
+// Analytic lights the next ref_render calls add to their scene (yrh_analytic_light
+// records, converted field by field); n = 0 clears them.
+std::vector<yr::AnalyticLight> g_analytic;
+
+void ref_set_analytic_lights(const yrh_analytic_light* l, int32_t n) {
+    g_analytic.clear();
+    for (int32_t k = 0; k < n; ++k) {
+        yr::AnalyticLight a;
+        a.kind = static_cast<yr::AnalyticLightKind>(l[k].kind);
+        a.position = yr::Point3f{l[k].position[0], l[k].position[1], l[k].position[2]};
+        a.direction = yr::Vec3f{l[k].direction[0], l[k].direction[1], l[k].direction[2]};
+        a.intensity = yr::Color3f{l[k].intensity[0], l[k].intensity[1], l[k].intensity[2]};
+        a.cone_angle = l[k].cone_angle;
+        a.cone_cos_inner = l[k].cone_cos_inner;
+        g_analytic.push_back(a);
+    }
+}
+
+// yr::PrepareLightSampling's analytic tables for `n` lights: the power alias
+// table (n entries each) and the light tree (at most 2n - 1 nodes, 40-byte
+// LightTreeNode records); returns the node count.
+int32_t ref_analytic_tables(const yrh_analytic_light* l, int32_t n, float* prob, int32_t* alias, float* pmf,
+                            void* tree, int32_t* root, int32_t* use_tree) {
+    ref_set_analytic_lights(l, n);
+    yr::RenderSceneIR sc;
+    sc.analytic_lights = g_analytic;
+    g_analytic.clear();
+    yr::PrepareLightSampling(sc);
+    const yr::LightSamplingCache& c = sc.light_sampling;
+    for (size_t k = 0; k < c.analytic_power.pmf.size(); ++k) {
+        prob[k] = c.analytic_power.probability[k];
+        alias[k] = c.analytic_power.alias[k];
+        pmf[k] = c.analytic_power.pmf[k];
+    }
+    static_assert(sizeof(yr::LightTreeNode) == 40);
+    if (!c.analytic_tree.empty()) std::memcpy(tree, c.analytic_tree.data(), c.analytic_tree.size() * 40);
+    *root = c.analytic_tree_root;
+    *use_tree = c.use_analytic_tree ? 1 : 0;
+    return static_cast<int32_t>(c.analytic_tree.size());
 }
 
 // vertices: 48-byte RenderVertex records; primitives: 16-byte RenderPrimitive.
@@ -271,6 +313,7 @@ int ref_render(const yrh_vertex* vertices, int64_t nv, const uint32_t* indices, 
         sc.instances.push_back(inst);
     }
     sc.spheres = g_spheres;
+    sc.analytic_lights = g_analytic;
     for (int32_t k = 0; k < sd->material_count; ++k) sc.materials.push_back(ToRefMaterial(sd->materials[k]));
     for (int32_t k = 0; k < sd->emissive_count; ++k) {
         const yrh_emissive& e = sd->emissives[k];

@@ -231,7 +231,52 @@ def spheres_fixture():
     return out
 
 
-FIXTURES = {"spheres": spheres_fixture, "sampler": sampler_fixture, "bsdf": bsdf_fixture, "zoo_film": zoo_film_fixture, "flat_trace": flat_fixture, "two_level_trace": two_level_fixture,
+def read_yrir(path):
+    """Sections of a `dropin_probe dump` file: name -> raw bytes (and count)."""
+    with open(path, "rb") as f:
+        data = f.read()
+    assert data[:4] == b"YRIR"
+    off, out = 4, {}
+    while True:
+        name = data[off:off + 16].split(b"\0")[0].decode()
+        count = int(np.frombuffer(data, np.int64, 1, off + 16)[0])
+        elem = int(np.frombuffer(data, np.int32, 1, off + 24)[0])
+        off += 28
+        if name == "end":
+            return out
+        out[name] = np.frombuffer(data, np.uint8, count * elem, off).copy()
+        off += count * elem
+
+
+def cornell_pbrt_fixture():
+    """The reference's own Cornell scene (scenes/pbrt/cornell_box_pbrt: mirror
+    and glass spheres, a ceiling area light and a point light) compiled by its
+    PBRT frontend (the drop-in probe's `dump`), and the reference CPU backend's
+    film + statistics of its 64 x 64 smoke variant at 32 spp, depth 6, rendered
+    by the same probe."""
+    import subprocess
+    import tempfile
+
+    from tests import dropin as D
+    scenes = "/root/reference/scenes/pbrt/cornell_box_pbrt/"
+    with tempfile.TemporaryDirectory() as tmp:
+        out = {}
+        for tag, name in (("full", "cornell_box_pbrt.pbrt"), ("smoke", "cornell_box_pbrt_smoke.pbrt")):
+            path = os.path.join(tmp, tag + ".bin")
+            subprocess.run([D.PROBE, "dump", scenes + name, path], check=True, cwd=tmp)
+            ir = read_yrir(path)
+            for k, v in ir.items():
+                out[f"{tag}_{k}"] = v
+        r, res = D.probe(["render", scenes + "cornell_box_pbrt_smoke.pbrt", "cpu", "32"],
+                         os.path.join(tmp, "film.bin"), tmp)
+        assert r.returncode == 0 and res["ok"], r.stderr
+    out["film"] = res["film"].reshape(-1).view(np.uint8)
+    out["stats"] = np.array([res["stats"][k] for k in O.STAT_KEYS], np.uint64)
+    out["spp"] = np.int32(32)
+    return out
+
+
+FIXTURES = {"cornell_pbrt": cornell_pbrt_fixture, "spheres": spheres_fixture, "sampler": sampler_fixture, "bsdf": bsdf_fixture, "zoo_film": zoo_film_fixture, "flat_trace": flat_fixture, "two_level_trace": two_level_fixture,
             "cornell_film": cornell_fixture, "cornell_adaptive": cornell_adaptive_fixture}
 
 

@@ -220,3 +220,47 @@ def sphere_zoo(instances=150):
                    ((-1.0, 1.0, 22.0), 0.9, 4)])
     s.build_accel()
     return s
+
+
+# ---- analytic lights ---------------------------------------------------------------
+
+def lit_sphere_scene(lights):
+    """MakeLitSphereJob (tests/backends/cpu/cpu_path_tracer_test_support.hpp:59-84):
+    a diffuse 0.8 sphere of radius 0.5 at the origin, camera at z = 3, no
+    environment, lit by `lights` (Scene.set_analytic_lights dicts)."""
+    s = MeshArrays().scene()
+    s.set_camera((0.0, 0.0, 3.0), (0.0, 0.0, -1.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), 1.04719758)
+    s.set_materials([{"kind": 0, "reflectance": (0.8, 0.8, 0.8)}])
+    s.set_constant_environment((0.0, 0.0, 0.0), 1.0)
+    s.set_spheres([((0.0, 0.0, 0.0), 0.5, 0)])
+    s.set_analytic_lights(lights)
+    s.build_accel()
+    return s
+
+
+CORNELL_LIGHTS = [
+    {"kind": "point", "position": (0.3, 1.6, 0.5), "intensity": (1.5, 1.4, 1.2)},
+    {"kind": "spot", "position": (-0.5, 1.9, -0.3), "direction": (0.1, -1.0, 0.05), "intensity": (3.0, 2.5, 2.0),
+     "cone_angle": float(np.cos(np.radians(35.0))), "cone_cos_inner": float(np.cos(np.radians(25.0)))},
+    {"kind": "distant", "direction": (0.1, -0.45, -1.0), "intensity": (0.6, 0.65, 0.7)},
+]
+
+
+def cornell_lights_scene(spheres=True):
+    """The sphere Cornell box (or the plain one) with a point, a spot and a
+    distant light (the distant one shines in through the open front)."""
+    s = sphere_cornell() if spheres else Scene.cornell(1)
+    s.set_analytic_lights(CORNELL_LIGHTS)
+    s.build_accel()
+    return s
+
+
+def many_point_lights_scene(count=12, seed=3):
+    """Cornell box with `count` point lights and no others kinds: more than 8
+    local lights, so light selection descends the light tree."""
+    rng = np.random.default_rng(seed)
+    s = Scene.cornell(1)
+    s.set_analytic_lights([{"kind": "point", "position": tuple(rng.uniform((-0.9, 0.2, -0.9), (0.9, 1.9, 0.9))),
+                            "intensity": tuple(rng.uniform(0.05, 0.6, 3))} for _ in range(count)])
+    s.build_accel()
+    return s

@@ -9,7 +9,10 @@ machine.
 * the Cornell workload's FP32 film and statistics from the oracle, bit for bit;
 * analytic spheres: per-ray hits (t, SphereUv, id) and node / triangle /
   sphere counters of the oracle on flat and TLAS spheres, and the sphere
-  Cornell box's FP32 film and statistics, bit for bit.
+  Cornell box's FP32 film and statistics, bit for bit;
+* the reference's own Cornell scene (scenes/pbrt/cornell_box_pbrt: mirror and
+  glass spheres, an area light and a point light) as its PBRT frontend
+  compiles it: the oracle's film and statistics equal its CPU backend's.
 """
 import numpy as np
 import pytest
@@ -143,3 +146,16 @@ def test_oracle_matches_spheres_fixture():
     film, st = O.render(c, w, h, int(f["spp"]), max_depth=int(f["depth"]), sampler=2, seed=int(f["seed"]))
     assert film.tobytes() == G.film(f["film"]).tobytes()
     assert st == G.stats(f["stats"])
+
+
+def test_oracle_matches_reference_cornell_pbrt_fixture():
+    f = G.load("cornell_pbrt")
+    for k in f:
+        if k.startswith("full_") and k != "full_settings":
+            assert np.array_equal(f[k], f["smoke_" + k[5:]]), k  # the smoke variant is the same scene
+    s, st = G.pbrt_scene(f)
+    assert s.info.sphere_count == 2 and s.info.analytic_light_count == 1 and s.info.emissive_count == 1
+    film, stats = O.render(s, st["width"], st["height"], int(f["spp"]), max_depth=st["max_depth"],
+                           sampler=st["sampler"], seed=st["seed"])
+    assert film.tobytes() == G.film(f["film"]).tobytes()
+    assert stats == G.stats(f["stats"])

@@ -256,6 +256,24 @@ class Scene:
         if self._lib.yrhost_scene_set_spheres(self._h, C.cast(arr, C.c_void_p), len(spheres)) != 0:
             raise YrhError("set_spheres failed")
 
+    LIGHT_KINDS = {"point": 0, "spot": 1, "distant": 2}
+
+    def set_analytic_lights(self, lights):
+        """lights: list of dicts (yr::AnalyticLight): kind ("point" | "spot" |
+        "distant"), position, direction, intensity (rgb), and for spot lights
+        cone_angle / cone_cos_inner (cosines of the outer / inner cone). The
+        selection tables (power alias table, light tree) are rebuilt."""
+        arr = (N.AnalyticLight * max(1, len(lights)))()
+        for k, l in enumerate(lights):
+            arr[k].kind = self.LIGHT_KINDS[l["kind"]] if isinstance(l["kind"], str) else int(l["kind"])
+            arr[k].position[:] = [float(v) for v in l.get("position", (0.0, 0.0, 0.0))]
+            arr[k].direction[:] = [float(v) for v in l.get("direction", (0.0, 0.0, 0.0))]
+            arr[k].intensity[:] = [float(v) for v in l["intensity"]]
+            arr[k].cone_angle = float(l.get("cone_angle", 0.0))
+            arr[k].cone_cos_inner = float(l.get("cone_cos_inner", 0.0))
+        if self._lib.yrhost_scene_set_analytic_lights(self._h, C.cast(arr, C.c_void_p), len(lights)) != 0:
+            raise YrhError("set_analytic_lights failed")
+
     def spheres(self) -> np.ndarray:
         """The scene's spheres as a SPHERE_DTYPE array (a copy)."""
         p, n = C.c_void_p(), C.c_int32()

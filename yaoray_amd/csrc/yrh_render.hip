@@ -105,6 +105,20 @@ struct RenderState {
     DevBuf alias_prob, alias_index, alias_pmf;
     int32_t emissive_count = 0;
     float total_emissive_area = 0.0f;
+    // analytic lights: three float4 per light (position + kind bits,
+    // direction + cone_angle, intensity + cone_cos_inner), the power alias
+    // table, and the light tree (three float4 per node: bounds min + power,
+    // bounds max, (left, right, light index) bits)
+    DevBuf alight, alight_prob, alight_alias, alight_pmf, alight_tree;
+    int32_t analytic_count = 0;
+    int32_t analytic_pmf_count = 0;
+    int32_t analytic_tree_count = 0;
+    int32_t analytic_tree_root = -1;
+    int32_t use_analytic_tree = 0;
+    // Shadow-queue entries per path slot: emissive, analytic (scenes with
+    // analytic lights), environment.
+    int shadow_slots = 2;
+    int cap_shadow_slots = 0;
     // environment
     int32_t env_active = 0;
     float env_radiance[3] = {0, 0, 0};
@@ -132,7 +146,7 @@ struct RenderState {
     yrh_adaptive_desc act_desc{};
     // ---- pass buffers (capacity = paths per pass) ----
     int64_t capacity = 0;
-    DevBuf p_pixel, p_tp, p_rad, p_prev, p_flags, p_med, p_alb, p_nrm, p_nee, p_slot;
+    DevBuf p_pixel, p_tp, p_rad, p_prev, p_flags, p_med, p_alb, p_nrm, p_nee, p_slot;  // p_slot: int4 shadow entries
     DevBuf q_org[2], q_dir[2], q_path[2];
     DevBuf hits, hit_inst;
     DevBuf s_org, s_dir, s_li, s_bsdf, s_occ, s_trans;
@@ -197,6 +211,16 @@ struct LightView {
     const float* alias_pmf;
     int32_t emissive_count;
     float total_emissive_area;
+    const float4* alight;
+    const float* alight_prob;
+    const int32_t* alight_alias;
+    const float* alight_pmf;
+    const float4* alight_tree;
+    int32_t analytic_count;
+    int32_t analytic_pmf_count;
+    int32_t analytic_tree_count;
+    int32_t analytic_tree_root;
+    int32_t use_analytic_tree;
     int32_t env_active;
     float3 env_radiance;
     float env_strength;
@@ -249,7 +273,7 @@ struct PassView {
     float4* p_alb;
     float4* p_nrm;
     float4* p_nee;
-    int2* p_slot;
+    int4* p_slot;  // shadow-queue entries of the path's vertex: emissive, analytic, environment
     // queues
     float4* q_org[2];
     float4* q_dir[2];
@@ -600,6 +624,117 @@ __device__ __forceinline__ float EmissivePdfSolidAngle(const LightView& L, float
     return area_pdf * d2 / c;
 }
 
+// ---- analytic lights (src/lighting/analytic_light_sampling.cpp:22-140) --------------
+
+struct AnalyticSample {
+    float3 wi, radiance;
+    float distance;
+    float selection_pdf;
+    bool valid;
+};
+
+__device__ __forceinline__ int4 AsInt4(float4 v) {
+    return make_int4(__float_as_int(v.x), __float_as_int(v.y), __float_as_int(v.z), __float_as_int(v.w));
+}
+
+// TreeImportance: power over the squared distance to the node's bounds.
+__device__ __forceinline__ float TreeImportance(const float4* node, float3 p) {
+    const float4 lo = node[0], hi = node[1];
+    const float dx = fmaxf(fmaxf(lo.x - p.x, 0.0f), p.x - hi.x);
+    const float dy = fmaxf(fmaxf(lo.y - p.y, 0.0f), p.y - hi.y);
+    const float dz = fmaxf(fmaxf(lo.z - p.z, 0.0f), p.z - hi.z);
+    return lo.w / fmaxf(1.0e-4f, dx * dx + dy * dy + dz * dz);
+}
+
+// SampleAnalyticLight: pick one light (light tree descent by importance, else
+// the power alias table, else uniformly), then its delta sample.
+__device__ __forceinline__ AnalyticSample SampleAnalytic(const LightView& L, float3 p, float select) {
+    AnalyticSample out;
+    out.valid = false;
+    out.selection_pdf = 0.0f;
+    const int n = L.analytic_count;
+    if (n <= 0) return out;
+    int light = -1;
+    float pdf = 1.0f;
+    float u = Clampf(select, 0.0f, kOneMinusEps);
+    if (L.use_analytic_tree && L.analytic_tree_root >= 0 && L.analytic_tree_root < L.analytic_tree_count) {
+        // Children sit after their parent (checked at upload), so the descent ends.
+        int node = L.analytic_tree_root;
+        while (node >= 0) {
+            const int4 link = AsInt4(L.alight_tree[3 * node + 2]);
+            if (link.z >= 0) {
+                light = link.z;
+                break;
+            }
+            if (link.x < 0 || link.y < 0) return out;
+            const float lw = TreeImportance(L.alight_tree + 3 * link.x, p);
+            const float rw = TreeImportance(L.alight_tree + 3 * link.y, p);
+            const float total = lw + rw;
+            const float lp = total > 0.0f ? lw / total : 0.5f;
+            if (u < lp) {
+                pdf *= lp;
+                u = lp > 0.0f ? u / lp : 0.0f;
+                node = link.x;
+            } else {
+                const float rp = 1.0f - lp;
+                pdf *= rp;
+                u = rp > 0.0f ? (u - lp) / rp : 0.0f;
+                node = link.y;
+            }
+        }
+    } else if (L.analytic_pmf_count == n) {
+        // SampleAliasTable over the analytic power table.
+        const float scaled = fminf(ClampUnitSample(u), kOneMinusEps) * static_cast<float>(n);
+        const int column = min(static_cast<int>(scaled), n - 1);
+        const float fraction = scaled - static_cast<float>(column);
+        const float prob = L.alight_prob[column];
+        light = fraction < prob ? column : L.alight_alias[column];
+        pdf = L.alight_pmf[light];
+    } else {
+        light = min(static_cast<int>(u * static_cast<float>(n)), n - 1);
+        pdf = 1.0f / static_cast<float>(n);
+    }
+    if (light < 0 || light >= n || pdf <= 0.0f) return out;
+    const float4 a = L.alight[3 * light], b = L.alight[3 * light + 1], c = L.alight[3 * light + 2];
+    const int kind = __float_as_int(a.w);
+    const float3 intensity = XYZ(c);
+    if (kind == YRH_LIGHT_DISTANT) {
+        const float3 to_light = Neg(XYZ(b));
+        if (LengthSq(to_light) == 0.0f) return out;
+        out.wi = Normalize(to_light);
+        out.distance = 1.0e6f;
+        out.radiance = intensity;
+    } else {
+        const float3 delta = Sub(XYZ(a), p);
+        const float d2 = Dot(delta, delta);
+        if (d2 <= 0.0f) return out;
+        if (kind == YRH_LIGHT_POINT) {
+            out.distance = sqrtf(d2);
+            out.wi = Normalize(delta);
+            out.radiance = Div(intensity, d2);
+        } else {
+            const float dist = sqrtf(d2);
+            const float3 wi = Div(delta, dist);
+            const float3 axis = LengthSq(XYZ(b)) > 0.0f ? Normalize(XYZ(b)) : V(0.0f, 0.0f, 1.0f);
+            const float cosine = -Dot(wi, axis);
+            const float outer = b.w, inner = c.w;
+            float falloff = 0.0f;
+            if (cosine >= inner) {
+                falloff = 1.0f;
+            } else if (cosine > outer) {
+                const float t = (cosine - outer) / (inner - outer);
+                falloff = t * t * (3.0f - 2.0f * t);
+            }
+            out.wi = wi;
+            out.distance = dist;
+            out.radiance = Mul(intensity, falloff / d2);
+        }
+    }
+    out.selection_pdf = pdf;
+    out.valid = true;
+    return out;
+}
+
 // ---- kernels -----------------------------------------------------------------------
 
 __global__ void RaygenKernel(PassView P) {
@@ -687,7 +822,7 @@ __global__ void RaygenKernel(PassView P) {
     P.p_med[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     P.p_alb[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.p_nrm[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.p_slot[p] = make_int2(-1, -1);
+    P.p_slot[p] = make_int4(-1, -1, -1, -1);
 }
 
 // Zeroes the counters a bounce appends to before anything uses them.
@@ -788,14 +923,20 @@ __device__ __forceinline__ SurfacePoint ResolveSurface(const SurfaceView& S, con
     return out;
 }
 
+// Shadow entries of a vertex, in EstimateDirectLight's summation order.
+constexpr int kShadowEmissive = 0;
+constexpr int kShadowAnalytic = 1;
+constexpr int kShadowEnv = 2;
+constexpr int kShadowKinds = 3;
+
 struct ShadeOut {
     bool next;
     float3 next_org, next_dir;
-    bool shadow[2];
-    float3 s_org[2], s_dir[2];
-    float s_tmax[2];
-    float3 s_li[2], s_bsdf[2];
-    float s_scale[2];
+    bool shadow[kShadowKinds];
+    float3 s_org[kShadowKinds], s_dir[kShadowKinds];
+    float s_tmax[kShadowKinds];
+    float3 s_li[kShadowKinds], s_bsdf[kShadowKinds];
+    float s_scale[kShadowKinds];
 };
 
 // ---- guided sampling (learned proposal, yrh_guide.h) ----------------------------
@@ -840,7 +981,7 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
                                           const yrg::GuideView& G, int depth, int i, int cur, ShadeOut& o,
                                           unsigned& hit_count, unsigned& miss_count) {
     o.next = false;
-    o.shadow[0] = o.shadow[1] = false;
+    o.shadow[kShadowEmissive] = o.shadow[kShadowAnalytic] = o.shadow[kShadowEnv] = false;
     const int p = P.q_path[cur][i];
     const float4 o4 = P.q_org[cur][i];
     const float4 d4 = P.q_dir[cur][i];
@@ -853,7 +994,7 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
     float3 tp = XYZ(P.p_tp[p]);
     float3 rad = XYZ(P.p_rad[p]);
     const float4 prev = P.p_prev[p];
-    P.p_slot[p] = make_int2(-1, -1);
+    P.p_slot[p] = make_int4(-1, -1, -1, -1);
     const int pix = P.p_pixel[p];
     PixelSampler smp;
     if (P.jobs != nullptr) {
@@ -944,16 +1085,36 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
                                     float pdf_bsdf = PdfBsdf(mat, wo, wi, sn);
                                     if (guided) pdf_bsdf = alpha * GuidePdf(gm, wi) + (1.0f - alpha) * pdf_bsdf;
                                     const float mis = PowerHeuristic(1, pdf_light, 1, pdf_bsdf);
-                                    o.shadow[0] = true;
-                                    o.s_org[0] = so;
-                                    o.s_dir[0] = wi;
-                                    o.s_tmax[0] = sd - bias;
-                                    o.s_li[0] = ls.radiance;
-                                    o.s_bsdf[0] = f;
-                                    o.s_scale[0] = c * mis / pdf_light;
+                                    o.shadow[kShadowEmissive] = true;
+                                    o.s_org[kShadowEmissive] = so;
+                                    o.s_dir[kShadowEmissive] = wi;
+                                    o.s_tmax[kShadowEmissive] = sd - bias;
+                                    o.s_li[kShadowEmissive] = ls.radiance;
+                                    o.s_bsdf[kShadowEmissive] = f;
+                                    o.s_scale[kShadowEmissive] = c * mis / pdf_light;
                                 }
                             }
                         }
+                    }
+                }
+            }
+        }
+        // Analytic lights: one delta sample, no MIS (path_direct_lighting.cpp:250-281).
+        if (L.analytic_count > 0) {
+            const AnalyticSample as = SampleAnalytic(L, hp, smp.Get1D(kDimAnalyticSelect));
+            if (as.valid && as.selection_pdf > 0.0f && !IsBlack(as.radiance)) {
+                const float c = fmaxf(0.0f, Dot(sn, as.wi));
+                if (c > 0.0f) {
+                    const float3 f = EvalBsdf(mat, wo, as.wi, sn);
+                    if (!IsBlack(f)) {
+                        const float bias = SurfaceBias(hp);
+                        o.shadow[kShadowAnalytic] = true;
+                        o.s_org[kShadowAnalytic] = Add(hp, Mul(sn, bias));
+                        o.s_dir[kShadowAnalytic] = as.wi;
+                        o.s_tmax[kShadowAnalytic] = as.distance - bias;
+                        o.s_li[kShadowAnalytic] = as.radiance;
+                        o.s_bsdf[kShadowAnalytic] = f;
+                        o.s_scale[kShadowAnalytic] = c / as.selection_pdf;
                     }
                 }
             }
@@ -968,13 +1129,13 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
                         float pdf_bsdf = PdfBsdf(mat, wo, es.dir, sn);
                         if (guided) pdf_bsdf = alpha * GuidePdf(gm, es.dir) + (1.0f - alpha) * pdf_bsdf;
                         const float mis = PowerHeuristic(1, es.pdf, 1, pdf_bsdf);
-                        o.shadow[1] = true;
-                        o.s_org[1] = Add(hp, Mul(sn, SurfaceBias(hp)));
-                        o.s_dir[1] = es.dir;
-                        o.s_tmax[1] = __int_as_float(0x7f800000);
-                        o.s_li[1] = es.radiance;
-                        o.s_bsdf[1] = f;
-                        o.s_scale[1] = c * mis / es.pdf;
+                        o.shadow[kShadowEnv] = true;
+                        o.s_org[kShadowEnv] = Add(hp, Mul(sn, SurfaceBias(hp)));
+                        o.s_dir[kShadowEnv] = es.dir;
+                        o.s_tmax[kShadowEnv] = __int_as_float(0x7f800000);
+                        o.s_li[kShadowEnv] = es.radiance;
+                        o.s_bsdf[kShadowEnv] = f;
+                        o.s_scale[kShadowEnv] = c * mis / es.pdf;
                     }
                 }
             }
@@ -1046,7 +1207,7 @@ __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S,
     const int n = static_cast<int>(P.counters[cur]);
     ShadeOut o;
     o.next = false;
-    o.shadow[0] = o.shadow[1] = false;
+    o.shadow[kShadowEmissive] = o.shadow[kShadowAnalytic] = o.shadow[kShadowEnv] = false;
     unsigned hit_count = 0, miss_count = 0;
     const bool valid = i < n;
     if (valid) ShadePath<kGuided>(P, S, L, G, depth, i, cur, o, hit_count, miss_count);
@@ -1059,24 +1220,24 @@ __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S,
         P.q_dir[nxt][slot] = F4(o.next_dir, __int_as_float(0x7f800000));
         P.q_path[nxt][slot] = p;
     }
-    int2 slots = make_int2(-1, -1);
-    int32_t s = BlockAppend(P.counters + kCountShadow, (o.shadow[0] ? 1u : 0u) + (o.shadow[1] ? 1u : 0u), append_lds);
-    for (int k = 0; k < 2; ++k) {
+    int slot_of[kShadowKinds] = {-1, -1, -1};
+    uint32_t wanted = 0;
+    for (int k = 0; k < kShadowKinds; ++k) wanted += o.shadow[k] ? 1u : 0u;
+    int32_t s = BlockAppend(P.counters + kCountShadow, wanted, append_lds);
+    for (int k = 0; k < kShadowKinds; ++k) {
         if (o.shadow[k]) {
             P.s_org[s] = F4(o.s_org[k], 1.0e-5f);
             P.s_dir[s] = F4(o.s_dir[k], o.s_tmax[k]);
             P.s_li[s] = F4(o.s_li[k], o.s_scale[k]);
             P.s_bsdf[s] = F4(o.s_bsdf[k], 0.0f);
             P.s_trans[s] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-            if (k == 0) {
-                slots.x = s;
-            } else {
-                slots.y = s;
-            }
+            slot_of[k] = s;
             ++s;
         }
     }
-    if (valid && (slots.x >= 0 || slots.y >= 0)) P.p_slot[P.q_path[cur][i]] = slots;
+    if (valid && wanted > 0) {
+        P.p_slot[P.q_path[cur][i]] = make_int4(slot_of[kShadowEmissive], slot_of[kShadowAnalytic], slot_of[kShadowEnv], -1);
+    }
     WaveStat(P.stats, kStatHits, hit_count);
     WaveStat(P.stats, kStatMisses, miss_count);
     if (i == 0) atomicAdd(P.stats + kStatRays, static_cast<unsigned long long>(n));
@@ -1253,19 +1414,19 @@ __global__ __launch_bounds__(kBlock) void GuideCollectKernel(PassView P, yrg::Gu
     }
 }
 
-// Adds the visible next-event contributions of this bounce, emissive then
-// environment, as EstimateDirectLight sums them.
+// Adds the visible next-event contributions of this bounce, emissive, then
+// analytic, then environment, as EstimateDirectLight sums them.
 __global__ __launch_bounds__(kBlock) void ResolveKernel(PassView P, yrg::GuideView G, int depth, int cur) {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const int n = static_cast<int>(P.counters[cur]);
     unsigned shadow = 0, occluded = 0;
     if (i < n) {
         const int p = P.q_path[cur][i];
-        const int2 slots = P.p_slot[p];
-        if (slots.x >= 0 || slots.y >= 0) {
+        const int4 slots = P.p_slot[p];
+        if (slots.x >= 0 || slots.y >= 0 || slots.z >= 0) {
             float3 direct = V(0.0f, 0.0f, 0.0f);
-            for (int k = 0; k < 2; ++k) {
-                const int s = k == 0 ? slots.x : slots.y;
+            for (int k = 0; k < kShadowKinds; ++k) {
+                const int s = k == kShadowEmissive ? slots.x : (k == kShadowAnalytic ? slots.y : slots.z);
                 if (s < 0) continue;
                 ++shadow;
                 if (P.s_occ[s] != 0u) {
@@ -1274,7 +1435,7 @@ __global__ __launch_bounds__(kBlock) void ResolveKernel(PassView P, yrg::GuideVi
                 }
                 const float4 li = P.s_li[s];
                 const float3 c = Mul(Hadamard(XYZ(P.s_bsdf[s]), Hadamard(XYZ(P.s_trans[s]), XYZ(li))), li.w);
-                direct = k == 0 ? c : Add(direct, c);
+                direct = Add(direct, c);
             }
             const float3 tp = XYZ(P.p_nee[p]);
             const float3 rad = Add(XYZ(P.p_rad[p]), Hadamard(tp, direct));
@@ -1574,7 +1735,7 @@ PassView MakePassView(RenderState& R, const yrh_render_desc& d) {
     P.p_alb = R.p_alb.as<float4>();
     P.p_nrm = R.p_nrm.as<float4>();
     P.p_nee = R.p_nee.as<float4>();
-    P.p_slot = R.p_slot.as<int2>();
+    P.p_slot = R.p_slot.as<int4>();
     for (int k = 0; k < 2; ++k) {
         P.q_org[k] = R.q_org[k].as<float4>();
         P.q_dir[k] = R.q_dir[k].as<float4>();
@@ -1633,6 +1794,16 @@ LightView MakeLightView(RenderState& R) {
     L.alias_pmf = R.alias_pmf.as<float>();
     L.emissive_count = R.emissive_count;
     L.total_emissive_area = R.total_emissive_area;
+    L.alight = R.alight.as<float4>();
+    L.alight_prob = R.alight_prob.as<float>();
+    L.alight_alias = R.alight_alias.as<int32_t>();
+    L.alight_pmf = R.alight_pmf.as<float>();
+    L.alight_tree = R.alight_tree.as<float4>();
+    L.analytic_count = R.analytic_count;
+    L.analytic_pmf_count = R.analytic_pmf_count;
+    L.analytic_tree_count = R.analytic_tree_count;
+    L.analytic_tree_root = R.analytic_tree_root;
+    L.use_analytic_tree = R.use_analytic_tree;
     L.env_active = R.env_active;
     L.env_radiance = make_float3(R.env_radiance[0], R.env_radiance[1], R.env_radiance[2]);
     L.env_strength = R.env_strength;
@@ -1698,22 +1869,23 @@ __global__ void BsdfQueryKernel(const DevMaterial* mats, int64_t n, const int32_
 
 // Bytes per path slot and per shadow slot of the arrays EnsureCapacity
 // allocates, and per guided row (mixture + valid flag).
-constexpr size_t kPathSlotBytes = 4 + 4 + 4 + 16 + 16 + 16 + 4 + 16 + 16 + 16 + 16 + 8 + 2 * 16 + 2 * 16 + 2 * 4 + 16 + 4;
+constexpr size_t kPathSlotBytes = 4 + 4 + 4 + 16 + 16 + 16 + 4 + 16 + 16 + 16 + 16 + 16 + 2 * 16 + 2 * 16 + 2 * 4 + 16 + 4;
 constexpr size_t kShadowSlotBytes = 16 + 16 + 16 + 16 + 4 + 16 + 16 + 16;
 constexpr size_t kGuideRowBytes = yrg::kGuideMixtureRow * 4 + 4;
 
 int EnsureCapacity(yrh_context* ctx, RenderState& R, int64_t paths) {
-    if (R.capacity >= paths && R.counters.ptr != nullptr) return YRH_OK;
+    if (R.capacity >= paths && R.cap_shadow_slots >= R.shadow_slots && R.counters.ptr != nullptr) return YRH_OK;
     int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     if (rc != YRH_OK) return rc;
+    paths = std::max(paths, R.capacity);
     const size_t P = static_cast<size_t>(paths);
-    const size_t S = 2 * P;
+    const size_t S = static_cast<size_t>(R.shadow_slots) * P;
     struct Item {
         DevBuf* buf;
         size_t bytes;
     } items[] = {
         {&R.p_pixel, 4 * P}, {&R.p_job, 4 * P}, {&R.p_sample, 4 * P}, {&R.p_tp, 16 * P}, {&R.p_rad, 16 * P}, {&R.p_prev, 16 * P}, {&R.p_flags, 4 * P},
-        {&R.p_med, 16 * P}, {&R.p_alb, 16 * P}, {&R.p_nrm, 16 * P}, {&R.p_nee, 16 * P}, {&R.p_slot, 8 * P},
+        {&R.p_med, 16 * P}, {&R.p_alb, 16 * P}, {&R.p_nrm, 16 * P}, {&R.p_nee, 16 * P}, {&R.p_slot, 16 * P},
         {&R.q_org[0], 16 * P}, {&R.q_org[1], 16 * P}, {&R.q_dir[0], 16 * P}, {&R.q_dir[1], 16 * P},
         {&R.q_path[0], 4 * P}, {&R.q_path[1], 4 * P}, {&R.hits, 16 * P}, {&R.hit_inst, 4 * P},
         {&R.s_org, 16 * S}, {&R.s_dir, 16 * S}, {&R.s_li, 16 * S}, {&R.s_bsdf, 16 * S}, {&R.s_occ, 4 * S},
@@ -1731,6 +1903,7 @@ int EnsureCapacity(yrh_context* ctx, RenderState& R, int64_t paths) {
     }
     rc = CheckHip(ctx, hipMemsetAsync(R.counters.ptr, 0, 4 * kCounterWords, ctx->stream), "hipMemsetAsync");
     R.capacity = paths;
+    R.cap_shadow_slots = R.shadow_slots;
     return rc;
 }
 
@@ -1810,7 +1983,7 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
             sr.org = P.s_org;
             sr.dir = P.s_dir;
             sr.n_device = P.counters + kCountShadow;
-            sr.capacity = 2 * paths;
+            sr.capacity = R.shadow_slots * paths;
             sr.expected = paths;  // about one shadow ray per path in flight
             sr.work_counter = P.counters + kWorkShadow;
             sr.parts = trav::kParts;
@@ -1824,7 +1997,7 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
             wr.org = P.s_org;
             wr.dir = P.s_dir;
             wr.n_device = P.counters + kCountShadow;
-            wr.capacity = 2 * paths;
+            wr.capacity = R.shadow_slots * paths;
             wr.expected = paths;
             wr.work_counter = P.counters + kWorkShadow;
             wr.parts = trav::kParts;
@@ -2054,6 +2227,63 @@ int yrh_upload_shading(yrh_context* ctx, const yrh_shading_desc* sd) {
     if ((rc = UploadVec(ctx, R.alias_prob, sd->alias_probability, static_cast<size_t>(ne))) != YRH_OK) return rc;
     if ((rc = UploadVec(ctx, R.alias_index, sd->alias_index, static_cast<size_t>(ne))) != YRH_OK) return rc;
     if ((rc = UploadVec(ctx, R.alias_pmf, sd->alias_pmf, static_cast<size_t>(ne))) != YRH_OK) return rc;
+    // Analytic lights and their selection tables (alias table, light tree).
+    {
+        const auto bits = [](int32_t v) {
+            float f;
+            std::memcpy(&f, &v, 4);
+            return f;
+        };
+        const int32_t na = std::max(0, sd->analytic_light_count);
+        if (na > 0 && sd->analytic_lights == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "analytic lights missing");
+        const bool has_alias = na > 0 && sd->analytic_alias_probability != nullptr &&
+                               sd->analytic_alias_index != nullptr && sd->analytic_alias_pmf != nullptr;
+        if (has_alias) {
+            for (int32_t k = 0; k < na; ++k) {
+                if (sd->analytic_alias_index[k] < 0 || sd->analytic_alias_index[k] >= na) {
+                    return Fail(ctx, YRH_E_INVALID_ARGUMENT, "analytic alias table index out of range");
+                }
+            }
+        }
+        const int32_t nt = na > 0 && sd->use_analytic_tree && sd->analytic_tree != nullptr
+                               ? std::max(0, sd->analytic_tree_count) : 0;
+        std::vector<float4> lights(3 * static_cast<size_t>(na)), tree(3 * static_cast<size_t>(nt));
+        for (int32_t k = 0; k < na; ++k) {
+            const yrh_analytic_light& l = sd->analytic_lights[k];
+            if (l.kind < YRH_LIGHT_POINT || l.kind > YRH_LIGHT_DISTANT) {
+                return Fail(ctx, YRH_E_INVALID_ARGUMENT, "unknown analytic light kind");
+            }
+            float4* o = lights.data() + 3 * static_cast<size_t>(k);
+            o[0] = make_float4(l.position[0], l.position[1], l.position[2], bits(l.kind));
+            o[1] = make_float4(l.direction[0], l.direction[1], l.direction[2], l.cone_angle);
+            o[2] = make_float4(l.intensity[0], l.intensity[1], l.intensity[2], l.cone_cos_inner);
+        }
+        for (int32_t k = 0; k < nt; ++k) {
+            const yrh_light_tree_node& t = sd->analytic_tree[k];
+            // A leaf names a light; an interior node's children come after it
+            // (the reference's recursive build order), so a descent ends.
+            if (t.light_index >= na || (t.light_index < 0 && ((t.left >= 0 && (t.left <= k || t.left >= nt)) ||
+                                                               (t.right >= 0 && (t.right <= k || t.right >= nt))))) {
+                return Fail(ctx, YRH_E_INVALID_ARGUMENT, "analytic light tree node out of range");
+            }
+            float4* o = tree.data() + 3 * static_cast<size_t>(k);
+            o[0] = make_float4(t.bounds_min[0], t.bounds_min[1], t.bounds_min[2], t.power);
+            o[1] = make_float4(t.bounds_max[0], t.bounds_max[1], t.bounds_max[2], 0.0f);
+            o[2] = make_float4(bits(t.left), bits(t.right), bits(t.light_index), 0.0f);
+        }
+        if ((rc = UploadVec(ctx, R.alight, lights.data(), lights.size())) != YRH_OK) return rc;
+        if ((rc = UploadVec(ctx, R.alight_tree, tree.data(), tree.size())) != YRH_OK) return rc;
+        const size_t nalias = has_alias ? static_cast<size_t>(na) : 0;
+        if ((rc = UploadVec(ctx, R.alight_prob, sd->analytic_alias_probability, nalias)) != YRH_OK) return rc;
+        if ((rc = UploadVec(ctx, R.alight_alias, sd->analytic_alias_index, nalias)) != YRH_OK) return rc;
+        if ((rc = UploadVec(ctx, R.alight_pmf, sd->analytic_alias_pmf, nalias)) != YRH_OK) return rc;
+        R.analytic_count = na;
+        R.analytic_pmf_count = static_cast<int32_t>(nalias);
+        R.analytic_tree_count = nt;
+        R.analytic_tree_root = nt > 0 ? sd->analytic_tree_root : -1;
+        R.use_analytic_tree = nt > 0 ? 1 : 0;
+        R.shadow_slots = na > 0 ? 3 : 2;
+    }
     // Primitive -> first emitter record (the reference's linear scan).
     {
         std::vector<int4> info(static_cast<size_t>(R.primitive_count));
@@ -2219,8 +2449,9 @@ int yrh_suggest_paths_per_pass(yrh_context* ctx, int32_t max_depth, int64_t* pat
     size_t free_bytes = 0, total_bytes = 0;
     int rc = CheckHip(ctx, hipMemGetInfo(&free_bytes, &total_bytes), "hipMemGetInfo");
     if (rc != YRH_OK) return rc;
-    // Per path: the path slot state (EnsureCapacity) and its two shadow slots.
-    size_t per_path = kPathSlotBytes + 2 * kShadowSlotBytes;
+    // Per path: the path slot state (EnsureCapacity) and its shadow slots.
+    const int shadow_slots = ctx->render != nullptr ? ctx->render->shadow_slots : 2;
+    size_t per_path = kPathSlotBytes + static_cast<size_t>(shadow_slots) * kShadowSlotBytes;
     if (ctx->guide != nullptr) per_path += kGuideRowBytes + 64 * static_cast<size_t>(std::max(1, max_depth));
     const size_t held = ctx->render != nullptr ? static_cast<size_t>(ctx->render->capacity) * per_path : 0;
     const int64_t fit = static_cast<int64_t>((free_bytes + held) / 2 / per_path);
@@ -2382,7 +2613,7 @@ int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* d, float* job_ms, i
         warm.org = P.s_org;
         warm.dir = P.s_dir;
         warm.n_device = P.counters + kCountShadow;
-        warm.capacity = 2 * capacity;
+        warm.capacity = R.shadow_slots * capacity;
         trav::TraceArgs wa;
         int g = 0, b = 0;
         if ((rc = PrepareTraceArgs(ctx, warm, wa, g, b)) != YRH_OK) return rc;

@@ -53,6 +53,7 @@ enum Dim : uint32_t {
     kDimLightSurface = 3,
     kDimBsdf = 5,
     kDimRoulette = 7,
+    kDimAnalyticSelect = 14,
     kDimGuide = 15,  // guide-or-BSDF choice (and lobe) of the guided proposal; the reference leaves it unused
 };
 

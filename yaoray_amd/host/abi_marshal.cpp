@@ -21,6 +21,17 @@ static_assert(sizeof(yr::RenderSphere) == sizeof(yrh_sphere));
 static_assert(offsetof(yr::RenderSphere, radius) == offsetof(yrh_sphere, radius));
 static_assert(offsetof(yr::RenderSphere, material_index) == offsetof(yrh_sphere, material_index));
 static_assert(offsetof(yr::RenderSphere, flip_normals) == offsetof(yrh_sphere, flip_normals));
+static_assert(sizeof(yr::AnalyticLight) == sizeof(yrh_analytic_light));
+static_assert(offsetof(yr::AnalyticLight, position) == offsetof(yrh_analytic_light, position));
+static_assert(offsetof(yr::AnalyticLight, direction) == offsetof(yrh_analytic_light, direction));
+static_assert(offsetof(yr::AnalyticLight, intensity) == offsetof(yrh_analytic_light, intensity));
+static_assert(offsetof(yr::AnalyticLight, cone_cos_inner) == offsetof(yrh_analytic_light, cone_cos_inner));
+static_assert(static_cast<int>(yr::AnalyticLightKind::Point) == YRH_LIGHT_POINT);
+static_assert(static_cast<int>(yr::AnalyticLightKind::Spot) == YRH_LIGHT_SPOT);
+static_assert(static_cast<int>(yr::AnalyticLightKind::Distant) == YRH_LIGHT_DISTANT);
+static_assert(sizeof(yr::LightTreeNode) == sizeof(yrh_light_tree_node));
+static_assert(offsetof(yr::LightTreeNode, power) == offsetof(yrh_light_tree_node, power));
+static_assert(offsetof(yr::LightTreeNode, light_index) == offsetof(yrh_light_tree_node, light_index));
 
 yrh_bvh_desc DescribeBvh(const yr::RenderBvh& b) {
     yrh_bvh_desc d{};
@@ -112,6 +123,18 @@ std::string DescribeShading(const yr::RenderSceneIR& ir, Marshal& m, yrh_shading
     d.alias_probability = power.probability.data();
     d.alias_index = power.alias.data();
     d.alias_pmf = power.pmf.data();
+    const yr::LightSamplingCache& cache = ir.light_sampling;
+    d.analytic_lights = reinterpret_cast<const yrh_analytic_light*>(ir.analytic_lights.data());
+    d.analytic_light_count = static_cast<int32_t>(ir.analytic_lights.size());
+    if (cache.analytic_power.pmf.size() == ir.analytic_lights.size()) {
+        d.analytic_alias_probability = cache.analytic_power.probability.data();
+        d.analytic_alias_index = cache.analytic_power.alias.data();
+        d.analytic_alias_pmf = cache.analytic_power.pmf.data();
+    }
+    d.analytic_tree = reinterpret_cast<const yrh_light_tree_node*>(cache.analytic_tree.data());
+    d.analytic_tree_count = static_cast<int32_t>(cache.analytic_tree.size());
+    d.analytic_tree_root = cache.analytic_tree_root;
+    d.use_analytic_tree = cache.use_analytic_tree ? 1 : 0;
     yrh_environment_desc& e = d.environment;
     const yr::RenderEnvironment& env = ir.environment;
     e.active = env.active ? 1 : 0;

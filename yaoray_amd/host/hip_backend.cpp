@@ -235,22 +235,23 @@ RenderResult HipBackend::Render(const PreparedScene& scene, const RenderRequest&
         result.stats.samples_rendered = film.TotalSampleCount();
         result.stats.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     };
-    // Adaptive sampling (RenderCpuPathTrace's active-pixel loop,
-    // cpu_path_tracer.cpp:95-106, 183-199): the device keeps the list of
-    // pixels below spp samples that Film::IsConverged has not stopped, and
-    // each pass renders the next sample of every listed pixel.
+    // RenderCpuPathTrace's active-pixel loop (cpu_path_tracer.cpp:95-106,
+    // 120-157, 183-199): the device keeps the list of pixels below spp samples
+    // (that Film::IsConverged has not stopped, with adaptive sampling), and
+    // each pass renders the next sample of every listed pixel, its index the
+    // pixel's film sample count. A non-finite sample is not added, so that
+    // pixel renders the same index again, as in the reference.
     const AdaptiveSamplingSettings& as = settings.adaptive_sampling;
     const bool adaptive = as.enabled;
-    const yrh_adaptive_desc ad{1, as.min_spp, as.relative_error, as.absolute_error, as.confidence};
+    const yrh_adaptive_desc ad{adaptive ? 1 : 0, as.min_spp, as.relative_error, as.absolute_error, as.confidence};
     int64_t active = npix, converged = 0;
-    if (adaptive) {
-        if (yrh_film_update_active(ctx, &ad, spp, &active, &converged) != YRH_OK) return fail(LastError(ctx, "active pixels"));
-        result.stats.converged_pixels = static_cast<std::uint64_t>(converged);
-        d.pixel_list = 1;
-    }
-    // Otherwise, without a progress callback nothing observes the film between
-    // passes, so consecutive passes are batched into one wavefront as large as
-    // the device memory suggests (same per-pixel sample order, so the same film).
+    if (yrh_film_update_active(ctx, &ad, spp, &active, &converged) != YRH_OK) return fail(LastError(ctx, "active pixels"));
+    result.stats.converged_pixels = static_cast<std::uint64_t>(converged);
+    d.pixel_list = 1;
+    // Without adaptive sampling or a progress callback nothing observes the
+    // film between passes, so consecutive passes are batched into one
+    // wavefront as large as the device memory suggests (each listed pixel's
+    // samples are added in index order, so the film is the same).
     int64_t batch_paths = int64_t{1} << 23;
     if (!request.progress_callback && yrh_suggest_paths_per_pass(ctx, settings.max_depth, &batch_paths) != YRH_OK) {
         return fail(LastError(ctx, "suggest paths per pass"));
@@ -284,14 +285,10 @@ RenderResult HipBackend::Render(const PreparedScene& scene, const RenderRequest&
         }
         d.sample_count = std::min(take, spp - pass);
         if (yrh_render_pass(ctx, &d) != YRH_OK) return fail(LastError(ctx, "render pass"));
-        if (adaptive) {
-            if (yrh_film_update_active(ctx, &ad, spp, &active, &converged) != YRH_OK) {
-                return fail(LastError(ctx, "active pixels"));
-            }
-            result.stats.converged_pixels = static_cast<std::uint64_t>(converged);
-        } else {
-            active = pass + d.sample_count < spp ? npix : 0;
+        if (yrh_film_update_active(ctx, &ad, spp, &active, &converged) != YRH_OK) {
+            return fail(LastError(ctx, "active pixels"));
         }
+        if (adaptive) result.stats.converged_pixels = static_cast<std::uint64_t>(converged);
         if (request.progress_callback) {
             if (!sync_film()) return fail(LastError(ctx, "film read"));
             fill_stats();

@@ -162,6 +162,24 @@ int yrhost_scene_set_emissives(yrhost_scene* s, const yrh_emissive* e, int32_t c
     return YRH_OK;
 }
 
+int yrhost_scene_set_analytic_lights(yrhost_scene* s, const yrh_analytic_light* l, int32_t count) {
+    if (s == nullptr || count < 0 || (count > 0 && l == nullptr)) return YRH_E_INVALID_ARGUMENT;
+    s->ir.analytic_lights.clear();
+    for (int32_t k = 0; k < count; ++k) {
+        if (l[k].kind < YRH_LIGHT_POINT || l[k].kind > YRH_LIGHT_DISTANT) return YRH_E_INVALID_ARGUMENT;
+        yr::AnalyticLight a;
+        a.kind = static_cast<yr::AnalyticLightKind>(l[k].kind);
+        a.position = yr::Point3f{l[k].position[0], l[k].position[1], l[k].position[2]};
+        a.direction = yr::Vec3f{l[k].direction[0], l[k].direction[1], l[k].direction[2]};
+        a.intensity = yr::Color3f{l[k].intensity[0], l[k].intensity[1], l[k].intensity[2]};
+        a.cone_angle = l[k].cone_angle;
+        a.cone_cos_inner = l[k].cone_cos_inner;
+        s->ir.analytic_lights.push_back(a);
+    }
+    yr::PrepareLightSampling(s->ir);
+    return YRH_OK;
+}
+
 int yrhost_scene_set_constant_environment(yrhost_scene* s, const float* radiance3, float strength) {
     if (s == nullptr || radiance3 == nullptr) return YRH_E_INVALID_ARGUMENT;
     s->ir.environment = yr::RenderEnvironment{};
@@ -189,6 +207,7 @@ int yrhost_scene_info_get(const yrhost_scene* s, yrhost_scene_info* info) {
     info->accel_built = s->accel_built ? 1 : 0;
     info->environment_active = s->ir.environment.active ? 1 : 0;
     info->sphere_count = static_cast<int32_t>(s->ir.spheres.size());
+    info->analytic_light_count = static_cast<int32_t>(s->ir.analytic_lights.size());
     if (s->accel_built) {
         info->accel_kind = s->accel.kind == yr::RenderAccelerationKind::TwoLevel ? YRH_ACCEL_TWO_LEVEL : YRH_ACCEL_FLAT;
         info->node_count = s->accel.NodeCount();

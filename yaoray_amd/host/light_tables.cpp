@@ -1,5 +1,6 @@
 // Light-sampling tables built on the host and consumed as-is by the device:
 //   emissive power alias table   src/lighting/light_sampling.cpp:148-216
+//   analytic power alias table + analytic light tree   :206-269
 //   environment distribution     src/lighting/environment.cpp:73-123
 // Both feed pdfs that the MIS weights compare against, so they are built with
 // the reference's operation order (Vose's alias method with the same small /
@@ -56,11 +57,92 @@ AliasTable BuildAliasTable(std::span<const float> weights) {
     return t;
 }
 
+namespace {
+
+// Analytic light tree over the local (point / spot) lights: nodes in
+// pre-order, each splitting its lights at the median position along the
+// widest axis of their positions (src/lighting/light_sampling.cpp:235-268).
+// The node bounds and powers are what light selection's importance reads,
+// and the split order fixes which light each descent reaches, so the steps
+// are the reference's.
+struct AnalyticTreeBuilder {
+    const std::vector<AnalyticLight>& lights;
+    const std::vector<float>& power;
+    std::vector<int>& order;
+    std::vector<LightTreeNode>& nodes;
+
+    int Build(size_t begin, size_t end) {
+        Bounds3f bounds, centroids;
+        float sum = 0.0f;
+        for (size_t i = begin; i < end; ++i) {
+            const int l = order[i];
+            const Point3f p = lights[static_cast<size_t>(l)].position;
+            bounds = Union(bounds, p);
+            centroids = Union(centroids, p);
+            sum += power[static_cast<size_t>(l)];
+        }
+        const int index = static_cast<int>(nodes.size());
+        nodes.push_back(LightTreeNode{bounds, sum});
+        if (end - begin == 1) {
+            nodes[static_cast<size_t>(index)].light_index = order[begin];
+            return index;
+        }
+        const Vec3f e = centroids.max - centroids.min;
+        const int axis = e.x >= e.y && e.x >= e.z ? 0 : (e.y >= e.z ? 1 : 2);
+        const size_t mid = begin + (end - begin) / 2;
+        const auto key = [&](int l) {
+            const Point3f p = lights[static_cast<size_t>(l)].position;
+            return axis == 0 ? p.x : (axis == 1 ? p.y : p.z);
+        };
+        std::nth_element(order.begin() + static_cast<std::ptrdiff_t>(begin),
+                         order.begin() + static_cast<std::ptrdiff_t>(mid),
+                         order.begin() + static_cast<std::ptrdiff_t>(end), [&](int a, int b) { return key(a) < key(b); });
+        const int left = Build(begin, mid);
+        const int right = Build(mid, end);
+        nodes[static_cast<size_t>(index)].left = left;
+        nodes[static_cast<size_t>(index)].right = right;
+        return index;
+    }
+
+    static Bounds3f Union(const Bounds3f& b, Point3f p) {
+        return Bounds3f{Point3f{std::min(b.min.x, p.x), std::min(b.min.y, p.y), std::min(b.min.z, p.z)},
+                        Point3f{std::max(b.max.x, p.x), std::max(b.max.y, p.y), std::max(b.max.z, p.z)}};
+    }
+};
+
+}  // namespace
+
 void PrepareLightSampling(RenderSceneIR& scene) {
     std::vector<float> w;
     w.reserve(scene.emissive_primitives.size());
     for (const EmissivePrimitive& ep : scene.emissive_primitives) w.push_back(EmissiveArea(ep) * Luminance(ep.radiance));
-    scene.light_sampling.emissive_power = BuildAliasTable(w);
+    LightSamplingCache& cache = scene.light_sampling;
+    cache.emissive_power = BuildAliasTable(w);
+
+    // Analytic power: luminance x solid-angle extent (4 pi for a point light,
+    // the cone's 2 pi (1 - cos) for a spot light, 1 for a distant light).
+    std::vector<float> power;
+    std::vector<int> local;
+    power.reserve(scene.analytic_lights.size());
+    for (size_t i = 0; i < scene.analytic_lights.size(); ++i) {
+        const AnalyticLight& l = scene.analytic_lights[i];
+        float p = Luminance(l.intensity);
+        if (l.kind == AnalyticLightKind::Spot) {
+            p *= std::max(0.0f, 2.0f * kPi * (1.0f - l.cone_angle));
+        } else if (l.kind == AnalyticLightKind::Point) {
+            p *= 4.0f * kPi;
+        }
+        power.push_back(p);
+        if (l.kind != AnalyticLightKind::Distant && p > 0.0f) local.push_back(static_cast<int>(i));
+    }
+    cache.analytic_power = BuildAliasTable(power);
+    cache.analytic_tree.clear();
+    cache.analytic_tree_root = -1;
+    cache.use_analytic_tree = local.size() == scene.analytic_lights.size() && local.size() > 8;
+    if (!cache.use_analytic_tree) return;
+    cache.analytic_tree.reserve(local.size() * 2);
+    AnalyticTreeBuilder b{scene.analytic_lights, power, local, cache.analytic_tree};
+    cache.analytic_tree_root = b.Build(0, local.size());
 }
 
 RenderEnvironmentDistribution BuildEnvironmentDistribution(const RenderTexture& tex) {

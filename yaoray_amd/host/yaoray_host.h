@@ -41,6 +41,7 @@ typedef struct yrhost_scene_info {
     int32_t environment_active;
     float camera[13];       /* origin, forward, right, up, fov_y_radians */
     int32_t sphere_count;
+    int32_t analytic_light_count;
 } yrhost_scene_info;
 
 int yrhost_scene_create(int kind, const int64_t* params, int param_count, uint64_t seed, yrhost_scene** out);
@@ -67,6 +68,9 @@ int yrhost_scene_set_constant_environment(yrhost_scene* scene, const float* radi
 int yrhost_scene_set_spheres(yrhost_scene* scene, const yrh_sphere* spheres, int32_t count);
 /* Read-only view of the spheres (valid until the scene changes). */
 int yrhost_scene_spheres(const yrhost_scene* scene, const yrh_sphere** spheres, int32_t* count);
+/* Replaces the analytic (point / spot / distant) lights (copied) and rebuilds
+ * their selection tables (power alias table, light tree). */
+int yrhost_scene_set_analytic_lights(yrhost_scene* scene, const yrh_analytic_light* lights, int32_t count);
 
 /* Read-only views of the scene arrays (valid until the scene is destroyed). */
 int yrhost_scene_geometry(const yrhost_scene* scene, const yrh_vertex** vertices, const uint32_t** indices,

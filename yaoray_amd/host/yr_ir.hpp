@@ -180,9 +180,35 @@ struct AliasTable {
     std::vector<float> pmf;
 };
 
-// LightSamplingCache (light.hpp:61-67): the emissive-power alias table.
+// AnalyticLight (light.hpp:36-45): point, spot or distant light.
+enum class AnalyticLightKind { Point, Spot, Distant };
+
+struct AnalyticLight {
+    AnalyticLightKind kind = AnalyticLightKind::Point;
+    Point3f position;
+    Vec3f direction;
+    Color3f intensity;
+    float cone_angle = 0.0f;      // cosine of the outer cone
+    float cone_cos_inner = 0.0f;  // cosine of the full-intensity cone
+};
+
+// LightTreeNode (light.hpp:53-59).
+struct LightTreeNode {
+    Bounds3f bounds;
+    float power = 0.0f;
+    int left = -1;
+    int right = -1;
+    int light_index = -1;
+};
+
+// LightSamplingCache (light.hpp:61-67): the emissive-power alias table, the
+// analytic-power alias table and the analytic light tree.
 struct LightSamplingCache {
     AliasTable emissive_power;
+    AliasTable analytic_power;
+    std::vector<LightTreeNode> analytic_tree;
+    int analytic_tree_root = -1;
+    bool use_analytic_tree = false;
 };
 
 struct RenderTexture {
@@ -253,6 +279,7 @@ struct RenderSceneIR {
     std::vector<RenderMaterial> materials;
     std::vector<RenderTexture> textures;
     std::vector<EmissivePrimitive> emissive_primitives;
+    std::vector<AnalyticLight> analytic_lights;
     RenderEnvironment environment;
     std::vector<RenderEnvironmentDistribution> environment_distributions;
     LightSamplingCache light_sampling;

@@ -112,10 +112,24 @@ class EnvironmentDesc(C.Structure):
                 ("row_cdf", C.c_void_p), ("conditional_cdfs", C.c_void_p), ("total_weight", C.c_float)]
 
 
+class AnalyticLight(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("position", C.c_float * 3), ("direction", C.c_float * 3),
+                ("intensity", C.c_float * 3), ("cone_angle", C.c_float), ("cone_cos_inner", C.c_float)]
+
+
+class LightTreeNode(C.Structure):
+    _fields_ = [("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3), ("power", C.c_float),
+                ("left", C.c_int32), ("right", C.c_int32), ("light_index", C.c_int32)]
+
+
 class ShadingDesc(C.Structure):
     _fields_ = [("materials", C.c_void_p), ("material_count", C.c_int32), ("emissives", C.c_void_p),
                 ("emissive_count", C.c_int32), ("alias_probability", C.c_void_p), ("alias_index", C.c_void_p),
-                ("alias_pmf", C.c_void_p), ("environment", EnvironmentDesc)]
+                ("alias_pmf", C.c_void_p), ("environment", EnvironmentDesc), ("analytic_lights", C.c_void_p),
+                ("analytic_light_count", C.c_int32), ("analytic_alias_probability", C.c_void_p),
+                ("analytic_alias_index", C.c_void_p), ("analytic_alias_pmf", C.c_void_p),
+                ("analytic_tree", C.c_void_p), ("analytic_tree_count", C.c_int32),
+                ("analytic_tree_root", C.c_int32), ("use_analytic_tree", C.c_int32)]
 
 
 class RenderDesc(C.Structure):
@@ -149,7 +163,7 @@ class SceneInfo(C.Structure):
                 ("instance_count", C.c_int32), ("material_count", C.c_int32), ("emissive_count", C.c_int32),
                 ("accel_built", C.c_int32), ("accel_kind", C.c_int32), ("node_count", C.c_int32),
                 ("max_depth", C.c_int32), ("blas_count", C.c_int32), ("environment_active", C.c_int32),
-                ("camera", C.c_float * 13), ("sphere_count", C.c_int32)]
+                ("camera", C.c_float * 13), ("sphere_count", C.c_int32), ("analytic_light_count", C.c_int32)]
 
 
 class BatchJob(C.Structure):
@@ -279,6 +293,7 @@ HOST_SIGNATURES = {
     "yrhost_scene_set_camera": (_I, [_VP, _VP]),
     "yrhost_scene_set_materials": (_I, [_VP, _VP, _I32]),
     "yrhost_scene_set_spheres": (_I, [_VP, _VP, _I32]),
+    "yrhost_scene_set_analytic_lights": (_I, [_VP, _VP, _I32]),
     "yrhost_scene_spheres": (_I, [_VP, C.POINTER(_VP), C.POINTER(_I32)]),
     "yrhost_scene_set_emissives": (_I, [_VP, _VP, _I32]),
     "yrhost_scene_set_constant_environment": (_I, [_VP, _VP, _F]),
