@@ -4,8 +4,11 @@ in test_analytic_lights.py), through the C ABI.
 
   * depth-1 renders (camera hit + next-event estimation of the emissive,
     analytic and environment lights; light selection by alias table or light
-    tree uses + - * / only): every counter, sample count and depth AOV equal,
-    radiance within 1e-5 relative (the conductor's Schlick term calls pow);
+    tree uses + - * / only): ray, hit, shadow and occlusion counters, sample
+    counts and depth AOVs equal, radiance within 1e-5 relative (the
+    conductor's Schlick term calls pow); traversal counters equal where shadow
+    rays walk closest hits (scenes with glass), not above the oracle's where
+    they stop at their first hit (opaque scenes);
   * full renders: statistical parity (image mean within 0.5 %, per-pixel RMSE
     below 0.35 x the oracle's per-pixel standard error, counters within 1 %).
 """
@@ -34,9 +37,13 @@ def test_depth1_analytic_render_matches_oracle(device, kind):
     print(f"depth-1 analytic render {kind} 64x48x2", flush=True)
     g, gs = gpu_render(device, s, 64, 48, 2, max_depth=1)
     o, os_ = O.render(s, 64, 48, 2, max_depth=1)
-    for k in ("rays_traced", "hits", "misses", "shadow_rays", "occluded_shadow_rays", "bad_samples",
-              "triangle_tests", "sphere_tests", "bvh_node_tests"):
+    for k in ("rays_traced", "hits", "misses", "shadow_rays", "occluded_shadow_rays", "bad_samples"):
         assert gs[k] == os_[k], k
+    for k in ("triangle_tests", "sphere_tests", "bvh_node_tests"):
+        if kind == "cornell_spheres":  # glass: shadow rays walk closest hits, as the reference's do
+            assert gs[k] == os_[k], k
+        else:  # opaque: shadow rays stop at their first hit (same visibility, fewer tests)
+            assert gs[k] <= os_[k], k
     assert np.array_equal(g["samples"], o["samples"]) and np.array_equal(g["aov_samples"], o["aov_samples"])
     assert np.array_equal(g["depth_sum"].astype(np.float32), o["depth_sum"])
     assert np.allclose(g["sum"], o["sum"], rtol=1e-5, atol=1e-7)
