@@ -614,6 +614,18 @@ int yrh_upload_geometry(yrh_context* ctx, const yrh_geometry_desc* g) {
         geo.empty = packed.empty;
     }
     geo.accel_kind = g->accel_kind;
+    // Sphere slots exist if any flat-BVH or TLAS primitive is a sphere (an
+    // invalid sphere handle still counts a sphere test, as in the reference).
+    geo.has_spheres = g->sphere_count > 0;
+    if (g->accel_kind == YRH_ACCEL_FLAT) {
+        for (int32_t k = 0; !geo.has_spheres && k < g->flat.primitive_count; ++k) {
+            geo.has_spheres = g->flat.primitives[k].kind != 0;
+        }
+    } else {
+        for (int32_t k = 0; !geo.has_spheres && k < g->tlas_primitive_count; ++k) {
+            geo.has_spheres = g->tlas_primitives[k].kind != 0;
+        }
+    }
     geo.slot_count = static_cast<int64_t>(packed.leaf_count.size());
     if ((rc = Upload(ctx, geo.nodes, packed.nodes)) != YRH_OK) return rc;
     if ((rc = Upload(ctx, geo.slots, packed.slots)) != YRH_OK) return rc;

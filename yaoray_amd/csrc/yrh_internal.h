@@ -109,6 +109,7 @@ struct DevGeometry {
     float root_lo[3] = {0, 0, 0};
     float root_hi[3] = {0, 0, 0};
     bool empty = true;
+    bool has_spheres = false;  // sphere slots exist: launch the traversal variant that tests them
     // two-level
     DevBuf tlas_nodes;
     DevBuf tlas_leaf_count;

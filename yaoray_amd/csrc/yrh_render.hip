@@ -976,7 +976,8 @@ __device__ __forceinline__ BsdfSample SampleGuided(const DevMaterial& m, float3 
     return b;
 }
 
-template <bool kGuided>
+// kAnalytic: the scene has analytic lights (a third shadow entry per vertex).
+template <bool kGuided, bool kAnalytic>
 __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& S, const LightView& L,
                                           const yrg::GuideView& G, int depth, int i, int cur, ShadeOut& o,
                                           unsigned& hit_count, unsigned& miss_count) {
@@ -1100,7 +1101,7 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
             }
         }
         // Analytic lights: one delta sample, no MIS (path_direct_lighting.cpp:250-281).
-        if (L.analytic_count > 0) {
+        if (kAnalytic && L.analytic_count > 0) {
             const AnalyticSample as = SampleAnalytic(L, hp, smp.Get1D(kDimAnalyticSelect));
             if (as.valid && as.selection_pdf > 0.0f && !IsBlack(as.radiance)) {
                 const float c = fmaxf(0.0f, Dot(sn, as.wi));
@@ -1200,7 +1201,7 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
     P.p_flags[p] = flags;
 }
 
-template <bool kGuided>
+template <bool kGuided, bool kAnalytic>
 __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S, LightView L, yrg::GuideView G,
                                                      int depth, int cur) {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -1210,7 +1211,7 @@ __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S,
     o.shadow[kShadowEmissive] = o.shadow[kShadowAnalytic] = o.shadow[kShadowEnv] = false;
     unsigned hit_count = 0, miss_count = 0;
     const bool valid = i < n;
-    if (valid) ShadePath<kGuided>(P, S, L, G, depth, i, cur, o, hit_count, miss_count);
+    if (valid) ShadePath<kGuided, kAnalytic>(P, S, L, G, depth, i, cur, o, hit_count, miss_count);
     const int nxt = 1 - cur;
     __shared__ uint32_t append_lds[kBlock / 64 + 1];
     const int32_t slot = BlockAppend(P.counters + nxt, o.next ? 1u : 0u, append_lds);
@@ -1222,10 +1223,10 @@ __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S,
     }
     int slot_of[kShadowKinds] = {-1, -1, -1};
     uint32_t wanted = 0;
-    for (int k = 0; k < kShadowKinds; ++k) wanted += o.shadow[k] ? 1u : 0u;
+    for (int k = 0; k < kShadowKinds; ++k) wanted += (k != kShadowAnalytic || kAnalytic) && o.shadow[k] ? 1u : 0u;
     int32_t s = BlockAppend(P.counters + kCountShadow, wanted, append_lds);
     for (int k = 0; k < kShadowKinds; ++k) {
-        if (o.shadow[k]) {
+        if ((k != kShadowAnalytic || kAnalytic) && o.shadow[k]) {
             P.s_org[s] = F4(o.s_org[k], 1.0e-5f);
             P.s_dir[s] = F4(o.s_dir[k], o.s_tmax[k]);
             P.s_li[s] = F4(o.s_li[k], o.s_scale[k]);
@@ -1547,15 +1548,25 @@ struct ShadowWalk {
     }
 };
 
-template <bool kTwoLevel>
+template <bool kTwoLevel, bool kSpheres>
 __global__ __launch_bounds__(trav::kMaxBlock) void ShadowWalkKernel(trav::TraceArgs a, PassView P, SurfaceView S,
                                                                     LightView L) {
     extern __shared__ int2 lds_stack[];
     const ShadowWalk pol{&P, &S, &L};
     if (kTwoLevel) {
-        trav::TraceTwoLevelLoop<false>(a, pol, lds_stack);
+        trav::TraceTwoLevelLoop<false, kSpheres>(a, pol, lds_stack);
     } else {
-        trav::TraceFlatLoop<false>(a, pol, lds_stack);
+        trav::TraceFlatLoop<false, kSpheres>(a, pol, lds_stack);
+    }
+}
+
+template <bool kTwoLevel>
+void LaunchShadowWalk(bool spheres, int grid, int block, size_t lds, hipStream_t st, const trav::TraceArgs& a,
+                      const PassView& P, const SurfaceView& S, const LightView& L) {
+    if (spheres) {
+        hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, true>), dim3(grid), dim3(block), lds, st, a, P, S, L);
+    } else {
+        hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, false>), dim3(grid), dim3(block), lds, st, a, P, S, L);
     }
 }
 
@@ -1968,10 +1979,17 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
             return rc;
         }
         hipLaunchKernelGGL(BeginBounceKernel, dim3(1), dim3(64), 0, st, P.counters, 1 - cur, kBeginShade);
+        const bool analytic = L.analytic_count > 0;
         if (guide_here) {
-            hipLaunchKernelGGL(ShadeKernel<true>, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
+            if (analytic) {
+                hipLaunchKernelGGL((ShadeKernel<true, true>), dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
+            } else {
+                hipLaunchKernelGGL((ShadeKernel<true, false>), dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
+            }
+        } else if (analytic) {
+            hipLaunchKernelGGL((ShadeKernel<false, true>), dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
         } else {
-            hipLaunchKernelGGL(ShadeKernel<false>, dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
+            hipLaunchKernelGGL((ShadeKernel<false, false>), dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
         }
         if (overlap) {
             if ((rc = CheckHip(ctx, hipEventRecord(shaded, st), "hipEventRecord")) != YRH_OK) return rc;
@@ -2008,11 +2026,9 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
             int wgrid = 0, wblock = 0;
             if ((rc = PrepareTraceArgs(ctx, wr, wa, wgrid, wblock)) != YRH_OK) return rc;
             if (ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL) {
-                hipLaunchKernelGGL(ShadowWalkKernel<true>, dim3(wgrid), dim3(wblock), LdsBytes(wa, wblock), ss, wa, P, S,
-                                   L);
+                LaunchShadowWalk<true>(ctx->geo.has_spheres, wgrid, wblock, LdsBytes(wa, wblock), ss, wa, P, S, L);
             } else {
-                hipLaunchKernelGGL(ShadowWalkKernel<false>, dim3(wgrid), dim3(wblock), LdsBytes(wa, wblock), ss, wa, P, S,
-                                   L);
+                LaunchShadowWalk<false>(ctx->geo.has_spheres, wgrid, wblock, LdsBytes(wa, wblock), ss, wa, P, S, L);
             }
         }
         hipLaunchKernelGGL(ResolveKernel, dim3(Grid(paths)), dim3(kBlock), 0, ss, P, GV, depth, cur);

@@ -10,19 +10,28 @@ using trav::TraceArgs;
 
 // 8 waves per SIMD (at most 64 VGPRs and 100 SGPRs): with 8 workgroups of 256
 // per CU every workgroup of the persistent grid is resident at once.
-template <bool kAny>
+template <bool kAny, bool kSpheres>
 __global__ __launch_bounds__(kMaxBlock, 8) void TraceFlatKernel(TraceArgs a) {
     extern __shared__ int2 lds_stack[];
     const trav::RayArrayPolicy<kAny> pol{&a};
-    trav::TraceFlatLoop<kAny>(a, pol, lds_stack);
+    trav::TraceFlatLoop<kAny, kSpheres>(a, pol, lds_stack);
 }
 
 // At most 96 VGPRs (5 waves per SIMD, the most that 16 LDS stack levels allow).
-template <bool kAny>
+template <bool kAny, bool kSpheres>
 __global__ __launch_bounds__(kMaxBlock, 5) void TraceTwoLevelKernel(TraceArgs a) {
     extern __shared__ int2 lds_stack[];
     const trav::RayArrayPolicy<kAny> pol{&a};
-    trav::TraceTwoLevelLoop<kAny>(a, pol, lds_stack);
+    trav::TraceTwoLevelLoop<kAny, kSpheres>(a, pol, lds_stack);
+}
+
+template <bool kAny, bool kSpheres>
+void LaunchTraceKernel(bool two, int grid, int block, size_t lds, hipStream_t st, const TraceArgs& a) {
+    if (two) {
+        hipLaunchKernelGGL((TraceTwoLevelKernel<kAny, kSpheres>), dim3(grid), dim3(block), lds, st, a);
+    } else {
+        hipLaunchKernelGGL((TraceFlatKernel<kAny, kSpheres>), dim3(grid), dim3(block), lds, st, a);
+    }
 }
 
 bool Aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
@@ -145,18 +154,20 @@ int LaunchTrace(yrh_context* ctx, const TraceRequest& r) {
     int rc = PrepareTraceArgs(ctx, r, a, grid, block);
     if (rc != YRH_OK) return rc;
     const bool two = ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL;
+    const bool spheres = ctx->geo.has_spheres;
     hipStream_t st = r.aux ? ctx->aux_stream : ctx->stream;
+    const size_t lds = LdsBytes(a, block);
     if (r.any) {
-        if (two) {
-            hipLaunchKernelGGL(TraceTwoLevelKernel<true>, dim3(grid), dim3(block), LdsBytes(a, block), st, a);
+        if (spheres) {
+            LaunchTraceKernel<true, true>(two, grid, block, lds, st, a);
         } else {
-            hipLaunchKernelGGL(TraceFlatKernel<true>, dim3(grid), dim3(block), LdsBytes(a, block), st, a);
+            LaunchTraceKernel<true, false>(two, grid, block, lds, st, a);
         }
     } else {
-        if (two) {
-            hipLaunchKernelGGL(TraceTwoLevelKernel<false>, dim3(grid), dim3(block), LdsBytes(a, block), st, a);
+        if (spheres) {
+            LaunchTraceKernel<false, true>(two, grid, block, lds, st, a);
         } else {
-            hipLaunchKernelGGL(TraceFlatKernel<false>, dim3(grid), dim3(block), LdsBytes(a, block), st, a);
+            LaunchTraceKernel<false, false>(two, grid, block, lds, st, a);
         }
     }
     rc = CheckHip(ctx, hipGetLastError(), r.any ? "trace any launch" : "trace closest launch");

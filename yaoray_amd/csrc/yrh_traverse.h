@@ -451,7 +451,9 @@ struct RayArrayPolicy {
     }
 };
 
-template <bool kAny, class Policy>
+// kSpheres: the tree may hold sphere slots (a scene without spheres runs the
+// variant without the sphere branch, which costs its triangle tests ~2 %).
+template <bool kAny, bool kSpheres, class Policy>
 __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& pol, int2* lds_stack) {
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int64_t gtid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -558,7 +560,7 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
             } else if (tri) {
                 ++cursor;
                 bool changed;
-                if (IsSphereId(__float_as_int(f.q0.w))) {
+                if (kSpheres && IsSphereId(__float_as_int(f.q0.w))) {
                     ++ts;
                     changed = SphereTest(f.q0.x, f.q0.y, f.q0.z, f.q1.x, __float_as_int(f.q0.w), ox, oy, oz, dx, dy,
                                          dz, t_min, !kAny, best);
@@ -582,7 +584,7 @@ __device__ __forceinline__ void TraceFlatLoop(const TraceArgs& a, const Policy& 
     if (a.totals != nullptr) AddTotals(a.totals, sum_nt, sum_tt, sum_ts, lane);
 }
 
-template <bool kAny, class Policy>
+template <bool kAny, bool kSpheres, class Policy>
 __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Policy& pol, int2* lds_stack) {
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int64_t gtid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -735,7 +737,7 @@ __device__ __forceinline__ void TraceTwoLevelLoop(const TraceArgs& a, const Poli
             if (try_instance && cursor < leaf_end) {
                 const DevInstanceSlot m = a.instance_slots[cursor];
                 const int32_t slot_id = __float_as_int(m.hi.w);
-                if (IsSphereId(slot_id)) {
+                if (kSpheres && IsSphereId(slot_id)) {
                     // A TLAS sphere (IntersectTlasSphere, two_level_bvh.cpp:231-253):
                     // tested against the world ray, no instance.
                     ++ts;
