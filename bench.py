@@ -137,7 +137,7 @@ def main():
     ap.add_argument("--incoherent", type=int, default=4_194_304)
     ap.add_argument("--cpu-stride", type=int, default=4, help="CPU baseline traces every k-th ray")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--renders", default="cornell,two_level,cornell_adaptive",
+    ap.add_argument("--renders", default="cornell,two_level,cornell_adaptive,cornell_pbrt",
                     help="render workloads timed after the ray-cast microbench (comma list, empty for none)")
     ap.add_argument("--render-frames", type=int, default=3)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-camera workload")
@@ -270,6 +270,9 @@ def main():
 
     renders = {}
     for key in [k for k in args.renders.split(",") if k]:
+        if key == "cornell_pbrt":
+            renders[key] = bench_pbrt(args.render_frames, host_cores()[0], not args.no_cpu_baseline)
+            continue
         renders[key] = bench_render(dev, key, args.render_frames, min(16, len(os.sched_getaffinity(0))),
                                     not args.no_cpu_baseline)
     batch = (bench_batch(dev, not args.no_cpu_baseline, host_cores()[0], use_graph=not args.no_batch_graph)
@@ -616,6 +619,75 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
     out["time_to_threshold_speedup"] = round(tp / tg, 3) if tp and tg else None
     log(f"guiding: equal-time relMSE improvement {out['equal_time_improvement']}x at {guided[-1]['spp']} spp; "
         f"time to relMSE <= {threshold}: plain {tp}s, guided {tg}s")
+    return out
+
+
+def bench_pbrt(frames, cpu_threads, with_cpu):
+    """The reference's own Cornell scene (scenes/pbrt/cornell_box_pbrt: mirror and
+    glass spheres, an area light and a point light; 256x256, 64 spp, depth 6,
+    independent sampler) as its PBRT frontend compiles it (the committed
+    RenderSceneIR dump, tests/golden/cornell_pbrt.npz), rendered by HipBackend
+    (Prepare + Render, the reference's pass loop); Render seconds per frame."""
+    from yaoray_amd import Scene
+
+    f = np.load(os.path.join(ROOT, "tests", "golden", "cornell_pbrt.npz"), allow_pickle=False)
+    sections = {k[len("full_"):]: f[k] for k in f.files if k.startswith("full_")}
+    scene, st = Scene.from_ir(sections)
+    w, h, spp, depth = st["width"], st["height"], st["spp"], st["max_depth"]
+    # No progress callback (cancel_after_passes < 0): Render batches its passes.
+    kw = dict(max_depth=depth, sampler=st["sampler"], seed=st["seed"], cancel_after_passes=-1)
+    scene.backend_render(w, h, spp, **kw)  # warm-up
+    reps = [scene.backend_render(w, h, spp, **kw) for _ in range(frames)]
+    secs = [r[0]["elapsed_seconds"] for r in reps]
+    i = int(np.argsort(secs)[len(secs) // 2])
+    rep, film = reps[i]
+    sec = secs[i]
+    out = {
+        "workload": f"the reference's Cornell scene (cornell_box_pbrt.pbrt: 24 tris, mirror + glass spheres, area + "
+                    f"point light), {w}x{h}, {spp} spp independent, depth {depth}, HipBackend Render",
+        "frames": frames,
+        "ms_per_frame": round(sec * 1e3, 3),
+        "value": round(rep["rays_traced"] / sec / 1e6, 2),
+        "unit": "Mrays/s (path segments, rays_traced)",
+        "shadow_Mrays_s": round(rep["shadow_rays"] / sec / 1e6, 2),
+        "rays_traced": rep["rays_traced"],
+        "bad_samples": rep["bad_samples"],
+        "prepare_ms": round(rep["prepare_seconds"] * 1e3, 3),
+        "cpu_baseline": None,
+    }
+    log(f"render cornell_pbrt: {out['ms_per_frame']} ms/frame, {out['value']} Mrays/s (+{out['shadow_Mrays_s']} shadow)")
+    if with_cpu:
+        from oracle import oracle_py as O
+
+        if not O.ref_available():
+            return out
+        c_spp = max(1, spp // 4)
+        rsecs = []
+
+        def run():
+            fr, st_, sec_ = O.ref_render(scene, w, h, c_spp, max_depth=depth, sampler=st["sampler"], seed=st["seed"],
+                                         threads=cpu_threads)
+            rsecs.append(sec_)
+            return st_
+
+        run()
+        rsecs.clear()
+        rst = [run() for _ in range(5)][-1]
+        rsec = float(np.median(rsecs))
+        _, nproc, aff = host_cores()
+        out["cpu_baseline"] = {
+            "value": round(rst["rays_traced"] / rsec / 1e6, 4),
+            "unit": "Mrays/s (path segments, rays_traced)",
+            "cores": cpu_threads,
+            "nproc": nproc,
+            "affinity_cores": aff,
+            "kind": "reference",
+            "sample": f"{w}x{h} at {c_spp} spp of the same scene and settings, RenderCpuPathTrace render seconds, "
+                      f"warm-up + median of 5 ({rsec:.3f}s)",
+            "runs_s": [round(x, 4) for x in rsecs],
+            "frame_seconds_extrapolated": round(rsec * spp / c_spp, 3),
+        }
+        log(f"render cornell_pbrt: cpu reference {out['cpu_baseline']['value']} Mrays/s on {cpu_threads} threads")
     return out
 
 

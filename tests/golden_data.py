@@ -114,37 +114,6 @@ def sphere_scenes(f):
 
 def pbrt_scene(f, tag="smoke"):
     """A Scene from a PBRT-compiled RenderSceneIR fixture (the drop-in probe's
-    dump): arrays, camera, materials, spheres, emitters and analytic lights;
-    returns (scene, settings dict)."""
-    import ctypes
-
-    from yaoray_amd import native as N
-    from yaoray_amd.api import PRIMITIVE_DTYPE, SPHERE_DTYPE, VERTEX_DTYPE
-    g = lambda k: f[f"{tag}_{k}"]
-    v = g("vertices").view(VERTEX_DTYPE)
-    i = g("indices").view(np.uint32)
-    p = g("primitives").view(PRIMITIVE_DTYPE)
-    ip = g("inst_prim").view(np.int32)
-    im = g("inst_mat").view(np.float32).reshape(-1, 16)
-    s = Scene.from_arrays(v, i, p, ip if len(ip) else None, im if len(ip) else None)
-    cam = g("camera").view(np.float32)
-    s.set_camera(cam[0:3], cam[3:6], cam[6:9], cam[9:12], float(cam[12]))
-    mats = (N.Material * (len(g("materials")) // ctypes.sizeof(N.Material))).from_buffer_copy(g("materials").tobytes())
-    s.set_materials([{k: (list(getattr(m, k)) if isinstance(getattr(m, k), ctypes.Array) else getattr(m, k))
-                      for k, _ in N.Material._fields_ if k != "reserved"} for m in mats])
-    sph = g("spheres").view(SPHERE_DTYPE)
-    s.set_spheres([(tuple(x["center"]), float(x["radius"]), int(x["material_index"]), bool(x["flip_normals"]))
-                   for x in sph])
-    em = (N.Emissive * (len(g("emissives")) // ctypes.sizeof(N.Emissive))).from_buffer_copy(g("emissives").tobytes())
-    s.set_emissives([(e.primitive_index, tuple(e.radiance), e.area) for e in em])
-    al = (N.AnalyticLight * (len(g("analytic")) // ctypes.sizeof(N.AnalyticLight))).from_buffer_copy(
-        g("analytic").tobytes())
-    s.set_analytic_lights([{"kind": a.kind, "position": tuple(a.position), "direction": tuple(a.direction),
-                            "intensity": tuple(a.intensity), "cone_angle": a.cone_angle,
-                            "cone_cos_inner": a.cone_cos_inner} for a in al])
-    env = g("environment").view(np.float32)
-    s.set_constant_environment(env[:3], float(env[3]))
-    s.build_accel()
-    st = g("settings").view(np.int64)
-    return s, {"width": int(st[0]), "height": int(st[1]), "spp": int(st[2]), "max_depth": int(st[3]),
-               "sampler": int(st[4]), "seed": int(st[5])}
+    dump); returns (scene, settings dict)."""
+    prefix = tag + "_"
+    return Scene.from_ir({k[len(prefix):]: v for k, v in f.items() if k.startswith(prefix)})

@@ -167,6 +167,42 @@ class Scene:
             raise YrhError(f"yrhost_scene_from_arrays failed: {rc}")
         return cls(h)
 
+    @classmethod
+    def from_ir(cls, sections):
+        """A Scene from the sections of a compiled RenderSceneIR as raw byte arrays
+        (`oracle/dropin_probe.cpp dump`: vertices, indices, primitives, spheres,
+        inst_prim, inst_mat, camera, materials, emissives, analytic, environment,
+        settings; yrh.h records). Returns (scene with its acceleration built,
+        settings dict: width, height, spp, max_depth, sampler, seed)."""
+        g = lambda k: np.ascontiguousarray(sections[k], np.uint8)
+        ip = g("inst_prim").view(np.int32)
+        im = g("inst_mat").view(np.float32).reshape(-1, 16)
+        s = cls.from_arrays(g("vertices").view(VERTEX_DTYPE), g("indices").view(np.uint32),
+                            g("primitives").view(PRIMITIVE_DTYPE), ip if len(ip) else None, im if len(ip) else None)
+        cam = g("camera").view(np.float32)
+        s.set_camera(cam[0:3], cam[3:6], cam[6:9], cam[9:12], float(cam[12]))
+        raw = g("materials").tobytes()
+        mats = (N.Material * (len(raw) // C.sizeof(N.Material))).from_buffer_copy(raw)
+        s.set_materials([{k: (list(getattr(m, k)) if isinstance(getattr(m, k), C.Array) else getattr(m, k))
+                          for k, _ in N.Material._fields_ if k != "reserved"} for m in mats])
+        sph = g("spheres").view(SPHERE_DTYPE)
+        s.set_spheres([(tuple(x["center"]), float(x["radius"]), int(x["material_index"]), bool(x["flip_normals"]))
+                       for x in sph])
+        raw = g("emissives").tobytes()
+        em = (N.Emissive * (len(raw) // C.sizeof(N.Emissive))).from_buffer_copy(raw)
+        s.set_emissives([(e.primitive_index, tuple(e.radiance), e.area) for e in em])
+        raw = g("analytic").tobytes()
+        al = (N.AnalyticLight * (len(raw) // C.sizeof(N.AnalyticLight))).from_buffer_copy(raw)
+        s.set_analytic_lights([{"kind": a.kind, "position": tuple(a.position), "direction": tuple(a.direction),
+                                "intensity": tuple(a.intensity), "cone_angle": a.cone_angle,
+                                "cone_cos_inner": a.cone_cos_inner} for a in al])
+        env = g("environment").view(np.float32)
+        s.set_constant_environment(env[:3], float(env[3]))
+        s.build_accel()
+        st = g("settings").view(np.int64)
+        return s, {"width": int(st[0]), "height": int(st[1]), "spp": int(st[2]), "max_depth": int(st[3]),
+                   "sampler": int(st[4]), "seed": int(st[5])}
+
     def close(self):
         if self._h:
             self._lib.yrhost_scene_destroy(self._h)
