@@ -203,30 +203,59 @@ __device__ inline float3 MixtureSample(const MixtureView& m, float u0, float u1,
 
 }  // namespace yrg
 
-// ---- network on one wave (MFMA 16x16x16 BF16 -> FP32) -------------------------------
+// ---- network on one wave (MFMA BF16 -> FP32) ---------------------------------------
 //
 // Formulation: Z^T = W^T H^T, so a layer's output block in the MFMA result
 // layout (lane l holds features 4(l/16)+r, r = 0..3, of row l%16) is exactly
 // the B operand of the next layer; activations never leave registers.
-// Weights come as pre-packed A-operand fragments (64 lanes x 4 BF16 each):
-//   forward fragment (layer, mb over outputs, kb over inputs): lane l holds
-//     W[kb*16 + 4(l/16) + j][mb*16 + l%16], j = 0..3;
-//   backward fragment (layer, mb over inputs, kb over outputs): lane l holds
-//     W[mb*16 + l%16][kb*16 + 4(l/16) + j].
+// The forward pass uses gfx950's 16x16x32 BF16 MFMA: its B operand (8 BF16
+// per lane) is a lane's values of two consecutive 16-feature output blocks,
+// 2kb2 and 2kb2+1, side by side. The K index a lane's j-th value stands for
+// only has to agree between A and B, so the weight fragments take the same
+// order and no value moves between lanes. Weights come as pre-packed
+// A-operand fragments:
+//   forward fragment (layer, mb over outputs, kb2 over 32-input blocks), 8 BF16
+//     per lane: lane l holds W[32 kb2 + 4(l/16) + j][mb*16 + l%16] for j = 0..3
+//     and W[32 kb2 + 16 + 4(l/16) + j - 4][mb*16 + l%16] for j = 4..7;
+//   backward fragment (layer, mb over inputs, kb over outputs; 16x16x16 BF16
+//     MFMA), 4 BF16 per lane: lane l holds W[mb*16 + l%16][kb*16 + 4(l/16) + j].
 
 namespace yrg {
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef short bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_mfma_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kLayerIn[4] = {kGuideIn, kGuideHidden, kGuideHidden, kGuideHidden};
 constexpr int kLayerOut[4] = {kGuideHidden, kGuideHidden, kGuideHidden, kGuideOut};
 constexpr int kLayerParam[4] = {kW0, kW1, kW2, kW3};
 constexpr int kFragBase[4] = {0, 8, 24, 40};
-constexpr int kFragCount = 52;  // per direction
+constexpr int kFragCount = 52;  // backward fragments (16 x 16 blocks)
+constexpr int kFwdFragBase[4] = {0, 4, 12, 20};
+constexpr int kFwdFragCount = 26;  // forward fragments (16 outputs x 32 inputs): the same bytes
 
 __device__ __forceinline__ f32x4_t Mfma(bf16x4_t a, bf16x4_t b, f32x4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ f32x4_t Mfma32(bf16x8_t a, bf16x8_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_mfma_t, a),
+                                                   __builtin_bit_cast(bf16x8_mfma_t, b), c, 0, 0, 0);
+}
+
+// B operand of a 32-input block: a lane's values of two 16-feature blocks.
+__device__ __forceinline__ bf16x8_t Cat(bf16x4_t lo, bf16x4_t hi) {
+    bf16x8_t r;
+    r[0] = lo[0];
+    r[1] = lo[1];
+    r[2] = lo[2];
+    r[3] = lo[3];
+    r[4] = hi[0];
+    r[5] = hi[1];
+    r[6] = hi[2];
+    r[7] = hi[3];
+    return r;
 }
 
 __device__ __forceinline__ bf16x4_t PackBf16(f32x4_t d, bool relu) {
@@ -255,42 +284,37 @@ __device__ __forceinline__ f32x4_t Zero4() {
 // Forward pass of RB row blocks. x: input B operands (2 blocks of 16 inputs);
 // h0..h2: hidden activations (B operands); o: FP32 outputs (3 blocks of 16).
 template <int RB>
-__device__ __forceinline__ void MlpForward(const bf16x4_t* __restrict__ frag, int lane, const bf16x4_t (&x)[RB][2],
+__device__ __forceinline__ void MlpForward(const bf16x8_t* __restrict__ frag, int lane, const bf16x4_t (&x)[RB][2],
                                            bf16x4_t (&h0)[RB][4], bf16x4_t (&h1)[RB][4], bf16x4_t (&h2)[RB][4],
                                            f32x4_t (&o)[RB][3]) {
     for (int mb = 0; mb < 4; ++mb) {
-        f32x4_t acc[RB];
-        for (int r = 0; r < RB; ++r) acc[r] = Zero4();
-        for (int kb = 0; kb < 2; ++kb) {
-            const bf16x4_t a = frag[(kFragBase[0] + mb * 2 + kb) * 64 + lane];
-            for (int r = 0; r < RB; ++r) acc[r] = Mfma(a, x[r][kb], acc[r]);
-        }
-        for (int r = 0; r < RB; ++r) h0[r][mb] = PackBf16(acc[r], true);
+        const bf16x8_t a = frag[(kFwdFragBase[0] + mb) * 64 + lane];
+        for (int r = 0; r < RB; ++r) h0[r][mb] = PackBf16(Mfma32(a, Cat(x[r][0], x[r][1]), Zero4()), true);
     }
     for (int mb = 0; mb < 4; ++mb) {
         f32x4_t acc[RB];
         for (int r = 0; r < RB; ++r) acc[r] = Zero4();
-        for (int kb = 0; kb < 4; ++kb) {
-            const bf16x4_t a = frag[(kFragBase[1] + mb * 4 + kb) * 64 + lane];
-            for (int r = 0; r < RB; ++r) acc[r] = Mfma(a, h0[r][kb], acc[r]);
+        for (int kb2 = 0; kb2 < 2; ++kb2) {
+            const bf16x8_t a = frag[(kFwdFragBase[1] + mb * 2 + kb2) * 64 + lane];
+            for (int r = 0; r < RB; ++r) acc[r] = Mfma32(a, Cat(h0[r][2 * kb2], h0[r][2 * kb2 + 1]), acc[r]);
         }
         for (int r = 0; r < RB; ++r) h1[r][mb] = PackBf16(acc[r], true);
     }
     for (int mb = 0; mb < 4; ++mb) {
         f32x4_t acc[RB];
         for (int r = 0; r < RB; ++r) acc[r] = Zero4();
-        for (int kb = 0; kb < 4; ++kb) {
-            const bf16x4_t a = frag[(kFragBase[2] + mb * 4 + kb) * 64 + lane];
-            for (int r = 0; r < RB; ++r) acc[r] = Mfma(a, h1[r][kb], acc[r]);
+        for (int kb2 = 0; kb2 < 2; ++kb2) {
+            const bf16x8_t a = frag[(kFwdFragBase[2] + mb * 2 + kb2) * 64 + lane];
+            for (int r = 0; r < RB; ++r) acc[r] = Mfma32(a, Cat(h1[r][2 * kb2], h1[r][2 * kb2 + 1]), acc[r]);
         }
         for (int r = 0; r < RB; ++r) h2[r][mb] = PackBf16(acc[r], true);
     }
     for (int mb = 0; mb < 3; ++mb) {
         f32x4_t acc[RB];
         for (int r = 0; r < RB; ++r) acc[r] = Zero4();
-        for (int kb = 0; kb < 4; ++kb) {
-            const bf16x4_t a = frag[(kFragBase[3] + mb * 4 + kb) * 64 + lane];
-            for (int r = 0; r < RB; ++r) acc[r] = Mfma(a, h2[r][kb], acc[r]);
+        for (int kb2 = 0; kb2 < 2; ++kb2) {
+            const bf16x8_t a = frag[(kFwdFragBase[3] + mb * 2 + kb2) * 64 + lane];
+            for (int r = 0; r < RB; ++r) acc[r] = Mfma32(a, Cat(h2[r][2 * kb2], h2[r][2 * kb2 + 1]), acc[r]);
         }
         for (int r = 0; r < RB; ++r) o[r][mb] = acc[r];
     }
@@ -333,7 +357,7 @@ struct GuideView {
     int32_t mode;  // YRH_GUIDE_*
     float alpha;   // probability of sampling the proposal at a guidable vertex
     GuideGrid grid;
-    const bf16x4_t* frag;  // forward fragments
+    const bf16x8_t* frag;  // forward fragments
     const float2* table;   // hash-grid features
     float* mix;            // per queue row i: value k (pi[8], kappa[8], mu[8][3], VmfNorm(kappa)[8]) at mix[k * mix_stride + i]
     int64_t mix_stride;

@@ -28,6 +28,7 @@
 namespace yrh {
 
 using yrg::bf16x4_t;
+using yrg::bf16x8_t;
 using yrg::f32x4_t;
 
 // Device memory the per-cell grid copies may take (levels coarse first).
@@ -105,22 +106,33 @@ __global__ void GuideCellPackKernel(const float2* __restrict__ table, yrg::Guide
     for (int j = 0; j < 4; ++j) o[j] = make_float4(v[2 * j].x, v[2 * j].y, v[2 * j + 1].x, v[2 * j + 1].y);
 }
 
-__global__ void GuidePackKernel(const float* __restrict__ mlp, bf16x4_t* __restrict__ fwd, bf16x4_t* __restrict__ bwd) {
+// Layer of a forward (16 x 32) fragment index.
+__device__ __forceinline__ int LayerOfFwdFrag(int f) { return f < 4 ? 0 : (f < 12 ? 1 : (f < 20 ? 2 : 3)); }
+
+__global__ void GuidePackKernel(const float* __restrict__ mlp, bf16x8_t* __restrict__ fwd, bf16x4_t* __restrict__ bwd) {
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (t >= yrg::kFragCount * 64) return;
     const int f = t / 64, lane = t % 64;
-    const int l = LayerOfFrag(f);
-    const int local = f - yrg::kFragBase[l];
-    const int in = yrg::kLayerIn[l], out = yrg::kLayerOut[l];
-    const float* w = mlp + yrg::kLayerParam[l];
     const int r = lane % 16, g = 4 * (lane / 16);
-    {
-        const int nkb = in / 16;
-        const int mb = local / nkb, kb = local % nkb;
-        bf16x4_t v;
-        for (int j = 0; j < 4; ++j) v[j] = static_cast<short>(yrg::ToBf16(w[(kb * 16 + g + j) * out + mb * 16 + r]));
+    if (f < yrg::kFwdFragCount) {
+        // Forward fragment (mb, kb2): the values of input blocks 2 kb2 and 2 kb2 + 1.
+        const int l = LayerOfFwdFrag(f);
+        const int local = f - yrg::kFwdFragBase[l];
+        const int in = yrg::kLayerIn[l], out = yrg::kLayerOut[l];
+        const float* w = mlp + yrg::kLayerParam[l];
+        const int nkb2 = in / 32;
+        const int mb = local / nkb2, kb2 = local % nkb2;
+        bf16x8_t v;
+        for (int j = 0; j < 8; ++j) {
+            const int feature = 32 * kb2 + (j < 4 ? g + j : 16 + g + j - 4);
+            v[j] = static_cast<short>(yrg::ToBf16(w[feature * out + mb * 16 + r]));
+        }
         fwd[t] = v;
     }
+    const int l = LayerOfFrag(f);
+    const int local = f - yrg::kFragBase[l];
+    const int out = yrg::kLayerOut[l];
+    const float* w = mlp + yrg::kLayerParam[l];
     {
         const int nkb = out / 16;
         const int mb = local / nkb, kb = local % nkb;
@@ -139,7 +151,7 @@ struct TrainArgs {
     int64_t offset;  // first sample of this batch (mod count)
     int64_t rows;    // batch rows
     int32_t spread;  // 1: row k of the run takes sample floor(frac(k phi) count), k = offset + row
-    const bf16x4_t* fwd;
+    const bf16x8_t* fwd;
     const bf16x4_t* bwd;
     const float2* table;
     float* table_grad;
@@ -445,7 +457,7 @@ __global__ void ResetHeaderKernel(uint32_t* hdr) {
 // Network evaluation for host-supplied rows (position, normal, outgoing):
 // one wave per 64 rows.
 __global__ __launch_bounds__(kBlock) void GuideQueryRowsKernel(const float* __restrict__ rows, int64_t n,
-                                                               const bf16x4_t* __restrict__ fwd,
+                                                               const bf16x8_t* __restrict__ fwd,
                                                                const float2* __restrict__ table, yrg::GuideGrid grid,
                                                                float* __restrict__ mixture) {
     __shared__ float lds_o[4][64 * yrg::kOutStride];
@@ -536,7 +548,7 @@ int NeedGuide(yrh_context* ctx) {
 
 int Pack(yrh_context* ctx, GuideState& G) {
     hipLaunchKernelGGL(GuidePackKernel, dim3(Grid(yrg::kFragCount * 64)), dim3(kBlock), 0, ctx->stream,
-                       G.mlp.as<float>(), G.frag_fwd.as<bf16x4_t>(), G.frag_bwd.as<bf16x4_t>());
+                       G.mlp.as<float>(), G.frag_fwd.as<bf16x8_t>(), G.frag_bwd.as<bf16x4_t>());
     return CheckHip(ctx, hipGetLastError(), "guide pack");
 }
 
@@ -602,7 +614,7 @@ int BackwardPass(yrh_context* ctx, GuideState& G, const float4* samples, const u
     A.offset = offset;
     A.spread = spread;
     A.rows = rows;
-    A.fwd = G.frag_fwd.as<bf16x4_t>();
+    A.fwd = G.frag_fwd.as<bf16x8_t>();
     A.bwd = G.frag_bwd.as<bf16x4_t>();
     A.table = G.table.as<float2>();
     A.table_grad = G.table_grad.as<float>();
@@ -654,7 +666,7 @@ int GuidePrepareView(yrh_context* ctx, int64_t paths, int max_depth, yrg::GuideV
     view->mode = G->mode;
     view->alpha = G->desc.guide_fraction;
     view->grid = G->grid;
-    view->frag = G->frag_fwd.as<bf16x4_t>();
+    view->frag = G->frag_fwd.as<bf16x8_t>();
     view->table = G->table.as<float2>();
     view->mix = G->mix.as<float>();
     view->mix_stride = G->rows;
@@ -915,7 +927,7 @@ int yrh_guide_query(yrh_context* ctx, int64_t n, const float* inputs, float* mix
     if ((rc = Alloc(ctx, out, static_cast<size_t>(n) * 160)) != YRH_OK) return rc;
     if ((rc = yrh_copy_to_device(ctx, in.ptr, inputs, static_cast<size_t>(n) * 36)) != YRH_OK) return rc;
     hipLaunchKernelGGL(GuideQueryRowsKernel, dim3(Grid(n, 256)), dim3(kBlock), 0, ctx->stream, in.as<float>(), n,
-                       G.frag_fwd.as<bf16x4_t>(), G.table.as<float2>(), G.grid, out.as<float>());
+                       G.frag_fwd.as<bf16x8_t>(), G.table.as<float2>(), G.grid, out.as<float>());
     if ((rc = CheckHip(ctx, hipGetLastError(), "guide query")) != YRH_OK) return rc;
     return yrh_copy_to_host(ctx, mixture, out.ptr, static_cast<size_t>(n) * 160);
 }

@@ -1270,10 +1270,10 @@ __global__ __launch_bounds__(kQueryBlock) void GuideQueryKernel(PassView P, Surf
     // conflicts), then the network outputs of 16 rows at a time.
     __shared__ float lds_stage[kQueryWaves][16 * yrg::kOutStride];
     static_assert(16 * yrg::kOutStride >= 64 * 11, "staging area holds the inputs");
-    __shared__ yrg::bf16x4_t lds_w[yrg::kFragCount * 64];  // weight fragments, shared by the block
+    __shared__ yrg::bf16x8_t lds_w[yrg::kFwdFragCount * 64];  // forward weight fragments, shared by the block
     const int n = static_cast<int>(P.counters[cur]);
     if (static_cast<int64_t>(blockIdx.x) * blockDim.x >= n) return;  // whole block idle
-    for (int k = static_cast<int>(threadIdx.x); k < yrg::kFragCount * 64; k += kQueryBlock) lds_w[k] = G.frag[k];
+    for (int k = static_cast<int>(threadIdx.x); k < yrg::kFwdFragCount * 64; k += kQueryBlock) lds_w[k] = G.frag[k];
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const int wave = static_cast<int>(threadIdx.x >> 6);
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
