@@ -278,20 +278,62 @@ int yrh_set_trace_order(yrh_context* ctx, int order);
 #define YRH_MATERIAL_DIELECTRIC 2
 #define YRH_MATERIAL_THIN_DIELECTRIC 3
 
-/* The untextured subset of yr::RenderMaterial (include/yaoray/scene/material.hpp:26-65)
- * that the in-scope BSDFs read (src/shading/bsdf.cpp:17-329). */
+/* The subset of yr::RenderMaterial (include/yaoray/scene/material.hpp:26-65)
+ * that the in-scope BSDFs and EvaluateMaterialAtSurface read
+ * (src/shading/bsdf.cpp:17-329; src/shading/material_evaluator.cpp:112-180).
+ * Texture fields index yrh_shading_desc::textures; an index outside
+ * [0, texture_count) means "no texture" (TextureIndexValid,
+ * material_evaluator.cpp:19-21). Textures apply only to triangles of
+ * primitives with has_uvs: reflectance is multiplied by the texture's RGB,
+ * alpha by its first channel; a hit whose alpha is below 0.5 is skipped
+ * (IsAlphaVisible, :182-184); normal_map (also needs has_tangents) perturbs
+ * the shading normal in the interpolated tangent frame, tangent-space x and
+ * y scaled by normal_scale (:35-92). Spheres are never textured. */
 typedef struct yrh_material {
     int32_t kind; /* YRH_MATERIAL_* */
     float reflectance[3];
     float ior;
     float uroughness;
     float vroughness;
-    float alpha; /* must be >= 0.5: alpha cut-outs are not supported */
+    float alpha;
     float emission[3];
     float absorption_distance;
     float absorption_color[3];
     float reserved;
+    int32_t reflectance_texture;
+    int32_t alpha_texture;
+    int32_t normal_map;
+    float normal_scale;
 } yrh_material;
+
+#define YRH_FILTER_NEAREST 0
+#define YRH_FILTER_BILINEAR 1
+#define YRH_FILTER_TRILINEAR 2
+#define YRH_FILTER_EWA 3
+#define YRH_WRAP_REPEAT 0
+#define YRH_WRAP_CLAMP 1
+#define YRH_WRAP_MIRRORED 2
+
+/* == yr::RenderTexture (include/yaoray/scene/texture.hpp:38-51): linear RGBA
+ * texels of level 0 (row-major, width * height * 4 floats), the filter
+ * (YRH_FILTER_*, include/yaoray/scene/texture_options.hpp:7-12) and the wrap
+ * modes (YRH_WRAP_*). The library builds the mip chain at upload
+ * (BuildTextureMipChain, src/shading/texture.cpp:241-272). A texture without
+ * texels (width or height <= 0, or texels NULL) samples as zero, as
+ * SampleTexture4 returns for one (texture.cpp:312-313). Trilinear and EWA
+ * filtering apply where the reference has a texture footprint: hits of camera
+ * rays (ray differentials one pixel apart, src/scene/camera.cpp:20-39;
+ * ComputeTextureFootprint, src/integrators/surface_query.cpp:35-97); every
+ * other lookup is bilinear. */
+typedef struct yrh_texture {
+    int32_t width;
+    int32_t height;
+    int32_t filter;
+    int32_t wrap_s;
+    int32_t wrap_t;
+    int32_t reserved;
+    const float* texels;
+} yrh_texture;
 
 /* == yr::EmissivePrimitive (include/yaoray/scene/light.hpp:30-34). */
 typedef struct yrh_emissive {
@@ -300,8 +342,11 @@ typedef struct yrh_emissive {
     float area;
 } yrh_emissive;
 
-/* yr::RenderEnvironment + its RenderTexture (RGBA float, bilinear, repeat)
- * + RenderEnvironmentDistribution (include/yaoray/scene/light.hpp:10-28). */
+/* yr::RenderEnvironment + its RenderTexture (RGBA float) + RenderEnvironmentDistribution
+ * (include/yaoray/scene/light.hpp:10-28). The map is read as SampleTexture
+ * reads it without a footprint (src/lighting/environment.cpp:136): bilinear,
+ * or nearest when `nearest` is set, with the texture's wrap modes (YRH_WRAP_*;
+ * the reference's HDR / EXR loaders set repeat in u, clamp in v). */
 typedef struct yrh_environment_desc {
     int32_t active;
     float radiance[3];
@@ -318,6 +363,9 @@ typedef struct yrh_environment_desc {
     const float* row_cdf;          /* dist_height */
     const float* conditional_cdfs; /* dist_width * dist_height */
     float total_weight;
+    int32_t nearest;
+    int32_t wrap_s;
+    int32_t wrap_t;
 } yrh_environment_desc;
 
 #define YRH_LIGHT_POINT 0
@@ -373,6 +421,9 @@ typedef struct yrh_shading_desc {
     int32_t analytic_tree_count;
     int32_t analytic_tree_root;
     int32_t use_analytic_tree;
+    /* RenderSceneIR::textures that materials index (NULL / 0 when none). */
+    const yrh_texture* textures;
+    int32_t texture_count;
 } yrh_shading_desc;
 
 /* Uploads materials, emitters and the environment for the current geometry
@@ -549,6 +600,14 @@ int yrh_sample_values(yrh_context* ctx, int32_t kind, uint64_t seed, int32_t spp
  * Host arrays; synchronises. */
 int yrh_bsdf_query(yrh_context* ctx, const yrh_material* materials, int32_t material_count, int64_t n,
                    const int32_t* material_index, const float* queries, float* out);
+
+/* SampleTexture4 (src/shading/texture.cpp:312-322) of `texture` (its mip
+ * chain built as at upload) at n points: uv[2i..] = (u, v), footprint[4i..] =
+ * (dudx, dvdx, dudy, dvdy) or footprint NULL for none; out[4i..] receives
+ * RGBA. Host arrays; synchronises. The texture lookups of the shading
+ * kernels, exposed for parity tests. */
+int yrh_texture_query(yrh_context* ctx, const yrh_texture* texture, int64_t n, const float* uv, const float* footprint,
+                      float* out);
 
 /* Device ray generation for one pass (GeneratePerspectiveCameraRay with the
  * sampler's Pixel dimension): writes width*height rays into device arrays. */

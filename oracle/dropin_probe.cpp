@@ -16,7 +16,8 @@
 //   dropin_probe dump <scene.pbrt> <out.bin>
 //       The compiled RenderSceneIR and RenderSettings as the C ABI carries them
 //       (raw vertex / index / primitive / sphere / instance arrays, materials,
-//       emitters and analytic lights through the backend's own marshalling):
+//       emitters and analytic lights through the backend's own marshalling,
+//       textures with their level-0 texels, the environment map's index):
 //       "YRIR", then sections (16-byte name, int64 count, int32 element size,
 //       bytes) up to a section named "end".
 //
@@ -174,10 +175,6 @@ int Dump(int argc, char** argv) {
     if (!loaded) return 2;
     yr::RenderSceneIR& ir = loaded->scene;
     const yr::RenderSettings& st = loaded->settings;
-    if (!ir.textures.empty() || ir.environment.active) {
-        std::cerr << "dump: textures and environment maps are not carried\n";
-        return 1;
-    }
     yr::PrepareLightSampling(ir);
     yrhip::Marshal m;
     yrh_shading_desc sd{};
@@ -219,9 +216,33 @@ int Dump(int argc, char** argv) {
     const float env[4] = {ir.environment.radiance.x, ir.environment.radiance.y, ir.environment.radiance.z,
                           ir.environment.strength};
     Section(f, "environment", env, 4, 4);
+    // Textures (level 0; RenderSceneIR::textures order, the environment map
+    // among them) and the environment's texture index and rotation.
+    std::vector<yrh_texture> tex(ir.textures.size());
+    std::vector<float> texels;
+    for (size_t k = 0; k < tex.size(); ++k) {
+        const yr::RenderTexture& t = ir.textures[k];
+        tex[k] = yrh_texture{t.width, t.height, static_cast<int32_t>(t.filter), static_cast<int32_t>(t.wrap_s),
+                             static_cast<int32_t>(t.wrap_t), 0, nullptr};
+        const std::vector<yr::Color4f>& px = t.sampling_cache ? t.sampling_cache->texels : t.texels;
+        if (static_cast<int64_t>(px.size()) != static_cast<int64_t>(t.width) * t.height) {
+            std::cerr << "dump: texture " << k << " holds " << px.size() << " texels for " << t.width << " x "
+                      << t.height << '\n';
+            return 1;
+        }
+        for (const yr::Color4f& c : px) texels.insert(texels.end(), {c.x, c.y, c.z, c.w});
+    }
+    // 24-byte records: width, height, filter, wrap_s, wrap_t, reserved.
+    std::vector<int32_t> tex_rec;
+    for (const yrh_texture& t : tex) tex_rec.insert(tex_rec.end(), {t.width, t.height, t.filter, t.wrap_s, t.wrap_t, 0});
+    Section(f, "textures", tex_rec.data(), static_cast<int64_t>(tex.size()), 24);
+    Section(f, "texels", texels.data(), static_cast<int64_t>(texels.size()), 4);
+    int32_t env_map[4] = {ir.environment.active ? 1 : 0, ir.environment.texture_index, 0, 0};
+    std::memcpy(&env_map[2], &ir.environment.rotation_radians, 4);
+    Section(f, "env_map", env_map, 4, 4);
     Section(f, "end", nullptr, 0, 0);
     std::cout << "dumped " << ir.vertices.size() << " vertices, " << ir.spheres.size() << " spheres, "
-              << ir.analytic_lights.size() << " analytic lights\n";
+              << ir.analytic_lights.size() << " analytic lights, " << ir.textures.size() << " textures\n";
     return f ? 0 : 1;
 }
 

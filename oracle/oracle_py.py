@@ -397,6 +397,60 @@ def ref_render(scene, width, height, spp, max_depth=8, sampler=2, seed=0x59414F5
     return film, out, secs.value
 
 
+def _texture(texels, filter, wrap_s, wrap_t):
+    from yaoray_amd import native as N
+
+    t = np.ascontiguousarray(texels, np.float32)
+    d = N.Texture()
+    d.height, d.width = (int(t.shape[0]), int(t.shape[1])) if t.size else (0, 0)
+    d.filter, d.wrap_s, d.wrap_t = int(filter), int(wrap_s), int(wrap_s if wrap_t is None else wrap_t)
+    d.texels = t.ctypes.data if t.size else None
+    return d, t
+
+
+def _sample_texture(lib, name, texels, uv, footprint, filter, wrap_s, wrap_t):
+    d, keep = _texture(texels, filter, wrap_s, wrap_t)
+    uv = _c(uv)
+    fp = None if footprint is None else _c(footprint)
+    out = np.zeros((len(uv), 4), np.float32)
+    fn = getattr(lib, name)
+    fn.argtypes = [_VP, _I64, _VP, _VP, _VP]
+    fn(C.cast(C.pointer(d), _VP), len(uv), _p(uv), None if fp is None else _p(fp), _p(out))
+    return out
+
+
+def sample_texture(texels, uv, footprint=None, filter=1, wrap_s=0, wrap_t=None):
+    """The oracle's SampleTexture4 of an RGBA texture (height x width x 4) at uv
+    (n x 2) with footprints (n x 4: dudx, dvdx, dudy, dvdy) or none; n x 4."""
+    return _sample_texture(oracle_lib(), "oracle_sample_texture", texels, uv, footprint, filter, wrap_s, wrap_t)
+
+
+def ref_sample_texture(texels, uv, footprint=None, filter=1, wrap_s=0, wrap_t=None):
+    """The reference's SampleTexture4 (after BuildTextureSamplingCaches) on the same inputs."""
+    return _sample_texture(ref_lib(), "ref_sample_texture", texels, uv, footprint, filter, wrap_s, wrap_t)
+
+
+def _mips(lib, name, texels, wrap_s, wrap_t):
+    d, keep = _texture(texels, 1, wrap_s, wrap_t)
+    fn = getattr(lib, name)
+    fn.argtypes = [_VP, C.POINTER(_I32), C.POINTER(_I64), _VP]
+    levels, count = _I32(), _I64()
+    fn(C.cast(C.pointer(d), _VP), C.byref(levels), C.byref(count), None)
+    out = np.zeros((count.value, 4), np.float32)
+    fn(C.cast(C.pointer(d), _VP), C.byref(levels), C.byref(count), _p(out))
+    return levels.value, out
+
+
+def texture_mips(texels, wrap_s=0, wrap_t=None):
+    """(level count, every level's texels one after another) of the oracle's mip chain."""
+    return _mips(oracle_lib(), "oracle_texture_mips", texels, wrap_s, wrap_t)
+
+
+def ref_texture_mips(texels, wrap_s=0, wrap_t=None):
+    """The same from the reference's BuildTextureMipChain."""
+    return _mips(ref_lib(), "ref_texture_mips", texels, wrap_s, wrap_t)
+
+
 def ref_sample_values(kind, seed, spp, x, y, sample, dim, dims):
     """The reference's yr::Sampler at bounce 0 (Sample1D / Sample2D of raw dimensions)."""
     lib = ref_lib()

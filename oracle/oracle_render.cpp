@@ -5,8 +5,9 @@
 //   Sampler                 src/sampling/sampler.cpp:34-237
 //   camera ray              src/scene/camera.cpp:9-39
 //   TraceVisibleSurface     src/integrators/surface_query.cpp:101-200
-//   material / normals      src/shading/material_evaluator.cpp:112-180,
+//   material / normals      src/shading/material_evaluator.cpp:35-184,
 //                           src/geometry/intersection.cpp:8-106
+//   textures, alpha skip    oracle_texture.hpp; surface_query.cpp:35-97, 187-194
 //   BSDFs                   src/shading/bsdf.cpp:17-329, bsdf_microfacet.cpp:20-298,
 //                           bsdf_common.cpp:13-47
 //   lights                  src/lighting/light_sampling.cpp:60-396,
@@ -31,6 +32,7 @@
 #include <thread>
 #include <vector>
 
+#include "oracle_texture.hpp"
 #include "oracle_trace.hpp"
 
 namespace {
@@ -59,6 +61,9 @@ bool NearBlack(V3 c) { return c.x <= 0.0f && c.y <= 0.0f && c.z <= 0.0f; }
 float MaxComponent(V3 c) { return std::max(c.x, std::max(c.y, c.z)); }
 bool Finite(V3 c) { return std::isfinite(c.x) && std::isfinite(c.y) && std::isfinite(c.z); }
 V3 Load3(const float* p) { return V3{p[0], p[1], p[2]}; }
+struct V2 {
+    float x, y;
+};
 
 // ---- sampler ------------------------------------------------------------------
 
@@ -185,6 +190,10 @@ struct Scene {
     const yrh_shading_desc* sd;
     bool two_level;
     orc::TwoLevel tl;
+    std::vector<ortex::Texture> tex;
+    const ortex::Texture* Tex(int index) const {
+        return index >= 0 && static_cast<size_t>(index) < tex.size() ? &tex[static_cast<size_t>(index)] : nullptr;
+    }
 };
 
 struct Hit {
@@ -271,50 +280,184 @@ V3 ResolveShadingNormal(const Scene& s, TriRef t, float u, float v, V3 geo) {
 
 struct Surface {
     bool hit = false;
+    bool exhausted = false;
     Hit h;
-    const yrh_material* mat = nullptr;
+    yrh_material m{};  // the material with its textures applied
     V3 shading;
     int prim = -1;
     int sphere = -1;
     TriRef tri;
 };
 
-// TraceVisibleSurface without alpha cut-outs (all in-scope materials are opaque
-// to the alpha test). Sphere hits (src/integrators/surface_query.cpp:129-147):
-// material from the sphere, shading normal = SphereNormal (negated when
-// flip_normals), not face-forwarded.
-Surface TraceSurface(const Scene& s, V3 o, V3 d, float t_min, float t_max, orc::Counters& c) {
+// A camera ray's differentials: the directions one pixel to the right and one
+// pixel down, from the same origin (src/scene/camera.cpp:20-39).
+struct Differentials {
+    V3 dx, dy;
+};
+
+V2 InterpolateUv(const Scene& s, TriRef t, float u, float v) {
+    uint32_t i0, i1, i2;
+    TriVerts(s, t, i0, i1, i2);
+    const float w = 1.0f - u - v;
+    const float* a = s.g.v[i0].uv;
+    const float* b = s.g.v[i1].uv;
+    const float* c = s.g.v[i2].uv;
+    return V2{a[0] * w + b[0] * u + c[0] * v, a[1] * w + b[1] * u + c[1] * v};
+}
+
+// ComputeTextureFootprint (src/integrators/surface_query.cpp:35-97): the uv
+// change per pixel from where the differential rays meet the hit's plane.
+ortex::Footprint TextureFootprint(const Scene& s, TriRef t, V3 o, const Differentials* rd, V3 hp, V3 geo,
+                                  const float* o2w) {
+    if (rd == nullptr || !s.g.prim[t.prim].has_uvs) return {};
+    uint32_t i0, i1, i2;
+    TriVerts(s, t, i0, i1, i2);
+    V3 p0 = VertexPos(s, i0), p1 = VertexPos(s, i1), p2 = VertexPos(s, i2);
+    if (o2w != nullptr) {
+        p0 = orc::XformPoint(o2w, p0);
+        p1 = orc::XformPoint(o2w, p1);
+        p2 = orc::XformPoint(o2w, p2);
+    }
+    const float* uv0 = s.g.v[i0].uv;
+    const float* uv1 = s.g.v[i1].uv;
+    const float* uv2 = s.g.v[i2].uv;
+    const float du1 = uv1[0] - uv0[0], dv1 = uv1[1] - uv0[1];
+    const float du2 = uv2[0] - uv0[0], dv2 = uv2[1] - uv0[1];
+    const float uv_det = du1 * dv2 - dv1 * du2;
+    if (std::fabs(uv_det) < 1.0e-12f) return {};
+    const float inv_uv = 1.0f / uv_det;
+    const V3 e1 = Sub(p1, p0), e2 = Sub(p2, p0);
+    const V3 dpdu = Mul(Sub(Mul(e1, dv2), Mul(e2, dv1)), inv_uv);
+    const V3 dpdv = Mul(Sub(Mul(e2, du1), Mul(e1, du2)), inv_uv);
+    auto on_plane = [&](V3 dir) {
+        const float den = Dot(geo, dir);
+        if (std::fabs(den) < 1.0e-12f) return hp;
+        const float tt = Dot(geo, Sub(hp, o)) / den;
+        return Add(o, Mul(dir, tt));
+    };
+    const V3 dpdx = Sub(on_plane(rd->dx), hp);
+    const V3 dpdy = Sub(on_plane(rd->dy), hp);
+    const float a = Dot(dpdu, dpdu), b = Dot(dpdu, dpdv), c = Dot(dpdv, dpdv);
+    const float gram = a * c - b * b;
+    if (std::fabs(gram) < 1.0e-20f) return {};
+    const float inv_gram = 1.0f / gram;
+    auto solve = [&](V3 dp, float& du, float& dv) {
+        const float ur = Dot(dp, dpdu), vr = Dot(dp, dpdv);
+        du = (ur * c - vr * b) * inv_gram;
+        dv = (vr * a - ur * b) * inv_gram;
+    };
+    ortex::Footprint f;
+    solve(dpdx, f.dudx, f.dvdx);
+    solve(dpdy, f.dudy, f.dvdy);
+    if (!std::isfinite(f.dudx) || !std::isfinite(f.dvdx) || !std::isfinite(f.dudy) || !std::isfinite(f.dvdy)) return {};
+    return f;
+}
+
+// ResolveNormalMap (src/shading/material_evaluator.cpp:35-92): the map's
+// tangent-space normal (x, y scaled) in the frame of the interpolated
+// tangent, made orthogonal to the shading normal, and its handedness
+// (negated for an instance whose transform mirrors).
+V3 NormalMap(const Scene& s, TriRef t, float u, float v, const yrh_material& m, V2 uv, V3 sn, const float* o2w,
+             ortex::Footprint fp) {
+    const yrh_primitive& prim = s.g.prim[t.prim];
+    const ortex::Texture* nm = s.Tex(m.normal_map);
+    if (!prim.has_tangents || nm == nullptr) return sn;
+    const ortex::C4 e = ortex::Sample(*nm, uv.x, uv.y, fp);
+    V3 ts{e.x * 2.0f - 1.0f, e.y * 2.0f - 1.0f, e.z * 2.0f - 1.0f};
+    ts.x *= m.normal_scale;
+    ts.y *= m.normal_scale;
+    ts = Normalize(ts);
+    if (LengthSq(ts) == 0.0f) return sn;
+    uint32_t i0, i1, i2;
+    TriVerts(s, t, i0, i1, i2);
+    const float w = 1.0f - u - v;
+    const yrh_vertex &v0 = s.g.v[i0], &v1 = s.g.v[i1], &v2 = s.g.v[i2];
+    V3 tan = Normalize(V3{v0.tangent[0] * w + v1.tangent[0] * u + v2.tangent[0] * v,
+                          v0.tangent[1] * w + v1.tangent[1] * u + v2.tangent[1] * v,
+                          v0.tangent[2] * w + v1.tangent[2] * u + v2.tangent[2] * v});
+    if (o2w != nullptr) tan = orc::XformVector(o2w, tan);
+    const V3 proj = Sub(tan, Mul(sn, Dot(tan, sn)));
+    if (LengthSq(proj) <= 1.0e-12f) return sn;
+    tan = Normalize(proj);
+    if (LengthSq(tan) == 0.0f) return sn;
+    float orientation = 1.0f;
+    if (o2w != nullptr) {
+        const float* q = o2w;
+        const float det = q[0] * (q[5] * q[10] - q[9] * q[6]) - q[4] * (q[1] * q[10] - q[9] * q[2]) +
+                          q[8] * (q[1] * q[6] - q[5] * q[2]);
+        orientation = det < 0.0f ? -1.0f : 1.0f;
+    }
+    const float hand = (v0.tangent_handedness * w + v1.tangent_handedness * u + v2.tangent_handedness * v) * orientation;
+    const V3 bit = Mul(Cross(sn, tan), hand);
+    const V3 mapped = Normalize(Add(Add(Mul(tan, ts.x), Mul(bit, ts.y)), Mul(sn, ts.z)));
+    return LengthSq(mapped) > 0.0f ? mapped : sn;
+}
+
+constexpr int MaxAlphaSkips = 64;         // surface_query.cpp:12
+constexpr float AlphaSkipOffset = 1.0e-4f;  // surface_query.cpp:13
+
+// TraceVisibleSurface (src/integrators/surface_query.cpp:101-200): the closest
+// hit, its material with textures applied (EvaluateMaterialAtSurface,
+// material_evaluator.cpp:112-180) and shading normal; a hit whose alpha is
+// below 0.5 is skipped by tracing again from just past it, at most 64 times
+// (then `exhausted`). Sphere hits (:129-147): material from the sphere,
+// shading normal = SphereNormal (negated when flip_normals), not
+// face-forwarded, always opaque. `rd`: the camera ray's differentials, for
+// texture footprints (NULL for every other ray).
+Surface TraceSurface(const Scene& s, V3 o, V3 d, float t_min, float t_max, orc::Counters& c,
+                     const Differentials* rd = nullptr) {
+    for (int skip = 0; skip < MaxAlphaSkips; ++skip) {
+        Surface out;
+        const Hit h = Trace(s, o, d, t_min, t_max, c);
+        if (!h.hit || h.tri == -1) return out;
+        out.hit = true;
+        out.h = h;
+        if (h.tri <= -2) {
+            const yrh_sphere& sp = s.g.sph[-2 - h.tri];
+            const V3 hp = Add(o, Mul(d, h.t));
+            const V3 gn = Normalize(Sub(hp, Load3(sp.center)));
+            out.sphere = -2 - h.tri;
+            out.m = s.sd->materials[sp.material_index];
+            out.shading = sp.flip_normals ? Neg(gn) : gn;
+            return out;
+        }
+        out.tri = Locate(s, h.tri);
+        out.prim = out.tri.prim;
+        const yrh_primitive& prim = s.g.prim[out.prim];
+        out.m = s.sd->materials[prim.material_index];
+        const bool inst = h.inst >= 0;
+        const float* o2w = inst ? s.tl.inst[h.inst].object_to_world : nullptr;
+        V3 raw = GeometricNormal(s, out.tri);
+        if (inst) raw = TransformNormal(o2w, raw);
+        const V3 geo = Dot(raw, Neg(d)) < 0.0f ? Neg(raw) : raw;
+        const ortex::Footprint fp = TextureFootprint(s, out.tri, o, rd, Add(o, Mul(d, h.t)), geo, o2w);
+        const V2 uv = prim.has_uvs ? InterpolateUv(s, out.tri, h.u, h.v) : V2{0.0f, 0.0f};
+        float alpha = out.m.alpha;
+        if (prim.has_uvs) {
+            if (const ortex::Texture* rt = s.Tex(out.m.reflectance_texture)) {
+                const ortex::C4 tc = ortex::Sample(*rt, uv.x, uv.y, fp);
+                out.m.reflectance[0] *= tc.x;
+                out.m.reflectance[1] *= tc.y;
+                out.m.reflectance[2] *= tc.z;
+            }
+            if (const ortex::Texture* at = s.Tex(out.m.alpha_texture)) alpha *= ortex::Sample(*at, uv.x, uv.y, fp).x;
+        }
+        V3 sn;
+        if (inst) {
+            sn = TransformNormal(o2w, ResolveShadingNormal(s, out.tri, h.u, h.v, GeometricNormal(s, out.tri)));
+            if (Dot(sn, geo) < 0.0f) sn = Neg(sn);
+        } else {
+            sn = ResolveShadingNormal(s, out.tri, h.u, h.v, geo);
+        }
+        if (prim.has_uvs) sn = NormalMap(s, out.tri, h.u, h.v, out.m, uv, sn, o2w, fp);
+        const V3 wo = Neg(d);
+        out.shading = Dot(sn, wo) < 0.0f ? Neg(sn) : sn;
+        if (alpha >= 0.5f) return out;
+        t_min = h.t + AlphaSkipOffset;
+        if (t_min >= t_max) return Surface{};
+    }
     Surface out;
-    const Hit h = Trace(s, o, d, t_min, t_max, c);
-    if (!h.hit || h.tri == -1) return out;
-    out.hit = true;
-    out.h = h;
-    if (h.tri <= -2) {
-        const yrh_sphere& sp = s.g.sph[-2 - h.tri];
-        const V3 hp = Add(o, Mul(d, h.t));
-        const V3 gn = Normalize(Sub(hp, Load3(sp.center)));
-        out.sphere = -2 - h.tri;
-        out.mat = &s.sd->materials[sp.material_index];
-        out.shading = sp.flip_normals ? Neg(gn) : gn;
-        return out;
-    }
-    out.tri = Locate(s, h.tri);
-    out.prim = out.tri.prim;
-    out.mat = &s.sd->materials[s.g.prim[out.prim].material_index];
-    const bool inst = h.inst >= 0;
-    const float* o2w = inst ? s.tl.inst[h.inst].object_to_world : nullptr;
-    V3 raw = GeometricNormal(s, out.tri);
-    if (inst) raw = TransformNormal(o2w, raw);
-    const V3 geo = Dot(raw, Neg(d)) < 0.0f ? Neg(raw) : raw;
-    V3 sn;
-    if (inst) {
-        sn = TransformNormal(o2w, ResolveShadingNormal(s, out.tri, h.u, h.v, GeometricNormal(s, out.tri)));
-        if (Dot(sn, geo) < 0.0f) sn = Neg(sn);
-    } else {
-        sn = ResolveShadingNormal(s, out.tri, h.u, h.v, geo);
-    }
-    const V3 wo = Neg(d);
-    out.shading = Dot(sn, wo) < 0.0f ? Neg(sn) : sn;
+    out.exhausted = true;
     return out;
 }
 
@@ -756,26 +899,25 @@ V3 UvToDir(float u, float v, float rot) {
     const float st = std::sin(th);
     return Normalize(V3{st * std::sin(phi), std::cos(th), st * std::cos(phi)});
 }
-int Wrap(int v, int n) {
-    int w = v % n;
-    if (w < 0) w += n;
-    return w;
-}
-float Repeat(float v) {
-    const float b = std::floor(v);
-    const float f = v - b;
-    return f < 0.0f ? f + 1.0f : f;
-}
 V3 Texel(const yrh_environment_desc& e, int x, int y) {
-    const float* t = e.texels + 4 * (static_cast<size_t>(Wrap(y, e.height)) * static_cast<size_t>(e.width) +
-                                     static_cast<size_t>(Wrap(x, e.width)));
+    const float* t = e.texels + 4 * (static_cast<size_t>(ortex::WrapIndex(y, e.height, e.wrap_t)) *
+                                         static_cast<size_t>(e.width) +
+                                     static_cast<size_t>(ortex::WrapIndex(x, e.width, e.wrap_s)));
     return V3{t[0], t[1], t[2]};
 }
+// SampleTexture without a footprint on the map (texture.cpp:107-130, 312-322).
 V3 Bilinear(const yrh_environment_desc& e, float u, float v) {
     if (e.width <= 0 || e.height <= 0 || e.texels == nullptr) return V3{0, 0, 0};
+    if (e.nearest) {
+        const int x = std::clamp(static_cast<int>(std::floor(ortex::WrapCoord(u, e.wrap_s) * static_cast<float>(e.width))),
+                                 0, e.width - 1);
+        const int y = std::clamp(static_cast<int>(std::floor(ortex::WrapCoord(v, e.wrap_t) * static_cast<float>(e.height))),
+                                 0, e.height - 1);
+        return Texel(e, x, y);
+    }
     if (e.width == 1 && e.height == 1) return Load3(e.texels);
-    const float s = Repeat(u) * static_cast<float>(e.width) - 0.5f;
-    const float t = Repeat(v) * static_cast<float>(e.height) - 0.5f;
+    const float s = ortex::WrapCoord(u, e.wrap_s) * static_cast<float>(e.width) - 0.5f;
+    const float t = ortex::WrapCoord(v, e.wrap_t) * static_cast<float>(e.height) - 0.5f;
     const int s0 = static_cast<int>(std::floor(s)), t0 = static_cast<int>(std::floor(t));
     const float ds = s - static_cast<float>(s0), dt = t - static_cast<float>(t0);
     const V3 a = Add(Mul(Texel(e, s0, t0), 1.0f - ds), Mul(Texel(e, s0 + 1, t0), ds));
@@ -886,6 +1028,7 @@ bool ShadowVisible(const Scene& s, V3 o, V3 d, float max_dist, Stats& st, V3& T)
         const Surface sf = TraceSurface(s, o, d, 1.0e-5f, remaining, c);
         AddTrace(st, c);
         const bool finite = std::isfinite(remaining);
+        if (sf.exhausted) return false;
         if (!sf.hit) {
             if (med.active) {
                 if (!finite) return false;
@@ -898,7 +1041,7 @@ bool ShadowVisible(const Scene& s, V3 o, V3 d, float max_dist, Stats& st, V3& T)
             if (TransBlack(T)) return false;
         }
         const V3 hp = Add(o, Mul(d, sf.h.t));
-        const yrh_material& m = *sf.mat;
+        const yrh_material& m = sf.m;
         if (m.kind != YRH_MATERIAL_DIELECTRIC && m.kind != YRH_MATERIAL_THIN_DIELECTRIC) return false;
         if (m.kind == YRH_MATERIAL_THIN_DIELECTRIC) {
             T = Had(T, V3{std::clamp(m.reflectance[0], 0.0f, 1.0f), std::clamp(m.reflectance[1], 0.0f, 1.0f),
@@ -1120,7 +1263,8 @@ struct PathOut {
     bool primary = false;
 };
 
-PathOut Integrate(const Scene& s, const yrh_render_desc& rd, V3 o, V3 d, Sampler& smp, Stats& st) {
+PathOut Integrate(const Scene& s, const yrh_render_desc& rd, V3 o, V3 d, const Differentials& camera_diff,
+                  Sampler& smp, Stats& st) {
     PathOut out;
     V3 rad{0, 0, 0}, tp{1, 1, 1};
     bool prev_valid = false, prev_delta = false;
@@ -1132,9 +1276,9 @@ PathOut Integrate(const Scene& s, const yrh_render_desc& rd, V3 o, V3 d, Sampler
         smp.bounce = depth;
         ++st.rays;
         orc::Counters c;
-        const Surface sf = TraceSurface(s, o, d, 1.0e-5f, kInf, c);
+        const Surface sf = TraceSurface(s, o, d, 1.0e-5f, kInf, c, depth == 0 ? &camera_diff : nullptr);
         AddTrace(st, c);
-        if (!sf.hit) {
+        if (!sf.hit) {  // a miss, or 64 alpha-skipped hits (exhausted)
             ++st.misses;
             float w = 1.0f;
             if (prev_valid && !prev_delta) w = PowerHeuristic(1, prev_pdf, 1, EnvPdf(s, d));
@@ -1146,7 +1290,7 @@ PathOut Integrate(const Scene& s, const yrh_render_desc& rd, V3 o, V3 d, Sampler
         if (NearBlack(tp)) break;
         const V3 hp = Add(o, Mul(d, sf.h.t));
         const V3 wo = Neg(d);
-        const yrh_material& m = *sf.mat;
+        const yrh_material& m = sf.m;
         const V3 n = sf.shading;
         if (depth == 0) {
             out.primary = true;
@@ -1250,6 +1394,9 @@ Scene MakeScene(const yrh_geometry_desc* gd, const yrh_shading_desc* sd) {
     s.two_level = gd->accel_kind == YRH_ACCEL_TWO_LEVEL;
     s.tl = orc::TwoLevel{gd->blases, gd->blas_count, gd->instances, gd->instance_count, &gd->tlas, gd->tlas_primitives,
                          gd->tlas_primitive_count};
+    if (sd != nullptr && sd->textures != nullptr) {
+        for (int32_t k = 0; k < sd->texture_count; ++k) s.tex.push_back(ortex::Build(sd->textures[k]));
+    }
     return s;
 }
 
@@ -1279,10 +1426,13 @@ void RenderPixels(const Scene& s, const yrh_render_desc& rd, const std::vector<i
                 smp.At2(DimPixel, 0, sx, sy);
                 const float fx = static_cast<float>(x) + sx, fy = static_cast<float>(y) + sy;
                 const float aspect = w / h;
-                const float scx = (2.0f * fx / w - 1.0f) * aspect * hh;
-                const float scy = (1.0f - 2.0f * fy / h) * hh;
-                const V3 dir = Normalize(Add(Add(cf, Mul(cr, scx)), Mul(cu, scy)));
-                out[static_cast<size_t>(p)] = Integrate(s, rd, co, dir, smp, st);
+                auto direction = [&](float px, float py) {
+                    const float scx = (2.0f * px / w - 1.0f) * aspect * hh;
+                    const float scy = (1.0f - 2.0f * py / h) * hh;
+                    return Normalize(Add(Add(cf, Mul(cr, scx)), Mul(cu, scy)));
+                };
+                const Differentials diff{direction(fx + 1.0f, fy), direction(fx, fy + 1.0f)};
+                out[static_cast<size_t>(p)] = Integrate(s, rd, co, direction(fx, fy), diff, smp, st);
             }
         }
     };
@@ -1472,6 +1622,47 @@ int oracle_sample_analytic(const yrh_shading_desc* sd, int64_t n, const float* p
                                 a.valid ? 1.0f : 0.0f};
         std::memcpy(out + 10 * i, vals, sizeof(vals));
     }
+    return 0;
+}
+
+// SampleTexture4 of texture `t` at n points: uv (n x 2) and footprints
+// (n x 4: dudx, dvdx, dudy, dvdy; NULL = none); out n x 4 (RGBA).
+int oracle_sample_texture(const yrh_texture* t, int64_t n, const float* uv, const float* footprint, float* out) {
+    if (t == nullptr) return -1;
+    const ortex::Texture tex = ortex::Build(*t);
+    for (int64_t i = 0; i < n; ++i) {
+        ortex::Footprint f;
+        if (footprint != nullptr) f = ortex::Footprint{footprint[4 * i], footprint[4 * i + 1], footprint[4 * i + 2],
+                                                       footprint[4 * i + 3]};
+        const ortex::C4 c = ortex::Sample(tex, uv[2 * i], uv[2 * i + 1], f);
+        out[4 * i] = c.x;
+        out[4 * i + 1] = c.y;
+        out[4 * i + 2] = c.z;
+        out[4 * i + 3] = c.w;
+    }
+    return 0;
+}
+
+// The mip chain of `t` (levels 1..N after level 0): writes the level count to
+// *levels and, when out is not NULL, every level's texels (RGBA) one after
+// another (out must hold them all: the caller asks for the count first).
+int oracle_texture_mips(const yrh_texture* t, int32_t* levels, int64_t* texels, float* out) {
+    if (t == nullptr || levels == nullptr || texels == nullptr) return -1;
+    const ortex::Texture tex = ortex::Build(*t);
+    *levels = tex.HasData() ? static_cast<int32_t>(tex.levels.size()) : 0;
+    int64_t k = 0;
+    for (const ortex::Level& l : tex.levels) {
+        for (const ortex::C4& c : l.t) {
+            if (out != nullptr) {
+                out[4 * k] = c.x;
+                out[4 * k + 1] = c.y;
+                out[4 * k + 2] = c.z;
+                out[4 * k + 3] = c.w;
+            }
+            ++k;
+        }
+    }
+    *texels = k;
     return 0;
 }
 

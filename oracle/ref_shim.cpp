@@ -14,6 +14,7 @@
 #include <yaoray/core/rng.hpp>
 #include <yaoray/sampling/sampler.hpp>
 #include <yaoray/shading/bsdf.hpp>
+#include <yaoray/shading/texture.hpp>
 #include <yaoray/scene/render_scene.hpp>
 
 #include "../include/yrh.h"
@@ -33,8 +34,10 @@ struct RefScene {
     yr::RenderAcceleration accel;
 };
 
-// The reference material a yrh_material describes (untextured parameters).
-yr::RenderMaterial ToRefMaterial(const yrh_material& m) {
+// The reference material a yrh_material describes; texture indices outside
+// [0, ntex) become -1, which the reference reads as "no texture" as well
+// (the render appends its environment texture after the ntex material ones).
+yr::RenderMaterial ToRefMaterial(const yrh_material& m, int32_t ntex = 0) {
     yr::RenderMaterial rm;
     rm.kind = m.kind == YRH_MATERIAL_DIFFUSE ? yr::RenderMaterialKind::Diffuse
             : m.kind == YRH_MATERIAL_CONDUCTOR ? yr::RenderMaterialKind::Conductor
@@ -48,7 +51,35 @@ yr::RenderMaterial ToRefMaterial(const yrh_material& m) {
     rm.emission = yr::Color3f{m.emission[0], m.emission[1], m.emission[2]};
     rm.absorption_color = yr::Color3f{m.absorption_color[0], m.absorption_color[1], m.absorption_color[2]};
     rm.absorption_distance = m.absorption_distance;
+    auto tex = [&](int32_t k) { return k >= 0 && k < ntex ? k : -1; };
+    rm.reflectance.texture = tex(m.reflectance_texture);
+    rm.alpha.texture = tex(m.alpha_texture);
+    rm.normal_map = tex(m.normal_map);
+    rm.normal_scale = m.normal_scale;
     return rm;
+}
+
+yr::RenderTexture ToRefTexture(const yrh_texture& t) {
+    yr::RenderTexture r;
+    r.kind = yr::RenderTextureKind::Image;
+    r.filter = t.filter == YRH_FILTER_NEAREST     ? yr::TextureFilter::Nearest
+             : t.filter == YRH_FILTER_BILINEAR    ? yr::TextureFilter::Bilinear
+             : t.filter == YRH_FILTER_TRILINEAR   ? yr::TextureFilter::Trilinear
+                                                  : yr::TextureFilter::Ewa;
+    auto wrap = [](int32_t w) {
+        return w == YRH_WRAP_CLAMP ? yr::TextureWrap::ClampToEdge
+             : w == YRH_WRAP_MIRRORED ? yr::TextureWrap::MirroredRepeat
+                                      : yr::TextureWrap::Repeat;
+    };
+    r.wrap_s = wrap(t.wrap_s);
+    r.wrap_t = wrap(t.wrap_t);
+    if (t.width > 0 && t.height > 0 && t.texels != nullptr) {
+        r.width = t.width;
+        r.height = t.height;
+        r.texels.resize(static_cast<size_t>(t.width) * static_cast<size_t>(t.height));
+        std::memcpy(r.texels.data(), t.texels, r.texels.size() * 16);
+    }
+    return r;
 }
 
 }  // namespace
@@ -314,7 +345,9 @@ int ref_render(const yrh_vertex* vertices, int64_t nv, const uint32_t* indices, 
     }
     sc.spheres = g_spheres;
     sc.analytic_lights = g_analytic;
-    for (int32_t k = 0; k < sd->material_count; ++k) sc.materials.push_back(ToRefMaterial(sd->materials[k]));
+    const int32_t ntex = sd->textures != nullptr ? sd->texture_count : 0;
+    for (int32_t k = 0; k < ntex; ++k) sc.textures.push_back(ToRefTexture(sd->textures[k]));
+    for (int32_t k = 0; k < sd->material_count; ++k) sc.materials.push_back(ToRefMaterial(sd->materials[k], ntex));
     for (int32_t k = 0; k < sd->emissive_count; ++k) {
         const yrh_emissive& e = sd->emissives[k];
         sc.emissive_primitives.push_back(
@@ -329,12 +362,17 @@ int ref_render(const yrh_vertex* vertices, int64_t nv, const uint32_t* indices, 
         yr::RenderTexture tex;
         tex.width = env.width;
         tex.height = env.height;
+        tex.filter = env.nearest ? yr::TextureFilter::Nearest : yr::TextureFilter::Bilinear;
+        tex.wrap_s = env.wrap_s == YRH_WRAP_CLAMP ? yr::TextureWrap::ClampToEdge
+                   : env.wrap_s == YRH_WRAP_MIRRORED ? yr::TextureWrap::MirroredRepeat : yr::TextureWrap::Repeat;
+        tex.wrap_t = env.wrap_t == YRH_WRAP_CLAMP ? yr::TextureWrap::ClampToEdge
+                   : env.wrap_t == YRH_WRAP_MIRRORED ? yr::TextureWrap::MirroredRepeat : yr::TextureWrap::Repeat;
         tex.texels.resize(static_cast<size_t>(env.width) * static_cast<size_t>(env.height));
         std::memcpy(tex.texels.data(), env.texels, tex.texels.size() * 16);
         sc.textures.push_back(std::move(tex));
-        sc.environment.texture_index = 0;
+        sc.environment.texture_index = ntex;
         if (env.has_distribution) {
-            sc.environment_distributions.push_back(yr::BuildEnvironmentDistribution(sc.textures[0]));
+            sc.environment_distributions.push_back(yr::BuildEnvironmentDistribution(sc.textures[static_cast<size_t>(ntex)]));
             sc.environment.distribution_index = 0;
         }
     }
@@ -368,6 +406,53 @@ int ref_render(const yrh_vertex* vertices, int64_t nv, const uint32_t* indices, 
                                r.stats.misses, r.film.BadSampleCount(), static_cast<uint64_t>(r.stats.threads)};
     std::memcpy(stats, vals, sizeof(vals));
     if (seconds != nullptr) *seconds = r.stats.elapsed_seconds;
+    return 0;
+}
+
+// yr::SampleTexture4 (src/shading/texture.cpp:312-322) of texture `t` after
+// BuildTextureSamplingCaches (its mip chain), at n points: uv (n x 2),
+// footprints (n x 4: dudx, dvdx, dudy, dvdy; NULL = none); out n x 4.
+int ref_sample_texture(const yrh_texture* t, int64_t n, const float* uv, const float* footprint, float* out) {
+    std::vector<yr::RenderTexture> texs{ToRefTexture(*t)};
+    yr::BuildTextureSamplingCaches(texs);
+    for (int64_t i = 0; i < n; ++i) {
+        yr::TextureFootprint f;
+        if (footprint != nullptr) {
+            f.dudx = footprint[4 * i];
+            f.dvdx = footprint[4 * i + 1];
+            f.dudy = footprint[4 * i + 2];
+            f.dvdy = footprint[4 * i + 3];
+        }
+        const yr::Color4f c = yr::SampleTexture4(texs[0], yr::Vec2f{uv[2 * i], uv[2 * i + 1]}, f);
+        out[4 * i] = c.x;
+        out[4 * i + 1] = c.y;
+        out[4 * i + 2] = c.z;
+        out[4 * i + 3] = c.w;
+    }
+    return 0;
+}
+
+// The mip chain yr::BuildTextureMipChain (texture.cpp:241-272) makes for `t`:
+// level count (with level 0) and all texels, as oracle_texture_mips.
+int ref_texture_mips(const yrh_texture* t, int32_t* levels, int64_t* texels, float* out) {
+    yr::RenderTexture r = ToRefTexture(*t);
+    yr::BuildTextureMipChain(r);
+    *levels = r.texels.empty() ? 0 : 1 + static_cast<int32_t>(r.mip_levels.size());
+    int64_t k = 0;
+    auto put = [&](const std::vector<yr::Color4f>& v) {
+        for (const yr::Color4f& c : v) {
+            if (out != nullptr) {
+                out[4 * k] = c.x;
+                out[4 * k + 1] = c.y;
+                out[4 * k + 2] = c.z;
+                out[4 * k + 3] = c.w;
+            }
+            ++k;
+        }
+    };
+    put(r.texels);
+    for (const yr::TextureMipLevel& l : r.mip_levels) put(l.texels);
+    *texels = k;
     return 0;
 }
 

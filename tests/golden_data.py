@@ -92,7 +92,13 @@ def scene_digest(scene):
     if len(sph):
         h.update(sph.tobytes())
     sd = scene.shading_desc()
-    h.update(ctypes.string_at(sd.materials, sd.material_count * ctypes.sizeof(N.Material)))
+    size = ctypes.sizeof(N.Material)
+    raw = ctypes.string_at(sd.materials, sd.material_count * size)
+    untextured = np.array([-1, -1, -1, np.float32(1.0).view(np.int32)], np.int32).tobytes()
+    for k in range(sd.material_count):
+        rec = raw[k * size:(k + 1) * size]
+        # the texture fields enter the digest only when a material has textures
+        h.update(rec[:64] if rec[64:] == untextured else rec)
     return h.hexdigest()
 
 

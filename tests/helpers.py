@@ -264,3 +264,149 @@ def many_point_lights_scene(count=12, seed=3):
                             "intensity": tuple(rng.uniform(0.05, 0.6, 3))} for _ in range(count)])
     s.build_accel()
     return s
+
+
+# ---- textures, alpha cut-outs, normal maps ----------------------------------------
+
+
+class TexturedMesh:
+    """Quads (two triangles each) with per-vertex uvs and tangents, one
+    primitive per quad."""
+
+    def __init__(self):
+        self.v = []
+        self.i = []
+        self.p = []
+        self.areas = []
+
+    def quad(self, corners, uvs=None, material=0, normals=None, tangent=None, handedness=(1.0, 1.0, 1.0, 1.0)):
+        c = np.asarray(corners, np.float32)
+        base, first = len(self.v), len(self.i)
+        for k in range(4):
+            rec = np.zeros((), VERTEX_DTYPE)
+            rec["position"] = c[k]
+            if normals is not None:
+                rec["normal"] = normals[k] if np.ndim(normals) == 2 else normals
+            if uvs is not None:
+                rec["uv"] = uvs[k]
+            if tangent is not None:
+                rec["tangent"] = tangent[k] if np.ndim(tangent) == 2 else tangent
+            rec["tangent_handedness"] = handedness[k]
+            self.v.append(rec)
+        self.i += [base, base + 1, base + 2, base, base + 2, base + 3]
+        self.p.append((first, 6, material, normals is not None, uvs is not None, tangent is not None))
+        a = 0.5 * (np.linalg.norm(np.cross(c[1] - c[0], c[2] - c[0])) + np.linalg.norm(np.cross(c[2] - c[0], c[3] - c[0])))
+        self.areas.append(float(a))
+        return len(self.p) - 1
+
+    def arrays(self):
+        v = np.array(self.v, VERTEX_DTYPE)
+        p = np.zeros(len(self.p), PRIMITIVE_DTYPE)
+        for k, (fi, ic, m, hn, hu, ht) in enumerate(self.p):
+            p[k]["first_index"], p[k]["index_count"], p[k]["material_index"] = fi, ic, m
+            p[k]["has_normals"], p[k]["has_uvs"], p[k]["has_tangents"] = hn, hu, ht
+        return v, np.array(self.i, np.uint32), p
+
+
+def _checker(h, w, seed, lo=0.15, hi=0.95):
+    rng = np.random.default_rng(seed)
+    t = rng.uniform(lo, hi, (h, w, 4)).astype(np.float32)
+    t[..., 3] = 1.0
+    return t
+
+
+def _normal_texels(h, w, seed):
+    """Encoded tangent-space normals ((n + 1) / 2) leaning up to ~40 degrees."""
+    rng = np.random.default_rng(seed)
+    xy = rng.uniform(-0.6, 0.6, (h, w, 2))
+    n = np.concatenate([xy, np.ones((h, w, 1))], axis=2)
+    n /= np.linalg.norm(n, axis=2, keepdims=True)
+    t = np.ones((h, w, 4), np.float32)
+    t[..., :3] = (n + 1.0) * 0.5
+    return t
+
+
+def textured_scene(two_level=False, filters=None):
+    """Textured planes under an area light: an EWA-filtered repeat ground seen
+    at a grazing angle (anisotropic footprints), trilinear clamp, nearest
+    mirrored and bilinear repeat walls (uv from -1 to 2), a normal-mapped
+    quad with tangents, an alpha cut-out quad (alpha texture), a quad with
+    constant alpha 0.3 (never visible), and a thin-dielectric quad whose
+    textured reflectance tints the shadows behind it. `two_level`: every
+    quad is an instance, and the normal-mapped and cut-out quads appear a
+    second time under a mirroring, rotating transform. `filters`: the four
+    image textures' filters (default EWA, trilinear, nearest, bilinear)."""
+    f = filters or (N.FILTER_EWA, N.FILTER_TRILINEAR, N.FILTER_NEAREST, N.FILTER_BILINEAR)
+    m = TexturedMesh()
+    up = (0.0, 1.0, 0.0)
+    uvq = [(-1.0, -1.0), (2.0, -1.0), (2.0, 2.0), (-1.0, 2.0)]
+    m.quad([(-4, 0, 3), (4, 0, 3), (4, 0, -6), (-4, 0, -6)], [(-2, 1.5), (2, 1.5), (2, -3), (-2, -3)], 0)
+    m.quad([(-3, 0, -2.5), (-0.5, 0, -2.5), (-0.5, 2, -2.5), (-3, 2, -2.5)], uvq, 1)
+    m.quad([(-0.5, 0, -2.5), (1.5, 0, -2.5), (1.5, 2, -2.5), (-0.5, 2, -2.5)], uvq, 2)
+    nm = m.quad([(1.5, 0, -2.4), (3, 0, -2.4), (3, 2, -2.4), (1.5, 2, -2.4)], [(0, 0), (1, 0), (1, 1), (0, 1)], 3,
+                normals=(0.0, 0.0, 1.0), tangent=(1.0, 0.0, 0.0), handedness=(1.0, 1.0, -1.0, 1.0))
+    cut = m.quad([(-2, 0.2, 0.5), (0, 0.2, 0.5), (0, 1.6, 0.5), (-2, 1.6, 0.5)], [(0, 0), (1, 0), (1, 1), (0, 1)], 4)
+    m.quad([(0.5, 0, 1.0), (2, 0, 1.0), (2, 1, 1.0), (0.5, 1, 1.0)], [(0, 0), (1, 0), (1, 1), (0, 1)], 5)
+    m.quad([(0.3, 0.3, -0.5), (1.3, 0.3, -0.5), (1.3, 1.3, -0.5), (0.3, 1.3, -0.5)], [(0, 0), (1, 0), (1, 1), (0, 1)], 6)
+    light = m.quad([(-1, 3, -1), (1, 3, -1), (1, 3, 1), (-1, 3, 1)], None, 7)
+    v, i, p = m.arrays()
+    if two_level:
+        ip = list(range(len(p))) + [nm, cut]
+        mats = [np.eye(4, dtype=np.float32).reshape(16)] * len(p)
+        mats += [trs((4.2, 0.0, -3.5), (0.0, 1.0, 0.0), 35.0, (-1.0, 1.0, 1.0)),
+                 trs((-1.0, 0.1, 1.8), (0.3, 1.0, 0.0), -25.0, (1.0, 1.2, -0.8))]
+        s = Scene.from_arrays(v, i, p, np.array(ip, np.int32), np.array(mats, np.float32))
+    else:
+        s = Scene.from_arrays(v, i, p)
+    t = [s.add_texture(_checker(8, 8, 1), f[0], N.WRAP_REPEAT),
+         s.add_texture(_checker(3, 5, 2), f[1], N.WRAP_CLAMP),
+         s.add_texture(_checker(6, 4, 3), f[2], N.WRAP_MIRRORED),
+         s.add_texture(_checker(3, 3, 4), f[3], N.WRAP_REPEAT, N.WRAP_MIRRORED)]
+    alpha = np.ones((4, 4, 4), np.float32)
+    alpha[..., 0] = np.indices((4, 4)).sum(0) % 2
+    t.append(s.add_texture(alpha, N.FILTER_BILINEAR, N.WRAP_REPEAT))
+    t.append(s.add_texture(_normal_texels(8, 8, 5), N.FILTER_BILINEAR, N.WRAP_REPEAT))
+    s.set_materials([
+        {"kind": 0, "reflectance": (1.0, 1.0, 1.0), "reflectance_texture": t[0]},
+        {"kind": 0, "reflectance": (0.9, 0.8, 0.9), "reflectance_texture": t[1]},
+        {"kind": 1, "reflectance": (1.0, 0.9, 0.8), "uroughness": 0.3, "reflectance_texture": t[2]},
+        {"kind": 0, "reflectance": (0.8, 0.8, 0.8), "reflectance_texture": t[3], "normal_map": t[5],
+         "normal_scale": 0.8},
+        {"kind": 0, "reflectance": (0.7, 0.3, 0.2), "alpha_texture": t[4]},
+        {"kind": 0, "reflectance": (0.2, 0.7, 0.3), "alpha": 0.3},
+        {"kind": 3, "reflectance": (1.0, 1.0, 1.0), "ior": 1.5, "reflectance_texture": t[3]},
+        {"kind": 0, "reflectance": (0.0, 0.0, 0.0), "emission": (6.0, 6.0, 6.0)},
+    ])
+    s.set_emissives([(light, (6.0, 6.0, 6.0), m.areas[light])])
+    s.set_constant_environment((0.10, 0.12, 0.15), 1.0)
+    fwd = np.array([0.0, -0.4, -1.0])
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, up)
+    right /= np.linalg.norm(right)
+    s.set_camera((0.3, 2.2, 5.0), fwd, right, np.cross(right, fwd), 0.9)
+    s.build_accel()
+    return s
+
+
+def alpha_stack_scene(layers=70, alpha=0.3):
+    """`layers` parallel quads of constant alpha `alpha` between the camera and
+    a lit diffuse wall, the light off to the side on the camera's side of the
+    stack (part of the wall sees it through the stack): with more than 64 layers
+    every camera ray and shadow ray through the stack exhausts the alpha skips
+    (a miss for the path, blocked for the shadow)."""
+    m = TexturedMesh()
+    m.quad([(-2, -2, -4), (2, -2, -4), (2, 2, -4), (-2, 2, -4)], None, 0)
+    for k in range(layers):
+        z = -3.0 + 2.5 * k / layers
+        m.quad([(-1, -1, z), (1, -1, z), (1, 1, z), (-1, 1, z)], [(0, 0), (1, 0), (1, 1), (0, 1)], 1)
+    light = m.quad([(1.5, -0.5, 1.0), (2.5, -0.5, 1.0), (2.5, 0.5, 1.0), (1.5, 0.5, 1.0)][::-1], None, 2)
+    v, i, p = m.arrays()
+    s = Scene.from_arrays(v, i, p)
+    s.set_materials([{"kind": 0, "reflectance": (0.8, 0.7, 0.6)}, {"kind": 0, "reflectance": (0.9, 0.1, 0.1),
+                                                                    "alpha": alpha},
+                     {"kind": 0, "reflectance": (0.0, 0.0, 0.0), "emission": (4.0, 4.0, 4.0)}])
+    s.set_emissives([(light, (4.0, 4.0, 4.0), m.areas[light])])
+    s.set_constant_environment((0.05, 0.05, 0.05), 1.0)
+    s.set_camera((0.0, 0.0, 3.0), (0.0, 0.0, -1.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), 0.9)
+    s.build_accel()
+    return s

@@ -97,7 +97,8 @@ def validate_geometry(desc: N.GeometryDesc):
 
 def material_array(materials):
     """ctypes array of native.Material from dicts (kind, reflectance, ior, uroughness, vroughness,
-    alpha, emission, absorption_distance, absorption_color; missing keys take the defaults)."""
+    alpha, emission, absorption_distance, absorption_color, reflectance_texture, alpha_texture,
+    normal_map, normal_scale; missing keys take the defaults, -1 = no texture)."""
     arr = (N.Material * max(1, len(materials)))()
     for k, m in enumerate(materials):
         a = arr[k]
@@ -110,6 +111,10 @@ def material_array(materials):
         a.emission[:] = m.get("emission", (0.0, 0.0, 0.0))
         a.absorption_distance = m.get("absorption_distance", 1.0)
         a.absorption_color[:] = m.get("absorption_color", (1.0, 1.0, 1.0))
+        a.reflectance_texture = m.get("reflectance_texture", -1)
+        a.alpha_texture = m.get("alpha_texture", -1)
+        a.normal_map = m.get("normal_map", -1)
+        a.normal_scale = m.get("normal_scale", 1.0)
     return arr
 
 
@@ -198,6 +203,17 @@ class Scene:
                                 "cone_cos_inner": a.cone_cos_inner} for a in al])
         env = g("environment").view(np.float32)
         s.set_constant_environment(env[:3], float(env[3]))
+        recs = g("textures").view(np.int32).reshape(-1, 6) if "textures" in sections else np.zeros((0, 6), np.int32)
+        texels = g("texels").view(np.float32) if "texels" in sections else np.zeros(0, np.float32)
+        off = 0
+        for w, h, filt, ws, wt, _ in recs:
+            n = int(w) * int(h) * 4
+            s.add_texture(texels[off:off + n].reshape(int(h), int(w), 4), int(filt), int(ws), int(wt))
+            off += n
+        if "env_map" in sections:
+            em = g("env_map").view(np.int32)
+            if em[0]:
+                s.set_environment_map(int(em[1]), env[:3], float(env[3]), float(em[2:3].view(np.float32)[0]))
         s.build_accel()
         st = g("settings").view(np.int64)
         return s, {"width": int(st[0]), "height": int(st[1]), "spp": int(st[2]), "max_depth": int(st[3]),
@@ -278,6 +294,32 @@ class Scene:
         arr = material_array(materials)
         if self._lib.yrhost_scene_set_materials(self._h, C.cast(arr, C.c_void_p), len(materials)) != 0:
             raise YrhError("set_materials failed")
+
+    def add_texture(self, texels, filter=N.FILTER_BILINEAR, wrap_s=N.WRAP_REPEAT, wrap_t=None) -> int:
+        """Appends an RGBA texture (texels: height x width x 4 linear floats, row 0 first;
+        filter / wraps: native.FILTER_* / WRAP_*; wrap_t defaults to wrap_s) to the
+        scene's texture list and returns its index, which materials name in
+        reflectance_texture / alpha_texture / normal_map."""
+        t = np.ascontiguousarray(texels, np.float32)
+        if t.ndim != 3 or t.shape[2] != 4:
+            raise ValueError("texels must be height x width x 4")
+        d = N.Texture()
+        d.height, d.width = int(t.shape[0]), int(t.shape[1])
+        d.filter = int(filter)
+        d.wrap_s = int(wrap_s)
+        d.wrap_t = int(wrap_s if wrap_t is None else wrap_t)
+        d.texels = _ptr(t)
+        rc = self._lib.yrhost_scene_add_texture(self._h, C.byref(d))
+        if rc < 0:
+            raise YrhError("add_texture failed: " + self._lib.yrhost_scene_error(self._h).decode())
+        return rc
+
+    def set_environment_map(self, texture_index, radiance=(1.0, 1.0, 1.0), strength=1.0, rotation_radians=0.0):
+        """An active equirect environment on texture `texture_index` (bilinear, repeat)."""
+        r = (C.c_float * 3)(*radiance)
+        if self._lib.yrhost_scene_set_environment_map(self._h, int(texture_index), r, float(strength),
+                                                      float(rotation_radians)) != 0:
+            raise YrhError("set_environment_map failed: " + self._lib.yrhost_scene_error(self._h).decode())
 
     def set_spheres(self, spheres):
         """spheres: list of (center (3), radius, material_index[, flip_normals]) (yr::RenderSphere);
@@ -675,6 +717,23 @@ class Device:
         out = np.zeros((len(q), 12), np.float32)
         self._check(self._lib.yrh_bsdf_query(self._ctx, C.cast(arr, C.c_void_p), len(materials), len(q),
                                              idx.ctypes.data, q.ctypes.data, out.ctypes.data), "bsdf_query")
+        return out
+
+    def texture_query(self, texels, uv, footprint=None, filter=N.FILTER_BILINEAR, wrap_s=N.WRAP_REPEAT, wrap_t=None):
+        """Device SampleTexture4 of an RGBA texture (height x width x 4) at uv (n x 2) with
+        footprints (n x 4: dudx, dvdx, dudy, dvdy) or none; returns n x 4."""
+        t = np.ascontiguousarray(texels, np.float32)
+        d = N.Texture()
+        d.height, d.width = (int(t.shape[0]), int(t.shape[1])) if t.size else (0, 0)
+        d.filter, d.wrap_s = int(filter), int(wrap_s)
+        d.wrap_t = int(wrap_s if wrap_t is None else wrap_t)
+        d.texels = t.ctypes.data if t.size else None
+        q = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        fp = None if footprint is None else np.ascontiguousarray(footprint, np.float32).reshape(-1, 4)
+        out = np.zeros((len(q), 4), np.float32)
+        self._check(self._lib.yrh_texture_query(self._ctx, C.byref(d), len(q), q.ctypes.data,
+                                                None if fp is None else fp.ctypes.data, out.ctypes.data),
+                    "texture_query")
         return out
 
     def camera_rays(self, desc: N.RenderDesc):

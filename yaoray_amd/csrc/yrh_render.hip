@@ -16,6 +16,7 @@
 #include "yrh_internal.h"
 #include "yrh_guide.h"
 #include "yrh_shading.h"
+#include "yrh_texture.h"
 #include "yrh_traverse.h"
 
 #include <algorithm>
@@ -31,6 +32,12 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxTransparentShadowHits = 16;  // src/integrators/path_direct_lighting.cpp:19
+constexpr int kMaxAlphaSkips = 64;              // src/integrators/surface_query.cpp:12
+constexpr float kAlphaSkipOffset = 1.0e-4f;     // surface_query.cpp:13
+// prim_flags bits
+constexpr int kHasNormals = 1;
+constexpr int kHasUvs = 2;
+constexpr int kHasTangents = 4;
 
 // Counter words of one pass (uint32).
 enum Counter : int {
@@ -82,8 +89,12 @@ struct RenderState {
     DevBuf spheres;      // float4 per sphere: centre, radius
     DevBuf sphere_info;  // int2 per sphere: material index, flip_normals
     DevBuf prim_info;    // int4 per primitive: material, emissive index, first index, flat offset
-    DevBuf prim_flags;   // int32 per primitive: bit0 has_normals
-    DevBuf inst_normal;  // float4[6] per instance: normal-matrix rows (+inv det, singular), o2w rows
+    DevBuf prim_flags;   // int32 per primitive: kHasNormals | kHasUvs | kHasTangents
+    DevBuf uvs;          // float2 per vertex (when a primitive has uvs)
+    DevBuf tangents;     // float4 per vertex: tangent, handedness (when a primitive has tangents)
+    // float4[6] per instance: normal-matrix rows (.w: 1 / det, singular flag,
+    // orientation sign), object-to-world rows (.w: translation)
+    DevBuf inst_normal;
     int32_t primitive_count = 0;
     int64_t triangle_count = 0;
     int64_t vertex_count = 0;
@@ -98,6 +109,14 @@ struct RenderState {
     bool transparent = false;  // any dielectric / thin dielectric material
     DevBuf materials;          // DevMaterial
     int32_t material_count = 0;
+    // textures (yrh_texture.h layout) and per material (reflectance texture,
+    // alpha texture, normal map, normal scale bits); texture indices are -1
+    // where the material has none
+    DevBuf tex_records, tex_levels, tex_texels, tex_gauss, mat_tex;
+    int32_t texture_count = 0;
+    bool textured = false;    // a material reads a texture
+    bool alpha = false;       // a material's hits may be alpha-skipped
+    bool footprints = false;  // a material's texture filters with camera-ray footprints
     DevBuf emit;               // float4 (radiance, stored area)
     DevBuf emit_info;          // int4 (primitive, cdf offset, cdf count, 0)
     DevBuf emit_area;          // float: summed triangle area (SelectTriangleByArea)
@@ -126,6 +145,7 @@ struct RenderState {
     float env_rotation = 0.0f;
     DevBuf env_texels;  // float4
     int32_t env_w = 0, env_h = 0;
+    int32_t env_nearest = 0, env_ws = 0, env_wt = 0;
     int32_t env_sampleable = 0;
     DevBuf env_weights, env_row_weights;
     // CDFs padded with +inf to a multiple of kCdfBlock entries (rows of the
@@ -147,6 +167,7 @@ struct RenderState {
     // ---- pass buffers (capacity = paths per pass) ----
     int64_t capacity = 0;
     DevBuf p_pixel, p_tp, p_rad, p_prev, p_flags, p_med, p_alb, p_nrm, p_nee, p_slot;  // p_slot: int4 shadow entries
+    DevBuf p_diff;  // float4[2] per path: the camera ray's differential directions (footprints only)
     DevBuf q_org[2], q_dir[2], q_path[2];
     DevBuf hits, hit_inst;
     DevBuf s_org, s_dir, s_li, s_bsdf, s_occ, s_trans;
@@ -198,10 +219,14 @@ struct SurfaceView {
     const float4* inst_normal;
     const float4* spheres;     // centre, radius
     const int2* sphere_info;   // material index, flip_normals
+    const float2* uvs;
+    const float4* tangents;
 };
 
 struct LightView {
     const DevMaterial* materials;
+    const int4* mat_tex;  // per material: reflectance texture, alpha texture, normal map, normal scale bits
+    tex::TexView tex;
     const float4* emit;
     const int4* emit_info;
     const float* emit_area;
@@ -227,6 +252,7 @@ struct LightView {
     float env_rotation;
     const float4* env_texels;
     int32_t env_w, env_h;
+    int32_t env_nearest, env_ws, env_wt;
     int32_t env_sampleable;
     const float* env_weights;
     const float* env_row_weights;
@@ -274,6 +300,7 @@ struct PassView {
     float4* p_nrm;
     float4* p_nee;
     int4* p_slot;  // shadow-queue entries of the path's vertex: emissive, analytic, environment
+    float4* p_diff;  // camera-ray differential directions (2 per path), or nullptr
     // queues
     float4* q_org[2];
     float4* q_dir[2];
@@ -390,41 +417,19 @@ __device__ __forceinline__ float3 UvToDir(float2 uv, float rot) {
     return Normalize(V(st * sinf(phi), cosf(theta), st * cosf(phi)));
 }
 
-__device__ __forceinline__ int Wrap(int v, int count) {
-    int w = v % count;
-    if (w < 0) w += count;
-    return w;
-}
-
-__device__ __forceinline__ float ApplyRepeat(float v) {
-    const float base = floorf(v);
-    const float f = v - base;
-    return f < 0.0f ? f + 1.0f : f;
-}
-
-// SampleBilinear at level 0 with repeat wrap (src/shading/texture.cpp:116-130).
-__device__ __forceinline__ float3 Bilinear(const float4* tex, int w, int h, float2 uv) {
-    if (w <= 0 || h <= 0) return V(0, 0, 0);
-    if (w == 1 && h == 1) return XYZ(tex[0]);
-    const float s = ApplyRepeat(uv.x) * static_cast<float>(w) - 0.5f;
-    const float t = ApplyRepeat(uv.y) * static_cast<float>(h) - 0.5f;
-    const int s0 = static_cast<int>(floorf(s));
-    const int t0 = static_cast<int>(floorf(t));
-    const float ds = s - static_cast<float>(s0);
-    const float dt = t - static_cast<float>(t0);
-    const float4 c00 = tex[Wrap(t0, h) * w + Wrap(s0, w)];
-    const float4 c10 = tex[Wrap(t0, h) * w + Wrap(s0 + 1, w)];
-    const float4 c01 = tex[Wrap(t0 + 1, h) * w + Wrap(s0, w)];
-    const float4 c11 = tex[Wrap(t0 + 1, h) * w + Wrap(s0 + 1, w)];
-    const float3 a = Add(Mul(XYZ(c00), 1.0f - ds), Mul(XYZ(c10), ds));
-    const float3 b = Add(Mul(XYZ(c01), 1.0f - ds), Mul(XYZ(c11), ds));
-    return Add(Mul(a, 1.0f - dt), Mul(b, dt));
+// SampleTexture without a footprint on the map (src/shading/texture.cpp:107-130,
+// 312-322): bilinear, or nearest, with the map's wrap modes.
+__device__ __forceinline__ float3 EnvLookup(const LightView& L, float2 uv) {
+    if (L.env_w <= 0 || L.env_h <= 0) return V(0, 0, 0);
+    const tex::Level l{L.env_texels, L.env_w, L.env_h};
+    return XYZ(L.env_nearest ? tex::Nearest(l, uv.x, uv.y, L.env_ws, L.env_wt)
+                             : tex::Bilinear(l, uv.x, uv.y, L.env_ws, L.env_wt));
 }
 
 __device__ __forceinline__ float3 EnvColor(const LightView& L, float3 d) {
     if (!L.env_active) return Mul(L.env_radiance, L.env_strength);
     if (L.env_strength <= 0.0f || L.env_texels == nullptr) return V(0, 0, 0);
-    const float3 c = Bilinear(L.env_texels, L.env_w, L.env_h, DirToUv(d, L.env_rotation));
+    const float3 c = EnvLookup(L, DirToUv(d, L.env_rotation));
     return Mul(Hadamard(c, L.env_radiance), L.env_strength);
 }
 
@@ -813,6 +818,13 @@ __global__ void RaygenKernel(PassView P) {
     const float3 d = CameraDirection(camera, w, h, static_cast<float>(x) + s.x, static_cast<float>(y) + s.y);
     P.q_org[0][p] = F4(camera.origin, 1.0e-5f);
     P.q_dir[0][p] = F4(d, __int_as_float(0x7f800000));
+    if (P.p_diff != nullptr) {
+        // The ray differentials one pixel right and down (GeneratePerspectiveCameraRay,
+        // src/scene/camera.cpp:20-39), for texture footprints at the first hit.
+        const float fx = static_cast<float>(x) + s.x, fy = static_cast<float>(y) + s.y;
+        P.p_diff[2 * p] = F4(CameraDirection(camera, w, h, fx + 1.0f, fy), 0.0f);
+        P.p_diff[2 * p + 1] = F4(CameraDirection(camera, w, h, fx, fy + 1.0f), 0.0f);
+    }
     P.q_path[0][p] = static_cast<int32_t>(p);
     P.p_pixel[p] = static_cast<int32_t>(pixel);
     P.p_tp[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -860,11 +872,132 @@ struct SurfacePoint {
     DevMaterial mat;
 };
 
+// ---- textures at a triangle hit (material_evaluator.cpp:35-149, surface_query.cpp:35-97)
+
+__device__ __forceinline__ float2 HitUv(const SurfaceView& S, uint32_t i0, uint32_t i1, uint32_t i2, float u, float v) {
+    const float w = 1.0f - u - v;
+    const float2 a = S.uvs[i0], b = S.uvs[i1], c = S.uvs[i2];
+    return make_float2(a.x * w + b.x * u + c.x * v, a.y * w + b.y * u + c.y * v);
+}
+
+// TransformPoint / TransformVector (src/core/transform.cpp:30-44) with the
+// instance's object-to-world rows.
+__device__ __forceinline__ float3 XformPointRows(const float4* rows, float3 p) {
+    const float4 m0 = rows[3], m1 = rows[4], m2 = rows[5];
+    return V(m0.x * p.x + m0.y * p.y + m0.z * p.z + m0.w, m1.x * p.x + m1.y * p.y + m1.z * p.z + m1.w,
+             m2.x * p.x + m2.y * p.y + m2.z * p.z + m2.w);
+}
+__device__ __forceinline__ float3 XformVectorRows(const float4* rows, float3 v) {
+    const float4 m0 = rows[3], m1 = rows[4], m2 = rows[5];
+    return V(m0.x * v.x + m0.y * v.y + m0.z * v.z, m1.x * v.x + m1.y * v.y + m1.z * v.z,
+             m2.x * v.x + m2.y * v.y + m2.z * v.z);
+}
+
+// ComputeTextureFootprint (surface_query.cpp:35-97): the uv change per pixel
+// where the camera ray's differential rays (directions diff[0], diff[1] from
+// the same origin o) meet the hit's plane. p0..p2: object-space corners.
+__device__ tex::Footprint HitFootprint(const SurfaceView& S, const float4* nrows, uint32_t i0, uint32_t i1, uint32_t i2,
+                                       float3 p0, float3 p1, float3 p2, float3 o, const float4* diff, float3 hp,
+                                       float3 geo) {
+    const tex::Footprint none = tex::NoFootprint();
+    if (nrows != nullptr) {
+        p0 = XformPointRows(nrows, p0);
+        p1 = XformPointRows(nrows, p1);
+        p2 = XformPointRows(nrows, p2);
+    }
+    const float2 uv0 = S.uvs[i0], uv1 = S.uvs[i1], uv2 = S.uvs[i2];
+    const float du1 = uv1.x - uv0.x, dv1 = uv1.y - uv0.y;
+    const float du2 = uv2.x - uv0.x, dv2 = uv2.y - uv0.y;
+    const float uv_det = du1 * dv2 - dv1 * du2;
+    if (fabsf(uv_det) < 1.0e-12f) return none;
+    const float inv_uv = 1.0f / uv_det;
+    const float3 e1 = Sub(p1, p0), e2 = Sub(p2, p0);
+    const float3 dpdu = Mul(Sub(Mul(e1, dv2), Mul(e2, dv1)), inv_uv);
+    const float3 dpdv = Mul(Sub(Mul(e2, du1), Mul(e1, du2)), inv_uv);
+    float3 on[2];
+    for (int k = 0; k < 2; ++k) {
+        const float3 dir = XYZ(diff[k]);
+        const float den = Dot(geo, dir);
+        if (fabsf(den) < 1.0e-12f) {
+            on[k] = hp;
+        } else {
+            const float tt = Dot(geo, Sub(hp, o)) / den;
+            on[k] = Add(o, Mul(dir, tt));
+        }
+    }
+    const float3 dpdx = Sub(on[0], hp), dpdy = Sub(on[1], hp);
+    const float a = Dot(dpdu, dpdu), b = Dot(dpdu, dpdv), c = Dot(dpdv, dpdv);
+    const float gram = a * c - b * b;
+    if (fabsf(gram) < 1.0e-20f) return none;
+    const float inv_gram = 1.0f / gram;
+    const float ux = Dot(dpdx, dpdu), vx = Dot(dpdx, dpdv), uy = Dot(dpdy, dpdu), vy = Dot(dpdy, dpdv);
+    const tex::Footprint f{(ux * c - vx * b) * inv_gram, (vx * a - ux * b) * inv_gram, (uy * c - vy * b) * inv_gram,
+                           (vy * a - uy * b) * inv_gram};
+    if (!isfinite(f.dudx) || !isfinite(f.dvdx) || !isfinite(f.dudy) || !isfinite(f.dvdy)) return none;
+    return f;
+}
+
+// ResolveNormalMap (material_evaluator.cpp:35-92): the map's tangent-space
+// normal (x, y scaled) in the frame of the interpolated tangent made
+// orthogonal to the shading normal, with the interpolated handedness
+// (negated under a mirroring instance transform).
+__device__ float3 NormalMapped(const SurfaceView& S, const LightView& L, int map, float scale, uint32_t i0, uint32_t i1,
+                               uint32_t i2, float u, float v, float2 uv, float3 sn, const float4* nrows,
+                               const tex::Footprint& fp) {
+    const float4 e = tex::Sample(L.tex, map, uv.x, uv.y, fp);
+    float3 ts = V(e.x * 2.0f - 1.0f, e.y * 2.0f - 1.0f, e.z * 2.0f - 1.0f);
+    ts.x *= scale;
+    ts.y *= scale;
+    ts = Normalize(ts);
+    if (LengthSq(ts) == 0.0f) return sn;
+    const float w = 1.0f - u - v;
+    const float4 t0 = S.tangents[i0], t1 = S.tangents[i1], t2 = S.tangents[i2];
+    float3 tan = Normalize(V(t0.x * w + t1.x * u + t2.x * v, t0.y * w + t1.y * u + t2.y * v,
+                             t0.z * w + t1.z * u + t2.z * v));
+    if (nrows != nullptr) tan = XformVectorRows(nrows, tan);
+    const float3 proj = Sub(tan, Mul(sn, Dot(tan, sn)));
+    if (LengthSq(proj) <= 1.0e-12f) return sn;
+    tan = Normalize(proj);
+    if (LengthSq(tan) == 0.0f) return sn;
+    const float orientation = nrows != nullptr ? nrows[2].w : 1.0f;
+    const float hand = (t0.w * w + t1.w * u + t2.w * v) * orientation;
+    const float3 bit = Mul(Cross(sn, tan), hand);
+    const float3 mapped = Normalize(Add(Add(Mul(tan, ts.x), Mul(bit, ts.y)), Mul(sn, ts.z)));
+    return LengthSq(mapped) > 0.0f ? mapped : sn;
+}
+
+// The alpha EvaluateMaterialAtSurface gives a triangle hit (material_evaluator.cpp:135-149):
+// the material's alpha, times the alpha texture's first channel where the
+// primitive has uvs. `diff`: the camera ray's differentials (footprint), or nullptr.
+__device__ float HitAlpha(const SurfaceView& S, const LightView& L, int tri, int inst, float u, float v, float3 o,
+                          float3 d, float t, const float4* diff) {
+    const int prim = S.tri_prim[tri];
+    const int4 pinfo = S.prim_info[prim];
+    const float alpha = L.materials[pinfo.x].alpha;
+    const int at = L.mat_tex[pinfo.x].y;
+    if (at < 0 || !(S.prim_flags[prim] & kHasUvs)) return alpha;
+    const int64_t base = static_cast<int64_t>(pinfo.z) + static_cast<int64_t>(tri - pinfo.w) * 3;
+    const uint32_t i0 = S.indices[base], i1 = S.indices[base + 1], i2 = S.indices[base + 2];
+    const float2 uv = HitUv(S, i0, i1, i2, u, v);
+    tex::Footprint fp = tex::NoFootprint();
+    if (diff != nullptr) {
+        const float3 p0 = Pos(S, i0), p1 = Pos(S, i1), p2 = Pos(S, i2);
+        const float3 obj_geo = Normalize(Cross(Sub(p1, p0), Sub(p2, p0)));
+        const float4* nrows = inst >= 0 ? S.inst_normal + 6 * static_cast<int64_t>(inst) : nullptr;
+        const float3 raw = nrows != nullptr ? TransformNormalDev(nrows, obj_geo) : obj_geo;
+        const float3 geo = Dot(raw, Neg(d)) < 0.0f ? Neg(raw) : raw;
+        fp = HitFootprint(S, nrows, i0, i1, i2, p0, p1, p2, o, diff, Add(o, Mul(d, t)), geo);
+    }
+    return alpha * tex::Sample(L.tex, at, uv.x, uv.y, fp).x;
+}
+
 // TraceVisibleSurface's hit resolution (src/integrators/surface_query.cpp:101-200):
-// material, hit point, object-space geometric normal and the face-forwarded
-// shading normal.
+// material (kTex: with its reflectance texture applied), hit point,
+// object-space geometric normal and the face-forwarded shading normal (kTex:
+// normal-mapped). `diff`: the camera ray's differentials, or nullptr.
+template <bool kTex>
 __device__ __forceinline__ SurfacePoint ResolveSurface(const SurfaceView& S, const LightView& L, float3 org,
-                                                       float3 dir, float4 h, int tri, int inst) {
+                                                       float3 dir, float4 h, int tri, int inst, const float4* diff) {
     const float t = h.x;
     if (trav::IsSphereId(tri)) {
         // Sphere hit (surface_query.cpp:129-146): outward normal at ray.At(t)
@@ -885,7 +1018,7 @@ __device__ __forceinline__ SurfacePoint ResolveSurface(const SurfaceView& S, con
     }
     const int prim = S.tri_prim[tri];
     const int4 pinfo = S.prim_info[prim];
-    const DevMaterial mat = LoadMaterial(L.materials, pinfo.x);
+    DevMaterial mat = LoadMaterial(L.materials, pinfo.x);
     const float3 hp = Add(org, Mul(dir, t));
     const float3 wo = Neg(dir);
     const int64_t base = static_cast<int64_t>(pinfo.z) + static_cast<int64_t>(tri - pinfo.w) * 3;
@@ -895,11 +1028,28 @@ __device__ __forceinline__ SurfacePoint ResolveSurface(const SurfaceView& S, con
     const float4* nrows = inst >= 0 ? S.inst_normal + 6 * static_cast<int64_t>(inst) : nullptr;
     const float3 raw = inst >= 0 ? TransformNormalDev(nrows, obj_geo) : obj_geo;
     const float3 geo = Dot(raw, wo) < 0.0f ? Neg(raw) : raw;
+    const int pflags = S.prim_flags[prim];
+    int normal_map = -1;
+    float normal_scale = 1.0f;
+    float2 uv = make_float2(0.0f, 0.0f);
+    tex::Footprint fp = tex::NoFootprint();
+    if (kTex && (pflags & kHasUvs)) {
+        const int4 mt = L.mat_tex[pinfo.x];
+        if (mt.x >= 0 || mt.z >= 0) {
+            uv = HitUv(S, i0, i1, i2, h.y, h.z);
+            if (diff != nullptr) fp = HitFootprint(S, nrows, i0, i1, i2, p0, p1, p2, org, diff, hp, geo);
+            if (mt.x >= 0) mat.reflectance = Hadamard(mat.reflectance, XYZ(tex::Sample(L.tex, mt.x, uv.x, uv.y, fp)));
+            if (pflags & kHasTangents) {
+                normal_map = mt.z;
+                normal_scale = __int_as_float(mt.w);
+            }
+        }
+    }
     // ResolveShadingNormal (src/geometry/intersection.cpp:84-106) on the object-space
     // normal for instances, on the face-forwarded normal otherwise.
     const float3 ref = inst >= 0 ? obj_geo : geo;
     float3 sn = Normalize(ref);
-    if (S.prim_flags[prim] & 1) {
+    if (pflags & kHasNormals) {
         const float u = h.y, v = h.z, w = 1.0f - u - v;
         const float3 n0 = XYZ(S.normals[i0]), n1 = XYZ(S.normals[i1]), n2 = XYZ(S.normals[i2]);
         const float3 ni = Normalize(V(n0.x * w + n1.x * u + n2.x * v, n0.y * w + n1.y * u + n2.y * v,
@@ -912,6 +1062,7 @@ __device__ __forceinline__ SurfacePoint ResolveSurface(const SurfaceView& S, con
         sn = TransformNormalDev(nrows, sn);
         if (Dot(sn, geo) < 0.0f) sn = Neg(sn);
     }
+    if (kTex && normal_map >= 0) sn = NormalMapped(S, L, normal_map, normal_scale, i0, i1, i2, h.y, h.z, uv, sn, nrows, fp);
     if (Dot(sn, wo) < 0.0f) sn = Neg(sn);
     SurfacePoint out;
     out.hp = hp;
@@ -977,7 +1128,7 @@ __device__ __forceinline__ BsdfSample SampleGuided(const DevMaterial& m, float3 
 }
 
 // kAnalytic: the scene has analytic lights (a third shadow entry per vertex).
-template <bool kGuided, bool kAnalytic>
+template <bool kGuided, bool kAnalytic, bool kTex>
 __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& S, const LightView& L,
                                           const yrg::GuideView& G, int depth, int i, int cur, ShadeOut& o,
                                           unsigned& hit_count, unsigned& miss_count) {
@@ -1028,7 +1179,8 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
         P.p_tp[p] = F4(tp, 0.0f);
         return;
     }
-    const SurfacePoint sp = ResolveSurface(S, L, org, dir, h, tri, inst);
+    // P.p_diff is set for the camera rays' bounce only (texture footprints).
+    const SurfacePoint sp = ResolveSurface<kTex>(S, L, org, dir, h, tri, inst, P.p_diff != nullptr ? P.p_diff + 2 * p : nullptr);
     const int4 pinfo = sp.pinfo;
     const DevMaterial& mat = sp.mat;
     const float3 hp = sp.hp;
@@ -1201,7 +1353,7 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
     P.p_flags[p] = flags;
 }
 
-template <bool kGuided, bool kAnalytic>
+template <bool kGuided, bool kAnalytic, bool kTex>
 __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S, LightView L, yrg::GuideView G,
                                                      int depth, int cur) {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -1211,7 +1363,7 @@ __global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S,
     o.shadow[kShadowEmissive] = o.shadow[kShadowAnalytic] = o.shadow[kShadowEnv] = false;
     unsigned hit_count = 0, miss_count = 0;
     const bool valid = i < n;
-    if (valid) ShadePath<kGuided, kAnalytic>(P, S, L, G, depth, i, cur, o, hit_count, miss_count);
+    if (valid) ShadePath<kGuided, kAnalytic, kTex>(P, S, L, G, depth, i, cur, o, hit_count, miss_count);
     const int nxt = 1 - cur;
     __shared__ uint32_t append_lds[kBlock / 64 + 1];
     const int32_t slot = BlockAppend(P.counters + nxt, o.next ? 1u : 0u, append_lds);
@@ -1264,6 +1416,7 @@ __device__ __forceinline__ void WaveSync() {
 }
 constexpr int kQueryWaves = kQueryBlock / 64;
 
+template <bool kTex>
 __global__ __launch_bounds__(kQueryBlock) void GuideQueryKernel(PassView P, SurfaceView S, LightView L,
                                                                yrg::GuideView G, int cur) {
     // Per wave, first the inputs of its 64 rows (stride 11: odd, no bank
@@ -1284,7 +1437,9 @@ __global__ __launch_bounds__(kQueryBlock) void GuideQueryKernel(PassView P, Surf
         const int tri = __float_as_int(h.w);
         if (tri != -1) {
             const int inst = P.hit_inst != nullptr ? P.hit_inst[i] : -1;
-            const SurfacePoint sp = ResolveSurface(S, L, XYZ(P.q_org[cur][i]), XYZ(P.q_dir[cur][i]), h, tri, inst);
+            const float4* diff = P.p_diff != nullptr ? P.p_diff + 2 * P.q_path[cur][i] : nullptr;
+            const SurfacePoint sp =
+                ResolveSurface<kTex>(S, L, XYZ(P.q_org[cur][i]), XYZ(P.q_dir[cur][i]), h, tri, inst, diff);
             guidable = Guidable(sp.mat);
             hp = sp.hp;
             sn = sp.sn;
@@ -1463,6 +1618,7 @@ __global__ __launch_bounds__(kBlock) void ResolveKernel(PassView P, yrg::GuideVi
 
 __device__ __forceinline__ bool TransBlack(float3 c) { return MaxC(c) <= 1.0e-6f; }
 
+template <bool kAlpha>
 struct ShadowWalk {
     const PassView* P;
     const SurfaceView* S;
@@ -1476,8 +1632,8 @@ struct ShadowWalk {
         P->s_trans[e] = make_float4(1.0f, 1.0f, 1.0f, d.w);
     }
 
-    __device__ __forceinline__ bool Finish(int32_t e, const trav::Best& best, int32_t, uint32_t, uint32_t, uint32_t,
-                                           const float4& lo, const float4& ld, float4& next_org,
+    __device__ __forceinline__ bool Finish(int32_t e, const trav::Best& best, int32_t inst, uint32_t, uint32_t,
+                                           uint32_t, const float4& lo, const float4& ld, float4& next_org,
                                            float4& next_dir) const {
         int4 st = P->l_state[e];
         const float4 tr = P->s_trans[e];
@@ -1487,7 +1643,29 @@ struct ShadowWalk {
         const bool finite = st.z != 0;
         uint32_t occ = 1u;
         bool again = false;
-        if (!best.hit) {
+        bool hit = best.hit;
+        if (kAlpha && hit && !trav::IsSphereId(best.id) &&
+            !(HitAlpha(*S, *L, best.id, inst, best.u, best.v, XYZ(lo), XYZ(ld), best.t, nullptr) >= 0.5f)) {
+            // TraceVisibleSurface's alpha skip within this segment
+            // (surface_query.cpp:187-199): trace again from just past the
+            // hit; past the segment's end it is a miss, and the 64th skipped
+            // hit leaves the segment exhausted, which blocks the shadow ray.
+            const float t_min = best.t + kAlphaSkipOffset;
+            st.w += 1;
+            if (t_min >= remaining) {
+                hit = false;
+            } else if (st.w >= kMaxAlphaSkips) {
+                P->s_occ[e] = 1u;
+                P->l_state[e] = st;
+                return false;
+            } else {
+                P->l_state[e] = st;
+                next_org = make_float4(lo.x, lo.y, lo.z, t_min);
+                next_dir = make_float4(ld.x, ld.y, ld.z, remaining);
+                return true;
+            }
+        }
+        if (!hit) {
             bool ok = true;
             if (med.w >= 0.0f) {
                 if (!finite) {
@@ -1505,8 +1683,22 @@ struct ShadowWalk {
             }
             const int mat_index = trav::IsSphereId(best.id) ? S->sphere_info[-2 - best.id].x
                                                              : S->prim_info[S->tri_prim[best.id]].x;
-            const DevMaterial mat = L->materials[mat_index];
+            DevMaterial mat = L->materials[mat_index];
             if (ok && mat.kind != kDielectric && mat.kind != kThinDielectric) ok = false;
+            if (ok && mat.kind == kThinDielectric && L->mat_tex != nullptr && !trav::IsSphereId(best.id)) {
+                // The hit's material has its reflectance texture applied
+                // (EvaluateMaterialAtSurface); shadow rays carry no footprint.
+                const int rt = L->mat_tex[mat_index].x;
+                const int prim = S->tri_prim[best.id];
+                if (rt >= 0 && (S->prim_flags[prim] & kHasUvs)) {
+                    const int4 pinfo = S->prim_info[prim];
+                    const int64_t base = static_cast<int64_t>(pinfo.z) + static_cast<int64_t>(best.id - pinfo.w) * 3;
+                    const float2 uv = HitUv(*S, S->indices[base], S->indices[base + 1], S->indices[base + 2], best.u,
+                                            best.v);
+                    mat.reflectance =
+                        Hadamard(mat.reflectance, XYZ(tex::Sample(L->tex, rt, uv.x, uv.y, tex::NoFootprint())));
+                }
+            }
             if (ok) {
                 if (mat.kind == kThinDielectric) {
                     T = Hadamard(T, V(Clampf(mat.reflectance.x, 0.0f, 1.0f), Clampf(mat.reflectance.y, 0.0f, 1.0f),
@@ -1534,6 +1726,7 @@ struct ShadowWalk {
                         occ = 1u;
                     } else {
                         again = true;
+                        st.w = 0;  // a new segment: its own alpha skips
                         next_org = F4(Add(hp, Mul(XYZ(ld), bias)), 1.0e-5f);
                         next_dir = make_float4(ld.x, ld.y, ld.z, remaining);
                     }
@@ -1548,11 +1741,13 @@ struct ShadowWalk {
     }
 };
 
-template <bool kTwoLevel, bool kSpheres>
+// kAlpha: the scene has alpha cut-outs, which the walk skips as
+// TraceVisibleSurface does within each segment.
+template <bool kTwoLevel, bool kSpheres, bool kAlpha>
 __global__ __launch_bounds__(trav::kMaxBlock) void ShadowWalkKernel(trav::TraceArgs a, PassView P, SurfaceView S,
                                                                     LightView L) {
     extern __shared__ int2 lds_stack[];
-    const ShadowWalk pol{&P, &S, &L};
+    const ShadowWalk<kAlpha> pol{&P, &S, &L};
     if (kTwoLevel) {
         trav::TraceTwoLevelLoop<false, kSpheres>(a, pol, lds_stack);
     } else {
@@ -1561,12 +1756,102 @@ __global__ __launch_bounds__(trav::kMaxBlock) void ShadowWalkKernel(trav::TraceA
 }
 
 template <bool kTwoLevel>
-void LaunchShadowWalk(bool spheres, int grid, int block, size_t lds, hipStream_t st, const trav::TraceArgs& a,
-                      const PassView& P, const SurfaceView& S, const LightView& L) {
-    if (spheres) {
-        hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, true>), dim3(grid), dim3(block), lds, st, a, P, S, L);
+void LaunchShadowWalk(bool spheres, bool alpha, int grid, int block, size_t lds, hipStream_t st,
+                      const trav::TraceArgs& a, const PassView& P, const SurfaceView& S, const LightView& L) {
+    if (alpha) {
+        if (spheres) {
+            hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, true, true>), dim3(grid), dim3(block), lds, st, a, P, S, L);
+        } else {
+            hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, false, true>), dim3(grid), dim3(block), lds, st, a, P, S, L);
+        }
+    } else if (spheres) {
+        hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, true, false>), dim3(grid), dim3(block), lds, st, a, P, S, L);
     } else {
-        hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, false>), dim3(grid), dim3(block), lds, st, a, P, S, L);
+        hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, false, false>), dim3(grid), dim3(block), lds, st, a, P, S, L);
+    }
+}
+
+// ---- path segments through alpha cut-outs (TraceVisibleSurface, surface_query.cpp:101-200)
+// The closest-hit traversal of a bounce's path queue hands each closest hit
+// to AlphaPath::Finish: a triangle hit whose alpha is below 0.5 is skipped by
+// tracing again from just past it (same lane), at most 64 times; running out
+// of skips, or past t_max, the segment is a miss, which is what the path loop
+// makes of both. hit[ray].w holds the skip count until the ray is done.
+struct AlphaPath {
+    const trav::TraceArgs* a;
+    const SurfaceView* S;
+    const LightView* L;
+    const float4* diff;     // camera-ray differentials per path (first bounce only), or nullptr
+    const int32_t* q_path;  // queue entry -> path
+
+    __device__ __forceinline__ void Begin(int32_t ray, float4& o, float4& d) const {
+        o = a->org[ray];
+        d = a->dir[ray];
+        a->hit[ray] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
+    }
+
+    __device__ __forceinline__ bool Finish(int32_t ray, const trav::Best& best, int32_t inst, uint32_t, uint32_t,
+                                           uint32_t, const float4& lo, const float4& ld, float4& next_org,
+                                           float4& next_dir) const {
+        bool hit = best.hit;
+        if (hit && !trav::IsSphereId(best.id)) {
+            const float4* df = diff != nullptr ? diff + 2 * q_path[ray] : nullptr;
+            if (!(HitAlpha(*S, *L, best.id, inst, best.u, best.v, XYZ(lo), XYZ(ld), best.t, df) >= 0.5f)) {
+                const int skips = __float_as_int(a->hit[ray].w) + 1;
+                const float t_min = best.t + kAlphaSkipOffset;
+                const float t_max = a->dir[ray].w;
+                if (t_min < t_max && skips < kMaxAlphaSkips) {
+                    a->hit[ray] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(skips));
+                    next_org = make_float4(lo.x, lo.y, lo.z, t_min);
+                    next_dir = make_float4(ld.x, ld.y, ld.z, t_max);
+                    return true;
+                }
+                hit = false;
+            }
+        }
+        a->hit[ray] = make_float4(best.t, best.u, best.v, __int_as_float(hit ? best.id : -1));
+        if (a->inst_out != nullptr) a->inst_out[ray] = hit ? inst : -1;
+        return false;
+    }
+};
+
+template <bool kTwoLevel, bool kSpheres>
+__global__ __launch_bounds__(trav::kMaxBlock) void AlphaTraceKernel(trav::TraceArgs a, SurfaceView S, LightView L,
+                                                                    const float4* diff, const int32_t* q_path) {
+    extern __shared__ int2 lds_stack[];
+    const AlphaPath pol{&a, &S, &L, diff, q_path};
+    if (kTwoLevel) {
+        trav::TraceTwoLevelLoop<false, kSpheres>(a, pol, lds_stack);
+    } else {
+        trav::TraceFlatLoop<false, kSpheres>(a, pol, lds_stack);
+    }
+}
+
+void LaunchAlphaTrace(bool two_level, bool spheres, int grid, int block, size_t lds, hipStream_t st,
+                      const trav::TraceArgs& a, const SurfaceView& S, const LightView& L, const float4* diff,
+                      const int32_t* q_path) {
+    if (two_level) {
+        if (spheres) {
+            hipLaunchKernelGGL((AlphaTraceKernel<true, true>), dim3(grid), dim3(block), lds, st, a, S, L, diff, q_path);
+        } else {
+            hipLaunchKernelGGL((AlphaTraceKernel<true, false>), dim3(grid), dim3(block), lds, st, a, S, L, diff, q_path);
+        }
+    } else if (spheres) {
+        hipLaunchKernelGGL((AlphaTraceKernel<false, true>), dim3(grid), dim3(block), lds, st, a, S, L, diff, q_path);
+    } else {
+        hipLaunchKernelGGL((AlphaTraceKernel<false, false>), dim3(grid), dim3(block), lds, st, a, S, L, diff, q_path);
+    }
+}
+
+template <bool kGuided, bool kAnalytic>
+void LaunchShade(bool textured, int grid, hipStream_t st, const PassView& P, const SurfaceView& S, const LightView& L,
+                 const yrg::GuideView& G, int depth, int cur) {
+    if (textured) {
+        hipLaunchKernelGGL((ShadeKernel<kGuided, kAnalytic, true>), dim3(grid), dim3(kBlock), 0, st, P, S, L, G, depth,
+                           cur);
+    } else {
+        hipLaunchKernelGGL((ShadeKernel<kGuided, kAnalytic, false>), dim3(grid), dim3(kBlock), 0, st, P, S, L, G, depth,
+                           cur);
     }
 }
 
@@ -1747,6 +2032,7 @@ PassView MakePassView(RenderState& R, const yrh_render_desc& d) {
     P.p_nrm = R.p_nrm.as<float4>();
     P.p_nee = R.p_nee.as<float4>();
     P.p_slot = R.p_slot.as<int4>();
+    P.p_diff = R.footprints ? R.p_diff.as<float4>() : nullptr;
     for (int k = 0; k < 2; ++k) {
         P.q_org[k] = R.q_org[k].as<float4>();
         P.q_dir[k] = R.q_dir[k].as<float4>();
@@ -1790,12 +2076,17 @@ SurfaceView MakeSurfaceView(RenderState& R) {
     S.prim_info = R.prim_info.as<int4>();
     S.prim_flags = R.prim_flags.as<int32_t>();
     S.inst_normal = R.inst_normal.as<float4>();
+    S.uvs = R.uvs.as<float2>();
+    S.tangents = R.tangents.as<float4>();
     return S;
 }
 
 LightView MakeLightView(RenderState& R) {
     LightView L{};
     L.materials = R.materials.as<DevMaterial>();
+    L.mat_tex = R.textured || R.alpha ? R.mat_tex.as<int4>() : nullptr;
+    L.tex = tex::TexView{R.tex_records.as<int4>(), R.tex_levels.as<int4>(), R.tex_texels.as<float4>(),
+                         R.tex_gauss.as<float>(), R.texture_count};
     L.emit = R.emit.as<float4>();
     L.emit_info = R.emit_info.as<int4>();
     L.emit_area = R.emit_area.as<float>();
@@ -1822,6 +2113,9 @@ LightView MakeLightView(RenderState& R) {
     L.env_texels = R.env_w > 0 && R.env_h > 0 ? R.env_texels.as<float4>() : nullptr;
     L.env_w = R.env_w;
     L.env_h = R.env_h;
+    L.env_nearest = R.env_nearest;
+    L.env_ws = R.env_ws;
+    L.env_wt = R.env_wt;
     L.env_sampleable = R.env_sampleable;
     L.env_weights = R.env_weights.as<float>();
     L.env_row_weights = R.env_row_weights.as<float>();
@@ -1854,6 +2148,15 @@ DevMaterial ToDevMaterial(const yrh_material& m) {
 
 // One BSDF query per thread (yrh_bsdf_query): the device EvalBsdf, PdfBsdf,
 // SampleBsdf and IsDelta the shading kernels run, on caller tuples.
+// SampleTexture4 of texture 0 at n points (uv 2 floats, footprint 4 floats or none).
+__global__ void TextureQueryKernel(tex::TexView T, int64_t n, const float* uv, const float* fp, float4* out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const tex::Footprint f = fp != nullptr ? tex::Footprint{fp[4 * i], fp[4 * i + 1], fp[4 * i + 2], fp[4 * i + 3]}
+                                           : tex::NoFootprint();
+    out[i] = tex::Sample(T, 0, uv[2 * i], uv[2 * i + 1], f);
+}
+
 __global__ void BsdfQueryKernel(const DevMaterial* mats, int64_t n, const int32_t* mat, const float* q, float* out) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1884,8 +2187,19 @@ constexpr size_t kPathSlotBytes = 4 + 4 + 4 + 16 + 16 + 16 + 4 + 16 + 16 + 16 + 
 constexpr size_t kShadowSlotBytes = 16 + 16 + 16 + 16 + 4 + 16 + 16 + 16;
 constexpr size_t kGuideRowBytes = yrg::kGuideMixtureRow * 4 + 4;
 
+// The camera-ray differentials (two float4 per path) of scenes whose textures
+// filter with footprints.
+int EnsureDifferentials(yrh_context* ctx, RenderState& R) {
+    const size_t bytes = 32 * static_cast<size_t>(R.capacity);
+    if (!R.footprints || R.p_diff.bytes >= bytes) return YRH_OK;
+    int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return rc != YRH_OK ? rc : Alloc(ctx, R.p_diff, bytes);
+}
+
 int EnsureCapacity(yrh_context* ctx, RenderState& R, int64_t paths) {
-    if (R.capacity >= paths && R.cap_shadow_slots >= R.shadow_slots && R.counters.ptr != nullptr) return YRH_OK;
+    if (R.capacity >= paths && R.cap_shadow_slots >= R.shadow_slots && R.counters.ptr != nullptr) {
+        return EnsureDifferentials(ctx, R);
+    }
     int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     if (rc != YRH_OK) return rc;
     paths = std::max(paths, R.capacity);
@@ -1915,7 +2229,7 @@ int EnsureCapacity(yrh_context* ctx, RenderState& R, int64_t paths) {
     rc = CheckHip(ctx, hipMemsetAsync(R.counters.ptr, 0, 4 * kCounterWords, ctx->stream), "hipMemsetAsync");
     R.capacity = paths;
     R.cap_shadow_slots = R.shadow_slots;
-    return rc;
+    return rc != YRH_OK ? rc : EnsureDifferentials(ctx, R);
 }
 
 // Raygen and the bounces of one pass over `paths` path slots (the queue
@@ -1962,16 +2276,32 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
         tr.hit = P.hits;
         tr.instance = ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL ? P.hit_inst : nullptr;
         tr.totals = P.stats + kStatTraceNodes;
-        if ((rc = LaunchTrace(ctx, tr)) != YRH_OK) return rc;
         PassView Pk = P;
         if (ctx->geo.accel_kind != YRH_ACCEL_TWO_LEVEL) Pk.hit_inst = nullptr;
+        if (depth > 0) Pk.p_diff = nullptr;  // only camera rays carry differentials
+        if (R.alpha) {
+            // Alpha cut-outs: the traversal skips invisible hits itself.
+            trav::TraceArgs ta;
+            int tgrid = 0, tblock = 0;
+            if ((rc = PrepareTraceArgs(ctx, tr, ta, tgrid, tblock)) != YRH_OK) return rc;
+            LaunchAlphaTrace(ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL, ctx->geo.has_spheres, tgrid, tblock,
+                             LdsBytes(ta, tblock), st, ta, S, L, Pk.p_diff, P.q_path[cur]);
+        } else if ((rc = LaunchTrace(ctx, tr)) != YRH_OK) {
+            return rc;
+        }
         // Guided bounces: all that may continue, or the first depth_limit. A
         // plain vertex after guided ones is the plain estimator's vertex (its
         // NEE and emission weights use the pdf its direction was drawn from).
         const bool guide_here = guided && depth + 1 < max_depth && (GV.depth_limit <= 0 || depth < GV.depth_limit);
+        const bool textured = L.mat_tex != nullptr;
         if (guide_here) {
-            hipLaunchKernelGGL(GuideQueryKernel, dim3(Grid(paths, kQueryBlock)), dim3(kQueryBlock), 0, st, Pk, S, L, GV,
-                               cur);
+            if (textured) {
+                hipLaunchKernelGGL(GuideQueryKernel<true>, dim3(Grid(paths, kQueryBlock)), dim3(kQueryBlock), 0, st, Pk,
+                                   S, L, GV, cur);
+            } else {
+                hipLaunchKernelGGL(GuideQueryKernel<false>, dim3(Grid(paths, kQueryBlock)), dim3(kQueryBlock), 0, st, Pk,
+                                   S, L, GV, cur);
+            }
         }
         // The previous bounce's Resolve reads this bounce's path queue count,
         // the path slots and the shadow queue, which Shade rewrites.
@@ -1980,16 +2310,17 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
         }
         hipLaunchKernelGGL(BeginBounceKernel, dim3(1), dim3(64), 0, st, P.counters, 1 - cur, kBeginShade);
         const bool analytic = L.analytic_count > 0;
+        const int sgrid = Grid(paths);
         if (guide_here) {
             if (analytic) {
-                hipLaunchKernelGGL((ShadeKernel<true, true>), dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
+                LaunchShade<true, true>(textured, sgrid, st, Pk, S, L, GV, depth, cur);
             } else {
-                hipLaunchKernelGGL((ShadeKernel<true, false>), dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
+                LaunchShade<true, false>(textured, sgrid, st, Pk, S, L, GV, depth, cur);
             }
         } else if (analytic) {
-            hipLaunchKernelGGL((ShadeKernel<false, true>), dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
+            LaunchShade<false, true>(textured, sgrid, st, Pk, S, L, GV, depth, cur);
         } else {
-            hipLaunchKernelGGL((ShadeKernel<false, false>), dim3(Grid(paths)), dim3(kBlock), 0, st, Pk, S, L, GV, depth, cur);
+            LaunchShade<false, false>(textured, sgrid, st, Pk, S, L, GV, depth, cur);
         }
         if (overlap) {
             if ((rc = CheckHip(ctx, hipEventRecord(shaded, st), "hipEventRecord")) != YRH_OK) return rc;
@@ -2026,9 +2357,9 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
             int wgrid = 0, wblock = 0;
             if ((rc = PrepareTraceArgs(ctx, wr, wa, wgrid, wblock)) != YRH_OK) return rc;
             if (ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL) {
-                LaunchShadowWalk<true>(ctx->geo.has_spheres, wgrid, wblock, LdsBytes(wa, wblock), ss, wa, P, S, L);
+                LaunchShadowWalk<true>(ctx->geo.has_spheres, R.alpha, wgrid, wblock, LdsBytes(wa, wblock), ss, wa, P, S, L);
             } else {
-                LaunchShadowWalk<false>(ctx->geo.has_spheres, wgrid, wblock, LdsBytes(wa, wblock), ss, wa, P, S, L);
+                LaunchShadowWalk<false>(ctx->geo.has_spheres, R.alpha, wgrid, wblock, LdsBytes(wa, wblock), ss, wa, P, S, L);
             }
         }
         hipLaunchKernelGGL(ResolveKernel, dim3(Grid(paths)), dim3(kBlock), 0, ss, P, GV, depth, cur);
@@ -2041,6 +2372,94 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
                            P, GV);
     }
     return YRH_OK;
+}
+
+// The device texture layout (yrh_texture.h) of `n` textures: level 0 as
+// given, then the box-filtered chain down to 1 x 1 (BuildTextureMipChain,
+// src/shading/texture.cpp:241-272), all in one pool; the EWA weight table as
+// the reference tabulates it (texture.cpp:140-152).
+struct TexturePool {
+    std::vector<int4> records, levels;
+    std::vector<float4> texels;
+    std::vector<char> footprint;  // per texture: filters with footprints (trilinear / EWA with mips)
+    float gauss[tex::kGaussEntries];
+    std::string error;
+};
+
+bool BuildTexturePool(const yrh_texture* textures, int32_t ntex, TexturePool& pool) {
+    std::vector<int4>& trec = pool.records;
+    std::vector<int4>& tlev = pool.levels;
+    std::vector<float4>& tpool = pool.texels;
+    std::vector<char>& tex_footprint = pool.footprint;
+    trec.assign(2 * static_cast<size_t>(ntex), make_int4(0, 0, 0, 0));
+    tex_footprint.assign(static_cast<size_t>(ntex), 0);
+    for (int32_t k = 0; k < ntex; ++k) {
+        const yrh_texture& t = textures[k];
+        if (t.filter < YRH_FILTER_NEAREST || t.filter > YRH_FILTER_EWA || t.wrap_s < YRH_WRAP_REPEAT ||
+            t.wrap_s > YRH_WRAP_MIRRORED || t.wrap_t < YRH_WRAP_REPEAT || t.wrap_t > YRH_WRAP_MIRRORED) {
+            pool.error = "texture filter or wrap mode out of range";
+            return false;
+        }
+        const int first = static_cast<int>(tlev.size());
+        int levels = 0;
+        if (t.width > 0 && t.height > 0 && t.texels != nullptr) {
+            int w = t.width, h = t.height;
+            size_t off = tpool.size();
+            tlev.push_back(make_int4(w, h, static_cast<int>(off), 0));
+            const float4* src = reinterpret_cast<const float4*>(t.texels);
+            tpool.insert(tpool.end(), src, src + static_cast<size_t>(w) * static_cast<size_t>(h));
+            levels = 1;
+            auto wrap = [](int i, int n, int mode) {
+                if (mode == YRH_WRAP_CLAMP) return std::clamp(i, 0, n - 1);
+                if (mode == YRH_WRAP_MIRRORED) {
+                    int q = i % (2 * n);
+                    if (q < 0) q += 2 * n;
+                    return q < n ? q : 2 * n - 1 - q;
+                }
+                int q = i % n;
+                return q < 0 ? q + n : q;
+            };
+            while (w > 1 || h > 1) {
+                const int nw = std::max(1, (w + 1) / 2), nh = std::max(1, (h + 1) / 2);
+                std::vector<float4> next(static_cast<size_t>(nw) * static_cast<size_t>(nh));
+                auto at = [&](int x, int y) {
+                    return tpool[off + static_cast<size_t>(wrap(y, h, t.wrap_t)) * static_cast<size_t>(w) +
+                                 static_cast<size_t>(wrap(x, w, t.wrap_s))];
+                };
+                for (int y = 0; y < nh; ++y) {
+                    for (int x = 0; x < nw; ++x) {
+                        const float4 a = at(2 * x, 2 * y), b = at(2 * x + 1, 2 * y), c = at(2 * x, 2 * y + 1),
+                                     d = at(2 * x + 1, 2 * y + 1);
+                        next[static_cast<size_t>(y) * static_cast<size_t>(nw) + static_cast<size_t>(x)] =
+                            make_float4((a.x + b.x + c.x + d.x) * 0.25f, (a.y + b.y + c.y + d.y) * 0.25f,
+                                        (a.z + b.z + c.z + d.z) * 0.25f, (a.w + b.w + c.w + d.w) * 0.25f);
+                    }
+                }
+                off = tpool.size();
+                tpool.insert(tpool.end(), next.begin(), next.end());
+                tlev.push_back(make_int4(nw, nh, static_cast<int>(off), 0));
+                w = nw;
+                h = nh;
+                ++levels;
+            }
+            if (tpool.size() > static_cast<size_t>(std::numeric_limits<int32_t>::max())) {
+                pool.error = "textures exceed 2^31 texels with their mip chains";
+                return false;
+            }
+        }
+        trec[2 * static_cast<size_t>(k)] = make_int4(t.width, t.height, t.filter | (t.wrap_s << 4) | (t.wrap_t << 8), levels);
+        trec[2 * static_cast<size_t>(k) + 1] = make_int4(first, 0, 0, 0);
+        tex_footprint[static_cast<size_t>(k)] = (t.filter == YRH_FILTER_TRILINEAR || t.filter == YRH_FILTER_EWA) && levels > 1;
+    }
+    {
+        const float falloff = 2.0f;
+        const float edge = std::exp(-falloff);
+        for (int k = 0; k < tex::kGaussEntries; ++k) {
+            const float r2 = static_cast<float>(k) / static_cast<float>(tex::kGaussEntries - 1);
+            pool.gauss[k] = std::exp(-falloff * r2) - edge;
+        }
+    }
+    return true;
 }
 
 }  // namespace
@@ -2079,11 +2498,15 @@ int UploadSurfaceGeometry(yrh_context* ctx, const yrh_geometry_desc* g) {
     R.host_prim_count.assign(static_cast<size_t>(g->primitive_count), 0);
     R.host_prim_material.assign(static_cast<size_t>(g->primitive_count), 0);
     int32_t flat = 0;
+    bool any_uvs = false, any_tangents = false;
     for (int32_t p = 0; p < g->primitive_count; ++p) {
         const yrh_primitive& mp = g->primitives[p];
         const int32_t tris = static_cast<int32_t>(mp.index_count / 3);
         info[static_cast<size_t>(p)] = make_int4(mp.material_index, -1, static_cast<int32_t>(mp.first_index), flat);
-        flags[static_cast<size_t>(p)] = mp.has_normals ? 1 : 0;
+        flags[static_cast<size_t>(p)] =
+            (mp.has_normals ? kHasNormals : 0) | (mp.has_uvs ? kHasUvs : 0) | (mp.has_tangents ? kHasTangents : 0);
+        any_uvs = any_uvs || mp.has_uvs;
+        any_tangents = any_tangents || mp.has_tangents;
         R.host_prim_first[static_cast<size_t>(p)] = static_cast<int32_t>(mp.first_index);
         R.host_prim_count[static_cast<size_t>(p)] = tris;
         R.host_prim_material[static_cast<size_t>(p)] = mp.material_index;
@@ -2105,12 +2528,14 @@ int UploadSurfaceGeometry(yrh_context* ctx, const yrh_geometry_desc* g) {
             const bool singular = std::fabs(det) <= 1.0e-12f;
             const float inv = singular ? 0.0f : 1.0f / det;
             float4* o = &inst[6 * static_cast<size_t>(k)];
+            // .w of row 2: the orientation ResolveNormalMap gives the
+            // tangent frame (material_evaluator.cpp:75-82), same determinant
             o[0] = make_float4(e * i - f * h, f * gg - d * i, d * h - e * gg, inv);
             o[1] = make_float4(c * h - b * i, a * i - c * gg, b * gg - a * h, singular ? 1.0f : 0.0f);
-            o[2] = make_float4(b * f - c * e, c * d - a * f, a * e - b * d, 0.0f);
-            o[3] = make_float4(m[0], m[4], m[8], 0.0f);
-            o[4] = make_float4(m[1], m[5], m[9], 0.0f);
-            o[5] = make_float4(m[2], m[6], m[10], 0.0f);
+            o[2] = make_float4(b * f - c * e, c * d - a * f, a * e - b * d, det < 0.0f ? -1.0f : 1.0f);
+            o[3] = make_float4(m[0], m[4], m[8], m[12]);
+            o[4] = make_float4(m[1], m[5], m[9], m[13]);
+            o[5] = make_float4(m[2], m[6], m[10], m[14]);
         }
     }
     int rc;
@@ -2121,6 +2546,19 @@ int UploadSurfaceGeometry(yrh_context* ctx, const yrh_geometry_desc* g) {
     if ((rc = UploadVec(ctx, R.prim_info, info.data(), info.size())) != YRH_OK) return rc;
     if ((rc = UploadVec(ctx, R.prim_flags, flags.data(), flags.size())) != YRH_OK) return rc;
     if ((rc = UploadVec(ctx, R.inst_normal, inst.data(), inst.size())) != YRH_OK) return rc;
+    // Texture coordinates and tangent frames, when some primitive has them.
+    std::vector<float2> uvs(any_uvs ? static_cast<size_t>(nv) : 0);
+    std::vector<float4> tangents(any_tangents ? static_cast<size_t>(nv) : 0);
+    for (int64_t v = 0; v < nv && (any_uvs || any_tangents); ++v) {
+        const yrh_vertex& vx = g->vertices[v];
+        if (any_uvs) uvs[static_cast<size_t>(v)] = make_float2(vx.uv[0], vx.uv[1]);
+        if (any_tangents) {
+            tangents[static_cast<size_t>(v)] =
+                make_float4(vx.tangent[0], vx.tangent[1], vx.tangent[2], vx.tangent_handedness);
+        }
+    }
+    if ((rc = UploadVec(ctx, R.uvs, uvs.data(), uvs.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.tangents, tangents.data(), tangents.size())) != YRH_OK) return rc;
     std::vector<float4> sph(static_cast<size_t>(g->sphere_count));
     std::vector<int2> sph_info(static_cast<size_t>(g->sphere_count));
     R.host_sphere_material.assign(static_cast<size_t>(g->sphere_count), 0);
@@ -2160,18 +2598,49 @@ int yrh_upload_shading(yrh_context* ctx, const yrh_shading_desc* sd) {
     (void)hipSetDevice(ctx->device);
     int rc = CheckHip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     if (rc != YRH_OK) return rc;
+    const int32_t ntex = sd->textures != nullptr ? std::max(0, sd->texture_count) : 0;
+    TexturePool pool;
+    if (!BuildTexturePool(sd->textures, ntex, pool)) return Fail(ctx, YRH_E_INVALID_ARGUMENT, pool.error);
+    std::vector<int4>& trec = pool.records;
+    std::vector<int4>& tlev = pool.levels;
+    std::vector<float4>& tpool = pool.texels;
+    std::vector<char>& tex_footprint = pool.footprint;
+    float* gauss = pool.gauss;
     std::vector<DevMaterial> mats(static_cast<size_t>(sd->material_count));
+    std::vector<int4> mtex(static_cast<size_t>(sd->material_count));
     R.transparent = false;
+    R.textured = false;
+    R.alpha = false;
+    R.footprints = false;
     for (int32_t k = 0; k < sd->material_count; ++k) {
         const yrh_material& m = sd->materials[k];
         if (m.kind < YRH_MATERIAL_DIFFUSE || m.kind > YRH_MATERIAL_THIN_DIELECTRIC) {
             return Fail(ctx, YRH_E_UNSUPPORTED, "material kind not supported by the HIP backend");
         }
-        if (!(m.alpha >= 0.5f)) return Fail(ctx, YRH_E_UNSUPPORTED, "alpha cut-out materials are not supported");
+        // TextureIndexValid (material_evaluator.cpp:19-21): other indices mean none.
+        auto valid = [&](int32_t t) { return t >= 0 && t < ntex ? t : -1; };
+        const int32_t rt = valid(m.reflectance_texture), at = valid(m.alpha_texture), nm = valid(m.normal_map);
+        float scale = m.normal_scale;
+        int32_t scale_bits;
+        std::memcpy(&scale_bits, &scale, 4);
+        mtex[static_cast<size_t>(k)] = make_int4(rt, at, nm, scale_bits);
         mats[static_cast<size_t>(k)] = ToDevMaterial(m);
         if (m.kind == YRH_MATERIAL_DIELECTRIC || m.kind == YRH_MATERIAL_THIN_DIELECTRIC) R.transparent = true;
+        if (rt >= 0 || nm >= 0) R.textured = true;
+        if (at >= 0 || !(m.alpha >= 0.5f)) R.alpha = true;
+        for (int32_t t : {rt, at, nm}) {
+            if (t >= 0 && tex_footprint[static_cast<size_t>(t)]) R.footprints = true;
+        }
     }
+    // Alpha cut-outs make shadow rays take the transparent walk (closest hits).
+    if (R.alpha) R.transparent = true;
     if ((rc = UploadVec(ctx, R.materials, mats.data(), mats.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.mat_tex, mtex.data(), mtex.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.tex_records, trec.data(), trec.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.tex_levels, tlev.data(), tlev.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.tex_texels, tpool.data(), tpool.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, R.tex_gauss, gauss, tex::kGaussEntries)) != YRH_OK) return rc;
+    R.texture_count = ntex;
     R.material_count = sd->material_count;
 
     // Emitters: per-emitter running triangle areas in index order (the
@@ -2317,9 +2786,16 @@ int yrh_upload_shading(yrh_context* ctx, const yrh_shading_desc* sd) {
     R.env_rotation = env.rotation_radians;
     R.env_w = env.texels != nullptr ? std::max(0, env.width) : 0;
     R.env_h = env.texels != nullptr ? std::max(0, env.height) : 0;
-    const size_t ntex = static_cast<size_t>(R.env_w) * static_cast<size_t>(R.env_h);
-    if ((rc = UploadVec(ctx, R.env_texels, reinterpret_cast<const float4*>(env.texels), ntex)) != YRH_OK) return rc;
-    const bool has_tex = ntex > 0;
+    if (env.wrap_s < YRH_WRAP_REPEAT || env.wrap_s > YRH_WRAP_MIRRORED || env.wrap_t < YRH_WRAP_REPEAT ||
+        env.wrap_t > YRH_WRAP_MIRRORED) {
+        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "environment wrap mode out of range");
+    }
+    R.env_nearest = env.nearest ? 1 : 0;
+    R.env_ws = env.wrap_s;
+    R.env_wt = env.wrap_t;
+    const size_t env_texels = static_cast<size_t>(R.env_w) * static_cast<size_t>(R.env_h);
+    if ((rc = UploadVec(ctx, R.env_texels, reinterpret_cast<const float4*>(env.texels), env_texels)) != YRH_OK) return rc;
+    const bool has_tex = env_texels > 0;
     R.env_sampleable = (env.active && env.strength > 0.0f && has_tex && env.has_distribution) ? 1 : 0;
     R.env_dw = env.has_distribution ? env.dist_width : 0;
     R.env_dh = env.has_distribution ? env.dist_height : 0;
@@ -2758,6 +3234,32 @@ int yrh_bsdf_query(yrh_context* ctx, const yrh_material* materials, int32_t mate
                        bi.as<int32_t>(), bq.as<float>(), bo.as<float>());
     if ((rc = CheckHip(ctx, hipGetLastError(), "bsdf query launch")) != YRH_OK) return rc;
     return yrh_copy_to_host(ctx, out, bo.ptr, static_cast<size_t>(12 * n) * sizeof(float));
+}
+
+int yrh_texture_query(yrh_context* ctx, const yrh_texture* texture, int64_t n, const float* uv, const float* footprint,
+                      float* out) {
+    if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
+    if (texture == nullptr || n < 0 || (n > 0 && (uv == nullptr || out == nullptr))) {
+        return Fail(ctx, YRH_E_INVALID_ARGUMENT, "bad texture_query arguments");
+    }
+    TexturePool pool;
+    if (!BuildTexturePool(texture, 1, pool)) return Fail(ctx, YRH_E_INVALID_ARGUMENT, pool.error);
+    if (n == 0) return YRH_OK;
+    (void)hipSetDevice(ctx->device);
+    DevBuf br, bl, bt, bg, bu, bf, bo;
+    int rc;
+    if ((rc = UploadVec(ctx, br, pool.records.data(), pool.records.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, bl, pool.levels.data(), pool.levels.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, bt, pool.texels.data(), pool.texels.size())) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, bg, pool.gauss, tex::kGaussEntries)) != YRH_OK) return rc;
+    if ((rc = UploadVec(ctx, bu, uv, static_cast<size_t>(2 * n))) != YRH_OK) return rc;
+    if (footprint != nullptr && (rc = UploadVec(ctx, bf, footprint, static_cast<size_t>(4 * n))) != YRH_OK) return rc;
+    if ((rc = Alloc(ctx, bo, static_cast<size_t>(n) * sizeof(float4))) != YRH_OK) return rc;
+    const tex::TexView T{br.as<int4>(), bl.as<int4>(), bt.as<float4>(), bg.as<float>(), 1};
+    hipLaunchKernelGGL(TextureQueryKernel, dim3(Grid(n)), dim3(kBlock), 0, ctx->stream, T, n, bu.as<float>(),
+                       footprint != nullptr ? bf.as<float>() : nullptr, bo.as<float4>());
+    if ((rc = CheckHip(ctx, hipGetLastError(), "texture query launch")) != YRH_OK) return rc;
+    return yrh_copy_to_host(ctx, out, bo.ptr, static_cast<size_t>(n) * sizeof(float4));
 }
 
 int yrh_generate_camera_rays(yrh_context* ctx, const yrh_render_desc* d, float* org_tmin, float* dir_tmax) {

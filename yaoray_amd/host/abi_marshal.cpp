@@ -33,6 +33,27 @@ static_assert(sizeof(yr::LightTreeNode) == sizeof(yrh_light_tree_node));
 static_assert(offsetof(yr::LightTreeNode, power) == offsetof(yrh_light_tree_node, power));
 static_assert(offsetof(yr::LightTreeNode, light_index) == offsetof(yrh_light_tree_node, light_index));
 
+static_assert(static_cast<int>(yr::TextureFilter::Nearest) == YRH_FILTER_NEAREST);
+static_assert(static_cast<int>(yr::TextureFilter::Bilinear) == YRH_FILTER_BILINEAR);
+static_assert(static_cast<int>(yr::TextureFilter::Trilinear) == YRH_FILTER_TRILINEAR);
+static_assert(static_cast<int>(yr::TextureFilter::Ewa) == YRH_FILTER_EWA);
+static_assert(static_cast<int>(yr::TextureWrap::Repeat) == YRH_WRAP_REPEAT);
+static_assert(static_cast<int>(yr::TextureWrap::ClampToEdge) == YRH_WRAP_CLAMP);
+static_assert(static_cast<int>(yr::TextureWrap::MirroredRepeat) == YRH_WRAP_MIRRORED);
+
+namespace {
+
+// Level-0 texels: the reference moves them into a shared sampling cache once
+// BuildTextureSamplingCaches ran (src/shading/texture.cpp:274-298).
+const std::vector<yr::Color4f>& Texels(const yr::RenderTexture& t) {
+#if defined(YAORAY_HIP_DROPIN)
+    if (t.sampling_cache) return t.sampling_cache->texels;
+#endif
+    return t.texels;
+}
+
+}  // namespace
+
 yrh_bvh_desc DescribeBvh(const yr::RenderBvh& b) {
     yrh_bvh_desc d{};
     d.nodes = reinterpret_cast<const yrh_bvh_node*>(b.nodes.data());
@@ -81,6 +102,13 @@ void DescribeGeometry(const yr::RenderSceneIR& ir, const yr::RenderAcceleration&
 
 std::string DescribeShading(const yr::RenderSceneIR& ir, Marshal& m, yrh_shading_desc& d) {
     m.materials.clear();
+    const int ntex = static_cast<int>(ir.textures.size());
+    std::vector<char> used(static_cast<size_t>(ntex), 0);
+    auto tex = [&](int k) {
+        if (k < 0 || k >= ntex) return -1;  // TextureIndexValid (material_evaluator.cpp:19-21)
+        used[static_cast<size_t>(k)] = 1;
+        return k;
+    };
     for (const yr::RenderMaterial& mat : ir.materials) {
         yrh_material a{};
         switch (mat.kind) {
@@ -90,10 +118,13 @@ std::string DescribeShading(const yr::RenderSceneIR& ir, Marshal& m, yrh_shading
             case yr::RenderMaterialKind::ThinDielectric: a.kind = YRH_MATERIAL_THIN_DIELECTRIC; break;
             default: return "material kind outside the HIP backend's set";
         }
-        if (mat.reflectance.texture != -1 || mat.uroughness.texture != -1 || mat.vroughness.texture != -1 ||
-            mat.alpha.texture != -1 || mat.normal_map != -1) {
-            return "textured material parameters are not supported by the HIP backend";
-        }
+        // Only reflectance, alpha and the normal map are textured at a hit
+        // (EvaluateMaterialAtSurface, material_evaluator.cpp:137-177); the
+        // other parameters' texture slots are not read there either.
+        a.reflectance_texture = tex(mat.reflectance.texture);
+        a.alpha_texture = tex(mat.alpha.texture);
+        a.normal_map = tex(mat.normal_map);
+        a.normal_scale = mat.normal_scale;
         a.reflectance[0] = mat.reflectance.value.x;
         a.reflectance[1] = mat.reflectance.value.y;
         a.reflectance[2] = mat.reflectance.value.z;
@@ -135,6 +166,27 @@ std::string DescribeShading(const yr::RenderSceneIR& ir, Marshal& m, yrh_shading
     d.analytic_tree_count = static_cast<int32_t>(cache.analytic_tree.size());
     d.analytic_tree_root = cache.analytic_tree_root;
     d.use_analytic_tree = cache.use_analytic_tree ? 1 : 0;
+    // Every texture keeps its index; ones no material reads (the environment
+    // map) go without texels, which nothing samples.
+    m.textures.assign(static_cast<size_t>(ntex), yrh_texture{});
+    for (int k = 0; k < ntex; ++k) {
+        const yr::RenderTexture& t = ir.textures[static_cast<size_t>(k)];
+        yrh_texture& o = m.textures[static_cast<size_t>(k)];
+        o.width = t.width;
+        o.height = t.height;
+        o.filter = static_cast<int32_t>(t.filter);
+        o.wrap_s = static_cast<int32_t>(t.wrap_s);
+        o.wrap_t = static_cast<int32_t>(t.wrap_t);
+        const std::vector<yr::Color4f>& texels = Texels(t);
+        if (used[static_cast<size_t>(k)] && !texels.empty()) {
+            if (static_cast<int64_t>(texels.size()) < static_cast<int64_t>(t.width) * static_cast<int64_t>(t.height)) {
+                return "texture has fewer texels than width * height";
+            }
+            o.texels = reinterpret_cast<const float*>(texels.data());
+        }
+    }
+    d.textures = m.textures.data();
+    d.texture_count = ntex;
     yrh_environment_desc& e = d.environment;
     const yr::RenderEnvironment& env = ir.environment;
     e.active = env.active ? 1 : 0;
@@ -145,9 +197,14 @@ std::string DescribeShading(const yr::RenderSceneIR& ir, Marshal& m, yrh_shading
     e.rotation_radians = env.rotation_radians;
     if (env.texture_index >= 0 && env.texture_index < static_cast<int>(ir.textures.size())) {
         const yr::RenderTexture& t = ir.textures[static_cast<size_t>(env.texture_index)];
+        // EvaluateEnvironment samples without a footprint (environment.cpp:136):
+        // bilinear unless the map is nearest-filtered, with its wrap modes.
+        e.nearest = t.filter == yr::TextureFilter::Nearest ? 1 : 0;
+        e.wrap_s = static_cast<int32_t>(t.wrap_s);
+        e.wrap_t = static_cast<int32_t>(t.wrap_t);
         e.width = t.width;
         e.height = t.height;
-        e.texels = reinterpret_cast<const float*>(t.texels.data());
+        e.texels = reinterpret_cast<const float*>(Texels(t).data());
     }
     if (env.distribution_index >= 0 && env.distribution_index < static_cast<int>(ir.environment_distributions.size())) {
         const yr::RenderEnvironmentDistribution& dist = ir.environment_distributions[static_cast<size_t>(env.distribution_index)];

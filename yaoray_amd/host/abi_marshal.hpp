@@ -19,6 +19,7 @@ struct Marshal {
     std::vector<yrh_bvh_desc> blas;
     std::vector<yrh_material> materials;
     std::vector<yrh_emissive> emissives;
+    std::vector<yrh_texture> textures;
 };
 
 yrh_bvh_desc DescribeBvh(const yr::RenderBvh& b);

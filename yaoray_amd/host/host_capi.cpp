@@ -145,6 +145,10 @@ int yrhost_scene_set_materials(yrhost_scene* s, const yrh_material* m, int32_t c
         r.emission = yr::Color3f{m[k].emission[0], m[k].emission[1], m[k].emission[2]};
         r.absorption_distance = m[k].absorption_distance;
         r.absorption_color = yr::Color3f{m[k].absorption_color[0], m[k].absorption_color[1], m[k].absorption_color[2]};
+        r.reflectance.texture = m[k].reflectance_texture;
+        r.alpha.texture = m[k].alpha_texture;
+        r.normal_map = m[k].normal_map;
+        r.normal_scale = m[k].normal_scale;
         s->ir.materials.push_back(r);
     }
     return YRH_OK;
@@ -186,9 +190,48 @@ int yrhost_scene_set_constant_environment(yrhost_scene* s, const float* radiance
     s->ir.environment.active = false;
     s->ir.environment.radiance = yr::Color3f{radiance3[0], radiance3[1], radiance3[2]};
     s->ir.environment.strength = strength;
-    s->ir.textures.clear();
     s->ir.environment_distributions.clear();
     return YRH_OK;
+}
+
+int yrhost_scene_set_environment_map(yrhost_scene* s, int32_t texture_index, const float* radiance3, float strength,
+                                     float rotation_radians) {
+    if (s == nullptr || radiance3 == nullptr || texture_index < 0 ||
+        texture_index >= static_cast<int32_t>(s->ir.textures.size())) {
+        return YRH_E_INVALID_ARGUMENT;
+    }
+    const yr::RenderTexture& t = s->ir.textures[static_cast<size_t>(texture_index)];
+    if (t.width <= 0 || t.height <= 0 || t.texels.empty()) return SetError(s, "environment map has no texels");
+    s->ir.environment = yr::RenderEnvironment{};
+    s->ir.environment.active = true;
+    s->ir.environment.radiance = yr::Color3f{radiance3[0], radiance3[1], radiance3[2]};
+    s->ir.environment.strength = strength;
+    s->ir.environment.rotation_radians = rotation_radians;
+    s->ir.environment.texture_index = texture_index;
+    s->ir.environment_distributions.clear();
+    s->ir.environment_distributions.push_back(yr::BuildEnvironmentDistribution(t));
+    s->ir.environment.distribution_index = 0;
+    return YRH_OK;
+}
+
+int yrhost_scene_add_texture(yrhost_scene* s, const yrh_texture* t) {
+    if (s == nullptr || t == nullptr) return YRH_E_INVALID_ARGUMENT;
+    if (t->filter < YRH_FILTER_NEAREST || t->filter > YRH_FILTER_EWA || t->wrap_s < YRH_WRAP_REPEAT ||
+        t->wrap_s > YRH_WRAP_MIRRORED || t->wrap_t < YRH_WRAP_REPEAT || t->wrap_t > YRH_WRAP_MIRRORED) {
+        return SetError(s, "texture filter or wrap mode out of range");
+    }
+    yr::RenderTexture r;
+    r.filter = static_cast<yr::TextureFilter>(t->filter);
+    r.wrap_s = static_cast<yr::TextureWrap>(t->wrap_s);
+    r.wrap_t = static_cast<yr::TextureWrap>(t->wrap_t);
+    if (t->width > 0 && t->height > 0 && t->texels != nullptr) {
+        r.width = t->width;
+        r.height = t->height;
+        r.texels.resize(static_cast<size_t>(t->width) * static_cast<size_t>(t->height));
+        std::memcpy(r.texels.data(), t->texels, r.texels.size() * sizeof(yr::Color4f));
+    }
+    s->ir.textures.push_back(std::move(r));
+    return static_cast<int>(s->ir.textures.size()) - 1;
 }
 
 void yrhost_scene_destroy(yrhost_scene* scene) { delete scene; }

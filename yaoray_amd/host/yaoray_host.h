@@ -61,6 +61,16 @@ int yrhost_scene_info_get(const yrhost_scene* scene, yrhost_scene_info* info);
  * current list and rebuild the emissive alias table. */
 int yrhost_scene_set_camera(yrhost_scene* scene, const float* camera13);
 int yrhost_scene_set_materials(yrhost_scene* scene, const yrh_material* materials, int32_t count);
+/* Appends a texture to the scene's texture list (RenderSceneIR::textures,
+ * include/yaoray/scene/render_scene.hpp) and returns its index (>= 0), which
+ * materials name in reflectance_texture / alpha_texture / normal_map; the
+ * texels are copied. Returns a negative YRH_E_* code on bad arguments. */
+int yrhost_scene_add_texture(yrhost_scene* scene, const yrh_texture* texture);
+/* An active equirect environment on texture `texture_index` (RenderEnvironment,
+ * include/yaoray/scene/light.hpp:10-17), with its importance-sampling
+ * distribution (BuildEnvironmentDistribution, src/lighting/environment.cpp:73-123). */
+int yrhost_scene_set_environment_map(yrhost_scene* scene, int32_t texture_index, const float* radiance3,
+                                     float strength, float rotation_radians);
 int yrhost_scene_set_emissives(yrhost_scene* scene, const yrh_emissive* emissives, int32_t count);
 int yrhost_scene_set_constant_environment(yrhost_scene* scene, const float* radiance3, float strength);
 /* Replaces the analytic spheres (copied; the acceleration structure must be

@@ -140,8 +140,8 @@ enum class RenderMaterialKind {
     SubsurfaceExit,
 };
 
-// A material parameter with an optional texture (material.hpp:21-30); this
-// backend reads `value` and rejects texture != -1.
+// A material parameter with an optional texture index into
+// RenderSceneIR::textures (material.hpp:21-30; -1 = none).
 struct TexParam1f {
     float value = 0.0f;
     int texture = -1;
@@ -162,6 +162,7 @@ struct RenderMaterial {
     TexParam1f uroughness{0.0f};
     TexParam1f vroughness{0.0f};
     int normal_map = -1;
+    float normal_scale = 1.0f;
     Color3f emission{0.0f, 0.0f, 0.0f};
     TexParam1f alpha{1.0f};
     Color3f absorption_color{1.0f, 1.0f, 1.0f};
@@ -211,10 +212,19 @@ struct LightSamplingCache {
     bool use_analytic_tree = false;
 };
 
+// TextureFilter / TextureWrap (include/yaoray/scene/texture_options.hpp:7-18).
+enum class TextureFilter { Nearest, Bilinear, Trilinear, Ewa };
+enum class TextureWrap { Repeat, ClampToEdge, MirroredRepeat };
+
+// RenderTexture (texture.hpp:38-51): linear RGBA texels of level 0 with the
+// filter and wrap modes; mip chains are built where the texture is sampled.
 struct RenderTexture {
     int width = 0;
     int height = 0;
-    std::vector<Color4f> texels;  // bilinear, repeat wrap (texture.cpp:116-130)
+    std::vector<Color4f> texels;
+    TextureFilter filter = TextureFilter::Bilinear;
+    TextureWrap wrap_s = TextureWrap::Repeat;
+    TextureWrap wrap_t = TextureWrap::Repeat;
 };
 
 struct RenderEnvironment {

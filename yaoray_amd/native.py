@@ -97,7 +97,18 @@ class HitSoa(C.Structure):
 class Material(C.Structure):
     _fields_ = [("kind", C.c_int32), ("reflectance", C.c_float * 3), ("ior", C.c_float), ("uroughness", C.c_float),
                 ("vroughness", C.c_float), ("alpha", C.c_float), ("emission", C.c_float * 3),
-                ("absorption_distance", C.c_float), ("absorption_color", C.c_float * 3), ("reserved", C.c_float)]
+                ("absorption_distance", C.c_float), ("absorption_color", C.c_float * 3), ("reserved", C.c_float),
+                ("reflectance_texture", C.c_int32), ("alpha_texture", C.c_int32), ("normal_map", C.c_int32),
+                ("normal_scale", C.c_float)]
+
+
+FILTER_NEAREST, FILTER_BILINEAR, FILTER_TRILINEAR, FILTER_EWA = 0, 1, 2, 3
+WRAP_REPEAT, WRAP_CLAMP, WRAP_MIRRORED = 0, 1, 2
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("filter", C.c_int32), ("wrap_s", C.c_int32),
+                ("wrap_t", C.c_int32), ("reserved", C.c_int32), ("texels", C.c_void_p)]
 
 
 class Emissive(C.Structure):
@@ -109,7 +120,8 @@ class EnvironmentDesc(C.Structure):
                 ("rotation_radians", C.c_float), ("width", C.c_int32), ("height", C.c_int32),
                 ("texels", C.c_void_p), ("has_distribution", C.c_int32), ("dist_width", C.c_int32),
                 ("dist_height", C.c_int32), ("texel_weights", C.c_void_p), ("row_weights", C.c_void_p),
-                ("row_cdf", C.c_void_p), ("conditional_cdfs", C.c_void_p), ("total_weight", C.c_float)]
+                ("row_cdf", C.c_void_p), ("conditional_cdfs", C.c_void_p), ("total_weight", C.c_float),
+                ("nearest", C.c_int32), ("wrap_s", C.c_int32), ("wrap_t", C.c_int32)]
 
 
 class AnalyticLight(C.Structure):
@@ -129,7 +141,8 @@ class ShadingDesc(C.Structure):
                 ("analytic_light_count", C.c_int32), ("analytic_alias_probability", C.c_void_p),
                 ("analytic_alias_index", C.c_void_p), ("analytic_alias_pmf", C.c_void_p),
                 ("analytic_tree", C.c_void_p), ("analytic_tree_count", C.c_int32),
-                ("analytic_tree_root", C.c_int32), ("use_analytic_tree", C.c_int32)]
+                ("analytic_tree_root", C.c_int32), ("use_analytic_tree", C.c_int32), ("textures", C.c_void_p),
+                ("texture_count", C.c_int32)]
 
 
 class RenderDesc(C.Structure):
@@ -265,6 +278,7 @@ YRH_SIGNATURES = {
     "yrh_render_stats_read": (_I, [_VP, C.POINTER(RenderStats)]),
     "yrh_sample_values": (_I, [_VP, _I32, _U64, _I32, _I64, _VP, _VP, _VP, _VP, _I32, _VP]),
     "yrh_bsdf_query": (_I, [_VP, _VP, _I32, _I64, _VP, _VP, _VP]),
+    "yrh_texture_query": (_I, [_VP, _VP, _I64, _VP, _VP, _VP]),
     "yrh_validate_geometry": (_I, [_VP, _VP, _I32]),
     "yrh_trace_closest_noalloc": (_I, [_VP, C.POINTER(RaySoa), _I64, C.POINTER(HitSoa)]),
     "yrh_generate_camera_rays": (_I, [_VP, C.POINTER(RenderDesc), _VP, _VP]),
@@ -292,6 +306,8 @@ HOST_SIGNATURES = {
     "yrhost_scene_destroy": (None, [_VP]),
     "yrhost_scene_set_camera": (_I, [_VP, _VP]),
     "yrhost_scene_set_materials": (_I, [_VP, _VP, _I32]),
+    "yrhost_scene_add_texture": (_I, [_VP, _VP]),
+    "yrhost_scene_set_environment_map": (_I, [_VP, _I32, _VP, C.c_float, C.c_float]),
     "yrhost_scene_set_spheres": (_I, [_VP, _VP, _I32]),
     "yrhost_scene_set_analytic_lights": (_I, [_VP, _VP, _I32]),
     "yrhost_scene_spheres": (_I, [_VP, C.POINTER(_VP), C.POINTER(_I32)]),
