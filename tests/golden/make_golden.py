@@ -276,7 +276,37 @@ def cornell_pbrt_fixture():
     return out
 
 
-FIXTURES = {"cornell_pbrt": cornell_pbrt_fixture, "spheres": spheres_fixture, "sampler": sampler_fixture, "bsdf": bsdf_fixture, "zoo_film": zoo_film_fixture, "flat_trace": flat_fixture, "two_level_trace": two_level_fixture,
+def textures_pbrt_fixture():
+    """Textured PBRT scenes compiled by the reference's frontend (the drop-in
+    probe's `dump`) with the reference CPU backend's film + statistics at
+    16 spp, rendered by the same probe: this repository's hip_textures.pbrt
+    (EWA / trilinear / nearest / bilinear image textures, a normal map with
+    tangents, textured thin glass, an HDR environment map) under "ours_", and
+    the reference's own texture_test smoke scene (scenes/pbrt/texture_test:
+    checker PNGs with repeat / clamp wraps, a normal-mapped plane without
+    tangents, an HDR sky) under "tt_"."""
+    import subprocess
+    import tempfile
+
+    from tests import dropin as D
+    scenes = (("ours", os.path.join(D.SCENES, "hip_textures.pbrt")),
+              ("tt", "/root/reference/scenes/pbrt/texture_test/texture_test_smoke.pbrt"))
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for tag, path in scenes:
+            ir_path = os.path.join(tmp, tag + ".bin")
+            subprocess.run([D.PROBE, "dump", path, ir_path], check=True, cwd=tmp, capture_output=True)
+            for k, v in read_yrir(ir_path).items():
+                out[f"{tag}_{k}"] = v
+            r, res = D.probe(["render", path, "cpu", "16"], os.path.join(tmp, tag + "_film.bin"), tmp)
+            assert r.returncode == 0 and res["ok"], r.stderr
+            out[f"{tag}_film"] = res["film"].reshape(-1).view(np.uint8)
+            out[f"{tag}_stats"] = np.array([res["stats"][k] for k in O.STAT_KEYS], np.uint64)
+    out["spp"] = np.int32(16)
+    return out
+
+
+FIXTURES = {"textures_pbrt": textures_pbrt_fixture, "cornell_pbrt": cornell_pbrt_fixture, "spheres": spheres_fixture, "sampler": sampler_fixture, "bsdf": bsdf_fixture, "zoo_film": zoo_film_fixture, "flat_trace": flat_fixture, "two_level_trace": two_level_fixture,
             "cornell_film": cornell_fixture, "cornell_adaptive": cornell_adaptive_fixture}
 
 

@@ -2,7 +2,9 @@
 film and checkpoint code driving HipBackend.
 
 * HIP vs the reference's CPU backend on the same PBRT scenes (every in-scope
-  material kind as triangle meshes; analytic spheres from Shape "sphere"): image mean within 0.5 %, per-pixel RMSE
+  material kind as triangle meshes; analytic spheres from Shape "sphere";
+  image textures with every filter, a normal map with tangents and an HDR
+  environment map): image mean within 0.5 %, per-pixel RMSE
   below 0.35 x the CPU film's standard error, counters within 1 %, sample
   counts equal (same sampler streams; beyond the first bounce the device's
   transcendentals differ from glibc in the last bits).
@@ -31,7 +33,7 @@ def lum_mean(film):
     return (film["sum"].reshape(-1, 3) / np.maximum(film["samples"].reshape(-1), 1)[:, None]) @ LUM
 
 
-@pytest.mark.parametrize("scene", ["hip_room.pbrt", "hip_spheres.pbrt"])
+@pytest.mark.parametrize("scene", ["hip_room.pbrt", "hip_spheres.pbrt", "hip_textures.pbrt"])
 def test_hip_matches_reference_cpu_backend(tmp_path, scene):
     rc, cpu = D.render(scene, "cpu", tmp_path)
     rh, hip = D.render(scene, "hip", tmp_path)

@@ -159,3 +159,17 @@ def test_oracle_matches_reference_cornell_pbrt_fixture():
                            sampler=st["sampler"], seed=st["seed"])
     assert film.tobytes() == G.film(f["film"]).tobytes()
     assert stats == G.stats(f["stats"])
+
+
+@pytest.mark.parametrize("tag", ["ours", "tt"])
+def test_oracle_matches_reference_textures_pbrt_fixture(tag):
+    """Textured PBRT scenes through the reference's frontend: this repository's
+    hip_textures.pbrt and the reference's texture_test smoke scene; the
+    oracle's film and counters equal the reference CPU backend's bit for bit."""
+    f = G.load("textures_pbrt")
+    s, st = G.pbrt_scene(f, tag)
+    assert s.info.texture_count >= 5
+    film, stats = O.render(s, st["width"], st["height"], int(f["spp"]), max_depth=st["max_depth"],
+                           sampler=st["sampler"], seed=st["seed"])
+    assert film.tobytes() == G.film(f[tag + "_film"]).tobytes()
+    assert stats == G.stats(f[tag + "_stats"])

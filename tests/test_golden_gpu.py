@@ -164,3 +164,24 @@ def test_backend_matches_reference_cornell_pbrt_fixture(device):
           f"rel mean {rel_mean:.5f} rmse {rmse:.5f} noise {noise:.5f} counts {counts}", flush=True)
     assert bad < 0.05 and mean_spp < 0.02 and same.mean() > 0.85
     assert rel_mean < 0.005 and rmse < 0.35 * noise and max(counts.values()) < 0.01
+
+
+@pytest.mark.parametrize("tag", ["ours", "tt"])
+def test_backend_matches_reference_textures_pbrt_fixture(device, tag):
+    """HipBackend on textured PBRT scenes (this repository's hip_textures.pbrt,
+    the reference's texture_test smoke scene) against the reference CPU
+    backend's film: sample counts equal, statistical parity."""
+    f = G.load("textures_pbrt")
+    s, st = G.pbrt_scene(f, tag)
+    w, h, spp = st["width"], st["height"], int(f["spp"])
+    print(f"textured PBRT {tag} {w}x{h}x{spp} depth {st['max_depth']}", flush=True)
+    rep, film = s.backend_render(w, h, spp, max_depth=st["max_depth"], sampler=st["sampler"], seed=st["seed"])
+    assert rep["ok"] == 1, rep["error"]
+    ref = G.film(f[tag + "_film"])
+    rs = G.stats(f[tag + "_stats"])
+    g = film.reshape(-1)
+    assert np.array_equal(g["samples"], ref["samples"]) and rep["bad_samples"] == rs["bad_samples"] == 0
+    st_ = {k: rep[k] for k in ("rays_traced", "hits", "misses", "shadow_rays")}
+    rel_mean, rmse, noise, counts = film_parity(g, st_, ref, rs)
+    print(f"textured PBRT {tag}: rel mean {rel_mean:.5f} rmse {rmse:.5f} noise {noise:.5f} counts {counts}", flush=True)
+    assert rel_mean < 0.005 and rmse < 0.35 * noise and max(counts.values()) < 0.01

@@ -251,6 +251,7 @@ int yrhost_scene_info_get(const yrhost_scene* s, yrhost_scene_info* info) {
     info->environment_active = s->ir.environment.active ? 1 : 0;
     info->sphere_count = static_cast<int32_t>(s->ir.spheres.size());
     info->analytic_light_count = static_cast<int32_t>(s->ir.analytic_lights.size());
+    info->texture_count = static_cast<int32_t>(s->ir.textures.size());
     if (s->accel_built) {
         info->accel_kind = s->accel.kind == yr::RenderAccelerationKind::TwoLevel ? YRH_ACCEL_TWO_LEVEL : YRH_ACCEL_FLAT;
         info->node_count = s->accel.NodeCount();

@@ -42,6 +42,7 @@ typedef struct yrhost_scene_info {
     float camera[13];       /* origin, forward, right, up, fov_y_radians */
     int32_t sphere_count;
     int32_t analytic_light_count;
+    int32_t texture_count;
 } yrhost_scene_info;
 
 int yrhost_scene_create(int kind, const int64_t* params, int param_count, uint64_t seed, yrhost_scene** out);

@@ -176,7 +176,8 @@ class SceneInfo(C.Structure):
                 ("instance_count", C.c_int32), ("material_count", C.c_int32), ("emissive_count", C.c_int32),
                 ("accel_built", C.c_int32), ("accel_kind", C.c_int32), ("node_count", C.c_int32),
                 ("max_depth", C.c_int32), ("blas_count", C.c_int32), ("environment_active", C.c_int32),
-                ("camera", C.c_float * 13), ("sphere_count", C.c_int32), ("analytic_light_count", C.c_int32)]
+                ("camera", C.c_float * 13), ("sphere_count", C.c_int32), ("analytic_light_count", C.c_int32),
+                ("texture_count", C.c_int32)]
 
 
 class BatchJob(C.Structure):
