@@ -14,7 +14,10 @@ After it, the render workloads (`renders` in the JSON line): the seeded
 Cornell box (256x256, 16 spp, depth 8) and the two-level instanced scene
 (1920x1080, 64 spp, depth 16); one step = one whole frame, timed with events
 on the context stream, beside the reference's RenderCpuPathTrace on the host
-cores (the whole Cornell frame; a 480x270 x 8 spp sample of the two-level one).
+cores (the whole Cornell frame; a 480x270 x 8 spp sample of the two-level one);
+the adaptive Cornell render; and two PBRT scenes as the reference's frontend
+compiles them, rendered by HipBackend: the reference's own Cornell scene and
+the textured scene of the drop-in tests at 1280x720 x 64 spp.
 
 Then (`batch`) 32 cameras of uneven size and spp on the two-level scene in
 one pass schedule with HIP-graph replay, wavefront size autotuned; and
@@ -137,7 +140,7 @@ def main():
     ap.add_argument("--incoherent", type=int, default=4_194_304)
     ap.add_argument("--cpu-stride", type=int, default=4, help="CPU baseline traces every k-th ray")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--renders", default="cornell,two_level,cornell_adaptive,cornell_pbrt",
+    ap.add_argument("--renders", default="cornell,two_level,cornell_adaptive,cornell_pbrt,textures_pbrt",
                     help="render workloads timed after the ray-cast microbench (comma list, empty for none)")
     ap.add_argument("--render-frames", type=int, default=3)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-camera workload")
@@ -270,8 +273,8 @@ def main():
 
     renders = {}
     for key in [k for k in args.renders.split(",") if k]:
-        if key == "cornell_pbrt":
-            renders[key] = bench_pbrt(args.render_frames, host_cores()[0], not args.no_cpu_baseline)
+        if key in PBRT_WORKLOADS:
+            renders[key] = bench_pbrt(key, args.render_frames, host_cores()[0], not args.no_cpu_baseline)
             continue
         renders[key] = bench_render(dev, key, args.render_frames, min(16, len(os.sched_getaffinity(0))),
                                     not args.no_cpu_baseline)
@@ -622,18 +625,35 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
     return out
 
 
-def bench_pbrt(frames, cpu_threads, with_cpu):
-    """The reference's own Cornell scene (scenes/pbrt/cornell_box_pbrt: mirror and
+PBRT_WORKLOADS = {
+    # key: (fixture, section prefix, frame override (w, h, spp) or None, description)
+    "cornell_pbrt": ("cornell_pbrt", "full_", None,
+                     "the reference's Cornell scene (cornell_box_pbrt.pbrt: 24 tris, mirror + glass spheres, area + "
+                     "point light)"),
+    "textures_pbrt": ("textures_pbrt", "ours_", (1280, 720, 64),
+                      "hip_textures.pbrt (EWA / trilinear / nearest / bilinear image textures, normal map with "
+                      "tangents, textured thin glass, 16x8 HDR environment map, area light)"),
+}
+
+
+def bench_pbrt(key, frames, cpu_threads, with_cpu):
+    """A PBRT scene as the reference's frontend compiles it (the committed
+    RenderSceneIR dump under tests/golden/), rendered by HipBackend (Prepare +
+    Render, the reference's pass loop); Render seconds per frame. cornell_pbrt:
+    the reference's own Cornell scene (scenes/pbrt/cornell_box_pbrt: mirror and
     glass spheres, an area light and a point light; 256x256, 64 spp, depth 6,
-    independent sampler) as its PBRT frontend compiles it (the committed
-    RenderSceneIR dump, tests/golden/cornell_pbrt.npz), rendered by HipBackend
-    (Prepare + Render, the reference's pass loop); Render seconds per frame."""
+    independent sampler); textures_pbrt: this repository's textured scene at
+    1280x720, 64 spp, depth 6 (camera-ray footprints, EWA and trilinear
+    lookups, normal mapping, textured shadows)."""
     from yaoray_amd import Scene
 
-    f = np.load(os.path.join(ROOT, "tests", "golden", "cornell_pbrt.npz"), allow_pickle=False)
-    sections = {k[len("full_"):]: f[k] for k in f.files if k.startswith("full_")}
+    fixture, prefix, frame, desc = PBRT_WORKLOADS[key]
+    f = np.load(os.path.join(ROOT, "tests", "golden", fixture + ".npz"), allow_pickle=False)
+    sections = {k[len(prefix):]: f[k] for k in f.files if k.startswith(prefix)}
     scene, st = Scene.from_ir(sections)
     w, h, spp, depth = st["width"], st["height"], st["spp"], st["max_depth"]
+    if frame is not None:
+        w, h, spp = frame
     # No progress callback (cancel_after_passes < 0): Render batches its passes.
     kw = dict(max_depth=depth, sampler=st["sampler"], seed=st["seed"], cancel_after_passes=-1)
     scene.backend_render(w, h, spp, **kw)  # warm-up
@@ -642,9 +662,9 @@ def bench_pbrt(frames, cpu_threads, with_cpu):
     i = int(np.argsort(secs)[len(secs) // 2])
     rep, film = reps[i]
     sec = secs[i]
+    sampler = {0: "independent", 1: "stratified", 2: "owen-sobol", 3: "zsobol"}.get(st["sampler"], "?")
     out = {
-        "workload": f"the reference's Cornell scene (cornell_box_pbrt.pbrt: 24 tris, mirror + glass spheres, area + "
-                    f"point light), {w}x{h}, {spp} spp independent, depth {depth}, HipBackend Render",
+        "workload": f"{desc}, {w}x{h}, {spp} spp {sampler}, depth {depth}, HipBackend Render",
         "frames": frames,
         "ms_per_frame": round(sec * 1e3, 3),
         "value": round(rep["rays_traced"] / sec / 1e6, 2),
@@ -655,13 +675,13 @@ def bench_pbrt(frames, cpu_threads, with_cpu):
         "prepare_ms": round(rep["prepare_seconds"] * 1e3, 3),
         "cpu_baseline": None,
     }
-    log(f"render cornell_pbrt: {out['ms_per_frame']} ms/frame, {out['value']} Mrays/s (+{out['shadow_Mrays_s']} shadow)")
+    log(f"render {key}: {out['ms_per_frame']} ms/frame, {out['value']} Mrays/s (+{out['shadow_Mrays_s']} shadow)")
     if with_cpu:
         from oracle import oracle_py as O
 
         if not O.ref_available():
             return out
-        c_spp = max(1, spp // 4)
+        c_spp = max(1, spp // (4 if frame is None else 16))
         rsecs = []
 
         def run():
@@ -687,7 +707,7 @@ def bench_pbrt(frames, cpu_threads, with_cpu):
             "runs_s": [round(x, 4) for x in rsecs],
             "frame_seconds_extrapolated": round(rsec * spp / c_spp, 3),
         }
-        log(f"render cornell_pbrt: cpu reference {out['cpu_baseline']['value']} Mrays/s on {cpu_threads} threads")
+        log(f"render {key}: cpu reference {out['cpu_baseline']['value']} Mrays/s on {cpu_threads} threads")
     return out
 
 
