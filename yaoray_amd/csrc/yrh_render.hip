@@ -1927,7 +1927,11 @@ __global__ __launch_bounds__(kBlock) void AccumulateKernel(PassView P) {
         const int64_t e = pixel;
         const int64_t n = *P.active_count;
         if (e < n) {
-            yrh_film_pixel& px = P.film[P.active_list[e]];
+            // The pixel's record stays in registers over its samples (one
+            // load, one store: a read-modify-write in memory per sample made
+            // each sample wait on the previous one's store).
+            yrh_film_pixel* dst = P.film + P.active_list[e];
+            yrh_film_pixel px = *dst;
             for (int k = 0; k < P.sample_count; ++k) {
                 if (px.samples >= static_cast<uint32_t>(P.spp)) break;
                 if (AddSample(P, static_cast<int64_t>(k) * n + e, px) != 0u) {
@@ -1936,11 +1940,12 @@ __global__ __launch_bounds__(kBlock) void AccumulateKernel(PassView P) {
                 }
                 if (P.act_on && PixelConverged(px, P.act_min_spp, P.act_rel, P.act_abs, P.act_conf)) break;
             }
+            *dst = px;
         }
-    } else {
-        for (int k = 0; pixel < P.npix && k < P.sample_count; ++k) {
-            bad += AddSample(P, static_cast<int64_t>(k) * P.npix + pixel, P.film[pixel]);
-        }
+    } else if (pixel < P.npix) {
+        yrh_film_pixel px = P.film[pixel];
+        for (int k = 0; k < P.sample_count; ++k) bad += AddSample(P, static_cast<int64_t>(k) * P.npix + pixel, px);
+        P.film[pixel] = px;
     }
     WaveStat(P.stats, kStatBad, bad);
 }

@@ -71,6 +71,11 @@ __device__ __forceinline__ float WrapCoord(float v, int wrap) {
 
 __device__ __forceinline__ int WrapIndex(int i, int n, int wrap) {
     if (wrap == kClamp) return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+    // In range, or one texel outside (every bilinear fetch, most EWA ones):
+    // the same index the modulo forms below give, without integer division.
+    if (i >= 0 && i < n) return i;
+    if (i == -1) return wrap == kMirrored ? 0 : n - 1;
+    if (i == n) return wrap == kMirrored ? n - 1 : 0;
     if (wrap == kMirrored) {
         const int period = 2 * n;
         int k = i % period;
