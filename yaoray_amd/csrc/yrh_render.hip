@@ -896,7 +896,7 @@ __device__ __forceinline__ float3 XformVectorRows(const float4* rows, float3 v) 
 // ComputeTextureFootprint (surface_query.cpp:35-97): the uv change per pixel
 // where the camera ray's differential rays (directions diff[0], diff[1] from
 // the same origin o) meet the hit's plane. p0..p2: object-space corners.
-__device__ tex::Footprint HitFootprint(const SurfaceView& S, const float4* nrows, uint32_t i0, uint32_t i1, uint32_t i2,
+__device__ __forceinline__ tex::Footprint HitFootprint(const SurfaceView& S, const float4* nrows, uint32_t i0, uint32_t i1, uint32_t i2,
                                        float3 p0, float3 p1, float3 p2, float3 o, const float4* diff, float3 hp,
                                        float3 geo) {
     const tex::Footprint none = tex::NoFootprint();
@@ -941,7 +941,7 @@ __device__ tex::Footprint HitFootprint(const SurfaceView& S, const float4* nrows
 // normal (x, y scaled) in the frame of the interpolated tangent made
 // orthogonal to the shading normal, with the interpolated handedness
 // (negated under a mirroring instance transform).
-__device__ float3 NormalMapped(const SurfaceView& S, const LightView& L, int map, float scale, uint32_t i0, uint32_t i1,
+__device__ __forceinline__ float3 NormalMapped(const SurfaceView& S, const LightView& L, int map, float scale, uint32_t i0, uint32_t i1,
                                uint32_t i2, float u, float v, float2 uv, float3 sn, const float4* nrows,
                                const tex::Footprint& fp) {
     const float4 e = tex::Sample(L.tex, map, uv.x, uv.y, fp);
@@ -1354,7 +1354,7 @@ __device__ __forceinline__ void ShadePath(const PassView& P, const SurfaceView& 
 }
 
 template <bool kGuided, bool kAnalytic, bool kTex>
-__global__ __launch_bounds__(kBlock) void ShadeKernel(PassView P, SurfaceView S, LightView L, yrg::GuideView G,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void ShadeKernel(PassView P, SurfaceView S, LightView L, yrg::GuideView G,
                                                      int depth, int cur) {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const int n = static_cast<int>(P.counters[cur]);

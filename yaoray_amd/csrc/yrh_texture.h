@@ -142,7 +142,7 @@ __device__ __forceinline__ void Axes(const Tex& t, const Footprint& f, float& ax
     ay = HypotF(f.dudy * w, f.dvdy * h);
 }
 
-__device__ float4 EwaLevel(const TexView& T, const Tex& t, float u, float v, const Footprint& f, int level) {
+__device__ __forceinline__ float4 EwaLevel(const TexView& T, const Tex& t, float u, float v, const Footprint& f, int level) {
     const Level l = LevelOf(T, t, level);
     const float W = static_cast<float>(l.w), H = static_cast<float>(l.h);
     float ax = f.dudx * W, ay = f.dvdx * H, bx = f.dudy * W, by = f.dvdy * H;
@@ -201,7 +201,7 @@ __device__ float4 EwaLevel(const TexView& T, const Tex& t, float u, float v, con
 }
 
 // SampleTexture4 of texture `index` (texture.cpp:312-322).
-__device__ float4 Sample(const TexView& T, int index, float u, float v, const Footprint& f) {
+__device__ __forceinline__ float4 Sample(const TexView& T, int index, float u, float v, const Footprint& f) {
     const Tex t = Load(T, index);
     if (t.levels <= 0) return make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     if (t.filter == kNearest) return Nearest(LevelOf(T, t, 0), u, v, t.ws, t.wt);
