@@ -969,7 +969,7 @@ __device__ __forceinline__ float3 NormalMapped(const SurfaceView& S, const Light
 // The alpha EvaluateMaterialAtSurface gives a triangle hit (material_evaluator.cpp:135-149):
 // the material's alpha, times the alpha texture's first channel where the
 // primitive has uvs. `diff`: the camera ray's differentials (footprint), or nullptr.
-__device__ float HitAlpha(const SurfaceView& S, const LightView& L, int tri, int inst, float u, float v, float3 o,
+__device__ __forceinline__ float HitAlpha(const SurfaceView& S, const LightView& L, int tri, int inst, float u, float v, float3 o,
                           float3 d, float t, const float4* diff) {
     const int prim = S.tri_prim[tri];
     const int4 pinfo = S.prim_info[prim];
@@ -1618,7 +1618,7 @@ __global__ __launch_bounds__(kBlock) void ResolveKernel(PassView P, yrg::GuideVi
 
 __device__ __forceinline__ bool TransBlack(float3 c) { return MaxC(c) <= 1.0e-6f; }
 
-template <bool kAlpha>
+template <bool kAlpha, bool kTex>
 struct ShadowWalk {
     const PassView* P;
     const SurfaceView* S;
@@ -1685,7 +1685,7 @@ struct ShadowWalk {
                                                              : S->prim_info[S->tri_prim[best.id]].x;
             DevMaterial mat = L->materials[mat_index];
             if (ok && mat.kind != kDielectric && mat.kind != kThinDielectric) ok = false;
-            if (ok && mat.kind == kThinDielectric && L->mat_tex != nullptr && !trav::IsSphereId(best.id)) {
+            if (kTex && ok && mat.kind == kThinDielectric && !trav::IsSphereId(best.id)) {
                 // The hit's material has its reflectance texture applied
                 // (EvaluateMaterialAtSurface); shadow rays carry no footprint.
                 const int rt = L->mat_tex[mat_index].x;
@@ -1742,12 +1742,15 @@ struct ShadowWalk {
 };
 
 // kAlpha: the scene has alpha cut-outs, which the walk skips as
-// TraceVisibleSurface does within each segment.
-template <bool kTwoLevel, bool kSpheres, bool kAlpha>
+// TraceVisibleSurface does within each segment; kTex: materials have
+// textures (thin glass's reflectance texture scales its shadow
+// transmittance). Scenes without either run the variant without that code
+// (fewer registers).
+template <bool kTwoLevel, bool kSpheres, bool kAlpha, bool kTex>
 __global__ __launch_bounds__(trav::kMaxBlock) void ShadowWalkKernel(trav::TraceArgs a, PassView P, SurfaceView S,
                                                                     LightView L) {
     extern __shared__ int2 lds_stack[];
-    const ShadowWalk<kAlpha> pol{&P, &S, &L};
+    const ShadowWalk<kAlpha, kTex> pol{&P, &S, &L};
     if (kTwoLevel) {
         trav::TraceTwoLevelLoop<false, kSpheres>(a, pol, lds_stack);
     } else {
@@ -1755,19 +1758,27 @@ __global__ __launch_bounds__(trav::kMaxBlock) void ShadowWalkKernel(trav::TraceA
     }
 }
 
-template <bool kTwoLevel>
-void LaunchShadowWalk(bool spheres, bool alpha, int grid, int block, size_t lds, hipStream_t st,
-                      const trav::TraceArgs& a, const PassView& P, const SurfaceView& S, const LightView& L) {
-    if (alpha) {
-        if (spheres) {
-            hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, true, true>), dim3(grid), dim3(block), lds, st, a, P, S, L);
-        } else {
-            hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, false, true>), dim3(grid), dim3(block), lds, st, a, P, S, L);
-        }
-    } else if (spheres) {
-        hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, true, false>), dim3(grid), dim3(block), lds, st, a, P, S, L);
+template <bool kTwoLevel, bool kSpheres>
+void LaunchShadowWalkAs(bool alpha, bool tex, int grid, int block, size_t lds, hipStream_t st,
+                        const trav::TraceArgs& a, const PassView& P, const SurfaceView& S, const LightView& L) {
+    if (alpha && tex) {
+        hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, kSpheres, true, true>), dim3(grid), dim3(block), lds, st, a, P, S, L);
+    } else if (alpha) {
+        hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, kSpheres, true, false>), dim3(grid), dim3(block), lds, st, a, P, S, L);
+    } else if (tex) {
+        hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, kSpheres, false, true>), dim3(grid), dim3(block), lds, st, a, P, S, L);
     } else {
-        hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, false, false>), dim3(grid), dim3(block), lds, st, a, P, S, L);
+        hipLaunchKernelGGL((ShadowWalkKernel<kTwoLevel, kSpheres, false, false>), dim3(grid), dim3(block), lds, st, a, P, S, L);
+    }
+}
+
+template <bool kTwoLevel>
+void LaunchShadowWalk(bool spheres, bool alpha, bool tex, int grid, int block, size_t lds, hipStream_t st,
+                      const trav::TraceArgs& a, const PassView& P, const SurfaceView& S, const LightView& L) {
+    if (spheres) {
+        LaunchShadowWalkAs<kTwoLevel, true>(alpha, tex, grid, block, lds, st, a, P, S, L);
+    } else {
+        LaunchShadowWalkAs<kTwoLevel, false>(alpha, tex, grid, block, lds, st, a, P, S, L);
     }
 }
 
@@ -2362,9 +2373,11 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
             int wgrid = 0, wblock = 0;
             if ((rc = PrepareTraceArgs(ctx, wr, wa, wgrid, wblock)) != YRH_OK) return rc;
             if (ctx->geo.accel_kind == YRH_ACCEL_TWO_LEVEL) {
-                LaunchShadowWalk<true>(ctx->geo.has_spheres, R.alpha, wgrid, wblock, LdsBytes(wa, wblock), ss, wa, P, S, L);
+                LaunchShadowWalk<true>(ctx->geo.has_spheres, R.alpha, R.textured, wgrid, wblock, LdsBytes(wa, wblock), ss,
+                                       wa, P, S, L);
             } else {
-                LaunchShadowWalk<false>(ctx->geo.has_spheres, R.alpha, wgrid, wblock, LdsBytes(wa, wblock), ss, wa, P, S, L);
+                LaunchShadowWalk<false>(ctx->geo.has_spheres, R.alpha, R.textured, wgrid, wblock, LdsBytes(wa, wblock), ss,
+                                        wa, P, S, L);
             }
         }
         hipLaunchKernelGGL(ResolveKernel, dim3(Grid(paths)), dim3(kBlock), 0, ss, P, GV, depth, cur);
