@@ -16,8 +16,11 @@ import os
 import time
 
 from . import native as N
+from .api import YrhError
 
-DEFAULT_PATHS = (1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27)
+# Up to 2^28 paths per pass (~128 GB of path state at ~480 B per path, within
+# one MI355X's 288 GB); a candidate the device cannot hold is skipped.
+DEFAULT_PATHS = (1 << 23, 1 << 24, 1 << 25, 1 << 26, 1 << 27, 1 << 28)
 DEFAULT_BLOCKS_PER_CU = (-1, 4)  # -1: the automatic launch shape
 
 
@@ -60,15 +63,23 @@ def tune(device, jobs, key, max_depth=8, sampler=N.SAMPLER_OWEN_SOBOL, paths_can
         for bpc in blocks_candidates:
             device.set_trace_config(bpc, 0)
             best_t = None
+            failed = None
             for _ in range(max(1, repeats)):
                 device.synchronize()
                 t0 = time.perf_counter()
-                device.render_batch(probe, max_depth=max_depth, sampler=sampler, paths_per_pass=ppp)
+                try:
+                    device.render_batch(probe, max_depth=max_depth, sampler=sampler, paths_per_pass=ppp)
+                except YrhError as e:  # e.g. the pass's path state does not fit the device
+                    failed = str(e)
+                    break
                 t = time.perf_counter() - t0
                 best_t = t if best_t is None else min(best_t, t)
+            if failed is not None:
+                trials.append({"paths_per_pass": ppp, "blocks_per_cu": bpc, "probe_paths": paths, "error": failed})
+                continue
             trials.append({"paths_per_pass": ppp, "blocks_per_cu": bpc, "probe_paths": paths,
                            "seconds": round(best_t, 5), "paths_per_s": round(paths / best_t, 1)})
-    best = max(trials, key=lambda r: r["paths_per_s"])
+    best = max((r for r in trials if "paths_per_s" in r), key=lambda r: r["paths_per_s"])
     device.set_trace_config(best["blocks_per_cu"], 0)
     if cache:
         data = _load(cache)
