@@ -51,7 +51,8 @@ def build_hip(jobs: int = 8) -> str:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     out = os.path.join(LIB, "libyrh.so")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp", *objs])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp", *objs, "-L/opt/rocm/lib",
+          "-lrocprofiler-sdk-roctx"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -482,6 +482,7 @@ int yrh_validate_geometry(const yrh_geometry_desc* g, char* error, int error_cap
 }
 
 int yrh_upload_geometry(yrh_context* ctx, const yrh_geometry_desc* g) {
+    const StageRange stage_range("yrh_upload_geometry");
     if (ctx == nullptr || g == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null argument");
     {
         std::string msg;
@@ -636,6 +637,7 @@ int yrh_upload_geometry(yrh_context* ctx, const yrh_geometry_desc* g) {
 }
 
 int yrh_trace_closest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, const yrh_hit_soa* hits) {
+    const StageRange stage_range("yrh_trace_closest");
     if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
     if (rays == nullptr || hits == nullptr || n < 0) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null rays/hits or negative count");
     if (n > 0 && (rays->org_tmin == nullptr || rays->dir_tmax == nullptr || hits->hit == nullptr)) {
@@ -649,6 +651,7 @@ int yrh_trace_closest(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, cons
 
 int yrh_trace_any(yrh_context* ctx, const yrh_ray_soa* rays, int64_t n, uint32_t* occluded,
                   uint32_t* node_tests, uint32_t* triangle_tests) {
+    const StageRange stage_range("yrh_trace_any");
     if (ctx == nullptr) return YRH_E_INVALID_ARGUMENT;
     if (rays == nullptr || n < 0) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null rays or negative count");
     if (n > 0 && (rays->org_tmin == nullptr || rays->dir_tmax == nullptr || occluded == nullptr)) {

@@ -798,6 +798,7 @@ int yrh_guide_set_cell_budget(yrh_context* ctx, int64_t bytes) {
 }
 
 int yrh_guide_train(yrh_context* ctx, int32_t steps, int32_t batch) {
+    const StageRange stage_range("yrh_guide_train");
     int rc = NeedGuide(ctx);
     if (rc != YRH_OK) return rc;
     if (steps < 0 || batch <= 0) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "bad steps or batch");

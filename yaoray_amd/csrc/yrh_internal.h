@@ -18,6 +18,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdint>
 #include <string>
@@ -26,6 +27,16 @@
 #include "../../include/yrh.h"
 
 namespace yrh {
+
+// A roctx range over a host-side stage (entry points, pass stages): shown by
+// `rocprofv3 --marker-trace` beside the kernels the stage enqueues; a
+// push / pop pair costs nanoseconds without a profiler.
+struct StageRange {
+    explicit StageRange(const char* name) { roctxRangePushA(name); }
+    ~StageRange() { roctxRangePop(); }
+    StageRange(const StageRange&) = delete;
+    StageRange& operator=(const StageRange&) = delete;
+};
 
 constexpr int kLeafCountEscape = 15;
 // Context work-counter words (yrh_context::work_counter).

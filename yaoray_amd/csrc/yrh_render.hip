@@ -2276,8 +2276,12 @@ int LaunchPassSequence(yrh_context* ctx, RenderState& R, const PassView& P, cons
     }
     hipEvent_t shaded = overlap ? ctx->bounce_events[0] : nullptr;
     hipEvent_t resolved = overlap ? ctx->bounce_events[1] : nullptr;
-    hipLaunchKernelGGL(RaygenKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P);
+    {
+        const StageRange r("raygen");
+        hipLaunchKernelGGL(RaygenKernel, dim3(Grid(paths)), dim3(kBlock), 0, st, P);
+    }
     for (int depth = 0; depth < max_depth; ++depth) {
+        const StageRange bounce_range("bounce");
         const int cur = depth & 1;
         hipLaunchKernelGGL(BeginBounceKernel, dim3(1), dim3(64), 0, st, P.counters, 1 - cur, kBeginTrace);
         TraceRequest tr;
@@ -2603,6 +2607,7 @@ using namespace yrh;
 extern "C" {
 
 int yrh_upload_shading(yrh_context* ctx, const yrh_shading_desc* sd) {
+    const StageRange stage_range("yrh_upload_shading");
     if (ctx == nullptr || sd == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null argument");
     if (!ctx->geo.loaded || ctx->render == nullptr) return Fail(ctx, YRH_E_NO_SCENE, "upload geometry first");
     RenderState& R = *ctx->render;
@@ -2970,6 +2975,7 @@ int yrh_suggest_paths_per_pass(yrh_context* ctx, int32_t max_depth, int64_t* pat
 }
 
 int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* d) {
+    const StageRange stage_range("yrh_render_pass");
     if (ctx == nullptr || d == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null argument");
     if (ctx->render == nullptr || !ctx->render->shading_loaded) return Fail(ctx, YRH_E_NO_SCENE, "upload geometry and shading first");
     RenderState& R = *ctx->render;
@@ -3005,6 +3011,7 @@ int yrh_render_pass(yrh_context* ctx, const yrh_render_desc* d) {
 // ---- batched multi-camera rendering ----------------------------------------------
 
 int yrh_batch_render(yrh_context* ctx, const yrh_batch_desc* d, float* job_ms, int32_t* passes_out) {
+    const StageRange stage_range("yrh_batch_render");
     if (ctx == nullptr || d == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "null argument");
     if (ctx->render == nullptr || !ctx->render->shading_loaded) return Fail(ctx, YRH_E_NO_SCENE, "upload geometry and shading first");
     if (d->job_count <= 0 || d->jobs == nullptr) return Fail(ctx, YRH_E_INVALID_ARGUMENT, "no jobs");
