@@ -1747,7 +1747,7 @@ struct ShadowWalk {
 // transmittance). Scenes without either run the variant without that code
 // (fewer registers).
 template <bool kTwoLevel, bool kSpheres, bool kAlpha, bool kTex>
-__global__ __launch_bounds__(trav::kMaxBlock) void ShadowWalkKernel(trav::TraceArgs a, PassView P, SurfaceView S,
+__global__ __launch_bounds__(trav::kMaxBlock) __attribute__((amdgpu_waves_per_eu(kAlpha || kTex ? 1 : 5))) void ShadowWalkKernel(trav::TraceArgs a, PassView P, SurfaceView S,
                                                                     LightView L) {
     extern __shared__ int2 lds_stack[];
     const ShadowWalk<kAlpha, kTex> pol{&P, &S, &L};
