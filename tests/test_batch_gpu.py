@@ -1,8 +1,9 @@
 """Batched multi-camera rendering (one pass schedule for many cameras, one
 HIP-graph-captured pass replayed per pass): every job's FP64 film equals the
 film of rendering that job alone, bit for bit, with and without the graph,
-for any wavefront size (passes may hold several sample layers of one job);
-counters add up; every pass but the last is full."""
+for any wavefront size (passes may hold several sample layers of one job),
+also for a textured scene with alpha cut-outs (per-job camera-ray
+footprints); counters add up; every pass but the last is full."""
 import numpy as np
 import pytest
 
@@ -43,11 +44,18 @@ def single(device, scene, desc, depth):
     return device.film_read(desc.width, desc.height), device.render_stats()
 
 
-@pytest.mark.parametrize("kind", ["cornell", "two_level"])
+@pytest.mark.parametrize("kind", ["cornell", "two_level", "textured"])
 def test_batch_films_equal_single_renders(device, kind):
     if kind == "cornell":
         s = Scene.cornell(1)
         depth = 8
+    elif kind == "textured":
+        # EWA / trilinear footprints from each job's own camera differentials,
+        # alpha skips in the traversal, normal maps
+        from tests.helpers import textured_scene
+
+        s = textured_scene()
+        depth = 5
     else:
         s = Scene.instanced(instances=40, subdivisions=2, variants=3, env_width=64, env_height=32, seed=5)
         depth = 10
