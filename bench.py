@@ -93,8 +93,8 @@ def load_counters():
 
 def host_cores():
     """(threads used, nproc, cores in this process's affinity). The CPU baselines
-    use every core of the affinity mask up to 16: a one-GPU box's share of the
-    host is 16 cores, whatever nproc reports."""
+    use every core of the affinity mask up to 16: one GPU's share of an
+    8-GPU MI355X host's cores, whatever nproc reports."""
     aff = len(os.sched_getaffinity(0))
     return min(16, aff), os.cpu_count(), aff
 
