@@ -153,8 +153,8 @@ def main():
     ap.add_argument("--guide-spp", default="16,32,64,128,256,512")
     ap.add_argument("--guide-train-passes", type=int, default=8)
     ap.add_argument("--guide-record-spp", type=int, default=2, help="sample indices per recording pass")
-    ap.add_argument("--guide-depths", type=int, default=3, help="guided bounces (0 = all that may continue)")
-    ap.add_argument("--guide-fraction", type=float, default=0.5)
+    ap.add_argument("--guide-depths", type=int, default=2, help="guided bounces (0 = all that may continue)")
+    ap.add_argument("--guide-fraction", type=float, default=0.65)
     ap.add_argument("--guide-train-steps", type=int, default=16)
     ap.add_argument("--guide-train-batch", type=int, default=32768)
     ap.add_argument("--guide-lr", type=float, default=3e-3)
