@@ -278,6 +278,8 @@ def main():
             continue
         renders[key] = bench_render(dev, key, args.render_frames, min(16, len(os.sched_getaffinity(0))),
                                     not args.no_cpu_baseline)
+        if key == "two_level":
+            renders[key]["binding"] = two_level_binding()
     batch = (bench_batch(dev, not args.no_cpu_baseline, host_cores()[0], use_graph=not args.no_batch_graph)
              if not args.no_batch else None)
     guiding = None
@@ -397,6 +399,28 @@ def batch_jobs(scene, count=32, seed=5, cameras=None):
     return jobs
 
 
+def two_level_binding():
+    """What bounds the two-level frame (and the batched pipeline, which runs the
+    same kernels): the committed rocprofv3 counters of its two traversal
+    kernels (profiles/counters_two_level_c3.json)."""
+    path = os.path.join(ROOT, "profiles", "counters_two_level_c3.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    keys = ("valu_busy", "valu_lane_utilization", "occupancy", "l2_hit_rate")
+    out = {"source": "profiles/counters_two_level_c3.json"}
+    for k, v in data.items():
+        if isinstance(v, dict) and "valu_busy" in v:
+            out[k] = {x: v[x] for x in keys if x in v}
+            out[k]["valu_lane_cycles_used"] = round(v["valu_busy"] * v["valu_lane_utilization"], 3)
+    out["binding"] = ("VALU issue with a quarter of the lanes active (TLAS / instance entry / BLAS node / triangle "
+                      "paths diverge within a wave) and launch tails (occupancy below the register and LDS limits); "
+                      "not HBM (L2 hit rate 0.93-0.97)")
+    return out
+
+
 def bench_batch(dev, with_cpu=True, cpu_threads=16, cpu_scale=8, use_graph=True):
     """32 cameras on the two-level scene in one pass schedule with HIP-graph replay;
     wavefront size and launch shape from the autotuner. CPU baseline: the 32 jobs
@@ -436,6 +460,7 @@ def bench_batch(dev, with_cpu=True, cpu_threads=16, cpu_scale=8, use_graph=True)
         "per_camera": [{"width": j.width, "height": j.height, "spp": j.spp, "done_ms": round(float(m), 2)}
                        for j, m in zip(jobs, ms)],
         "autotune_trials": trials,
+        "binding": two_level_binding(),
     }
     log(f"batch: {total_ms:.0f} ms, {out['value']} Mrays/s over {passes} passes")
     if with_cpu:
@@ -533,6 +558,8 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
     ref_s = time.perf_counter() - t0
     log(f"guiding: reference {ref_spp} spp in {ref_s:.1f}s")
 
+    train_events = []
+
     def progressive(guided):
         dev.film_reset(width, height)
         points = []
@@ -560,7 +587,11 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
                     dev.render_pass(d)
                     done += count
                     if guided and done <= train_passes * record_spp:
+                        e0, e1 = dev.event(), dev.event()
+                        dev.record(e0)
                         dev.guide_train(train_steps, train_batch)
+                        dev.record(e1)
+                        train_events.append((e0, e1))
                 dev.synchronize()
                 clock += time.perf_counter() - t
                 film = dev.film_read(width, height)
@@ -573,6 +604,34 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
 
     plain = progressive(False)
     guided = progressive(True)
+    dev.synchronize()
+    train_ms = [dev.elapsed_ms(a, b) for a, b in train_events]
+    for a, b in train_events:
+        dev.destroy_event(a)
+        dev.destroy_event(b)
+    # Training's MFMA work per sample: forward (2 x (32x64 + 64x64 + 64x64 + 64x48)
+    # multiply-adds) plus the backward's input and weight gradients (twice that).
+    flop_per_sample = 3 * 2 * (32 * 64 + 2 * 64 * 64 + 64 * 48)
+    steps_done = len(train_ms) * train_steps
+    train_s = sum(train_ms) / 1e3
+    training_rate = None
+    if steps_done and train_s > 0:
+        tflops = steps_done * train_batch * flop_per_sample / train_s / 1e12
+        training_rate = {
+            "kernels": "per Adam step: GuideBackwardKernel (encode, forward, mixture NLL, backward, grid-gradient "
+                       "atomics), GuideWeightGradKernel, GuideAdamMlpKernel, GuideAdamGridKernel, GuidePackKernel",
+            "flop_per_sample": flop_per_sample,
+            "steps": steps_done,
+            "batch": train_batch,
+            "ms_per_step": round(sum(train_ms) / steps_done, 4),
+            "achieved": round(tflops, 3),
+            "peak": 2500.0,
+            "unit": "TFLOP/s (BF16 MFMA, FP32 accumulation)",
+            "frac": round(tflops / 2500.0, 5),
+            "bound": "not MFMA: the hash-grid gradient's atomic adds and the latency of one wave per 16 samples "
+                     "(2 waves per SIMD); the matrix products are a few percent of the step",
+            "timing": "HIP events on the context stream around each yrh_guide_train call, inside the guided clock",
+        }
     px = [p["seconds"] for p in plain]
     py = [p["relmse"] for p in plain]
     for g in guided:
@@ -605,6 +664,9 @@ def bench_guiding(dev, width, height, ref_spp, spp_list, train_passes, train_ste
         "plain": plain,
         "guided": guided,
         "equal_time_improvement": guided[-1]["improvement"],
+        "training_mfma": training_rate,
+        "query_note": "GuideQueryKernel (hash-grid gathers + the forward MLP on MFMA, 26.6 kFLOP per row) runs "
+                      "inside each guided bounce; its share of the frame is in profiles/guide_kernels_room.md",
         "threshold": threshold,
         "time_to_threshold_plain_s": time_to(plain, threshold),
         "time_to_threshold_guided_s": time_to(guided, threshold),
