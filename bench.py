@@ -156,7 +156,7 @@ def main():
     ap.add_argument("--guide-depths", type=int, default=2, help="guided bounces (0 = all that may continue)")
     ap.add_argument("--guide-fraction", type=float, default=0.65)
     ap.add_argument("--guide-train-steps", type=int, default=16)
-    ap.add_argument("--guide-train-batch", type=int, default=32768)
+    ap.add_argument("--guide-train-batch", type=int, default=16384)
     ap.add_argument("--guide-lr", type=float, default=3e-3)
     ap.add_argument("--guide-threshold", type=float, default=0.1,
                     help="relMSE threshold of the headline time-to-threshold figure (0.2 is reported beside it)")
